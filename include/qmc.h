@@ -1,0 +1,139 @@
+/* qmc.h -- thin C ABI over the MI355X Metropolis sweep engine (libqmc.so).
+ *
+ * This is the only door between the C++ host solvers (SA / PT / PA, JSON
+ * API) and HIP.  Plain pointers and sizes; integer status codes; caller-owned
+ * host buffers are copied before a call returns; device buffers are owned by
+ * the handles.  A handle is used from one host thread at a time.
+ *
+ * Reference interfaces each entry point replaces (paths relative to the
+ * reference's cpp/ directory):
+ *   qmc_problem_create   graph::normalize_edges + Graph::populate_node_edge_ids
+ *                        (graph/graph.h:74-177, 517-534) feeding IsingTermCached
+ *                        (model/ising.h:305-342) / PuboWithCounter (model/pubo.h:285-399)
+ *   qmc_walkers_init     Walker::init -> get_random_spin_state / get_random_binary_state
+ *                        (markov/walker.h:50-58, model/ising.h:23-41, model/pubo.h:16-34)
+ *   qmc_walkers_sweep    Walker::make_sweep / attempt_transition / Metropolis::accept
+ *                        (markov/walker.h:144-153, 209-229; markov/metropolis.h:64-104),
+ *                        the OMP replica loop of SimulatedAnnealing::make_step
+ *                        (solver/simulated_annealing.h:76-110) and, with a PT ladder,
+ *                        ParallelTempering::make_step (solver/parallel_tempering.h:134-189)
+ *   qmc_walkers_exchange ParallelTempering::accept + the swap loop
+ *                        (solver/parallel_tempering.h:126-131, 171-188)
+ *   qmc_walkers_read     Walker::cost / get_lowest_cost / get_lowest_state
+ *                        (markov/walker.h:232-305)
+ *   qmc_energy           AbstractIsing::calculate_cost / PuboWithCounter::calculate_cost
+ *                        (model/ising.h:94-102, model/pubo.h:303-333)
+ */
+#ifndef QMC_H
+#define QMC_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes mirror utils::Error (utils/error_handling.h:25-52) */
+enum {
+  QMC_OK = 0,
+  QMC_MEMORY_LIMITED = 1,       /* admission / allocation refused */
+  QMC_INVALID_ARGUMENT = 101,   /* ConfigurationException family */
+  QMC_DUPLICATED_VARIABLE = 105,
+  QMC_RUNTIME_ERROR = 201,      /* RuntimeException family */
+  QMC_DEVICE_ERROR = 202
+};
+
+enum { QMC_ISING = 0, QMC_PUBO = 1 };
+enum { QMC_COUPLING_I32 = 0, QMC_COUPLING_F32 = 1 };
+/* beta source for a sweep launch */
+enum { QMC_BETA_SCHEDULE = 0, QMC_BETA_PER_REPLICA = 1 };
+
+typedef struct qmc_ctx_s* qmc_ctx;
+typedef struct qmc_problem_s* qmc_problem;
+typedef struct qmc_walkers_s* qmc_walkers;
+typedef struct qmc_batch_s* qmc_batch;
+
+typedef struct {
+  int32_t n_vars;
+  int64_t n_terms;
+  int64_t n_incidences;
+  int32_t n_classes;       /* colour classes of the sweep order */
+  int32_t lanes;           /* positions processed side by side in one class round */
+  int32_t coupling;        /* QMC_COUPLING_* chosen from the coefficients */
+  int32_t pairwise;        /* every term has at most two variables */
+  int32_t max_degree;      /* max incident terms per variable */
+  double mean_degree;
+} qmc_problem_info;
+
+/* thread-local message of the last failing call */
+const char* qmc_last_error(void);
+
+int qmc_ctx_create(int device, qmc_ctx* out);
+void qmc_ctx_destroy(qmc_ctx ctx);
+int qmc_sync(qmc_ctx ctx);
+
+/* Problem in internal ids 0..n_vars-1 (first-appearance order) as a term
+ * CSR; ids sorted inside each term, no constant terms.  lanes = 0 picks the
+ * class-round width automatically (64/128/256). */
+int qmc_problem_create(qmc_ctx ctx, int kind, int32_t n_vars, int64_t n_terms,
+                       const int64_t* term_offsets, const int32_t* term_vars,
+                       const double* coeffs, int32_t lanes, qmc_problem* out);
+void qmc_problem_destroy(qmc_problem p);
+int qmc_problem_get_info(qmc_problem p, qmc_problem_info* info);
+/* Sweep visit order (variable ids) and class offsets (n_classes + 1). */
+int qmc_problem_order(qmc_problem p, int32_t* order, int32_t* class_offsets);
+
+/* R walkers on one problem; replicas_per_group (1..32) replicas share one
+ * work-group.  Walkers form independent runs of replicas_per_seed
+ * consecutive replicas: global replica g = first_replica + r belongs to run
+ * k = g / replicas_per_seed, whose RNG key is (seed + k, stream) and whose
+ * replica ids are g % replicas_per_seed -- so run k reproduces a single run
+ * started with seed + k. */
+int qmc_walkers_create(qmc_problem p, int32_t n_replicas, int32_t replicas_per_group,
+                       uint32_t seed, uint32_t stream, int32_t first_replica,
+                       int32_t replicas_per_seed, qmc_walkers* out);
+void qmc_walkers_destroy(qmc_walkers w);
+/* random initial states (counter contract, INIT domain), cost, lowest = cost */
+int qmc_walkers_init(qmc_walkers w, uint32_t epoch);
+/* set states (n_replicas * n_vars bytes, 1 = the reference's `true` bit) */
+int qmc_walkers_set_states(qmc_walkers w, const uint8_t* states);
+int qmc_walkers_set_betas(qmc_walkers w, const double* betas /* n_replicas */);
+/* Run n_sweeps sweeps starting at global sweep index sweep0.
+ *   beta_mode QMC_BETA_SCHEDULE   : beta of sweep s = schedule[s - sweep0] (same for all replicas)
+ *   beta_mode QMC_BETA_PER_REPLICA: betas set by qmc_walkers_set_betas (or a PT ladder)
+ * ordered = 1 visits variables in index order (one lane per replica); 0 uses
+ * the coloured order of qmc_problem_order. */
+int qmc_walkers_sweep(qmc_walkers w, int64_t sweep0, int32_t n_sweeps, int beta_mode,
+                      const double* schedule, int ordered);
+/* Parallel tempering over ladders of `ladder` consecutive replicas with
+ * betas ladder_betas[0..ladder) (descending).  Runs n_steps steps of
+ * sweeps_per_replica sweeps, exchanging pairs (i,i+1), i = step mod 2, after
+ * each step; steps are numbered from step0. */
+int qmc_walkers_pt(qmc_walkers w, int32_t ladder, const double* ladder_betas, int64_t step0,
+                   int32_t n_steps, int32_t sweeps_per_replica);
+/* Read back per replica: current cost, lowest cost, current beta, accepted
+ * counts, current and lowest states (n_replicas*n_vars bytes).  Any pointer
+ * may be NULL. */
+int qmc_walkers_read(qmc_walkers w, double* cost, double* lowest, double* beta,
+                     uint64_t* accepted, uint8_t* states, uint8_t* lowest_states);
+/* Recompute energies (FP64, on device) of the current or lowest states. */
+int qmc_walkers_energy(qmc_walkers w, int lowest, double* energies);
+/* PT ladder labels: slot holding beta index k for every ladder (n_replicas) */
+int qmc_walkers_labels(qmc_walkers w, int32_t* labels);
+
+/* Energies of arbitrary states (FP64 on device). */
+int qmc_energy(qmc_problem p, int32_t n_states, const uint8_t* states, double* energies);
+
+/* Device-side delta-energy of every variable for given states (the GPU's own
+ * dE arithmetic: int32 or fp32 sums in CSR order), out[n_states*n_vars]. */
+int qmc_delta_all(qmc_problem p, int32_t n_states, const uint8_t* states, double* out);
+
+/* Philox4x32-10 words computed on the device: out[4*n] for counters ctr[4*n]. */
+int qmc_philox(qmc_ctx ctx, int32_t n, const uint32_t* ctr, const uint32_t key[2], uint32_t* out);
+
+/* Timing of the last sweep launch on this walker set (HIP events on the
+ * stream it ran on), milliseconds. */
+int qmc_walkers_last_ms(qmc_walkers w, double* ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,244 @@
+"""ctypes binding of libqmc.so (include/qmc.h).
+
+The library must be built (``python -m qiotoolkit_amd.build`` or
+``__graft_entry__.build()``); importing this module without it raises -- there
+is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libqmc.so")
+
+QMC_ISING, QMC_PUBO = 0, 1
+QMC_COUPLING_I32, QMC_COUPLING_F32 = 0, 1
+QMC_BETA_SCHEDULE, QMC_BETA_PER_REPLICA = 0, 1
+
+# every symbol include/qmc.h declares
+EXPORTS = [
+    "qmc_last_error", "qmc_ctx_create", "qmc_ctx_destroy", "qmc_sync", "qmc_problem_create",
+    "qmc_problem_destroy", "qmc_problem_get_info", "qmc_problem_order", "qmc_walkers_create",
+    "qmc_walkers_destroy", "qmc_walkers_init", "qmc_walkers_set_states", "qmc_walkers_set_betas",
+    "qmc_walkers_sweep", "qmc_walkers_pt", "qmc_walkers_read", "qmc_walkers_energy",
+    "qmc_walkers_labels", "qmc_energy", "qmc_delta_all", "qmc_philox", "qmc_walkers_last_ms",
+]
+
+
+class QmcError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("qmc error %d: %s" % (code, msg))
+        self.code = code
+
+
+class ProblemInfo(C.Structure):
+    _fields_ = [
+        ("n_vars", C.c_int32), ("n_terms", C.c_int64), ("n_incidences", C.c_int64),
+        ("n_classes", C.c_int32), ("lanes", C.c_int32), ("coupling", C.c_int32),
+        ("pairwise", C.c_int32), ("max_degree", C.c_int32), ("mean_degree", C.c_double),
+    ]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError("libqmc.so is not built (run __graft_entry__.build()); "
+                          "the sweep engine has no CPU fallback")
+    L = C.CDLL(LIB_PATH)
+    P = C.POINTER
+    vp = C.c_void_p
+    L.qmc_last_error.restype = C.c_char_p
+    L.qmc_ctx_create.argtypes = [C.c_int, P(vp)]
+    L.qmc_ctx_destroy.argtypes = [vp]
+    L.qmc_ctx_destroy.restype = None
+    L.qmc_sync.argtypes = [vp]
+    L.qmc_problem_create.argtypes = [vp, C.c_int, C.c_int32, C.c_int64, P(C.c_int64), P(C.c_int32),
+                                     P(C.c_double), C.c_int32, P(vp)]
+    L.qmc_problem_destroy.argtypes = [vp]
+    L.qmc_problem_destroy.restype = None
+    L.qmc_problem_get_info.argtypes = [vp, P(ProblemInfo)]
+    L.qmc_problem_order.argtypes = [vp, P(C.c_int32), P(C.c_int32)]
+    L.qmc_walkers_create.argtypes = [vp, C.c_int32, C.c_int32, C.c_uint32, C.c_uint32, C.c_int32,
+                                     C.c_int32, P(vp)]
+    L.qmc_walkers_destroy.argtypes = [vp]
+    L.qmc_walkers_destroy.restype = None
+    L.qmc_walkers_init.argtypes = [vp, C.c_uint32]
+    L.qmc_walkers_set_states.argtypes = [vp, P(C.c_uint8)]
+    L.qmc_walkers_set_betas.argtypes = [vp, P(C.c_double)]
+    L.qmc_walkers_sweep.argtypes = [vp, C.c_int64, C.c_int32, C.c_int, P(C.c_double), C.c_int]
+    L.qmc_walkers_pt.argtypes = [vp, C.c_int32, P(C.c_double), C.c_int64, C.c_int32, C.c_int32]
+    L.qmc_walkers_read.argtypes = [vp, P(C.c_double), P(C.c_double), P(C.c_double), P(C.c_uint64),
+                                   P(C.c_uint8), P(C.c_uint8)]
+    L.qmc_walkers_energy.argtypes = [vp, C.c_int, P(C.c_double)]
+    L.qmc_walkers_labels.argtypes = [vp, P(C.c_int32)]
+    L.qmc_energy.argtypes = [vp, C.c_int32, P(C.c_uint8), P(C.c_double)]
+    L.qmc_delta_all.argtypes = [vp, C.c_int32, P(C.c_uint8), P(C.c_double)]
+    L.qmc_philox.argtypes = [vp, C.c_int32, P(C.c_uint32), P(C.c_uint32), P(C.c_uint32)]
+    L.qmc_walkers_last_ms.argtypes = [vp, P(C.c_double)]
+    _lib = L
+    return L
+
+
+def _check(rc):
+    if rc != 0:
+        raise QmcError(rc, lib().qmc_last_error().decode(errors="replace"))
+
+
+def _p(a, t):
+    return a.ctypes.data_as(C.POINTER(t))
+
+
+class Context:
+    def __init__(self, device=0):
+        self.h = C.c_void_p()
+        _check(lib().qmc_ctx_create(device, C.byref(self.h)))
+
+    def close(self):
+        if self.h:
+            lib().qmc_ctx_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def sync(self):
+        _check(lib().qmc_sync(self.h))
+
+    def philox(self, ctrs, key):
+        ctrs = np.ascontiguousarray(ctrs, dtype=np.uint32).reshape(-1, 4)
+        k = np.ascontiguousarray(key, dtype=np.uint32)
+        out = np.zeros_like(ctrs)
+        _check(lib().qmc_philox(self.h, len(ctrs), _p(ctrs, C.c_uint32), _p(k, C.c_uint32),
+                                _p(out, C.c_uint32)))
+        return out
+
+
+class Problem:
+    """A problem in internal ids (terms with sorted unique ids, no constant terms)."""
+
+    def __init__(self, ctx, kind, n_vars, term_offsets, term_vars, coeffs, lanes=0):
+        self.ctx = ctx
+        self.kind = kind
+        off = np.ascontiguousarray(term_offsets, dtype=np.int64)
+        ids = np.ascontiguousarray(term_vars, dtype=np.int32)
+        c = np.ascontiguousarray(coeffs, dtype=np.float64)
+        self.h = C.c_void_p()
+        _check(lib().qmc_problem_create(ctx.h, kind, n_vars, len(c), _p(off, C.c_int64),
+                                        _p(ids, C.c_int32), _p(c, C.c_double), lanes,
+                                        C.byref(self.h)))
+        self.info = ProblemInfo()
+        _check(lib().qmc_problem_get_info(self.h, C.byref(self.info)))
+        self.n = self.info.n_vars
+
+    def close(self):
+        if self.h:
+            lib().qmc_problem_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def order(self):
+        o = np.zeros(self.n, dtype=np.int32)
+        co = np.zeros(self.info.n_classes + 1, dtype=np.int32)
+        _check(lib().qmc_problem_order(self.h, _p(o, C.c_int32), _p(co, C.c_int32)))
+        return o, co
+
+    def energy(self, states):
+        s = np.ascontiguousarray(states, dtype=np.uint8).reshape(-1, self.n)
+        out = np.zeros(len(s))
+        _check(lib().qmc_energy(self.h, len(s), _p(s, C.c_uint8), _p(out, C.c_double)))
+        return out
+
+    def delta_all(self, states):
+        s = np.ascontiguousarray(states, dtype=np.uint8).reshape(-1, self.n)
+        out = np.zeros((len(s), self.n))
+        _check(lib().qmc_delta_all(self.h, len(s), _p(s, C.c_uint8), _p(out, C.c_double)))
+        return out
+
+
+class Walkers:
+    def __init__(self, problem, n_replicas, replicas_per_group=8, seed=0, stream=0, first_replica=0,
+                 replicas_per_seed=None):
+        self.problem = problem
+        self.R = n_replicas
+        self.n = problem.n
+        rps = replicas_per_seed if replicas_per_seed else n_replicas + first_replica
+        self.h = C.c_void_p()
+        _check(lib().qmc_walkers_create(problem.h, n_replicas, replicas_per_group, seed & 0xFFFFFFFF,
+                                        stream, first_replica, rps, C.byref(self.h)))
+
+    def close(self):
+        if self.h:
+            lib().qmc_walkers_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def init(self, epoch=0):
+        _check(lib().qmc_walkers_init(self.h, epoch))
+
+    def set_states(self, states):
+        s = np.ascontiguousarray(states, dtype=np.uint8).reshape(self.R, self.n)
+        _check(lib().qmc_walkers_set_states(self.h, _p(s, C.c_uint8)))
+
+    def set_betas(self, betas):
+        b = np.ascontiguousarray(betas, dtype=np.float64)
+        _check(lib().qmc_walkers_set_betas(self.h, _p(b, C.c_double)))
+
+    def sweep(self, sweep0, schedule=None, n_sweeps=None, ordered=False):
+        if schedule is not None:
+            s = np.ascontiguousarray(schedule, dtype=np.float64)
+            _check(lib().qmc_walkers_sweep(self.h, sweep0, len(s), QMC_BETA_SCHEDULE, _p(s, C.c_double),
+                                           int(ordered)))
+        else:
+            _check(lib().qmc_walkers_sweep(self.h, sweep0, n_sweeps, QMC_BETA_PER_REPLICA, None,
+                                           int(ordered)))
+
+    def pt(self, ladder_betas, step0, n_steps, sweeps_per_replica=1):
+        b = np.ascontiguousarray(ladder_betas, dtype=np.float64)
+        _check(lib().qmc_walkers_pt(self.h, len(b), _p(b, C.c_double), step0, n_steps,
+                                    sweeps_per_replica))
+
+    def read(self, states=True):
+        R, N = self.R, self.n
+        cost, low, beta = np.zeros(R), np.zeros(R), np.zeros(R)
+        acc = np.zeros(R, dtype=np.uint64)
+        st = np.zeros((R, N), dtype=np.uint8) if states else None
+        ls = np.zeros((R, N), dtype=np.uint8) if states else None
+        _check(lib().qmc_walkers_read(self.h, _p(cost, C.c_double), _p(low, C.c_double),
+                                      _p(beta, C.c_double), _p(acc, C.c_uint64),
+                                      _p(st, C.c_uint8) if states else None,
+                                      _p(ls, C.c_uint8) if states else None))
+        return dict(cost=cost, lowest=low, beta=beta, accepted=acc, states=st, lowest_states=ls)
+
+    def energy(self, lowest=False):
+        out = np.zeros(self.R)
+        _check(lib().qmc_walkers_energy(self.h, int(lowest), _p(out, C.c_double)))
+        return out
+
+    def labels(self):
+        out = np.zeros(self.R, dtype=np.int32)
+        _check(lib().qmc_walkers_labels(self.h, _p(out, C.c_int32)))
+        return out
+
+    def last_ms(self):
+        v = C.c_double()
+        _check(lib().qmc_walkers_last_ms(self.h, C.byref(v)))
+        return v.value

@@ -1,0 +1,655 @@
+// C-ABI of the sweep engine (include/qmc.h): problem upload (colouring and
+// position-space CSR), walker sets, and launches.  Host-side HIP runtime code.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "qmc_internal.h"
+
+using namespace qmc;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                  \
+  do {                                                                                 \
+    hipError_t e_ = (expr);                                                            \
+    if (e_ != hipSuccess)                                                              \
+      return fail(QMC_DEVICE_ERROR, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+template <class T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  hipError_t alloc(size_t count) {
+    release();
+    n = count;
+    if (count == 0) return hipSuccess;
+    return hipMalloc((void**)&p, count * sizeof(T));
+  }
+  hipError_t upload(const T* src, size_t count) {
+    hipError_t e = alloc(count);
+    if (e != hipSuccess || count == 0) return e;
+    return hipMemcpy(p, src, count * sizeof(T), hipMemcpyHostToDevice);
+  }
+};
+
+}  // namespace
+
+struct qmc_ctx_s {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  DevBuf<double> lut;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+struct qmc_problem_s {
+  qmc_ctx ctx = nullptr;
+  int kind = QMC_ISING;
+  int32_t N = 0;
+  int64_t M = 0;
+  int coupling = QMC_COUPLING_I32;
+  int pairwise = 1;
+  int lanes = 64;
+  int max_degree = 0;
+  std::vector<int32_t> class_off, var_of_pos, pos_of_var;
+  DevBuf<int32_t> d_class_off, d_var_of_pos, d_pos_of_var, d_inc_off, d_nbr, d_oth_off, d_oth_pos;
+  DevBuf<int32_t> d_coef_i;
+  DevBuf<float> d_coef_f;
+  DevBuf<int64_t> d_term_off;
+  DevBuf<int32_t> d_term_var;
+  DevBuf<double> d_term_c;
+  int64_t n_inc = 0;
+  ProblemDev dev() const {
+    ProblemDev P{};
+    P.N = N;
+    P.n_classes = (int32_t)class_off.size() - 1;
+    P.lanes = lanes;
+    P.kind = kind;
+    P.pairwise = pairwise;
+    P.coupling = coupling;
+    P.M = M;
+    P.class_off = d_class_off.p;
+    P.var_of_pos = d_var_of_pos.p;
+    P.pos_of_var = d_pos_of_var.p;
+    P.inc_off = d_inc_off.p;
+    P.coef = coupling == QMC_COUPLING_I32 ? (const void*)d_coef_i.p : (const void*)d_coef_f.p;
+    P.nbr = d_nbr.p;
+    P.oth_off = d_oth_off.p;
+    P.oth_pos = d_oth_pos.p;
+    P.term_off = d_term_off.p;
+    P.term_var = d_term_var.p;
+    P.term_c = d_term_c.p;
+    return P;
+  }
+};
+
+struct qmc_walkers_s {
+  qmc_problem prob = nullptr;
+  int32_t R = 0, B = 1, G = 1, n_units = 0, stride = 0;
+  int32_t per_seed = 0;
+  uint32_t seed = 0, stream = 0;
+  int32_t first_replica = 0;
+  DevBuf<uint8_t> spins, best, scratch_states;
+  DevBuf<double> cost, best_cost, beta, sched, ladder_beta, energies;
+  DevBuf<unsigned long long> accepted;
+  DevBuf<int32_t> label;
+  DevBuf<UnitDev> d_units;
+  std::vector<UnitDev> units;
+  int ladder = 0;
+  float last_ms = 0.f;
+};
+
+extern "C" {
+
+const char* qmc_last_error(void) { return g_err.c_str(); }
+
+int qmc_ctx_create(int device, qmc_ctx* out) {
+  if (!out) return fail(QMC_INVALID_ARGUMENT, "null output handle");
+  int n = 0;
+  HIP_TRY(hipGetDeviceCount(&n));
+  if (device < 0 || device >= n) return fail(QMC_INVALID_ARGUMENT, "no such device");
+  HIP_TRY(hipSetDevice(device));
+  auto* c = new qmc_ctx_s();
+  c->device = device;
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return fail(QMC_DEVICE_ERROR, "hipStreamCreate failed");
+  }
+  // utils::uniform_ln (utils/utils.cc:12-32), computed with the host libm as the reference does
+  std::vector<double> lut(1024);
+  for (int r = 0; r < 4; ++r)
+    for (int i = 0; i < 256; ++i) lut[r * 256 + i] = std::log((double)i / std::pow(256.0, (double)(r + 1)));
+  if (c->lut.upload(lut.data(), lut.size()) != hipSuccess || hipEventCreate(&c->ev0) != hipSuccess ||
+      hipEventCreate(&c->ev1) != hipSuccess) {
+    qmc_ctx_destroy(c);
+    return fail(QMC_DEVICE_ERROR, "context allocation failed");
+  }
+  *out = c;
+  return QMC_OK;
+}
+
+void qmc_ctx_destroy(qmc_ctx c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  c->lut.release();
+  if (c->ev0) (void)hipEventDestroy(c->ev0);
+  if (c->ev1) (void)hipEventDestroy(c->ev1);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+int qmc_sync(qmc_ctx c) {
+  if (!c) return fail(QMC_INVALID_ARGUMENT, "null context");
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return QMC_OK;
+}
+
+int qmc_problem_create(qmc_ctx ctx, int kind, int32_t n_vars, int64_t n_terms,
+                       const int64_t* term_offsets, const int32_t* term_vars, const double* coeffs,
+                       int32_t lanes, qmc_problem* out) {
+  if (!ctx || !out) return fail(QMC_INVALID_ARGUMENT, "null handle");
+  if (kind != QMC_ISING && kind != QMC_PUBO) return fail(QMC_INVALID_ARGUMENT, "unknown model kind");
+  if (n_vars <= 0 || n_terms <= 0) return fail(QMC_INVALID_ARGUMENT, "empty problem");
+  if (lanes != 0 && (lanes % 64 != 0 || lanes < 64 || lanes > 256))
+    return fail(QMC_INVALID_ARGUMENT, "lanes must be 64, 128, 192 or 256");
+  const int32_t N = n_vars;
+  std::vector<std::vector<int32_t>> node_terms(N);
+  bool all_int = true;
+  double abs_sum = 0.0;
+  int max_term = 0;
+  for (int64_t j = 0; j < n_terms; ++j) {
+    const int64_t a = term_offsets[j], b = term_offsets[j + 1];
+    if (b <= a) return fail(QMC_INVALID_ARGUMENT, "constant terms must be folded into const_cost");
+    max_term = std::max<int>(max_term, (int)(b - a));
+    for (int64_t k = a; k < b; ++k) {
+      const int32_t v = term_vars[k];
+      if (v < 0 || v >= N) return fail(QMC_INVALID_ARGUMENT, "variable id out of range");
+      if (k > a && term_vars[k - 1] >= v)
+        return fail(QMC_DUPLICATED_VARIABLE, "term ids must be strictly increasing");
+      node_terms[v].push_back((int32_t)j);
+    }
+    const double c = coeffs[j];
+    if (!std::isfinite(c)) return fail(QMC_INVALID_ARGUMENT, "non-finite coefficient");
+    if (c != std::floor(c)) all_int = false;
+    abs_sum += std::fabs(c);
+  }
+  for (int32_t v = 0; v < N; ++v)
+    if (node_terms[v].empty()) return fail(QMC_INVALID_ARGUMENT, "variable without terms");
+  auto p = std::make_unique<qmc_problem_s>();
+  p->ctx = ctx;
+  p->kind = kind;
+  p->N = N;
+  p->M = n_terms;
+  p->coupling = (all_int && 2.0 * abs_sum < 2147483647.0) ? QMC_COUPLING_I32 : QMC_COUPLING_F32;
+  p->pairwise = max_term <= 2;
+
+  // Greedy colouring in variable order over "shares a term": members of a
+  // class can be updated simultaneously.
+  std::vector<int32_t> color(N, -1), stamp;
+  int n_colors = 0;
+  for (int32_t v = 0; v < N; ++v) {
+    for (int32_t j : node_terms[v])
+      for (int64_t k = term_offsets[j]; k < term_offsets[j + 1]; ++k) {
+        const int32_t u = term_vars[k];
+        if (u < v) {
+          const int32_t cu = color[u];
+          if ((int)stamp.size() <= cu) stamp.resize(cu + 1, -1);
+          stamp[cu] = v;
+        }
+      }
+    int cv = 0;
+    while (cv < (int)stamp.size() && stamp[cv] == v) ++cv;
+    color[v] = cv;
+    n_colors = std::max(n_colors, cv + 1);
+  }
+  std::vector<int32_t> csize(n_colors, 0);
+  for (int32_t v = 0; v < N; ++v) csize[color[v]]++;
+  p->class_off.assign(n_colors + 1, 0);
+  for (int c = 0; c < n_colors; ++c) p->class_off[c + 1] = p->class_off[c] + csize[c];
+  p->var_of_pos.assign(N, 0);
+  p->pos_of_var.assign(N, 0);
+  {
+    std::vector<int32_t> fillp(p->class_off.begin(), p->class_off.end() - 1);
+    for (int32_t v = 0; v < N; ++v) {
+      const int32_t q = fillp[color[v]]++;
+      p->var_of_pos[q] = v;
+      p->pos_of_var[v] = q;
+    }
+  }
+  const int32_t nmax = *std::max_element(csize.begin(), csize.end());
+  p->lanes = lanes ? lanes : (nmax > 512 ? 128 : 64);
+
+  // position-space incidence CSR
+  std::vector<int32_t> inc_off(N + 1, 0), nbr, oth_off(1, 0), oth_pos, coef_i;
+  std::vector<float> coef_f;
+  for (int32_t q = 0; q < N; ++q) {
+    const int32_t v = p->var_of_pos[q];
+    p->max_degree = std::max<int>(p->max_degree, (int)node_terms[v].size());
+    for (int32_t j : node_terms[v]) {
+      int32_t other = N;
+      int n_oth = 0;
+      for (int64_t k = term_offsets[j]; k < term_offsets[j + 1]; ++k) {
+        const int32_t u = term_vars[k];
+        if (u == v) continue;
+        other = p->pos_of_var[u];
+        oth_pos.push_back(other);
+        ++n_oth;
+      }
+      oth_off.push_back((int32_t)oth_pos.size());
+      nbr.push_back(n_oth <= 1 ? other : N);
+      coef_i.push_back(p->coupling == QMC_COUPLING_I32 ? (int32_t)coeffs[j] : 0);
+      coef_f.push_back((float)coeffs[j]);
+    }
+    inc_off[q + 1] = (int32_t)nbr.size();
+  }
+  p->n_inc = (int64_t)nbr.size();
+  std::vector<int64_t> toff(term_offsets, term_offsets + n_terms + 1);
+  std::vector<int32_t> tvar(term_vars, term_vars + term_offsets[n_terms]);
+  std::vector<double> tc(coeffs, coeffs + n_terms);
+  HIP_TRY(hipSetDevice(ctx->device));
+  HIP_TRY(p->d_class_off.upload(p->class_off.data(), p->class_off.size()));
+  HIP_TRY(p->d_var_of_pos.upload(p->var_of_pos.data(), N));
+  HIP_TRY(p->d_pos_of_var.upload(p->pos_of_var.data(), N));
+  HIP_TRY(p->d_inc_off.upload(inc_off.data(), inc_off.size()));
+  HIP_TRY(p->d_nbr.upload(nbr.data(), nbr.size()));
+  HIP_TRY(p->d_oth_off.upload(oth_off.data(), oth_off.size()));
+  oth_pos.push_back(0);  // never empty
+  HIP_TRY(p->d_oth_pos.upload(oth_pos.data(), oth_pos.size()));
+  if (p->coupling == QMC_COUPLING_I32) HIP_TRY(p->d_coef_i.upload(coef_i.data(), coef_i.size()));
+  else HIP_TRY(p->d_coef_f.upload(coef_f.data(), coef_f.size()));
+  HIP_TRY(p->d_term_off.upload(toff.data(), toff.size()));
+  HIP_TRY(p->d_term_var.upload(tvar.data(), tvar.size()));
+  HIP_TRY(p->d_term_c.upload(tc.data(), tc.size()));
+  *out = p.release();
+  return QMC_OK;
+}
+
+void qmc_problem_destroy(qmc_problem p) { delete p; }
+
+int qmc_problem_get_info(qmc_problem p, qmc_problem_info* info) {
+  if (!p || !info) return fail(QMC_INVALID_ARGUMENT, "null handle");
+  info->n_vars = p->N;
+  info->n_terms = p->M;
+  info->n_incidences = p->n_inc;
+  info->n_classes = (int32_t)p->class_off.size() - 1;
+  info->lanes = p->lanes;
+  info->coupling = p->coupling;
+  info->pairwise = p->pairwise;
+  info->max_degree = p->max_degree;
+  info->mean_degree = (double)p->n_inc / (double)p->N;
+  return QMC_OK;
+}
+
+int qmc_problem_order(qmc_problem p, int32_t* order, int32_t* class_offsets) {
+  if (!p) return fail(QMC_INVALID_ARGUMENT, "null handle");
+  const int nclass = (int)p->class_off.size() - 1;
+  int32_t k = 0;
+  for (int c = 0; c < nclass; ++c) {
+    const int32_t c0 = p->class_off[c], nc = p->class_off[c + 1] - c0;
+    if (class_offsets) class_offsets[c] = c0;
+    for (int32_t tp = 0; tp < p->lanes; ++tp)
+      for (int32_t rho = tp; rho < nc; rho += p->lanes)
+        if (order) order[k++] = p->var_of_pos[c0 + rho];
+  }
+  if (class_offsets) class_offsets[nclass] = p->N;
+  return QMC_OK;
+}
+
+static void build_units(qmc_walkers w) {
+  qmc_problem p = w->prob;
+  w->units.assign(w->n_units, UnitDev{});
+  for (int u = 0; u < w->n_units; ++u) {
+    UnitDev& U = w->units[u];
+    U.class_off = p->d_class_off.p;
+    U.var_of_pos = p->d_var_of_pos.p;
+    U.pos_of_var = p->d_pos_of_var.p;
+    U.inc_off = p->d_inc_off.p;
+    U.coef = p->coupling == QMC_COUPLING_I32 ? (const void*)p->d_coef_i.p : (const void*)p->d_coef_f.p;
+    U.nbr = p->d_nbr.p;
+    U.oth_off = p->d_oth_off.p;
+    U.oth_pos = p->d_oth_pos.p;
+    U.N = p->N;
+    U.n_classes = (int32_t)p->class_off.size() - 1;
+    U.lanes = p->lanes;
+    U.G = w->G;
+    U.stride = w->stride;
+    U.ladder = 0;
+    const int32_t r = u * w->B;
+    U.spins = w->spins.p + (size_t)u * w->stride;
+    U.best = w->best.p + (size_t)u * w->stride;
+    U.cost = w->cost.p + r;
+    U.best_cost = w->best_cost.p + r;
+    U.beta = w->beta.p + r;
+    U.accepted = w->accepted.p + r;
+    U.label = w->label.p + r;
+    U.ladder_beta = w->ladder_beta.p;
+    U.nrep = (uint32_t)std::min(w->B, w->R - r);
+    // replicas are grouped per seed: instance k = r / per_seed uses seed + k
+    const int32_t gr = w->first_replica + r;
+    U.seed = w->seed + (uint32_t)(gr / w->per_seed);
+    U.r0 = (uint32_t)(gr % w->per_seed);
+    U.stream = w->stream;
+  }
+}
+
+static int upload_units(qmc_walkers w) {
+  HIP_TRY(hipMemcpy(w->d_units.p, w->units.data(), w->units.size() * sizeof(UnitDev),
+                    hipMemcpyHostToDevice));
+  return QMC_OK;
+}
+
+int qmc_walkers_create(qmc_problem p, int32_t n_replicas, int32_t replicas_per_group,
+                        uint32_t seed, uint32_t stream, int32_t first_replica,
+                        int32_t replicas_per_seed, qmc_walkers* out) {
+  if (!p || !out) return fail(QMC_INVALID_ARGUMENT, "null handle");
+  if (n_replicas <= 0) return fail(QMC_INVALID_ARGUMENT, "n_replicas must be positive");
+  if (replicas_per_group < 1 || replicas_per_group > 32)
+    return fail(QMC_INVALID_ARGUMENT, "replicas_per_group must be in 1..32");
+  if (replicas_per_seed <= 0) return fail(QMC_INVALID_ARGUMENT, "replicas_per_seed must be positive");
+  auto w = std::make_unique<qmc_walkers_s>();
+  w->prob = p;
+  w->R = n_replicas;
+  w->B = replicas_per_group;
+  w->G = (w->B + 7) / 8;
+  if (p->lanes * w->G > 512)
+    return fail(QMC_INVALID_ARGUMENT, "lanes * replica bytes exceeds the 512-thread work-group");
+  w->n_units = (w->R + w->B - 1) / w->B;
+  w->stride = (int32_t)(((int64_t)(p->N + 1) * w->G + 3) / 4 * 4);
+  w->seed = seed;
+  w->stream = stream;
+  w->first_replica = first_replica;
+  w->per_seed = replicas_per_seed;
+  const size_t smem = sweep_smem_bytes(w->stride, 0);
+  if (smem > 160 * 1024) return fail(QMC_MEMORY_LIMITED, "problem too large for LDS residency at this group size");
+  HIP_TRY(hipSetDevice(p->ctx->device));
+  HIP_TRY(w->spins.alloc((size_t)w->n_units * w->stride));
+  HIP_TRY(w->best.alloc((size_t)w->n_units * w->stride));
+  HIP_TRY(hipMemset(w->spins.p, 0, w->spins.n));
+  HIP_TRY(hipMemset(w->best.p, 0, w->best.n));
+  HIP_TRY(w->cost.alloc(w->R));
+  HIP_TRY(w->best_cost.alloc(w->R));
+  HIP_TRY(w->beta.alloc(w->R));
+  HIP_TRY(w->accepted.alloc(w->R));
+  HIP_TRY(w->label.alloc(w->R));
+  HIP_TRY(w->ladder_beta.alloc(32));
+  HIP_TRY(hipMemset(w->cost.p, 0, w->R * sizeof(double)));
+  HIP_TRY(hipMemset(w->best_cost.p, 0, w->R * sizeof(double)));
+  HIP_TRY(hipMemset(w->beta.p, 0, w->R * sizeof(double)));
+  HIP_TRY(hipMemset(w->accepted.p, 0, w->R * sizeof(unsigned long long)));
+  HIP_TRY(hipMemset(w->label.p, 0, w->R * sizeof(int32_t)));
+  HIP_TRY(w->d_units.alloc(w->n_units));
+  build_units(w.get());
+  int rc = upload_units(w.get());
+  if (rc) return rc;
+  *out = w.release();
+  return QMC_OK;
+}
+
+void qmc_walkers_destroy(qmc_walkers w) {
+  if (w && w->prob) (void)hipStreamSynchronize(w->prob->ctx->stream);
+  delete w;
+}
+
+static int ensure_scratch(qmc_walkers w) {
+  const size_t need = (size_t)w->R * w->prob->N;
+  if (w->scratch_states.n < need) HIP_TRY(w->scratch_states.alloc(need));
+  if (w->energies.n < (size_t)w->R) HIP_TRY(w->energies.alloc(w->R));
+  return QMC_OK;
+}
+
+int qmc_walkers_init(qmc_walkers w, uint32_t epoch) {
+  if (!w) return fail(QMC_INVALID_ARGUMENT, "null handle");
+  qmc_problem p = w->prob;
+  hipStream_t st = p->ctx->stream;
+  HIP_TRY(launch_init(w->d_units.p, w->n_units, p->N, epoch, st));
+  int rc = ensure_scratch(w);
+  if (rc) return rc;
+  HIP_TRY(launch_unpack(w->d_units.p, w->B, w->R, p->N, 0, w->scratch_states.p, st));
+  HIP_TRY(launch_energy(p->dev(), w->scratch_states.p, w->R, w->cost.p, st));
+  HIP_TRY(hipMemcpyAsync(w->best_cost.p, w->cost.p, w->R * sizeof(double), hipMemcpyDeviceToDevice, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  return QMC_OK;
+}
+
+int qmc_walkers_set_states(qmc_walkers w, const uint8_t* states) {
+  if (!w || !states) return fail(QMC_INVALID_ARGUMENT, "null argument");
+  qmc_problem p = w->prob;
+  hipStream_t st = p->ctx->stream;
+  int rc = ensure_scratch(w);
+  if (rc) return rc;
+  const size_t n = (size_t)w->R * p->N;
+  HIP_TRY(hipMemcpyAsync(w->scratch_states.p, states, n, hipMemcpyHostToDevice, st));
+  HIP_TRY(launch_pack(w->d_units.p, w->n_units, w->B, w->R, p->N, w->scratch_states.p, 2, st));
+  HIP_TRY(launch_energy(p->dev(), w->scratch_states.p, w->R, w->cost.p, st));
+  HIP_TRY(hipMemcpyAsync(w->best_cost.p, w->cost.p, w->R * sizeof(double), hipMemcpyDeviceToDevice, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  return QMC_OK;
+}
+
+int qmc_walkers_set_betas(qmc_walkers w, const double* betas) {
+  if (!w || !betas) return fail(QMC_INVALID_ARGUMENT, "null argument");
+  HIP_TRY(hipMemcpyAsync(w->beta.p, betas, w->R * sizeof(double), hipMemcpyHostToDevice,
+                         w->prob->ctx->stream));
+  HIP_TRY(hipStreamSynchronize(w->prob->ctx->stream));
+  return QMC_OK;
+}
+
+static int do_sweep(qmc_walkers w, int64_t sweep0, int32_t n_sweeps, int beta_mode, int ordered,
+                    int spr, bool timed) {
+  qmc_problem p = w->prob;
+  qmc_ctx c = p->ctx;
+  SweepArgs a{};
+  a.sweep0 = sweep0;
+  a.n_sweeps = n_sweeps;
+  a.beta_mode = beta_mode;
+  a.sched = w->sched.p;
+  a.lut = c->lut.p;
+  a.spr = spr > 0 ? spr : 1;
+  const size_t smem = sweep_smem_bytes(w->stride, ordered);
+  if (timed) HIP_TRY(hipEventRecord(c->ev0, c->stream));
+  HIP_TRY(launch_sweep(p->kind, p->pairwise, p->coupling, w->d_units.p, w->n_units, p->lanes * w->G,
+                       smem, a, ordered, c->stream));
+  if (timed) HIP_TRY(hipEventRecord(c->ev1, c->stream));
+  return QMC_OK;
+}
+
+int qmc_walkers_sweep(qmc_walkers w, int64_t sweep0, int32_t n_sweeps, int beta_mode,
+                      const double* schedule, int ordered) {
+  if (!w) return fail(QMC_INVALID_ARGUMENT, "null handle");
+  if (n_sweeps <= 0) return QMC_OK;
+  if (beta_mode == QMC_BETA_SCHEDULE) {
+    if (!schedule) return fail(QMC_INVALID_ARGUMENT, "schedule required");
+    if (w->sched.n < (size_t)n_sweeps) HIP_TRY(w->sched.alloc(n_sweeps));
+    HIP_TRY(hipMemcpyAsync(w->sched.p, schedule, n_sweeps * sizeof(double), hipMemcpyHostToDevice,
+                           w->prob->ctx->stream));
+  }
+  if (w->ladder) {
+    for (auto& U : w->units) U.ladder = 0;
+    w->ladder = 0;
+    int rc = upload_units(w);
+    if (rc) return rc;
+  }
+  int rc = do_sweep(w, sweep0, n_sweeps, beta_mode, ordered, 1, true);
+  if (rc) return rc;
+  HIP_TRY(hipStreamSynchronize(w->prob->ctx->stream));
+  HIP_TRY(hipEventElapsedTime(&w->last_ms, w->prob->ctx->ev0, w->prob->ctx->ev1));
+  return QMC_OK;
+}
+
+int qmc_walkers_pt(qmc_walkers w, int32_t ladder, const double* ladder_betas, int64_t step0,
+                   int32_t n_steps, int32_t sweeps_per_replica) {
+  if (!w || !ladder_betas) return fail(QMC_INVALID_ARGUMENT, "null argument");
+  if (ladder < 1 || ladder > 32 || w->R % ladder != 0)
+    return fail(QMC_INVALID_ARGUMENT, "ladder must divide the replica count and be <= 32");
+  if (w->per_seed != ladder) return fail(QMC_INVALID_ARGUMENT, "create the walkers with replicas_per_seed == ladder");
+  if (sweeps_per_replica < 1) return fail(QMC_INVALID_ARGUMENT, "sweeps_per_replica must be >= 1");
+  qmc_problem p = w->prob;
+  hipStream_t st = p->ctx->stream;
+  const int n_ladders = w->R / ladder;
+  if (step0 == 0 || w->ladder != ladder) {
+    // fresh ladder: slot k holds beta index k
+    std::vector<int32_t> lab(w->R);
+    std::vector<double> be(w->R);
+    for (int r = 0; r < w->R; ++r) {
+      lab[r] = r % ladder;
+      be[r] = ladder_betas[r % ladder];
+    }
+    HIP_TRY(hipMemcpy(w->label.p, lab.data(), w->R * sizeof(int32_t), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(w->beta.p, be.data(), w->R * sizeof(double), hipMemcpyHostToDevice));
+  }
+  HIP_TRY(hipMemcpy(w->ladder_beta.p, ladder_betas, ladder * sizeof(double), hipMemcpyHostToDevice));
+  const bool in_unit = (w->B == ladder);
+  if (in_unit) {
+    if (w->ladder != ladder) {
+      for (auto& U : w->units) U.ladder = ladder;
+      w->ladder = ladder;
+      int rc = upload_units(w);
+      if (rc) return rc;
+    }
+    int rc = do_sweep(w, step0 * sweeps_per_replica, n_steps * sweeps_per_replica,
+                      QMC_BETA_PER_REPLICA, 0, sweeps_per_replica, true);
+    if (rc) return rc;
+  } else {
+    if (ladder % w->B != 0) return fail(QMC_INVALID_ARGUMENT, "ladder must be a multiple of replicas_per_group");
+    if (w->ladder) {
+      for (auto& U : w->units) U.ladder = 0;
+      int rc = upload_units(w);
+      if (rc) return rc;
+    }
+    w->ladder = ladder;
+    HIP_TRY(hipEventRecord(p->ctx->ev0, st));
+    for (int32_t k = 0; k < n_steps; ++k) {
+      const int64_t step = step0 + k;
+      int rc = do_sweep(w, step * sweeps_per_replica, sweeps_per_replica, QMC_BETA_PER_REPLICA, 0, 1, false);
+      if (rc) return rc;
+      HIP_TRY(launch_exchange(w->cost.p, w->beta.p, w->label.p, w->ladder_beta.p, n_ladders, ladder,
+                              (uint32_t)step, w->seed + (uint32_t)(w->first_replica / ladder),
+                              w->stream, 1, st));
+    }
+    HIP_TRY(hipEventRecord(p->ctx->ev1, st));
+  }
+  HIP_TRY(hipStreamSynchronize(st));
+  HIP_TRY(hipEventElapsedTime(&w->last_ms, p->ctx->ev0, p->ctx->ev1));
+  return QMC_OK;
+}
+
+int qmc_walkers_read(qmc_walkers w, double* cost, double* lowest, double* beta, uint64_t* accepted,
+                     uint8_t* states, uint8_t* lowest_states) {
+  if (!w) return fail(QMC_INVALID_ARGUMENT, "null handle");
+  qmc_problem p = w->prob;
+  hipStream_t st = p->ctx->stream;
+  HIP_TRY(hipStreamSynchronize(st));
+  if (cost) HIP_TRY(hipMemcpy(cost, w->cost.p, w->R * sizeof(double), hipMemcpyDeviceToHost));
+  if (lowest) HIP_TRY(hipMemcpy(lowest, w->best_cost.p, w->R * sizeof(double), hipMemcpyDeviceToHost));
+  if (beta) HIP_TRY(hipMemcpy(beta, w->beta.p, w->R * sizeof(double), hipMemcpyDeviceToHost));
+  if (accepted)
+    HIP_TRY(hipMemcpy(accepted, w->accepted.p, w->R * sizeof(uint64_t), hipMemcpyDeviceToHost));
+  if (states || lowest_states) {
+    int rc = ensure_scratch(w);
+    if (rc) return rc;
+    const size_t n = (size_t)w->R * p->N;
+    if (states) {
+      HIP_TRY(launch_unpack(w->d_units.p, w->B, w->R, p->N, 0, w->scratch_states.p, st));
+      HIP_TRY(hipMemcpyAsync(states, w->scratch_states.p, n, hipMemcpyDeviceToHost, st));
+      HIP_TRY(hipStreamSynchronize(st));
+    }
+    if (lowest_states) {
+      HIP_TRY(launch_unpack(w->d_units.p, w->B, w->R, p->N, 1, w->scratch_states.p, st));
+      HIP_TRY(hipMemcpyAsync(lowest_states, w->scratch_states.p, n, hipMemcpyDeviceToHost, st));
+      HIP_TRY(hipStreamSynchronize(st));
+    }
+  }
+  return QMC_OK;
+}
+
+int qmc_walkers_energy(qmc_walkers w, int lowest, double* energies) {
+  if (!w || !energies) return fail(QMC_INVALID_ARGUMENT, "null argument");
+  qmc_problem p = w->prob;
+  hipStream_t st = p->ctx->stream;
+  int rc = ensure_scratch(w);
+  if (rc) return rc;
+  HIP_TRY(launch_unpack(w->d_units.p, w->B, w->R, p->N, lowest ? 1 : 0, w->scratch_states.p, st));
+  HIP_TRY(launch_energy(p->dev(), w->scratch_states.p, w->R, w->energies.p, st));
+  HIP_TRY(hipMemcpyAsync(energies, w->energies.p, w->R * sizeof(double), hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  return QMC_OK;
+}
+
+int qmc_walkers_labels(qmc_walkers w, int32_t* labels) {
+  if (!w || !labels) return fail(QMC_INVALID_ARGUMENT, "null argument");
+  HIP_TRY(hipStreamSynchronize(w->prob->ctx->stream));
+  HIP_TRY(hipMemcpy(labels, w->label.p, w->R * sizeof(int32_t), hipMemcpyDeviceToHost));
+  return QMC_OK;
+}
+
+int qmc_walkers_last_ms(qmc_walkers w, double* ms) {
+  if (!w || !ms) return fail(QMC_INVALID_ARGUMENT, "null argument");
+  *ms = w->last_ms;
+  return QMC_OK;
+}
+
+int qmc_energy(qmc_problem p, int32_t n_states, const uint8_t* states, double* energies) {
+  if (!p || !states || !energies || n_states < 0) return fail(QMC_INVALID_ARGUMENT, "bad argument");
+  if (n_states == 0) return QMC_OK;
+  hipStream_t st = p->ctx->stream;
+  DevBuf<uint8_t> ds;
+  DevBuf<double> de;
+  HIP_TRY(ds.upload(states, (size_t)n_states * p->N));
+  HIP_TRY(de.alloc(n_states));
+  HIP_TRY(launch_energy(p->dev(), ds.p, n_states, de.p, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  HIP_TRY(hipMemcpy(energies, de.p, n_states * sizeof(double), hipMemcpyDeviceToHost));
+  return QMC_OK;
+}
+
+int qmc_delta_all(qmc_problem p, int32_t n_states, const uint8_t* states, double* out) {
+  if (!p || !states || !out || n_states < 0) return fail(QMC_INVALID_ARGUMENT, "bad argument");
+  if (n_states == 0) return QMC_OK;
+  hipStream_t st = p->ctx->stream;
+  DevBuf<uint8_t> ds;
+  DevBuf<double> d;
+  HIP_TRY(ds.upload(states, (size_t)n_states * p->N));
+  HIP_TRY(d.alloc((size_t)n_states * p->N));
+  HIP_TRY(launch_delta_all(p->dev(), n_states, ds.p, d.p, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  HIP_TRY(hipMemcpy(out, d.p, (size_t)n_states * p->N * sizeof(double), hipMemcpyDeviceToHost));
+  return QMC_OK;
+}
+
+int qmc_philox(qmc_ctx ctx, int32_t n, const uint32_t* ctr, const uint32_t key[2], uint32_t* out) {
+  if (!ctx || !ctr || !key || !out || n < 0) return fail(QMC_INVALID_ARGUMENT, "bad argument");
+  if (n == 0) return QMC_OK;
+  DevBuf<uint32_t> dc, dout;
+  HIP_TRY(dc.upload(ctr, (size_t)4 * n));
+  HIP_TRY(dout.alloc((size_t)4 * n));
+  HIP_TRY(launch_philox(n, dc.p, key[0], key[1], dout.p, ctx->stream));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  HIP_TRY(hipMemcpy(out, dout.p, (size_t)16 * n, hipMemcpyDeviceToHost));
+  return QMC_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,88 @@
+// Internal structures shared by the HIP kernels and the C-ABI implementation.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/qmc.h"
+
+namespace qmc {
+
+// Device copy of a problem in position space (see qmc_kernels.hip).
+struct ProblemDev {
+  int32_t N;
+  int32_t n_classes;
+  int32_t lanes;
+  int32_t kind;
+  int32_t pairwise;
+  int32_t coupling;
+  int64_t M;
+  const int32_t* class_off;   // n_classes + 1
+  const int32_t* var_of_pos;  // N
+  const int32_t* pos_of_var;  // N
+  const int32_t* inc_off;     // N + 1, incident entries of each position (term order)
+  const void* coef;           // int32 or float per entry
+  const int32_t* nbr;         // pairwise: the other member's position (N = ghost)
+  const int32_t* oth_off;     // entries + 1
+  const int32_t* oth_pos;     // other members' positions
+  const int64_t* term_off;    // M + 1 (variable ids, for FP64 energies)
+  const int32_t* term_var;
+  const double* term_c;
+};
+
+// One work-group's share: up to 32 replicas of one problem.
+struct UnitDev {
+  // problem
+  const int32_t* class_off;
+  const int32_t* var_of_pos;
+  const int32_t* pos_of_var;
+  const int32_t* inc_off;
+  const void* coef;
+  const int32_t* nbr;
+  const int32_t* oth_off;
+  const int32_t* oth_pos;
+  int32_t N, n_classes, lanes, G;
+  int32_t stride;   // bytes of one unit's spin array, multiple of 4
+  int32_t ladder;   // > 0: in-unit PT ladder of this many replicas
+  // state (unit-local pointers: element 0 is the unit's first replica)
+  uint8_t* spins;
+  uint8_t* best;
+  double* cost;
+  double* best_cost;
+  double* beta;
+  unsigned long long* accepted;
+  int32_t* label;
+  const double* ladder_beta;
+  uint32_t r0;      // replica id of the unit's first replica (RNG counter)
+  uint32_t nrep;
+  uint32_t seed, stream;
+};
+
+struct SweepArgs {
+  int64_t sweep0;
+  int32_t n_sweeps;
+  int32_t beta_mode;
+  const double* sched;   // device, n_sweeps values (schedule mode)
+  const double* lut;     // device, 1024 values
+  int32_t spr;           // PT: sweeps per exchange step
+  int32_t pad;
+};
+
+size_t sweep_smem_bytes(int max_stride, int ordered);
+hipError_t launch_sweep(int kind, int pairwise, int coupling, const UnitDev* units, int n_units,
+                        int block, size_t smem, const SweepArgs& a, int ordered, hipStream_t st);
+hipError_t launch_exchange(double* cost, double* beta, int32_t* label, const double* lbeta,
+                           int n_ladders, int L, uint32_t step, uint32_t seed, uint32_t stream,
+                           int seed_per_ladder, hipStream_t st);
+hipError_t launch_init(const UnitDev* units, int n_units, int N, uint32_t epoch, hipStream_t st);
+hipError_t launch_unpack(const UnitDev* units, int B, int R, int N, int which, uint8_t* out,
+                         hipStream_t st);
+hipError_t launch_pack(const UnitDev* units, int n_units, int B, int R, int N, const uint8_t* in,
+                       int which, hipStream_t st);
+hipError_t launch_energy(const ProblemDev& P, const uint8_t* states, int n, double* out,
+                         hipStream_t st);
+hipError_t launch_delta_all(const ProblemDev& P, int n_states, const uint8_t* states, double* out,
+                            hipStream_t st);
+hipError_t launch_philox(int n, const uint32_t* ctr, uint32_t k0, uint32_t k1, uint32_t* out,
+                         hipStream_t st);
+
+}  // namespace qmc

@@ -1,0 +1,752 @@
+// MI355X (gfx950) kernels of the Metropolis sweep engine.
+//
+// Data layout (per problem, in "position space"): variables are renumbered so
+// that every colour class is a contiguous range of positions; inside a class
+// the positions follow ascending variable id.  Position N is a ghost that is
+// always 0, so one-variable terms need no special case.
+//
+// Per walker group ("unit"): up to 32 replicas of one problem share a
+// work-group.  Spins are packed bits, byte g of position q holding replicas
+// 8g..8g+7 of the unit: spins[q*G + g].  Thread (g, tp) owns byte lane g and
+// positions rho = k*lanes + tp of every class, so a wavefront reads
+// consecutive bytes (conflict-free LDS) and one neighbour byte serves eight
+// replicas.
+//
+// Sweep order: classes in order; inside a class, position-lanes tp in order,
+// each lane's positions k = 0,1,.. in order.  Members of one class share no
+// term, so the parallel update equals that sequential order exactly.  The
+// reference's per-flip best tracking (markov/walker.h:277-280) is kept exact:
+// each thread accumulates the prefix sum S and minimum prefix m of its
+// accepted dE; a wave scan + min-reduction gives the minimum over the class's
+// sequence and where it is attained, and the lowest state is rebuilt from the
+// pre-class state and the flips recorded in LDS.
+#include <hip/hip_runtime.h>
+
+#include "device_common.h"
+#include "qmc_internal.h"
+
+namespace qmc {
+
+constexpr int kMaxRep = 32;          // replicas per unit
+constexpr int kMaxWavesPerGroup = 4; // lanes <= 256
+constexpr int kBlock = 512;
+
+template <typename CT>
+__device__ __forceinline__ CT min_ct(CT a, CT b) { return b < a ? b : a; }
+
+struct Smem {
+  double lut[1024];
+  double rcost[kMaxRep];
+  double rbest[kMaxRep];
+  double rbeta[kMaxRep];
+  double lbeta[kMaxRep];
+  int32_t label[kMaxRep];
+  uint32_t racc[kMaxRep];
+  int32_t nb_tp[kMaxRep];   // thread lane holding the new-best argmin, -1 none
+  int32_t kstar[kMaxRep];
+  int32_t any_new_best;
+  int32_t pad_[3];
+  // reduction scratch [group][replica-in-group][wave]
+  float wsum_f[4][8][kMaxWavesPerGroup];
+  float wmin_f[4][8][kMaxWavesPerGroup];
+  int32_t wsum_i[4][8][kMaxWavesPerGroup];
+  int32_t wmin_i[4][8][kMaxWavesPerGroup];
+  int32_t wlane[4][8][kMaxWavesPerGroup];
+};
+
+template <typename CT> struct RedSlots;
+template <> struct RedSlots<float> {
+  __device__ static float* sum(Smem& s, int g, int b) { return s.wsum_f[g][b]; }
+  __device__ static float* mn(Smem& s, int g, int b) { return s.wmin_f[g][b]; }
+};
+template <> struct RedSlots<int32_t> {
+  __device__ static int32_t* sum(Smem& s, int g, int b) { return s.wsum_i[g][b]; }
+  __device__ static int32_t* mn(Smem& s, int g, int b) { return s.wmin_i[g][b]; }
+};
+
+// Local field of the 8 replicas of byte lane g at position q:
+//   Ising: h_b = sum_e c_e * (-1)^(parity of the other members, replica b)
+//   PUBO : h_b = sum_e c_e * [all other members are 1 in replica b]
+// Sums run over the variable's incident terms in term order, as the
+// reference's calculate_cost_difference does.
+template <int KIND, bool PAIR, typename CT>
+__device__ __forceinline__ void local_field(const UnitDev& U, const uint8_t* sp, int q, int G,
+                                            int g, CT (&h)[8]) {
+#pragma unroll
+  for (int b = 0; b < 8; ++b) h[b] = CT(0);
+  const CT* coef = static_cast<const CT*>(U.coef);
+  const int e0 = U.inc_off[q], e1 = U.inc_off[q + 1];
+  for (int e = e0; e < e1; ++e) {
+    const CT cc = coef[e];
+    uint32_t w;
+    if (PAIR) {
+      w = sp[U.nbr[e] * G + g];
+    } else {
+      w = 0;
+      const int o0 = U.oth_off[e], o1 = U.oth_off[e + 1];
+      for (int o = o0; o < o1; ++o) {
+        const uint32_t x = sp[U.oth_pos[o] * G + g];
+        w = (KIND == QMC_ISING) ? (w ^ x) : (w | x);
+      }
+    }
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      const bool bit = (w >> b) & 1u;
+      if (KIND == QMC_ISING) h[b] += bit ? -cc : cc;
+      else h[b] += bit ? CT(0) : cc;
+    }
+  }
+}
+
+template <int KIND, typename CT>
+__device__ __forceinline__ CT delta_from_field(CT h, uint32_t own_bit) {
+  if (KIND == QMC_ISING) return own_bit ? CT(2) * h : CT(-2) * h;
+  return own_bit ? h : -h;
+}
+
+// Single-replica dE recomputation (rare path: locating the lowest state).
+template <int KIND, bool PAIR, typename CT>
+__device__ CT delta_one(const UnitDev& U, const uint8_t* sp, int q, int G, int g, int b,
+                        uint32_t own_bit) {
+  CT h[8];
+  local_field<KIND, PAIR, CT>(U, sp, q, G, g, h);
+  CT hb = h[0];
+#pragma unroll
+  for (int k = 1; k < 8; ++k)
+    if (k == b) hb = h[k];
+  return delta_from_field<KIND, CT>(hb, own_bit);
+}
+
+template <int KIND, bool PAIR, typename CT>
+__global__ void __launch_bounds__(kBlock)
+k_sweep(const UnitDev* __restrict__ units, SweepArgs a) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  Smem& S = *reinterpret_cast<Smem*>(smem_raw);
+  const UnitDev U = units[blockIdx.x];
+  const int N = U.N, G = U.G, TP = U.lanes;
+  const int stride = U.stride;
+  uint8_t* sp = smem_raw + sizeof(Smem);
+  uint8_t* fl = sp + stride;
+  const int tid = threadIdx.x;
+  const int nthreads = blockDim.x;
+  const bool active = tid < TP * G;
+  const int g = active ? tid / TP : 0;
+  const int tp = tid - g * TP;
+  const int lane = tid & 63;
+  const int wave_in_group = tp >> 6;
+  const int nrep = (int)U.nrep;
+  const int nb_g = active ? min(8, nrep - 8 * g) : 0;
+  const uint32_t rbase = U.r0 + 8u * (uint32_t)g;
+
+  for (int i = tid; i < 1024; i += nthreads) S.lut[i] = a.lut[i];
+  {
+    const uint32_t* src = reinterpret_cast<const uint32_t*>(U.spins);
+    uint32_t* dst = reinterpret_cast<uint32_t*>(sp);
+    for (int i = tid; i < stride / 4; i += nthreads) dst[i] = src[i];
+  }
+  if (tid < nrep) {
+    S.rcost[tid] = U.cost[tid];
+    S.rbest[tid] = U.best_cost[tid];
+    S.rbeta[tid] = U.beta[tid];
+    S.racc[tid] = 0;
+    if (U.ladder > 0) {
+      S.label[tid] = U.label[tid];
+      S.lbeta[tid] = U.ladder_beta[tid];
+    }
+  }
+  __syncthreads();
+
+  uint32_t acc[8];
+#pragma unroll
+  for (int b = 0; b < 8; ++b) acc[b] = 0;
+
+  for (int s = 0; s < a.n_sweeps; ++s) {
+    const uint32_t sweep = (uint32_t)(a.sweep0 + s);
+    if (a.beta_mode == QMC_BETA_SCHEDULE && tid < nrep) S.rbeta[tid] = a.sched[s];
+    __syncthreads();
+    double beta[8];
+#pragma unroll
+    for (int b = 0; b < 8; ++b) beta[b] = (b < nb_g) ? S.rbeta[8 * g + b] : 0.0;
+
+    for (int c = 0; c < U.n_classes; ++c) {
+      const int c0 = U.class_off[c];
+      const int nc = U.class_off[c + 1] - c0;
+      const int K = (nc + TP - 1) / TP;
+      CT Ssum[8], Smin[8];
+#pragma unroll
+      for (int b = 0; b < 8; ++b) {
+        Ssum[b] = CT(0);
+        Smin[b] = CT(0);
+      }
+      if (active) {
+        for (int k = 0; k < K; ++k) {
+          const int rho = k * TP + tp;
+          if (rho >= nc) break;
+          const int q = c0 + rho;
+          const uint32_t var = (uint32_t)U.var_of_pos[q];
+          const uint32_t own = sp[q * G + g];
+          CT h[8];
+          local_field<KIND, PAIR, CT>(U, sp, q, G, g, h);
+          uint32_t flips = 0;
+          uint4 quad = make_uint4(0, 0, 0, 0);
+#pragma unroll
+          for (int b = 0; b < 8; ++b) {
+            if (b < nb_g) {
+              const uint32_t r = rbase + (uint32_t)b;
+              if (b == 0 || (r & 3u) == 0u)
+                quad = philox4x32_10(var, sweep, r >> 2, DOM_SWEEP, U.seed, U.stream);
+              const uint32_t ob = (own >> b) & 1u;
+              const CT dE = delta_from_field<KIND, CT>(h[b], ob);
+              const bool ok = metropolis_accept((double)dE, beta[b], pick(quad, r & 3u), S.lut, [&]() {
+                return philox4x32_10(var, sweep, r, DOM_REDRAW, U.seed, U.stream);
+              });
+              if (ok) {
+                flips |= 1u << b;
+                Ssum[b] += dE;
+                Smin[b] = min_ct(Smin[b], Ssum[b]);
+                acc[b]++;
+              }
+            }
+          }
+          sp[q * G + g] = (uint8_t)(own ^ flips);
+          fl[q * G + g] = (uint8_t)flips;
+        }
+      }
+      __syncthreads();
+
+      // ---- per replica: sequence minimum of the cost over this class ----
+      if (active) {
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+          if (b < nb_g) {
+            CT incl = Ssum[b];
+#pragma unroll
+            for (int off = 1; off < 64; off <<= 1) {
+              const CT t = __shfl_up(incl, off, 64);
+              if (lane >= off) incl += t;
+            }
+            CT excl = __shfl_up(incl, 1, 64);
+            if (lane == 0) excl = CT(0);
+            CT mv = excl + Smin[b];
+            int ml = lane;
+#pragma unroll
+            for (int off = 32; off >= 1; off >>= 1) {
+              const CT ov = __shfl_xor(mv, off, 64);
+              const int ol = __shfl_xor(ml, off, 64);
+              if (ov < mv || (ov == mv && ol < ml)) {
+                mv = ov;
+                ml = ol;
+              }
+            }
+            const CT tot = __shfl(incl, 63, 64);
+            if (lane == 0) {
+              RedSlots<CT>::sum(S, g, b)[wave_in_group] = tot;
+              RedSlots<CT>::mn(S, g, b)[wave_in_group] = mv;
+              S.wlane[g][b][wave_in_group] = ml;
+            }
+          }
+        }
+      }
+      __syncthreads();
+      if (tid < nrep) {
+        const int gg = tid >> 3, bb = tid & 7;
+        const int W = TP >> 6;
+        CT run = CT(0);
+        CT bestc = CT(0);
+        int bt = -1;
+        for (int w = 0; w < W; ++w) {
+          const CT cnd = run + RedSlots<CT>::mn(S, gg, bb)[w];
+          if (bt < 0 || cnd < bestc) {
+            bestc = cnd;
+            bt = w * 64 + S.wlane[gg][bb][w];
+          }
+          run += RedSlots<CT>::sum(S, gg, bb)[w];
+        }
+        const double before = S.rcost[tid];
+        S.rcost[tid] = before + (double)run;
+        const double cand = before + (double)bestc;
+        if (cand < S.rbest[tid]) {
+          S.rbest[tid] = cand;
+          S.nb_tp[tid] = bt;
+          S.any_new_best = 1;
+        } else {
+          S.nb_tp[tid] = -1;
+        }
+      }
+      __syncthreads();
+      if (S.any_new_best) {
+        // locate k* in the owning thread, then rebuild the lowest states
+        if (active) {
+#pragma unroll
+          for (int b = 0; b < 8; ++b) {
+            if (b < nb_g && S.nb_tp[8 * g + b] == tp) {
+              CT pre = CT(0);
+              int ks = -1;  // Smin == 0: the empty prefix is the first minimum
+              if (Smin[b] < CT(0)) {
+                for (int k = 0; k < K; ++k) {
+                  const int rho = k * TP + tp;
+                  if (rho >= nc) break;
+                  const int q = c0 + rho;
+                  const uint32_t f = fl[q * G + g];
+                  if ((f >> b) & 1u) {
+                    const uint32_t ob = ((sp[q * G + g] ^ f) >> b) & 1u;  // pre-flip bit
+                    pre += delta_one<KIND, PAIR, CT>(U, sp, q, G, g, b, ob);
+                    if (pre == Smin[b]) {
+                      ks = k;
+                      break;
+                    }
+                  }
+                }
+              }
+              S.kstar[8 * g + b] = ks;
+            }
+          }
+        }
+        __syncthreads();
+        if (active) {
+          uint32_t newmask = 0;
+#pragma unroll
+          for (int b = 0; b < 8; ++b)
+            if (b < nb_g && S.nb_tp[8 * g + b] >= 0) newmask |= 1u << b;
+          if (newmask) {
+            for (int q = tp; q < N; q += TP) {
+              const uint32_t cur = sp[q * G + g];
+              uint32_t val = cur;
+              const int rho = q - c0;
+              if (rho >= 0 && rho < nc) {
+                const uint32_t f = fl[q * G + g];
+                const int tq = rho % TP, kq = rho / TP;
+#pragma unroll
+                for (int b = 0; b < 8; ++b) {
+                  if ((newmask >> b) & 1u) {
+                    const int ts = S.nb_tp[8 * g + b], ks = S.kstar[8 * g + b];
+                    const bool after = tq > ts || (tq == ts && kq > ks);
+                    if (after) val ^= f & (1u << b);
+                  }
+                }
+              }
+              const uint32_t old = U.best[q * G + g];
+              U.best[q * G + g] = (uint8_t)((old & ~newmask) | (val & newmask));
+            }
+          }
+        }
+        __syncthreads();
+        if (tid == 0) S.any_new_best = 0;
+      }
+    }  // classes
+
+    // ---- parallel tempering exchange inside the unit (ladder == nrep) ----
+    if (U.ladder > 0 && ((uint32_t)(s + 1) % (uint32_t)a.spr) == 0u) {
+      __syncthreads();
+      const uint32_t step = (uint32_t)((a.sweep0 + s) / a.spr);
+      const int L = U.ladder;
+      const int i = (int)(step & 1u) + 2 * tid;
+      if (i < L - 1) {
+        const int sa = S.label[i], sb = S.label[i + 1];
+        const uint4 w = philox4x32_10((uint32_t)i, step, 0u, DOM_EXCHANGE, U.seed, U.stream);
+        const double u = canonical2(w.x, w.y);
+        const double bi = S.lbeta[i], bj = S.lbeta[i + 1];
+        const double arg = __dmul_rn(__dsub_rn(S.rcost[sa], S.rcost[sb]), __dsub_rn(bi, bj));
+        if (u < exp(arg)) {
+          S.label[i] = sb;
+          S.label[i + 1] = sa;
+          S.rbeta[sa] = bj;
+          S.rbeta[sb] = bi;
+        }
+      }
+    }
+    __syncthreads();
+  }  // sweeps
+
+  // ---- write back ----
+  if (active) {
+#pragma unroll
+    for (int b = 0; b < 8; ++b)
+      if (b < nb_g && acc[b]) atomicAdd(&S.racc[8 * g + b], acc[b]);
+  }
+  {
+    const uint32_t* src = reinterpret_cast<const uint32_t*>(sp);
+    uint32_t* dst = reinterpret_cast<uint32_t*>(U.spins);
+    for (int i = tid; i < stride / 4; i += nthreads) dst[i] = src[i];
+  }
+  __syncthreads();
+  if (tid < nrep) {
+    U.cost[tid] = S.rcost[tid];
+    U.best_cost[tid] = S.rbest[tid];
+    U.beta[tid] = S.rbeta[tid];
+    U.accepted[tid] += S.racc[tid];
+    if (U.ladder > 0) U.label[tid] = S.label[tid];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Reference-order sweep: one lane per replica, variables i = 0..N-1 in index
+// order (markov/walker.h:144-153).  Graph reads are wave-uniform.  The
+// lowest state uses the ordered-sweep merge: the position of the sweep's
+// running minimum selects end-of-sweep bits up to it and start-of-sweep bits
+// after it (each variable is touched once per sweep).
+template <int KIND, bool PAIR, typename CT>
+__global__ void __launch_bounds__(64)
+k_sweep_ordered(const UnitDev* __restrict__ units, SweepArgs a) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  double* lut = reinterpret_cast<double*>(smem_raw);
+  uint8_t* sp = smem_raw + 8192;            // unit layout, stride bytes
+  const UnitDev U = units[blockIdx.x];
+  const int G = U.G, N = U.N, stride = U.stride;
+  uint8_t* start = sp + stride;             // start-of-sweep copy
+  const int lane = threadIdx.x;
+  const int nrep = (int)U.nrep;
+  for (int i = lane; i < 1024; i += 64) lut[i] = a.lut[i];
+  for (int i = lane; i < stride / 4; i += 64)
+    reinterpret_cast<uint32_t*>(sp)[i] = reinterpret_cast<const uint32_t*>(U.spins)[i];
+  __syncthreads();
+  const bool act = lane < nrep;
+  const int graw = lane >> 3;
+  const int g = min(graw, G - 1), b = lane & 7;
+  const bool byte_writer = (b == 0) && (graw < G) && (8 * graw < nrep);
+  const uint32_t r = U.r0 + (uint32_t)lane;
+  double cost = act ? U.cost[lane] : 0.0;
+  double best = act ? U.best_cost[lane] : 0.0;
+  double beta = act ? U.beta[lane] : 0.0;
+  unsigned long long accn = 0;
+  const CT* coef = static_cast<const CT*>(U.coef);
+  for (int s = 0; s < a.n_sweeps; ++s) {
+    const uint32_t sweep = (uint32_t)(a.sweep0 + s);
+    if (a.beta_mode == QMC_BETA_SCHEDULE) beta = a.sched[s];
+    for (int i = lane; i < stride / 4; i += 64)
+      reinterpret_cast<uint32_t*>(start)[i] = reinterpret_cast<uint32_t*>(sp)[i];
+    __syncthreads();
+    double run = cost;
+    double mincost = best;
+    int kmin = -1;
+    for (int i = 0; i < N; ++i) {
+      const int q = U.pos_of_var[i];
+      const int e0 = U.inc_off[q], e1 = U.inc_off[q + 1];
+      CT h = CT(0);
+      for (int e = e0; e < e1; ++e) {
+        const CT cc = coef[e];
+        uint32_t bit;
+        if (PAIR) {
+          bit = (sp[U.nbr[e] * G + g] >> b) & 1u;
+        } else {
+          bit = 0;
+          for (int o = U.oth_off[e]; o < U.oth_off[e + 1]; ++o) {
+            const uint32_t x = (sp[U.oth_pos[o] * G + g] >> b) & 1u;
+            bit = (KIND == QMC_ISING) ? (bit ^ x) : (bit | x);
+          }
+        }
+        if (KIND == QMC_ISING) h += bit ? -cc : cc;
+        else h += bit ? CT(0) : cc;
+      }
+      const uint32_t own = (sp[q * G + g] >> b) & 1u;
+      const CT dE = delta_from_field<KIND, CT>(h, own);
+      bool ok = false;
+      if (act) {
+        const uint4 quad = philox4x32_10((uint32_t)i, sweep, r >> 2, DOM_SWEEP, U.seed, U.stream);
+        ok = metropolis_accept((double)dE, beta, pick(quad, r & 3u), lut, [&]() {
+          return philox4x32_10((uint32_t)i, sweep, r, DOM_REDRAW, U.seed, U.stream);
+        });
+      }
+      // one writer per byte: lanes 8g..8g+7 combine their flips
+      const unsigned long long bal = __ballot(ok);
+      if (ok) {
+        run += (double)dE;
+        accn++;
+        if (run < mincost) {
+          mincost = run;
+          kmin = i;
+        }
+      }
+      if (byte_writer) {
+        const uint32_t fm = (uint32_t)((bal >> (8 * g)) & 0xffu);
+        sp[q * G + g] ^= (uint8_t)fm;
+      }
+      __syncthreads();
+    }
+    cost = run;
+    // lowest state: bits of variables <= kmin from the end state, others from the start
+    const unsigned long long need = __ballot(act && kmin >= 0);
+    if (need) {
+      for (int q = 0; q < N; ++q) {
+        // every lane handles its own bit; bytes assembled by lane 8g
+        const int i = U.var_of_pos[q];
+        const uint32_t cb = (sp[q * G + g] >> b) & 1u;
+        const uint32_t sb = (start[q * G + g] >> b) & 1u;
+        const bool mine = act && kmin >= 0;
+        const uint32_t v = (i <= kmin) ? cb : sb;
+        const unsigned long long vb = __ballot(mine && v);
+        const unsigned long long mb = __ballot(mine);
+        if (byte_writer) {
+          const uint32_t m8 = (uint32_t)((mb >> (8 * g)) & 0xffu);
+          const uint32_t v8 = (uint32_t)((vb >> (8 * g)) & 0xffu);
+          if (m8) {
+            const uint32_t old = U.best[q * G + g];
+            U.best[q * G + g] = (uint8_t)((old & ~m8) | (v8 & m8));
+          }
+        }
+      }
+      if (act && kmin >= 0) best = mincost;
+    }
+    __syncthreads();
+  }
+  for (int i = lane; i < stride / 4; i += 64)
+    reinterpret_cast<uint32_t*>(U.spins)[i] = reinterpret_cast<uint32_t*>(sp)[i];
+  if (act) {
+    U.cost[lane] = cost;
+    U.best_cost[lane] = best;
+    U.beta[lane] = beta;
+    U.accepted[lane] += accn;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// PT exchange across units (ladders spanning several work-groups).
+// One thread per ladder pair.  costs/betas/labels are per global replica.
+__global__ void k_exchange(double* __restrict__ cost, double* __restrict__ beta,
+                           int32_t* __restrict__ label, const double* __restrict__ lbeta,
+                           int n_ladders, int L, uint32_t step, uint32_t seed0, uint32_t stream,
+                           int seed_per_ladder) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int pairs = L / 2;
+  if (t >= n_ladders * pairs) return;
+  const int ld = t / pairs, p = t - ld * pairs;
+  const int i = (int)(step & 1u) + 2 * p;
+  if (i >= L - 1) return;
+  int32_t* lab = label + ld * L;
+  const int sa = ld * L + lab[i], sb = ld * L + lab[i + 1];
+  const uint32_t seed = seed_per_ladder ? seed0 + (uint32_t)ld : seed0;
+  const uint4 w = philox4x32_10((uint32_t)i, step, 0u, DOM_EXCHANGE, seed, stream);
+  const double u = canonical2(w.x, w.y);
+  const double bi = lbeta[i], bj = lbeta[i + 1];
+  const double arg = __dmul_rn(__dsub_rn(cost[sa], cost[sb]), __dsub_rn(bi, bj));
+  if (u < exp(arg)) {
+    lab[i] = sb - ld * L;
+    lab[i + 1] = sa - ld * L;
+    beta[sa] = bj;
+    beta[sb] = bi;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Initial random states: flip variable i of replica r if bit i%32 of word
+// i/32 is set (model/ising.h:23-41), words from the INIT counter domain.
+__global__ void k_init_states(const UnitDev* __restrict__ units, uint32_t epoch) {
+  const UnitDev U = units[blockIdx.y];
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q > U.N) return;
+  for (int g = 0; g < U.G; ++g) {
+    uint32_t byte = 0;
+    if (q < U.N) {
+      const uint32_t i = (uint32_t)U.var_of_pos[q];
+      const uint32_t wi = i >> 5;
+      for (int b = 0; b < 8; ++b) {
+        const int rr = 8 * g + b;
+        if (rr < (int)U.nrep) {
+          const uint32_t r = U.r0 + (uint32_t)rr;
+          const uint4 w = philox4x32_10(wi >> 2, r, epoch, DOM_INIT, U.seed, U.stream);
+          const uint32_t word = pick(w, wi & 3u);
+          byte |= ((word >> (i & 31u)) & 1u) << b;
+        }
+      }
+    }
+    U.spins[q * U.G + g] = (uint8_t)byte;
+    U.best[q * U.G + g] = (uint8_t)byte;
+  }
+}
+
+// plain states [r][var] <-> unit bytes
+__global__ void k_unpack(const UnitDev* __restrict__ units, int B, int R, int N, int which,
+                         uint8_t* __restrict__ out) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)R * N) return;
+  const int r = (int)(t / N), v = (int)(t - (int64_t)r * N);
+  const UnitDev& U = units[r / B];
+  const int b = r % B;
+  const int q = U.pos_of_var[v];
+  const uint8_t* src = which ? U.best : U.spins;
+  out[t] = (src[q * U.G + (b >> 3)] >> (b & 7)) & 1u;
+}
+
+__global__ void k_pack(const UnitDev* __restrict__ units, int B, int R, const uint8_t* __restrict__ in,
+                       int which) {
+  const UnitDev U = units[blockIdx.y];
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q > U.N) return;
+  const int u = blockIdx.y;
+  for (int g = 0; g < U.G; ++g) {
+    uint32_t byte = 0;
+    if (q < U.N) {
+      const int v = U.var_of_pos[q];
+      for (int b = 0; b < 8; ++b) {
+        const int rr = 8 * g + b;
+        const int r = u * B + rr;
+        if (rr < (int)U.nrep && r < R) byte |= ((uint32_t)(in[(int64_t)r * U.N + v] & 1u)) << b;
+      }
+    }
+    if (which == 0 || which == 2) U.spins[q * U.G + g] = (uint8_t)byte;
+    if (which == 1 || which == 2) U.best[q * U.G + g] = (uint8_t)byte;
+  }
+}
+
+// FP64 energy of plain states [s][var]: term CSR in variable ids.
+__global__ void k_energy(const int64_t* __restrict__ term_off, const int32_t* __restrict__ term_var,
+                         const double* __restrict__ term_c, int64_t M, int kind, int N,
+                         const uint8_t* __restrict__ states, double* __restrict__ out) {
+  __shared__ double part[256];
+  const uint8_t* st = states + (int64_t)blockIdx.x * N;
+  double acc = 0.0;
+  for (int64_t j = threadIdx.x; j < M; j += blockDim.x) {
+    uint32_t x = 0;
+    for (int64_t k = term_off[j]; k < term_off[j + 1]; ++k) {
+      const uint32_t bit = st[term_var[k]] & 1u;
+      x = (kind == QMC_ISING) ? (x ^ bit) : (x | bit);
+    }
+    const double c = term_c[j];
+    if (kind == QMC_ISING) acc += x ? -c : c;
+    else if (!x) acc += c;
+  }
+  part[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[blockIdx.x] = part[0];
+}
+
+// dE of every variable for plain states, with the sweep kernels' arithmetic.
+template <int KIND, typename CT>
+__global__ void k_delta_all(const ProblemDev P, int n_states, const uint8_t* __restrict__ states,
+                            double* __restrict__ out) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)n_states * P.N) return;
+  const int s = (int)(t / P.N), i = (int)(t - (int64_t)s * P.N);
+  const uint8_t* st = states + (int64_t)s * P.N;
+  const int q = P.pos_of_var[i];
+  const CT* coef = static_cast<const CT*>(P.coef);
+  CT h = CT(0);
+  for (int e = P.inc_off[q]; e < P.inc_off[q + 1]; ++e) {
+    uint32_t bit = 0;
+    for (int o = P.oth_off[e]; o < P.oth_off[e + 1]; ++o) {
+      const uint32_t x = st[P.var_of_pos[P.oth_pos[o]]] & 1u;
+      bit = (KIND == QMC_ISING) ? (bit ^ x) : (bit | x);
+    }
+    const CT cc = coef[e];
+    if (KIND == QMC_ISING) h += bit ? -cc : cc;
+    else h += bit ? CT(0) : cc;
+  }
+  out[t] = (double)delta_from_field<KIND, CT>(h, st[i] & 1u);
+}
+
+__global__ void k_philox(int n, const uint32_t* __restrict__ ctr, uint32_t k0, uint32_t k1,
+                         uint32_t* __restrict__ out) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const uint4 w = philox4x32_10(ctr[4 * t], ctr[4 * t + 1], ctr[4 * t + 2], ctr[4 * t + 3], k0, k1);
+  out[4 * t] = w.x;
+  out[4 * t + 1] = w.y;
+  out[4 * t + 2] = w.z;
+  out[4 * t + 3] = w.w;
+}
+
+// ---------------------------------------------------------------------------
+// launch helpers (host side of this translation unit)
+size_t sweep_smem_bytes(int max_stride, int ordered) {
+  if (ordered) return 8192 + 2 * (size_t)max_stride;
+  return sizeof(Smem) + 2 * (size_t)max_stride;
+}
+
+template <int KIND, bool PAIR, typename CT>
+static hipError_t launch_sweep_t(const UnitDev* units, int n_units, int block, size_t smem,
+                                 const SweepArgs& a, int ordered, hipStream_t st) {
+  if (ordered) {
+    auto fn = k_sweep_ordered<KIND, PAIR, CT>;
+    hipError_t e = hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(fn, dim3(n_units), dim3(64), smem, st, units, a);
+  } else {
+    auto fn = k_sweep<KIND, PAIR, CT>;
+    hipError_t e = hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(fn, dim3(n_units), dim3(block), smem, st, units, a);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_sweep(int kind, int pairwise, int coupling, const UnitDev* units, int n_units,
+                        int block, size_t smem, const SweepArgs& a, int ordered, hipStream_t st) {
+  if (kind == QMC_ISING) {
+    if (pairwise) {
+      return coupling == QMC_COUPLING_I32 ? launch_sweep_t<QMC_ISING, true, int32_t>(units, n_units, block, smem, a, ordered, st)
+                                          : launch_sweep_t<QMC_ISING, true, float>(units, n_units, block, smem, a, ordered, st);
+    }
+    return coupling == QMC_COUPLING_I32 ? launch_sweep_t<QMC_ISING, false, int32_t>(units, n_units, block, smem, a, ordered, st)
+                                        : launch_sweep_t<QMC_ISING, false, float>(units, n_units, block, smem, a, ordered, st);
+  }
+  if (pairwise) {
+    return coupling == QMC_COUPLING_I32 ? launch_sweep_t<QMC_PUBO, true, int32_t>(units, n_units, block, smem, a, ordered, st)
+                                        : launch_sweep_t<QMC_PUBO, true, float>(units, n_units, block, smem, a, ordered, st);
+  }
+  return coupling == QMC_COUPLING_I32 ? launch_sweep_t<QMC_PUBO, false, int32_t>(units, n_units, block, smem, a, ordered, st)
+                                      : launch_sweep_t<QMC_PUBO, false, float>(units, n_units, block, smem, a, ordered, st);
+}
+
+hipError_t launch_exchange(double* cost, double* beta, int32_t* label, const double* lbeta,
+                           int n_ladders, int L, uint32_t step, uint32_t seed, uint32_t stream,
+                           int seed_per_ladder, hipStream_t st) {
+  const int n = n_ladders * (L / 2);
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_exchange, dim3((n + 127) / 128), dim3(128), 0, st, cost, beta, label, lbeta,
+                     n_ladders, L, step, seed, stream, seed_per_ladder);
+  return hipGetLastError();
+}
+
+hipError_t launch_init(const UnitDev* units, int n_units, int N, uint32_t epoch, hipStream_t st) {
+  hipLaunchKernelGGL(k_init_states, dim3((N + 1 + 127) / 128, n_units), dim3(128), 0, st, units, epoch);
+  return hipGetLastError();
+}
+
+hipError_t launch_unpack(const UnitDev* units, int B, int R, int N, int which, uint8_t* out,
+                         hipStream_t st) {
+  const int64_t n = (int64_t)R * N;
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_unpack, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, units, B, R, N, which, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_pack(const UnitDev* units, int n_units, int B, int R, int N, const uint8_t* in,
+                       int which, hipStream_t st) {
+  hipLaunchKernelGGL(k_pack, dim3((N + 1 + 127) / 128, n_units), dim3(128), 0, st, units, B, R, in, which);
+  return hipGetLastError();
+}
+
+hipError_t launch_energy(const ProblemDev& P, const uint8_t* states, int n, double* out, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_energy, dim3(n), dim3(256), 0, st, P.term_off, P.term_var, P.term_c, P.M,
+                     P.kind, P.N, states, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_delta_all(const ProblemDev& P, int n_states, const uint8_t* states, double* out,
+                            hipStream_t st) {
+  const int64_t n = (int64_t)n_states * P.N;
+  if (n == 0) return hipSuccess;
+  dim3 grid((unsigned)((n + 255) / 256));
+  if (P.kind == QMC_ISING) {
+    if (P.coupling == QMC_COUPLING_I32) hipLaunchKernelGGL((k_delta_all<QMC_ISING, int32_t>), grid, dim3(256), 0, st, P, n_states, states, out);
+    else hipLaunchKernelGGL((k_delta_all<QMC_ISING, float>), grid, dim3(256), 0, st, P, n_states, states, out);
+  } else {
+    if (P.coupling == QMC_COUPLING_I32) hipLaunchKernelGGL((k_delta_all<QMC_PUBO, int32_t>), grid, dim3(256), 0, st, P, n_states, states, out);
+    else hipLaunchKernelGGL((k_delta_all<QMC_PUBO, float>), grid, dim3(256), 0, st, P, n_states, states, out);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_philox(int n, const uint32_t* ctr, uint32_t k0, uint32_t k1, uint32_t* out,
+                         hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_philox, dim3((n + 255) / 256), dim3(256), 0, st, n, ctr, k0, k1, out);
+  return hipGetLastError();
+}
+
+}  // namespace qmc

@@ -1,0 +1,109 @@
+"""Seeded synthetic problems in the reference's JSON term format
+(``{"c": coefficient, "ids": [variable ids]}``, cpp/graph/cost_edge.h:26,
+cpp/graph/edge.cc:43).
+
+* ``lattice2d``   square L x L periodic Ising, J in {-1,+1}, h = 0
+* ``lattice3d``   cubic L^3 periodic Ising spin glass, J ~ N(0,1) rounded to fp32
+* ``random_pubo`` PUBO with given term-degree mix, ids without replacement per
+                  term, integer coefficients uniform in [-cmax, cmax] \\ {0}
+* ``er_ising``    Erdos-Renyi Ising, J uniform int in [-jmax, jmax] \\ {0}, every
+                  variable gets a one-variable field term (h uniform in {-1,0,1},
+                  zero kept so no variable vanishes)
+* ``er_ising_f32`` Erdos-Renyi Ising with J ~ N(0,1) fp32, h = 0
+"""
+import numpy as np
+
+
+def _f32(x):
+    return float(np.float32(x))
+
+
+def lattice2d(L=32, seed=0):
+    rng = np.random.default_rng(seed)
+    terms = []
+    for y in range(L):
+        for x in range(L):
+            i = x + L * y
+            right = (x + 1) % L + L * y
+            down = x + L * ((y + 1) % L)
+            terms.append({"c": int(rng.choice([-1, 1])), "ids": [i, right]})
+            terms.append({"c": int(rng.choice([-1, 1])), "ids": [i, down]})
+    return terms
+
+
+def lattice3d(L=16, seed=0):
+    rng = np.random.default_rng(seed)
+    terms = []
+    for z in range(L):
+        for y in range(L):
+            for x in range(L):
+                i = x + L * y + L * L * z
+                for nb in ((x + 1) % L + L * y + L * L * z,
+                           x + L * ((y + 1) % L) + L * L * z,
+                           x + L * y + L * L * ((z + 1) % L)):
+                    terms.append({"c": _f32(rng.standard_normal()), "ids": [i, nb]})
+    return terms
+
+
+def random_pubo(n_vars=2000, n_terms=20000, fractions=(0.1, 0.4, 0.3, 0.2), cmax=10, seed=0):
+    rng = np.random.default_rng(seed)
+    counts = [int(round(f * n_terms)) for f in fractions]
+    counts[-1] = n_terms - sum(counts[:-1])
+    vals = np.array([v for v in range(-cmax, cmax + 1) if v != 0])
+    terms = []
+    for deg, cnt in enumerate(counts, start=1):
+        for _ in range(cnt):
+            ids = rng.choice(n_vars, size=deg, replace=False)
+            terms.append({"c": int(rng.choice(vals)), "ids": [int(v) for v in ids]})
+    order = rng.permutation(len(terms))
+    return [terms[k] for k in order]
+
+
+def er_ising(N=256, mean_degree=6.0, jmax=3, seed=0, field=True):
+    rng = np.random.default_rng(seed)
+    p = min(1.0, mean_degree / max(1, N - 1))
+    terms = []
+    if field:
+        for i in range(N):
+            terms.append({"c": int(rng.integers(-1, 2)), "ids": [i]})
+    vals = np.array([v for v in range(-jmax, jmax + 1) if v != 0])
+    iu, ju = np.triu_indices(N, k=1)
+    mask = rng.random(len(iu)) < p
+    for i, j in zip(iu[mask], ju[mask]):
+        terms.append({"c": int(rng.choice(vals)), "ids": [int(i), int(j)]})
+    return terms
+
+
+def er_ising_f32(N=256, mean_degree=8.0, seed=0):
+    rng = np.random.default_rng(seed)
+    p = min(1.0, mean_degree / max(1, N - 1))
+    # sample edges without the O(N^2) mask for large N
+    m = rng.binomial(N * (N - 1) // 2, p)
+    seen = set()
+    terms = []
+    while len(terms) < m:
+        i, j = rng.integers(0, N, size=2)
+        if i == j:
+            continue
+        a, b = (int(i), int(j)) if i < j else (int(j), int(i))
+        if (a, b) in seen:
+            continue
+        seen.add((a, b))
+        terms.append({"c": _f32(rng.standard_normal()), "ids": [a, b]})
+    # make sure every variable appears
+    present = np.zeros(N, dtype=bool)
+    for t in terms:
+        present[t["ids"]] = True
+    for i in np.nonzero(~present)[0]:
+        terms.append({"c": 0.0, "ids": [int(i)]})
+    return terms
+
+
+def log_uniform_sizes(n, lo, hi, seed=0):
+    rng = np.random.default_rng(seed)
+    return np.rint(np.exp(rng.uniform(np.log(lo), np.log(hi), size=n))).astype(np.int64)
+
+
+def geometric_temperatures(t0=0.2, t1=2.0, count=32):
+    """Geometric schedule discretised to `count` values (schedule/geometric.cc:9-12)."""
+    return [t0 * (t1 / t0) ** (i / (count - 1)) for i in range(count)]

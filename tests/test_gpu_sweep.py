@@ -1,0 +1,159 @@
+"""GPU parity: the HIP sweep engine against the CPU oracle, through the C ABI.
+
+Integer couplings: bit-exact states, costs, lowest costs and lowest states
+against the oracle run in the same visit order with the same counter-based
+RNG contract.  FP32 couplings: identical trajectories (the oracle sums the
+fp32 coefficients in the same order) and energies within 1e-5 relative.
+"""
+import numpy as np
+import pytest
+
+from oracle import pyoracle as po
+from qiotoolkit_amd import _qmc, workloads as wl
+from tests._helpers import arith_of, oracle_initial, pair
+
+pytestmark = pytest.mark.gpu
+
+KAT = [
+    ([0, 0, 0, 0], [0, 0], [0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8]),
+    ([0xFFFFFFFF] * 4, [0xFFFFFFFF] * 2, [0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD]),
+    ([0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344], [0xA4093822, 0x299F31D0],
+     [0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1]),
+]
+
+
+def test_philox_device_matches_kat_and_oracle(ctx):
+    for c, k, want in KAT:
+        got = ctx.philox([c], k)[0]
+        assert list(got) == want
+    rng = np.random.default_rng(1)
+    ctrs = rng.integers(0, 2**32, size=(512, 4), dtype=np.uint64).astype(np.uint32)
+    key = [123456789, 987654321]
+    got = ctx.philox(ctrs, key)
+    want = np.stack([po.philox(c, key) for c in ctrs])
+    assert np.array_equal(got, want)
+
+
+PROBLEMS = {
+    "lattice2d_pm1": ("ising", lambda: wl.lattice2d(8, seed=3)),
+    "er_int": ("ising", lambda: wl.er_ising(150, 6.0, seed=4)),
+    "pubo_int": ("pubo", lambda: wl.random_pubo(120, 600, seed=5)),
+    "ising_4local": ("ising", lambda: [dict(c=int(c), ids=ids) for c, ids in zip(
+        np.random.default_rng(6).integers(-5, 6, 300),
+        [sorted(np.random.default_rng(100 + k).choice(40, size=1 + k % 4, replace=False).tolist())
+         for k in range(300)]) if c != 0]),
+    "lattice3d_f32": ("ising", lambda: wl.lattice3d(6, seed=7)),
+}
+
+
+@pytest.mark.parametrize("name", list(PROBLEMS))
+def test_energy_and_delta_match_oracle(ctx, name):
+    kind, gen = PROBLEMS[name]
+    op, qp = pair(ctx, kind, gen())
+    rng = np.random.default_rng(11)
+    states = rng.integers(0, 2, size=(16, op.n), dtype=np.uint8)
+    e_dev = qp.energy(states)
+    e_ref = np.array([op.cost(s) for s in states])
+    if qp.info.coupling == _qmc.QMC_COUPLING_I32:
+        assert np.array_equal(e_dev, e_ref)
+    else:
+        np.testing.assert_allclose(e_dev, e_ref, rtol=1e-12, atol=1e-9)
+    d_dev = qp.delta_all(states)
+    ar = arith_of(qp)
+    d_ref = np.array([[op.delta(s, i, ar) for i in range(op.n)] for s in states])
+    assert np.array_equal(d_dev, d_ref)
+    # the GPU's dE agrees with the reference's FP64 arithmetic within 1e-5 relative
+    d64 = np.array([[op.delta(s, i, po.ARITH_F64) for i in range(op.n)] for s in states])
+    np.testing.assert_allclose(d_dev, d64, rtol=1e-5, atol=1e-5 * np.abs(d64).max())
+
+
+def _run_pair(ctx, kind, terms, R, B, betas, seed=7, ordered=False, lanes=0):
+    op, qp = pair(ctx, kind, terms, lanes=lanes)
+    order, co = qp.order()
+    assert op.check_classes(order, co)
+    w = _qmc.Walkers(qp, R, B, seed=seed)
+    w.init()
+    r0 = w.read()
+    opts = po.make_opts(po.RNG_COUNTER, seed=seed, arith=arith_of(qp),
+                        order=None if ordered else order)
+    init = oracle_initial(op, opts, R)
+    assert np.array_equal(r0["states"], init)
+    costs = np.array([op.cost(s) for s in init])
+    if qp.info.coupling == _qmc.QMC_COUPLING_I32:
+        assert np.array_equal(r0["cost"], costs)
+    states = init.copy()
+    lowest = costs.copy()
+    lstates = init.copy()
+    w.sweep(0, schedule=betas, ordered=ordered)
+    r1 = w.read()
+    acc = op.run_sweeps(opts, states, costs, lowest, lstates, np.arange(R), 0,
+                        np.repeat(np.asarray(betas, dtype=np.float64), R))
+    return op, qp, w, r1, dict(states=states, cost=costs, lowest=lowest, lowest_states=lstates,
+                               accepted=acc)
+
+
+@pytest.mark.parametrize("name", ["lattice2d_pm1", "er_int", "pubo_int", "ising_4local"])
+@pytest.mark.parametrize("B", [1, 5, 8, 32])
+def test_sa_sweeps_bit_exact_int(ctx, name, B):
+    kind, gen = PROBLEMS[name]
+    betas = np.linspace(0.1, 3.0, 24)
+    op, qp, w, got, want = _run_pair(ctx, kind, gen(), R=37, B=B, betas=betas)
+    assert qp.info.coupling == _qmc.QMC_COUPLING_I32
+    assert np.array_equal(got["states"], want["states"])
+    assert np.array_equal(got["cost"], want["cost"])
+    assert np.array_equal(got["lowest"], want["lowest"])
+    assert np.array_equal(got["lowest_states"], want["lowest_states"])
+    assert np.array_equal(got["accepted"], want["accepted"])
+    # every returned configuration's energy equals the oracle cost function
+    assert np.array_equal(w.energy(lowest=True), np.array([op.cost(s) for s in got["lowest_states"]]))
+
+
+@pytest.mark.parametrize("name", ["lattice2d_pm1", "er_int", "pubo_int"])
+def test_ordered_sweeps_bit_exact_int(ctx, name):
+    kind, gen = PROBLEMS[name]
+    betas = np.linspace(0.1, 3.0, 12)
+    op, qp, w, got, want = _run_pair(ctx, kind, gen(), R=19, B=8, betas=betas, ordered=True)
+    assert np.array_equal(got["states"], want["states"])
+    assert np.array_equal(got["cost"], want["cost"])
+    assert np.array_equal(got["lowest"], want["lowest"])
+    assert np.array_equal(got["lowest_states"], want["lowest_states"])
+
+
+def test_sa_sweeps_f32(ctx):
+    kind, gen = PROBLEMS["lattice3d_f32"]
+    betas = np.linspace(0.2, 3.0, 16)
+    op, qp, w, got, want = _run_pair(ctx, kind, gen(), R=16, B=16, betas=betas)
+    assert qp.info.coupling == _qmc.QMC_COUPLING_F32
+    assert np.array_equal(got["states"], want["states"])
+    e = np.array([op.cost(s) for s in got["lowest_states"]])
+    np.testing.assert_allclose(w.energy(lowest=True), e, rtol=1e-12)
+    np.testing.assert_allclose(got["lowest"], e, rtol=1e-5)
+    np.testing.assert_allclose(got["cost"], want["cost"], rtol=1e-5)
+
+
+@pytest.mark.parametrize("name,B,L", [("er_int", 8, 8), ("lattice2d_pm1", 16, 16),
+                                      ("er_int", 4, 8), ("pubo_int", 8, 16)])
+def test_pt_matches_oracle(ctx, name, B, L):
+    kind, gen = PROBLEMS[name]
+    terms = gen()
+    op, qp = pair(ctx, kind, terms)
+    order, _ = qp.order()
+    temps = wl.geometric_temperatures(0.3, 3.0, L)
+    betas = sorted([1.0 / t for t in temps], reverse=True)
+    n_ladders = 3
+    steps = 40
+    w = _qmc.Walkers(qp, L * n_ladders, B, seed=21, replicas_per_seed=L)
+    w.init()
+    w.pt(betas, 0, steps)
+    got = w.read()
+    labels = w.labels()
+    for ld in range(n_ladders):
+        opts = po.make_opts(po.RNG_COUNTER, seed=21 + ld, arith=arith_of(qp), order=order,
+                            steps_per_tick=1)
+        want = op.pt(opts, temps, False, steps)
+        sl = slice(ld * L, (ld + 1) * L)
+        assert np.array_equal(got["states"][sl], want["final_states"])
+        assert np.array_equal(got["cost"][sl], want["final_costs"])
+        assert np.array_equal(got["lowest"][sl], want["lowest"])
+        assert np.array_equal(got["lowest_states"][sl], want["lowest_states"])
+        assert np.allclose(got["beta"][sl], want["slot_beta"])

@@ -1,0 +1,321 @@
+"""Pin the CPU oracle before trusting it (runs on CPU).
+
+Each check is independent of the oracle's own formulas: reference-held golden
+vectors (utils/test/random_generator_test.cc), published known-answer vectors
+(Random123 Philox), the reference's model/schedule/solver test answers, brute
+force with the textbook energy formula, and planted optima shipped with the
+reference (data/input_e2e_pipeline/*/energies.json).
+"""
+import itertools
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+from oracle import pyoracle as po
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def load(name):
+    with open(os.path.join(GOLD, name)) as f:
+        return json.load(f)["cost_function"]
+
+
+# ---------------------------------------------------------------- RNG
+def pct(xs):
+    return [int(x * 100) for x in xs]
+
+
+def test_pcg_default_seed():  # random_generator_test.cc:38-44
+    assert pct(po.pcg_uniform(3)) == [9, 83, 5]
+
+
+def test_pcg_custom_seed():  # random_generator_test.cc:46-57
+    assert pct(po.pcg_uniform(3, 42)) == [41, 26, 40]
+
+
+def test_pcg_fork():  # random_generator_test.cc:59-78
+    f = po.pcg_fork_uniform(42, 3, 12)
+    assert pct(f[0]) == [95, 79, 48, 70, 2, 89, 70, 10, 27, 66, 47, 18]
+    assert pct(f[1]) == [86, 95, 8, 65, 36, 92, 17, 14, 82, 9, 90, 19]
+    assert pct(f[2]) == [79, 27, 7, 13, 39, 85, 26, 66, 89, 5, 55, 13]
+    assert pct(f[3]) == [14, 82, 1, 6, 90, 27, 67, 82, 94, 24, 76, 35]
+
+
+def test_philox_known_answers():  # Random123 kat_vectors, philox4x32_10
+    assert list(po.philox([0, 0, 0, 0], [0, 0])) == [0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8]
+    assert list(po.philox([0xFFFFFFFF] * 4, [0xFFFFFFFF] * 2)) == [0x408F276D, 0x41C83B0E,
+                                                                   0xA20BC7C6, 0x6D5451FD]
+    assert list(po.philox([0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344],
+                          [0xA4093822, 0x299F31D0])) == [0xD16CFE09, 0x94FDCCEB, 0x5001E420,
+                                                         0x24126EA1]
+
+
+def test_poisson_moments():
+    # parity unpinned: libstdc++ std::poisson_distribution used as the reference
+    # uses it; only its distribution is checked here.
+    import ctypes as C
+    out = np.zeros(20000, dtype=np.uint32)
+    po.lib().orc_pcg_poisson(7, 3.5, len(out), out.ctypes.data_as(C.POINTER(C.c_uint32)))
+    assert abs(out.mean() - 3.5) < 0.06 and abs(out.var() - 3.5) < 0.15
+
+
+# ---------------------------------------------------------------- LUT accept
+def test_uniform_ln_closed_form():  # utils/utils.cc:12-32
+    t = po.uniform_ln()
+    for r in range(4):
+        assert t[r, 0] == -math.inf
+        for i in (1, 2, 100, 255):
+            assert t[r, i] == math.log(i / 256.0 ** (r + 1))
+
+
+def test_accept_rules():  # markov/metropolis.h:64-104
+    assert po.accept(0.0, 1.0, []) == (True, 0)          # downhill: no draw
+    assert po.accept(-3.0, 1.0, []) == (True, 0)
+    assert po.accept(1.0, 1.0, [1]) == (False, 1)        # bucket 1: ln(0) -> NaN -> reject
+    assert po.accept(1.0, 1.0, [0, 0, 0, 0]) == (True, 4)  # four zero buckets accept
+    assert po.accept(1e9, 1.0, [0, 0, 0, 0]) == (True, 4)
+    assert po.accept(1.0, 0.0, [200]) == (True, 1)       # beta 0 accepts any uphill move
+    assert po.accept(1.0, math.inf, [200]) == (False, 1)  # T = 0 rejects uphill
+
+
+def test_accept_exhaustive_cells():
+    """Every (row, bucket) cell and both ends of the interpolation, against the closed form."""
+    t = po.uniform_ln()
+    D = float(0xFFFFFE)
+    for row in range(4):
+        for bucket in range(1, 256):
+            for hi in (0, 0x7FFFFF, 0xFFFFFF):
+                r = (hi << 8) | bucket
+                y0, y1 = t[row, bucket - 1], t[row, bucket]
+                with np.errstate(invalid="ignore"):
+                    value = y0 + (hi / D) * (y1 - y0)
+                for bde in (1e-3, 0.5, 3.0, 5.5, 12.0):
+                    draws = [0] * row + [r]
+                    acc, used = po.accept(bde, 1.0, draws)
+                    assert used == row + 1
+                    assert acc == bool(value < -bde)
+
+
+def test_accept_probability_approximates_boltzmann():
+    rng = np.random.default_rng(5)
+    words = rng.integers(0, 2**32, size=(40000, 4), dtype=np.uint64).astype(np.uint32)
+    for bde in (0.1, 1.0, 3.0):
+        p = np.mean([po.accept(bde, 1.0, w)[0] for w in words])
+        assert abs(p - math.exp(-bde)) < 0.01
+
+
+# ---------------------------------------------------------------- models
+def ring(n, c=1):
+    return [{"c": c, "ids": [i, (i + 1) % n]} for i in range(n)]
+
+
+def test_ising_ring10_known_answers():  # ising_term_cached_test.cc:61-76
+    p = po.Problem("ising", ring(10))
+    s = np.zeros(10, dtype=np.uint8)
+    assert p.cost(s) == 10
+    assert p.delta(s, 0) == -4
+    s[0] = 1
+    assert p.cost(s) == 6
+    assert p.delta(s, 0) == 4
+
+
+def test_pubo_ring10_known_answers():  # pubo_with_counter_test.cc:59-74
+    p = po.Problem("pubo", ring(10))
+    s = np.zeros(10, dtype=np.uint8)
+    assert p.cost(s) == 10
+    assert p.delta(s, 0) == -2
+    s[0] = 1
+    assert p.cost(s) == 8
+    assert p.delta(s, 0) == 2
+
+
+def test_single_spin_terms():  # ising_test.cc:572-618
+    p = po.Problem("ising", [{"c": 1, "ids": [i]} for i in range(7)])
+    s = np.ones(7, dtype=np.uint8)
+    assert p.cost(s) == -7
+    for i in range(7):
+        assert p.delta(s, i) == 2
+        s[i] = 0
+        assert p.cost(s) == (i + 1) * 2 - 7
+
+
+def test_constant_terms_and_names():  # ising_test.cc:284-360, pubo_with_counter_test.cc:206-244
+    p = po.Problem("ising", [{"c": 1, "ids": [-1, 2]}, {"c": 2, "ids": [2, 100]}, {"c": 200, "ids": []}])
+    assert list(p.names()) == [-1, 2, 100]
+    assert p.const_cost == 200
+    p = po.Problem("pubo", [{"c": 1, "ids": []}, {"c": 2, "ids": []}])
+    assert p.n == 0 and p.const_cost == 3
+
+
+def test_duplicate_ids():  # graph.h:131-138; pubo.h:82 merges
+    with pytest.raises(ValueError, match="Duplicated"):
+        po.Problem("ising", [{"c": 1, "ids": [0, 0]}])
+    p = po.Problem("pubo", [{"c": 3, "ids": [4, 4, 5]}])
+    off, ids, c = p.edges()
+    assert list(ids) == [0, 1] and p.n == 2
+
+
+def test_empty_terms_rejected():  # graph.h:88-92
+    with pytest.raises(ValueError, match="greater than 0"):
+        po.Problem("ising", [])
+
+
+def textbook_energy(kind, terms, names, bits):
+    """H = sum_j c_j prod_{i in j} s_i, s = +1/-1 (Ising) or x = 1/0 (PUBO)."""
+    pos = {n: k for k, n in enumerate(names)}
+    e = 0.0
+    for t in terms:
+        if kind == "ising":
+            prod = 1
+            for i in set(t["ids"]):
+                prod *= -1 if bits[pos[i]] else 1
+        else:
+            prod = 1
+            for i in set(t["ids"]):
+                prod *= 0 if bits[pos[i]] else 1
+        e += t["c"] * prod
+    return e
+
+
+@pytest.mark.parametrize("kind", ["ising", "pubo"])
+def test_brute_force_cost_and_delta(kind):
+    rng = np.random.default_rng(3 if kind == "ising" else 4)
+    terms = []
+    for k in range(40):
+        size = int(rng.integers(1, 5))
+        ids = rng.choice(10, size=size, replace=False).tolist()
+        terms.append({"c": int(rng.integers(-7, 8)), "ids": ids})
+    terms.append({"c": 5, "ids": []})
+    p = po.Problem(kind, terms)
+    names = p.names()
+    n = p.n
+    for bits in itertools.product([0, 1], repeat=n):
+        b = np.array(bits, dtype=np.uint8)
+        want = textbook_energy(kind, terms, names, b) - p.const_cost
+        got = p.cost(b)
+        assert got == want
+        i = int(rng.integers(0, n))
+        b2 = b.copy()
+        b2[i] ^= 1
+        assert p.delta(b, i) == textbook_energy(kind, terms, names, b2) - textbook_energy(kind, terms, names, b)
+
+
+def test_f32_delta_is_fp32_rounding_of_f64():
+    rng = np.random.default_rng(9)
+    terms = [{"c": float(np.float32(rng.standard_normal())), "ids": [int(a), int(b)]}
+             for a, b in rng.integers(0, 30, size=(120, 2)) if a != b]
+    p = po.Problem("ising", terms)
+    s = rng.integers(0, 2, size=p.n, dtype=np.uint8)
+    for i in range(p.n):
+        d64 = p.delta(s, i, po.ARITH_F64)
+        d32 = p.delta(s, i, po.ARITH_F32)
+        assert abs(d32 - d64) <= 1e-5 * max(1.0, abs(d64))
+
+
+# ---------------------------------------------------------------- schedules
+def test_linear_schedule_known_values():  # schedule/test/linear_test.cc:16-61
+    v = lambda x: po.lib().orc_schedule_value(0, 3, 5, -1, -1, x)
+    assert [v(-1), v(0), v(0.5), v(1), v(2)] == [3, 3, 4, 5, 5]
+    v = lambda x: po.lib().orc_schedule_value(0, 5, 3, 2, 4, x)
+    assert [v(1), v(2), v(3), v(4), v(5)] == [5, 5, 4, 3, 3]
+
+
+def test_geometric_schedule_known_values():  # schedule/test/geometric_test.cc:16-124
+    v = lambda x: po.lib().orc_schedule_value(1, 3, 6, -1, -1, x)
+    assert v(0) == 3 and v(1) == 6 and abs(v(0.5) - 3 * 2 ** 0.5) < 1e-15
+    import ctypes as C
+    out = np.zeros(100)
+    po.lib().orc_schedule_discretized(1, 10, 1, 100, out.ctypes.data_as(C.POINTER(C.c_double)))
+    for k, want in [(1, 9.7700995729922528), (5, 8.9021508544503831), (12, 7.5646332755462842),
+                    (24, 5.7223676593502102), (37, 4.2292428743894899), (45, 3.5111917342151227),
+                    (52, 2.9836472402833309), (67, 2.1049041445120129), (70, 1.9630406500402637),
+                    (82, 1.4849682622544584), (95, 1.0974987654930506), (99, 0.9999999999999947)]:
+        assert abs(out[k] - want) < 1e-14
+
+
+# ---------------------------------------------------------------- solvers
+FIXTURES = [  # (file, kind, expected cost, expected configuration subset)
+    ("ising1.json", -10, None),
+    ("cpupubojobstest.json", -12, None),
+    ("isingemptymixed.json", -100, {1: -1, 10: 1}),
+    ("puboemptymixed.json", -104, {1: 1, 12: 1}),
+    ("isingempty.json", 12, None),
+    ("puboempty.json", 2, None),
+]
+
+
+def render(kind, names, bits):
+    if kind == "ising":
+        return {int(n): (-1 if b else 1) for n, b in zip(names, bits)}
+    return {int(n): (0 if b else 1) for n, b in zip(names, bits)}
+
+
+@pytest.mark.parametrize("fname,cost,config", FIXTURES)
+@pytest.mark.parametrize("solver", ["sa", "pt", "pa"])
+def test_solver_fixtures(fname, cost, config, solver):
+    # simulated_annealing_test.cc:194-260, parallel_tempering_test.cc:235-281,
+    # population_annealing_test.cc:243-329 with data/params1.json
+    params = json.load(open(os.path.join(GOLD, "params1.json")))["params"]
+    cf = load(fname)
+    p = po.Problem(cf["type"], cf["terms"])
+    o = po.make_opts(po.RNG_REFERENCE, seed=params["seed"], threads=1)
+    if solver == "sa":
+        r = p.sa(o, params["beta_start"], params["beta_stop"], params["sweeps"], params["restarts"])
+        k = int(np.argmin(r["lowest"])) if p.n else 0
+        best, state = (r["lowest"][k] if p.n else 0.0), r["lowest_states"][k]
+    elif solver == "pt":
+        r = p.pt(o, params["all_betas"], True, params["sweeps"])
+        k = int(np.argmin(r["lowest"])) if p.n else 0
+        best, state = (r["lowest"][k] if p.n else 0.0), r["lowest_states"][k]
+    else:
+        r = p.pa(o, params["beta_start"], params["beta_stop"], params["sweeps"], 16)
+        best, state = (r["best"] if p.n else 0.0), r["best_state"]
+    assert best + p.const_cost == cost
+    if config:
+        got = render(cf["type"], p.names(), state)
+        for key, val in config.items():
+            assert got[key] == val
+
+
+@pytest.mark.parametrize("family", ["wishart_planting_N_16_alpha_0.81_qualitytest",
+                                    "combined_4_local_N_16_qualitytest"])
+def test_planted_optima(family):
+    cases = json.load(open(os.path.join(GOLD, "e2e", family, "energies.json")))["cases"]
+    for case in cases:
+        cf = json.load(open(os.path.join(GOLD, "e2e", family, case["file"])))["cost_function"]
+        p = po.Problem(cf["type"], cf["terms"])
+        o = po.make_opts(po.RNG_REFERENCE, seed=22)
+        scale = max(abs(t["c"]) for t in cf["terms"])
+        r = p.sa(o, 0.1 / scale, 10.0 / scale, 300, 32)
+        assert r["lowest"].min() + p.const_cost == case["energy"]
+        # the optimum is a true minimum: exhaustive check on N = 16
+        best = np.inf
+        for bits in itertools.product([0, 1], repeat=p.n):
+            best = min(best, p.cost(np.array(bits, dtype=np.uint8)))
+            if best + p.const_cost < case["energy"]:
+                raise AssertionError("planted energy is not minimal")
+        assert best + p.const_cost == case["energy"]
+
+
+def test_counter_mode_initial_state_layout():
+    """Counter-mode initial state: bit i%32 of Philox INIT word i/32 (model/ising.h:23-41)."""
+    p = po.Problem("ising", ring(70))
+    o = po.make_opts(po.RNG_COUNTER, seed=99, stream=3)
+    s = p.initial_state(o, replica=5, epoch=2)
+    for i in range(70):
+        wi = i // 32
+        w = po.philox([wi >> 2, 5, 2, 2], [99, 3])[wi & 3]
+        assert s[i] == ((int(w) >> (i % 32)) & 1)
+
+
+def test_steps_per_tick_minimum():
+    # stepping_solver.h:112-268: the first tick always runs steps_per_tick steps
+    p = po.Problem("ising", ring(6))
+    r = p.sa(po.make_opts(seed=1), 0.1, 1.0, 3, 2)
+    assert r["steps"] == 10
+    r = p.sa(po.make_opts(seed=1), 0.1, 1.0, 37, 2)
+    assert r["steps"] == 37
