@@ -1,0 +1,61 @@
+/* qiotk.h -- C ABI of the host solver library (libqiotk.so).
+ *
+ * The request/response surface of the reference's Runner and CLI
+ * (app/runner.cc:35-429, 513-688; app/qiotoolkit.cc:128-222): a solver
+ * target, its params JSON ({"params": {...}}) and the cost function JSON
+ * ({"cost_function": {...}}), answered with {"benchmark": {...},
+ * "solutions": {...}}.  Errors return the reference's codes
+ * (utils/error_handling.h:25-52) with a message.
+ *
+ * Targets: simulatedannealing.qiotoolkit, paralleltempering.qiotoolkit,
+ * populationannealing.cpu (app/runner.cc:137-148).
+ */
+#ifndef QIOTK_H
+#define QIOTK_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct qiotk_engine_s* qiotk_engine;
+typedef struct qiotk_model_s* qiotk_model;
+
+/* one engine per device; owns the device context */
+int qiotk_engine_create(int device, qiotk_engine* out);
+void qiotk_engine_destroy(qiotk_engine e);
+
+/* One solve: Runner::configure + Runner::get_run_output.  *response is
+ * malloc'ed JSON (free with qiotk_free); on error it holds
+ * {"error": {"code": c, "message": m}}. */
+int qiotk_solve(qiotk_engine e, const char* target, const char* params_json,
+                const char* input_json, char** response);
+
+/* A model parsed, normalised and resident on the device, reused by many
+ * solves (timed solves then exclude ingest and upload). */
+int qiotk_model_create(qiotk_engine e, const char* input_json, qiotk_model* out);
+void qiotk_model_destroy(qiotk_model m);
+int qiotk_solve_model(qiotk_model m, const char* target, const char* params_json, char** response);
+
+/* Many independent solves of one target on resident models, fused into
+ * shared device launches (simulated annealing and parallel tempering):
+ * models[k] with params_json[k].  responses[k] as for qiotk_solve; *device_ms
+ * = summed device time of the fused launches. */
+int qiotk_solve_batch(qiotk_engine e, const char* target, int32_t n, const qiotk_model* models,
+                      const char* const* params_json, char** responses, double* device_ms);
+
+/* Device-free host logic, for tooling and tests. */
+/* graph::normalize_edges result as JSON: names, term CSR, const_cost, ... */
+int qiotk_parse_model(const char* input_json, char** response);
+/* Schedule::get_value at n points; *discretized = {"start","stop","count","values"} */
+int qiotk_schedule_eval(const char* schedule_json, int32_t n, const double* x, double* out,
+                        char** discretized);
+/* Solver::configure only: the resolved plan (steps, betas, restarts) */
+int qiotk_describe(const char* target, const char* params_json, char** response);
+
+void qiotk_free(char* p);
+const char* qiotk_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

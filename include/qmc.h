@@ -17,8 +17,11 @@
  *                        the OMP replica loop of SimulatedAnnealing::make_step
  *                        (solver/simulated_annealing.h:76-110) and, with a PT ladder,
  *                        ParallelTempering::make_step (solver/parallel_tempering.h:134-189)
- *   qmc_walkers_exchange ParallelTempering::accept + the swap loop
- *                        (solver/parallel_tempering.h:126-131, 171-188)
+ *   qmc_walkers_pt       ParallelTempering::make_step with ParallelTempering::accept
+ *                        and the swap loop (solver/parallel_tempering.h:126-189)
+ *   qmc_batch_run        the OMP loops over replicas of many independent solves
+ *                        fused into one launch (no counterpart: the reference runs
+ *                        one solve per process)
  *   qmc_walkers_read     Walker::cost / get_lowest_cost / get_lowest_state
  *                        (markov/walker.h:232-305)
  *   qmc_energy           AbstractIsing::calculate_cost / PuboWithCounter::calculate_cost
@@ -109,6 +112,31 @@ int qmc_walkers_sweep(qmc_walkers w, int64_t sweep0, int32_t n_sweeps, int beta_
  * each step; steps are numbered from step0. */
 int qmc_walkers_pt(qmc_walkers w, int32_t ladder, const double* ladder_betas, int64_t step0,
                    int32_t n_steps, int32_t sweeps_per_replica);
+/* Plan the next sweeps of every work-group without launching (for batches):
+ * next global sweep index sweep0, n_sweeps left, beta per sweep from
+ * `schedule` (n_sweeps values, shared by all replicas) or, when NULL, the
+ * per-replica betas. */
+int qmc_walkers_plan(qmc_walkers w, int64_t sweep0, int32_t n_sweeps, const double* schedule);
+/* Plan parallel tempering inside each work-group (replicas_per_group ==
+ * ladder): n_steps steps of sweeps_per_replica sweeps, exchanges after each. */
+int qmc_walkers_plan_pt(qmc_walkers w, int32_t ladder, const double* ladder_betas, int64_t step0,
+                        int32_t n_steps, int32_t sweeps_per_replica);
+/* Largest number of planned sweeps not yet run by any work-group. */
+int qmc_walkers_remaining(qmc_walkers w, int64_t* sweeps_left);
+
+/* A batch launches the work-groups of many walker sets -- different
+ * problems, sizes and progress -- together: one launch per kernel variant
+ * (model kind x pairwise x coupling type), largest problems first. */
+int qmc_batch_create(qmc_ctx ctx, qmc_batch* out);
+void qmc_batch_destroy(qmc_batch b);
+int qmc_batch_add(qmc_batch b, qmc_walkers w);
+int qmc_batch_remove(qmc_batch b, qmc_walkers w);
+int qmc_batch_clear(qmc_batch b);
+/* Asynchronous: every member work-group runs up to max_sweeps planned sweeps. */
+int qmc_batch_run(qmc_batch b, int32_t max_sweeps);
+/* Wait for the last qmc_batch_run; ms = its device time (HIP events). */
+int qmc_batch_wait(qmc_batch b, double* ms);
+
 /* Read back per replica: current cost, lowest cost, current beta, accepted
  * counts, current and lowest states (n_replicas*n_vars bytes).  Any pointer
  * may be NULL. */

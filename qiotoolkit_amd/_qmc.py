@@ -23,6 +23,9 @@ EXPORTS = [
     "qmc_walkers_destroy", "qmc_walkers_init", "qmc_walkers_set_states", "qmc_walkers_set_betas",
     "qmc_walkers_sweep", "qmc_walkers_pt", "qmc_walkers_read", "qmc_walkers_energy",
     "qmc_walkers_labels", "qmc_energy", "qmc_delta_all", "qmc_philox", "qmc_walkers_last_ms",
+    "qmc_walkers_plan", "qmc_walkers_plan_pt", "qmc_walkers_remaining", "qmc_batch_create",
+    "qmc_batch_destroy", "qmc_batch_add", "qmc_batch_remove", "qmc_batch_clear", "qmc_batch_run",
+    "qmc_batch_wait",
 ]
 
 
@@ -81,6 +84,17 @@ def lib():
     L.qmc_delta_all.argtypes = [vp, C.c_int32, P(C.c_uint8), P(C.c_double)]
     L.qmc_philox.argtypes = [vp, C.c_int32, P(C.c_uint32), P(C.c_uint32), P(C.c_uint32)]
     L.qmc_walkers_last_ms.argtypes = [vp, P(C.c_double)]
+    L.qmc_walkers_plan.argtypes = [vp, C.c_int64, C.c_int32, P(C.c_double)]
+    L.qmc_walkers_plan_pt.argtypes = [vp, C.c_int32, P(C.c_double), C.c_int64, C.c_int32, C.c_int32]
+    L.qmc_walkers_remaining.argtypes = [vp, P(C.c_int64)]
+    L.qmc_batch_create.argtypes = [vp, P(vp)]
+    L.qmc_batch_destroy.argtypes = [vp]
+    L.qmc_batch_destroy.restype = None
+    L.qmc_batch_add.argtypes = [vp, vp]
+    L.qmc_batch_remove.argtypes = [vp, vp]
+    L.qmc_batch_clear.argtypes = [vp]
+    L.qmc_batch_run.argtypes = [vp, C.c_int32]
+    L.qmc_batch_wait.argtypes = [vp, P(C.c_double)]
     _lib = L
     return L
 
@@ -241,4 +255,59 @@ class Walkers:
     def last_ms(self):
         v = C.c_double()
         _check(lib().qmc_walkers_last_ms(self.h, C.byref(v)))
+        return v.value
+
+    def plan(self, sweep0, n_sweeps, schedule=None):
+        if schedule is not None:
+            s = np.ascontiguousarray(schedule, dtype=np.float64)
+            assert len(s) >= n_sweeps
+            _check(lib().qmc_walkers_plan(self.h, sweep0, n_sweeps, _p(s, C.c_double)))
+        else:
+            _check(lib().qmc_walkers_plan(self.h, sweep0, n_sweeps, None))
+
+    def plan_pt(self, ladder_betas, step0, n_steps, sweeps_per_replica=1):
+        b = np.ascontiguousarray(ladder_betas, dtype=np.float64)
+        _check(lib().qmc_walkers_plan_pt(self.h, len(b), _p(b, C.c_double), step0, n_steps,
+                                         sweeps_per_replica))
+
+    def remaining(self):
+        v = C.c_int64()
+        _check(lib().qmc_walkers_remaining(self.h, C.byref(v)))
+        return v.value
+
+
+class Batch:
+    """Work-groups of many walker sets launched together."""
+
+    def __init__(self, ctx):
+        self.ctx = ctx
+        self.h = C.c_void_p()
+        self.members = []
+        _check(lib().qmc_batch_create(ctx.h, C.byref(self.h)))
+
+    def close(self):
+        if self.h:
+            lib().qmc_batch_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add(self, w):
+        _check(lib().qmc_batch_add(self.h, w.h))
+        self.members.append(w)
+
+    def remove(self, w):
+        _check(lib().qmc_batch_remove(self.h, w.h))
+        self.members = [m for m in self.members if m is not w]
+
+    def run(self, max_sweeps):
+        _check(lib().qmc_batch_run(self.h, max_sweeps))
+
+    def wait(self):
+        v = C.c_double()
+        _check(lib().qmc_batch_wait(self.h, C.byref(v)))
         return v.value

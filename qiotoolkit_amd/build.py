@@ -20,7 +20,8 @@ HIP_FLAGS = [
 ]
 
 DEVICE_SOURCES = ["qmc_kernels.hip", "qmc_api.cpp"]
-HOST_SOURCES = []
+HOST_SOURCES = ["host/json.cc", "host/model.cc", "host/schedule.cc", "host/solver.cc", "host/pa.cc",
+                "host/capi.cc"]
 
 
 def _run(cmd):
@@ -46,8 +47,8 @@ def build_host():
         return None
     out = os.path.join(HERE, "libqiotk.so")
     srcs = [os.path.join(CSRC, s) for s in HOST_SOURCES]
-    _run(["g++", "-std=c++17", "-O2", "-fPIC", "-shared", "-I", os.path.join(ROOT, "include"),
-          "-o", out] + srcs + ["-L", HERE, "-lqmc", "-Wl,-rpath,$ORIGIN"])
+    _run(["g++", "-std=c++17", "-O2", "-fPIC", "-shared", "-Wall", "-I", os.path.join(ROOT, "include"),
+          "-o", out] + srcs + [os.path.join(HERE, "libqmc.so"), "-Wl,-rpath,$ORIGIN"])
     return out
 
 

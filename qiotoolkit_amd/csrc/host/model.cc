@@ -1,0 +1,147 @@
+#include "model.h"
+
+#include <algorithm>
+#include <cmath>
+#include <set>
+
+namespace qiotk {
+
+Model make_model(int kind, const std::vector<Term>& terms) {
+  // graph::normalize_edges (graph/graph.h:74-177)
+  if (terms.empty())
+    throw Error(kValueError, "parameter `terms`: size must be greater than 0, found 0");
+  Model m;
+  m.kind = kind;
+  m.term_off.push_back(0);
+  std::vector<int32_t> members;
+  for (const Term& t : terms) {
+    members.clear();
+    for (int32_t name : t.ids) {
+      auto it = m.name_to_id.find(name);
+      int32_t id;
+      if (it == m.name_to_id.end()) {
+        id = (int32_t)m.names.size();
+        m.name_to_id.emplace(name, id);
+        m.names.push_back(name);
+      } else {
+        id = it->second;
+      }
+      if (std::find(members.begin(), members.end(), id) == members.end()) {
+        members.push_back(id);
+      } else if (kind == QMC_ISING) {
+        throw Error(kDuplicatedVariable, "Duplicated ids detected in term!");
+      }
+    }
+    std::sort(members.begin(), members.end());
+    const uint32_t loc = (uint32_t)members.size();
+    m.max_locality = std::max(m.max_locality, loc);
+    m.min_locality = std::min(m.min_locality, loc);
+    m.total_locality += loc;
+    if (loc > 0) {
+      const double a = std::fabs(t.c);
+      m.max_coupling = std::max(m.max_coupling, a);
+      m.min_coupling = std::min(m.min_coupling, a);
+      m.term_var.insert(m.term_var.end(), members.begin(), members.end());
+      m.term_off.push_back((int64_t)m.term_var.size());
+      m.coeff.push_back(t.c);
+    } else {
+      m.const_cost += t.c;
+    }
+  }
+  m.n_vars = (int32_t)m.names.size();
+  return m;
+}
+
+Model parse_model(const Json& doc) {
+  const Json* cf = &doc;
+  if (doc.is_object() && doc.has("cost_function")) cf = &doc.at("cost_function");
+  if (!cf->is_object() || !cf->has("type"))
+    throw Error(kMissingInput, "The configuration must contain a `cost_function` entry.");
+  const std::string& type = cf->at("type").as_string();
+  int kind;
+  if (type == "ising") kind = QMC_ISING;
+  else if (type == "pubo") kind = QMC_PUBO;
+  else throw Error(kValueError, "unsupported cost_function type: " + type);
+  std::string version = cf->has("version") ? cf->at("version").as_string() : std::string("1.0");
+  // AbstractIsing/AbstractPubo::match_version (model/ising.h:80-89, model/pubo.h:69-78)
+  if (version != "1.0" && version != "1.1")
+    throw Error(kValueError, "Expecting `version` equals 1.0 or 1.1, but found: " + version);
+  if (!cf->has("terms")) throw Error(kMissingInput, "parameter `terms` is required");
+  const Json& jt = cf->at("terms");
+  std::vector<Term> terms;
+  terms.reserve(jt.size());
+  for (const Json& t : jt.items()) {
+    Term term;
+    term.c = t.at("c").as_double();
+    for (const Json& id : t.at("ids").items()) term.ids.push_back((int32_t)id.as_int());
+    terms.push_back(std::move(term));
+  }
+  Model m = make_model(kind, terms);
+  m.version = version;
+  if (cf->has("initial_configuration")) {
+    // GraphModelConfiguration::map_initial_configuration (model/graph_model.h:64-95)
+    const Json& ic = cf->at("initial_configuration");
+    for (const auto& kv : ic.members()) {
+      const int32_t name = (int32_t)std::stol(kv.first);
+      auto it = m.name_to_id.find(name);
+      if (it == m.name_to_id.end())
+        throw Error(kInitialConfig, "parameter `initial_configuration` have node name: " + kv.first +
+                                        " which is not found in model node names");
+      const int64_t v = kv.second.as_int();
+      uint8_t bit;
+      if (kind == QMC_ISING) {
+        if (v != 1 && v != -1)
+          throw Error(kValueError, "parameter `initial_configuration` may have 1 and -1 values for ising models, found: " + std::to_string(v));
+        bit = v == -1;
+      } else {
+        if (v != 0 && v != 1)
+          throw Error(kValueError, "parameter `initial_configuration` may have 1 and 0 values for pubo models, found: " + std::to_string(v));
+        bit = v == 0;
+      }
+      m.initial.emplace_back(it->second, bit);
+    }
+    if (!m.initial.empty() && (int32_t)m.initial.size() != m.n_vars)
+      throw Error(kInitialConfig, "parameter `initial_configuration` have size: " +
+                                      std::to_string(m.initial.size()) +
+                                      " which is different from model nodes size: " +
+                                      std::to_string(m.n_vars));
+    m.has_initial = !m.initial.empty();
+  }
+  return m;
+}
+
+Json Model::render(const uint8_t* bits) const {
+  // IsingState::render / BinaryWithCounter::render with the id map
+  // (model/ising.cc:16-24, model/pubo.h:240-249)
+  Json s = Json::object();
+  for (int32_t i = 0; i < n_vars; ++i) {
+    const int v = kind == QMC_ISING ? (bits[i] ? -1 : 1) : (bits[i] ? 0 : 1);
+    s[std::to_string(names[i])] = Json(v);
+  }
+  return s;
+}
+
+double Model::cost(const uint8_t* bits) const {
+  double total = 0.0;
+  for (size_t j = 0; j < coeff.size(); ++j) {
+    uint32_t x = 0;
+    for (int64_t k = term_off[j]; k < term_off[j + 1]; ++k) {
+      const uint32_t b = bits[term_var[k]] & 1u;
+      x = kind == QMC_ISING ? (x ^ b) : (x | b);
+    }
+    if (kind == QMC_ISING) total += x ? -coeff[j] : coeff[j];
+    else if (!x) total += coeff[j];
+  }
+  return total;
+}
+
+DeviceModel::DeviceModel(qmc_ctx ctx, const Model& m, int lanes) {
+  device_check(qmc_problem_create(ctx, m.kind, m.n_vars, m.n_terms(), m.term_off.data(),
+                                  m.term_var.data(), m.coeff.data(), lanes, &p_),
+               "qmc_problem_create");
+  device_check(qmc_problem_get_info(p_, &info_), "qmc_problem_get_info");
+}
+
+DeviceModel::~DeviceModel() { qmc_problem_destroy(p_); }
+
+}  // namespace qiotk

@@ -1,0 +1,72 @@
+// Ising / PUBO cost functions in the reference's JSON format, normalised as
+// graph::normalize_edges does (graph/graph.h:74-177) and uploaded to the
+// device through the C ABI.
+#pragma once
+#include <cstdint>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../../include/qmc.h"
+#include "errors.h"
+#include "json.h"
+
+namespace qiotk {
+
+struct Term {
+  double c;
+  std::vector<int32_t> ids;
+};
+
+// A cost function after normalisation: internal variable ids 0..N-1 by first
+// appearance (this is the reference's sweep order), ids sorted inside each
+// term, constant terms folded into const_cost; duplicate ids are an error for
+// Ising and merged for PUBO (model/pubo.h:82).
+struct Model {
+  int kind = QMC_ISING;          // QMC_ISING or QMC_PUBO
+  std::string version = "1.0";
+  int32_t n_vars = 0;
+  std::vector<int64_t> term_off;  // n_terms + 1
+  std::vector<int32_t> term_var;
+  std::vector<double> coeff;
+  double const_cost = 0.0;
+  std::vector<int32_t> names;     // internal id -> input id
+  std::map<int32_t, int32_t> name_to_id;
+  // graph properties (graph/graph_properties.h)
+  uint32_t max_locality = 0, min_locality = UINT32_MAX;
+  uint64_t total_locality = 0;
+  double max_coupling = 0.0, min_coupling = 1.7976931348623157e308;
+  // initial_configuration, internal ids -> reference bit (true = -1 / 0)
+  std::vector<std::pair<int32_t, uint8_t>> initial;
+  bool has_initial = false;
+
+  int64_t n_terms() const { return (int64_t)coeff.size(); }
+  bool empty() const { return n_vars == 0; }
+  // scale_factor used by population annealing (graph_properties.h:24-40)
+  double scale_factor() const { return max_coupling <= 1 ? 1.0 : 1.0 / max_coupling; }
+  // render a state (one byte per internal variable, 1 = reference true bit)
+  Json render(const uint8_t* bits) const;
+  // FP64 cost of a state (host-side, used for the const-cost-free checks)
+  double cost(const uint8_t* bits) const;
+};
+
+// Parse {"cost_function": {...}} (or the inner object) and normalise it.
+Model parse_model(const Json& doc);
+Model make_model(int kind, const std::vector<Term>& terms);
+
+// Device copy of a model, owned.
+class DeviceModel {
+ public:
+  DeviceModel(qmc_ctx ctx, const Model& m, int lanes = 0);
+  ~DeviceModel();
+  DeviceModel(const DeviceModel&) = delete;
+  DeviceModel& operator=(const DeviceModel&) = delete;
+  qmc_problem handle() const { return p_; }
+  const qmc_problem_info& info() const { return info_; }
+
+ private:
+  qmc_problem p_ = nullptr;
+  qmc_problem_info info_{};
+};
+
+}  // namespace qiotk

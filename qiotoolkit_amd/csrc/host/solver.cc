@@ -1,0 +1,457 @@
+#include "solver.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <limits>
+#include <numeric>
+
+#include "rng.h"
+
+namespace qiotk {
+
+namespace {
+
+double wall_s() {
+  using namespace std::chrono;
+  return duration<double>(steady_clock::now().time_since_epoch()).count();
+}
+
+const Json& params_of(const Json& doc) {
+  if (!doc.is_object() || !doc.has("params"))
+    throw Error(kMissingInput, "Input field `params` is required.");
+  return doc.at("params");
+}
+
+// Metropolis::set_beta / set_temperature (markov/metropolis.h:35-61)
+double beta_from(double v, bool use_inverse) {
+  if (use_inverse) return v <= 0 ? 0.0 : v;
+  return v <= 0 ? std::numeric_limits<double>::infinity() : 1.0 / v;
+}
+
+// Replicas per work-group when the caller does not pick one: enough
+// work-groups to spread over the CUs, up to 8 replicas sharing each
+// neighbour byte.
+int default_group(int64_t R) {
+  int B = 1;
+  while (B < 8 && R / (2 * B) >= 128) B *= 2;
+  return (int)std::min<int64_t>(B, R);
+}
+
+}  // namespace
+
+void Solver::configure(const Json& doc) {
+  const Json& p = params_of(doc);
+  // Solver::configure (solver.h:156-218)
+  if (p.has("threads")) threads_ = (int)p.at("threads").as_int();
+  if (p.has("time_limit")) time_limit_ = p.at("time_limit").as_double();
+  else if (p.has("timeout")) time_limit_ = p.at("timeout").as_double();
+  if (p.has("cost_limit")) cost_limit_ = p.at("cost_limit").as_double();
+  if (p.has("eval_limit")) eval_limit_ = p.at("eval_limit").as_int();
+  if (p.has("number_of_solutions")) {
+    const int64_t k = p.at("number_of_solutions").as_int();
+    if (k < 1 || k > 1000) throw Error(kValueError, "`number_of_solutions` must be in [1, 1000]");
+    number_of_solutions_ = (int)k;
+  }
+  // SteppingSolver::configure (stepping_solver.h:43-81)
+  if (p.has("step_limit")) step_limit_ = p.at("step_limit").as_int();
+  else if (p.has("sweeps")) step_limit_ = p.at("sweeps").as_int();
+  if (p.has("steps_per_tick")) {
+    steps_per_tick_ = p.at("steps_per_tick").as_int();
+    if (steps_per_tick_ <= 0) throw Error(kValueError, "`steps_per_tick` must be greater than 0");
+  }
+  if (p.has("seed")) {
+    seed_ = (uint32_t)p.at("seed").as_int();
+  } else {
+    seed_ = (uint32_t)(std::chrono::duration_cast<std::chrono::microseconds>(
+                           std::chrono::system_clock::now().time_since_epoch())
+                           .count());
+  }
+  set_output("seed", Json((int64_t)seed_));
+  set_output("sweeps", Json(step_limit_));
+}
+
+int64_t Solver::steps_to_run() const { return std::max(step_limit_, steps_per_tick_); }
+
+Json Solver::describe() const {
+  Json d = Json::object();
+  d["target"] = target();
+  d["steps"] = Json(steps_to_run());
+  d["seed"] = Json((int64_t)seed_);
+  return d;
+}
+
+static Json to_json(const std::vector<double>& v) {
+  Json a = Json::array();
+  for (double x : v) a.push_back(Json(x));
+  return a;
+}
+
+void Solver::record(double cost, const std::vector<uint8_t>& state) {
+  // ModelSolver::update_lowest_cost (model_solver.h:247-258)
+  if (!have_lowest_ || cost < lowest_cost_) {
+    have_lowest_ = true;
+    lowest_cost_ = cost;
+    lowest_state_ = state;
+  }
+}
+
+void Solver::populate(const std::vector<double>& lowest, const std::vector<uint8_t>& states,
+                      int64_t R) {
+  // ModelSolver::copy_solutions + populate_solutions (model_solver.h:261-308)
+  const int64_t N = model_->n_vars;
+  std::vector<int64_t> idx(R);
+  std::iota(idx.begin(), idx.end(), 0);
+  const int64_t k = std::min<int64_t>(number_of_solutions_, R);
+  std::partial_sort(idx.begin(), idx.begin() + k, idx.end(),
+                    [&](int64_t a, int64_t b) { return lowest[a] < lowest[b]; });
+  lowest_costs_.clear();
+  lowest_states_.clear();
+  for (int64_t i = 0; i < k; ++i) {
+    lowest_costs_.push_back(lowest[idx[i]]);
+    lowest_states_.emplace_back(states.begin() + idx[i] * N, states.begin() + (idx[i] + 1) * N);
+  }
+  if (k > 0) record(lowest_costs_[0], lowest_states_[0]);
+}
+
+Json Solver::get_solutions() const {
+  Json s = Json::object();
+  s["version"] = "1.0";
+  if (have_lowest_ && !model_->empty()) {
+    s["cost"] = lowest_cost_ + model_->const_cost;
+    s["configuration"] = model_->render(lowest_state_.data());
+    Json arr = Json::array();
+    for (size_t i = 0; i < lowest_costs_.size(); ++i) {
+      Json e = Json::object();
+      e["cost"] = lowest_costs_[i] + model_->const_cost;
+      e["configuration"] = model_->render(lowest_states_[i].data());
+      arr.push_back(e);
+    }
+    s["solutions"] = arr;
+  } else {
+    s["cost"] = model_->const_cost;
+  }
+  s["parameters"] = out_params_;
+  return s;
+}
+
+Json Solver::get_result() const {
+  Json b = Json::object();
+  b["cost_function_evaluation_count"] = Json((int64_t)0);
+  b["cost_difference_evaluation_count"] = Json(attempts_);
+  b["max_locality"] = Json((int64_t)(model_->empty() ? 0 : model_->max_locality));
+  b["min_locality"] = Json((int64_t)(model_->empty() ? 0 : model_->min_locality));
+  b["avg_locality"] = model_->n_terms() ? (double)model_->total_locality / (double)model_->n_terms() : 0.0;
+  b["variable_count"] = Json((int64_t)model_->n_vars);
+  b["term_count"] = Json(model_->n_terms());
+  Json ext = Json::array();
+  Json counters = Json::object();
+  counters["accepted_transitions"] = Json(accepted_);
+  counters["difference_evaluations"] = Json(attempts_);
+  counters["function_evaluations"] = Json((int64_t)0);
+  Json c1 = Json::object();
+  c1["name"] = "counters";
+  c1["value"] = counters;
+  ext.push_back(c1);
+  Json sol = Json::object();
+  sol["last_step"] = Json(steps_done_);
+  if (!exit_reason_.empty()) sol["exit_reason"] = exit_reason_;
+  Json c2 = Json::object();
+  c2["name"] = "solver";
+  c2["value"] = sol;
+  ext.push_back(c2);
+  Json dev = Json::object();
+  dev["device_time_ms"] = timing_.device_ms;
+  dev["launches"] = Json(timing_.launches);
+  dev["attempts_per_s"] = timing_.device_ms > 0 ? (double)attempts_ / (timing_.device_ms * 1e-3) : 0.0;
+  if (dmodel_) {
+    dev["colour_classes"] = Json((int64_t)dmodel_->info().n_classes);
+    dev["coupling"] = dmodel_->info().coupling == QMC_COUPLING_I32 ? "int32" : "fp32";
+  }
+  Json c3 = Json::object();
+  c3["name"] = "device";
+  c3["value"] = dev;
+  ext.push_back(c3);
+  b["extensions"] = ext;
+  Json r = Json::object();
+  r["benchmark"] = b;
+  r["solutions"] = get_solutions();
+  return r;
+}
+
+// ===========================================================================
+// SimulatedAnnealing (simulated_annealing.h:45-209)
+void SimulatedAnnealing::configure(const Json& doc) {
+  Solver::configure(doc);
+  const Json& p = params_of(doc);
+  use_inverse_temperature_ = p.has("use_inverse_temperature") && p.at("use_inverse_temperature").as_bool();
+  bool qio_format = false;
+  if (p.has("schedule")) {
+    schedule_.configure(p.at("schedule"));
+    set_output("schedule", schedule_.render());
+  } else {
+    const double b0 = p.has("beta_start") ? p.at("beta_start").as_double() : 0.0;
+    const double b1 = p.has("beta_stop") ? p.at("beta_stop").as_double() : 0.01;
+    if (b0 < 0) throw Error(kValueError, "parameter `beta_start` must be greater or equal 0");
+    if (!(b1 > b0)) throw Error(kValueError, "parameter `beta_stop` must be greater than `beta_start`");
+    set_output("beta_start", b0);
+    set_output("beta_stop", b1);
+    use_inverse_temperature_ = true;
+    schedule_.make_linear(b0, b1);
+    qio_format = true;
+  }
+  (void)qio_format;
+  if (schedule_.get_start() == 0 && schedule_.get_stop() == 1 && schedule_.get_count() == 1) {
+    if (step_limit_ > 1) schedule_.set_stop((int)std::max<int64_t>(1, step_limit_ - 1));
+    else throw Error(kValueError, "The number of sweeps must be greater than 1.");
+  }
+  restarts_ = threads_ > 0 ? threads_ : 64;
+  if (p.has("restarts")) {
+    const int64_t r = p.at("restarts").as_int();
+    if (r <= 0) throw Error(kValueError, "parameter `restarts` must be greater than 0");
+    restarts_ = (int)r;
+  }
+  set_output("restarts", Json((int64_t)restarts_));
+  if (p.has("replicas_per_group")) group_ = (int)p.at("replicas_per_group").as_int();
+}
+
+SimulatedAnnealing::~SimulatedAnnealing() {
+  if (w_) qmc_walkers_destroy(w_);
+}
+
+void SimulatedAnnealing::init() {
+  if (model_->empty()) return;
+  const int B = group_ > 0 ? group_ : default_group(restarts_);
+  device_check(qmc_walkers_create(dmodel_->handle(), restarts_, B, seed_, 0, 0, restarts_, &w_),
+               "qmc_walkers_create");
+  if (model_->has_initial) {
+    std::vector<uint8_t> st((size_t)restarts_ * model_->n_vars, 0);
+    for (int r = 0; r < restarts_; ++r)
+      for (auto& kv : model_->initial) st[(size_t)r * model_->n_vars + kv.first] = kv.second;
+    device_check(qmc_walkers_set_states(w_, st.data()), "qmc_walkers_set_states");
+  } else {
+    device_check(qmc_walkers_init(w_, 0), "qmc_walkers_init");
+  }
+  betas_ = schedule_betas();
+}
+
+std::vector<double> SimulatedAnnealing::schedule_betas() const {
+  // SimulatedAnnealing::make_step: one schedule value per step, as beta or T
+  const int64_t steps = steps_to_run();
+  std::vector<double> b(steps);
+  for (int64_t s = 0; s < steps; ++s)
+    b[s] = beta_from(schedule_.get_value((double)s), use_inverse_temperature_);
+  return b;
+}
+
+Json SimulatedAnnealing::describe() const {
+  Json d = Solver::describe();
+  d["restarts"] = Json((int64_t)restarts_);
+  d["betas"] = to_json(schedule_betas());
+  return d;
+}
+
+void SimulatedAnnealing::run() {
+  if (model_->empty()) return;
+  const int64_t steps = steps_to_run();
+  const int64_t N = model_->n_vars;
+  const double t0 = wall_s();
+  // Without time or cost limits all steps are one launch; otherwise ticks of
+  // steps with the limits checked in between (stepping_solver.h:127-258).
+  const bool ticking = time_limit_ || cost_limit_;
+  int64_t chunk = ticking ? std::max<int64_t>(1, steps_per_tick_) : steps;
+  int64_t s = 0;
+  while (s < steps) {
+    const int64_t n = std::min(chunk, steps - s);
+    device_check(qmc_walkers_sweep(w_, s, (int32_t)n, QMC_BETA_SCHEDULE, betas_.data() + s, 0),
+                 "qmc_walkers_sweep");
+    double ms = 0;
+    qmc_walkers_last_ms(w_, &ms);
+    timing_.device_ms += ms;
+    timing_.launches++;
+    s += n;
+    attempts_ += n * N * restarts_;
+    if (!ticking) continue;
+    if (cost_limit_) {
+      std::vector<double> low(restarts_);
+      device_check(qmc_walkers_read(w_, nullptr, low.data(), nullptr, nullptr, nullptr, nullptr), "read");
+      if (*std::min_element(low.begin(), low.end()) <= *cost_limit_) {
+        exit_reason_ = "Stop due to cost limit";
+        break;
+      }
+    }
+    if (time_limit_ && wall_s() - t0 >= *time_limit_) {
+      exit_reason_ = "Stop due to time_limit";
+      break;
+    }
+    chunk = std::min<int64_t>(chunk * 2, 1 << 14);
+  }
+  steps_done_ = s;
+}
+
+bool SimulatedAnnealing::batch_plan(qmc_batch b) {
+  if (time_limit_ || cost_limit_) return false;
+  if (model_->empty()) return true;
+  const int64_t steps = steps_to_run();
+  device_check(qmc_walkers_plan(w_, 0, (int32_t)steps, betas_.data()), "qmc_walkers_plan");
+  device_check(qmc_batch_add(b, w_), "qmc_batch_add");
+  steps_done_ = steps;
+  attempts_ = steps * model_->n_vars * restarts_;
+  return true;
+}
+
+void SimulatedAnnealing::finalize() {
+  if (model_->empty()) return;
+  const int64_t N = model_->n_vars;
+  lowest_.assign(restarts_, 0.0);
+  lowest_states_flat_.assign((size_t)restarts_ * N, 0);
+  std::vector<uint64_t> acc(restarts_);
+  device_check(qmc_walkers_read(w_, nullptr, lowest_.data(), nullptr, acc.data(), nullptr,
+                                lowest_states_flat_.data()),
+               "qmc_walkers_read");
+  // exact FP64 energy of every returned configuration
+  device_check(qmc_walkers_energy(w_, 1, lowest_.data()), "qmc_walkers_energy");
+  accepted_ = (int64_t)std::accumulate(acc.begin(), acc.end(), (uint64_t)0);
+  populate(lowest_, lowest_states_flat_, restarts_);
+}
+
+// ===========================================================================
+// ParallelTempering (parallel_tempering.h:68-234)
+void ParallelTempering::configure(const Json& doc) {
+  Solver::configure(doc);
+  const Json& p = params_of(doc);
+  if (p.has("sweeps_per_replica")) {
+    sweeps_per_replica_ = (int)p.at("sweeps_per_replica").as_int();
+    if (sweeps_per_replica_ <= 0) throw Error(kValueError, "`sweeps_per_replica` must be greater than 0");
+  }
+  if (p.has("temperatures")) {
+    temperatures_.configure(p.at("temperatures"));
+    use_inverse_temperatures_ = false;
+    set_output("temperatures", temperatures_.render());
+  } else if (p.has("all_betas")) {
+    temperatures_.configure(p.at("all_betas"));
+    use_inverse_temperatures_ = true;
+    set_output("all_betas", temperatures_.render());
+  } else {
+    temperatures_.make_list({0.1, 0.5, 1., 2., 4.});
+    use_inverse_temperatures_ = true;
+    set_output("all_betas", temperatures_.render());
+  }
+  if (p.has("replicas_per_group")) group_ = (int)p.at("replicas_per_group").as_int();
+}
+
+ParallelTempering::~ParallelTempering() {
+  if (w_) qmc_walkers_destroy(w_);
+}
+
+std::vector<double> ParallelTempering::ladder() const {
+  // ParallelTempering::init (parallel_tempering.h:68-81): betas descending or
+  // temperatures ascending, then one beta per replica
+  std::vector<double> t = temperatures_.get_discretized_values();
+  if (use_inverse_temperatures_) std::sort(t.begin(), t.end(), std::greater<double>());
+  else std::sort(t.begin(), t.end(), std::less<double>());
+  std::vector<double> b(t.size());
+  for (size_t k = 0; k < t.size(); ++k) b[k] = beta_from(t[k], use_inverse_temperatures_);
+  return b;
+}
+
+Json ParallelTempering::describe() const {
+  Json d = Solver::describe();
+  d["betas"] = to_json(ladder());
+  d["sweeps_per_replica"] = Json((int64_t)sweeps_per_replica_);
+  return d;
+}
+
+void ParallelTempering::init() {
+  if (model_->empty()) return;
+  betas_ = ladder();
+  const int L = (int)betas_.size();
+  if (L > 32) throw Error(kValueError, "at most 32 temperatures per ladder are supported");
+  int B = group_ > 0 ? group_ : L;
+  if (L % B != 0) B = L;
+  device_check(qmc_walkers_create(dmodel_->handle(), L, B, seed_, 0, 0, L, &w_), "qmc_walkers_create");
+  if (model_->has_initial) {
+    std::vector<uint8_t> st((size_t)L * model_->n_vars, 0);
+    for (int r = 0; r < L; ++r)
+      for (auto& kv : model_->initial) st[(size_t)r * model_->n_vars + kv.first] = kv.second;
+    device_check(qmc_walkers_set_states(w_, st.data()), "qmc_walkers_set_states");
+  } else {
+    device_check(qmc_walkers_init(w_, 0), "qmc_walkers_init");
+  }
+}
+
+void ParallelTempering::run() {
+  if (model_->empty()) return;
+  const int64_t steps = steps_to_run();
+  const int L = (int)betas_.size();
+  const int64_t N = model_->n_vars;
+  const double t0 = wall_s();
+  const bool ticking = time_limit_ || cost_limit_;
+  int64_t chunk = ticking ? std::max<int64_t>(1, steps_per_tick_) : steps;
+  int64_t s = 0;
+  while (s < steps) {
+    const int64_t n = std::min(chunk, steps - s);
+    device_check(qmc_walkers_pt(w_, L, betas_.data(), s, (int32_t)n, sweeps_per_replica_), "qmc_walkers_pt");
+    double ms = 0;
+    qmc_walkers_last_ms(w_, &ms);
+    timing_.device_ms += ms;
+    timing_.launches++;
+    s += n;
+    attempts_ += n * sweeps_per_replica_ * N * L;
+    if (!ticking) continue;
+    if (cost_limit_) {
+      std::vector<double> low(L);
+      device_check(qmc_walkers_read(w_, nullptr, low.data(), nullptr, nullptr, nullptr, nullptr), "read");
+      if (*std::min_element(low.begin(), low.end()) <= *cost_limit_) {
+        exit_reason_ = "Stop due to cost limit";
+        break;
+      }
+    }
+    if (time_limit_ && wall_s() - t0 >= *time_limit_) {
+      exit_reason_ = "Stop due to time_limit";
+      break;
+    }
+    chunk = std::min<int64_t>(chunk * 2, 1 << 14);
+  }
+  steps_done_ = s;
+}
+
+bool ParallelTempering::batch_plan(qmc_batch b) {
+  if (time_limit_ || cost_limit_) return false;
+  if (model_->empty()) return true;
+  const int L = (int)betas_.size();
+  const int64_t steps = steps_to_run();
+  if (qmc_walkers_plan_pt(w_, L, betas_.data(), 0, (int32_t)steps, sweeps_per_replica_) != QMC_OK)
+    return false;  // ladder split over work-groups: runs on its own
+  device_check(qmc_batch_add(b, w_), "qmc_batch_add");
+  steps_done_ = steps;
+  attempts_ = steps * sweeps_per_replica_ * model_->n_vars * L;
+  return true;
+}
+
+void ParallelTempering::finalize() {
+  if (model_->empty()) return;
+  const int L = (int)betas_.size();
+  const int64_t N = model_->n_vars;
+  lowest_.assign(L, 0.0);
+  lowest_states_flat_.assign((size_t)L * N, 0);
+  std::vector<uint64_t> acc(L);
+  device_check(qmc_walkers_read(w_, nullptr, lowest_.data(), nullptr, acc.data(), nullptr,
+                                lowest_states_flat_.data()),
+               "qmc_walkers_read");
+  device_check(qmc_walkers_energy(w_, 1, lowest_.data()), "qmc_walkers_energy");
+  accepted_ = (int64_t)std::accumulate(acc.begin(), acc.end(), (uint64_t)0);
+  populate(lowest_, lowest_states_flat_, L);
+}
+
+// ===========================================================================
+std::unique_ptr<Solver> create_solver(const std::string& target) {
+  // create_model_solver (app/runner.cc:133-192)
+  if (target == "simulatedannealing.qiotoolkit") return std::make_unique<SimulatedAnnealing>();
+  if (target == "paralleltempering.qiotoolkit") return std::make_unique<ParallelTempering>();
+  if (target == "populationannealing.cpu" || target == "populationannealing.qiotoolkit")
+    return std::make_unique<PopulationAnnealing>();
+  throw Error(kValueError, "Unknown solver: " + target);
+}
+
+}  // namespace qiotk

@@ -1,0 +1,159 @@
+// Host solvers over the device sweep engine, keeping the reference's
+// lifecycle configure(json) -> init() -> run() -> finalize() -> get_result()
+// (solver/solver.h:37-50,147-156,224) and parameter names (Appendix B of the
+// reference docs: solver.h:166-217, stepping_solver.h:55-79,
+// simulated_annealing.h:123-209, parallel_tempering.h:192-234,
+// population_annealing.h:632-778).
+#pragma once
+#include <memory>
+#include <optional>
+#include <string>
+#include <vector>
+
+#include "json.h"
+#include "model.h"
+#include "schedule.h"
+
+namespace qiotk {
+
+struct Timing {
+  double device_ms = 0.0;  // summed HIP-event time of sweep launches
+  int64_t launches = 0;
+};
+
+class Solver {
+ public:
+  virtual ~Solver() = default;
+  void set_model(const Model* m, DeviceModel* dm, qmc_ctx ctx) {
+    model_ = m;
+    dmodel_ = dm;
+    ctx_ = ctx;
+  }
+  virtual std::string target() const = 0;
+  // {"params": {...}} as the reference's Solver::configure takes it
+  virtual void configure(const Json& doc);
+  virtual void init() = 0;
+  virtual void run() = 0;
+  virtual void finalize() = 0;
+  // Fused launches with other solves: plan the whole run on the device and
+  // add the walkers to `b`; false when this solve needs host-side steps.
+  virtual bool batch_plan(qmc_batch) { return false; }
+  // Resolved run plan (steps, betas), computable without a device.
+  virtual Json describe() const;
+  Json get_result() const;
+  Json get_solutions() const;  // ModelSolver::get_solutions (model_solver.h:111-150)
+
+  int64_t attempts() const { return attempts_; }
+  int64_t accepted() const { return accepted_; }
+  const Timing& timing() const { return timing_; }
+  double lowest_cost() const { return lowest_cost_; }
+  uint32_t seed() const { return seed_; }
+
+ protected:
+  // SteppingSolver (stepping_solver.h:112-268): fewer than steps_per_tick steps
+  // are never run, because the step limit is checked after each tick.
+  int64_t steps_to_run() const;
+  bool has_time_limit() const { return (bool)time_limit_; }
+  void record(double cost, const std::vector<uint8_t>& state);
+  // ModelSolver::populate_solutions: best `number_of_solutions` walkers
+  void populate(const std::vector<double>& lowest, const std::vector<uint8_t>& states, int64_t R);
+  void set_output(const std::string& k, Json v) { out_params_[k] = std::move(v); }
+
+  const Model* model_ = nullptr;
+  DeviceModel* dmodel_ = nullptr;
+  qmc_ctx ctx_ = nullptr;
+  uint32_t seed_ = 0;
+  int64_t step_limit_ = 10000;
+  int64_t steps_per_tick_ = 10;
+  std::optional<double> time_limit_;
+  std::optional<double> cost_limit_;
+  std::optional<int64_t> eval_limit_;
+  int number_of_solutions_ = 1;
+  int threads_ = 0;
+  int64_t steps_done_ = 0;
+  int64_t attempts_ = 0, accepted_ = 0;
+  Timing timing_;
+  bool have_lowest_ = false;
+  double lowest_cost_ = 0.0;
+  std::vector<uint8_t> lowest_state_;
+  std::vector<double> lowest_costs_;
+  std::vector<std::vector<uint8_t>> lowest_states_;
+  Json out_params_ = Json::object();
+  std::string exit_reason_;
+};
+
+// simulated_annealing.h
+class SimulatedAnnealing : public Solver {
+ public:
+  std::string target() const override { return "simulatedannealing.qiotoolkit"; }
+  void configure(const Json& doc) override;
+  void init() override;
+  void run() override;
+  void finalize() override;
+  bool batch_plan(qmc_batch b) override;
+  Json describe() const override;
+  int restarts() const { return restarts_; }
+
+ private:
+  bool use_inverse_temperature_ = false;
+  int restarts_ = 64;
+  int group_ = 0;
+  Schedule schedule_;
+  qmc_walkers w_ = nullptr;
+  std::vector<double> betas_;
+  std::vector<double> schedule_betas() const;
+  std::vector<double> lowest_;
+  std::vector<uint8_t> lowest_states_flat_;
+
+ public:
+  ~SimulatedAnnealing() override;
+};
+
+// parallel_tempering.h
+class ParallelTempering : public Solver {
+ public:
+  std::string target() const override { return "paralleltempering.qiotoolkit"; }
+  void configure(const Json& doc) override;
+  void init() override;
+  void run() override;
+  void finalize() override;
+  bool batch_plan(qmc_batch b) override;
+  Json describe() const override;
+  ~ParallelTempering() override;
+  std::vector<double> ladder() const;
+
+ private:
+  bool use_inverse_temperatures_ = true;
+  Schedule temperatures_;
+  int sweeps_per_replica_ = 1;
+  int group_ = 0;
+  std::vector<double> betas_;  // descending
+  qmc_walkers w_ = nullptr;
+  std::vector<double> lowest_;
+  std::vector<uint8_t> lowest_states_flat_;
+};
+
+// population_annealing.h: device sweeps, host resampling (Poisson), linear
+// or geometric beta schedule.
+class PopulationAnnealing : public Solver {
+ public:
+  std::string target() const override { return "populationannealing.cpu"; }
+  void configure(const Json& doc) override;
+  void init() override;
+  void run() override;
+  void finalize() override;
+  ~PopulationAnnealing() override;
+
+ private:
+  double alpha_ = 2.0;
+  int64_t target_population_ = 64;
+  bool constant_population_ = false;
+  std::string strategy_ = "linear_schedule";
+  Schedule beta_;
+  double beta_start_ = 0.0, beta_stop_ = 5.0;
+  int sweeps_per_replica_ = 1;
+};
+
+std::unique_ptr<Solver> create_solver(const std::string& target);
+
+}  // namespace qiotk

@@ -116,9 +116,30 @@ struct qmc_walkers_s {
   DevBuf<double> cost, best_cost, beta, sched, ladder_beta, energies;
   DevBuf<unsigned long long> accepted;
   DevBuf<int32_t> label;
+  DevBuf<int64_t> progress;  // 2 per unit: next sweep, sweeps left
   DevBuf<UnitDev> d_units;
   std::vector<UnitDev> units;
   int ladder = 0;
+  int spr = 1;
+  int64_t sched_base = 0;
+  uint64_t version = 0;      // bumped whenever `units` changes
+  float last_ms = 0.f;
+};
+
+struct qmc_batch_s {
+  qmc_ctx ctx = nullptr;
+  std::vector<qmc_walkers> members;
+  std::vector<uint64_t> seen_version;
+  bool dirty = true;
+  // one launch group per kernel instantiation
+  struct Group {
+    int kind, pairwise, coupling;
+    int block = 64;
+    int stride = 0;
+    std::vector<UnitDev> units;
+    DevBuf<UnitDev> d_units;
+  };
+  std::vector<std::unique_ptr<Group>> groups;
   float last_ms = 0.f;
 };
 
@@ -347,6 +368,11 @@ static void build_units(qmc_walkers w) {
     U.accepted = w->accepted.p + r;
     U.label = w->label.p + r;
     U.ladder_beta = w->ladder_beta.p;
+    U.ladder = w->ladder;
+    U.spr = w->spr;
+    U.sched = w->sched.p;
+    U.sched_base = w->sched_base;
+    U.progress = w->progress.p + 2 * (size_t)u;
     U.nrep = (uint32_t)std::min(w->B, w->R - r);
     // replicas are grouped per seed: instance k = r / per_seed uses seed + k
     const int32_t gr = w->first_replica + r;
@@ -357,8 +383,21 @@ static void build_units(qmc_walkers w) {
 }
 
 static int upload_units(qmc_walkers w) {
+  build_units(w);
+  w->version++;
   HIP_TRY(hipMemcpy(w->d_units.p, w->units.data(), w->units.size() * sizeof(UnitDev),
                     hipMemcpyHostToDevice));
+  return QMC_OK;
+}
+
+// every unit: next sweep = sweep0, sweeps left = n
+static int set_progress(qmc_walkers w, int64_t sweep0, int64_t n) {
+  std::vector<int64_t> pr(2 * (size_t)w->n_units);
+  for (int u = 0; u < w->n_units; ++u) {
+    pr[2 * u] = sweep0;
+    pr[2 * u + 1] = n;
+  }
+  HIP_TRY(hipMemcpy(w->progress.p, pr.data(), pr.size() * sizeof(int64_t), hipMemcpyHostToDevice));
   return QMC_OK;
 }
 
@@ -402,7 +441,8 @@ int qmc_walkers_create(qmc_problem p, int32_t n_replicas, int32_t replicas_per_g
   HIP_TRY(hipMemset(w->accepted.p, 0, w->R * sizeof(unsigned long long)));
   HIP_TRY(hipMemset(w->label.p, 0, w->R * sizeof(int32_t)));
   HIP_TRY(w->d_units.alloc(w->n_units));
-  build_units(w.get());
+  HIP_TRY(w->progress.alloc(2 * (size_t)w->n_units));
+  HIP_TRY(hipMemset(w->progress.p, 0, w->progress.n * sizeof(int64_t)));
   int rc = upload_units(w.get());
   if (rc) return rc;
   *out = w.release();
@@ -458,58 +498,43 @@ int qmc_walkers_set_betas(qmc_walkers w, const double* betas) {
   return QMC_OK;
 }
 
-static int do_sweep(qmc_walkers w, int64_t sweep0, int32_t n_sweeps, int beta_mode, int ordered,
-                    int spr, bool timed) {
-  qmc_problem p = w->prob;
-  qmc_ctx c = p->ctx;
+static int launch_units(qmc_problem p, const UnitDev* d_units, int n_units, int block, int stride,
+                        int32_t max_sweeps, int ordered, hipStream_t st) {
   SweepArgs a{};
-  a.sweep0 = sweep0;
-  a.n_sweeps = n_sweeps;
-  a.beta_mode = beta_mode;
-  a.sched = w->sched.p;
-  a.lut = c->lut.p;
-  a.spr = spr > 0 ? spr : 1;
-  const size_t smem = sweep_smem_bytes(w->stride, ordered);
-  if (timed) HIP_TRY(hipEventRecord(c->ev0, c->stream));
-  HIP_TRY(launch_sweep(p->kind, p->pairwise, p->coupling, w->d_units.p, w->n_units, p->lanes * w->G,
-                       smem, a, ordered, c->stream));
-  if (timed) HIP_TRY(hipEventRecord(c->ev1, c->stream));
+  a.n_sweeps = max_sweeps;
+  a.lut = p->ctx->lut.p;
+  const size_t smem = sweep_smem_bytes(stride, ordered);
+  HIP_TRY(launch_sweep(p->kind, p->pairwise, p->coupling, d_units, n_units, block, smem, a, ordered, st));
   return QMC_OK;
 }
 
-int qmc_walkers_sweep(qmc_walkers w, int64_t sweep0, int32_t n_sweeps, int beta_mode,
-                      const double* schedule, int ordered) {
+static int upload_schedule(qmc_walkers w, int64_t sweep0, int32_t n, const double* schedule) {
+  if (w->sched.n < (size_t)n) HIP_TRY(w->sched.alloc(n));
+  HIP_TRY(hipMemcpy(w->sched.p, schedule, n * sizeof(double), hipMemcpyHostToDevice));
+  w->sched_base = sweep0;
+  return QMC_OK;
+}
+
+int qmc_walkers_plan(qmc_walkers w, int64_t sweep0, int32_t n_sweeps, const double* schedule) {
   if (!w) return fail(QMC_INVALID_ARGUMENT, "null handle");
-  if (n_sweeps <= 0) return QMC_OK;
-  if (beta_mode == QMC_BETA_SCHEDULE) {
-    if (!schedule) return fail(QMC_INVALID_ARGUMENT, "schedule required");
-    if (w->sched.n < (size_t)n_sweeps) HIP_TRY(w->sched.alloc(n_sweeps));
-    HIP_TRY(hipMemcpyAsync(w->sched.p, schedule, n_sweeps * sizeof(double), hipMemcpyHostToDevice,
-                           w->prob->ctx->stream));
+  if (n_sweeps < 0) return fail(QMC_INVALID_ARGUMENT, "n_sweeps must be >= 0");
+  int rc;
+  w->ladder = 0;
+  w->spr = 1;
+  if (schedule) {
+    if ((rc = upload_schedule(w, sweep0, n_sweeps, schedule))) return rc;
+  } else {
+    w->sched.release();
   }
-  if (w->ladder) {
-    for (auto& U : w->units) U.ladder = 0;
-    w->ladder = 0;
-    int rc = upload_units(w);
-    if (rc) return rc;
-  }
-  int rc = do_sweep(w, sweep0, n_sweeps, beta_mode, ordered, 1, true);
-  if (rc) return rc;
-  HIP_TRY(hipStreamSynchronize(w->prob->ctx->stream));
-  HIP_TRY(hipEventElapsedTime(&w->last_ms, w->prob->ctx->ev0, w->prob->ctx->ev1));
-  return QMC_OK;
+  if ((rc = upload_units(w))) return rc;
+  return set_progress(w, sweep0, n_sweeps);
 }
 
-int qmc_walkers_pt(qmc_walkers w, int32_t ladder, const double* ladder_betas, int64_t step0,
-                   int32_t n_steps, int32_t sweeps_per_replica) {
-  if (!w || !ladder_betas) return fail(QMC_INVALID_ARGUMENT, "null argument");
+static int set_ladder(qmc_walkers w, int32_t ladder, const double* ladder_betas, int64_t step0) {
   if (ladder < 1 || ladder > 32 || w->R % ladder != 0)
     return fail(QMC_INVALID_ARGUMENT, "ladder must divide the replica count and be <= 32");
-  if (w->per_seed != ladder) return fail(QMC_INVALID_ARGUMENT, "create the walkers with replicas_per_seed == ladder");
-  if (sweeps_per_replica < 1) return fail(QMC_INVALID_ARGUMENT, "sweeps_per_replica must be >= 1");
-  qmc_problem p = w->prob;
-  hipStream_t st = p->ctx->stream;
-  const int n_ladders = w->R / ladder;
+  if (w->per_seed != ladder)
+    return fail(QMC_INVALID_ARGUMENT, "create the walkers with replicas_per_seed == ladder");
   if (step0 == 0 || w->ladder != ladder) {
     // fresh ladder: slot k holds beta index k
     std::vector<int32_t> lab(w->R);
@@ -522,38 +547,199 @@ int qmc_walkers_pt(qmc_walkers w, int32_t ladder, const double* ladder_betas, in
     HIP_TRY(hipMemcpy(w->beta.p, be.data(), w->R * sizeof(double), hipMemcpyHostToDevice));
   }
   HIP_TRY(hipMemcpy(w->ladder_beta.p, ladder_betas, ladder * sizeof(double), hipMemcpyHostToDevice));
-  const bool in_unit = (w->B == ladder);
-  if (in_unit) {
-    if (w->ladder != ladder) {
-      for (auto& U : w->units) U.ladder = ladder;
-      w->ladder = ladder;
-      int rc = upload_units(w);
-      if (rc) return rc;
-    }
-    int rc = do_sweep(w, step0 * sweeps_per_replica, n_steps * sweeps_per_replica,
-                      QMC_BETA_PER_REPLICA, 0, sweeps_per_replica, true);
-    if (rc) return rc;
+  return QMC_OK;
+}
+
+int qmc_walkers_plan_pt(qmc_walkers w, int32_t ladder, const double* ladder_betas, int64_t step0,
+                        int32_t n_steps, int32_t sweeps_per_replica) {
+  if (!w || !ladder_betas) return fail(QMC_INVALID_ARGUMENT, "null argument");
+  if (w->B != ladder) return fail(QMC_INVALID_ARGUMENT, "batched PT needs replicas_per_group == ladder");
+  if (sweeps_per_replica < 1) return fail(QMC_INVALID_ARGUMENT, "sweeps_per_replica must be >= 1");
+  int rc = set_ladder(w, ladder, ladder_betas, step0);
+  if (rc) return rc;
+  w->ladder = ladder;
+  w->spr = sweeps_per_replica;
+  w->sched.release();
+  if ((rc = upload_units(w))) return rc;
+  return set_progress(w, step0 * sweeps_per_replica, (int64_t)n_steps * sweeps_per_replica);
+}
+
+int qmc_walkers_sweep(qmc_walkers w, int64_t sweep0, int32_t n_sweeps, int beta_mode,
+                      const double* schedule, int ordered) {
+  if (!w) return fail(QMC_INVALID_ARGUMENT, "null handle");
+  if (n_sweeps <= 0) return QMC_OK;
+  if (beta_mode == QMC_BETA_SCHEDULE && !schedule) return fail(QMC_INVALID_ARGUMENT, "schedule required");
+  int rc = qmc_walkers_plan(w, sweep0, n_sweeps, beta_mode == QMC_BETA_SCHEDULE ? schedule : nullptr);
+  if (rc) return rc;
+  qmc_ctx c = w->prob->ctx;
+  HIP_TRY(hipEventRecord(c->ev0, c->stream));
+  rc = launch_units(w->prob, w->d_units.p, w->n_units, ordered ? 64 : w->prob->lanes * w->G, w->stride,
+                    n_sweeps, ordered, c->stream);
+  if (rc) return rc;
+  HIP_TRY(hipEventRecord(c->ev1, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  HIP_TRY(hipEventElapsedTime(&w->last_ms, c->ev0, c->ev1));
+  return QMC_OK;
+}
+
+int qmc_walkers_pt(qmc_walkers w, int32_t ladder, const double* ladder_betas, int64_t step0,
+                   int32_t n_steps, int32_t sweeps_per_replica) {
+  if (!w || !ladder_betas) return fail(QMC_INVALID_ARGUMENT, "null argument");
+  if (sweeps_per_replica < 1) return fail(QMC_INVALID_ARGUMENT, "sweeps_per_replica must be >= 1");
+  qmc_problem p = w->prob;
+  qmc_ctx c = p->ctx;
+  hipStream_t st = c->stream;
+  const int n_ladders = w->R / std::max(1, ladder);
+  int rc;
+  if (w->B == ladder) {
+    if ((rc = qmc_walkers_plan_pt(w, ladder, ladder_betas, step0, n_steps, sweeps_per_replica))) return rc;
+    HIP_TRY(hipEventRecord(c->ev0, st));
+    if ((rc = launch_units(p, w->d_units.p, w->n_units, p->lanes * w->G, w->stride,
+                           n_steps * sweeps_per_replica, 0, st)))
+      return rc;
+    HIP_TRY(hipEventRecord(c->ev1, st));
   } else {
     if (ladder % w->B != 0) return fail(QMC_INVALID_ARGUMENT, "ladder must be a multiple of replicas_per_group");
-    if (w->ladder) {
-      for (auto& U : w->units) U.ladder = 0;
-      int rc = upload_units(w);
-      if (rc) return rc;
-    }
+    if ((rc = set_ladder(w, ladder, ladder_betas, step0))) return rc;
+    w->ladder = 0;  // exchanges run as their own launches
+    w->spr = 1;
+    w->sched.release();
+    if ((rc = upload_units(w))) return rc;
     w->ladder = ladder;
-    HIP_TRY(hipEventRecord(p->ctx->ev0, st));
+    if ((rc = set_progress(w, step0 * sweeps_per_replica, (int64_t)n_steps * sweeps_per_replica))) return rc;
+    HIP_TRY(hipEventRecord(c->ev0, st));
     for (int32_t k = 0; k < n_steps; ++k) {
       const int64_t step = step0 + k;
-      int rc = do_sweep(w, step * sweeps_per_replica, sweeps_per_replica, QMC_BETA_PER_REPLICA, 0, 1, false);
-      if (rc) return rc;
+      if ((rc = launch_units(p, w->d_units.p, w->n_units, p->lanes * w->G, w->stride, sweeps_per_replica,
+                             0, st)))
+        return rc;
       HIP_TRY(launch_exchange(w->cost.p, w->beta.p, w->label.p, w->ladder_beta.p, n_ladders, ladder,
                               (uint32_t)step, w->seed + (uint32_t)(w->first_replica / ladder),
                               w->stream, 1, st));
     }
-    HIP_TRY(hipEventRecord(p->ctx->ev1, st));
+    HIP_TRY(hipEventRecord(c->ev1, st));
   }
   HIP_TRY(hipStreamSynchronize(st));
-  HIP_TRY(hipEventElapsedTime(&w->last_ms, p->ctx->ev0, p->ctx->ev1));
+  HIP_TRY(hipEventElapsedTime(&w->last_ms, c->ev0, c->ev1));
+  return QMC_OK;
+}
+
+int qmc_walkers_remaining(qmc_walkers w, int64_t* sweeps_left) {
+  if (!w || !sweeps_left) return fail(QMC_INVALID_ARGUMENT, "null argument");
+  std::vector<int64_t> pr(w->progress.n);
+  HIP_TRY(hipStreamSynchronize(w->prob->ctx->stream));
+  HIP_TRY(hipMemcpy(pr.data(), w->progress.p, pr.size() * sizeof(int64_t), hipMemcpyDeviceToHost));
+  int64_t m = 0;
+  for (int u = 0; u < w->n_units; ++u) m = std::max(m, pr[2 * u + 1]);
+  *sweeps_left = m;
+  return QMC_OK;
+}
+
+// ---------------------------------------------------------------- batches
+int qmc_batch_create(qmc_ctx ctx, qmc_batch* out) {
+  if (!ctx || !out) return fail(QMC_INVALID_ARGUMENT, "null handle");
+  auto* b = new qmc_batch_s();
+  b->ctx = ctx;
+  *out = b;
+  return QMC_OK;
+}
+
+void qmc_batch_destroy(qmc_batch b) {
+  if (b && b->ctx) (void)hipStreamSynchronize(b->ctx->stream);
+  delete b;
+}
+
+int qmc_batch_add(qmc_batch b, qmc_walkers w) {
+  if (!b || !w) return fail(QMC_INVALID_ARGUMENT, "null handle");
+  if (w->prob->ctx != b->ctx) return fail(QMC_INVALID_ARGUMENT, "walkers belong to another context");
+  for (auto* m : b->members)
+    if (m == w) return QMC_OK;
+  b->members.push_back(w);
+  b->seen_version.push_back(~0ull);
+  b->dirty = true;
+  return QMC_OK;
+}
+
+int qmc_batch_remove(qmc_batch b, qmc_walkers w) {
+  if (!b || !w) return fail(QMC_INVALID_ARGUMENT, "null handle");
+  for (size_t i = 0; i < b->members.size(); ++i)
+    if (b->members[i] == w) {
+      b->members.erase(b->members.begin() + i);
+      b->seen_version.erase(b->seen_version.begin() + i);
+      b->dirty = true;
+      return QMC_OK;
+    }
+  return QMC_OK;
+}
+
+int qmc_batch_clear(qmc_batch b) {
+  if (!b) return fail(QMC_INVALID_ARGUMENT, "null handle");
+  b->members.clear();
+  b->seen_version.clear();
+  b->dirty = true;
+  return QMC_OK;
+}
+
+static int batch_rebuild(qmc_batch b) {
+  bool stale = b->dirty;
+  for (size_t i = 0; i < b->members.size(); ++i)
+    if (b->seen_version[i] != b->members[i]->version) stale = true;
+  if (!stale) return QMC_OK;
+  b->groups.clear();
+  // Largest problems first, so the longest work-groups start earliest.
+  std::vector<qmc_walkers> order(b->members.begin(), b->members.end());
+  std::stable_sort(order.begin(), order.end(), [](qmc_walkers x, qmc_walkers y) {
+    return (int64_t)x->prob->n_inc * x->B > (int64_t)y->prob->n_inc * y->B;
+  });
+  for (qmc_walkers w : order) {
+    qmc_problem p = w->prob;
+    qmc_batch_s::Group* g = nullptr;
+    for (auto& gg : b->groups)
+      if (gg->kind == p->kind && gg->pairwise == p->pairwise && gg->coupling == p->coupling) g = gg.get();
+    if (!g) {
+      b->groups.push_back(std::make_unique<qmc_batch_s::Group>());
+      g = b->groups.back().get();
+      g->kind = p->kind;
+      g->pairwise = p->pairwise;
+      g->coupling = p->coupling;
+    }
+    g->block = std::max(g->block, p->lanes * w->G);
+    g->stride = std::max(g->stride, w->stride);
+    g->units.insert(g->units.end(), w->units.begin(), w->units.end());
+  }
+  for (auto& g : b->groups) {
+    HIP_TRY(g->d_units.upload(g->units.data(), g->units.size()));
+  }
+  for (size_t i = 0; i < b->members.size(); ++i) b->seen_version[i] = b->members[i]->version;
+  b->dirty = false;
+  return QMC_OK;
+}
+
+int qmc_batch_run(qmc_batch b, int32_t max_sweeps) {
+  if (!b) return fail(QMC_INVALID_ARGUMENT, "null handle");
+  int rc = batch_rebuild(b);
+  if (rc) return rc;
+  qmc_ctx c = b->ctx;
+  HIP_TRY(hipEventRecord(c->ev0, c->stream));
+  for (auto& g : b->groups) {
+    if (g->units.empty()) continue;
+    SweepArgs a{};
+    a.n_sweeps = max_sweeps;
+    a.lut = c->lut.p;
+    const size_t smem = sweep_smem_bytes(g->stride, 0);
+    if (smem > 160 * 1024) return fail(QMC_MEMORY_LIMITED, "batch LDS footprint exceeds 160 KiB");
+    HIP_TRY(launch_sweep(g->kind, g->pairwise, g->coupling, g->d_units.p, (int)g->units.size(), g->block,
+                         smem, a, 0, c->stream));
+  }
+  HIP_TRY(hipEventRecord(c->ev1, c->stream));
+  return QMC_OK;
+}
+
+int qmc_batch_wait(qmc_batch b, double* ms) {
+  if (!b) return fail(QMC_INVALID_ARGUMENT, "null handle");
+  HIP_TRY(hipStreamSynchronize(b->ctx->stream));
+  HIP_TRY(hipEventElapsedTime(&b->last_ms, b->ctx->ev0, b->ctx->ev1));
+  if (ms) *ms = b->last_ms;
   return QMC_OK;
 }
 

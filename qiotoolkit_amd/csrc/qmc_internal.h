@@ -52,19 +52,20 @@ struct UnitDev {
   unsigned long long* accepted;
   int32_t* label;
   const double* ladder_beta;
+  const double* sched;   // beta of absolute sweep s = sched[s - sched_base]; null: per-replica betas
+  int64_t sched_base;
+  int64_t* progress;     // [0] next sweep index, [1] sweeps left (updated by the kernel)
+  int32_t spr;           // PT: sweeps per exchange step
+  int32_t pad0;
   uint32_t r0;      // replica id of the unit's first replica (RNG counter)
   uint32_t nrep;
   uint32_t seed, stream;
 };
 
 struct SweepArgs {
-  int64_t sweep0;
-  int32_t n_sweeps;
-  int32_t beta_mode;
-  const double* sched;   // device, n_sweeps values (schedule mode)
-  const double* lut;     // device, 1024 values
-  int32_t spr;           // PT: sweeps per exchange step
+  int32_t n_sweeps;      // at most this many sweeps per unit in this launch
   int32_t pad;
+  const double* lut;     // device, 1024 values
 };
 
 size_t sweep_smem_bytes(int max_stride, int ordered);

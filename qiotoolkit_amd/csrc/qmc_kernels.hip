@@ -160,9 +160,12 @@ k_sweep(const UnitDev* __restrict__ units, SweepArgs a) {
 #pragma unroll
   for (int b = 0; b < 8; ++b) acc[b] = 0;
 
-  for (int s = 0; s < a.n_sweeps; ++s) {
-    const uint32_t sweep = (uint32_t)(a.sweep0 + s);
-    if (a.beta_mode == QMC_BETA_SCHEDULE && tid < nrep) S.rbeta[tid] = a.sched[s];
+  const int64_t sweep_begin = U.progress[0];
+  const int n_run = (int)min((int64_t)a.n_sweeps, U.progress[1]);
+  for (int s = 0; s < n_run; ++s) {
+    const int64_t sweep64 = sweep_begin + s;
+    const uint32_t sweep = (uint32_t)sweep64;
+    if (U.sched && tid < nrep) S.rbeta[tid] = U.sched[sweep64 - U.sched_base];
     __syncthreads();
     double beta[8];
 #pragma unroll
@@ -336,9 +339,9 @@ k_sweep(const UnitDev* __restrict__ units, SweepArgs a) {
     }  // classes
 
     // ---- parallel tempering exchange inside the unit (ladder == nrep) ----
-    if (U.ladder > 0 && ((uint32_t)(s + 1) % (uint32_t)a.spr) == 0u) {
+    if (U.ladder > 0 && ((sweep64 + 1) % U.spr) == 0) {
       __syncthreads();
-      const uint32_t step = (uint32_t)((a.sweep0 + s) / a.spr);
+      const uint32_t step = (uint32_t)(sweep64 / U.spr);
       const int L = U.ladder;
       const int i = (int)(step & 1u) + 2 * tid;
       if (i < L - 1) {
@@ -377,6 +380,10 @@ k_sweep(const UnitDev* __restrict__ units, SweepArgs a) {
     U.accepted[tid] += S.racc[tid];
     if (U.ladder > 0) U.label[tid] = S.label[tid];
   }
+  if (tid == 0) {
+    U.progress[0] = sweep_begin + n_run;
+    U.progress[1] -= n_run;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -410,9 +417,12 @@ k_sweep_ordered(const UnitDev* __restrict__ units, SweepArgs a) {
   double beta = act ? U.beta[lane] : 0.0;
   unsigned long long accn = 0;
   const CT* coef = static_cast<const CT*>(U.coef);
-  for (int s = 0; s < a.n_sweeps; ++s) {
-    const uint32_t sweep = (uint32_t)(a.sweep0 + s);
-    if (a.beta_mode == QMC_BETA_SCHEDULE) beta = a.sched[s];
+  const int64_t sweep_begin = U.progress[0];
+  const int n_run = (int)min((int64_t)a.n_sweeps, U.progress[1]);
+  for (int s = 0; s < n_run; ++s) {
+    const int64_t sweep64 = sweep_begin + s;
+    const uint32_t sweep = (uint32_t)sweep64;
+    if (U.sched) beta = U.sched[sweep64 - U.sched_base];
     for (int i = lane; i < stride / 4; i += 64)
       reinterpret_cast<uint32_t*>(start)[i] = reinterpret_cast<uint32_t*>(sp)[i];
     __syncthreads();
@@ -496,6 +506,10 @@ k_sweep_ordered(const UnitDev* __restrict__ units, SweepArgs a) {
     U.best_cost[lane] = best;
     U.beta[lane] = beta;
     U.accepted[lane] += accn;
+  }
+  if (lane == 0) {
+    U.progress[0] = sweep_begin + n_run;
+    U.progress[1] -= n_run;
   }
 }
 

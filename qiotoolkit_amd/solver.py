@@ -1,0 +1,178 @@
+"""Python binding of the host solver library (include/qiotk.h, libqiotk.so).
+
+Mirrors the reference's request/response surface: a target string, a params
+document ``{"params": {...}}`` and a cost function ``{"cost_function": {...}}``
+(app/runner.cc:35-429), answered with ``{"benchmark": ..., "solutions": ...}``.
+Errors raise :class:`SolverError` carrying the reference's error code
+(utils/error_handling.h:25-52).
+"""
+import ctypes as C
+import json
+import os
+
+from . import _qmc
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libqiotk.so")
+
+TARGETS = ("simulatedannealing.qiotoolkit", "paralleltempering.qiotoolkit", "populationannealing.cpu")
+
+EXPORTS = ["qiotk_engine_create", "qiotk_engine_destroy", "qiotk_solve", "qiotk_model_create",
+           "qiotk_model_destroy", "qiotk_solve_model", "qiotk_solve_batch", "qiotk_free",
+           "qiotk_last_error", "qiotk_parse_model", "qiotk_schedule_eval", "qiotk_describe"]
+
+
+class SolverError(RuntimeError):
+    def __init__(self, code, message):
+        super().__init__("error %d: %s" % (code, message))
+        self.code = code
+        self.message = message
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError("libqiotk.so is not built (run __graft_entry__.build())")
+    _qmc.lib()  # libqmc.so first
+    L = C.CDLL(LIB_PATH)
+    vp, P = C.c_void_p, C.POINTER
+    L.qiotk_engine_create.argtypes = [C.c_int, P(vp)]
+    L.qiotk_engine_destroy.argtypes = [vp]
+    L.qiotk_engine_destroy.restype = None
+    L.qiotk_solve.argtypes = [vp, C.c_char_p, C.c_char_p, C.c_char_p, P(vp)]
+    L.qiotk_model_create.argtypes = [vp, C.c_char_p, P(vp)]
+    L.qiotk_model_destroy.argtypes = [vp]
+    L.qiotk_model_destroy.restype = None
+    L.qiotk_solve_model.argtypes = [vp, C.c_char_p, C.c_char_p, P(vp)]
+    L.qiotk_solve_batch.argtypes = [vp, C.c_char_p, C.c_int32, P(vp), P(C.c_char_p), P(vp),
+                                    P(C.c_double)]
+    L.qiotk_free.argtypes = [vp]
+    L.qiotk_free.restype = None
+    L.qiotk_last_error.restype = C.c_char_p
+    L.qiotk_parse_model.argtypes = [C.c_char_p, P(vp)]
+    L.qiotk_schedule_eval.argtypes = [C.c_char_p, C.c_int32, P(C.c_double), P(C.c_double), P(vp)]
+    L.qiotk_describe.argtypes = [C.c_char_p, C.c_char_p, P(vp)]
+    _lib = L
+    return L
+
+
+def _take(ptr):
+    s = C.cast(ptr, C.c_char_p).value.decode()
+    lib().qiotk_free(ptr)
+    return json.loads(s)
+
+
+def _doc(x):
+    return x if isinstance(x, (str, bytes)) else json.dumps(x)
+
+
+def _enc(x):
+    x = _doc(x)
+    return x.encode() if isinstance(x, str) else x
+
+
+def _raise_from(resp):
+    if isinstance(resp, dict) and "error" in resp:
+        raise SolverError(resp["error"]["code"], resp["error"]["message"])
+    return resp
+
+
+class Engine:
+    """One device context."""
+
+    def __init__(self, device=0):
+        self.h = C.c_void_p()
+        rc = lib().qiotk_engine_create(device, C.byref(self.h))
+        if rc:
+            raise SolverError(rc, lib().qiotk_last_error().decode())
+
+    def close(self):
+        if self.h:
+            lib().qiotk_engine_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def solve(self, target, params, input_data):
+        out = C.c_void_p()
+        lib().qiotk_solve(self.h, target.encode(), _enc(params), _enc(input_data), C.byref(out))
+        return _raise_from(_take(out))
+
+    def model(self, input_data):
+        return Model(self, input_data)
+
+    def solve_batch(self, target, models, params_list):
+        n = len(models)
+        hs = (C.c_void_p * n)(*[m.h for m in models])
+        ps = (C.c_char_p * n)(*[_enc(p) for p in params_list])
+        outs = (C.c_void_p * n)()
+        ms = C.c_double()
+        rc = lib().qiotk_solve_batch(self.h, target.encode(), n, hs, ps, outs, C.byref(ms))
+        if rc:
+            for o in outs:
+                if o:
+                    lib().qiotk_free(o)
+            raise SolverError(rc, lib().qiotk_last_error().decode())
+        return [_raise_from(_take(o)) for o in outs], ms.value
+
+
+class Model:
+    """A cost function parsed, normalised and resident on the device."""
+
+    def __init__(self, engine, input_data):
+        self.engine = engine
+        self.h = C.c_void_p()
+        rc = lib().qiotk_model_create(engine.h, _enc(input_data), C.byref(self.h))
+        if rc:
+            raise SolverError(rc, lib().qiotk_last_error().decode())
+
+    def close(self):
+        if self.h:
+            lib().qiotk_model_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def solve(self, target, params):
+        out = C.c_void_p()
+        lib().qiotk_solve_model(self.h, target.encode(), _enc(params), C.byref(out))
+        return _raise_from(_take(out))
+
+
+def parse_model(input_data):
+    """graph::normalize_edges of a cost function (no device needed)."""
+    out = C.c_void_p()
+    lib().qiotk_parse_model(_enc(input_data), C.byref(out))
+    return _raise_from(_take(out))
+
+
+def schedule_eval(schedule, xs):
+    """Schedule::get_value at xs, plus start/stop/count/discretised values."""
+    import numpy as np
+    x = np.ascontiguousarray(xs, dtype=np.float64)
+    y = np.zeros_like(x)
+    out = C.c_void_p()
+    lib().qiotk_schedule_eval(_enc(schedule), len(x), x.ctypes.data_as(C.POINTER(C.c_double)),
+                              y.ctypes.data_as(C.POINTER(C.c_double)), C.byref(out))
+    meta = _raise_from(_take(out))
+    return y, meta
+
+
+def describe(target, params):
+    """Solver::configure without a device: the resolved steps and betas."""
+    out = C.c_void_p()
+    lib().qiotk_describe(target.encode(), _enc(params), C.byref(out))
+    return _raise_from(_take(out))

@@ -157,3 +157,37 @@ def test_pt_matches_oracle(ctx, name, B, L):
         assert np.array_equal(got["lowest"][sl], want["lowest"])
         assert np.array_equal(got["lowest_states"][sl], want["lowest_states"])
         assert np.allclose(got["beta"][sl], want["slot_beta"])
+
+
+def test_batch_mixed_problems_same_as_alone(ctx):
+    """Work-groups of different problems, sizes and kernel variants launched
+    together, in chunks, give exactly the results of each run alone."""
+    specs = [("ising", wl.er_ising(60, 6.0, seed=31), 16, 8, 3),
+             ("ising", wl.er_ising(300, 6.0, seed=32), 24, 8, 4),
+             ("pubo", wl.random_pubo(100, 400, seed=33), 8, 8, 5),
+             ("ising", wl.lattice2d(8, seed=34), 32, 32, 6)]
+    sweeps = 30
+    alone, mixed, keep = [], [], []
+    for kind, terms, R, B, seed in specs:
+        op, qp = pair(ctx, kind, terms)
+        keep.append(qp)
+        betas = np.linspace(0.2, 2.5, sweeps) * (1 + 0.1 * seed)
+        w = _qmc.Walkers(qp, R, B, seed=seed)
+        w.init()
+        w.sweep(0, schedule=betas)
+        alone.append(w.read())
+        w2 = _qmc.Walkers(qp, R, B, seed=seed)
+        w2.init()
+        w2.plan(0, sweeps, betas)
+        mixed.append(w2)
+    batch = _qmc.Batch(ctx)
+    for w2 in mixed:
+        batch.add(w2)
+    for _ in range(4):
+        batch.run(8)
+        batch.wait()
+    assert all(w2.remaining() == 0 for w2 in mixed)
+    for a, w2 in zip(alone, mixed):
+        b = w2.read()
+        for k in ("states", "cost", "lowest", "lowest_states", "accepted"):
+            assert np.array_equal(a[k], b[k]), k

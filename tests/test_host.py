@@ -1,0 +1,135 @@
+"""Host-side logic of the product (no GPU): libraries load and export every
+declared symbol, and the C++ host layer agrees with the oracle on
+normalisation, schedules and the resolved solver plans."""
+import ctypes
+import json
+import os
+import re
+
+import numpy as np
+import pytest
+
+from oracle import pyoracle as po
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLD = os.path.join(ROOT, "tests", "golden")
+
+
+def declared(header):
+    text = open(os.path.join(ROOT, "include", header)).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(q(?:mc|iotk)_[a-z0-9_]+)\s*\(", text)))
+
+
+@pytest.mark.parametrize("header,libname", [("qmc.h", "libqmc.so"), ("qiotk.h", "libqiotk.so")])
+def test_library_exports_every_declared_symbol(header, libname):
+    from qiotoolkit_amd import _qmc, solver  # noqa: F401  (loads libqmc first)
+    _qmc.lib()
+    lib = ctypes.CDLL(os.path.join(ROOT, "qiotoolkit_amd", libname))
+    names = declared(header)
+    assert len(names) > 5
+    for n in names:
+        assert hasattr(lib, n), n
+    listed = _qmc.EXPORTS if header == "qmc.h" else solver.EXPORTS
+    assert sorted(listed) == names
+
+
+def test_no_device_fails_loudly():
+    from qiotoolkit_amd import _qmc
+    try:
+        import torch
+        if torch.cuda.is_available():
+            pytest.skip("a GPU is present")
+    except ImportError:
+        pass
+    with pytest.raises(_qmc.QmcError):
+        _qmc.Context(0)
+
+
+def load(name):
+    return json.load(open(os.path.join(GOLD, name)))
+
+
+@pytest.mark.parametrize("fname", ["ising1.json", "cpupubojobstest.json", "isingemptymixed.json",
+                                   "puboemptymixed.json", "isingempty.json",
+                                   "e2e/combined_4_local_N_16_qualitytest/combined_4_local_N_16_inst_1.json"])
+def test_normalisation_matches_oracle(fname):
+    from qiotoolkit_amd import solver as S
+    doc = load(fname)
+    cf = doc["cost_function"]
+    m = S.parse_model(doc)
+    op = po.Problem(cf["type"], cf["terms"])
+    assert m["n_vars"] == op.n
+    assert m["const_cost"] == op.const_cost
+    if op.n:
+        assert m["names"] == op.names().tolist()
+        off, ids, c = op.edges()
+        assert m["term_offsets"] == off.tolist()
+        assert m["term_vars"] == ids.tolist()
+        assert m["coeffs"] == c.tolist()
+
+
+def test_normalisation_errors():
+    from qiotoolkit_amd import solver as S
+    with pytest.raises(S.SolverError) as e:
+        S.parse_model({"cost_function": {"type": "ising", "terms": [{"c": 1, "ids": [3, 3]}]}})
+    assert e.value.code == 105 and "Duplicated" in e.value.message
+    m = S.parse_model({"cost_function": {"type": "pubo", "terms": [{"c": 1, "ids": [3, 3, 4]}]}})
+    assert m["term_vars"] == [0, 1]
+    with pytest.raises(S.SolverError, match="greater than 0"):
+        S.parse_model({"cost_function": {"type": "ising", "terms": []}})
+    with pytest.raises(S.SolverError, match="version"):
+        S.parse_model({"cost_function": {"type": "ising", "version": "-7", "terms": [{"c": 1, "ids": [0]}]}})
+    with pytest.raises(S.SolverError, match="not found"):
+        S.parse_model({"cost_function": {"type": "ising", "terms": [{"c": 1, "ids": [0, 1]}],
+                                         "initial_configuration": {"2": 1}}})
+
+
+def test_schedules_known_values():
+    from qiotoolkit_amd import solver as S
+    y, _ = S.schedule_eval({"type": "linear", "initial": 3, "final": 5}, [-1, 0, 0.5, 1, 2])
+    assert y.tolist() == [3, 3, 4, 5, 5]
+    y, _ = S.schedule_eval({"type": "geometric", "initial": 3, "final": 6}, [0, 0.5, 1])
+    assert y[0] == 3 and y[2] == 6 and abs(y[1] - 3 * 2 ** 0.5) < 1e-15
+    _, meta = S.schedule_eval({"type": "geometric", "initial": 10, "final": 1, "count": 100}, [0])
+    assert abs(meta["values"][70] - 1.9630406500402637) < 1e-14
+    y, meta = S.schedule_eval({"type": "linear", "initial": 3, "final": 5, "start": 2, "stop": 4,
+                               "repeat": True}, [1.5, 2.5, 6.5])
+    assert y.tolist() == [4.5, 3.5, 3.5]  # schedule/test/linear_test.cc:83-92
+    _, meta = S.schedule_eval([0.1, 0.5, 1, 2, 4], [0])
+    assert meta["values"] == [0.1, 0.5, 1, 2, 4]
+
+
+@pytest.mark.parametrize("b0,b1,sweeps", [(0.1, 3.0, 1000), (0.01, 5.0, 2000), (0.5, 1.0, 3)])
+def test_sa_plan_matches_oracle_schedule(b0, b1, sweeps):
+    from qiotoolkit_amd import solver as S
+    d = S.describe("simulatedannealing.qiotoolkit",
+                   {"params": {"beta_start": b0, "beta_stop": b1, "sweeps": sweeps, "restarts": 7}})
+    steps = max(sweeps, 10)
+    assert d["steps"] == steps and d["restarts"] == 7
+    stop = max(1, sweeps - 1)
+    want = [po.lib().orc_schedule_value(0, b0, b1, 0, stop, s) for s in range(steps)]
+    assert d["betas"] == want
+
+
+def test_sa_plan_errors():
+    from qiotoolkit_amd import solver as S
+    with pytest.raises(S.SolverError, match="greater than 1"):
+        S.describe("simulatedannealing.qiotoolkit", {"params": {"sweeps": 1}})
+    with pytest.raises(S.SolverError, match="beta_stop"):
+        S.describe("simulatedannealing.qiotoolkit", {"params": {"beta_start": 2, "beta_stop": 1}})
+    with pytest.raises(S.SolverError, match="Unknown solver"):
+        S.describe("tabu.qiotoolkit", {"params": {}})
+
+
+def test_pt_plan_ladder():
+    from qiotoolkit_amd import solver as S
+    d = S.describe("paralleltempering.qiotoolkit",
+                   {"params": {"temperatures": {"type": "geometric", "initial": 0.2, "final": 2.0,
+                                                "count": 32}, "sweeps": 5000}})
+    t = np.zeros(32)
+    import ctypes as C
+    po.lib().orc_schedule_discretized(1, 0.2, 2.0, 32, t.ctypes.data_as(C.POINTER(C.c_double)))
+    assert d["betas"] == [1.0 / x for x in sorted(t)]
+    d = S.describe("paralleltempering.qiotoolkit", {"params": {"sweeps": 100}})
+    assert d["betas"] == [4, 2, 1, 0.5, 0.1]  # default all_betas, descending
