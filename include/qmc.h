@@ -121,6 +121,13 @@ int qmc_walkers_plan(qmc_walkers w, int64_t sweep0, int32_t n_sweeps, const doub
  * ladder): n_steps steps of sweeps_per_replica sweeps, exchanges after each. */
 int qmc_walkers_plan_pt(qmc_walkers w, int32_t ladder, const double* ladder_betas, int64_t step0,
                         int32_t n_steps, int32_t sweeps_per_replica);
+/* Population resampling (solver/population.h:171-218, Population::resample):
+ * replica r of the new population, r < new_R, becomes a copy of replica
+ * src[r] of the current one -- state, lowest state, costs, beta and counters.
+ * new_R may not exceed the replica count the walkers were created with. */
+int qmc_walkers_resample(qmc_walkers w, const int32_t* src, int32_t new_R);
+/* Current number of replicas. */
+int qmc_walkers_count(qmc_walkers w, int32_t* n_replicas);
 /* Largest number of planned sweeps not yet run by any work-group. */
 int qmc_walkers_remaining(qmc_walkers w, int64_t* sweeps_left);
 

@@ -25,7 +25,7 @@ EXPORTS = [
     "qmc_walkers_labels", "qmc_energy", "qmc_delta_all", "qmc_philox", "qmc_walkers_last_ms",
     "qmc_walkers_plan", "qmc_walkers_plan_pt", "qmc_walkers_remaining", "qmc_batch_create",
     "qmc_batch_destroy", "qmc_batch_add", "qmc_batch_remove", "qmc_batch_clear", "qmc_batch_run",
-    "qmc_batch_wait",
+    "qmc_batch_wait", "qmc_walkers_resample", "qmc_walkers_count",
 ]
 
 
@@ -95,6 +95,8 @@ def lib():
     L.qmc_batch_clear.argtypes = [vp]
     L.qmc_batch_run.argtypes = [vp, C.c_int32]
     L.qmc_batch_wait.argtypes = [vp, P(C.c_double)]
+    L.qmc_walkers_resample.argtypes = [vp, P(C.c_int32), C.c_int32]
+    L.qmc_walkers_count.argtypes = [vp, P(C.c_int32)]
     _lib = L
     return L
 
@@ -269,6 +271,11 @@ class Walkers:
         b = np.ascontiguousarray(ladder_betas, dtype=np.float64)
         _check(lib().qmc_walkers_plan_pt(self.h, len(b), _p(b, C.c_double), step0, n_steps,
                                          sweeps_per_replica))
+
+    def resample(self, src):
+        s = np.ascontiguousarray(src, dtype=np.int32)
+        _check(lib().qmc_walkers_resample(self.h, _p(s, C.c_int32), len(s)))
+        self.R = len(s)
 
     def remaining(self):
         v = C.c_int64()

@@ -5,6 +5,7 @@
 // simulated_annealing.h:123-209, parallel_tempering.h:192-234,
 // population_annealing.h:632-778).
 #pragma once
+#include <limits>
 #include <memory>
 #include <optional>
 #include <string>
@@ -152,6 +153,10 @@ class PopulationAnnealing : public Solver {
   Schedule beta_;
   double beta_start_ = 0.0, beta_stop_ = 5.0;
   int sweeps_per_replica_ = 1;
+  double cur_beta_ = 0.0;
+  int64_t restarts_ = 0;
+  std::vector<size_t> family_;
+  std::shared_ptr<void> impl_;
 };
 
 std::unique_ptr<Solver> create_solver(const std::string& target);

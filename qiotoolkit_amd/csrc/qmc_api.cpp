@@ -109,6 +109,7 @@ struct qmc_problem_s {
 struct qmc_walkers_s {
   qmc_problem prob = nullptr;
   int32_t R = 0, B = 1, G = 1, n_units = 0, stride = 0;
+  int32_t cap = 0, cap_units = 0;  // allocated replicas / work-groups
   int32_t per_seed = 0;
   uint32_t seed = 0, stream = 0;
   int32_t first_replica = 0;
@@ -417,6 +418,8 @@ int qmc_walkers_create(qmc_problem p, int32_t n_replicas, int32_t replicas_per_g
   if (p->lanes * w->G > 512)
     return fail(QMC_INVALID_ARGUMENT, "lanes * replica bytes exceeds the 512-thread work-group");
   w->n_units = (w->R + w->B - 1) / w->B;
+  w->cap = w->R;
+  w->cap_units = w->n_units;
   w->stride = (int32_t)(((int64_t)(p->N + 1) * w->G + 3) / 4 * 4);
   w->seed = seed;
   w->stream = stream;
@@ -455,9 +458,54 @@ void qmc_walkers_destroy(qmc_walkers w) {
 }
 
 static int ensure_scratch(qmc_walkers w) {
-  const size_t need = (size_t)w->R * w->prob->N;
+  const size_t need = (size_t)w->cap * w->prob->N;
   if (w->scratch_states.n < need) HIP_TRY(w->scratch_states.alloc(need));
-  if (w->energies.n < (size_t)w->R) HIP_TRY(w->energies.alloc(w->R));
+  if (w->energies.n < (size_t)w->cap) HIP_TRY(w->energies.alloc(w->cap));
+  return QMC_OK;
+}
+
+int qmc_walkers_resample(qmc_walkers w, const int32_t* src, int32_t new_R) {
+  if (!w || (!src && new_R > 0)) return fail(QMC_INVALID_ARGUMENT, "null argument");
+  if (new_R < 1 || new_R > w->cap)
+    return fail(QMC_MEMORY_LIMITED, "population exceeds the walker capacity");
+  for (int32_t r = 0; r < new_R; ++r)
+    if (src[r] < 0 || src[r] >= w->R) return fail(QMC_INVALID_ARGUMENT, "resample source out of range");
+  qmc_problem p = w->prob;
+  hipStream_t st = p->ctx->stream;
+  int rc = ensure_scratch(w);
+  if (rc) return rc;
+  const size_t n = (size_t)w->R * p->N;
+  DevBuf<uint8_t> low;
+  DevBuf<int32_t> dsrc;
+  DevBuf<double> c0, c1, c2;
+  DevBuf<unsigned long long> a0;
+  HIP_TRY(low.alloc(std::max<size_t>(n, 1)));
+  HIP_TRY(dsrc.upload(src, new_R));
+  HIP_TRY(c0.alloc(new_R));
+  HIP_TRY(c1.alloc(new_R));
+  HIP_TRY(c2.alloc(new_R));
+  HIP_TRY(a0.alloc(new_R));
+  HIP_TRY(launch_unpack(w->d_units.p, w->B, w->R, p->N, 0, w->scratch_states.p, st));
+  HIP_TRY(launch_unpack(w->d_units.p, w->B, w->R, p->N, 1, low.p, st));
+  HIP_TRY(launch_gather_replicas(new_R, dsrc.p, w->cost.p, w->best_cost.p, w->beta.p, w->accepted.p,
+                                 c0.p, c1.p, c2.p, a0.p, st));
+  HIP_TRY(hipMemcpyAsync(w->cost.p, c0.p, new_R * sizeof(double), hipMemcpyDeviceToDevice, st));
+  HIP_TRY(hipMemcpyAsync(w->best_cost.p, c1.p, new_R * sizeof(double), hipMemcpyDeviceToDevice, st));
+  HIP_TRY(hipMemcpyAsync(w->beta.p, c2.p, new_R * sizeof(double), hipMemcpyDeviceToDevice, st));
+  HIP_TRY(hipMemcpyAsync(w->accepted.p, a0.p, new_R * sizeof(unsigned long long), hipMemcpyDeviceToDevice, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  w->R = new_R;
+  w->n_units = (new_R + w->B - 1) / w->B;
+  if ((rc = upload_units(w))) return rc;
+  HIP_TRY(launch_pack_gather(w->d_units.p, w->n_units, w->B, new_R, p->N, dsrc.p, w->scratch_states.p,
+                             low.p, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  return QMC_OK;
+}
+
+int qmc_walkers_count(qmc_walkers w, int32_t* n_replicas) {
+  if (!w || !n_replicas) return fail(QMC_INVALID_ARGUMENT, "null argument");
+  *n_replicas = w->R;
   return QMC_OK;
 }
 

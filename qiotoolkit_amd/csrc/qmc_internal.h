@@ -79,6 +79,11 @@ hipError_t launch_unpack(const UnitDev* units, int B, int R, int N, int which, u
                          hipStream_t st);
 hipError_t launch_pack(const UnitDev* units, int n_units, int B, int R, int N, const uint8_t* in,
                        int which, hipStream_t st);
+hipError_t launch_pack_gather(const UnitDev* units, int n_units, int B, int R, int N,
+                              const int32_t* src, const uint8_t* cur, const uint8_t* low, hipStream_t st);
+hipError_t launch_gather_replicas(int n, const int32_t* src, const double* c0, const double* c1,
+                                  const double* c2, const unsigned long long* a0, double* o0,
+                                  double* o1, double* o2, unsigned long long* oa, hipStream_t st);
 hipError_t launch_energy(const ProblemDev& P, const uint8_t* states, int n, double* out,
                          hipStream_t st);
 hipError_t launch_delta_all(const ProblemDev& P, int n_states, const uint8_t* states, double* out,

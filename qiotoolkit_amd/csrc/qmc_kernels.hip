@@ -602,6 +602,48 @@ __global__ void k_pack(const UnitDev* __restrict__ units, int B, int R, const ui
   }
 }
 
+// Population resampling: replica r of the new population is a copy of
+// replica src[r] of the old one (plain layout in, unit layout out).
+__global__ void k_pack_gather(const UnitDev* __restrict__ units, int B, int R,
+                              const int32_t* __restrict__ src, const uint8_t* __restrict__ cur,
+                              const uint8_t* __restrict__ low) {
+  const UnitDev U = units[blockIdx.y];
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q > U.N) return;
+  const int u = blockIdx.y;
+  for (int g = 0; g < U.G; ++g) {
+    uint32_t bc = 0, bl = 0;
+    if (q < U.N) {
+      const int v = U.var_of_pos[q];
+      for (int b = 0; b < 8; ++b) {
+        const int rr = 8 * g + b;
+        const int r = u * B + rr;
+        if (rr < (int)U.nrep && r < R) {
+          const int64_t s = (int64_t)src[r] * U.N + v;
+          bc |= ((uint32_t)(cur[s] & 1u)) << b;
+          bl |= ((uint32_t)(low[s] & 1u)) << b;
+        }
+      }
+    }
+    U.spins[q * U.G + g] = (uint8_t)bc;
+    U.best[q * U.G + g] = (uint8_t)bl;
+  }
+}
+
+__global__ void k_gather_replicas(int n, const int32_t* __restrict__ src, const double* __restrict__ c0,
+                                  const double* __restrict__ c1, const double* __restrict__ c2,
+                                  const unsigned long long* __restrict__ a0, double* __restrict__ o0,
+                                  double* __restrict__ o1, double* __restrict__ o2,
+                                  unsigned long long* __restrict__ oa) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  const int s = src[r];
+  o0[r] = c0[s];
+  o1[r] = c1[s];
+  o2[r] = c2[s];
+  oa[r] = a0[s];
+}
+
 // FP64 energy of plain states [s][var]: term CSR in variable ids.
 __global__ void k_energy(const int64_t* __restrict__ term_off, const int32_t* __restrict__ term_var,
                          const double* __restrict__ term_c, int64_t M, int kind, int N,
@@ -731,6 +773,22 @@ hipError_t launch_unpack(const UnitDev* units, int B, int R, int N, int which, u
 hipError_t launch_pack(const UnitDev* units, int n_units, int B, int R, int N, const uint8_t* in,
                        int which, hipStream_t st) {
   hipLaunchKernelGGL(k_pack, dim3((N + 1 + 127) / 128, n_units), dim3(128), 0, st, units, B, R, in, which);
+  return hipGetLastError();
+}
+
+hipError_t launch_pack_gather(const UnitDev* units, int n_units, int B, int R, int N,
+                              const int32_t* src, const uint8_t* cur, const uint8_t* low, hipStream_t st) {
+  hipLaunchKernelGGL(k_pack_gather, dim3((N + 1 + 127) / 128, n_units), dim3(128), 0, st, units, B, R,
+                     src, cur, low);
+  return hipGetLastError();
+}
+
+hipError_t launch_gather_replicas(int n, const int32_t* src, const double* c0, const double* c1,
+                                  const double* c2, const unsigned long long* a0, double* o0,
+                                  double* o1, double* o2, unsigned long long* oa, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_gather_replicas, dim3((n + 255) / 256), dim3(256), 0, st, n, src, c0, c1, c2,
+                     a0, o0, o1, o2, oa);
   return hipGetLastError();
 }
 

@@ -1,0 +1,149 @@
+"""End-to-end solves through the host library's JSON API on the GPU.
+
+* the reference's solver fixtures (data/*.json with params1.json) reach the
+  asserted costs and configurations;
+* planted optima shipped with the reference are found;
+* every returned configuration's cost equals the oracle's cost function
+  (bit-exact for integer couplings, 1e-5 relative for fp32);
+* distribution parity: best energies over 100 seeds, GPU vs the oracle running
+  the reference algorithm (pcg32, index order), two-sample KS p > 0.05 and
+  means within 0.5%.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from oracle import pyoracle as po
+from qiotoolkit_amd import workloads as wl
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+@pytest.fixture(scope="module")
+def engine():
+    from qiotoolkit_amd.solver import Engine
+    e = Engine(0)
+    yield e
+    e.close()
+
+
+def load(name):
+    return json.load(open(os.path.join(GOLD, name)))
+
+
+def oracle_cost(doc, configuration):
+    """Reference cost function of a rendered configuration (+ const cost)."""
+    cf = doc["cost_function"]
+    p = po.Problem(cf["type"], cf["terms"])
+    names = p.names()
+    if cf["type"] == "ising":
+        bits = np.array([1 if configuration[str(n)] == -1 else 0 for n in names], dtype=np.uint8)
+    else:
+        bits = np.array([1 if configuration[str(n)] == 0 else 0 for n in names], dtype=np.uint8)
+    return p.cost(bits) + p.const_cost
+
+
+FIXTURES = [
+    ("ising1.json", -10, None),
+    ("cpupubojobstest.json", -12, None),
+    ("isingemptymixed.json", -100, {"1": -1, "10": 1}),
+    ("puboemptymixed.json", -104, {"1": 1, "12": 1}),
+    ("isingempty.json", 12, None),
+    ("puboempty.json", 2, None),
+]
+
+
+@pytest.mark.parametrize("fname,cost,config", FIXTURES)
+@pytest.mark.parametrize("target", ["simulatedannealing.qiotoolkit", "paralleltempering.qiotoolkit",
+                                    "populationannealing.cpu"])
+def test_reference_fixtures(engine, fname, cost, config, target):
+    params = load("params1.json")
+    if target == "populationannealing.cpu":
+        params = {"params": dict(params["params"], population=64)}
+    doc = load(fname)
+    res = engine.solve(target, params, doc)
+    sol = res["solutions"]
+    assert sol["cost"] == cost
+    if config:
+        for k, v in config.items():
+            assert sol["configuration"][k] == v
+    if "configuration" in sol:
+        assert oracle_cost(doc, sol["configuration"]) == sol["cost"]
+        for s in sol["solutions"]:
+            assert oracle_cost(doc, s["configuration"]) == s["cost"]
+
+
+def test_solver_errors(engine):
+    from qiotoolkit_amd.solver import SolverError
+    doc = load("ising1.json")
+    with pytest.raises(SolverError) as e:
+        engine.solve("simulatedannealing.qiotoolkit", {"params": {"sweeps": 1}}, doc)
+    assert e.value.code == 103
+    with pytest.raises(SolverError):
+        engine.solve("simulatedannealing.qiotoolkit", {}, doc)
+
+
+@pytest.mark.parametrize("family", ["wishart_planting_N_16_alpha_0.81_qualitytest",
+                                    "combined_4_local_N_16_qualitytest"])
+@pytest.mark.parametrize("target", ["simulatedannealing.qiotoolkit", "paralleltempering.qiotoolkit"])
+def test_planted_optima(engine, family, target):
+    cases = json.load(open(os.path.join(GOLD, "e2e", family, "energies.json")))["cases"]
+    for case in cases:
+        doc = json.load(open(os.path.join(GOLD, "e2e", family, case["file"])))
+        scale = max(abs(t["c"]) for t in doc["cost_function"]["terms"])
+        if target.startswith("simulated"):
+            p = {"beta_start": 0.1 / scale, "beta_stop": 10.0 / scale, "sweeps": 400, "restarts": 64,
+                 "seed": 5}
+        else:
+            p = {"all_betas": [10.0 / scale * 0.7 ** k for k in range(16)], "sweeps": 400, "seed": 5}
+        res = engine.solve(target, {"params": p}, doc)
+        assert res["solutions"]["cost"] == case["energy"]
+        assert oracle_cost(doc, res["solutions"]["configuration"]) == case["energy"]
+
+
+def test_fp32_returned_energies_within_tolerance(engine):
+    terms = wl.lattice3d(8, seed=3)
+    doc = {"cost_function": {"type": "ising", "terms": terms}}
+    res = engine.solve("paralleltempering.qiotoolkit",
+                       {"params": {"temperatures": {"type": "geometric", "initial": 0.2, "final": 2.0,
+                                                    "count": 16}, "sweeps": 300, "seed": 9,
+                                   "number_of_solutions": 5}}, doc)
+    for s in res["solutions"]["solutions"]:
+        ref = oracle_cost(doc, s["configuration"])
+        assert abs(ref - s["cost"]) <= 1e-5 * abs(ref)
+
+
+def ks_2samp(a, b):
+    from scipy.stats import ks_2samp as ks
+    return ks(a, b).pvalue
+
+
+@pytest.mark.parametrize("kind", ["lattice2d", "pubo"])
+def test_best_energy_distribution_matches_reference(engine, kind):
+    if kind == "lattice2d":
+        terms = wl.lattice2d(12, seed=41)
+        p0 = {"beta_start": 0.1, "beta_stop": 3.0, "sweeps": 60, "restarts": 4}
+    else:
+        terms = wl.random_pubo(150, 1500, seed=42)
+        p0 = {"beta_start": 0.01, "beta_stop": 5.0, "sweeps": 60, "restarts": 4}
+    ptype = "ising" if kind == "lattice2d" else "pubo"
+    doc = {"cost_function": {"type": ptype, "terms": terms}}
+    seeds = list(range(100, 200))
+    model = engine.model(doc)
+    res, _ = engine.solve_batch("simulatedannealing.qiotoolkit", [model] * len(seeds),
+                                [{"params": dict(p0, seed=s)} for s in seeds])
+    gpu = np.array([r["solutions"]["cost"] for r in res])
+    op = po.Problem(ptype, terms)
+    cpu = []
+    for s in seeds:
+        r = op.sa(po.make_opts(po.RNG_REFERENCE, seed=s), p0["beta_start"], p0["beta_stop"],
+                  p0["sweeps"], p0["restarts"])
+        cpu.append(r["lowest"].min() + op.const_cost)
+    cpu = np.array(cpu)
+    p = ks_2samp(gpu, cpu)
+    assert p > 0.05, (p, gpu.mean(), cpu.mean())
+    assert abs(gpu.mean() - cpu.mean()) <= 0.005 * abs(cpu.mean())
