@@ -33,6 +33,12 @@ int qiotk_solve(qiotk_engine e, const char* target, const char* params_json,
 /* A model parsed, normalised and resident on the device, reused by many
  * solves (timed solves then exclude ingest and upload). */
 int qiotk_model_create(qiotk_engine e, const char* input_json, qiotk_model* out);
+/* The same from a term list already in memory (kind 0 ising, 1 pubo; term k
+ * has coefficient coeffs[k] and ids ids[offsets[k]..offsets[k+1])), as the
+ * reference's protobuf ingest provides an alternative to JSON
+ * (utils/proto_reader.h:196-228). */
+int qiotk_model_create_terms(qiotk_engine e, int kind, int64_t n_terms, const int64_t* offsets,
+                             const int32_t* ids, const double* coeffs, qiotk_model* out);
 void qiotk_model_destroy(qiotk_model m);
 int qiotk_solve_model(qiotk_model m, const char* target, const char* params_json, char** response);
 

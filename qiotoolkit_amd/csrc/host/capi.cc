@@ -117,6 +117,23 @@ int qiotk_model_create(qiotk_engine e, const char* input_json, qiotk_model* out)
   });
 }
 
+int qiotk_model_create_terms(qiotk_engine e, int kind, int64_t n_terms, const int64_t* offsets,
+                             const int32_t* ids, const double* coeffs, qiotk_model* out) {
+  return guarded(nullptr, [&] {
+    std::vector<Term> terms(n_terms);
+    for (int64_t k = 0; k < n_terms; ++k) {
+      terms[k].c = coeffs[k];
+      terms[k].ids.assign(ids + offsets[k], ids + offsets[k + 1]);
+    }
+    auto m = std::make_unique<qiotk_model_s>();
+    m->e = e;
+    m->model = make_model(kind == 0 ? QMC_ISING : QMC_PUBO, terms);
+    if (!m->model.empty()) m->dev = std::make_unique<DeviceModel>(e->ctx, m->model);
+    *out = m.release();
+    return 0;
+  });
+}
+
 void qiotk_model_destroy(qiotk_model m) { delete m; }
 
 int qiotk_solve_model(qiotk_model m, const char* target, const char* params_json, char** response) {

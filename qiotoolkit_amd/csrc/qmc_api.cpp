@@ -118,6 +118,10 @@ struct qmc_walkers_s {
   DevBuf<unsigned long long> accepted;
   DevBuf<int32_t> label;
   DevBuf<int64_t> progress;  // 2 per unit: next sweep, sweeps left
+  DevBuf<int64_t> last_step; // per PT ladder: last exchanged step
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  int graph_key[4] = {0, 0, 0, 0};
   DevBuf<UnitDev> d_units;
   std::vector<UnitDev> units;
   int ladder = 0;
@@ -452,8 +456,16 @@ int qmc_walkers_create(qmc_problem p, int32_t n_replicas, int32_t replicas_per_g
   return QMC_OK;
 }
 
+static void drop_graph(qmc_walkers w) {
+  if (w->graph_exec) (void)hipGraphExecDestroy(w->graph_exec);
+  if (w->graph) (void)hipGraphDestroy(w->graph);
+  w->graph_exec = nullptr;
+  w->graph = nullptr;
+}
+
 void qmc_walkers_destroy(qmc_walkers w) {
   if (w && w->prob) (void)hipStreamSynchronize(w->prob->ctx->stream);
+  if (w) drop_graph(w);
   delete w;
 }
 
@@ -648,6 +660,7 @@ int qmc_walkers_pt(qmc_walkers w, int32_t ladder, const double* ladder_betas, in
     HIP_TRY(hipEventRecord(c->ev1, st));
   } else {
     if (ladder % w->B != 0) return fail(QMC_INVALID_ARGUMENT, "ladder must be a multiple of replicas_per_group");
+    const bool fresh = (step0 == 0 || w->ladder != ladder);
     if ((rc = set_ladder(w, ladder, ladder_betas, step0))) return rc;
     w->ladder = 0;  // exchanges run as their own launches
     w->spr = 1;
@@ -655,16 +668,38 @@ int qmc_walkers_pt(qmc_walkers w, int32_t ladder, const double* ladder_betas, in
     if ((rc = upload_units(w))) return rc;
     w->ladder = ladder;
     if ((rc = set_progress(w, step0 * sweeps_per_replica, (int64_t)n_steps * sweeps_per_replica))) return rc;
-    HIP_TRY(hipEventRecord(c->ev0, st));
-    for (int32_t k = 0; k < n_steps; ++k) {
-      const int64_t step = step0 + k;
-      if ((rc = launch_units(p, w->d_units.p, w->n_units, p->lanes * w->G, w->stride, sweeps_per_replica,
-                             0, st)))
-        return rc;
-      HIP_TRY(launch_exchange(w->cost.p, w->beta.p, w->label.p, w->ladder_beta.p, n_ladders, ladder,
-                              (uint32_t)step, w->seed + (uint32_t)(w->first_replica / ladder),
-                              w->stream, 1, st));
+    if (w->last_step.n < (size_t)n_ladders) HIP_TRY(w->last_step.alloc(n_ladders));
+    {
+      std::vector<int64_t> ls(n_ladders, fresh ? -1 : step0 - 1);
+      HIP_TRY(hipMemcpy(w->last_step.p, ls.data(), n_ladders * sizeof(int64_t), hipMemcpyHostToDevice));
     }
+    const int upl = ladder / w->B;
+    const uint32_t seed0 = w->seed + (uint32_t)(w->first_replica / ladder);
+    // Steps are captured once as a graph of K (sweep, exchange) pairs and
+    // replayed; the kernels read their step from the device-side progress,
+    // and replays past the planned steps do nothing.
+    const int K = std::min<int32_t>(n_steps, 64);
+    const int key[4] = {K, sweeps_per_replica, w->n_units, ladder};
+    if (!w->graph_exec || std::memcmp(key, w->graph_key, sizeof key) != 0) {
+      drop_graph(w);
+      HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+      for (int k = 0; k < K; ++k) {
+        rc = launch_units(p, w->d_units.p, w->n_units, p->lanes * w->G, w->stride, sweeps_per_replica, 0, st);
+        if (rc) {
+          hipGraph_t g;
+          (void)hipStreamEndCapture(st, &g);
+          return rc;
+        }
+        HIP_TRY(launch_exchange(w->cost.p, w->beta.p, w->label.p, w->ladder_beta.p, n_ladders, ladder,
+                                w->progress.p, upl, sweeps_per_replica, w->last_step.p, seed0,
+                                w->stream, st));
+      }
+      HIP_TRY(hipStreamEndCapture(st, &w->graph));
+      HIP_TRY(hipGraphInstantiate(&w->graph_exec, w->graph, nullptr, nullptr, 0));
+      std::memcpy(w->graph_key, key, sizeof key);
+    }
+    HIP_TRY(hipEventRecord(c->ev0, st));
+    for (int32_t done = 0; done < n_steps; done += K) HIP_TRY(hipGraphLaunch(w->graph_exec, st));
     HIP_TRY(hipEventRecord(c->ev1, st));
   }
   HIP_TRY(hipStreamSynchronize(st));

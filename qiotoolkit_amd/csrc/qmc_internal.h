@@ -72,8 +72,9 @@ size_t sweep_smem_bytes(int max_stride, int ordered);
 hipError_t launch_sweep(int kind, int pairwise, int coupling, const UnitDev* units, int n_units,
                         int block, size_t smem, const SweepArgs& a, int ordered, hipStream_t st);
 hipError_t launch_exchange(double* cost, double* beta, int32_t* label, const double* lbeta,
-                           int n_ladders, int L, uint32_t step, uint32_t seed, uint32_t stream,
-                           int seed_per_ladder, hipStream_t st);
+                           int n_ladders, int L, const int64_t* progress, int units_per_ladder,
+                           int spr, int64_t* last_step, uint32_t seed, uint32_t stream,
+                           hipStream_t st);
 hipError_t launch_init(const UnitDev* units, int n_units, int N, uint32_t epoch, hipStream_t st);
 hipError_t launch_unpack(const UnitDev* units, int B, int R, int N, int which, uint8_t* out,
                          hipStream_t st);

@@ -516,29 +516,42 @@ k_sweep_ordered(const UnitDev* __restrict__ units, SweepArgs a) {
 // ---------------------------------------------------------------------------
 // PT exchange across units (ladders spanning several work-groups).
 // One thread per ladder pair.  costs/betas/labels are per global replica.
+// The step is derived from the ladder's sweep progress (next sweep / spr - 1)
+// so graph replays need no host-side argument updates; last_step[ladder]
+// keeps an exchange from running twice for one step.
 __global__ void k_exchange(double* __restrict__ cost, double* __restrict__ beta,
                            int32_t* __restrict__ label, const double* __restrict__ lbeta,
-                           int n_ladders, int L, uint32_t step, uint32_t seed0, uint32_t stream,
-                           int seed_per_ladder) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  const int pairs = L / 2;
-  if (t >= n_ladders * pairs) return;
-  const int ld = t / pairs, p = t - ld * pairs;
+                           int n_ladders, int L, const int64_t* __restrict__ progress,
+                           int units_per_ladder, int spr, int64_t* __restrict__ last_step,
+                           uint32_t seed0, uint32_t stream) {
+  const int ld = blockIdx.x;
+  if (ld >= n_ladders) return;
+  const int64_t next = progress[2 * (int64_t)ld * units_per_ladder];
+  const int64_t step64 = next / spr - 1;
+  __shared__ int go;
+  if (threadIdx.x == 0) go = (next % spr == 0) && step64 > last_step[ld];
+  __syncthreads();
+  if (!go) return;
+  const uint32_t step = (uint32_t)step64;
+  const int p = threadIdx.x;
   const int i = (int)(step & 1u) + 2 * p;
-  if (i >= L - 1) return;
-  int32_t* lab = label + ld * L;
-  const int sa = ld * L + lab[i], sb = ld * L + lab[i + 1];
-  const uint32_t seed = seed_per_ladder ? seed0 + (uint32_t)ld : seed0;
-  const uint4 w = philox4x32_10((uint32_t)i, step, 0u, DOM_EXCHANGE, seed, stream);
-  const double u = canonical2(w.x, w.y);
-  const double bi = lbeta[i], bj = lbeta[i + 1];
-  const double arg = __dmul_rn(__dsub_rn(cost[sa], cost[sb]), __dsub_rn(bi, bj));
-  if (u < exp(arg)) {
-    lab[i] = sb - ld * L;
-    lab[i + 1] = sa - ld * L;
-    beta[sa] = bj;
-    beta[sb] = bi;
+  if (i < L - 1) {
+    int32_t* lab = label + ld * L;
+    const int sa = ld * L + lab[i], sb = ld * L + lab[i + 1];
+    const uint32_t seed = seed0 + (uint32_t)ld;
+    const uint4 w = philox4x32_10((uint32_t)i, step, 0u, DOM_EXCHANGE, seed, stream);
+    const double u = canonical2(w.x, w.y);
+    const double bi = lbeta[i], bj = lbeta[i + 1];
+    const double arg = __dmul_rn(__dsub_rn(cost[sa], cost[sb]), __dsub_rn(bi, bj));
+    if (u < exp(arg)) {
+      lab[i] = sb - ld * L;
+      lab[i + 1] = sa - ld * L;
+      beta[sa] = bj;
+      beta[sb] = bi;
+    }
   }
+  __syncthreads();
+  if (threadIdx.x == 0) last_step[ld] = step64;
 }
 
 // ---------------------------------------------------------------------------
@@ -712,17 +725,26 @@ size_t sweep_smem_bytes(int max_stride, int ordered) {
   return sizeof(Smem) + 2 * (size_t)max_stride;
 }
 
+// The dynamic-LDS limit is raised once per kernel to the full 160 KiB, so
+// launches (also inside graph capture) need no attribute calls.
+template <class F>
+static hipError_t raise_lds_limit(F fn) {
+  static hipError_t once = hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                               160 * 1024);
+  return once;
+}
+
 template <int KIND, bool PAIR, typename CT>
 static hipError_t launch_sweep_t(const UnitDev* units, int n_units, int block, size_t smem,
                                  const SweepArgs& a, int ordered, hipStream_t st) {
   if (ordered) {
     auto fn = k_sweep_ordered<KIND, PAIR, CT>;
-    hipError_t e = hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
+    hipError_t e = raise_lds_limit(fn);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(fn, dim3(n_units), dim3(64), smem, st, units, a);
   } else {
     auto fn = k_sweep<KIND, PAIR, CT>;
-    hipError_t e = hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
+    hipError_t e = raise_lds_limit(fn);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(fn, dim3(n_units), dim3(block), smem, st, units, a);
   }
@@ -748,12 +770,12 @@ hipError_t launch_sweep(int kind, int pairwise, int coupling, const UnitDev* uni
 }
 
 hipError_t launch_exchange(double* cost, double* beta, int32_t* label, const double* lbeta,
-                           int n_ladders, int L, uint32_t step, uint32_t seed, uint32_t stream,
-                           int seed_per_ladder, hipStream_t st) {
-  const int n = n_ladders * (L / 2);
-  if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_exchange, dim3((n + 127) / 128), dim3(128), 0, st, cost, beta, label, lbeta,
-                     n_ladders, L, step, seed, stream, seed_per_ladder);
+                           int n_ladders, int L, const int64_t* progress, int units_per_ladder,
+                           int spr, int64_t* last_step, uint32_t seed, uint32_t stream,
+                           hipStream_t st) {
+  if (n_ladders <= 0 || L < 2) return hipSuccess;
+  hipLaunchKernelGGL(k_exchange, dim3(n_ladders), dim3(32), 0, st, cost, beta, label, lbeta,
+                     n_ladders, L, progress, units_per_ladder, spr, last_step, seed, stream);
   return hipGetLastError();
 }
 

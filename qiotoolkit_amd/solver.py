@@ -19,7 +19,8 @@ TARGETS = ("simulatedannealing.qiotoolkit", "paralleltempering.qiotoolkit", "pop
 
 EXPORTS = ["qiotk_engine_create", "qiotk_engine_destroy", "qiotk_solve", "qiotk_model_create",
            "qiotk_model_destroy", "qiotk_solve_model", "qiotk_solve_batch", "qiotk_free",
-           "qiotk_last_error", "qiotk_parse_model", "qiotk_schedule_eval", "qiotk_describe"]
+           "qiotk_last_error", "qiotk_parse_model", "qiotk_schedule_eval", "qiotk_describe",
+           "qiotk_model_create_terms"]
 
 
 class SolverError(RuntimeError):
@@ -57,6 +58,8 @@ def lib():
     L.qiotk_parse_model.argtypes = [C.c_char_p, P(vp)]
     L.qiotk_schedule_eval.argtypes = [C.c_char_p, C.c_int32, P(C.c_double), P(C.c_double), P(vp)]
     L.qiotk_describe.argtypes = [C.c_char_p, C.c_char_p, P(vp)]
+    L.qiotk_model_create_terms.argtypes = [vp, C.c_int, C.c_int64, P(C.c_int64), P(C.c_int32),
+                                           P(C.c_double), P(vp)]
     _lib = L
     return L
 
@@ -110,6 +113,10 @@ class Engine:
     def model(self, input_data):
         return Model(self, input_data)
 
+    def model_from_arrays(self, kind, offsets, ids, coeffs):
+        """Model from a term CSR (kind "ising"/"pubo"), without JSON."""
+        return Model(self, None, arrays=(kind, offsets, ids, coeffs))
+
     def solve_batch(self, target, models, params_list):
         n = len(models)
         hs = (C.c_void_p * n)(*[m.h for m in models])
@@ -128,10 +135,21 @@ class Engine:
 class Model:
     """A cost function parsed, normalised and resident on the device."""
 
-    def __init__(self, engine, input_data):
+    def __init__(self, engine, input_data, arrays=None):
         self.engine = engine
         self.h = C.c_void_p()
-        rc = lib().qiotk_model_create(engine.h, _enc(input_data), C.byref(self.h))
+        if arrays is not None:
+            import numpy as np
+            kind, off, ids, c = arrays
+            off = np.ascontiguousarray(off, dtype=np.int64)
+            ids = np.ascontiguousarray(ids, dtype=np.int32)
+            c = np.ascontiguousarray(c, dtype=np.float64)
+            rc = lib().qiotk_model_create_terms(
+                engine.h, 0 if kind == "ising" else 1, len(c), off.ctypes.data_as(C.POINTER(C.c_int64)),
+                ids.ctypes.data_as(C.POINTER(C.c_int32)), c.ctypes.data_as(C.POINTER(C.c_double)),
+                C.byref(self.h))
+        else:
+            rc = lib().qiotk_model_create(engine.h, _enc(input_data), C.byref(self.h))
         if rc:
             raise SolverError(rc, lib().qiotk_last_error().decode())
 

@@ -107,3 +107,33 @@ def log_uniform_sizes(n, lo, hi, seed=0):
 def geometric_temperatures(t0=0.2, t1=2.0, count=32):
     """Geometric schedule discretised to `count` values (schedule/geometric.cc:9-12)."""
     return [t0 * (t1 / t0) ** (i / (count - 1)) for i in range(count)]
+
+
+def er_ising_arrays(N, mean_degree=6.0, jmax=3, seed=0):
+    """Erdos-Renyi Ising as a term CSR (offsets, ids, coeffs) in O(edges):
+    N one-variable field terms h in {-1,0,1} (zero kept), then G(N, p) edges
+    with J uniform in [-jmax, jmax] \\ {0}."""
+    rng = np.random.default_rng(seed)
+    p = min(1.0, mean_degree / max(1, N - 1))
+    pairs = N * (N - 1) // 2
+    m = int(rng.binomial(pairs, p))
+    if m * 4 > pairs:
+        iu, ju = np.triu_indices(N, k=1)
+        pick = rng.choice(len(iu), size=m, replace=False)
+        a, b = iu[pick], ju[pick]
+    else:
+        got = np.zeros((0, 2), dtype=np.int64)
+        while len(got) < m:
+            x = rng.integers(0, N, size=(2 * (m - len(got)) + 16, 2))
+            x = x[x[:, 0] != x[:, 1]]
+            x.sort(axis=1)
+            got = np.unique(np.concatenate([got, x]), axis=0)
+        sel = rng.permutation(len(got))[:m]
+        a, b = got[sel, 0], got[sel, 1]
+    vals = np.array([v for v in range(-jmax, jmax + 1) if v != 0])
+    h = rng.integers(-1, 2, size=N)
+    J = rng.choice(vals, size=m)
+    offsets = np.concatenate([np.arange(N + 1), N + 2 * np.arange(1, m + 1)]).astype(np.int64)
+    ids = np.concatenate([np.arange(N), np.stack([a, b], axis=1).ravel()]).astype(np.int32)
+    coeffs = np.concatenate([h, J]).astype(np.float64)
+    return offsets, ids, coeffs
