@@ -53,6 +53,8 @@ def run_c2(args):
         p = {"temperatures": temps, "sweeps": sweeps, "sweeps_per_replica": 1, "seed": 1000 + k}
         if args.group:
             p["replicas_per_group"] = args.group
+        if args.lanes:
+            p["lanes"] = args.lanes
         params.append({"params": p})
     attempts_per_step = restarts * L * sweeps * N
 
@@ -69,6 +71,9 @@ def run_c2(args):
         dev_ms.append(ms)
         best = [r["solutions"]["cost"] for r in res]
     t = time.perf_counter() - t0
+    b0 = res[0]["benchmark"]
+    log("phases ms: init %.1f run %.1f finalize %.1f device %.1f" % (
+        b0["batch_init_ms"], b0["batch_run_ms"], b0["batch_finalize_ms"], b0["batch_device_ms"]))
     value = attempts_per_step * args.steps / t
     avg_ms = float(np.mean(dev_ms))
     B = 6 * (4 + 4 + 1) + 2
@@ -136,9 +141,23 @@ def main():
     ap.add_argument("--workload", default="c2")
     ap.add_argument("--sweeps", type=int, default=5000)
     ap.add_argument("--group", type=int, default=0, help="replicas per work-group (0 = auto)")
+    ap.add_argument("--lanes", type=int, default=0, help="positions per class round (0 = auto)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB (calibrates the HBM byte counters)")
     args = ap.parse_args()
+    if args.copy_probe:
+        import torch
+        n = args.copy_probe * (1 << 20) // 4
+        x = torch.ones(n, dtype=torch.float32, device="cuda")
+        torch.cuda.synchronize()
+        for _ in range(3):
+            y = x.clone()
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes_read": n * 4, "copy_probe_bytes_written": n * 4,
+                          "copies": 3, "check": float(y[-1].item())}))
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: every box here has a single MI355X; "
               "multi-GPU runs are not part of this build (--gpus %d refused)" % args.gpus,

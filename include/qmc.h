@@ -74,15 +74,16 @@ void qmc_ctx_destroy(qmc_ctx ctx);
 int qmc_sync(qmc_ctx ctx);
 
 /* Problem in internal ids 0..n_vars-1 (first-appearance order) as a term
- * CSR; ids sorted inside each term, no constant terms.  lanes = 0 picks the
- * class-round width automatically (64/128/256). */
+ * CSR; ids sorted inside each term, no constant terms.  lanes = default
+ * class-round width of walkers on it (0: from the largest colour class). */
 int qmc_problem_create(qmc_ctx ctx, int kind, int32_t n_vars, int64_t n_terms,
                        const int64_t* term_offsets, const int32_t* term_vars,
                        const double* coeffs, int32_t lanes, qmc_problem* out);
 void qmc_problem_destroy(qmc_problem p);
 int qmc_problem_get_info(qmc_problem p, qmc_problem_info* info);
-/* Sweep visit order (variable ids) and class offsets (n_classes + 1). */
-int qmc_problem_order(qmc_problem p, int32_t* order, int32_t* class_offsets);
+/* Sweep visit order (variable ids) for `lanes` positions per class round
+ * (0: the problem's default) and class offsets (n_classes + 1). */
+int qmc_problem_order(qmc_problem p, int32_t lanes, int32_t* order, int32_t* class_offsets);
 
 /* R walkers on one problem; replicas_per_group (1..32) replicas share one
  * work-group.  Walkers form independent runs of replicas_per_seed
@@ -143,6 +144,16 @@ int qmc_batch_clear(qmc_batch b);
 int qmc_batch_run(qmc_batch b, int32_t max_sweeps);
 /* Wait for the last qmc_batch_run; ms = its device time (HIP events). */
 int qmc_batch_wait(qmc_batch b, double* ms);
+
+/* Positions processed side by side in a class round (multiple of 64,
+ * lanes * ceil(replicas_per_group / 8) <= 512; 0 = the problem default).
+ * Changes the coloured visit order from the next sweep on. */
+int qmc_walkers_set_lanes(qmc_walkers w, int32_t lanes);
+int qmc_walkers_get_lanes(qmc_walkers w, int32_t* lanes);
+/* How ladders split over several work-groups exchange: 0 (default) inside
+ * one cooperative launch with a per-ladder barrier after every step; 1
+ * through graph-replayed (sweep, exchange) launches. */
+int qmc_walkers_set_pt_mode(qmc_walkers w, int mode);
 
 /* Read back per replica: current cost, lowest cost, current beta, accepted
  * counts, current and lowest states (n_replicas*n_vars bytes).  Any pointer

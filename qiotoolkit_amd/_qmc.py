@@ -25,7 +25,8 @@ EXPORTS = [
     "qmc_walkers_labels", "qmc_energy", "qmc_delta_all", "qmc_philox", "qmc_walkers_last_ms",
     "qmc_walkers_plan", "qmc_walkers_plan_pt", "qmc_walkers_remaining", "qmc_batch_create",
     "qmc_batch_destroy", "qmc_batch_add", "qmc_batch_remove", "qmc_batch_clear", "qmc_batch_run",
-    "qmc_batch_wait", "qmc_walkers_resample", "qmc_walkers_count",
+    "qmc_batch_wait", "qmc_walkers_resample", "qmc_walkers_count", "qmc_walkers_set_pt_mode",
+    "qmc_walkers_set_lanes", "qmc_walkers_get_lanes",
 ]
 
 
@@ -66,7 +67,7 @@ def lib():
     L.qmc_problem_destroy.argtypes = [vp]
     L.qmc_problem_destroy.restype = None
     L.qmc_problem_get_info.argtypes = [vp, P(ProblemInfo)]
-    L.qmc_problem_order.argtypes = [vp, P(C.c_int32), P(C.c_int32)]
+    L.qmc_problem_order.argtypes = [vp, C.c_int32, P(C.c_int32), P(C.c_int32)]
     L.qmc_walkers_create.argtypes = [vp, C.c_int32, C.c_int32, C.c_uint32, C.c_uint32, C.c_int32,
                                      C.c_int32, P(vp)]
     L.qmc_walkers_destroy.argtypes = [vp]
@@ -97,6 +98,9 @@ def lib():
     L.qmc_batch_wait.argtypes = [vp, P(C.c_double)]
     L.qmc_walkers_resample.argtypes = [vp, P(C.c_int32), C.c_int32]
     L.qmc_walkers_count.argtypes = [vp, P(C.c_int32)]
+    L.qmc_walkers_set_pt_mode.argtypes = [vp, C.c_int]
+    L.qmc_walkers_set_lanes.argtypes = [vp, C.c_int32]
+    L.qmc_walkers_get_lanes.argtypes = [vp, P(C.c_int32)]
     _lib = L
     return L
 
@@ -166,10 +170,10 @@ class Problem:
         except Exception:
             pass
 
-    def order(self):
+    def order(self, lanes=0):
         o = np.zeros(self.n, dtype=np.int32)
         co = np.zeros(self.info.n_classes + 1, dtype=np.int32)
-        _check(lib().qmc_problem_order(self.h, _p(o, C.c_int32), _p(co, C.c_int32)))
+        _check(lib().qmc_problem_order(self.h, lanes, _p(o, C.c_int32), _p(co, C.c_int32)))
         return o, co
 
     def energy(self, states):
@@ -226,6 +230,17 @@ class Walkers:
         else:
             _check(lib().qmc_walkers_sweep(self.h, sweep0, n_sweeps, QMC_BETA_PER_REPLICA, None,
                                            int(ordered)))
+
+    def set_lanes(self, lanes):
+        _check(lib().qmc_walkers_set_lanes(self.h, lanes))
+
+    def lanes(self):
+        v = C.c_int32()
+        _check(lib().qmc_walkers_get_lanes(self.h, C.byref(v)))
+        return v.value
+
+    def set_pt_mode(self, mode):
+        _check(lib().qmc_walkers_set_pt_mode(self.h, mode))
 
     def pt(self, ladder_betas, step0, n_steps, sweeps_per_replica=1):
         b = np.ascontiguousarray(ladder_betas, dtype=np.float64)

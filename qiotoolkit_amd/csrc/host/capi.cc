@@ -176,6 +176,7 @@ int qiotk_solve_batch(qiotk_engine e, const char* target, int32_t n, const qiotk
     device_check(qmc_batch_create(e->ctx, &b), "qmc_batch_create");
     std::unique_ptr<qmc_batch_s, void (*)(qmc_batch)> hold(b, qmc_batch_destroy);
     std::vector<int> fused(n, 0);
+    const double t0 = now_ms();
     for (int32_t k = 0; k < n; ++k) {
       auto s = create_solver(target);
       s->set_model(&models[k]->model, models[k]->dev.get(), e->ctx);
@@ -184,17 +185,26 @@ int qiotk_solve_batch(qiotk_engine e, const char* target, int32_t n, const qiotk
       fused[k] = s->batch_plan(b) ? 1 : 0;
       solvers.push_back(std::move(s));
     }
+    const double t1 = now_ms();
     double total_ms = 0;
     // every member finishes within its planned sweeps; one launch runs them all
     device_check(qmc_batch_run(b, 1 << 30), "qmc_batch_run");
     double ms = 0;
     device_check(qmc_batch_wait(b, &ms), "qmc_batch_wait");
     total_ms += ms;
+    const double t2 = now_ms();
     for (int32_t k = 0; k < n; ++k) {
       if (!fused[k]) solvers[k]->run();
       solvers[k]->finalize();
+    }
+    const double t3 = now_ms();
+    for (int32_t k = 0; k < n; ++k) {
       Json r = solvers[k]->get_result();
       r["benchmark"]["fused"] = Json((bool)fused[k]);
+      r["benchmark"]["batch_init_ms"] = t1 - t0;
+      r["benchmark"]["batch_run_ms"] = t2 - t1;
+      r["benchmark"]["batch_finalize_ms"] = t3 - t2;
+      r["benchmark"]["batch_device_ms"] = ms;
       responses[k] = dup(r.dump());
     }
     if (device_ms) *device_ms = total_ms;

@@ -266,6 +266,12 @@ Json& Json::operator[](const std::string& key) {
   obj_.emplace_back(key, Json());
   return obj_.back().second;
 }
+void Json::append(std::string key, Json v) {
+  if (type_ == NUL) type_ = OBJECT;
+  if (type_ != OBJECT) throw JsonError("expected object");
+  obj_.emplace_back(std::move(key), std::move(v));
+}
+
 void Json::remove(const std::string& key) {
   if (type_ != OBJECT) return;
   for (auto it = obj_.begin(); it != obj_.end(); ++it)

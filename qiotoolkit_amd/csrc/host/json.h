@@ -70,6 +70,8 @@ class Json {
   bool has(const std::string& key) const;
   const Json& at(const std::string& key) const;
   Json& operator[](const std::string& key);  // inserts null if absent
+  // append a member whose key the caller knows to be new (no lookup)
+  void append(std::string key, Json v);
   const Json& operator[](const std::string& key) const { return at(key); }
   void remove(const std::string& key);
   const std::vector<std::pair<std::string, Json>>& members() const { return obj_; }

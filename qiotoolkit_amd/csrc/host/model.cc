@@ -116,7 +116,7 @@ Json Model::render(const uint8_t* bits) const {
   Json s = Json::object();
   for (int32_t i = 0; i < n_vars; ++i) {
     const int v = kind == QMC_ISING ? (bits[i] ? -1 : 1) : (bits[i] ? 0 : 1);
-    s[std::to_string(names[i])] = Json(v);
+    s.append(std::to_string(names[i]), Json(v));  // names are unique
   }
   return s;
 }

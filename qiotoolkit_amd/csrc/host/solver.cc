@@ -44,6 +44,7 @@ void Solver::configure(const Json& doc) {
   const Json& p = params_of(doc);
   // Solver::configure (solver.h:156-218)
   if (p.has("threads")) threads_ = (int)p.at("threads").as_int();
+  if (p.has("lanes")) lanes_ = (int)p.at("lanes").as_int();
   if (p.has("time_limit")) time_limit_ = p.at("time_limit").as_double();
   else if (p.has("timeout")) time_limit_ = p.at("timeout").as_double();
   if (p.has("cost_limit")) cost_limit_ = p.at("cost_limit").as_double();
@@ -224,6 +225,7 @@ void SimulatedAnnealing::init() {
   const int B = group_ > 0 ? group_ : default_group(restarts_);
   device_check(qmc_walkers_create(dmodel_->handle(), restarts_, B, seed_, 0, 0, restarts_, &w_),
                "qmc_walkers_create");
+  if (lanes_) device_check(qmc_walkers_set_lanes(w_, lanes_), "qmc_walkers_set_lanes");
   if (model_->has_initial) {
     std::vector<uint8_t> st((size_t)restarts_ * model_->n_vars, 0);
     for (int r = 0; r < restarts_; ++r)
@@ -370,6 +372,7 @@ void ParallelTempering::init() {
   int B = group_ > 0 ? group_ : L;
   if (L % B != 0) B = L;
   device_check(qmc_walkers_create(dmodel_->handle(), L, B, seed_, 0, 0, L, &w_), "qmc_walkers_create");
+  if (lanes_) device_check(qmc_walkers_set_lanes(w_, lanes_), "qmc_walkers_set_lanes");
   if (model_->has_initial) {
     std::vector<uint8_t> st((size_t)L * model_->n_vars, 0);
     for (int r = 0; r < L; ++r)

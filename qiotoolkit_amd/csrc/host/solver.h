@@ -71,6 +71,7 @@ class Solver {
   std::optional<int64_t> eval_limit_;
   int number_of_solutions_ = 1;
   int threads_ = 0;
+  int lanes_ = 0;  // positions per class round on the device (0: automatic)
   int64_t steps_done_ = 0;
   int64_t attempts_ = 0, accepted_ = 0;
   Timing timing_;

@@ -5,8 +5,11 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -30,6 +33,48 @@ int fail(int code, const std::string& msg) {
       return fail(QMC_DEVICE_ERROR, std::string(#expr) + ": " + hipGetErrorString(e_)); \
   } while (0)
 
+// Device allocations are recycled through size buckets (powers of two from
+// 256 B): hipMalloc/hipFree synchronise and cost far more than a solve's
+// setup, and a serving loop must not call them per request.
+class DevicePool {
+ public:
+  static DevicePool& get() {
+    static DevicePool* p = new DevicePool();  // lives until process exit
+    return *p;
+  }
+  hipError_t alloc(void** out, size_t bytes) {
+    const size_t b = bucket(bytes);
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      int dev = 0;
+      (void)hipGetDevice(&dev);
+      auto& fl = free_[{dev, b}];
+      if (!fl.empty()) {
+        *out = fl.back();
+        fl.pop_back();
+        return hipSuccess;
+      }
+    }
+    return hipMalloc(out, b);
+  }
+  void release(void* p, size_t bytes) {
+    if (!p) return;
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    std::lock_guard<std::mutex> lk(mu_);
+    free_[{dev, bucket(bytes)}].push_back(p);
+  }
+  static size_t bucket(size_t bytes) {
+    size_t b = 256;
+    while (b < bytes) b <<= 1;
+    return b;
+  }
+
+ private:
+  std::mutex mu_;
+  std::map<std::pair<int, size_t>, std::vector<void*>> free_;
+};
+
 template <class T>
 struct DevBuf {
   T* p = nullptr;
@@ -39,7 +84,7 @@ struct DevBuf {
   DevBuf& operator=(const DevBuf&) = delete;
   ~DevBuf() { release(); }
   void release() {
-    if (p) (void)hipFree(p);
+    if (p) DevicePool::get().release(p, n * sizeof(T));
     p = nullptr;
     n = 0;
   }
@@ -47,7 +92,7 @@ struct DevBuf {
     release();
     n = count;
     if (count == 0) return hipSuccess;
-    return hipMalloc((void**)&p, count * sizeof(T));
+    return DevicePool::get().alloc((void**)&p, count * sizeof(T));
   }
   hipError_t upload(const T* src, size_t count) {
     hipError_t e = alloc(count);
@@ -65,8 +110,16 @@ struct qmc_ctx_s {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
 };
 
+struct EllLayout {
+  DevBuf<int32_t> slice_base, slice_off, slice_w, nbr, coef_i;
+  DevBuf<float> coef_f;
+};
+
 struct qmc_problem_s {
   qmc_ctx ctx = nullptr;
+  std::map<int, std::unique_ptr<EllLayout>> ell;  // per lane count (pairwise problems)
+  std::vector<int32_t> h_inc_off, h_nbr;          // host copies to build layouts
+  std::vector<double> h_coef;
   int kind = QMC_ISING;
   int32_t N = 0;
   int64_t M = 0;
@@ -110,6 +163,7 @@ struct qmc_walkers_s {
   qmc_problem prob = nullptr;
   int32_t R = 0, B = 1, G = 1, n_units = 0, stride = 0;
   int32_t cap = 0, cap_units = 0;  // allocated replicas / work-groups
+  int32_t lanes = 64;               // positions side by side in a class round
   int32_t per_seed = 0;
   uint32_t seed = 0, stream = 0;
   int32_t first_replica = 0;
@@ -119,6 +173,12 @@ struct qmc_walkers_s {
   DevBuf<int32_t> label;
   DevBuf<int64_t> progress;  // 2 per unit: next sweep, sweeps left
   DevBuf<int64_t> last_step; // per PT ladder: last exchanged step
+  DevBuf<int32_t> sync;      // per PT ladder: barrier counter (cooperative ladders)
+  DevBuf<double> cbuf;       // per PT ladder: [2][L] published energies
+  DevBuf<int32_t> err;       // barrier time-out flag
+  int coop = 0;              // ladders split over work-groups, exchanged in-kernel
+  int force_graph = 0;       // split ladders through graph-replayed launches instead
+  int nb = 8;                // compile-time replicas per byte lane (1, 2, 4, 8)
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
   int graph_key[4] = {0, 0, 0, 0};
@@ -138,7 +198,7 @@ struct qmc_batch_s {
   bool dirty = true;
   // one launch group per kernel instantiation
   struct Group {
-    int kind, pairwise, coupling;
+    int kind, pairwise, coupling, nb, coop;
     int block = 64;
     int stride = 0;
     std::vector<UnitDev> units;
@@ -200,8 +260,8 @@ int qmc_problem_create(qmc_ctx ctx, int kind, int32_t n_vars, int64_t n_terms,
   if (!ctx || !out) return fail(QMC_INVALID_ARGUMENT, "null handle");
   if (kind != QMC_ISING && kind != QMC_PUBO) return fail(QMC_INVALID_ARGUMENT, "unknown model kind");
   if (n_vars <= 0 || n_terms <= 0) return fail(QMC_INVALID_ARGUMENT, "empty problem");
-  if (lanes != 0 && (lanes % 64 != 0 || lanes < 64 || lanes > 256))
-    return fail(QMC_INVALID_ARGUMENT, "lanes must be 64, 128, 192 or 256");
+  if (lanes != 0 && (lanes % 64 != 0 || lanes < 64 || lanes > 512))
+    return fail(QMC_INVALID_ARGUMENT, "lanes must be a multiple of 64 in [64, 512]");
   const int32_t N = n_vars;
   std::vector<std::vector<int32_t>> node_terms(N);
   bool all_int = true;
@@ -259,15 +319,24 @@ int qmc_problem_create(qmc_ctx ctx, int kind, int32_t n_vars, int64_t n_terms,
   p->var_of_pos.assign(N, 0);
   p->pos_of_var.assign(N, 0);
   {
+    // inside a class: descending degree, then variable id -- members of a
+    // class are independent, so any order is a valid sweep; degree order
+    // keeps the padding of the sliced layout small
     std::vector<int32_t> fillp(p->class_off.begin(), p->class_off.end() - 1);
-    for (int32_t v = 0; v < N; ++v) {
-      const int32_t q = fillp[color[v]]++;
-      p->var_of_pos[q] = v;
-      p->pos_of_var[v] = q;
-    }
+    for (int32_t v = 0; v < N; ++v) p->var_of_pos[fillp[color[v]]++] = v;
+    for (int c = 0; c < n_colors; ++c)
+      std::stable_sort(p->var_of_pos.begin() + p->class_off[c], p->var_of_pos.begin() + p->class_off[c + 1],
+                       [&](int32_t a, int32_t b) { return node_terms[a].size() > node_terms[b].size(); });
+    for (int32_t q = 0; q < N; ++q) p->pos_of_var[p->var_of_pos[q]] = q;
   }
   const int32_t nmax = *std::max_element(csize.begin(), csize.end());
-  p->lanes = lanes ? lanes : (nmax > 512 ? 128 : 64);
+  if (lanes) {
+    p->lanes = lanes;
+  } else {
+    int l = 64;
+    while (l < 512 && l * 4 < nmax) l *= 2;
+    p->lanes = l;
+  }
 
   // position-space incidence CSR
   std::vector<int32_t> inc_off(N + 1, 0), nbr, oth_off(1, 0), oth_pos, coef_i;
@@ -293,6 +362,12 @@ int qmc_problem_create(qmc_ctx ctx, int kind, int32_t n_vars, int64_t n_terms,
     inc_off[q + 1] = (int32_t)nbr.size();
   }
   p->n_inc = (int64_t)nbr.size();
+  p->h_inc_off = inc_off;
+  p->h_nbr = nbr;
+  p->h_coef.resize(nbr.size());
+  for (size_t e = 0; e < nbr.size(); ++e) p->h_coef[e] = coef_f[e];
+  if (p->coupling == QMC_COUPLING_I32)
+    for (size_t e = 0; e < nbr.size(); ++e) p->h_coef[e] = coef_i[e];
   std::vector<int64_t> toff(term_offsets, term_offsets + n_terms + 1);
   std::vector<int32_t> tvar(term_vars, term_vars + term_offsets[n_terms]);
   std::vector<double> tc(coeffs, coeffs + n_terms);
@@ -330,26 +405,96 @@ int qmc_problem_get_info(qmc_problem p, qmc_problem_info* info) {
   return QMC_OK;
 }
 
-int qmc_problem_order(qmc_problem p, int32_t* order, int32_t* class_offsets) {
+int qmc_problem_order(qmc_problem p, int32_t lanes, int32_t* order, int32_t* class_offsets) {
   if (!p) return fail(QMC_INVALID_ARGUMENT, "null handle");
+  if (lanes == 0) lanes = p->lanes;
+  if (lanes < 64 || lanes > 512 || lanes % 64) return fail(QMC_INVALID_ARGUMENT, "bad lane count");
   const int nclass = (int)p->class_off.size() - 1;
   int32_t k = 0;
   for (int c = 0; c < nclass; ++c) {
     const int32_t c0 = p->class_off[c], nc = p->class_off[c + 1] - c0;
     if (class_offsets) class_offsets[c] = c0;
-    for (int32_t tp = 0; tp < p->lanes; ++tp)
-      for (int32_t rho = tp; rho < nc; rho += p->lanes)
+    for (int32_t tp = 0; tp < lanes; ++tp)
+      for (int32_t rho = tp; rho < nc; rho += lanes)
         if (order) order[k++] = p->var_of_pos[c0 + rho];
   }
   if (class_offsets) class_offsets[nclass] = p->N;
   return QMC_OK;
 }
 
+// Sliced ELL of the pairwise form for `lanes` lanes: slice (c, k) covers
+// positions c0 + k*lanes .. + lanes of class c; width = its max degree.
+static EllLayout* get_layout(qmc_problem p, int lanes) {
+  auto it = p->ell.find(lanes);
+  if (it != p->ell.end()) return it->second.get();
+  auto L = std::make_unique<EllLayout>();
+  const int nclass = (int)p->class_off.size() - 1;
+  std::vector<int32_t> sbase(nclass + 1, 0), soff, sw, nbr;
+  std::vector<double> coef;
+  for (int c = 0; c < nclass; ++c) {
+    const int32_t c0 = p->class_off[c], nc = p->class_off[c + 1] - c0;
+    sbase[c] = (int32_t)sw.size();
+    for (int32_t k = 0; k * lanes < nc; ++k) {
+      int32_t w = 0;
+      for (int32_t t = 0; t < lanes && k * lanes + t < nc; ++t) {
+        const int32_t q = c0 + k * lanes + t;
+        w = std::max(w, p->h_inc_off[q + 1] - p->h_inc_off[q]);
+      }
+      soff.push_back((int32_t)nbr.size());
+      sw.push_back(w);
+      for (int32_t e = 0; e < w; ++e)
+        for (int32_t t = 0; t < lanes; ++t) {
+          const int32_t rho = k * lanes + t;
+          int32_t nb = p->N;
+          double cf = 0.0;
+          if (rho < nc) {
+            const int32_t q = c0 + rho;
+            const int32_t e0 = p->h_inc_off[q], deg = p->h_inc_off[q + 1] - e0;
+            if (e < deg) {
+              nb = p->h_nbr[e0 + e];
+              cf = p->h_coef[e0 + e];
+            }
+          }
+          nbr.push_back(nb);
+          coef.push_back(cf);
+        }
+    }
+  }
+  sbase[nclass] = (int32_t)sw.size();
+  soff.push_back((int32_t)nbr.size());
+  sw.push_back(0);
+  nbr.push_back(p->N);
+  coef.push_back(0.0);
+  bool ok = L->slice_base.upload(sbase.data(), sbase.size()) == hipSuccess &&
+            L->slice_off.upload(soff.data(), soff.size()) == hipSuccess &&
+            L->slice_w.upload(sw.data(), sw.size()) == hipSuccess &&
+            L->nbr.upload(nbr.data(), nbr.size()) == hipSuccess;
+  if (p->coupling == QMC_COUPLING_I32) {
+    std::vector<int32_t> ci(coef.begin(), coef.end());
+    ok = ok && L->coef_i.upload(ci.data(), ci.size()) == hipSuccess;
+  } else {
+    std::vector<float> cf(coef.begin(), coef.end());
+    ok = ok && L->coef_f.upload(cf.data(), cf.size()) == hipSuccess;
+  }
+  if (!ok) return nullptr;
+  EllLayout* raw = L.get();
+  p->ell[lanes] = std::move(L);
+  return raw;
+}
+
 static void build_units(qmc_walkers w) {
   qmc_problem p = w->prob;
   w->units.assign(w->n_units, UnitDev{});
+  EllLayout* ell = p->pairwise ? get_layout(p, w->lanes) : nullptr;
   for (int u = 0; u < w->n_units; ++u) {
     UnitDev& U = w->units[u];
+    if (ell) {
+      U.ell_slice_base = ell->slice_base.p;
+      U.ell_slice_off = ell->slice_off.p;
+      U.ell_slice_w = ell->slice_w.p;
+      U.ell_nbr = ell->nbr.p;
+      U.ell_coef = p->coupling == QMC_COUPLING_I32 ? (const void*)ell->coef_i.p : (const void*)ell->coef_f.p;
+    }
     U.class_off = p->d_class_off.p;
     U.var_of_pos = p->d_var_of_pos.p;
     U.pos_of_var = p->d_pos_of_var.p;
@@ -360,7 +505,7 @@ static void build_units(qmc_walkers w) {
     U.oth_pos = p->d_oth_pos.p;
     U.N = p->N;
     U.n_classes = (int32_t)p->class_off.size() - 1;
-    U.lanes = p->lanes;
+    U.lanes = w->lanes;
     U.G = w->G;
     U.stride = w->stride;
     U.ladder = 0;
@@ -374,6 +519,22 @@ static void build_units(qmc_walkers w) {
     U.label = w->label.p + r;
     U.ladder_beta = w->ladder_beta.p;
     U.ladder = w->ladder;
+    U.slot0 = 0;
+    U.upl = 1;
+    U.sync = nullptr;
+    U.cbuf = nullptr;
+    U.err = w->err.p;
+    if (w->ladder > 0) {
+      const int L = w->ladder;
+      const int ld = r / L;
+      U.slot0 = r % L;
+      U.upl = L / w->B;
+      U.label = w->label.p + (size_t)ld * L;
+      if (U.upl > 1) {
+        U.sync = w->sync.p + ld;
+        U.cbuf = w->cbuf.p + (size_t)ld * 2 * L;
+      }
+    }
     U.spr = w->spr;
     U.sched = w->sched.p;
     U.sched_base = w->sched_base;
@@ -419,8 +580,9 @@ int qmc_walkers_create(qmc_problem p, int32_t n_replicas, int32_t replicas_per_g
   w->R = n_replicas;
   w->B = replicas_per_group;
   w->G = (w->B + 7) / 8;
-  if (p->lanes * w->G > 512)
-    return fail(QMC_INVALID_ARGUMENT, "lanes * replica bytes exceeds the 512-thread work-group");
+  w->lanes = p->lanes;
+  while (w->lanes * w->G > 512) w->lanes /= 2;
+  if (w->lanes < 64) return fail(QMC_INVALID_ARGUMENT, "replica bytes exceed the work-group");
   w->n_units = (w->R + w->B - 1) / w->B;
   w->cap = w->R;
   w->cap_units = w->n_units;
@@ -429,6 +591,7 @@ int qmc_walkers_create(qmc_problem p, int32_t n_replicas, int32_t replicas_per_g
   w->stream = stream;
   w->first_replica = first_replica;
   w->per_seed = replicas_per_seed;
+  w->nb = w->B > 4 ? 8 : (w->B > 2 ? 4 : (w->B > 1 ? 2 : 1));
   const size_t smem = sweep_smem_bytes(w->stride, 0);
   if (smem > 160 * 1024) return fail(QMC_MEMORY_LIMITED, "problem too large for LDS residency at this group size");
   HIP_TRY(hipSetDevice(p->ctx->device));
@@ -442,6 +605,8 @@ int qmc_walkers_create(qmc_problem p, int32_t n_replicas, int32_t replicas_per_g
   HIP_TRY(w->accepted.alloc(w->R));
   HIP_TRY(w->label.alloc(w->R));
   HIP_TRY(w->ladder_beta.alloc(32));
+  HIP_TRY(w->err.alloc(1));
+  HIP_TRY(hipMemset(w->err.p, 0, sizeof(int32_t)));
   HIP_TRY(hipMemset(w->cost.p, 0, w->R * sizeof(double)));
   HIP_TRY(hipMemset(w->best_cost.p, 0, w->R * sizeof(double)));
   HIP_TRY(hipMemset(w->beta.p, 0, w->R * sizeof(double)));
@@ -558,13 +723,15 @@ int qmc_walkers_set_betas(qmc_walkers w, const double* betas) {
   return QMC_OK;
 }
 
-static int launch_units(qmc_problem p, const UnitDev* d_units, int n_units, int block, int stride,
-                        int32_t max_sweeps, int ordered, hipStream_t st) {
+static int launch_units(qmc_problem p, int nb, const UnitDev* d_units, int n_units, int block, int stride,
+                        int32_t max_sweeps, int ordered, hipStream_t st, int coop = 0) {
   SweepArgs a{};
   a.n_sweeps = max_sweeps;
   a.lut = p->ctx->lut.p;
+  a.unit_of_block = nullptr;
   const size_t smem = sweep_smem_bytes(stride, ordered);
-  HIP_TRY(launch_sweep(p->kind, p->pairwise, p->coupling, d_units, n_units, block, smem, a, ordered, st));
+  HIP_TRY(launch_sweep(p->kind, p->pairwise, p->coupling, nb, d_units, n_units, block, smem, a, ordered,
+                       coop, st));
   return QMC_OK;
 }
 
@@ -580,6 +747,7 @@ int qmc_walkers_plan(qmc_walkers w, int64_t sweep0, int32_t n_sweeps, const doub
   if (n_sweeps < 0) return fail(QMC_INVALID_ARGUMENT, "n_sweeps must be >= 0");
   int rc;
   w->ladder = 0;
+  w->coop = 0;
   w->spr = 1;
   if (schedule) {
     if ((rc = upload_schedule(w, sweep0, n_sweeps, schedule))) return rc;
@@ -613,15 +781,35 @@ static int set_ladder(qmc_walkers w, int32_t ladder, const double* ladder_betas,
 int qmc_walkers_plan_pt(qmc_walkers w, int32_t ladder, const double* ladder_betas, int64_t step0,
                         int32_t n_steps, int32_t sweeps_per_replica) {
   if (!w || !ladder_betas) return fail(QMC_INVALID_ARGUMENT, "null argument");
-  if (w->B != ladder) return fail(QMC_INVALID_ARGUMENT, "batched PT needs replicas_per_group == ladder");
+  if (ladder < 1 || ladder % w->B != 0)
+    return fail(QMC_INVALID_ARGUMENT, "the ladder must be a multiple of replicas_per_group");
   if (sweeps_per_replica < 1) return fail(QMC_INVALID_ARGUMENT, "sweeps_per_replica must be >= 1");
   int rc = set_ladder(w, ladder, ladder_betas, step0);
   if (rc) return rc;
+  const int n_ladders = w->R / ladder;
+  const int upl = ladder / w->B;
+  w->coop = upl > 1;
+  if (w->coop) {
+    // ladders split over work-groups exchange inside a cooperative launch;
+    // the barrier counter holds steps * work-groups-per-ladder
+    if (w->sync.n < (size_t)n_ladders) HIP_TRY(w->sync.alloc(n_ladders));
+    if (w->cbuf.n < (size_t)n_ladders * 2 * ladder) HIP_TRY(w->cbuf.alloc((size_t)n_ladders * 2 * ladder));
+    std::vector<int32_t> ctr(n_ladders, (int32_t)(step0 * upl));
+    HIP_TRY(hipMemcpy(w->sync.p, ctr.data(), n_ladders * sizeof(int32_t), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemset(w->err.p, 0, sizeof(int32_t)));
+  }
   w->ladder = ladder;
   w->spr = sweeps_per_replica;
   w->sched.release();
   if ((rc = upload_units(w))) return rc;
   return set_progress(w, step0 * sweeps_per_replica, (int64_t)n_steps * sweeps_per_replica);
+}
+
+static int check_barrier_error(qmc_walkers w) {
+  int32_t e = 0;
+  HIP_TRY(hipMemcpy(&e, w->err.p, sizeof e, hipMemcpyDeviceToHost));
+  if (e) return fail(QMC_RUNTIME_ERROR, "ladder barrier timed out (work-groups not co-resident)");
+  return QMC_OK;
 }
 
 int qmc_walkers_sweep(qmc_walkers w, int64_t sweep0, int32_t n_sweeps, int beta_mode,
@@ -633,7 +821,7 @@ int qmc_walkers_sweep(qmc_walkers w, int64_t sweep0, int32_t n_sweeps, int beta_
   if (rc) return rc;
   qmc_ctx c = w->prob->ctx;
   HIP_TRY(hipEventRecord(c->ev0, c->stream));
-  rc = launch_units(w->prob, w->d_units.p, w->n_units, ordered ? 64 : w->prob->lanes * w->G, w->stride,
+  rc = launch_units(w->prob, w->nb, w->d_units.p, w->n_units, ordered ? 64 : w->lanes * w->G, w->stride,
                     n_sweeps, ordered, c->stream);
   if (rc) return rc;
   HIP_TRY(hipEventRecord(c->ev1, c->stream));
@@ -651,17 +839,23 @@ int qmc_walkers_pt(qmc_walkers w, int32_t ladder, const double* ladder_betas, in
   hipStream_t st = c->stream;
   const int n_ladders = w->R / std::max(1, ladder);
   int rc;
-  if (w->B == ladder) {
+  const size_t smem = sweep_smem_bytes(w->stride, 0);
+  const bool can_coop = w->B != ladder && ladder % w->B == 0 &&
+      w->n_units <= max_coresident_blocks(p->kind, p->pairwise, p->coupling, w->nb, w->lanes * w->G, smem);
+  if (w->B == ladder || (can_coop && !w->force_graph)) {
     if ((rc = qmc_walkers_plan_pt(w, ladder, ladder_betas, step0, n_steps, sweeps_per_replica))) return rc;
     HIP_TRY(hipEventRecord(c->ev0, st));
-    if ((rc = launch_units(p, w->d_units.p, w->n_units, p->lanes * w->G, w->stride,
-                           n_steps * sweeps_per_replica, 0, st)))
+    if ((rc = launch_units(p, w->nb, w->d_units.p, w->n_units, w->lanes * w->G, w->stride,
+                           n_steps * sweeps_per_replica, 0, st, w->coop)))
       return rc;
     HIP_TRY(hipEventRecord(c->ev1, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    if ((rc = check_barrier_error(w))) return rc;
   } else {
     if (ladder % w->B != 0) return fail(QMC_INVALID_ARGUMENT, "ladder must be a multiple of replicas_per_group");
     const bool fresh = (step0 == 0 || w->ladder != ladder);
     if ((rc = set_ladder(w, ladder, ladder_betas, step0))) return rc;
+    w->coop = 0;
     w->ladder = 0;  // exchanges run as their own launches
     w->spr = 1;
     w->sched.release();
@@ -684,7 +878,7 @@ int qmc_walkers_pt(qmc_walkers w, int32_t ladder, const double* ladder_betas, in
       drop_graph(w);
       HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
       for (int k = 0; k < K; ++k) {
-        rc = launch_units(p, w->d_units.p, w->n_units, p->lanes * w->G, w->stride, sweeps_per_replica, 0, st);
+        rc = launch_units(p, w->nb, w->d_units.p, w->n_units, w->lanes * w->G, w->stride, sweeps_per_replica, 0, st);
         if (rc) {
           hipGraph_t g;
           (void)hipStreamEndCapture(st, &g);
@@ -778,15 +972,19 @@ static int batch_rebuild(qmc_batch b) {
     qmc_problem p = w->prob;
     qmc_batch_s::Group* g = nullptr;
     for (auto& gg : b->groups)
-      if (gg->kind == p->kind && gg->pairwise == p->pairwise && gg->coupling == p->coupling) g = gg.get();
+      if (gg->kind == p->kind && gg->pairwise == p->pairwise && gg->coupling == p->coupling &&
+          gg->nb == w->nb && gg->coop == w->coop)
+        g = gg.get();
     if (!g) {
       b->groups.push_back(std::make_unique<qmc_batch_s::Group>());
       g = b->groups.back().get();
       g->kind = p->kind;
       g->pairwise = p->pairwise;
       g->coupling = p->coupling;
+      g->nb = w->nb;
+      g->coop = w->coop;
     }
-    g->block = std::max(g->block, p->lanes * w->G);
+    g->block = std::max(g->block, w->lanes * w->G);
     g->stride = std::max(g->stride, w->stride);
     g->units.insert(g->units.end(), w->units.begin(), w->units.end());
   }
@@ -809,10 +1007,31 @@ int qmc_batch_run(qmc_batch b, int32_t max_sweeps) {
     SweepArgs a{};
     a.n_sweeps = max_sweeps;
     a.lut = c->lut.p;
+    a.unit_of_block = nullptr;
     const size_t smem = sweep_smem_bytes(g->stride, 0);
     if (smem > 160 * 1024) return fail(QMC_MEMORY_LIMITED, "batch LDS footprint exceeds 160 KiB");
-    HIP_TRY(launch_sweep(g->kind, g->pairwise, g->coupling, g->d_units.p, (int)g->units.size(), g->block,
-                         smem, a, 0, c->stream));
+    const int n = (int)g->units.size();
+    if (!g->coop) {
+      HIP_TRY(launch_sweep(g->kind, g->pairwise, g->coupling, g->nb, g->d_units.p, n, g->block, smem, a, 0, 0,
+                           c->stream));
+      continue;
+    }
+    // Cooperative ladders: launches of at most the co-resident number of
+    // work-groups, never splitting a ladder (its work-groups are adjacent).
+    const int cap = max_coresident_blocks(g->kind, g->pairwise, g->coupling, g->nb, g->block, smem);
+    int start = 0;
+    while (start < n) {
+      int end = start;
+      while (end < n) {
+        const int upl = g->units[end].upl;
+        if (end + upl - start > cap) break;
+        end += upl;
+      }
+      if (end == start) return fail(QMC_MEMORY_LIMITED, "a ladder exceeds the co-resident work-group limit");
+      HIP_TRY(launch_sweep(g->kind, g->pairwise, g->coupling, g->nb, g->d_units.p + start, end - start,
+                           g->block, smem, a, 0, 1, c->stream));
+      start = end;
+    }
   }
   HIP_TRY(hipEventRecord(c->ev1, c->stream));
   return QMC_OK;
@@ -823,6 +1042,33 @@ int qmc_batch_wait(qmc_batch b, double* ms) {
   HIP_TRY(hipStreamSynchronize(b->ctx->stream));
   HIP_TRY(hipEventElapsedTime(&b->last_ms, b->ctx->ev0, b->ctx->ev1));
   if (ms) *ms = b->last_ms;
+  for (qmc_walkers w : b->members)
+    if (w->coop) {
+      int rc = check_barrier_error(w);
+      if (rc) return rc;
+    }
+  return QMC_OK;
+}
+
+int qmc_walkers_set_lanes(qmc_walkers w, int32_t lanes) {
+  if (!w) return fail(QMC_INVALID_ARGUMENT, "null handle");
+  if (lanes == 0) lanes = w->prob->lanes;
+  if (lanes < 64 || lanes > 512 || lanes % 64 || lanes * w->G > 512)
+    return fail(QMC_INVALID_ARGUMENT, "lanes must be a multiple of 64 with lanes * ceil(B/8) <= 512");
+  w->lanes = lanes;
+  drop_graph(w);
+  return upload_units(w);
+}
+
+int qmc_walkers_get_lanes(qmc_walkers w, int32_t* lanes) {
+  if (!w || !lanes) return fail(QMC_INVALID_ARGUMENT, "null argument");
+  *lanes = w->lanes;
+  return QMC_OK;
+}
+
+int qmc_walkers_set_pt_mode(qmc_walkers w, int mode) {
+  if (!w || mode < 0 || mode > 1) return fail(QMC_INVALID_ARGUMENT, "bad argument");
+  w->force_graph = mode;
   return QMC_OK;
 }
 

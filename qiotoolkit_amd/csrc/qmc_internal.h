@@ -31,7 +31,14 @@ struct ProblemDev {
 
 // One work-group's share: up to 32 replicas of one problem.
 struct UnitDev {
-  // problem
+  // problem: sliced ELL of the pairwise form for this unit's lane count --
+  // slice (class c, round k) = ell_slice_base[c] + k holds width w entries per
+  // lane at ell_*[off + e*lanes + lane], padded with the ghost position
+  const int32_t* ell_slice_base;
+  const int32_t* ell_slice_off;
+  const int32_t* ell_slice_w;
+  const int32_t* ell_nbr;
+  const void* ell_coef;
   const int32_t* class_off;
   const int32_t* var_of_pos;
   const int32_t* pos_of_var;
@@ -42,7 +49,12 @@ struct UnitDev {
   const int32_t* oth_pos;
   int32_t N, n_classes, lanes, G;
   int32_t stride;   // bytes of one unit's spin array, multiple of 4
-  int32_t ladder;   // > 0: in-unit PT ladder of this many replicas
+  int32_t ladder;   // > 0: PT ladder of this many replicas (exchanges in the kernel)
+  int32_t slot0;    // ladder slot of the unit's first replica
+  int32_t upl;      // work-groups per ladder (> 1: cooperative, per-step barrier)
+  int32_t* sync;    // per-ladder barrier counter (upl > 1)
+  double* cbuf;     // per-ladder [2][ladder] published energies (upl > 1)
+  int32_t* err;     // set when a barrier wait times out
   // state (unit-local pointers: element 0 is the unit's first replica)
   uint8_t* spins;
   uint8_t* best;
@@ -50,7 +62,7 @@ struct UnitDev {
   double* best_cost;
   double* beta;
   unsigned long long* accepted;
-  int32_t* label;
+  int32_t* label;   // the ladder's labels: slot holding beta index k
   const double* ladder_beta;
   const double* sched;   // beta of absolute sweep s = sched[s - sched_base]; null: per-replica betas
   int64_t sched_base;
@@ -66,11 +78,15 @@ struct SweepArgs {
   int32_t n_sweeps;      // at most this many sweeps per unit in this launch
   int32_t pad;
   const double* lut;     // device, 1024 values
+  const int32_t* unit_of_block;  // optional block -> unit map (null: identity)
 };
 
 size_t sweep_smem_bytes(int max_stride, int ordered);
-hipError_t launch_sweep(int kind, int pairwise, int coupling, const UnitDev* units, int n_units,
-                        int block, size_t smem, const SweepArgs& a, int ordered, hipStream_t st);
+hipError_t launch_sweep(int kind, int pairwise, int coupling, int nb, const UnitDev* units, int n_units,
+                        int block, size_t smem, const SweepArgs& a, int ordered, int cooperative,
+                        hipStream_t st);
+// largest grid of cooperative sweep work-groups that can be co-resident
+int max_coresident_blocks(int kind, int pairwise, int coupling, int nb, int block, size_t smem);
 hipError_t launch_exchange(double* cost, double* beta, int32_t* label, const double* lbeta,
                            int n_ladders, int L, const int64_t* progress, int units_per_ladder,
                            int spr, int64_t* last_step, uint32_t seed, uint32_t stream,

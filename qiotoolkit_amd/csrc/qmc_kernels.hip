@@ -28,7 +28,7 @@
 namespace qmc {
 
 constexpr int kMaxRep = 32;          // replicas per unit
-constexpr int kMaxWavesPerGroup = 4; // lanes <= 256
+constexpr int kMaxWavesPerGroup = 8; // lanes <= 512
 constexpr int kBlock = 512;
 
 template <typename CT>
@@ -41,6 +41,10 @@ struct Smem {
   double rbeta[kMaxRep];
   double lbeta[kMaxRep];
   int32_t label[kMaxRep];
+  int32_t inv[kMaxRep];     // ladder slot -> beta index
+  double lcost[kMaxRep];    // ladder energies at the exchange
+  int32_t abort_flag;
+  int32_t pad2_;
   uint32_t racc[kMaxRep];
   int32_t nb_tp[kMaxRep];   // thread lane holding the new-best argmin, -1 none
   int32_t kstar[kMaxRep];
@@ -69,28 +73,59 @@ template <> struct RedSlots<int32_t> {
 //   PUBO : h_b = sum_e c_e * [all other members are 1 in replica b]
 // Sums run over the variable's incident terms in term order, as the
 // reference's calculate_cost_difference does.
-template <int KIND, bool PAIR, typename CT>
-__device__ __forceinline__ void local_field(const UnitDev& U, const uint8_t* sp, int q, int G,
-                                            int g, CT (&h)[8]) {
+template <int KIND, bool PAIR, typename CT, int NB>
+__device__ __forceinline__ void local_field(const UnitDev& U, const uint8_t* sp, int q, int slice, int tp,
+                                            int G, int g, CT (&h)[8]) {
 #pragma unroll
-  for (int b = 0; b < 8; ++b) h[b] = CT(0);
+  for (int b = 0; b < NB; ++b) h[b] = CT(0);
+  if (PAIR) {
+    // sliced ELL: the slice width is wave-uniform; entries of one lane are
+    // `lanes` apart, so each batch of loads is coalesced and independent
+    const int base = U.ell_slice_off[slice];
+    const int w = U.ell_slice_w[slice];
+    const int TP = U.lanes;
+    const CT* coef = static_cast<const CT*>(U.ell_coef);
+    for (int e0 = 0; e0 < w; e0 += 8) {
+      int nb[8];
+      CT cc[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        if (e0 + j < w) {
+          const int idx = base + (e0 + j) * TP + tp;
+          nb[j] = U.ell_nbr[idx];
+          cc[j] = coef[idx];
+        }
+      }
+      uint32_t wv[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j)
+        if (e0 + j < w) wv[j] = sp[nb[j] * G + g];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        if (e0 + j < w) {
+#pragma unroll
+          for (int b = 0; b < NB; ++b) {
+            const bool bit = (wv[j] >> b) & 1u;
+            if (KIND == QMC_ISING) h[b] += bit ? -cc[j] : cc[j];
+            else h[b] += bit ? CT(0) : cc[j];
+          }
+        }
+      }
+    }
+    return;
+  }
   const CT* coef = static_cast<const CT*>(U.coef);
   const int e0 = U.inc_off[q], e1 = U.inc_off[q + 1];
   for (int e = e0; e < e1; ++e) {
     const CT cc = coef[e];
-    uint32_t w;
-    if (PAIR) {
-      w = sp[U.nbr[e] * G + g];
-    } else {
-      w = 0;
-      const int o0 = U.oth_off[e], o1 = U.oth_off[e + 1];
-      for (int o = o0; o < o1; ++o) {
-        const uint32_t x = sp[U.oth_pos[o] * G + g];
-        w = (KIND == QMC_ISING) ? (w ^ x) : (w | x);
-      }
+    uint32_t w = 0;
+    const int o0 = U.oth_off[e], o1 = U.oth_off[e + 1];
+    for (int o = o0; o < o1; ++o) {
+      const uint32_t x = sp[U.oth_pos[o] * G + g];
+      w = (KIND == QMC_ISING) ? (w ^ x) : (w | x);
     }
 #pragma unroll
-    for (int b = 0; b < 8; ++b) {
+    for (int b = 0; b < NB; ++b) {
       const bool bit = (w >> b) & 1u;
       if (KIND == QMC_ISING) h[b] += bit ? -cc : cc;
       else h[b] += bit ? CT(0) : cc;
@@ -105,24 +140,57 @@ __device__ __forceinline__ CT delta_from_field(CT h, uint32_t own_bit) {
 }
 
 // Single-replica dE recomputation (rare path: locating the lowest state).
-template <int KIND, bool PAIR, typename CT>
-__device__ CT delta_one(const UnitDev& U, const uint8_t* sp, int q, int G, int g, int b,
-                        uint32_t own_bit) {
+template <int KIND, bool PAIR, typename CT, int NB>
+__device__ CT delta_one(const UnitDev& U, const uint8_t* sp, int q, int slice, int tp, int G, int g,
+                        int b, uint32_t own_bit) {
   CT h[8];
-  local_field<KIND, PAIR, CT>(U, sp, q, G, g, h);
+  local_field<KIND, PAIR, CT, NB>(U, sp, q, slice, tp, G, g, h);
   CT hb = h[0];
 #pragma unroll
-  for (int k = 1; k < 8; ++k)
+  for (int k = 1; k < NB; ++k)
     if (k == b) hb = h[k];
   return delta_from_field<KIND, CT>(hb, own_bit);
 }
 
-template <int KIND, bool PAIR, typename CT>
+// Barrier of the work-groups of one PT ladder (cooperative launches only, so
+// every member is resident).  Bounded: after ~2 s of waiting the error flag
+// is raised and all members abandon the run instead of hanging.
+// The exchanged energies travel as agent-scope atomics; one thread per
+// work-group publishes them with an agent-scope release fence before it
+// arrives and takes an agent-scope acquire fence after the wait.  (Relaxed
+// atomics alone are not enough across the XCDs: without the fences a
+// work-group can read an energy of the previous step.)  Only one thread per
+// work-group fences, because each agent-scope fence writes back and
+// invalidates the L2 of its XCD.
+__device__ bool ladder_barrier(const UnitDev& U, int64_t target) {
+  __hip_atomic_fetch_add(U.sync, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
+  while (__hip_atomic_load(U.sync, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+    if (__hip_atomic_load(U.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return false;
+    if (__builtin_amdgcn_s_memrealtime() - t0 > 200000000ull) {  // 100 MHz clock: 2 s
+      __hip_atomic_store(U.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      return false;
+    }
+    __builtin_amdgcn_s_sleep(2);
+  }
+  return true;
+}
+
+__device__ __forceinline__ void store_agent(double* p, double v) {
+  __hip_atomic_store(reinterpret_cast<unsigned long long*>(p), (unsigned long long)__double_as_longlong(v),
+                     __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ double load_agent(const double* p) {
+  return __longlong_as_double((long long)__hip_atomic_load(
+      reinterpret_cast<const unsigned long long*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+}
+
+template <int KIND, bool PAIR, typename CT, int NB>
 __global__ void __launch_bounds__(kBlock)
 k_sweep(const UnitDev* __restrict__ units, SweepArgs a) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   Smem& S = *reinterpret_cast<Smem*>(smem_raw);
-  const UnitDev U = units[blockIdx.x];
+  const UnitDev U = units[a.unit_of_block ? a.unit_of_block[blockIdx.x] : (int)blockIdx.x];
   const int N = U.N, G = U.G, TP = U.lanes;
   const int stride = U.stride;
   uint8_t* sp = smem_raw + sizeof(Smem);
@@ -135,7 +203,7 @@ k_sweep(const UnitDev* __restrict__ units, SweepArgs a) {
   const int lane = tid & 63;
   const int wave_in_group = tp >> 6;
   const int nrep = (int)U.nrep;
-  const int nb_g = active ? min(8, nrep - 8 * g) : 0;
+  const int nb_g = active ? min(NB, nrep - 8 * g) : 0;
   const uint32_t rbase = U.r0 + 8u * (uint32_t)g;
 
   for (int i = tid; i < 1024; i += nthreads) S.lut[i] = a.lut[i];
@@ -149,16 +217,19 @@ k_sweep(const UnitDev* __restrict__ units, SweepArgs a) {
     S.rbest[tid] = U.best_cost[tid];
     S.rbeta[tid] = U.beta[tid];
     S.racc[tid] = 0;
-    if (U.ladder > 0) {
-      S.label[tid] = U.label[tid];
-      S.lbeta[tid] = U.ladder_beta[tid];
-    }
   }
+  if (tid == 0) S.abort_flag = 0;
+  if (U.ladder > 0 && tid < U.ladder) {
+    S.label[tid] = U.label[tid];
+    S.lbeta[tid] = U.ladder_beta[tid];
+  }
+  __syncthreads();
+  if (U.ladder > 0 && tid < U.ladder) S.inv[S.label[tid]] = tid;
   __syncthreads();
 
   uint32_t acc[8];
 #pragma unroll
-  for (int b = 0; b < 8; ++b) acc[b] = 0;
+  for (int b = 0; b < NB; ++b) acc[b] = 0;
 
   const int64_t sweep_begin = U.progress[0];
   const int n_run = (int)min((int64_t)a.n_sweeps, U.progress[1]);
@@ -169,15 +240,16 @@ k_sweep(const UnitDev* __restrict__ units, SweepArgs a) {
     __syncthreads();
     double beta[8];
 #pragma unroll
-    for (int b = 0; b < 8; ++b) beta[b] = (b < nb_g) ? S.rbeta[8 * g + b] : 0.0;
+    for (int b = 0; b < NB; ++b) beta[b] = (b < nb_g) ? S.rbeta[8 * g + b] : 0.0;
 
     for (int c = 0; c < U.n_classes; ++c) {
       const int c0 = U.class_off[c];
       const int nc = U.class_off[c + 1] - c0;
       const int K = (nc + TP - 1) / TP;
+      const int sbase = PAIR ? U.ell_slice_base[c] : 0;
       CT Ssum[8], Smin[8];
 #pragma unroll
-      for (int b = 0; b < 8; ++b) {
+      for (int b = 0; b < NB; ++b) {
         Ssum[b] = CT(0);
         Smin[b] = CT(0);
       }
@@ -189,11 +261,11 @@ k_sweep(const UnitDev* __restrict__ units, SweepArgs a) {
           const uint32_t var = (uint32_t)U.var_of_pos[q];
           const uint32_t own = sp[q * G + g];
           CT h[8];
-          local_field<KIND, PAIR, CT>(U, sp, q, G, g, h);
+          local_field<KIND, PAIR, CT, NB>(U, sp, q, sbase + k, tp, G, g, h);
           uint32_t flips = 0;
           uint4 quad = make_uint4(0, 0, 0, 0);
 #pragma unroll
-          for (int b = 0; b < 8; ++b) {
+          for (int b = 0; b < NB; ++b) {
             if (b < nb_g) {
               const uint32_t r = rbase + (uint32_t)b;
               if (b == 0 || (r & 3u) == 0u)
@@ -220,7 +292,7 @@ k_sweep(const UnitDev* __restrict__ units, SweepArgs a) {
       // ---- per replica: sequence minimum of the cost over this class ----
       if (active) {
 #pragma unroll
-        for (int b = 0; b < 8; ++b) {
+        for (int b = 0; b < NB; ++b) {
           if (b < nb_g) {
             CT incl = Ssum[b];
 #pragma unroll
@@ -281,7 +353,7 @@ k_sweep(const UnitDev* __restrict__ units, SweepArgs a) {
         // locate k* in the owning thread, then rebuild the lowest states
         if (active) {
 #pragma unroll
-          for (int b = 0; b < 8; ++b) {
+          for (int b = 0; b < NB; ++b) {
             if (b < nb_g && S.nb_tp[8 * g + b] == tp) {
               CT pre = CT(0);
               int ks = -1;  // Smin == 0: the empty prefix is the first minimum
@@ -293,7 +365,7 @@ k_sweep(const UnitDev* __restrict__ units, SweepArgs a) {
                   const uint32_t f = fl[q * G + g];
                   if ((f >> b) & 1u) {
                     const uint32_t ob = ((sp[q * G + g] ^ f) >> b) & 1u;  // pre-flip bit
-                    pre += delta_one<KIND, PAIR, CT>(U, sp, q, G, g, b, ob);
+                    pre += delta_one<KIND, PAIR, CT, NB>(U, sp, q, sbase + k, tp, G, g, b, ob);
                     if (pre == Smin[b]) {
                       ks = k;
                       break;
@@ -309,7 +381,7 @@ k_sweep(const UnitDev* __restrict__ units, SweepArgs a) {
         if (active) {
           uint32_t newmask = 0;
 #pragma unroll
-          for (int b = 0; b < 8; ++b)
+          for (int b = 0; b < NB; ++b)
             if (b < nb_g && S.nb_tp[8 * g + b] >= 0) newmask |= 1u << b;
           if (newmask) {
             for (int q = tp; q < N; q += TP) {
@@ -320,7 +392,7 @@ k_sweep(const UnitDev* __restrict__ units, SweepArgs a) {
                 const uint32_t f = fl[q * G + g];
                 const int tq = rho % TP, kq = rho / TP;
 #pragma unroll
-                for (int b = 0; b < 8; ++b) {
+                for (int b = 0; b < NB; ++b) {
                   if ((newmask >> b) & 1u) {
                     const int ts = S.nb_tp[8 * g + b], ks = S.kstar[8 * g + b];
                     const bool after = tq > ts || (tq == ts && kq > ks);
@@ -338,25 +410,49 @@ k_sweep(const UnitDev* __restrict__ units, SweepArgs a) {
       }
     }  // classes
 
-    // ---- parallel tempering exchange inside the unit (ladder == nrep) ----
+    // ---- parallel tempering exchange (parallel_tempering.h:126-189) ----
+    // Every work-group of the ladder computes the same decisions from the
+    // same energies, so labels stay identical without another barrier.
     if (U.ladder > 0 && ((sweep64 + 1) % U.spr) == 0) {
-      __syncthreads();
-      const uint32_t step = (uint32_t)(sweep64 / U.spr);
+      const int64_t step64 = sweep64 / U.spr;
+      const uint32_t step = (uint32_t)step64;
       const int L = U.ladder;
+      if (U.upl > 1) {
+        double* cb = U.cbuf + (step64 & 1) * L;  // double-buffered by step parity
+        if (tid < nrep) store_agent(cb + U.slot0 + tid, S.rcost[tid]);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __syncthreads();
+        if (tid == 0) {
+          // release: the work-group's energy stores before the arrival
+          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+          if (!ladder_barrier(U, (step64 + 1) * U.upl)) S.abort_flag = 1;
+          // acquire: the other work-groups' energies after the barrier
+          __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+        __syncthreads();
+        if (S.abort_flag) break;
+        if (tid < L) S.lcost[tid] = load_agent(cb + tid);
+      } else if (tid < L) {
+        S.lcost[tid] = S.rcost[tid];
+      }
+      __syncthreads();
       const int i = (int)(step & 1u) + 2 * tid;
       if (i < L - 1) {
         const int sa = S.label[i], sb = S.label[i + 1];
         const uint4 w = philox4x32_10((uint32_t)i, step, 0u, DOM_EXCHANGE, U.seed, U.stream);
         const double u = canonical2(w.x, w.y);
         const double bi = S.lbeta[i], bj = S.lbeta[i + 1];
-        const double arg = __dmul_rn(__dsub_rn(S.rcost[sa], S.rcost[sb]), __dsub_rn(bi, bj));
+        const double arg = __dmul_rn(__dsub_rn(S.lcost[sa], S.lcost[sb]), __dsub_rn(bi, bj));
         if (u < exp(arg)) {
           S.label[i] = sb;
           S.label[i + 1] = sa;
-          S.rbeta[sa] = bj;
-          S.rbeta[sb] = bi;
+          S.inv[sa] = i + 1;
+          S.inv[sb] = i;
         }
       }
+      __syncthreads();
+      if (tid < nrep) S.rbeta[tid] = S.lbeta[S.inv[U.slot0 + tid]];
     }
     __syncthreads();
   }  // sweeps
@@ -364,7 +460,7 @@ k_sweep(const UnitDev* __restrict__ units, SweepArgs a) {
   // ---- write back ----
   if (active) {
 #pragma unroll
-    for (int b = 0; b < 8; ++b)
+    for (int b = 0; b < NB; ++b)
       if (b < nb_g && acc[b]) atomicAdd(&S.racc[8 * g + b], acc[b]);
   }
   {
@@ -378,8 +474,8 @@ k_sweep(const UnitDev* __restrict__ units, SweepArgs a) {
     U.best_cost[tid] = S.rbest[tid];
     U.beta[tid] = S.rbeta[tid];
     U.accepted[tid] += S.racc[tid];
-    if (U.ladder > 0) U.label[tid] = S.label[tid];
   }
+  if (U.ladder > 0 && U.slot0 == 0 && tid < U.ladder) U.label[tid] = S.label[tid];
   if (tid == 0) {
     U.progress[0] = sweep_begin + n_run;
     U.progress[1] -= n_run;
@@ -734,39 +830,88 @@ static hipError_t raise_lds_limit(F fn) {
   return once;
 }
 
-template <int KIND, bool PAIR, typename CT>
+template <int KIND, bool PAIR, typename CT, int NB>
 static hipError_t launch_sweep_t(const UnitDev* units, int n_units, int block, size_t smem,
-                                 const SweepArgs& a, int ordered, hipStream_t st) {
+                                 const SweepArgs& a, int ordered, int cooperative, hipStream_t st) {
   if (ordered) {
     auto fn = k_sweep_ordered<KIND, PAIR, CT>;
     hipError_t e = raise_lds_limit(fn);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(fn, dim3(n_units), dim3(64), smem, st, units, a);
-  } else {
-    auto fn = k_sweep<KIND, PAIR, CT>;
-    hipError_t e = raise_lds_limit(fn);
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(fn, dim3(n_units), dim3(block), smem, st, units, a);
+    return hipGetLastError();
   }
+  auto fn = k_sweep<KIND, PAIR, CT, NB>;
+  hipError_t e = raise_lds_limit(fn);
+  if (e != hipSuccess) return e;
+  if (cooperative) {
+    SweepArgs args = a;
+    const UnitDev* u = units;
+    void* kargs[] = {(void*)&u, (void*)&args};
+    return hipLaunchCooperativeKernel((const void*)fn, dim3(n_units), dim3(block), kargs, (unsigned)smem, st);
+  }
+  hipLaunchKernelGGL(fn, dim3(n_units), dim3(block), smem, st, units, a);
   return hipGetLastError();
 }
 
-hipError_t launch_sweep(int kind, int pairwise, int coupling, const UnitDev* units, int n_units,
-                        int block, size_t smem, const SweepArgs& a, int ordered, hipStream_t st) {
+template <int KIND, bool PAIR, typename CT>
+static hipError_t launch_nb(int nb, const UnitDev* units, int n_units, int block, size_t smem,
+                            const SweepArgs& a, int ordered, int coop, hipStream_t st) {
+  if (nb <= 1) return launch_sweep_t<KIND, PAIR, CT, 1>(units, n_units, block, smem, a, ordered, coop, st);
+  if (nb <= 2) return launch_sweep_t<KIND, PAIR, CT, 2>(units, n_units, block, smem, a, ordered, coop, st);
+  if (nb <= 4) return launch_sweep_t<KIND, PAIR, CT, 4>(units, n_units, block, smem, a, ordered, coop, st);
+  return launch_sweep_t<KIND, PAIR, CT, 8>(units, n_units, block, smem, a, ordered, coop, st);
+}
+
+hipError_t launch_sweep(int kind, int pairwise, int coupling, int nb, const UnitDev* units, int n_units,
+                        int block, size_t smem, const SweepArgs& a, int ordered, int coop,
+                        hipStream_t st) {
+  const bool i32 = coupling == QMC_COUPLING_I32;
   if (kind == QMC_ISING) {
-    if (pairwise) {
-      return coupling == QMC_COUPLING_I32 ? launch_sweep_t<QMC_ISING, true, int32_t>(units, n_units, block, smem, a, ordered, st)
-                                          : launch_sweep_t<QMC_ISING, true, float>(units, n_units, block, smem, a, ordered, st);
-    }
-    return coupling == QMC_COUPLING_I32 ? launch_sweep_t<QMC_ISING, false, int32_t>(units, n_units, block, smem, a, ordered, st)
-                                        : launch_sweep_t<QMC_ISING, false, float>(units, n_units, block, smem, a, ordered, st);
+    if (pairwise)
+      return i32 ? launch_nb<QMC_ISING, true, int32_t>(nb, units, n_units, block, smem, a, ordered, coop, st)
+                 : launch_nb<QMC_ISING, true, float>(nb, units, n_units, block, smem, a, ordered, coop, st);
+    return i32 ? launch_nb<QMC_ISING, false, int32_t>(nb, units, n_units, block, smem, a, ordered, coop, st)
+               : launch_nb<QMC_ISING, false, float>(nb, units, n_units, block, smem, a, ordered, coop, st);
   }
-  if (pairwise) {
-    return coupling == QMC_COUPLING_I32 ? launch_sweep_t<QMC_PUBO, true, int32_t>(units, n_units, block, smem, a, ordered, st)
-                                        : launch_sweep_t<QMC_PUBO, true, float>(units, n_units, block, smem, a, ordered, st);
+  if (pairwise)
+    return i32 ? launch_nb<QMC_PUBO, true, int32_t>(nb, units, n_units, block, smem, a, ordered, coop, st)
+               : launch_nb<QMC_PUBO, true, float>(nb, units, n_units, block, smem, a, ordered, coop, st);
+  return i32 ? launch_nb<QMC_PUBO, false, int32_t>(nb, units, n_units, block, smem, a, ordered, coop, st)
+             : launch_nb<QMC_PUBO, false, float>(nb, units, n_units, block, smem, a, ordered, coop, st);
+}
+
+template <int KIND, bool PAIR, typename CT, int NB>
+static int coresident_t(int block, size_t smem) {
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_sweep<KIND, PAIR, CT, NB>, block,
+                                                   smem) != hipSuccess)
+    return 0;
+  int dev = 0, cus = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+  return per_cu * cus;
+}
+
+template <int KIND, bool PAIR, typename CT>
+static int coresident_nb(int nb, int block, size_t smem) {
+  if (nb <= 1) return coresident_t<KIND, PAIR, CT, 1>(block, smem);
+  if (nb <= 2) return coresident_t<KIND, PAIR, CT, 2>(block, smem);
+  if (nb <= 4) return coresident_t<KIND, PAIR, CT, 4>(block, smem);
+  return coresident_t<KIND, PAIR, CT, 8>(block, smem);
+}
+
+int max_coresident_blocks(int kind, int pairwise, int coupling, int nb, int block, size_t smem) {
+  const bool i32 = coupling == QMC_COUPLING_I32;
+  if (kind == QMC_ISING) {
+    if (pairwise) return i32 ? coresident_nb<QMC_ISING, true, int32_t>(nb, block, smem)
+                             : coresident_nb<QMC_ISING, true, float>(nb, block, smem);
+    return i32 ? coresident_nb<QMC_ISING, false, int32_t>(nb, block, smem)
+               : coresident_nb<QMC_ISING, false, float>(nb, block, smem);
   }
-  return coupling == QMC_COUPLING_I32 ? launch_sweep_t<QMC_PUBO, false, int32_t>(units, n_units, block, smem, a, ordered, st)
-                                      : launch_sweep_t<QMC_PUBO, false, float>(units, n_units, block, smem, a, ordered, st);
+  if (pairwise) return i32 ? coresident_nb<QMC_PUBO, true, int32_t>(nb, block, smem)
+                           : coresident_nb<QMC_PUBO, true, float>(nb, block, smem);
+  return i32 ? coresident_nb<QMC_PUBO, false, int32_t>(nb, block, smem)
+             : coresident_nb<QMC_PUBO, false, float>(nb, block, smem);
 }
 
 hipError_t launch_exchange(double* cost, double* beta, int32_t* label, const double* lbeta,

@@ -68,10 +68,12 @@ def test_energy_and_delta_match_oracle(ctx, name):
 
 
 def _run_pair(ctx, kind, terms, R, B, betas, seed=7, ordered=False, lanes=0):
-    op, qp = pair(ctx, kind, terms, lanes=lanes)
-    order, co = qp.order()
-    assert op.check_classes(order, co)
+    op, qp = pair(ctx, kind, terms)
     w = _qmc.Walkers(qp, R, B, seed=seed)
+    if lanes:
+        w.set_lanes(lanes)
+    order, co = qp.order(w.lanes())
+    assert op.check_classes(order, co)
     w.init()
     r0 = w.read()
     opts = po.make_opts(po.RNG_COUNTER, seed=seed, arith=arith_of(qp),
@@ -108,6 +110,18 @@ def test_sa_sweeps_bit_exact_int(ctx, name, B):
     assert np.array_equal(w.energy(lowest=True), np.array([op.cost(s) for s in got["lowest_states"]]))
 
 
+@pytest.mark.parametrize("lanes", [64, 128, 256, 512])
+@pytest.mark.parametrize("B", [1, 8])
+def test_sa_sweeps_lanes_and_slices(ctx, lanes, B):
+    """Wider class rounds (several waves per replica group), ragged slices."""
+    terms = wl.er_ising(1500, 7.0, seed=8)
+    betas = np.linspace(0.1, 3.0, 10)
+    op, qp, w, got, want = _run_pair(ctx, "ising", terms, R=9, B=B, betas=betas, lanes=lanes)
+    assert w.lanes() == lanes
+    for k in ("states", "cost", "lowest", "lowest_states", "accepted"):
+        assert np.array_equal(got[k], want[k]), k
+
+
 @pytest.mark.parametrize("name", ["lattice2d_pm1", "er_int", "pubo_int"])
 def test_ordered_sweeps_bit_exact_int(ctx, name):
     kind, gen = PROBLEMS[name]
@@ -131,20 +145,24 @@ def test_sa_sweeps_f32(ctx):
     np.testing.assert_allclose(got["cost"], want["cost"], rtol=1e-5)
 
 
-@pytest.mark.parametrize("name,B,L", [("er_int", 8, 8), ("lattice2d_pm1", 16, 16),
-                                      ("er_int", 4, 8), ("pubo_int", 8, 16)])
-def test_pt_matches_oracle(ctx, name, B, L):
+@pytest.mark.parametrize("name,B,L,mode", [("er_int", 8, 8, 0), ("lattice2d_pm1", 16, 16, 0),
+                                           ("er_int", 4, 8, 0), ("pubo_int", 8, 16, 0),
+                                           ("er_int", 1, 8, 0), ("lattice2d_pm1", 2, 32, 0),
+                                           ("er_int", 4, 8, 1), ("pubo_int", 8, 16, 1)])
+def test_pt_matches_oracle(ctx, name, B, L, mode):
     kind, gen = PROBLEMS[name]
     terms = gen()
     op, qp = pair(ctx, kind, terms)
-    order, _ = qp.order()
     temps = wl.geometric_temperatures(0.3, 3.0, L)
     betas = sorted([1.0 / t for t in temps], reverse=True)
     n_ladders = 3
     steps = 40
     w = _qmc.Walkers(qp, L * n_ladders, B, seed=21, replicas_per_seed=L)
+    order, _ = qp.order(w.lanes())
+    w.set_pt_mode(mode)
     w.init()
-    w.pt(betas, 0, steps)
+    w.pt(betas, 0, steps // 2)
+    w.pt(betas, steps // 2, steps - steps // 2)  # continuation keeps the ladder
     got = w.read()
     labels = w.labels()
     for ld in range(n_ladders):
@@ -191,3 +209,32 @@ def test_batch_mixed_problems_same_as_alone(ctx):
         b = w2.read()
         for k in ("states", "cost", "lowest", "lowest_states", "accepted"):
             assert np.array_equal(a[k], b[k]), k
+
+
+@pytest.mark.parametrize("kind,terms,n_ladders,steps,groups",
+                         [("ising", "lattice3d", 6, 300, (32, 16, 8, 4, 2, 1)),
+                          ("ising", "er", 6, 300, (32, 8, 1)),
+                          ("ising", "lattice16", 16, 1500, (32, 2, 1))])
+def test_pt_independent_of_replica_grouping(ctx, kind, terms, n_ladders, steps, groups):
+    """Cooperative ladders: the same run for every split of a 32-replica
+    ladder over work-groups (the RNG contract does not see the grouping), so
+    the cross-work-group energy hand-off cannot have lost an update.  The
+    last case is the full-size benchmark problem with 512 work-groups."""
+    tr = {"lattice3d": lambda: wl.lattice3d(8, seed=12), "er": lambda: wl.er_ising(700, 6.0, seed=13),
+          "lattice16": lambda: wl.lattice3d(16, seed=14)}[terms]()
+    op, qp = pair(ctx, kind, tr)
+    temps = wl.geometric_temperatures(0.2, 2.0, 32)
+    betas = sorted([1.0 / t for t in temps], reverse=True)
+    ref = None
+    for B in groups:
+        w = _qmc.Walkers(qp, 32 * n_ladders, B, seed=5, replicas_per_seed=32)
+        w.set_lanes(128)
+        w.init()
+        w.pt(betas, 0, steps)
+        got = w.read()
+        got["labels"] = w.labels()
+        if ref is None:
+            ref = got
+            continue
+        for k in ("states", "cost", "lowest", "labels", "beta"):
+            assert np.array_equal(got[k], ref[k]), (B, k)
