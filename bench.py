@@ -29,6 +29,13 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 LDS_PEAK_GBS = 256 * 128 * 2.4  # 256 CUs x 128 B/clk x 2.4 GHz
 
+# HBM bytes per sweep-kernel launch of the default workload (5000 sweeps,
+# automatic grouping and lanes), from rocprofv3 --pmc FETCH_SIZE and
+# WRITE_SIZE in separate runs, scaled by what the same counters report for a
+# plain 512 MiB copy (FETCH_SIZE x 2, WRITE_SIZE x 1):
+# 214164 KiB x 2 + 428366 KiB.  Summary: profiles/k_sweep_pt_16cube_hbm_traffic.md
+MEASURED_TRAFFIC_BYTES = {"c2": (214164 * 2 + 428366) * 1024}
+
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
@@ -78,6 +85,8 @@ def run_c2(args):
     avg_ms = float(np.mean(dev_ms))
     B = 6 * (4 + 4 + 1) + 2
     achieved = attempts_per_step * B / (avg_ms * 1e-3) / 1e9
+    default_shape = args.sweeps == 5000 and not args.group and not args.lanes
+    traffic = MEASURED_TRAFFIC_BYTES["c2"] if default_shape else None
     out = {
         "metric": "spin-flip attempts/s at equal final-energy quality; p50/p99 solve latency",
         "value": value,
@@ -96,7 +105,7 @@ def run_c2(args):
                    "n_spins": N, "couplings": 12288, "replicas": L, "restarts": restarts,
                    "sweeps": sweeps, "parallelism": "none (1 GPU)"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "bytes_per_attempt": B, "kernel": "k_sweep<ISING,pairwise,f32>",
                      "kernel_ms": avg_ms, "lds_frac": achieved / LDS_PEAK_GBS},
         "quality": {"best_energy_mean": float(np.mean(best)), "best_energy_min": float(np.min(best)),
