@@ -116,6 +116,119 @@ def run_c2(args):
     return out
 
 
+def c4_instances(n=4096, seed=4096):
+    """4096 Erdos-Renyi instances, N log-uniform in [64, 4096], mean degree 6,
+    J uniform in {-3..3}\\{0}, one field term h in {-1,0,1} per spin."""
+    from qiotoolkit_amd import workloads as wl
+    sizes = wl.log_uniform_sizes(n, 64, 4096, seed=seed)
+    return [(int(N),) + wl.er_ising_arrays(int(N), 6.0, 3, seed=seed * 16 + k)
+            for k, N in enumerate(sizes)]
+
+
+def run_c4(args):
+    """Batched uneven SA: every instance 8 restarts x `sweeps` sweeps, linear
+    beta 0.1 -> 3.0, fused into shared launches, against the same solves run
+    one instance per launch (a sample of them) -- same seeds, same results."""
+    from qiotoolkit_amd.solver import Engine
+    sweeps = args.sweeps if args.sweeps != 5000 else 1000
+    t0 = time.perf_counter()
+    insts = c4_instances(args.instances)
+    eng = Engine(0)
+    models = [eng.model_from_arrays("ising", off, ids, c) for (_, off, ids, c) in insts]
+    log("c4: %d instances, %d spins, ingest %.1f s" % (len(insts), sum(i[0] for i in insts),
+                                                       time.perf_counter() - t0))
+    extra = {}
+    if args.group:
+        extra["replicas_per_group"] = args.group
+    if args.lanes:
+        extra["lanes"] = args.lanes
+    if args.positions_per_lane:
+        extra["positions_per_lane"] = args.positions_per_lane
+    target = "simulatedannealing.qiotoolkit"
+
+    def params(k):
+        p = {"beta_start": 0.1, "beta_stop": 3.0, "restarts": 8, "sweeps": sweeps, "seed": 7000 + k}
+        p.update(extra)
+        return {"params": p}
+
+    plist = [params(k) for k in range(len(models))]
+    tune = None
+    if not (args.group or args.lanes or args.positions_per_lane or args.no_tune):
+        t1 = time.perf_counter()
+        tune = eng.tune_batch(target, models, plist, trial_sweeps=20)
+        tune["seconds"] = time.perf_counter() - t1
+        ch = tune["params"]
+        extra["replicas_per_group"] = ch["replicas_per_group"]
+        extra["positions_per_lane"] = ch["positions_per_lane"]
+        if ch["sweeps_per_launch_fraction"] > 0:
+            extra["sweeps_per_launch"] = max(1, int(round(ch["sweeps_per_launch_fraction"] * sweeps)))
+        log("c4 autotune %.1f s: %s" % (tune["seconds"], json.dumps(extra)))
+        plist = [params(k) for k in range(len(models))]
+    attempts = 8 * sweeps * sum(i[0] for i in insts)
+    for _ in range(args.warmup):
+        eng.solve_batch_arrays(target, models, plist, configs=False)
+    t0 = time.perf_counter()
+    dev_ms, tms = [], None
+    for _ in range(args.steps):
+        costs, cfg, tms = eng.solve_batch_arrays(target, models, plist, configs=True)
+        dev_ms.append(tms["device_ms"])
+    t = time.perf_counter() - t0
+    log("c4 phases ms: init %.1f run %.1f finalize %.1f device %.1f" % (
+        tms["init_ms"], tms["run_ms"], tms["finalize_ms"], tms["device_ms"]))
+    value = attempts * args.steps / t
+    avg_ms = float(np.mean(dev_ms))
+    B = 7 * (4 + 1 + 1) + 2   # d = 6 couplings + 1 field term, int8 coefficients
+    achieved = attempts * B / (avg_ms * 1e-3) / 1e9
+    # plain form: the same solves, one instance per launch, on a sample --
+    # with the batch's parameters, and with one replica per work-group (the
+    # plain form's best shape; the grouping does not change results)
+    stride = max(1, len(models) // args.plain_sample)
+    idx = list(range(0, len(models), stride))
+    plain = {}
+    same = True
+    for label, over in (("batch_params", {}), ("one_replica_per_group", {"replicas_per_group": 1})):
+        plain_t, plain_att = 0.0, 0
+        for k in idx:
+            pk = {"params": dict(plist[k]["params"], **over)}
+            pk["params"].pop("sweeps_per_launch", None)
+            t1 = time.perf_counter()
+            c1, _, _ = eng.solve_batch_arrays(target, [models[k]], [pk], configs=False)
+            plain_t += time.perf_counter() - t1
+            plain_att += 8 * sweeps * insts[k][0]
+            same &= bool(c1[0] == costs[k])
+        plain[label] = plain_att / plain_t
+    plain_best = max(plain.values())
+    out = {
+        "metric": "spin-flip attempts/s at equal final-energy quality; p50/p99 solve latency",
+        "value": value,
+        "unit": "spin-flip attempts/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": t * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32",
+        "data": "synthetic (seeded Erdos-Renyi instances)",
+        "config": {"workload": "batched uneven SA: %d ER instances N log-uniform [64,4096], mean degree 6, "
+                               "J int in [-3,3]\\{0}, h in {-1,0,1}, 8 restarts x %d sweeps, beta 0.1->3.0"
+                               % (len(models), sweeps),
+                   "instances": len(models), "spins": sum(i[0] for i in insts), "restarts": 8,
+                   "sweeps": sweeps, "params": extra, "parallelism": "none (1 GPU)"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "bytes_per_attempt": B,
+                     "kernel": "k_sweep<ISING,pairwise,i32>", "kernel_ms": avg_ms,
+                     "lds_frac": achieved / LDS_PEAK_GBS},
+        "plain_per_instance": {"value": plain_best, "by_shape": plain, "instances_sampled": len(idx),
+                               "speedup_of_batched": value / plain_best,
+                               "identical_best_costs": same},
+        "autotune": tune,
+        "quality": {"best_cost_mean": float(np.mean(costs))},
+    }
+    return out
+
+
 def cpu_baseline_c2(inp, temps, args):
     """Reference algorithm (oracle) on a bounded sample: one 32-replica ladder."""
     from oracle import pyoracle as po
@@ -151,6 +264,12 @@ def main():
     ap.add_argument("--sweeps", type=int, default=5000)
     ap.add_argument("--group", type=int, default=0, help="replicas per work-group (0 = auto)")
     ap.add_argument("--lanes", type=int, default=0, help="positions per class round (0 = auto)")
+    ap.add_argument("--positions-per-lane", type=int, default=0,
+                    help="lanes cover the largest colour class in this many rounds (0 = auto)")
+    ap.add_argument("--instances", type=int, default=4096, help="c4: number of instances")
+    ap.add_argument("--no-tune", action="store_true", help="c4: skip the autotuner")
+    ap.add_argument("--plain-sample", type=int, default=128,
+                    help="c4: instances run one per launch for the plain-form rate")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -172,10 +291,10 @@ def main():
               "multi-GPU runs are not part of this build (--gpus %d refused)" % args.gpus,
               file=sys.stderr)
         sys.exit(2)
-    if args.workload != "c2":
+    if args.workload not in ("c2", "c4"):
         print("unknown workload %s" % args.workload, file=sys.stderr)
         sys.exit(2)
-    out = run_c2(args)
+    out = run_c2(args) if args.workload == "c2" else run_c4(args)
     print(json.dumps(out), flush=True)
 
 

@@ -48,6 +48,30 @@ int qiotk_solve_model(qiotk_model m, const char* target, const char* params_json
  * = summed device time of the fused launches. */
 int qiotk_solve_batch(qiotk_engine e, const char* target, int32_t n, const qiotk_model* models,
                       const char* const* params_json, char** responses, double* device_ms);
+/* The same solves with results as arrays instead of JSON documents:
+ * cost[k] = the response's solutions.cost; configs (may be NULL) receives
+ * n_vars(k) values per solve, concatenated in k order, holding the
+ * response's solutions.configuration by internal variable id (+1/-1 Ising,
+ * 1/0 PUBO; names from qiotk_model_vars).  timings (may be NULL) = {host
+ * init ms, run ms, finalize ms, device ms}. */
+int qiotk_solve_batch_arrays(qiotk_engine e, const char* target, int32_t n, const qiotk_model* models,
+                             const char* const* params_json, double* cost, int8_t* configs,
+                             double* timings);
+/* Autotuner for batched solves: short trials (trial_sweeps sweeps each, the
+ * schedule compressed accordingly) of candidate launch shapes -- replicas per
+ * work-group x lane depth (positions each lane covers per colour class,
+ * which sets the LDS tile of a class round) -- on a sample of the solves (at
+ * most 512), timed with HIP events; the fastest is returned as
+ * {"params": {"replicas_per_group": B, "positions_per_lane": k}, "trials":
+ * [...]} to merge into every solve's params.  Sweeps per launch need no
+ * tuning: a batch runs every planned sweep in one persistent launch.
+ * replicas_per_group never changes results (the RNG counters do not see the
+ * grouping); positions_per_lane changes the colour-class visit order. */
+int qiotk_tune_batch(qiotk_engine e, const char* target, int32_t n, const qiotk_model* models,
+                     const char* const* params_json, int32_t trial_sweeps, char** response);
+/* Variable count and the original variable names in internal-id order
+ * (first appearance in the term list, graph/graph.h:97-124); names may be NULL. */
+int qiotk_model_vars(qiotk_model m, int32_t* n_vars, int32_t* names);
 
 /* Device-free host logic, for tooling and tests. */
 /* graph::normalize_edges result as JSON: names, term CSR, const_cost, ... */

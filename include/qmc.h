@@ -64,6 +64,8 @@ typedef struct {
   int32_t pairwise;        /* every term has at most two variables */
   int32_t max_degree;      /* max incident terms per variable */
   double mean_degree;
+  int32_t max_class;       /* size of the largest colour class */
+  int32_t pad_;
 } qmc_problem_info;
 
 /* thread-local message of the last failing call */
@@ -142,8 +144,21 @@ int qmc_batch_remove(qmc_batch b, qmc_walkers w);
 int qmc_batch_clear(qmc_batch b);
 /* Asynchronous: every member work-group runs up to max_sweeps planned sweeps. */
 int qmc_batch_run(qmc_batch b, int32_t max_sweeps);
+/* Asynchronous: every planned sweep as launches of sweeps_per_launch sweeps,
+ * recorded once into a HIP graph (again only when the batch changes) and
+ * replayed; the units' progress lives on the device, so replays take no
+ * argument updates.  Not for split PT ladders. */
+int qmc_batch_run_chunked(qmc_batch b, int32_t sweeps_per_launch);
 /* Wait for the last qmc_batch_run; ms = its device time (HIP events). */
 int qmc_batch_wait(qmc_batch b, double* ms);
+/* Read back every member in the order added, in one launch and one copy per
+ * array: member k owns the next R_k entries of lowest_energy (FP64 energy of
+ * each replica's lowest state, recomputed from the term list as
+ * qmc_walkers_energy does) and accepted, and the next R_k * n_vars_k bytes
+ * of lowest_states.  Replaces the per-solver Walker::get_lowest_cost /
+ * get_lowest_state reads (markov/walker.h:232-305) of many solves.  Any
+ * pointer may be NULL. */
+int qmc_batch_read(qmc_batch b, double* lowest_energy, uint64_t* accepted, uint8_t* lowest_states);
 
 /* Positions processed side by side in a class round (multiple of 64,
  * lanes * ceil(replicas_per_group / 8) <= 512; 0 = the problem default).

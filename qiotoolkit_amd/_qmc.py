@@ -25,7 +25,7 @@ EXPORTS = [
     "qmc_walkers_labels", "qmc_energy", "qmc_delta_all", "qmc_philox", "qmc_walkers_last_ms",
     "qmc_walkers_plan", "qmc_walkers_plan_pt", "qmc_walkers_remaining", "qmc_batch_create",
     "qmc_batch_destroy", "qmc_batch_add", "qmc_batch_remove", "qmc_batch_clear", "qmc_batch_run",
-    "qmc_batch_wait", "qmc_walkers_resample", "qmc_walkers_count", "qmc_walkers_set_pt_mode",
+    "qmc_batch_wait", "qmc_batch_read", "qmc_batch_run_chunked", "qmc_walkers_resample", "qmc_walkers_count", "qmc_walkers_set_pt_mode",
     "qmc_walkers_set_lanes", "qmc_walkers_get_lanes",
 ]
 
@@ -41,6 +41,7 @@ class ProblemInfo(C.Structure):
         ("n_vars", C.c_int32), ("n_terms", C.c_int64), ("n_incidences", C.c_int64),
         ("n_classes", C.c_int32), ("lanes", C.c_int32), ("coupling", C.c_int32),
         ("pairwise", C.c_int32), ("max_degree", C.c_int32), ("mean_degree", C.c_double),
+        ("max_class", C.c_int32), ("pad_", C.c_int32),
     ]
 
 
@@ -96,6 +97,8 @@ def lib():
     L.qmc_batch_clear.argtypes = [vp]
     L.qmc_batch_run.argtypes = [vp, C.c_int32]
     L.qmc_batch_wait.argtypes = [vp, P(C.c_double)]
+    L.qmc_batch_read.argtypes = [vp, vp, vp, vp]
+    L.qmc_batch_run_chunked.argtypes = [vp, C.c_int32]
     L.qmc_walkers_resample.argtypes = [vp, P(C.c_int32), C.c_int32]
     L.qmc_walkers_count.argtypes = [vp, P(C.c_int32)]
     L.qmc_walkers_set_pt_mode.argtypes = [vp, C.c_int]
@@ -329,7 +332,30 @@ class Batch:
     def run(self, max_sweeps):
         _check(lib().qmc_batch_run(self.h, max_sweeps))
 
+    def run_chunked(self, sweeps_per_launch):
+        _check(lib().qmc_batch_run_chunked(self.h, sweeps_per_launch))
+
     def wait(self):
         v = C.c_double()
         _check(lib().qmc_batch_wait(self.h, C.byref(v)))
         return v.value
+
+    def read(self):
+        """Lowest-state energies, accepted counts and lowest states of every
+        member (a list of per-member dicts, in the order added)."""
+        import numpy as np
+        R = [m.R for m in self.members]
+        N = [m.problem.n for m in self.members]
+        tot_r = sum(R)
+        tot_s = sum(r * n for r, n in zip(R, N))
+        e = np.zeros(tot_r, dtype=np.float64)
+        a = np.zeros(tot_r, dtype=np.uint64)
+        s = np.zeros(max(tot_s, 1), dtype=np.uint8)
+        _check(lib().qmc_batch_read(self.h, e.ctypes.data, a.ctypes.data, s.ctypes.data))
+        out, ro, so = [], 0, 0
+        for r, n in zip(R, N):
+            out.append({"lowest": e[ro:ro + r], "accepted": a[ro:ro + r],
+                        "lowest_states": s[so:so + r * n].reshape(r, n)})
+            ro += r
+            so += r * n
+        return out

@@ -1,5 +1,6 @@
 // C ABI of the host library (include/qiotk.h): Runner-style solves over the
 // device engine.
+#include <algorithm>
 #include <chrono>
 #include <cstdlib>
 #include <cstring>
@@ -77,6 +78,75 @@ int guarded(char** response, F&& f) {
   } catch (const std::exception& e) {
     return report(Error(kRuntimeError, e.what()), response);
   }
+}
+
+// Many solves of one target fused into shared device launches: configure
+// and init every solve, plan the fusible ones into one batch, run it, read
+// every fused result back in one transfer, then run the rest one by one.
+struct BatchRun {
+  std::vector<std::unique_ptr<Solver>> solvers;
+  std::vector<int> fused;
+  double init_ms = 0, run_ms = 0, finalize_ms = 0, device_ms = 0;
+};
+
+void run_batch(qiotk_engine e, const char* target, int32_t n, const qiotk_model* models,
+               const char* const* params_json, BatchRun& br) {
+  qmc_batch b = nullptr;
+  device_check(qmc_batch_create(e->ctx, &b), "qmc_batch_create");
+  std::unique_ptr<qmc_batch_s, void (*)(qmc_batch)> hold(b, qmc_batch_destroy);
+  br.fused.assign(n, 0);
+  const double t0 = now_ms();
+  for (int32_t k = 0; k < n; ++k) {
+    auto s = create_solver(target);
+    s->set_model(&models[k]->model, models[k]->dev.get(), e->ctx);
+    s->configure(Json::parse(params_json[k]));
+    s->init();
+    br.fused[k] = s->batch_plan(b) ? 1 : 0;
+    br.solvers.push_back(std::move(s));
+  }
+  const double t1 = now_ms();
+  // every member finishes within its planned sweeps: one persistent launch
+  // runs them all, or graph-replayed launches of the smallest requested
+  // sweeps_per_launch
+  int64_t spl = 0;
+  for (int32_t k = 0; k < n; ++k)
+    if (br.fused[k] && br.solvers[k]->sweeps_per_launch() > 0)
+      spl = spl ? std::min(spl, br.solvers[k]->sweeps_per_launch()) : br.solvers[k]->sweeps_per_launch();
+  if (spl > 0)
+    device_check(qmc_batch_run_chunked(b, (int32_t)std::min<int64_t>(spl, 1 << 30)), "qmc_batch_run_chunked");
+  else
+    device_check(qmc_batch_run(b, 1 << 30), "qmc_batch_run");
+  double ms = 0;
+  device_check(qmc_batch_wait(b, &ms), "qmc_batch_wait");
+  const double t2 = now_ms();
+  int64_t tot_r = 0, tot_s = 0;
+  for (int32_t k = 0; k < n; ++k) {
+    const int64_t r = br.solvers[k]->batch_replicas();
+    tot_r += r;
+    tot_s += r * models[k]->model.n_vars;
+  }
+  std::vector<double> low(tot_r);
+  std::vector<uint64_t> acc(tot_r);
+  std::vector<uint8_t> st(tot_s);
+  if (tot_r) device_check(qmc_batch_read(b, low.data(), acc.data(), st.data()), "qmc_batch_read");
+  int64_t ro = 0, so = 0;
+  for (int32_t k = 0; k < n; ++k) {
+    Solver& s = *br.solvers[k];
+    const int64_t r = s.batch_replicas();
+    if (br.fused[k] && r) {
+      s.finalize_from(low.data() + ro, acc.data() + ro, st.data() + so);
+      ro += r;
+      so += r * models[k]->model.n_vars;
+    } else if (!br.fused[k]) {
+      s.run();
+      s.finalize();
+    }
+  }
+  const double t3 = now_ms();
+  br.init_ms = t1 - t0;
+  br.run_ms = t2 - t1;
+  br.finalize_ms = t3 - t2;
+  br.device_ms = ms;
 }
 
 }  // namespace
@@ -171,43 +241,162 @@ int qiotk_solve_batch(qiotk_engine e, const char* target, int32_t n, const qiotk
                       const char* const* params_json, char** responses, double* device_ms) {
   for (int32_t k = 0; k < n; ++k) responses[k] = nullptr;
   return guarded(nullptr, [&] {
-    std::vector<std::unique_ptr<Solver>> solvers;
-    qmc_batch b = nullptr;
-    device_check(qmc_batch_create(e->ctx, &b), "qmc_batch_create");
-    std::unique_ptr<qmc_batch_s, void (*)(qmc_batch)> hold(b, qmc_batch_destroy);
-    std::vector<int> fused(n, 0);
-    const double t0 = now_ms();
+    BatchRun br;
+    run_batch(e, target, n, models, params_json, br);
     for (int32_t k = 0; k < n; ++k) {
-      auto s = create_solver(target);
-      s->set_model(&models[k]->model, models[k]->dev.get(), e->ctx);
-      s->configure(Json::parse(params_json[k]));
-      s->init();
-      fused[k] = s->batch_plan(b) ? 1 : 0;
-      solvers.push_back(std::move(s));
-    }
-    const double t1 = now_ms();
-    double total_ms = 0;
-    // every member finishes within its planned sweeps; one launch runs them all
-    device_check(qmc_batch_run(b, 1 << 30), "qmc_batch_run");
-    double ms = 0;
-    device_check(qmc_batch_wait(b, &ms), "qmc_batch_wait");
-    total_ms += ms;
-    const double t2 = now_ms();
-    for (int32_t k = 0; k < n; ++k) {
-      if (!fused[k]) solvers[k]->run();
-      solvers[k]->finalize();
-    }
-    const double t3 = now_ms();
-    for (int32_t k = 0; k < n; ++k) {
-      Json r = solvers[k]->get_result();
-      r["benchmark"]["fused"] = Json((bool)fused[k]);
-      r["benchmark"]["batch_init_ms"] = t1 - t0;
-      r["benchmark"]["batch_run_ms"] = t2 - t1;
-      r["benchmark"]["batch_finalize_ms"] = t3 - t2;
-      r["benchmark"]["batch_device_ms"] = ms;
+      Json r = br.solvers[k]->get_result();
+      r["benchmark"]["fused"] = Json((bool)br.fused[k]);
+      r["benchmark"]["batch_init_ms"] = br.init_ms;
+      r["benchmark"]["batch_run_ms"] = br.run_ms;
+      r["benchmark"]["batch_finalize_ms"] = br.finalize_ms;
+      r["benchmark"]["batch_device_ms"] = br.device_ms;
       responses[k] = dup(r.dump());
     }
-    if (device_ms) *device_ms = total_ms;
+    if (device_ms) *device_ms = br.device_ms;
+    return 0;
+  });
+}
+
+int qiotk_solve_batch_arrays(qiotk_engine e, const char* target, int32_t n, const qiotk_model* models,
+                             const char* const* params_json, double* cost, int8_t* configs,
+                             double* timings) {
+  return guarded(nullptr, [&] {
+    BatchRun br;
+    run_batch(e, target, n, models, params_json, br);
+    int64_t off = 0;
+    for (int32_t k = 0; k < n; ++k) {
+      const int32_t nv = models[k]->model.n_vars;
+      br.solvers[k]->best(cost ? cost + k : nullptr, configs ? configs + off : nullptr);
+      off += nv;
+    }
+    if (timings) {
+      timings[0] = br.init_ms;
+      timings[1] = br.run_ms;
+      timings[2] = br.finalize_ms;
+      timings[3] = br.device_ms;
+    }
+    return 0;
+  });
+}
+
+int qiotk_tune_batch(qiotk_engine e, const char* target, int32_t n, const qiotk_model* models,
+                     const char* const* params_json, int32_t trial_sweeps, char** response) {
+  return guarded(response, [&] {
+    if (n <= 0) throw Error(kValueError, "nothing to tune");
+    if (trial_sweeps < 2) throw Error(kValueError, "trial_sweeps must be >= 2");
+    // sample: every k-th solve, at most 512
+    const int32_t stride = std::max<int32_t>(1, (n + 511) / 512);
+    std::vector<int32_t> idx;
+    for (int32_t k = 0; k < n; k += stride) idx.push_back(k);
+    std::vector<qiotk_model> ms;
+    std::vector<Json> base;
+    int64_t min_group = 32;  // replicas of the smallest run (SA restarts, PT ladder)
+    const bool pt = std::string(target) == "paralleltempering.qiotoolkit";
+    for (int32_t k : idx) {
+      ms.push_back(models[k]);
+      Json doc = Json::parse(params_json[k]);
+      if (!doc.has("params")) throw Error(kMissingInput, "Input field `params` is required.");
+      base.push_back(doc);
+      auto s = create_solver(target);
+      s->set_model(&models[k]->model, models[k]->dev.get(), e->ctx);
+      s->configure(doc);
+      min_group = std::min<int64_t>(min_group, s->run_replicas());
+    }
+    // candidates: replicas per work-group x lane depth (0 = library default)
+    std::vector<std::pair<int, int>> cands;
+    for (int B : {1, 2, 4, 8, 16, 32}) {
+      if (B > min_group) break;
+      if (pt && min_group % B) continue;
+      for (int depth : {0, 2, 8}) cands.emplace_back(B, depth);
+    }
+    Json trials = Json::array();
+    double best_rate = -1;
+    int best_B = 0, best_depth = 0;
+    for (auto [B, depth] : cands) {
+      std::vector<std::string> ps;
+      for (Json doc : base) {
+        Json& p = doc["params"];
+        p["sweeps"] = Json((int64_t)trial_sweeps);
+        p.remove("step_limit");
+        p["replicas_per_group"] = Json((int64_t)B);
+        p["positions_per_lane"] = Json((int64_t)depth);
+        ps.push_back(doc.dump());
+      }
+      std::vector<const char*> pp;
+      for (auto& x : ps) pp.push_back(x.c_str());
+      BatchRun br;
+      run_batch(e, target, (int32_t)ms.size(), ms.data(), pp.data(), br);
+      int64_t att = 0;
+      for (auto& s : br.solvers) att += s->attempts();
+      const double rate = br.device_ms > 0 ? (double)att / (br.device_ms * 1e-3) : 0.0;
+      Json t = Json::object();
+      t["replicas_per_group"] = Json((int64_t)B);
+      t["positions_per_lane"] = Json((int64_t)depth);
+      t["device_ms"] = br.device_ms;
+      t["attempts_per_s"] = rate;
+      trials.push_back(t);
+      if (rate > best_rate) {
+        best_rate = rate;
+        best_B = B;
+        best_depth = depth;
+      }
+    }
+    // sweeps per launch: one persistent launch (0) against graph-replayed
+    // chunks, for the chosen shape (split PT ladders always run persistent)
+    int64_t best_spl = 0;
+    if (!pt || best_B == min_group) {
+      for (int64_t spl : {(int64_t)std::max(1, trial_sweeps / 4), (int64_t)std::max(1, trial_sweeps / 2)}) {
+        std::vector<std::string> ps;
+        for (Json doc : base) {
+          Json& p = doc["params"];
+          p["sweeps"] = Json((int64_t)trial_sweeps);
+          p.remove("step_limit");
+          p["replicas_per_group"] = Json((int64_t)best_B);
+          p["positions_per_lane"] = Json((int64_t)best_depth);
+          p["sweeps_per_launch"] = Json(spl);
+          ps.push_back(doc.dump());
+        }
+        std::vector<const char*> pp;
+        for (auto& x : ps) pp.push_back(x.c_str());
+        BatchRun br;
+        run_batch(e, target, (int32_t)ms.size(), ms.data(), pp.data(), br);
+        int64_t att = 0;
+        for (auto& s : br.solvers) att += s->attempts();
+        const double rate = br.device_ms > 0 ? (double)att / (br.device_ms * 1e-3) : 0.0;
+        Json t = Json::object();
+        t["replicas_per_group"] = Json((int64_t)best_B);
+        t["positions_per_lane"] = Json((int64_t)best_depth);
+        t["sweeps_per_launch"] = Json(spl);
+        t["device_ms"] = br.device_ms;
+        t["attempts_per_s"] = rate;
+        trials.push_back(t);
+        if (rate > best_rate) {
+          best_rate = rate;
+          best_spl = spl;
+        }
+      }
+    }
+    Json r = Json::object();
+    Json chosen = Json::object();
+    chosen["replicas_per_group"] = Json((int64_t)best_B);
+    chosen["positions_per_lane"] = Json((int64_t)best_depth);
+    // a chunk length chosen on a short trial scales with the full run
+    chosen["sweeps_per_launch"] = Json(best_spl);
+    chosen["sweeps_per_launch_fraction"] = best_spl ? (double)best_spl / trial_sweeps : 0.0;
+    r["params"] = chosen;
+    r["sample"] = Json((int64_t)idx.size());
+    r["trial_sweeps"] = Json((int64_t)trial_sweeps);
+    r["trials"] = trials;
+    *response = dup(r.dump());
+    return 0;
+  });
+}
+
+int qiotk_model_vars(qiotk_model m, int32_t* n_vars, int32_t* names) {
+  return guarded(nullptr, [&] {
+    if (!m) throw Error(kValueError, "null model");
+    if (n_vars) *n_vars = m->model.n_vars;
+    if (names) std::memcpy(names, m->model.names.data(), m->model.names.size() * sizeof(int32_t));
     return 0;
   });
 }

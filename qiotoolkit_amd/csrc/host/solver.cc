@@ -45,6 +45,14 @@ void Solver::configure(const Json& doc) {
   // Solver::configure (solver.h:156-218)
   if (p.has("threads")) threads_ = (int)p.at("threads").as_int();
   if (p.has("lanes")) lanes_ = (int)p.at("lanes").as_int();
+  if (p.has("sweeps_per_launch")) {
+    sweeps_per_launch_ = p.at("sweeps_per_launch").as_int();
+    if (sweeps_per_launch_ < 0) throw Error(kValueError, "`sweeps_per_launch` must be >= 0");
+  }
+  if (p.has("positions_per_lane")) {
+    positions_per_lane_ = (int)p.at("positions_per_lane").as_int();
+    if (positions_per_lane_ < 0) throw Error(kValueError, "`positions_per_lane` must be >= 0");
+  }
   if (p.has("time_limit")) time_limit_ = p.at("time_limit").as_double();
   else if (p.has("timeout")) time_limit_ = p.at("timeout").as_double();
   if (p.has("cost_limit")) cost_limit_ = p.at("cost_limit").as_double();
@@ -113,6 +121,47 @@ void Solver::populate(const std::vector<double>& lowest, const std::vector<uint8
     lowest_states_.emplace_back(states.begin() + idx[i] * N, states.begin() + (idx[i] + 1) * N);
   }
   if (k > 0) record(lowest_costs_[0], lowest_states_[0]);
+}
+
+int Solver::lanes_for(int replicas_per_group) const {
+  // explicit lane count, else the smallest power-of-two multiple of 64 that
+  // covers the largest colour class in `positions_per_lane` rounds
+  if (lanes_) return lanes_;
+  if (!positions_per_lane_ || !dmodel_) return 0;
+  const int G = (replicas_per_group + 7) / 8;
+  const int cap = std::max(64, 512 / G / 64 * 64);
+  int l = 64;
+  while (l < cap && (int64_t)l * positions_per_lane_ < dmodel_->info().max_class) l *= 2;
+  return std::min(l, cap);
+}
+
+void Solver::apply_lanes(qmc_walkers w, int replicas_per_group) const {
+  const int l = lanes_for(replicas_per_group);
+  if (l) device_check(qmc_walkers_set_lanes(w, l), "qmc_walkers_set_lanes");
+}
+
+void Solver::finalize_from(const double* lowest_energy, const uint64_t* accepted, const uint8_t* states) {
+  const int64_t R = batched_replicas_;
+  if (R == 0) return;
+  const int64_t N = model_->n_vars;
+  std::vector<double> low(lowest_energy, lowest_energy + R);
+  std::vector<uint8_t> st(states, states + R * N);
+  accepted_ = (int64_t)std::accumulate(accepted, accepted + R, (uint64_t)0);
+  populate(low, st, R);
+}
+
+bool Solver::best(double* cost, int8_t* config) const {
+  if (!have_lowest_ || model_->empty()) {
+    if (cost) *cost = model_->const_cost;
+    return false;
+  }
+  if (cost) *cost = lowest_cost_ + model_->const_cost;
+  if (config) {
+    const bool ising = model_->kind == QMC_ISING;
+    for (int32_t i = 0; i < model_->n_vars; ++i)
+      config[i] = (int8_t)(ising ? (lowest_state_[i] ? -1 : 1) : (lowest_state_[i] ? 0 : 1));
+  }
+  return true;
 }
 
 Json Solver::get_solutions() const {
@@ -225,7 +274,7 @@ void SimulatedAnnealing::init() {
   const int B = group_ > 0 ? group_ : default_group(restarts_);
   device_check(qmc_walkers_create(dmodel_->handle(), restarts_, B, seed_, 0, 0, restarts_, &w_),
                "qmc_walkers_create");
-  if (lanes_) device_check(qmc_walkers_set_lanes(w_, lanes_), "qmc_walkers_set_lanes");
+  apply_lanes(w_, B);
   if (model_->has_initial) {
     std::vector<uint8_t> st((size_t)restarts_ * model_->n_vars, 0);
     for (int r = 0; r < restarts_; ++r)
@@ -297,6 +346,7 @@ bool SimulatedAnnealing::batch_plan(qmc_batch b) {
   const int64_t steps = steps_to_run();
   device_check(qmc_walkers_plan(w_, 0, (int32_t)steps, betas_.data()), "qmc_walkers_plan");
   device_check(qmc_batch_add(b, w_), "qmc_batch_add");
+  batched_replicas_ = restarts_;
   steps_done_ = steps;
   attempts_ = steps * model_->n_vars * restarts_;
   return true;
@@ -375,7 +425,7 @@ void ParallelTempering::init() {
   int B = group_ > 0 ? group_ : ((L >= 4 && L % 2 == 0) ? 2 : L);
   if (L % B != 0) B = L;
   device_check(qmc_walkers_create(dmodel_->handle(), L, B, seed_, 0, 0, L, &w_), "qmc_walkers_create");
-  if (lanes_) device_check(qmc_walkers_set_lanes(w_, lanes_), "qmc_walkers_set_lanes");
+  apply_lanes(w_, B);
   if (model_->has_initial) {
     std::vector<uint8_t> st((size_t)L * model_->n_vars, 0);
     for (int r = 0; r < L; ++r)
@@ -430,6 +480,7 @@ bool ParallelTempering::batch_plan(qmc_batch b) {
   if (qmc_walkers_plan_pt(w_, L, betas_.data(), 0, (int32_t)steps, sweeps_per_replica_) != QMC_OK)
     return false;  // ladder split over work-groups: runs on its own
   device_check(qmc_batch_add(b, w_), "qmc_batch_add");
+  batched_replicas_ = L;
   steps_done_ = steps;
   attempts_ = steps * sweeps_per_replica_ * model_->n_vars * L;
   return true;

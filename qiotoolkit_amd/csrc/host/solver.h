@@ -39,6 +39,21 @@ class Solver {
   // Fused launches with other solves: plan the whole run on the device and
   // add the walkers to `b`; false when this solve needs host-side steps.
   virtual bool batch_plan(qmc_batch) { return false; }
+  // Replicas this solve added to a batch; their results come back from
+  // qmc_batch_read in the order the solves were added (0: none added).
+  int64_t batch_replicas() const { return batched_replicas_; }
+  // Replicas of one independent run as configured (SA restarts, PT ladder
+  // length, PA population); known after configure().
+  virtual int64_t run_replicas() const { return 1; }
+  // Sweeps per device launch when fused into a batch (0: one persistent launch).
+  int64_t sweeps_per_launch() const { return sweeps_per_launch_; }
+  // finalize() from this solve's slices of a batched read-back: FP64 energy
+  // of each replica's lowest state, accepted counts, lowest states.
+  void finalize_from(const double* lowest_energy, const uint64_t* accepted, const uint8_t* states);
+  // Best cost (constant terms included) and its configuration in the
+  // reference's rendering (Ising +1/-1, PUBO 1/0) by internal variable id;
+  // false for an empty model.
+  bool best(double* cost, int8_t* config) const;
   // Resolved run plan (steps, betas), computable without a device.
   virtual Json describe() const;
   Json get_result() const;
@@ -72,6 +87,10 @@ class Solver {
   int number_of_solutions_ = 1;
   int threads_ = 0;
   int lanes_ = 0;  // positions per class round on the device (0: automatic)
+  int positions_per_lane_ = 0;  // else lanes from the largest class (0: library default)
+  int64_t sweeps_per_launch_ = 0;
+  int lanes_for(int replicas_per_group) const;
+  void apply_lanes(qmc_walkers w, int replicas_per_group) const;
   int64_t steps_done_ = 0;
   int64_t attempts_ = 0, accepted_ = 0;
   Timing timing_;
@@ -82,6 +101,7 @@ class Solver {
   std::vector<std::vector<uint8_t>> lowest_states_;
   Json out_params_ = Json::object();
   std::string exit_reason_;
+  int64_t batched_replicas_ = 0;
 };
 
 // simulated_annealing.h
@@ -95,6 +115,7 @@ class SimulatedAnnealing : public Solver {
   bool batch_plan(qmc_batch b) override;
   Json describe() const override;
   int restarts() const { return restarts_; }
+  int64_t run_replicas() const override { return restarts_; }
 
  private:
   bool use_inverse_temperature_ = false;
@@ -123,6 +144,7 @@ class ParallelTempering : public Solver {
   Json describe() const override;
   ~ParallelTempering() override;
   std::vector<double> ladder() const;
+  int64_t run_replicas() const override { return (int64_t)ladder().size(); }
 
  private:
   bool use_inverse_temperatures_ = true;

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -94,6 +95,8 @@ struct DevBuf {
     if (count == 0) return hipSuccess;
     return DevicePool::get().alloc((void**)&p, count * sizeof(T));
   }
+  // at least `count` elements, keeping the current buffer when it is large enough
+  hipError_t reserve(size_t count) { return count <= n && p ? hipSuccess : alloc(count); }
   hipError_t upload(const T* src, size_t count) {
     hipError_t e = alloc(count);
     if (e != hipSuccess || count == 0) return e;
@@ -103,11 +106,21 @@ struct DevBuf {
 
 }  // namespace
 
+constexpr int kAuxStreams = 3;
+
+// Handles are reference counted: a problem keeps its context alive, a
+// walker set its problem, a batch its context and members.  *_destroy drops
+// the caller's reference, so handles may be destroyed in any order.
 struct qmc_ctx_s {
+  std::atomic<int> refs{1};
   int device = 0;
   hipStream_t stream = nullptr;
   DevBuf<double> lut;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  // batches fan their launch groups out over the main stream and these
+  // (created on first use, default priority), joined back by events
+  hipStream_t aux[kAuxStreams] = {};
+  hipEvent_t fork_ev = nullptr, join_ev[kAuxStreams] = {};
 };
 
 struct EllLayout {
@@ -116,6 +129,7 @@ struct EllLayout {
 };
 
 struct qmc_problem_s {
+  std::atomic<int> refs{1};
   qmc_ctx ctx = nullptr;
   std::map<int, std::unique_ptr<EllLayout>> ell;  // per lane count (pairwise problems)
   std::vector<int32_t> h_inc_off, h_nbr;          // host copies to build layouts
@@ -160,6 +174,7 @@ struct qmc_problem_s {
 };
 
 struct qmc_walkers_s {
+  std::atomic<int> refs{1};
   qmc_problem prob = nullptr;
   int32_t R = 0, B = 1, G = 1, n_units = 0, stride = 0;
   int32_t cap = 0, cap_units = 0;  // allocated replicas / work-groups
@@ -187,6 +202,7 @@ struct qmc_walkers_s {
   int ladder = 0;
   int spr = 1;
   int64_t sched_base = 0;
+  int64_t planned = 0;       // sweeps of the last plan
   uint64_t version = 0;      // bumped whenever `units` changes
   float last_ms = 0.f;
 };
@@ -201,11 +217,27 @@ struct qmc_batch_s {
     int kind, pairwise, coupling, nb, coop;
     int block = 64;
     int stride = 0;
+    double work = 0;  // attempts planned (for spreading groups over streams)
     std::vector<UnitDev> units;
     DevBuf<UnitDev> d_units;
   };
   std::vector<std::unique_ptr<Group>> groups;
   float last_ms = 0.f;
+  // chunked runs: launches of `graph_spl` sweeps captured once and replayed
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  int64_t graph_key[3] = {0, 0, 0};  // sweeps per launch, launches, rebuild count
+  int64_t rebuilds = 0;
+  ~qmc_batch_s() {
+    if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+    if (graph) (void)hipGraphDestroy(graph);
+  }
+  // batched read-back
+  DevBuf<ReadDesc> d_desc;
+  DevBuf<int32_t> d_rep_member;
+  DevBuf<uint8_t> d_states;
+  DevBuf<double> d_energy;
+  DevBuf<unsigned long long> d_acc;
 };
 
 extern "C" {
@@ -237,16 +269,24 @@ int qmc_ctx_create(int device, qmc_ctx* out) {
   return QMC_OK;
 }
 
-void qmc_ctx_destroy(qmc_ctx c) {
-  if (!c) return;
+static void ctx_release(qmc_ctx c) {
+  if (!c || --c->refs > 0) return;
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   c->lut.release();
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
+  if (c->fork_ev) (void)hipEventDestroy(c->fork_ev);
+  for (int i = 0; i < kAuxStreams; ++i) {
+    if (c->aux[i]) (void)hipStreamSynchronize(c->aux[i]);
+    if (c->join_ev[i]) (void)hipEventDestroy(c->join_ev[i]);
+    if (c->aux[i]) (void)hipStreamDestroy(c->aux[i]);
+  }
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
+
+void qmc_ctx_destroy(qmc_ctx c) { ctx_release(c); }
 
 int qmc_sync(qmc_ctx c) {
   if (!c) return fail(QMC_INVALID_ARGUMENT, "null context");
@@ -385,11 +425,19 @@ int qmc_problem_create(qmc_ctx ctx, int kind, int32_t n_vars, int64_t n_terms,
   HIP_TRY(p->d_term_off.upload(toff.data(), toff.size()));
   HIP_TRY(p->d_term_var.upload(tvar.data(), tvar.size()));
   HIP_TRY(p->d_term_c.upload(tc.data(), tc.size()));
+  ctx->refs++;
   *out = p.release();
   return QMC_OK;
 }
 
-void qmc_problem_destroy(qmc_problem p) { delete p; }
+static void problem_release(qmc_problem p) {
+  if (!p || --p->refs > 0) return;
+  qmc_ctx c = p->ctx;
+  delete p;
+  ctx_release(c);
+}
+
+void qmc_problem_destroy(qmc_problem p) { problem_release(p); }
 
 int qmc_problem_get_info(qmc_problem p, qmc_problem_info* info) {
   if (!p || !info) return fail(QMC_INVALID_ARGUMENT, "null handle");
@@ -402,6 +450,10 @@ int qmc_problem_get_info(qmc_problem p, qmc_problem_info* info) {
   info->pairwise = p->pairwise;
   info->max_degree = p->max_degree;
   info->mean_degree = (double)p->n_inc / (double)p->N;
+  info->max_class = 0;
+  for (size_t c = 0; c + 1 < p->class_off.size(); ++c)
+    info->max_class = std::max(info->max_class, p->class_off[c + 1] - p->class_off[c]);
+  info->pad_ = 0;
   return QMC_OK;
 }
 
@@ -564,6 +616,7 @@ static int set_progress(qmc_walkers w, int64_t sweep0, int64_t n) {
     pr[2 * u + 1] = n;
   }
   HIP_TRY(hipMemcpy(w->progress.p, pr.data(), pr.size() * sizeof(int64_t), hipMemcpyHostToDevice));
+  w->planned = n;
   return QMC_OK;
 }
 
@@ -617,6 +670,7 @@ int qmc_walkers_create(qmc_problem p, int32_t n_replicas, int32_t replicas_per_g
   HIP_TRY(hipMemset(w->progress.p, 0, w->progress.n * sizeof(int64_t)));
   int rc = upload_units(w.get());
   if (rc) return rc;
+  p->refs++;
   *out = w.release();
   return QMC_OK;
 }
@@ -628,11 +682,16 @@ static void drop_graph(qmc_walkers w) {
   w->graph = nullptr;
 }
 
-void qmc_walkers_destroy(qmc_walkers w) {
-  if (w && w->prob) (void)hipStreamSynchronize(w->prob->ctx->stream);
-  if (w) drop_graph(w);
+static void walkers_release(qmc_walkers w) {
+  if (!w || --w->refs > 0) return;
+  qmc_problem p = w->prob;
+  (void)hipStreamSynchronize(p->ctx->stream);
+  drop_graph(w);
   delete w;
+  problem_release(p);
 }
+
+void qmc_walkers_destroy(qmc_walkers w) { walkers_release(w); }
 
 static int ensure_scratch(qmc_walkers w) {
   const size_t need = (size_t)w->cap * w->prob->N;
@@ -917,13 +976,18 @@ int qmc_batch_create(qmc_ctx ctx, qmc_batch* out) {
   if (!ctx || !out) return fail(QMC_INVALID_ARGUMENT, "null handle");
   auto* b = new qmc_batch_s();
   b->ctx = ctx;
+  ctx->refs++;
   *out = b;
   return QMC_OK;
 }
 
 void qmc_batch_destroy(qmc_batch b) {
-  if (b && b->ctx) (void)hipStreamSynchronize(b->ctx->stream);
+  if (!b) return;
+  qmc_ctx c = b->ctx;
+  (void)hipStreamSynchronize(c->stream);
+  for (qmc_walkers w : b->members) walkers_release(w);
   delete b;
+  ctx_release(c);
 }
 
 int qmc_batch_add(qmc_batch b, qmc_walkers w) {
@@ -932,6 +996,7 @@ int qmc_batch_add(qmc_batch b, qmc_walkers w) {
   for (auto* m : b->members)
     if (m == w) return QMC_OK;
   b->members.push_back(w);
+  w->refs++;
   b->seen_version.push_back(~0ull);
   b->dirty = true;
   return QMC_OK;
@@ -944,6 +1009,7 @@ int qmc_batch_remove(qmc_batch b, qmc_walkers w) {
       b->members.erase(b->members.begin() + i);
       b->seen_version.erase(b->seen_version.begin() + i);
       b->dirty = true;
+      walkers_release(w);
       return QMC_OK;
     }
   return QMC_OK;
@@ -951,6 +1017,7 @@ int qmc_batch_remove(qmc_batch b, qmc_walkers w) {
 
 int qmc_batch_clear(qmc_batch b) {
   if (!b) return fail(QMC_INVALID_ARGUMENT, "null handle");
+  for (qmc_walkers w : b->members) walkers_release(w);
   b->members.clear();
   b->seen_version.clear();
   b->dirty = true;
@@ -962,7 +1029,10 @@ static int batch_rebuild(qmc_batch b) {
   for (size_t i = 0; i < b->members.size(); ++i)
     if (b->seen_version[i] != b->members[i]->version) stale = true;
   if (!stale) return QMC_OK;
+  // launches still running may read the descriptors released below
+  HIP_TRY(hipStreamSynchronize(b->ctx->stream));
   b->groups.clear();
+  b->rebuilds++;
   // Largest problems first, so the longest work-groups start earliest.
   std::vector<qmc_walkers> order(b->members.begin(), b->members.end());
   std::stable_sort(order.begin(), order.end(), [](qmc_walkers x, qmc_walkers y) {
@@ -970,10 +1040,13 @@ static int batch_rebuild(qmc_batch b) {
   });
   for (qmc_walkers w : order) {
     qmc_problem p = w->prob;
+    // one group per kernel instantiation and work-group size, so no launch
+    // carries idle threads for its smaller members
+    const int block = w->lanes * w->G;
     qmc_batch_s::Group* g = nullptr;
     for (auto& gg : b->groups)
       if (gg->kind == p->kind && gg->pairwise == p->pairwise && gg->coupling == p->coupling &&
-          gg->nb == w->nb && gg->coop == w->coop)
+          gg->nb == w->nb && gg->coop == w->coop && gg->block == block)
         g = gg.get();
     if (!g) {
       b->groups.push_back(std::make_unique<qmc_batch_s::Group>());
@@ -983,9 +1056,10 @@ static int batch_rebuild(qmc_batch b) {
       g->coupling = p->coupling;
       g->nb = w->nb;
       g->coop = w->coop;
+      g->block = block;
     }
-    g->block = std::max(g->block, w->lanes * w->G);
     g->stride = std::max(g->stride, w->stride);
+    g->work += (double)p->N * w->R * (double)w->planned;
     g->units.insert(g->units.end(), w->units.begin(), w->units.end());
   }
   for (auto& g : b->groups) {
@@ -996,43 +1070,133 @@ static int batch_rebuild(qmc_batch b) {
   return QMC_OK;
 }
 
+static int ensure_aux_streams(qmc_ctx c) {
+  if (c->fork_ev) return QMC_OK;
+  for (int i = 0; i < kAuxStreams; ++i) {
+    HIP_TRY(hipStreamCreateWithFlags(&c->aux[i], hipStreamNonBlocking));
+    HIP_TRY(hipEventCreateWithFlags(&c->join_ev[i], hipEventDisableTiming));
+  }
+  HIP_TRY(hipEventCreateWithFlags(&c->fork_ev, hipEventDisableTiming));
+  return QMC_OK;
+}
+
+static int launch_group(qmc_batch_s::Group& g, int32_t max_sweeps, const double* lut, hipStream_t st) {
+  SweepArgs a{};
+  a.n_sweeps = max_sweeps;
+  a.lut = lut;
+  a.unit_of_block = nullptr;
+  const size_t smem = sweep_smem_bytes(g.stride, 0);
+  if (smem > 160 * 1024) return fail(QMC_MEMORY_LIMITED, "batch LDS footprint exceeds 160 KiB");
+  const int n = (int)g.units.size();
+  if (!g.coop) {
+    HIP_TRY(launch_sweep(g.kind, g.pairwise, g.coupling, g.nb, g.d_units.p, n, g.block, smem, a, 0, 0, st));
+    return QMC_OK;
+  }
+  // Cooperative ladders: launches of at most the co-resident number of
+  // work-groups, never splitting a ladder (its work-groups are adjacent).
+  const int cap = max_coresident_blocks(g.kind, g.pairwise, g.coupling, g.nb, g.block, smem);
+  int start = 0;
+  while (start < n) {
+    int end = start;
+    while (end < n) {
+      const int upl = g.units[end].upl;
+      if (end + upl - start > cap) break;
+      end += upl;
+    }
+    if (end == start) return fail(QMC_MEMORY_LIMITED, "a ladder exceeds the co-resident work-group limit");
+    HIP_TRY(launch_sweep(g.kind, g.pairwise, g.coupling, g.nb, g.d_units.p + start, end - start, g.block,
+                         smem, a, 0, 1, st));
+    start = end;
+  }
+  return QMC_OK;
+}
+
+// Enqueue one launch per group (max_sweeps sweeps per unit) on the context's
+// streams; everything joins back into the main stream.
+static int enqueue_groups(qmc_batch b, int32_t max_sweeps) {
+  int rc;
+  qmc_ctx c = b->ctx;
+  std::vector<qmc_batch_s::Group*> gs;
+  for (auto& g : b->groups)
+    if (!g->units.empty()) gs.push_back(g.get());
+  // Groups of different kernels or work-group sizes run concurrently on up
+  // to 1 + kAuxStreams streams: largest first, each onto the least loaded
+  // stream.  Cooperative groups stay on the main stream, alone, so their
+  // co-residency is not shared with other kernels.
+  bool coop_any = false;
+  for (auto* g : gs) coop_any |= g->coop != 0;
+  const int n_streams = (gs.size() > 1 && !coop_any) ? 1 + kAuxStreams : 1;
+  if (n_streams > 1) {
+    HIP_TRY(hipEventRecord(c->fork_ev, c->stream));
+    for (int i = 0; i < kAuxStreams; ++i) HIP_TRY(hipStreamWaitEvent(c->aux[i], c->fork_ev, 0));
+  }
+  std::stable_sort(gs.begin(), gs.end(), [](auto* x, auto* y) { return x->work > y->work; });
+  std::vector<double> load(n_streams, 0.0);
+  for (auto* g : gs) {
+    const int s = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+    load[s] += g->work;
+    if ((rc = launch_group(*g, max_sweeps, c->lut.p, s == 0 ? c->stream : c->aux[s - 1]))) return rc;
+  }
+  if (n_streams > 1)
+    for (int i = 0; i < kAuxStreams; ++i) {
+      HIP_TRY(hipEventRecord(c->join_ev[i], c->aux[i]));
+      HIP_TRY(hipStreamWaitEvent(c->stream, c->join_ev[i], 0));
+    }
+  return QMC_OK;
+}
+
 int qmc_batch_run(qmc_batch b, int32_t max_sweeps) {
   if (!b) return fail(QMC_INVALID_ARGUMENT, "null handle");
   int rc = batch_rebuild(b);
   if (rc) return rc;
   qmc_ctx c = b->ctx;
+  if ((rc = ensure_aux_streams(c))) return rc;
   HIP_TRY(hipEventRecord(c->ev0, c->stream));
-  for (auto& g : b->groups) {
-    if (g->units.empty()) continue;
-    SweepArgs a{};
-    a.n_sweeps = max_sweeps;
-    a.lut = c->lut.p;
-    a.unit_of_block = nullptr;
-    const size_t smem = sweep_smem_bytes(g->stride, 0);
-    if (smem > 160 * 1024) return fail(QMC_MEMORY_LIMITED, "batch LDS footprint exceeds 160 KiB");
-    const int n = (int)g->units.size();
-    if (!g->coop) {
-      HIP_TRY(launch_sweep(g->kind, g->pairwise, g->coupling, g->nb, g->d_units.p, n, g->block, smem, a, 0, 0,
-                           c->stream));
-      continue;
-    }
-    // Cooperative ladders: launches of at most the co-resident number of
-    // work-groups, never splitting a ladder (its work-groups are adjacent).
-    const int cap = max_coresident_blocks(g->kind, g->pairwise, g->coupling, g->nb, g->block, smem);
-    int start = 0;
-    while (start < n) {
-      int end = start;
-      while (end < n) {
-        const int upl = g->units[end].upl;
-        if (end + upl - start > cap) break;
-        end += upl;
-      }
-      if (end == start) return fail(QMC_MEMORY_LIMITED, "a ladder exceeds the co-resident work-group limit");
-      HIP_TRY(launch_sweep(g->kind, g->pairwise, g->coupling, g->nb, g->d_units.p + start, end - start,
-                           g->block, smem, a, 0, 1, c->stream));
-      start = end;
-    }
+  if ((rc = enqueue_groups(b, max_sweeps))) return rc;
+  HIP_TRY(hipEventRecord(c->ev1, c->stream));
+  return QMC_OK;
+}
+
+static void drop_batch_graph(qmc_batch b) {
+  if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->graph_exec = nullptr;
+  b->graph = nullptr;
+}
+
+int qmc_batch_run_chunked(qmc_batch b, int32_t sweeps_per_launch) {
+  if (!b || sweeps_per_launch < 1) return fail(QMC_INVALID_ARGUMENT, "bad argument");
+  int rc = batch_rebuild(b);
+  if (rc) return rc;
+  qmc_ctx c = b->ctx;
+  if ((rc = ensure_aux_streams(c))) return rc;
+  int64_t longest = 0;
+  for (qmc_walkers w : b->members) {
+    if (w->coop) return fail(QMC_INVALID_ARGUMENT, "chunked runs take no split PT ladders");
+    longest = std::max(longest, w->planned);
   }
+  const int64_t launches = (longest + sweeps_per_launch - 1) / sweeps_per_launch;
+  const int64_t key[3] = {sweeps_per_launch, launches, b->rebuilds};
+  if (!b->graph_exec || !std::equal(key, key + 3, b->graph_key)) {
+    drop_batch_graph(b);
+    // The kernels read each unit's progress on the device, so one recorded
+    // sequence of launches serves every replay without argument updates.
+    HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    for (int64_t i = 0; i < launches; ++i) {
+      rc = enqueue_groups(b, sweeps_per_launch);
+      if (rc) {
+        hipGraph_t g = nullptr;
+        (void)hipStreamEndCapture(c->stream, &g);
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+      }
+    }
+    HIP_TRY(hipStreamEndCapture(c->stream, &b->graph));
+    HIP_TRY(hipGraphInstantiate(&b->graph_exec, b->graph, nullptr, nullptr, 0));
+    std::copy(key, key + 3, b->graph_key);
+  }
+  HIP_TRY(hipEventRecord(c->ev0, c->stream));
+  HIP_TRY(hipGraphLaunch(b->graph_exec, c->stream));
   HIP_TRY(hipEventRecord(c->ev1, c->stream));
   return QMC_OK;
 }
@@ -1047,6 +1211,51 @@ int qmc_batch_wait(qmc_batch b, double* ms) {
       int rc = check_barrier_error(w);
       if (rc) return rc;
     }
+  return QMC_OK;
+}
+
+int qmc_batch_read(qmc_batch b, double* lowest_energy, uint64_t* accepted, uint8_t* lowest_states) {
+  if (!b) return fail(QMC_INVALID_ARGUMENT, "null handle");
+  qmc_ctx c = b->ctx;
+  std::vector<ReadDesc> desc;
+  std::vector<int32_t> rep_member;
+  desc.reserve(b->members.size());
+  int64_t roff = 0, soff = 0;
+  for (qmc_walkers w : b->members) {
+    qmc_problem p = w->prob;
+    ReadDesc d{};
+    d.units = w->d_units.p;
+    d.soff = soff;
+    d.roff = roff;
+    d.B = w->B;
+    d.R = w->R;
+    d.N = p->N;
+    d.kind = p->kind;
+    d.M = p->M;
+    d.term_off = p->d_term_off.p;
+    d.term_var = p->d_term_var.p;
+    d.term_c = p->d_term_c.p;
+    rep_member.insert(rep_member.end(), (size_t)w->R, (int32_t)desc.size());
+    desc.push_back(d);
+    roff += w->R;
+    soff += (int64_t)w->R * p->N;
+  }
+  if (roff == 0) return QMC_OK;
+  HIP_TRY(hipStreamSynchronize(c->stream));  // pooled buffers below may be reused
+  HIP_TRY(b->d_desc.upload(desc.data(), desc.size()));
+  HIP_TRY(b->d_rep_member.upload(rep_member.data(), rep_member.size()));
+  HIP_TRY(b->d_states.reserve((size_t)std::max<int64_t>(soff, 1)));
+  HIP_TRY(b->d_energy.reserve((size_t)roff));
+  HIP_TRY(b->d_acc.reserve((size_t)roff));
+  HIP_TRY(launch_batch_read(b->d_desc.p, b->d_rep_member.p, roff, b->d_states.p, b->d_energy.p, b->d_acc.p,
+                            c->stream));
+  if (lowest_energy)
+    HIP_TRY(hipMemcpyAsync(lowest_energy, b->d_energy.p, roff * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  if (accepted)
+    HIP_TRY(hipMemcpyAsync(accepted, b->d_acc.p, roff * sizeof(uint64_t), hipMemcpyDeviceToHost, c->stream));
+  if (lowest_states && soff)
+    HIP_TRY(hipMemcpyAsync(lowest_states, b->d_states.p, (size_t)soff, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
   return QMC_OK;
 }
 

@@ -105,6 +105,25 @@ hipError_t launch_energy(const ProblemDev& P, const uint8_t* states, int n, doub
                          hipStream_t st);
 hipError_t launch_delta_all(const ProblemDev& P, int n_states, const uint8_t* states, double* out,
                             hipStream_t st);
+
+// One walker set of a batch read-back: its replicas occupy global replica
+// slots roff .. roff+R-1 and its states bytes soff .. soff+R*N-1.
+struct ReadDesc {
+  const UnitDev* units;
+  int64_t soff;
+  int64_t roff;
+  int32_t B, R, N, kind;
+  int64_t M;
+  const int64_t* term_off;
+  const int32_t* term_var;
+  const double* term_c;
+};
+// Lowest states (plain layout), their FP64 energies and accepted counts of
+// every replica of every walker set; rep_member maps a global replica to its
+// walker set.
+hipError_t launch_batch_read(const ReadDesc* desc, const int32_t* rep_member, int64_t n_replicas,
+                             uint8_t* states, double* energy, unsigned long long* accepted,
+                             hipStream_t st);
 hipError_t launch_philox(int n, const uint32_t* ctr, uint32_t k0, uint32_t k1, uint32_t* out,
                          hipStream_t st);
 

@@ -754,13 +754,14 @@ __global__ void k_gather_replicas(int n, const int32_t* __restrict__ src, const 
 }
 
 // FP64 energy of plain states [s][var]: term CSR in variable ids.
-__global__ void k_energy(const int64_t* __restrict__ term_off, const int32_t* __restrict__ term_var,
-                         const double* __restrict__ term_c, int64_t M, int kind, int N,
-                         const uint8_t* __restrict__ states, double* __restrict__ out) {
-  __shared__ double part[256];
-  const uint8_t* st = states + (int64_t)blockIdx.x * N;
+// FP64 energy of one plain state by a 256-thread work-group: strided partial
+// sums over the terms, then a fixed tree (the same order for every caller, so
+// single and batched read-backs agree bit for bit).
+__device__ double state_energy_256(const int64_t* __restrict__ term_off, const int32_t* __restrict__ term_var,
+                                   const double* __restrict__ term_c, int64_t M, int kind,
+                                   const uint8_t* st, double* part) {
   double acc = 0.0;
-  for (int64_t j = threadIdx.x; j < M; j += blockDim.x) {
+  for (int64_t j = threadIdx.x; j < M; j += 256) {
     uint32_t x = 0;
     for (int64_t k = term_off[j]; k < term_off[j + 1]; ++k) {
       const uint32_t bit = st[term_var[k]] & 1u;
@@ -772,11 +773,48 @@ __global__ void k_energy(const int64_t* __restrict__ term_off, const int32_t* __
   }
   part[threadIdx.x] = acc;
   __syncthreads();
-  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+  for (int w = 128; w > 0; w >>= 1) {
     if ((int)threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
     __syncthreads();
   }
-  if (threadIdx.x == 0) out[blockIdx.x] = part[0];
+  return part[0];
+}
+
+__global__ void __launch_bounds__(256)
+k_energy(const int64_t* __restrict__ term_off, const int32_t* __restrict__ term_var,
+         const double* __restrict__ term_c, int64_t M, int kind, int N,
+         const uint8_t* __restrict__ states, double* __restrict__ out) {
+  __shared__ double part[256];
+  const uint8_t* st = states + (int64_t)blockIdx.x * N;
+  const double e = state_energy_256(term_off, term_var, term_c, M, kind, st, part);
+  if (threadIdx.x == 0) out[blockIdx.x] = e;
+}
+
+// Batched read-back: work-group = one global replica.  Unpacks its lowest
+// state into the plain output (variable order), then sums its energy from
+// that copy (written by this work-group, visible after the barrier).
+__global__ void __launch_bounds__(256)
+k_batch_read(const ReadDesc* __restrict__ desc, const int32_t* __restrict__ rep_member,
+             uint8_t* __restrict__ states, double* __restrict__ energy,
+             unsigned long long* __restrict__ accepted) {
+  __shared__ double part[256];
+  const int64_t gr = blockIdx.x;
+  const ReadDesc D = desc[rep_member[gr]];
+  const int r = (int)(gr - D.roff);
+  const UnitDev& U = D.units[r / D.B];
+  const int b = r % D.B;
+  const int G = U.G;
+  uint8_t* st = states + D.soff + (int64_t)r * D.N;
+  for (int v = threadIdx.x; v < D.N; v += 256) {
+    const int q = U.pos_of_var[v];
+    st[v] = (U.best[q * G + (b >> 3)] >> (b & 7)) & 1u;
+  }
+  __syncthreads();
+  const double e = state_energy_256(D.term_off, D.term_var, D.term_c, D.M, D.kind, st, part);
+  if (threadIdx.x == 0) {
+    energy[gr] = e;
+    accepted[gr] = U.accepted[b];
+  }
 }
 
 // dE of every variable for plain states, with the sweep kernels' arithmetic.
@@ -963,6 +1001,16 @@ hipError_t launch_energy(const ProblemDev& P, const uint8_t* states, int n, doub
   if (n <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_energy, dim3(n), dim3(256), 0, st, P.term_off, P.term_var, P.term_c, P.M,
                      P.kind, P.N, states, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_batch_read(const ReadDesc* desc, const int32_t* rep_member, int64_t n_replicas,
+                             uint8_t* states, double* energy, unsigned long long* accepted,
+                             hipStream_t st) {
+  if (n_replicas <= 0) return hipSuccess;
+  if (n_replicas > 0x7fffffffLL) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_batch_read, dim3((unsigned)n_replicas), dim3(256), 0, st, desc, rep_member, states,
+                     energy, accepted);
   return hipGetLastError();
 }
 

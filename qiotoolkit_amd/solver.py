@@ -20,7 +20,13 @@ TARGETS = ("simulatedannealing.qiotoolkit", "paralleltempering.qiotoolkit", "pop
 EXPORTS = ["qiotk_engine_create", "qiotk_engine_destroy", "qiotk_solve", "qiotk_model_create",
            "qiotk_model_destroy", "qiotk_solve_model", "qiotk_solve_batch", "qiotk_free",
            "qiotk_last_error", "qiotk_parse_model", "qiotk_schedule_eval", "qiotk_describe",
-           "qiotk_model_create_terms"]
+           "qiotk_model_create_terms", "qiotk_solve_batch_arrays", "qiotk_model_vars",
+           "qiotk_tune_batch"]
+
+
+def _check_rc(rc):
+    if rc:
+        raise SolverError(rc, lib().qiotk_last_error().decode())
 
 
 class SolverError(RuntimeError):
@@ -61,6 +67,9 @@ def lib():
     L.qiotk_model_create_terms.argtypes = [vp, C.c_int, C.c_int64, P(C.c_int64), P(C.c_int32),
                                            P(C.c_double), P(vp)]
     _lib = L
+    L.qiotk_solve_batch_arrays.argtypes = [vp, C.c_char_p, C.c_int32, P(vp), P(C.c_char_p), vp, vp, vp]
+    L.qiotk_model_vars.argtypes = [vp, P(C.c_int32), vp]
+    L.qiotk_tune_batch.argtypes = [vp, C.c_char_p, C.c_int32, P(vp), P(C.c_char_p), C.c_int32, P(vp)]
     return L
 
 
@@ -131,6 +140,40 @@ class Engine:
             raise SolverError(rc, lib().qiotk_last_error().decode())
         return [_raise_from(_take(o)) for o in outs], ms.value
 
+    def tune_batch(self, target, models, params_list, trial_sweeps=20):
+        """Pick replicas per work-group and lane depth for a batch by short
+        timed trials on a sample; returns {"params": {...}, "trials": [...]}."""
+        n = len(models)
+        hs = (C.c_void_p * n)(*[m.h for m in models])
+        ps = (C.c_char_p * n)(*[_enc(p) for p in params_list])
+        out = C.c_void_p()
+        lib().qiotk_tune_batch(self.h, target.encode(), n, hs, ps, trial_sweeps, C.byref(out))
+        return _raise_from(_take(out))
+
+    def solve_batch_arrays(self, target, models, params_list, configs=True):
+        """solve_batch with array results: (costs[n], configurations or None,
+        timings {init_ms, run_ms, finalize_ms, device_ms}).  configurations is
+        a list of int8 arrays by internal variable id (+1/-1 Ising, 1/0 PUBO)."""
+        import numpy as np
+        n = len(models)
+        hs = (C.c_void_p * n)(*[m.h for m in models])
+        ps = (C.c_char_p * n)(*[_enc(p) for p in params_list])
+        cost = np.zeros(n, dtype=np.float64)
+        sizes = [m.n_vars() for m in models]
+        cfg = np.zeros(max(1, sum(sizes)), dtype=np.int8) if configs else None
+        tm = np.zeros(4, dtype=np.float64)
+        rc = lib().qiotk_solve_batch_arrays(self.h, target.encode(), n, hs, ps, cost.ctypes.data,
+                                            cfg.ctypes.data if configs else None, tm.ctypes.data)
+        if rc:
+            raise SolverError(rc, lib().qiotk_last_error().decode())
+        out = None
+        if configs:
+            out, o = [], 0
+            for nv in sizes:
+                out.append(cfg[o:o + nv])
+                o += nv
+        return cost, out, {"init_ms": tm[0], "run_ms": tm[1], "finalize_ms": tm[2], "device_ms": tm[3]}
+
 
 class Model:
     """A cost function parsed, normalised and resident on the device."""
@@ -163,6 +206,17 @@ class Model:
             self.close()
         except Exception:
             pass
+
+    def n_vars(self):
+        n = C.c_int32()
+        _check_rc(lib().qiotk_model_vars(self.h, C.byref(n), None))
+        return n.value
+
+    def names(self):
+        import numpy as np
+        out = np.zeros(max(1, self.n_vars()), dtype=np.int32)
+        _check_rc(lib().qiotk_model_vars(self.h, None, out.ctypes.data))
+        return out[:self.n_vars()]
 
     def solve(self, target, params):
         out = C.c_void_p()

@@ -122,14 +122,15 @@ def er_ising_arrays(N, mean_degree=6.0, jmax=3, seed=0):
         pick = rng.choice(len(iu), size=m, replace=False)
         a, b = iu[pick], ju[pick]
     else:
-        got = np.zeros((0, 2), dtype=np.int64)
-        while len(got) < m:
-            x = rng.integers(0, N, size=(2 * (m - len(got)) + 16, 2))
+        # distinct unordered pairs, encoded a*N + b (a < b)
+        keys = np.zeros(0, dtype=np.int64)
+        while len(keys) < m:
+            x = rng.integers(0, N, size=(2 * (m - len(keys)) + 16, 2))
             x = x[x[:, 0] != x[:, 1]]
-            x.sort(axis=1)
-            got = np.unique(np.concatenate([got, x]), axis=0)
-        sel = rng.permutation(len(got))[:m]
-        a, b = got[sel, 0], got[sel, 1]
+            lo, hi = np.minimum(x[:, 0], x[:, 1]), np.maximum(x[:, 0], x[:, 1])
+            keys = np.unique(np.concatenate([keys, lo * N + hi]))
+        keys = keys[rng.permutation(len(keys))[:m]]
+        a, b = keys // N, keys % N
     vals = np.array([v for v in range(-jmax, jmax + 1) if v != 0])
     h = rng.integers(-1, 2, size=N)
     J = rng.choice(vals, size=m)

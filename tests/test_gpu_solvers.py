@@ -147,3 +147,34 @@ def test_best_energy_distribution_matches_reference(engine, kind):
     p = ks_2samp(gpu, cpu)
     assert p > 0.05, (p, gpu.mean(), cpu.mean())
     assert abs(gpu.mean() - cpu.mean()) <= 0.005 * abs(cpu.mean())
+
+
+@pytest.mark.parametrize("target", ["simulatedannealing.qiotoolkit", "paralleltempering.qiotoolkit"])
+def test_batch_arrays_match_json_and_single_solves(engine, target):
+    """Batched solves of uneven instances (arrays and JSON results) equal the
+    same solves run one per launch; every returned configuration has the
+    returned cost under the oracle's cost function."""
+    insts = []
+    for k, n in enumerate([64, 300, 1000, 90, 2000, 150]):
+        off, ids, c = wl.er_ising_arrays(n, 6.0, 3, seed=300 + k)
+        insts.append((off, ids, c))
+    models = [engine.model_from_arrays("ising", *t) for t in insts]
+    if target.startswith("simulated"):
+        base = {"beta_start": 0.1, "beta_stop": 3.0, "restarts": 8, "sweeps": 60}
+    else:
+        base = {"all_betas": [3.0, 2.0, 1.0, 0.5], "sweeps": 60}
+    plist = [{"params": dict(base, seed=11 + k, positions_per_lane=2 if k % 2 else 0)}
+             for k in range(len(models))]
+    cost, cfg, tm = engine.solve_batch_arrays(target, models, plist)
+    res, _ = engine.solve_batch(target, models, plist)
+    for k, m in enumerate(models):
+        single = m.solve(target, plist[k])
+        assert single["solutions"]["cost"] == cost[k] == res[k]["solutions"]["cost"]
+        names = m.names()
+        want = {str(n): int(v) for n, v in zip(names, cfg[k])}
+        assert want == res[k]["solutions"]["configuration"]
+        off, ids, c = insts[k]
+        terms = [{"c": float(c[j]), "ids": [int(x) for x in ids[off[j]:off[j + 1]]]} for j in range(len(c))]
+        doc = {"cost_function": {"type": "ising", "terms": terms}}
+        assert oracle_cost(doc, want) == cost[k]
+    assert tm["device_ms"] > 0

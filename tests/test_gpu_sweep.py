@@ -209,6 +209,67 @@ def test_batch_mixed_problems_same_as_alone(ctx):
         b = w2.read()
         for k in ("states", "cost", "lowest", "lowest_states", "accepted"):
             assert np.array_equal(a[k], b[k]), k
+    # one batched read-back of every member: energies recomputed from the
+    # lowest states (integer couplings: equal to the tracked lowest costs)
+    for a, w2, r in zip(alone, mixed, batch.read()):
+        assert np.array_equal(r["lowest"], a["lowest"])
+        assert np.array_equal(r["lowest"], w2.energy(lowest=True))
+        assert np.array_equal(r["accepted"], a["accepted"])
+        assert np.array_equal(r["lowest_states"], a["lowest_states"])
+
+
+def test_batch_groups_on_several_streams(ctx):
+    """Members with different work-group sizes (lane counts) and kernel
+    variants run as separate launches spread over several streams; each
+    result equals its run alone."""
+    specs = [("ising", wl.er_ising(900, 6.0, seed=41), 8, 8, 512),
+             ("ising", wl.er_ising(400, 6.0, seed=42), 8, 8, 128),
+             ("ising", wl.er_ising(200, 6.0, seed=43), 8, 1, 64),
+             ("pubo", wl.random_pubo(150, 600, seed=44), 4, 4, 64),
+             ("ising", wl.lattice3d(8, seed=45), 16, 2, 256)]
+    sweeps = 40
+    alone, mixed, keep = [], [], []
+    for i, (kind, terms, R, B, lanes) in enumerate(specs):
+        op, qp = pair(ctx, kind, terms)
+        keep.append(qp)
+        betas = np.linspace(0.1, 3.0, sweeps)
+        w = _qmc.Walkers(qp, R, B, seed=100 + i)
+        w.set_lanes(lanes)
+        w.init()
+        w.sweep(0, schedule=betas)
+        alone.append(w.read())
+        w2 = _qmc.Walkers(qp, R, B, seed=100 + i)
+        w2.set_lanes(lanes)
+        w2.init()
+        w2.plan(0, sweeps, betas)
+        mixed.append(w2)
+    batch = _qmc.Batch(ctx)
+    for w2 in mixed:
+        batch.add(w2)
+    batch.run(1 << 20)
+    batch.wait()
+    for a, w2, r in zip(alone, mixed, batch.read()):
+        b = w2.read()
+        for k in ("states", "cost", "lowest", "lowest_states", "accepted"):
+            assert np.array_equal(a[k], b[k]), k
+        assert np.array_equal(r["lowest_states"], a["lowest_states"])
+    # the same plans as graph-replayed launches of 7 sweeps (replayed twice:
+    # re-planned walkers reuse the recorded graph)
+    for rep in range(2):
+        acc0 = []
+        for i, w2 in enumerate(mixed):
+            acc0.append(w2.read()["accepted"])
+            w2.init()
+            w2.plan(0, sweeps, np.linspace(0.1, 3.0, sweeps))
+        batch.run_chunked(7)
+        batch.wait()
+        for a, w2, a0 in zip(alone, mixed, acc0):
+            b = w2.read()
+            assert w2.remaining() == 0
+            for k in ("states", "cost", "lowest", "lowest_states"):
+                assert np.array_equal(a[k], b[k]), (rep, k)
+            # accepted counters accumulate over runs
+            assert np.array_equal(a["accepted"], b["accepted"] - a0), rep
 
 
 @pytest.mark.parametrize("kind,terms,n_ladders,steps,groups",
@@ -238,3 +299,25 @@ def test_pt_independent_of_replica_grouping(ctx, kind, terms, n_ladders, steps, 
             continue
         for k in ("states", "cost", "lowest", "labels", "beta"):
             assert np.array_equal(got[k], ref[k]), (B, k)
+
+
+def test_handles_destroyed_in_any_order():
+    """Handles are reference counted: closing the context or the problem
+    first leaves walkers and batches usable until they are destroyed."""
+    c = _qmc.Context(0)
+    op, qp = pair(c, "ising", wl.er_ising(120, 6.0, seed=51))
+    w = _qmc.Walkers(qp, 8, 8, seed=3)
+    w.init()
+    betas = np.linspace(0.1, 3.0, 10)
+    w.plan(0, 10, betas)
+    b = _qmc.Batch(c)
+    b.add(w)
+    c.close()
+    qp.close()
+    b.run(1 << 20)
+    b.wait()
+    got = w.read()
+    assert got["lowest"].shape == (8,)
+    w.close()
+    assert b.read()[0]["lowest"].shape == (8,)
+    b.close()
