@@ -229,6 +229,110 @@ def run_c4(args):
     return out
 
 
+def c5_requests(rate, duration, seed=5005):
+    """Poisson arrivals at `rate` per second over `duration` seconds.  70 %
+    Ising ER (N log-uniform in [100, 50000], mean degree 8, J ~ N(0,1) fp32),
+    30 % random PUBO (V log-uniform in [500, 20000] vars, 10 V terms of degree
+    1/2/3/4 at 10/40/30/20 %, c in [-10, 10] \\ {0}); each request is
+    population annealing (population 512, beta 0 -> 3 over 200 steps of 10
+    sweeps) or simulated annealing (64 restarts x 1000 sweeps, beta 0.1 ->
+    3.0) by a fair coin."""
+    from qiotoolkit_amd import workloads as wl
+    rng = np.random.default_rng(seed)
+    t, out = 0.0, []
+    while True:
+        t += rng.exponential(1.0 / rate)
+        if t > duration:
+            break
+        k = len(out)
+        ising = rng.random() < 0.7
+        pa = rng.random() < 0.5
+        if ising:
+            n = int(round(np.exp(rng.uniform(np.log(100), np.log(50000)))))
+            arrays = wl.er_ising_f32_arrays(n, 8.0, seed=seed * 1000 + k)
+        else:
+            n = int(round(np.exp(rng.uniform(np.log(500), np.log(20000)))))
+            arrays = wl.random_pubo_arrays(n, 10 * n, seed=seed * 1000 + k)
+        if pa:
+            target = "populationannealing.cpu"
+            params = {"population": 512, "beta_start": 0.0, "beta_stop": 3.0, "sweeps": 200,
+                      "sweeps_per_replica": 10, "seed": 9000 + k}
+            attempts = 512 * 200 * 10 * n  # nominal: the population fluctuates around 512
+        else:
+            target = "simulatedannealing.qiotoolkit"
+            params = {"beta_start": 0.1, "beta_stop": 3.0, "restarts": 64, "sweeps": 1000, "seed": 9000 + k}
+            attempts = 64 * 1000 * n
+        out.append({"t": t, "kind": "ising" if ising else "pubo", "n": n, "target": target,
+                    "params": params, "arrays": arrays, "nominal_attempts": attempts})
+    return out
+
+
+def run_c5(args):
+    """Serving stream: requests submitted at their Poisson arrival times to
+    the serving loop; latency = finish - scheduled arrival."""
+    from qiotoolkit_amd.solver import Server
+    t_gen = time.perf_counter()
+    reqs = c5_requests(args.rate, args.duration)
+    log("c5: %d requests generated in %.1f s (%d ising, %d pubo, %d PA, %d SA)" % (
+        len(reqs), time.perf_counter() - t_gen, sum(r["kind"] == "ising" for r in reqs),
+        sum(r["kind"] == "pubo" for r in reqs), sum(r["target"].startswith("pop") for r in reqs),
+        sum(r["target"].startswith("sim") for r in reqs)))
+    cfg = {"max_active": args.max_active, "sweeps_per_round": args.sweeps_per_round}
+    if args.hbm_budget_gib:
+        cfg["hbm_budget_bytes"] = int(args.hbm_budget_gib * (1 << 30))
+    srv = Server(0, **cfg)
+    t0 = time.perf_counter()
+    ids, lag = [], 0.0
+    for r in reqs:
+        now = time.perf_counter() - t0
+        if r["t"] > now:
+            time.sleep(r["t"] - now)
+        lag = max(lag, time.perf_counter() - t0 - r["t"])
+        ids.append(srv.submit(r["target"], {"params": r["params"]}, r["kind"], *r["arrays"]))
+    log("c5: all submitted at %.1f s (max submit lag %.3f s)" % (time.perf_counter() - t0, lag))
+    lat, att, fails = [], 0, 0
+    for r, rid in zip(reqs, ids):
+        rc, doc = srv.wait(rid, -1)
+        if rc:
+            fails += 1
+            continue
+        sv = doc["benchmark"]["serving"]
+        lat.append(sv["finished_ms"] - r["t"] * 1e3)
+        att += doc["benchmark"]["cost_difference_evaluation_count"]
+    t_end = time.perf_counter() - t0
+    st = srv.stats()
+    srv.close()
+    lat = np.array(lat)
+    value = att / t_end
+    out = {
+        "metric": "spin-flip attempts/s at equal final-energy quality; p50/p99 solve latency",
+        "value": value,
+        "unit": "spin-flip attempts/s",
+        "n_gpus": 1,
+        "steps": 1,
+        "warmup": 0,
+        "ms_per_step": t_end * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32/int32",
+        "data": "synthetic (seeded request stream)",
+        "config": {"workload": "serving stream: Poisson arrivals %.2f/s for %.0f s; 70%% Ising ER N 100-50000 "
+                               "deg 8 fp32, 30%% PUBO 500-20000 vars 10x terms; PA (pop 512, 200 steps x 10 "
+                               "sweeps, beta 0->3) or SA (64 x 1000 sweeps, beta 0.1->3) 50/50"
+                               % (args.rate, args.duration),
+                   "requests": len(reqs), "offered_rate_per_s": args.rate, "server": cfg,
+                   "parallelism": "none (1 GPU)"},
+        "latency_ms": {"p50": float(np.percentile(lat, 50)) if len(lat) else None,
+                       "p99": float(np.percentile(lat, 99)) if len(lat) else None,
+                       "max": float(lat.max()) if len(lat) else None},
+        "requests_per_s": len(lat) / t_end,
+        "failed": fails,
+        "server_stats": st,
+    }
+    return out
+
+
 def cpu_baseline_c2(inp, temps, args):
     """Reference algorithm (oracle) on a bounded sample: one 32-replica ladder."""
     from oracle import pyoracle as po
@@ -268,6 +372,11 @@ def main():
                     help="lanes cover the largest colour class in this many rounds (0 = auto)")
     ap.add_argument("--instances", type=int, default=4096, help="c4: number of instances")
     ap.add_argument("--no-tune", action="store_true", help="c4: skip the autotuner")
+    ap.add_argument("--rate", type=float, default=1.0, help="c5: offered requests per second")
+    ap.add_argument("--duration", type=float, default=30.0, help="c5: seconds of arrivals")
+    ap.add_argument("--max-active", type=int, default=64, help="c5: concurrently admitted requests")
+    ap.add_argument("--sweeps-per-round", type=int, default=10, help="c5: sweeps per shared launch")
+    ap.add_argument("--hbm-budget-gib", type=float, default=0.0, help="c5: device memory for requests")
     ap.add_argument("--plain-sample", type=int, default=128,
                     help="c4: instances run one per launch for the plain-form rate")
     ap.add_argument("--no-cpu", action="store_true")
@@ -291,10 +400,11 @@ def main():
               "multi-GPU runs are not part of this build (--gpus %d refused)" % args.gpus,
               file=sys.stderr)
         sys.exit(2)
-    if args.workload not in ("c2", "c4"):
+    runs = {"c2": run_c2, "c4": run_c4, "c5": run_c5}
+    if args.workload not in runs:
         print("unknown workload %s" % args.workload, file=sys.stderr)
         sys.exit(2)
-    out = run_c2(args) if args.workload == "c2" else run_c4(args)
+    out = runs[args.workload](args)
     print(json.dumps(out), flush=True)
 
 

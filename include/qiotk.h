@@ -73,6 +73,36 @@ int qiotk_tune_batch(qiotk_engine e, const char* target, int32_t n, const qiotk_
  * (first appearance in the term list, graph/graph.h:97-124); names may be NULL. */
 int qiotk_model_vars(qiotk_model m, int32_t* n_vars, int32_t* names);
 
+/* Serving loop (no counterpart in the reference, whose Runner solves one
+ * request per process, app/runner.cc:513-688): a background thread with its
+ * own device context admits submitted solves in arrival order while their
+ * device footprint fits the HBM budget and a slot is free, then advances every
+ * admitted solve in shared launches, round by round (sweeps_per_round sweeps
+ * of each solve, or one population-annealing step), keeping its state on the
+ * device until it completes.  Results are the qiotk_solve response plus
+ * benchmark.serving {arrival_ms, admitted_ms, finished_ms, latency_ms,
+ * queue_ms, rounds}; a result does not depend on what else was in flight.
+ * config_json: {"hbm_budget_bytes": 68719476736, "max_active": 64,
+ * "sweeps_per_round": 10} (all optional). */
+typedef struct qiotk_server_s* qiotk_server;
+int qiotk_server_create(int device, const char* config_json, qiotk_server* out);
+/* Enqueue a solve (model as a term CSR, kind 0 ising / 1 pubo, as in
+ * qiotk_model_create_terms); returns at once with its request id. */
+int qiotk_server_submit(qiotk_server s, const char* target, const char* params_json, int kind,
+                        int64_t n_terms, const int64_t* offsets, const int32_t* ids, const double* coeffs,
+                        int64_t* request_id);
+int qiotk_server_submit_json(qiotk_server s, const char* target, const char* params_json,
+                             const char* input_json, int64_t* request_id);
+/* Wait up to timeout_ms (< 0: no limit) for a request; *response is NULL
+ * when it is not finished yet, else its response (free with qiotk_free),
+ * and the return value is its status code. */
+int qiotk_server_wait(qiotk_server s, int64_t request_id, double timeout_ms, char** response);
+/* Counters and latency percentiles of the finished requests as JSON. */
+int qiotk_server_stats(qiotk_server s, char** response);
+/* Finishes every submitted request, then stops the loop. */
+void qiotk_server_destroy(qiotk_server s);
+const char* qiotk_server_last_error(void);
+
 /* Device-free host logic, for tooling and tests. */
 /* graph::normalize_edges result as JSON: names, term CSR, const_cost, ... */
 int qiotk_parse_model(const char* input_json, char** response);

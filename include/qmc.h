@@ -175,6 +175,8 @@ int qmc_walkers_set_pt_mode(qmc_walkers w, int mode);
  * may be NULL. */
 int qmc_walkers_read(qmc_walkers w, double* cost, double* lowest, double* beta,
                      uint64_t* accepted, uint8_t* states, uint8_t* lowest_states);
+/* One replica's current and/or lowest state (n_vars bytes each; NULL skips). */
+int qmc_walkers_read_replica(qmc_walkers w, int32_t replica, uint8_t* state, uint8_t* lowest_state);
 /* Recompute energies (FP64, on device) of the current or lowest states. */
 int qmc_walkers_energy(qmc_walkers w, int lowest, double* energies);
 /* PT ladder labels: slot holding beta index k for every ladder (n_replicas) */

@@ -25,7 +25,7 @@ EXPORTS = [
     "qmc_walkers_labels", "qmc_energy", "qmc_delta_all", "qmc_philox", "qmc_walkers_last_ms",
     "qmc_walkers_plan", "qmc_walkers_plan_pt", "qmc_walkers_remaining", "qmc_batch_create",
     "qmc_batch_destroy", "qmc_batch_add", "qmc_batch_remove", "qmc_batch_clear", "qmc_batch_run",
-    "qmc_batch_wait", "qmc_batch_read", "qmc_batch_run_chunked", "qmc_walkers_resample", "qmc_walkers_count", "qmc_walkers_set_pt_mode",
+    "qmc_batch_wait", "qmc_walkers_read_replica", "qmc_batch_read", "qmc_batch_run_chunked", "qmc_walkers_resample", "qmc_walkers_count", "qmc_walkers_set_pt_mode",
     "qmc_walkers_set_lanes", "qmc_walkers_get_lanes",
 ]
 
@@ -98,6 +98,7 @@ def lib():
     L.qmc_batch_run.argtypes = [vp, C.c_int32]
     L.qmc_batch_wait.argtypes = [vp, P(C.c_double)]
     L.qmc_batch_read.argtypes = [vp, vp, vp, vp]
+    L.qmc_walkers_read_replica.argtypes = [vp, C.c_int32, vp, vp]
     L.qmc_batch_run_chunked.argtypes = [vp, C.c_int32]
     L.qmc_walkers_resample.argtypes = [vp, P(C.c_int32), C.c_int32]
     L.qmc_walkers_count.argtypes = [vp, P(C.c_int32)]

@@ -21,7 +21,7 @@ HIP_FLAGS = [
 
 DEVICE_SOURCES = ["qmc_kernels.hip", "qmc_api.cpp"]
 HOST_SOURCES = ["host/json.cc", "host/model.cc", "host/schedule.cc", "host/solver.cc", "host/pa.cc",
-                "host/capi.cc"]
+                "host/capi.cc", "host/server.cc"]
 
 
 def _run(cmd):
@@ -47,7 +47,7 @@ def build_host():
         return None
     out = os.path.join(HERE, "libqiotk.so")
     srcs = [os.path.join(CSRC, s) for s in HOST_SOURCES]
-    _run(["g++", "-std=c++17", "-O2", "-fPIC", "-shared", "-Wall", "-I", os.path.join(ROOT, "include"),
+    _run(["g++", "-std=c++17", "-O2", "-fPIC", "-shared", "-Wall", "-pthread", "-I", os.path.join(ROOT, "include"),
           "-o", out] + srcs + [os.path.join(HERE, "libqmc.so"), "-Wl,-rpath,$ORIGIN"])
     return out
 

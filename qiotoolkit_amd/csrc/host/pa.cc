@@ -107,134 +107,157 @@ void PopulationAnnealing::run() {
   if (model_->empty()) return;
   PaState& st = pa_state(impl_);
   const int64_t steps = steps_to_run();
-  const int64_t N = model_->n_vars;
-  const double scale = model_->scale_factor();
-  int64_t base_step = 0;
-  uint32_t epoch = 0;
-  int64_t sweep = 0;
-  std::vector<double> after, lowest;
-  std::vector<uint8_t> lstates;
-  for (int64_t step = 0; step < steps; ++step) {
-    int32_t R = 0;
-    qmc_walkers_count(st.w, &R);
-    const double Rd = (double)R;
-    device_check(qmc_walkers_sweep(st.w, sweep, sweeps_per_replica_, QMC_BETA_PER_REPLICA, nullptr, 0),
+  while (step_ < steps) {
+    device_check(qmc_walkers_sweep(st.w, sweep_, sweeps_per_replica_, QMC_BETA_PER_REPLICA, nullptr, 0),
                  "qmc_walkers_sweep");
     double ms = 0;
     qmc_walkers_last_ms(st.w, &ms);
     timing_.device_ms += ms;
     timing_.launches++;
-    sweep += sweeps_per_replica_;
-    attempts_ += (int64_t)R * sweeps_per_replica_ * N;
-    after.assign(R, 0.0);
-    lowest.assign(R, 0.0);
-    device_check(qmc_walkers_read(st.w, after.data(), lowest.data(), nullptr, nullptr, nullptr, nullptr),
-                 "qmc_walkers_read");
-    // update_lowest_cost over the citizens (:301-308)
-    const int64_t kbest = std::min_element(lowest.begin(), lowest.end()) - lowest.begin();
-    if (!have_lowest_ || lowest[kbest] < lowest_cost_) {
-      lstates.assign((size_t)R * N, 0);
-      device_check(qmc_walkers_read(st.w, nullptr, nullptr, nullptr, nullptr, nullptr, lstates.data()),
-                   "qmc_walkers_read");
-      record(lowest[kbest], std::vector<uint8_t>(lstates.begin() + kbest * N, lstates.begin() + (kbest + 1) * N));
-    }
-    double avg_after = 0, min_cost = std::numeric_limits<double>::max();
-    for (double c : after) {
-      avg_after += c;
-      min_cost = c < min_cost ? c : min_cost;
-    }
-    avg_after /= Rd;
-    double var_after = 0;
-    for (double c : after) var_after += (c - avg_after) * (c - avg_after);
-    var_after /= Rd;
-    for (double& c : after) c -= min_cost;
-    const double delta_beta = beta_.get_value((double)(step + 1 - base_step)) -
-                              beta_.get_value((double)(step - base_step));
-    double rho_t = 1.0;
-    if (delta_beta > 0) {
-      // resample (:490-616), Poisson branch
-      std::vector<double> weight(R);
-      double total = 0;
-      for (int32_t i = 0; i < R; ++i) {
-        weight[i] = std::exp(-delta_beta * after[i] * scale);
-        total += weight[i];
-      }
-      const double RRQ = (double)target_population_ / total;
-      std::vector<uint32_t> count(R, 1);
-      int64_t current = R;
-      for (int32_t i = 0; i < R; ++i) {
-        const double mean = RRQ * weight[i];
-        uint32_t nc = 0;
-        if (mean > 0.0) {
-          nc = poisson_draw(mean, [&](uint32_t k) {
-            Words4 w = philox4x32_10((uint32_t)i, (uint32_t)step, k, DOM_POISSON, seed_, 0u);
-            return canonical2(w.w[0], w.w[1]);
-          });
-        }
-        if (nc == 0) {
-          current -= count[i];
-          count[i] = 0;
-        } else if (nc > 1) {
-          current += nc - 1;
-          count[i] += nc - 1;
-        }
-      }
-      if (current == 0) {  // keep the population alive
-        Words4 w = philox4x32_10(0u, (uint32_t)step, 0u, DOM_PICK, seed_, 0u);
-        const int32_t i = (int32_t)std::floor(canonical2(w.w[0], w.w[1]) * Rd);
-        count[i] += 1;
-        current = 1;
-      }
-      if (current > st.cap)
-        throw Error(kMemoryLimited, "population grew beyond the reserved capacity");
-      std::vector<int32_t> src;
-      std::vector<size_t> fam;
-      src.reserve(current);
-      for (int32_t i = 0; i < R; ++i) {
-        for (uint32_t k = 0; k < count[i]; ++k) {
-          src.push_back(i);
-          fam.push_back(family_[i]);
-        }
-      }
-      device_check(qmc_walkers_resample(st.w, src.data(), (int32_t)src.size()), "qmc_walkers_resample");
-      family_.swap(fam);
-      std::map<size_t, size_t> census;
-      for (size_t f : family_) census[f]++;
-      rho_t = 0;
-      for (auto& kv : census) rho_t += (double)kv.second * (double)kv.second;
-      rho_t /= Rd;
-    }
-    if (Rd / rho_t < alpha_ || var_after == 0) {
-      // restart (:461-475): fresh random citizens; existing ones keep their
-      // beta until the next step sets it, as init_population does
-      cur_beta_ = beta_start_;
-      base_step = step;
-      int32_t Rnow = 0;
-      qmc_walkers_count(st.w, &Rnow);
-      if (Rnow < target_population_) {
-        std::vector<double> b(Rnow);
-        device_check(qmc_walkers_read(st.w, nullptr, nullptr, b.data(), nullptr, nullptr, nullptr), "read");
-        std::vector<int32_t> src(target_population_);
-        for (int64_t i = 0; i < target_population_; ++i) src[i] = (int32_t)std::min<int64_t>(i, Rnow - 1);
-        device_check(qmc_walkers_resample(st.w, src.data(), (int32_t)target_population_), "resample");
-        b.resize(target_population_, beta_start_);
-        device_check(qmc_walkers_set_betas(st.w, b.data()), "qmc_walkers_set_betas");
-        Rnow = (int32_t)target_population_;
-      }
-      device_check(qmc_walkers_init(st.w, ++epoch), "qmc_walkers_init");
-      family_.resize(Rnow);
-      for (int64_t i = 0; i < Rnow; ++i)
-        if (i < target_population_) family_[i] = (size_t)i;
-      restarts_++;
-    } else {
-      cur_beta_ += delta_beta;
-      int32_t Rnow = 0;
-      qmc_walkers_count(st.w, &Rnow);
-      std::vector<double> b(Rnow, cur_beta_);
-      device_check(qmc_walkers_set_betas(st.w, b.data()), "qmc_walkers_set_betas");
-    }
-    steps_done_ = step + 1;
+    finish_step();
   }
+}
+
+bool PopulationAnnealing::serve_plan(qmc_batch b) {
+  if (model_->empty() || step_ >= steps_to_run()) return false;
+  PaState& st = pa_state(impl_);
+  device_check(qmc_walkers_plan(st.w, sweep_, sweeps_per_replica_, nullptr), "qmc_walkers_plan");
+  if (!in_batch_) {
+    device_check(qmc_batch_add(b, st.w), "qmc_batch_add");
+    in_batch_ = true;
+  }
+  return true;
+}
+
+bool PopulationAnnealing::serve_after() {
+  if (model_->empty()) return true;
+  finish_step();
+  return step_ >= steps_to_run();
+}
+
+qmc_walkers PopulationAnnealing::walkers() const {
+  return impl_ ? static_cast<PaState*>(impl_.get())->w : nullptr;
+}
+
+// The host part of one step after its sweeps (population_annealing.h:262-488):
+// lowest-cost update, weights, Poisson offspring, resampling and restarts.
+void PopulationAnnealing::finish_step() {
+  PaState& st = pa_state(impl_);
+  const int64_t step = step_;
+  const int64_t N = model_->n_vars;
+  const double scale = model_->scale_factor();
+  int32_t R = 0;
+  qmc_walkers_count(st.w, &R);
+  const double Rd = (double)R;
+  sweep_ += sweeps_per_replica_;
+  attempts_ += (int64_t)R * sweeps_per_replica_ * N;
+  std::vector<double> after(R, 0.0), lowest(R, 0.0);
+  device_check(qmc_walkers_read(st.w, after.data(), lowest.data(), nullptr, nullptr, nullptr, nullptr),
+               "qmc_walkers_read");
+  // update_lowest_cost over the citizens (:301-308)
+  const int64_t kbest = std::min_element(lowest.begin(), lowest.end()) - lowest.begin();
+  if (!have_lowest_ || lowest[kbest] < lowest_cost_) {
+    std::vector<uint8_t> ls((size_t)N);
+    device_check(qmc_walkers_read_replica(st.w, (int32_t)kbest, nullptr, ls.data()), "qmc_walkers_read_replica");
+    record(lowest[kbest], ls);
+  }
+  double avg_after = 0, min_cost = std::numeric_limits<double>::max();
+  for (double c : after) {
+    avg_after += c;
+    min_cost = c < min_cost ? c : min_cost;
+  }
+  avg_after /= Rd;
+  double var_after = 0;
+  for (double c : after) var_after += (c - avg_after) * (c - avg_after);
+  var_after /= Rd;
+  for (double& c : after) c -= min_cost;
+  const double delta_beta = beta_.get_value((double)(step + 1 - base_step_)) -
+                            beta_.get_value((double)(step - base_step_));
+  double rho_t = 1.0;
+  if (delta_beta > 0) {
+    // resample (:490-616), Poisson branch
+    std::vector<double> weight(R);
+    double total = 0;
+    for (int32_t i = 0; i < R; ++i) {
+      weight[i] = std::exp(-delta_beta * after[i] * scale);
+      total += weight[i];
+    }
+    const double RRQ = (double)target_population_ / total;
+    std::vector<uint32_t> count(R, 1);
+    int64_t current = R;
+    for (int32_t i = 0; i < R; ++i) {
+      const double mean = RRQ * weight[i];
+      uint32_t nc = 0;
+      if (mean > 0.0) {
+        nc = poisson_draw(mean, [&](uint32_t k) {
+          Words4 w = philox4x32_10((uint32_t)i, (uint32_t)step, k, DOM_POISSON, seed_, 0u);
+          return canonical2(w.w[0], w.w[1]);
+        });
+      }
+      if (nc == 0) {
+        current -= count[i];
+        count[i] = 0;
+      } else if (nc > 1) {
+        current += nc - 1;
+        count[i] += nc - 1;
+      }
+    }
+    if (current == 0) {  // keep the population alive
+      Words4 w = philox4x32_10(0u, (uint32_t)step, 0u, DOM_PICK, seed_, 0u);
+      const int32_t i = (int32_t)std::floor(canonical2(w.w[0], w.w[1]) * Rd);
+      count[i] += 1;
+      current = 1;
+    }
+    if (current > st.cap)
+      throw Error(kMemoryLimited, "population grew beyond the reserved capacity");
+    std::vector<int32_t> src;
+    std::vector<size_t> fam;
+    src.reserve(current);
+    for (int32_t i = 0; i < R; ++i) {
+      for (uint32_t k = 0; k < count[i]; ++k) {
+        src.push_back(i);
+        fam.push_back(family_[i]);
+      }
+    }
+    device_check(qmc_walkers_resample(st.w, src.data(), (int32_t)src.size()), "qmc_walkers_resample");
+    family_.swap(fam);
+    std::map<size_t, size_t> census;
+    for (size_t f : family_) census[f]++;
+    rho_t = 0;
+    for (auto& kv : census) rho_t += (double)kv.second * (double)kv.second;
+    rho_t /= Rd;
+  }
+  if (Rd / rho_t < alpha_ || var_after == 0) {
+    // restart (:461-475): fresh random citizens; existing ones keep their
+    // beta until the next step sets it, as init_population does
+    cur_beta_ = beta_start_;
+    base_step_ = step;
+    int32_t Rnow = 0;
+    qmc_walkers_count(st.w, &Rnow);
+    if (Rnow < target_population_) {
+      std::vector<double> b(Rnow);
+      device_check(qmc_walkers_read(st.w, nullptr, nullptr, b.data(), nullptr, nullptr, nullptr), "read");
+      std::vector<int32_t> src(target_population_);
+      for (int64_t i = 0; i < target_population_; ++i) src[i] = (int32_t)std::min<int64_t>(i, Rnow - 1);
+      device_check(qmc_walkers_resample(st.w, src.data(), (int32_t)target_population_), "resample");
+      b.resize(target_population_, beta_start_);
+      device_check(qmc_walkers_set_betas(st.w, b.data()), "qmc_walkers_set_betas");
+      Rnow = (int32_t)target_population_;
+    }
+    device_check(qmc_walkers_init(st.w, ++epoch_), "qmc_walkers_init");
+    family_.resize(Rnow);
+    for (int64_t i = 0; i < Rnow; ++i)
+      if (i < target_population_) family_[i] = (size_t)i;
+    restarts_++;
+  } else {
+    cur_beta_ += delta_beta;
+    int32_t Rnow = 0;
+    qmc_walkers_count(st.w, &Rnow);
+    std::vector<double> b(Rnow, cur_beta_);
+    device_check(qmc_walkers_set_betas(st.w, b.data()), "qmc_walkers_set_betas");
+  }
+  step_ = step + 1;
+  steps_done_ = step_;
 }
 
 void PopulationAnnealing::finalize() {

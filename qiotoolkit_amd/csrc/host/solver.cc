@@ -140,6 +140,20 @@ void Solver::apply_lanes(qmc_walkers w, int replicas_per_group) const {
   if (l) device_check(qmc_walkers_set_lanes(w, l), "qmc_walkers_set_lanes");
 }
 
+bool Solver::serve_plan(qmc_batch b) {
+  if (served_) return false;
+  served_ = true;
+  return batch_plan(b);
+}
+
+bool Solver::serve_after() {
+  qmc_walkers w = walkers();
+  if (!w) return true;
+  int64_t left = 0;
+  device_check(qmc_walkers_remaining(w, &left), "qmc_walkers_remaining");
+  return left == 0;
+}
+
 void Solver::finalize_from(const double* lowest_energy, const uint64_t* accepted, const uint8_t* states) {
   const int64_t R = batched_replicas_;
   if (R == 0) return;

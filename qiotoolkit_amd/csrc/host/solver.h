@@ -45,6 +45,14 @@ class Solver {
   // Replicas of one independent run as configured (SA restarts, PT ladder
   // length, PA population); known after configure().
   virtual int64_t run_replicas() const { return 1; }
+  // Serving form: every round the server calls serve_plan() to add this
+  // solve's next device work to the shared batch (false: nothing to add),
+  // runs the batch, then serve_after(), which does the host part of the
+  // step and returns true once the solve is complete (finalize() next).
+  virtual bool serve_plan(qmc_batch b);
+  virtual bool serve_after();
+  // the solve's walker set on the device (null before init or when empty)
+  virtual qmc_walkers walkers() const { return nullptr; }
   // Sweeps per device launch when fused into a batch (0: one persistent launch).
   int64_t sweeps_per_launch() const { return sweeps_per_launch_; }
   // finalize() from this solve's slices of a batched read-back: FP64 energy
@@ -102,6 +110,7 @@ class Solver {
   Json out_params_ = Json::object();
   std::string exit_reason_;
   int64_t batched_replicas_ = 0;
+  bool served_ = false;
 };
 
 // simulated_annealing.h
@@ -116,6 +125,7 @@ class SimulatedAnnealing : public Solver {
   Json describe() const override;
   int restarts() const { return restarts_; }
   int64_t run_replicas() const override { return restarts_; }
+  qmc_walkers walkers() const override { return w_; }
 
  private:
   bool use_inverse_temperature_ = false;
@@ -145,6 +155,7 @@ class ParallelTempering : public Solver {
   ~ParallelTempering() override;
   std::vector<double> ladder() const;
   int64_t run_replicas() const override { return (int64_t)ladder().size(); }
+  qmc_walkers walkers() const override { return w_; }
 
  private:
   bool use_inverse_temperatures_ = true;
@@ -166,9 +177,17 @@ class PopulationAnnealing : public Solver {
   void init() override;
   void run() override;
   void finalize() override;
+  bool serve_plan(qmc_batch b) override;
+  bool serve_after() override;
+  qmc_walkers walkers() const override;
+  int64_t run_replicas() const override { return target_population_; }
   ~PopulationAnnealing() override;
 
  private:
+  void finish_step();
+  int64_t step_ = 0, sweep_ = 0, base_step_ = 0;
+  uint32_t epoch_ = 0;
+  bool in_batch_ = false;
   double alpha_ = 2.0;
   int64_t target_population_ = 64;
   bool constant_population_ = false;

@@ -721,8 +721,8 @@ int qmc_walkers_resample(qmc_walkers w, const int32_t* src, int32_t new_R) {
   HIP_TRY(c1.alloc(new_R));
   HIP_TRY(c2.alloc(new_R));
   HIP_TRY(a0.alloc(new_R));
-  HIP_TRY(launch_unpack(w->d_units.p, w->B, w->R, p->N, 0, w->scratch_states.p, st));
-  HIP_TRY(launch_unpack(w->d_units.p, w->B, w->R, p->N, 1, low.p, st));
+  HIP_TRY(launch_unpack(w->d_units.p, w->B, 0, w->R, p->N, 0, w->scratch_states.p, st));
+  HIP_TRY(launch_unpack(w->d_units.p, w->B, 0, w->R, p->N, 1, low.p, st));
   HIP_TRY(launch_gather_replicas(new_R, dsrc.p, w->cost.p, w->best_cost.p, w->beta.p, w->accepted.p,
                                  c0.p, c1.p, c2.p, a0.p, st));
   HIP_TRY(hipMemcpyAsync(w->cost.p, c0.p, new_R * sizeof(double), hipMemcpyDeviceToDevice, st));
@@ -752,7 +752,7 @@ int qmc_walkers_init(qmc_walkers w, uint32_t epoch) {
   HIP_TRY(launch_init(w->d_units.p, w->n_units, p->N, epoch, st));
   int rc = ensure_scratch(w);
   if (rc) return rc;
-  HIP_TRY(launch_unpack(w->d_units.p, w->B, w->R, p->N, 0, w->scratch_states.p, st));
+  HIP_TRY(launch_unpack(w->d_units.p, w->B, 0, w->R, p->N, 0, w->scratch_states.p, st));
   HIP_TRY(launch_energy(p->dev(), w->scratch_states.p, w->R, w->cost.p, st));
   HIP_TRY(hipMemcpyAsync(w->best_cost.p, w->cost.p, w->R * sizeof(double), hipMemcpyDeviceToDevice, st));
   HIP_TRY(hipStreamSynchronize(st));
@@ -1297,15 +1297,32 @@ int qmc_walkers_read(qmc_walkers w, double* cost, double* lowest, double* beta, 
     if (rc) return rc;
     const size_t n = (size_t)w->R * p->N;
     if (states) {
-      HIP_TRY(launch_unpack(w->d_units.p, w->B, w->R, p->N, 0, w->scratch_states.p, st));
+      HIP_TRY(launch_unpack(w->d_units.p, w->B, 0, w->R, p->N, 0, w->scratch_states.p, st));
       HIP_TRY(hipMemcpyAsync(states, w->scratch_states.p, n, hipMemcpyDeviceToHost, st));
       HIP_TRY(hipStreamSynchronize(st));
     }
     if (lowest_states) {
-      HIP_TRY(launch_unpack(w->d_units.p, w->B, w->R, p->N, 1, w->scratch_states.p, st));
+      HIP_TRY(launch_unpack(w->d_units.p, w->B, 0, w->R, p->N, 1, w->scratch_states.p, st));
       HIP_TRY(hipMemcpyAsync(lowest_states, w->scratch_states.p, n, hipMemcpyDeviceToHost, st));
       HIP_TRY(hipStreamSynchronize(st));
     }
+  }
+  return QMC_OK;
+}
+
+int qmc_walkers_read_replica(qmc_walkers w, int32_t replica, uint8_t* state, uint8_t* lowest_state) {
+  if (!w) return fail(QMC_INVALID_ARGUMENT, "null handle");
+  if (replica < 0 || replica >= w->R) return fail(QMC_INVALID_ARGUMENT, "replica out of range");
+  qmc_problem p = w->prob;
+  hipStream_t st = p->ctx->stream;
+  int rc = ensure_scratch(w);
+  if (rc) return rc;
+  for (int which = 0; which < 2; ++which) {
+    uint8_t* out = which ? lowest_state : state;
+    if (!out) continue;
+    HIP_TRY(launch_unpack(w->d_units.p, w->B, replica, 1, p->N, which, w->scratch_states.p, st));
+    HIP_TRY(hipMemcpyAsync(out, w->scratch_states.p, p->N, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
   }
   return QMC_OK;
 }
@@ -1316,7 +1333,7 @@ int qmc_walkers_energy(qmc_walkers w, int lowest, double* energies) {
   hipStream_t st = p->ctx->stream;
   int rc = ensure_scratch(w);
   if (rc) return rc;
-  HIP_TRY(launch_unpack(w->d_units.p, w->B, w->R, p->N, lowest ? 1 : 0, w->scratch_states.p, st));
+  HIP_TRY(launch_unpack(w->d_units.p, w->B, 0, w->R, p->N, lowest ? 1 : 0, w->scratch_states.p, st));
   HIP_TRY(launch_energy(p->dev(), w->scratch_states.p, w->R, w->energies.p, st));
   HIP_TRY(hipMemcpyAsync(energies, w->energies.p, w->R * sizeof(double), hipMemcpyDeviceToHost, st));
   HIP_TRY(hipStreamSynchronize(st));

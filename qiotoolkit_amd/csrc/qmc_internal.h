@@ -92,7 +92,7 @@ hipError_t launch_exchange(double* cost, double* beta, int32_t* label, const dou
                            int spr, int64_t* last_step, uint32_t seed, uint32_t stream,
                            hipStream_t st);
 hipError_t launch_init(const UnitDev* units, int n_units, int N, uint32_t epoch, hipStream_t st);
-hipError_t launch_unpack(const UnitDev* units, int B, int R, int N, int which, uint8_t* out,
+hipError_t launch_unpack(const UnitDev* units, int B, int r0, int R, int N, int which, uint8_t* out,
                          hipStream_t st);
 hipError_t launch_pack(const UnitDev* units, int n_units, int B, int R, int N, const uint8_t* in,
                        int which, hipStream_t st);

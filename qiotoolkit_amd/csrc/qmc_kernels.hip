@@ -678,11 +678,12 @@ __global__ void k_init_states(const UnitDev* __restrict__ units, uint32_t epoch)
 }
 
 // plain states [r][var] <-> unit bytes
-__global__ void k_unpack(const UnitDev* __restrict__ units, int B, int R, int N, int which,
+__global__ void k_unpack(const UnitDev* __restrict__ units, int B, int r0, int R, int N, int which,
                          uint8_t* __restrict__ out) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (int64_t)R * N) return;
-  const int r = (int)(t / N), v = (int)(t - (int64_t)r * N);
+  const int rr = (int)(t / N), v = (int)(t - (int64_t)rr * N);
+  const int r = r0 + rr;
   const UnitDev& U = units[r / B];
   const int b = r % B;
   const int q = U.pos_of_var[v];
@@ -967,11 +968,11 @@ hipError_t launch_init(const UnitDev* units, int n_units, int N, uint32_t epoch,
   return hipGetLastError();
 }
 
-hipError_t launch_unpack(const UnitDev* units, int B, int R, int N, int which, uint8_t* out,
+hipError_t launch_unpack(const UnitDev* units, int B, int r0, int R, int N, int which, uint8_t* out,
                          hipStream_t st) {
   const int64_t n = (int64_t)R * N;
   if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_unpack, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, units, B, R, N, which, out);
+  hipLaunchKernelGGL(k_unpack, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, units, B, r0, R, N, which, out);
   return hipGetLastError();
 }
 

@@ -21,7 +21,8 @@ EXPORTS = ["qiotk_engine_create", "qiotk_engine_destroy", "qiotk_solve", "qiotk_
            "qiotk_model_destroy", "qiotk_solve_model", "qiotk_solve_batch", "qiotk_free",
            "qiotk_last_error", "qiotk_parse_model", "qiotk_schedule_eval", "qiotk_describe",
            "qiotk_model_create_terms", "qiotk_solve_batch_arrays", "qiotk_model_vars",
-           "qiotk_tune_batch"]
+           "qiotk_tune_batch", "qiotk_server_create", "qiotk_server_submit", "qiotk_server_submit_json",
+           "qiotk_server_wait", "qiotk_server_stats", "qiotk_server_destroy", "qiotk_server_last_error"]
 
 
 def _check_rc(rc):
@@ -70,6 +71,15 @@ def lib():
     L.qiotk_solve_batch_arrays.argtypes = [vp, C.c_char_p, C.c_int32, P(vp), P(C.c_char_p), vp, vp, vp]
     L.qiotk_model_vars.argtypes = [vp, P(C.c_int32), vp]
     L.qiotk_tune_batch.argtypes = [vp, C.c_char_p, C.c_int32, P(vp), P(C.c_char_p), C.c_int32, P(vp)]
+    L.qiotk_server_create.argtypes = [C.c_int, C.c_char_p, P(vp)]
+    L.qiotk_server_submit.argtypes = [vp, C.c_char_p, C.c_char_p, C.c_int, C.c_int64, P(C.c_int64),
+                                      P(C.c_int32), P(C.c_double), P(C.c_int64)]
+    L.qiotk_server_submit_json.argtypes = [vp, C.c_char_p, C.c_char_p, C.c_char_p, P(C.c_int64)]
+    L.qiotk_server_wait.argtypes = [vp, C.c_int64, C.c_double, P(vp)]
+    L.qiotk_server_stats.argtypes = [vp, P(vp)]
+    L.qiotk_server_destroy.argtypes = [vp]
+    L.qiotk_server_destroy.restype = None
+    L.qiotk_server_last_error.restype = C.c_char_p
     return L
 
 
@@ -173,6 +183,63 @@ class Engine:
                 out.append(cfg[o:o + nv])
                 o += nv
         return cost, out, {"init_ms": tm[0], "run_ms": tm[1], "finalize_ms": tm[2], "device_ms": tm[3]}
+
+
+class Server:
+    """Serving loop on its own device context: submit() returns at once;
+    wait() returns a request's response document once it is finished."""
+
+    def __init__(self, device=0, **config):
+        self.h = C.c_void_p()
+        rc = lib().qiotk_server_create(device, json.dumps(config).encode(), C.byref(self.h))
+        if rc:
+            raise SolverError(rc, lib().qiotk_server_last_error().decode())
+
+    def close(self):
+        if self.h:
+            lib().qiotk_server_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def submit(self, target, params, kind, offsets, ids, coeffs):
+        import numpy as np
+        off = np.ascontiguousarray(offsets, dtype=np.int64)
+        ids = np.ascontiguousarray(ids, dtype=np.int32)
+        c = np.ascontiguousarray(coeffs, dtype=np.float64)
+        rid = C.c_int64()
+        rc = lib().qiotk_server_submit(self.h, target.encode(), _enc(params), 0 if kind == "ising" else 1,
+                                       len(c), off.ctypes.data_as(C.POINTER(C.c_int64)),
+                                       ids.ctypes.data_as(C.POINTER(C.c_int32)),
+                                       c.ctypes.data_as(C.POINTER(C.c_double)), C.byref(rid))
+        if rc:
+            raise SolverError(rc, lib().qiotk_server_last_error().decode())
+        return rid.value
+
+    def submit_json(self, target, params, input_data):
+        rid = C.c_int64()
+        rc = lib().qiotk_server_submit_json(self.h, target.encode(), _enc(params), _enc(input_data),
+                                            C.byref(rid))
+        if rc:
+            raise SolverError(rc, lib().qiotk_server_last_error().decode())
+        return rid.value
+
+    def wait(self, request_id, timeout_ms=-1.0):
+        """(status code, response document), or None while unfinished."""
+        out = C.c_void_p()
+        rc = lib().qiotk_server_wait(self.h, request_id, float(timeout_ms), C.byref(out))
+        if not out.value:
+            return None
+        return rc, _take(out)
+
+    def stats(self):
+        out = C.c_void_p()
+        lib().qiotk_server_stats(self.h, C.byref(out))
+        return _take(out)
 
 
 class Model:

@@ -138,3 +138,66 @@ def er_ising_arrays(N, mean_degree=6.0, jmax=3, seed=0):
     ids = np.concatenate([np.arange(N), np.stack([a, b], axis=1).ravel()]).astype(np.int32)
     coeffs = np.concatenate([h, J]).astype(np.float64)
     return offsets, ids, coeffs
+
+
+def _distinct_pairs(rng, N, m):
+    """m distinct unordered pairs (a < b) of 0..N-1, uniformly."""
+    pairs = N * (N - 1) // 2
+    if m * 4 > pairs:
+        iu, ju = np.triu_indices(N, k=1)
+        pick = rng.choice(len(iu), size=m, replace=False)
+        return iu[pick], ju[pick]
+    keys = np.zeros(0, dtype=np.int64)
+    while len(keys) < m:
+        x = rng.integers(0, N, size=(2 * (m - len(keys)) + 16, 2))
+        x = x[x[:, 0] != x[:, 1]]
+        lo, hi = np.minimum(x[:, 0], x[:, 1]), np.maximum(x[:, 0], x[:, 1])
+        keys = np.unique(np.concatenate([keys, lo * N + hi]))
+    keys = keys[rng.permutation(len(keys))[:m]]
+    return keys // N, keys % N
+
+
+def er_ising_f32_arrays(N, mean_degree=8.0, seed=0):
+    """Erdos-Renyi Ising with J ~ N(0,1) rounded to fp32, h = 0, as a term
+    CSR; variables without an edge get a zero one-variable term so that all
+    N exist."""
+    rng = np.random.default_rng(seed)
+    p = min(1.0, mean_degree / max(1, N - 1))
+    m = int(rng.binomial(N * (N - 1) // 2, p))
+    a, b = _distinct_pairs(rng, N, m)
+    J = rng.standard_normal(m).astype(np.float32).astype(np.float64)
+    present = np.zeros(N, dtype=bool)
+    present[a] = True
+    present[b] = True
+    lone = np.nonzero(~present)[0]
+    offsets = np.concatenate([2 * np.arange(m + 1), 2 * m + np.arange(1, len(lone) + 1)]).astype(np.int64)
+    ids = np.concatenate([np.stack([a, b], axis=1).ravel(), lone]).astype(np.int32)
+    coeffs = np.concatenate([J, np.zeros(len(lone))])
+    return offsets, ids, coeffs
+
+
+def random_pubo_arrays(n_vars, n_terms, fractions=(0.1, 0.4, 0.3, 0.2), cmax=10, seed=0):
+    """random_pubo as a term CSR, vectorised: per term `deg` distinct ids
+    drawn uniformly (rows with a repeated id are redrawn), integer
+    coefficients uniform in [-cmax, cmax] \\ {0}, terms in random order."""
+    rng = np.random.default_rng(seed)
+    counts = [int(round(f * n_terms)) for f in fractions]
+    counts[-1] = n_terms - sum(counts[:-1])
+    rows = []
+    for deg, cnt in enumerate(counts, start=1):
+        x = rng.integers(0, n_vars, size=(cnt, deg))
+        while deg > 1:
+            s = np.sort(x, axis=1)
+            bad = np.nonzero((s[:, 1:] == s[:, :-1]).any(axis=1))[0]
+            if len(bad) == 0:
+                break
+            x[bad] = rng.integers(0, n_vars, size=(len(bad), deg))
+        rows.append(x)
+    vals = np.array([v for v in range(-cmax, cmax + 1) if v != 0])
+    order = rng.permutation(n_terms)
+    lens = np.concatenate([np.full(c, d) for d, c in enumerate(counts, start=1)])[order]
+    flat = [r for x in rows for r in x]
+    ids = np.concatenate([flat[k] for k in order]).astype(np.int32) if n_terms else np.zeros(0, np.int32)
+    offsets = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    coeffs = rng.choice(vals, size=n_terms).astype(np.float64)
+    return offsets, ids, coeffs

@@ -44,6 +44,11 @@ def test_no_device_fails_loudly():
         pass
     with pytest.raises(_qmc.QmcError):
         _qmc.Context(0)
+    from qiotoolkit_amd import solver as S
+    with pytest.raises(S.SolverError):
+        S.Engine(0)
+    with pytest.raises(S.SolverError):
+        S.Server(0, max_active=4)
 
 
 def load(name):
