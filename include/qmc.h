@@ -160,7 +160,7 @@ int qmc_batch_wait(qmc_batch b, double* ms);
  * pointer may be NULL. */
 int qmc_batch_read(qmc_batch b, double* lowest_energy, uint64_t* accepted, uint8_t* lowest_states);
 
-/* Positions processed side by side in a class round (multiple of 64,
+/* Positions processed side by side in a class round (64, 128, 256 or 512,
  * lanes * ceil(replicas_per_group / 8) <= 512; 0 = the problem default).
  * Changes the coloured visit order from the next sweep on. */
 int qmc_walkers_set_lanes(qmc_walkers w, int32_t lanes);

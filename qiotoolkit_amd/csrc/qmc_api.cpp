@@ -124,8 +124,9 @@ struct qmc_ctx_s {
 };
 
 struct EllLayout {
-  DevBuf<int32_t> slice_base, slice_off, slice_w, nbr, coef_i;
-  DevBuf<float> coef_f;
+  DevBuf<int32_t> slice_base;
+  DevBuf<int2> slice;
+  DevBuf<uint2> ent;
 };
 
 struct qmc_problem_s {
@@ -300,8 +301,8 @@ int qmc_problem_create(qmc_ctx ctx, int kind, int32_t n_vars, int64_t n_terms,
   if (!ctx || !out) return fail(QMC_INVALID_ARGUMENT, "null handle");
   if (kind != QMC_ISING && kind != QMC_PUBO) return fail(QMC_INVALID_ARGUMENT, "unknown model kind");
   if (n_vars <= 0 || n_terms <= 0) return fail(QMC_INVALID_ARGUMENT, "empty problem");
-  if (lanes != 0 && (lanes % 64 != 0 || lanes < 64 || lanes > 512))
-    return fail(QMC_INVALID_ARGUMENT, "lanes must be a multiple of 64 in [64, 512]");
+  if (lanes != 0 && lanes != 64 && lanes != 128 && lanes != 256 && lanes != 512)
+    return fail(QMC_INVALID_ARGUMENT, "lanes must be 64, 128, 256 or 512");
   const int32_t N = n_vars;
   std::vector<std::vector<int32_t>> node_terms(N);
   bool all_int = true;
@@ -413,7 +414,13 @@ int qmc_problem_create(qmc_ctx ctx, int kind, int32_t n_vars, int64_t n_terms,
   std::vector<double> tc(coeffs, coeffs + n_terms);
   HIP_TRY(hipSetDevice(ctx->device));
   HIP_TRY(p->d_class_off.upload(p->class_off.data(), p->class_off.size()));
-  HIP_TRY(p->d_var_of_pos.upload(p->var_of_pos.data(), N));
+  {
+    // padded by two class rounds of the widest lane count, so sweep lanes
+    // prefetching past the end of the last class may read it unconditionally
+    std::vector<int32_t> vp(p->var_of_pos);
+    vp.resize((size_t)N + 1024, 0);
+    HIP_TRY(p->d_var_of_pos.upload(vp.data(), vp.size()));
+  }
   HIP_TRY(p->d_pos_of_var.upload(p->pos_of_var.data(), N));
   HIP_TRY(p->d_inc_off.upload(inc_off.data(), inc_off.size()));
   HIP_TRY(p->d_nbr.upload(nbr.data(), nbr.size()));
@@ -460,7 +467,8 @@ int qmc_problem_get_info(qmc_problem p, qmc_problem_info* info) {
 int qmc_problem_order(qmc_problem p, int32_t lanes, int32_t* order, int32_t* class_offsets) {
   if (!p) return fail(QMC_INVALID_ARGUMENT, "null handle");
   if (lanes == 0) lanes = p->lanes;
-  if (lanes < 64 || lanes > 512 || lanes % 64) return fail(QMC_INVALID_ARGUMENT, "bad lane count");
+  if (lanes != 64 && lanes != 128 && lanes != 256 && lanes != 512)
+    return fail(QMC_INVALID_ARGUMENT, "lanes must be 64, 128, 256 or 512");
   const int nclass = (int)p->class_off.size() - 1;
   int32_t k = 0;
   for (int c = 0; c < nclass; ++c) {
@@ -481,53 +489,49 @@ static EllLayout* get_layout(qmc_problem p, int lanes) {
   if (it != p->ell.end()) return it->second.get();
   auto L = std::make_unique<EllLayout>();
   const int nclass = (int)p->class_off.size() - 1;
-  std::vector<int32_t> sbase(nclass + 1, 0), soff, sw, nbr;
-  std::vector<double> coef;
+  const bool i32 = p->coupling == QMC_COUPLING_I32;
+  auto bits = [&](double c) -> uint32_t {
+    if (i32) return (uint32_t)(int32_t)c;
+    const float f = (float)c;
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    return u;
+  };
+  std::vector<int32_t> sbase(nclass + 1, 0);
+  std::vector<int2> slice;
+  std::vector<uint2> ent;
   for (int c = 0; c < nclass; ++c) {
     const int32_t c0 = p->class_off[c], nc = p->class_off[c + 1] - c0;
-    sbase[c] = (int32_t)sw.size();
+    sbase[c] = (int32_t)slice.size();
     for (int32_t k = 0; k * lanes < nc; ++k) {
       int32_t w = 0;
       for (int32_t t = 0; t < lanes && k * lanes + t < nc; ++t) {
         const int32_t q = c0 + k * lanes + t;
         w = std::max(w, p->h_inc_off[q + 1] - p->h_inc_off[q]);
       }
-      soff.push_back((int32_t)nbr.size());
-      sw.push_back(w);
+      slice.push_back(make_int2((int32_t)ent.size(), w));
       for (int32_t e = 0; e < w; ++e)
         for (int32_t t = 0; t < lanes; ++t) {
           const int32_t rho = k * lanes + t;
-          int32_t nb = p->N;
-          double cf = 0.0;
+          uint2 x = make_uint2((uint32_t)p->N, bits(0.0));
           if (rho < nc) {
             const int32_t q = c0 + rho;
             const int32_t e0 = p->h_inc_off[q], deg = p->h_inc_off[q + 1] - e0;
-            if (e < deg) {
-              nb = p->h_nbr[e0 + e];
-              cf = p->h_coef[e0 + e];
-            }
+            if (e < deg) x = make_uint2((uint32_t)p->h_nbr[e0 + e], bits(p->h_coef[e0 + e]));
           }
-          nbr.push_back(nb);
-          coef.push_back(cf);
+          ent.push_back(x);
         }
     }
   }
-  sbase[nclass] = (int32_t)sw.size();
-  soff.push_back((int32_t)nbr.size());
-  sw.push_back(0);
-  nbr.push_back(p->N);
-  coef.push_back(0.0);
-  bool ok = L->slice_base.upload(sbase.data(), sbase.size()) == hipSuccess &&
-            L->slice_off.upload(soff.data(), soff.size()) == hipSuccess &&
-            L->slice_w.upload(sw.data(), sw.size()) == hipSuccess &&
-            L->nbr.upload(nbr.data(), nbr.size()) == hipSuccess;
-  if (p->coupling == QMC_COUPLING_I32) {
-    std::vector<int32_t> ci(coef.begin(), coef.end());
-    ok = ok && L->coef_i.upload(ci.data(), ci.size()) == hipSuccess;
-  } else {
-    std::vector<float> cf(coef.begin(), coef.end());
-    ok = ok && L->coef_f.upload(cf.data(), cf.size()) == hipSuccess;
-  }
+  sbase[nclass] = (int32_t)slice.size();
+  // two sentinel slices and 8 rows of tail padding: the sweep prefetches up
+  // to two rounds past a class end and always reads 8 rows of a slice
+  slice.push_back(make_int2((int32_t)ent.size(), 0));
+  slice.push_back(make_int2((int32_t)ent.size(), 0));
+  ent.insert(ent.end(), (size_t)8 * lanes + 1, make_uint2((uint32_t)p->N, bits(0.0)));
+  const bool ok = L->slice_base.upload(sbase.data(), sbase.size()) == hipSuccess &&
+                  L->slice.upload(slice.data(), slice.size()) == hipSuccess &&
+                  L->ent.upload(ent.data(), ent.size()) == hipSuccess;
   if (!ok) return nullptr;
   EllLayout* raw = L.get();
   p->ell[lanes] = std::move(L);
@@ -542,10 +546,8 @@ static void build_units(qmc_walkers w) {
     UnitDev& U = w->units[u];
     if (ell) {
       U.ell_slice_base = ell->slice_base.p;
-      U.ell_slice_off = ell->slice_off.p;
-      U.ell_slice_w = ell->slice_w.p;
-      U.ell_nbr = ell->nbr.p;
-      U.ell_coef = p->coupling == QMC_COUPLING_I32 ? (const void*)ell->coef_i.p : (const void*)ell->coef_f.p;
+      U.ell_slice = ell->slice.p;
+      U.ell_ent = ell->ent.p;
     }
     U.class_off = p->d_class_off.p;
     U.var_of_pos = p->d_var_of_pos.p;
@@ -558,6 +560,7 @@ static void build_units(qmc_walkers w) {
     U.N = p->N;
     U.n_classes = (int32_t)p->class_off.size() - 1;
     U.lanes = w->lanes;
+    U.lanes_log2 = 31 - __builtin_clz((unsigned)w->lanes);
     U.G = w->G;
     U.stride = w->stride;
     U.ladder = 0;
@@ -1262,8 +1265,8 @@ int qmc_batch_read(qmc_batch b, double* lowest_energy, uint64_t* accepted, uint8
 int qmc_walkers_set_lanes(qmc_walkers w, int32_t lanes) {
   if (!w) return fail(QMC_INVALID_ARGUMENT, "null handle");
   if (lanes == 0) lanes = w->prob->lanes;
-  if (lanes < 64 || lanes > 512 || lanes % 64 || lanes * w->G > 512)
-    return fail(QMC_INVALID_ARGUMENT, "lanes must be a multiple of 64 with lanes * ceil(B/8) <= 512");
+  if ((lanes != 64 && lanes != 128 && lanes != 256 && lanes != 512) || lanes * w->G > 512)
+    return fail(QMC_INVALID_ARGUMENT, "lanes must be 64, 128, 256 or 512 with lanes * ceil(B/8) <= 512");
   w->lanes = lanes;
   drop_graph(w);
   return upload_units(w);

@@ -32,13 +32,13 @@ struct ProblemDev {
 // One work-group's share: up to 32 replicas of one problem.
 struct UnitDev {
   // problem: sliced ELL of the pairwise form for this unit's lane count --
-  // slice (class c, round k) = ell_slice_base[c] + k holds width w entries per
-  // lane at ell_*[off + e*lanes + lane], padded with the ghost position
+  // slice (class c, round k) = ell_slice_base[c] + k; ell_slice[slice] =
+  // {offset, width}; entry e of lane t is ell_ent[offset + e*lanes + t] =
+  // {neighbour position, coefficient bits (int32 or fp32)}, padded with the
+  // ghost position and a zero coefficient
   const int32_t* ell_slice_base;
-  const int32_t* ell_slice_off;
-  const int32_t* ell_slice_w;
-  const int32_t* ell_nbr;
-  const void* ell_coef;
+  const int2* ell_slice;
+  const uint2* ell_ent;
   const int32_t* class_off;
   const int32_t* var_of_pos;
   const int32_t* pos_of_var;
@@ -48,6 +48,7 @@ struct UnitDev {
   const int32_t* oth_off;
   const int32_t* oth_pos;
   int32_t N, n_classes, lanes, G;
+  int32_t lanes_log2;
   int32_t stride;   // bytes of one unit's spin array, multiple of 4
   int32_t ladder;   // > 0: PT ladder of this many replicas (exchanges in the kernel)
   int32_t slot0;    // ladder slot of the unit's first replica

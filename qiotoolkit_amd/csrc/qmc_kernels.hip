@@ -68,68 +68,101 @@ template <> struct RedSlots<int32_t> {
   __device__ static int32_t* mn(Smem& s, int g, int b) { return s.wmin_i[g][b]; }
 };
 
-// Local field of the 8 replicas of byte lane g at position q:
+// Global (addrspace 1) and constant (addrspace 4, read-only for the whole
+// launch: scalar loads when the address is uniform) views of device memory.
+// Plain pointers loaded from the unit descriptor are generic, and generic
+// loads are flat loads that also wait on LDS traffic.
+template <class T> using gptr = const __attribute__((address_space(1))) T*;
+template <class T> using cptr = const __attribute__((address_space(4))) T*;
+template <class T> __device__ __forceinline__ gptr<T> glob(const T* p) { return (gptr<T>)p; }
+template <class T> __device__ __forceinline__ cptr<T> cnst(const T* p) { return (cptr<T>)p; }
+__device__ __forceinline__ int uni(int x) { return __builtin_amdgcn_readfirstlane(x); }
+
+template <typename CT> __device__ __forceinline__ CT coef_of(uint32_t bits);
+template <> __device__ __forceinline__ float coef_of<float>(uint32_t bits) { return __uint_as_float(bits); }
+template <> __device__ __forceinline__ int32_t coef_of<int32_t>(uint32_t bits) { return (int32_t)bits; }
+
+// The first (up to) 8 sliced-ELL entries of one position of a lane (low
+// word: neighbour position, high word: coefficient bits) and the position's
+// variable id, loaded ahead of use.
+struct EllRow {
+  uint64_t ent[8];
+  uint32_t var;
+};
+
+__device__ __forceinline__ int2 load_slice(const UnitDev& U, int s) {
+  cptr<int32_t> t = cnst(reinterpret_cast<const int32_t*>(U.ell_slice));
+  return make_int2(t[2 * s], t[2 * s + 1]);
+}
+
+// slice = {offset, width} (uniform); lane tp of a slice with `lanes` = 1 << lg.
+// Always 8 loads, without branches, so the compiler can keep them in flight
+// across the processing of the previous row: entries past the slice width
+// belong to the next slice (or the tail padding) and are masked at use.
+__device__ __forceinline__ void ell_fetch(const UnitDev& U, int2 slice, int tp, int lg, EllRow& e) {
+  gptr<uint64_t> p = glob(reinterpret_cast<const uint64_t*>(U.ell_ent)) + slice.x + tp;
+  const int step = 1 << lg;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    e.ent[j] = *p;
+    p += step;
+  }
+}
+
+template <int KIND, typename CT, int NB>
+__device__ __forceinline__ void accumulate(uint32_t wv, CT cc, CT (&h)[8]) {
+#pragma unroll
+  for (int b = 0; b < NB; ++b) {
+    const bool bit = (wv >> b) & 1u;
+    if (KIND == QMC_ISING) h[b] += bit ? -cc : cc;
+    else h[b] += bit ? CT(0) : cc;
+  }
+}
+
+// Local field of the replicas of byte lane g at position q:
 //   Ising: h_b = sum_e c_e * (-1)^(parity of the other members, replica b)
 //   PUBO : h_b = sum_e c_e * [all other members are 1 in replica b]
 // Sums run over the variable's incident terms in term order, as the
-// reference's calculate_cost_difference does.
+// reference's calculate_cost_difference does.  Pairwise problems read the
+// prefetched ELL row (entries beyond the first 8 are loaded here).
 template <int KIND, bool PAIR, typename CT, int NB>
-__device__ __forceinline__ void local_field(const UnitDev& U, const uint8_t* sp, int q, int slice, int tp,
-                                            int G, int g, CT (&h)[8]) {
+__device__ __forceinline__ void local_field(const UnitDev& U, const uint8_t* sp, int q, const EllRow& e,
+                                            int2 slice, int tp, int lg, int G, int g, CT (&h)[8]) {
 #pragma unroll
   for (int b = 0; b < NB; ++b) h[b] = CT(0);
   if (PAIR) {
-    // sliced ELL: the slice width is wave-uniform; entries of one lane are
-    // `lanes` apart, so each batch of loads is coalesced and independent
-    const int base = U.ell_slice_off[slice];
-    const int w = U.ell_slice_w[slice];
-    const int TP = U.lanes;
-    const CT* coef = static_cast<const CT*>(U.ell_coef);
-    for (int e0 = 0; e0 < w; e0 += 8) {
-      int nb[8];
-      CT cc[8];
+    uint32_t wv[8];
 #pragma unroll
-      for (int j = 0; j < 8; ++j) {
-        if (e0 + j < w) {
-          const int idx = base + (e0 + j) * TP + tp;
-          nb[j] = U.ell_nbr[idx];
-          cc[j] = coef[idx];
-        }
-      }
-      uint32_t wv[8];
+    for (int j = 0; j < 8; ++j) wv[j] = sp[(int)(uint32_t)e.ent[j] * G + g];
 #pragma unroll
-      for (int j = 0; j < 8; ++j)
-        if (e0 + j < w) wv[j] = sp[nb[j] * G + g];
-#pragma unroll
-      for (int j = 0; j < 8; ++j) {
-        if (e0 + j < w) {
-#pragma unroll
-          for (int b = 0; b < NB; ++b) {
-            const bool bit = (wv[j] >> b) & 1u;
-            if (KIND == QMC_ISING) h[b] += bit ? -cc[j] : cc[j];
-            else h[b] += bit ? CT(0) : cc[j];
-          }
-        }
+    for (int j = 0; j < 8; ++j) {
+      // a zero coefficient past the width adds nothing (at most the sign of a zero field)
+      const uint32_t cb = j < slice.y ? (uint32_t)(e.ent[j] >> 32) : 0u;
+      accumulate<KIND, CT, NB>(wv[j], coef_of<CT>(cb), h);
+    }
+    if (slice.y > 8) {
+      gptr<uint64_t> p = glob(reinterpret_cast<const uint64_t*>(U.ell_ent)) + slice.x + tp;
+      for (int j = 8; j < slice.y; ++j) {
+        const uint64_t t = p[j << lg];
+        accumulate<KIND, CT, NB>(sp[(int)(uint32_t)t * G + g], coef_of<CT>((uint32_t)(t >> 32)), h);
       }
     }
     return;
   }
-  const CT* coef = static_cast<const CT*>(U.coef);
-  const int e0 = U.inc_off[q], e1 = U.inc_off[q + 1];
+  gptr<CT> coef = glob(static_cast<const CT*>(U.coef));
+  gptr<int32_t> inc = glob(U.inc_off);
+  gptr<int32_t> ooff = glob(U.oth_off);
+  gptr<int32_t> opos = glob(U.oth_pos);
+  const int e0 = inc[q], e1 = inc[q + 1];
   for (int e = e0; e < e1; ++e) {
     const CT cc = coef[e];
     uint32_t w = 0;
-    const int o0 = U.oth_off[e], o1 = U.oth_off[e + 1];
+    const int o0 = ooff[e], o1 = ooff[e + 1];
     for (int o = o0; o < o1; ++o) {
-      const uint32_t x = sp[U.oth_pos[o] * G + g];
+      const uint32_t x = sp[opos[o] * G + g];
       w = (KIND == QMC_ISING) ? (w ^ x) : (w | x);
     }
-#pragma unroll
-    for (int b = 0; b < NB; ++b) {
-      const bool bit = (w >> b) & 1u;
-      if (KIND == QMC_ISING) h[b] += bit ? -cc : cc;
-      else h[b] += bit ? CT(0) : cc;
-    }
+    accumulate<KIND, CT, NB>(w, cc, h);
   }
 }
 
@@ -141,10 +174,12 @@ __device__ __forceinline__ CT delta_from_field(CT h, uint32_t own_bit) {
 
 // Single-replica dE recomputation (rare path: locating the lowest state).
 template <int KIND, bool PAIR, typename CT, int NB>
-__device__ CT delta_one(const UnitDev& U, const uint8_t* sp, int q, int slice, int tp, int G, int g,
+__device__ CT delta_one(const UnitDev& U, const uint8_t* sp, int q, int2 slice, int tp, int lg, int G, int g,
                         int b, uint32_t own_bit) {
+  EllRow e;
+  if (PAIR) ell_fetch(U, slice, tp, lg, e);
   CT h[8];
-  local_field<KIND, PAIR, CT, NB>(U, sp, q, slice, tp, G, g, h);
+  local_field<KIND, PAIR, CT, NB>(U, sp, q, e, slice, tp, lg, G, g, h);
   CT hb = h[0];
 #pragma unroll
   for (int k = 1; k < NB; ++k)
@@ -155,13 +190,14 @@ __device__ CT delta_one(const UnitDev& U, const uint8_t* sp, int q, int slice, i
 // Barrier of the work-groups of one PT ladder (cooperative launches only, so
 // every member is resident).  Bounded: after ~2 s of waiting the error flag
 // is raised and all members abandon the run instead of hanging.
-// The exchanged energies travel as agent-scope atomics; one thread per
-// work-group publishes them with an agent-scope release fence before it
-// arrives and takes an agent-scope acquire fence after the wait.  (Relaxed
-// atomics alone are not enough across the XCDs: without the fences a
-// work-group can read an energy of the previous step.)  Only one thread per
-// work-group fences, because each agent-scope fence writes back and
-// invalidates the L2 of its XCD.
+// Only the exchanged energies cross work-groups, and they travel as
+// agent-scope atomics (coherent across the XCDs).  What orders them is the
+// writer waiting for its energy stores to complete (wait_stores) before its
+// work-group arrives, and the readers loading them only after the barrier
+// has been passed.  An agent-scope fence would also order them, but it writes
+// back and invalidates the whole L2 of the XCD, so every following read of
+// the coupling graph would miss; without either, a work-group can read an
+// energy of the previous step.
 __device__ bool ladder_barrier(const UnitDev& U, int64_t target) {
   __hip_atomic_fetch_add(U.sync, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
@@ -175,6 +211,10 @@ __device__ bool ladder_barrier(const UnitDev& U, int64_t target) {
   }
   return true;
 }
+
+// s_waitcnt vmcnt(0): every vector memory access of this lane issued so far,
+// stores included, has completed.
+__device__ __forceinline__ void wait_stores() { __builtin_amdgcn_s_waitcnt(0x0F70); }
 
 __device__ __forceinline__ void store_agent(double* p, double v) {
   __hip_atomic_store(reinterpret_cast<unsigned long long*>(p), (unsigned long long)__double_as_longlong(v),
@@ -191,8 +231,11 @@ k_sweep(const UnitDev* __restrict__ units, SweepArgs a) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   Smem& S = *reinterpret_cast<Smem*>(smem_raw);
   const UnitDev U = units[a.unit_of_block ? a.unit_of_block[blockIdx.x] : (int)blockIdx.x];
-  const int N = U.N, G = U.G, TP = U.lanes;
+  const int N = U.N, G = U.G, TP = U.lanes, lg = U.lanes_log2;
   const int stride = U.stride;
+  cptr<int32_t> class_off = cnst(U.class_off);
+  cptr<int32_t> slice_base = cnst(U.ell_slice_base);
+  gptr<int32_t> var_of_pos = glob(U.var_of_pos);
   uint8_t* sp = smem_raw + sizeof(Smem);
   uint8_t* fl = sp + stride;
   const int tid = threadIdx.x;
@@ -243,10 +286,10 @@ k_sweep(const UnitDev* __restrict__ units, SweepArgs a) {
     for (int b = 0; b < NB; ++b) beta[b] = (b < nb_g) ? S.rbeta[8 * g + b] : 0.0;
 
     for (int c = 0; c < U.n_classes; ++c) {
-      const int c0 = U.class_off[c];
-      const int nc = U.class_off[c + 1] - c0;
-      const int K = (nc + TP - 1) / TP;
-      const int sbase = PAIR ? U.ell_slice_base[c] : 0;
+      const int c0 = class_off[c];
+      const int nc = class_off[c + 1] - c0;
+      const int K = (nc + TP - 1) >> lg;
+      const int sbase = PAIR ? slice_base[c] : 0;
       CT Ssum[8], Smin[8];
 #pragma unroll
       for (int b = 0; b < NB; ++b) {
@@ -254,24 +297,39 @@ k_sweep(const UnitDev* __restrict__ units, SweepArgs a) {
         Smin[b] = CT(0);
       }
       if (active) {
-        for (int k = 0; k < K; ++k) {
-          const int rho = k * TP + tp;
-          if (rho >= nc) break;
+        // Position k of this lane is rho = k * lanes + tp.  The rounds are
+        // uniform; the graph row of round k + 1 is fetched into the other
+        // buffer while round k is processed.
+        auto fetch = [&](int k, int2& sl, EllRow& e) {
+          const int rho = (k << lg) + tp;
+          if (PAIR) {
+            sl = load_slice(U, sbase + k);
+            ell_fetch(U, sl, tp, lg, e);
+          }
+          e.var = (uint32_t)var_of_pos[c0 + rho];  // padded past position N - 1
+        };
+        auto process = [&](int k, const int2& sl, const EllRow& e) {
+          const int rho = (k << lg) + tp;
           const int q = c0 + rho;
-          const uint32_t var = (uint32_t)U.var_of_pos[q];
-          const uint32_t own = sp[q * G + g];
+          // Pairwise rows: the field is formed on every lane (past the class
+          // end the row is ghost padding), so the row's loads are consumed on
+          // all paths.  The term lists of general problems end at N - 1.
+          if (!PAIR && rho >= nc) return;
           CT h[8];
-          local_field<KIND, PAIR, CT, NB>(U, sp, q, sbase + k, tp, G, g, h);
+          local_field<KIND, PAIR, CT, NB>(U, sp, q, e, sl, tp, lg, G, g, h);
+          if (rho >= nc) return;
+          const uint32_t var = e.var;
+          const uint32_t own = sp[q * G + g];
           uint32_t flips = 0;
           uint4 quad = make_uint4(0, 0, 0, 0);
 #pragma unroll
           for (int b = 0; b < NB; ++b) {
             if (b < nb_g) {
               const uint32_t r = rbase + (uint32_t)b;
-              if (b == 0 || (r & 3u) == 0u)
-                quad = philox4x32_10(var, sweep, r >> 2, DOM_SWEEP, U.seed, U.stream);
               const uint32_t ob = (own >> b) & 1u;
               const CT dE = delta_from_field<KIND, CT>(h[b], ob);
+              if (b == 0 || (r & 3u) == 0u)
+                quad = philox4x32_10(var, sweep, r >> 2, DOM_SWEEP, U.seed, U.stream);
               const bool ok = metropolis_accept((double)dE, beta[b], pick(quad, r & 3u), S.lut, [&]() {
                 return philox4x32_10(var, sweep, r, DOM_REDRAW, U.seed, U.stream);
               });
@@ -285,6 +343,19 @@ k_sweep(const UnitDev* __restrict__ units, SweepArgs a) {
           }
           sp[q * G + g] = (uint8_t)(own ^ flips);
           fl[q * G + g] = (uint8_t)flips;
+        };
+        // Fetches past the last round read sentinel slices and padding (the
+        // tables are padded for it), so no fetch sits behind a branch and
+        // the loads stay in flight while the other buffer is processed.
+        EllRow ea, eb;
+        int2 sla = make_int2(0, 0), slb = make_int2(0, 0);
+        fetch(0, sla, ea);
+        for (int k = 0; k < K; k += 2) {
+          fetch(k + 1, slb, eb);
+          process(k, sla, ea);
+          if (k + 1 >= K) break;
+          fetch(k + 2, sla, ea);
+          process(k + 1, slb, eb);
         }
       }
       __syncthreads();
@@ -359,13 +430,14 @@ k_sweep(const UnitDev* __restrict__ units, SweepArgs a) {
               int ks = -1;  // Smin == 0: the empty prefix is the first minimum
               if (Smin[b] < CT(0)) {
                 for (int k = 0; k < K; ++k) {
-                  const int rho = k * TP + tp;
+                  const int rho = (k << lg) + tp;
                   if (rho >= nc) break;
                   const int q = c0 + rho;
                   const uint32_t f = fl[q * G + g];
                   if ((f >> b) & 1u) {
                     const uint32_t ob = ((sp[q * G + g] ^ f) >> b) & 1u;  // pre-flip bit
-                    pre += delta_one<KIND, PAIR, CT, NB>(U, sp, q, sbase + k, tp, G, g, b, ob);
+                    const int2 sl = PAIR ? load_slice(U, sbase + k) : make_int2(0, 0);
+                    pre += delta_one<KIND, PAIR, CT, NB>(U, sp, q, sl, tp, lg, G, g, b, ob);
                     if (pre == Smin[b]) {
                       ks = k;
                       break;
@@ -390,7 +462,7 @@ k_sweep(const UnitDev* __restrict__ units, SweepArgs a) {
               const int rho = q - c0;
               if (rho >= 0 && rho < nc) {
                 const uint32_t f = fl[q * G + g];
-                const int tq = rho % TP, kq = rho / TP;
+                const int tq = rho & (TP - 1), kq = rho >> lg;
 #pragma unroll
                 for (int b = 0; b < NB; ++b) {
                   if ((newmask >> b) & 1u) {
@@ -419,17 +491,12 @@ k_sweep(const UnitDev* __restrict__ units, SweepArgs a) {
       const int L = U.ladder;
       if (U.upl > 1) {
         double* cb = U.cbuf + (step64 & 1) * L;  // double-buffered by step parity
-        if (tid < nrep) store_agent(cb + U.slot0 + tid, S.rcost[tid]);
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-        __syncthreads();
-        if (tid == 0) {
-          // release: the work-group's energy stores before the arrival
-          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-          if (!ladder_barrier(U, (step64 + 1) * U.upl)) S.abort_flag = 1;
-          // acquire: the other work-groups' energies after the barrier
-          __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        if (tid < nrep) {
+          store_agent(cb + U.slot0 + tid, S.rcost[tid]);
+          wait_stores();  // visible to the ladder before this work-group arrives
         }
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+        __syncthreads();
+        if (tid == 0 && !ladder_barrier(U, (step64 + 1) * U.upl)) S.abort_flag = 1;
         __syncthreads();
         if (S.abort_flag) break;
         if (tid < L) S.lcost[tid] = load_agent(cb + tid);
