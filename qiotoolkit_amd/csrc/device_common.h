@@ -55,11 +55,11 @@ __device__ __forceinline__ double div_ffffe(uint32_t k) {
 
 // Metropolis acceptance with the reference's 4x256 ln table
 // (markov/metropolis.h:64-104, utils/utils.cc:12-32).  `w0` is the first draw
-// of the attempt; `redraw` yields rows 1..3 (only when bucket 0 is hit).
-// lut: 1024 doubles, row-major.
-template <class Redraw>
-__device__ __forceinline__ bool metropolis_accept(double cost_diff, double beta, uint32_t w0,
-                                                  const double* lut, Redraw&& redraw) {
+// of the attempt; `redraw` yields rows 1..3 (only when bucket 0 is hit);
+// lut(row, i) reads entry i of table row `row`.
+template <class Lut, class Redraw>
+__device__ __forceinline__ bool metropolis_accept(double cost_diff, double beta, uint32_t w0, Lut&& lut,
+                                                  Redraw&& redraw) {
   if (cost_diff <= 0.0) return true;
   uint32_t r = w0;
   uint32_t bucket = r & 255u;
@@ -86,8 +86,8 @@ __device__ __forceinline__ bool metropolis_accept(double cost_diff, double beta,
   // never accept; bucket 1 interpolates from ln(0) = -inf and yields NaN,
   // which never accepts either.
   if (row == 0 && (bucket == 1u || t < -5.6)) return false;
-  const double y0 = lut[row * 256 + (int)bucket - 1];
-  const double y1 = lut[row * 256 + (int)bucket];
+  const double y0 = lut(row, (int)bucket - 1);
+  const double y1 = lut(row, (int)bucket);
   const double distance = div_ffffe(r >> 8);
   const double value = __dadd_rn(y0, __dmul_rn(distance, __dsub_rn(y1, y0)));
   return value < t;

@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <limits>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -190,7 +191,7 @@ struct qmc_walkers_s {
   DevBuf<int64_t> progress;  // 2 per unit: next sweep, sweeps left
   DevBuf<int64_t> last_step; // per PT ladder: last exchanged step
   DevBuf<int32_t> sync;      // per PT ladder: barrier counter (cooperative ladders)
-  DevBuf<double> cbuf;       // per PT ladder: [2][L] published energies
+  DevBuf<double> cbuf;       // per PT ladder: [3][L] published energies (NaN: not yet)
   DevBuf<int32_t> err;       // barrier time-out flag
   int coop = 0;              // ladders split over work-groups, exchanged in-kernel
   int force_graph = 0;       // split ladders through graph-replayed launches instead
@@ -587,7 +588,7 @@ static void build_units(qmc_walkers w) {
       U.label = w->label.p + (size_t)ld * L;
       if (U.upl > 1) {
         U.sync = w->sync.p + ld;
-        U.cbuf = w->cbuf.p + (size_t)ld * 2 * L;
+        U.cbuf = w->cbuf.p + (size_t)ld * 3 * L;
       }
     }
     U.spr = w->spr;
@@ -791,7 +792,7 @@ static int launch_units(qmc_problem p, int nb, const UnitDev* d_units, int n_uni
   a.n_sweeps = max_sweeps;
   a.lut = p->ctx->lut.p;
   a.unit_of_block = nullptr;
-  const size_t smem = sweep_smem_bytes(stride, ordered);
+  const size_t smem = sweep_smem_bytes(stride, ordered, &a.best_lds);
   HIP_TRY(launch_sweep(p->kind, p->pairwise, p->coupling, nb, d_units, n_units, block, smem, a, ordered,
                        coop, st));
   return QMC_OK;
@@ -855,7 +856,10 @@ int qmc_walkers_plan_pt(qmc_walkers w, int32_t ladder, const double* ladder_beta
     // ladders split over work-groups exchange inside a cooperative launch;
     // the barrier counter holds steps * work-groups-per-ladder
     if (w->sync.n < (size_t)n_ladders) HIP_TRY(w->sync.alloc(n_ladders));
-    if (w->cbuf.n < (size_t)n_ladders * 2 * ladder) HIP_TRY(w->cbuf.alloc((size_t)n_ladders * 2 * ladder));
+    if (w->cbuf.n < (size_t)n_ladders * 3 * ladder) HIP_TRY(w->cbuf.alloc((size_t)n_ladders * 3 * ladder));
+    // every energy slot starts "not yet published"
+    std::vector<double> nan((size_t)n_ladders * 3 * ladder, std::numeric_limits<double>::quiet_NaN());
+    HIP_TRY(hipMemcpy(w->cbuf.p, nan.data(), nan.size() * sizeof(double), hipMemcpyHostToDevice));
     std::vector<int32_t> ctr(n_ladders, (int32_t)(step0 * upl));
     HIP_TRY(hipMemcpy(w->sync.p, ctr.data(), n_ladders * sizeof(int32_t), hipMemcpyHostToDevice));
     HIP_TRY(hipMemset(w->err.p, 0, sizeof(int32_t)));
@@ -1088,7 +1092,7 @@ static int launch_group(qmc_batch_s::Group& g, int32_t max_sweeps, const double*
   a.n_sweeps = max_sweeps;
   a.lut = lut;
   a.unit_of_block = nullptr;
-  const size_t smem = sweep_smem_bytes(g.stride, 0);
+  const size_t smem = sweep_smem_bytes(g.stride, 0, &a.best_lds);
   if (smem > 160 * 1024) return fail(QMC_MEMORY_LIMITED, "batch LDS footprint exceeds 160 KiB");
   const int n = (int)g.units.size();
   if (!g.coop) {

@@ -54,7 +54,7 @@ struct UnitDev {
   int32_t slot0;    // ladder slot of the unit's first replica
   int32_t upl;      // work-groups per ladder (> 1: cooperative, per-step barrier)
   int32_t* sync;    // per-ladder barrier counter (upl > 1)
-  double* cbuf;     // per-ladder [2][ladder] published energies (upl > 1)
+  double* cbuf;     // per-ladder [3][ladder] published energies, NaN = not yet (upl > 1)
   int32_t* err;     // set when a barrier wait times out
   // state (unit-local pointers: element 0 is the unit's first replica)
   uint8_t* spins;
@@ -77,12 +77,14 @@ struct UnitDev {
 
 struct SweepArgs {
   int32_t n_sweeps;      // at most this many sweeps per unit in this launch
-  int32_t pad;
+  int32_t best_lds;      // lowest states kept in LDS during the launch (else in HBM)
   const double* lut;     // device, 1024 values
   const int32_t* unit_of_block;  // optional block -> unit map (null: identity)
 };
 
-size_t sweep_smem_bytes(int max_stride, int ordered);
+// Dynamic LDS of a sweep launch whose largest unit spin array is max_stride
+// bytes; *best_lds (optional) tells whether the lowest states fit in it too.
+size_t sweep_smem_bytes(int max_stride, int ordered, int* best_lds = nullptr);
 hipError_t launch_sweep(int kind, int pairwise, int coupling, int nb, const UnitDev* units, int n_units,
                         int block, size_t smem, const SweepArgs& a, int ordered, int cooperative,
                         hipStream_t st);

@@ -31,11 +31,16 @@ constexpr int kMaxRep = 32;          // replicas per unit
 constexpr int kMaxWavesPerGroup = 8; // lanes <= 512
 constexpr int kBlock = 512;
 
-template <typename CT>
-__device__ __forceinline__ CT min_ct(CT a, CT b) { return b < a ? b : a; }
+template <typename T>
+__device__ __forceinline__ T min_ct(T a, T b) { return b < a ? b : a; }
+
+// Accumulator of a class's accepted dE: exact int32 for integer couplings,
+// double for fp32 couplings (the reference accumulates its cost in double).
+template <typename CT> struct AccT { using T = int32_t; };
+template <> struct AccT<float> { using T = double; };
 
 struct Smem {
-  double lut[1024];
+  double lut0[256];         // row 0 of the ln table (rows 1..3 are read from HBM)
   double rcost[kMaxRep];
   double rbest[kMaxRep];
   double rbeta[kMaxRep];
@@ -44,29 +49,80 @@ struct Smem {
   int32_t inv[kMaxRep];     // ladder slot -> beta index
   double lcost[kMaxRep];    // ladder energies at the exchange
   int32_t abort_flag;
-  int32_t pad2_;
-  uint32_t racc[kMaxRep];
-  int32_t nb_tp[kMaxRep];   // thread lane holding the new-best argmin, -1 none
-  int32_t kstar[kMaxRep];
   int32_t any_new_best;
-  int32_t pad_[3];
-  // reduction scratch [group][replica-in-group][wave]
-  float wsum_f[4][8][kMaxWavesPerGroup];
-  float wmin_f[4][8][kMaxWavesPerGroup];
-  int32_t wsum_i[4][8][kMaxWavesPerGroup];
-  int32_t wmin_i[4][8][kMaxWavesPerGroup];
+  uint32_t racc[kMaxRep];
+  int32_t nb_tp[kMaxRep];   // lane of the new lowest cost in the class sequence, -1 none
+  int32_t kstar[kMaxRep];   // its round (-1: before the lane's first flip)
+  // per-wave partials [byte group][replica in group][wave]
+  union {
+    double d[4][8][kMaxWavesPerGroup];
+    int32_t i[4][8][kMaxWavesPerGroup];
+  } wsum, wmin;
   int32_t wlane[4][8][kMaxWavesPerGroup];
+  int32_t wk[4][8][kMaxWavesPerGroup];
 };
 
-template <typename CT> struct RedSlots;
-template <> struct RedSlots<float> {
-  __device__ static float* sum(Smem& s, int g, int b) { return s.wsum_f[g][b]; }
-  __device__ static float* mn(Smem& s, int g, int b) { return s.wmin_f[g][b]; }
+template <typename AT> struct Slots;
+template <> struct Slots<double> {
+  __device__ static double* sum(Smem& s, int g, int b) { return s.wsum.d[g][b]; }
+  __device__ static double* mn(Smem& s, int g, int b) { return s.wmin.d[g][b]; }
 };
-template <> struct RedSlots<int32_t> {
-  __device__ static int32_t* sum(Smem& s, int g, int b) { return s.wsum_i[g][b]; }
-  __device__ static int32_t* mn(Smem& s, int g, int b) { return s.wmin_i[g][b]; }
+template <> struct Slots<int32_t> {
+  __device__ static int32_t* sum(Smem& s, int g, int b) { return s.wsum.i[g][b]; }
+  __device__ static int32_t* mn(Smem& s, int g, int b) { return s.wmin.i[g][b]; }
 };
+
+// ---------------------------------------------------------------------------
+// Wavefront scans with DPP (gfx9 row_shr / row_bcast / wave_shr): lanes
+// without a source take `ident`.
+enum : int { DPP_ROW_SHR1 = 0x111, DPP_ROW_SHR2 = 0x112, DPP_ROW_SHR4 = 0x114, DPP_ROW_SHR8 = 0x118,
+             DPP_WAVE_SHR1 = 0x138, DPP_ROW_BCAST15 = 0x142, DPP_ROW_BCAST31 = 0x143 };
+
+template <int CTRL, int ROWS>
+__device__ __forceinline__ int32_t dpp(int32_t x, int32_t ident) {
+  return __builtin_amdgcn_update_dpp(ident, x, CTRL, ROWS, 0xf, false);
+}
+template <int CTRL, int ROWS>
+__device__ __forceinline__ double dpp(double x, double ident) {
+  const long long xi = __double_as_longlong(x), oi = __double_as_longlong(ident);
+  const int lo = __builtin_amdgcn_update_dpp((int)oi, (int)xi, CTRL, ROWS, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp((int)(oi >> 32), (int)(xi >> 32), CTRL, ROWS, 0xf, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+template <typename AT>
+__device__ __forceinline__ AT scan_add(AT x) {  // inclusive prefix sum over the wave
+  x += dpp<DPP_ROW_SHR1, 0xf>(x, AT(0));
+  x += dpp<DPP_ROW_SHR2, 0xf>(x, AT(0));
+  x += dpp<DPP_ROW_SHR4, 0xf>(x, AT(0));
+  x += dpp<DPP_ROW_SHR8, 0xf>(x, AT(0));
+  x += dpp<DPP_ROW_BCAST15, 0xa>(x, AT(0));
+  x += dpp<DPP_ROW_BCAST31, 0xc>(x, AT(0));
+  return x;
+}
+
+template <typename AT> __device__ __forceinline__ AT max_of();
+template <> __device__ __forceinline__ double max_of<double>() { return __builtin_huge_val(); }
+template <> __device__ __forceinline__ int32_t max_of<int32_t>() { return 0x7fffffff; }
+
+template <typename AT>
+__device__ __forceinline__ AT scan_min(AT x) {  // inclusive prefix minimum over the wave
+  const AT id = max_of<AT>();
+  x = min_ct(x, dpp<DPP_ROW_SHR1, 0xf>(x, id));
+  x = min_ct(x, dpp<DPP_ROW_SHR2, 0xf>(x, id));
+  x = min_ct(x, dpp<DPP_ROW_SHR4, 0xf>(x, id));
+  x = min_ct(x, dpp<DPP_ROW_SHR8, 0xf>(x, id));
+  x = min_ct(x, dpp<DPP_ROW_BCAST15, 0xa>(x, id));
+  x = min_ct(x, dpp<DPP_ROW_BCAST31, 0xc>(x, id));
+  return x;
+}
+
+__device__ __forceinline__ int32_t lane_value(int32_t x, int lane) { return __builtin_amdgcn_readlane(x, lane); }
+__device__ __forceinline__ double lane_value(double x, int lane) {
+  const long long v = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_readlane((int)v, lane), hi = __builtin_amdgcn_readlane((int)(v >> 32), lane);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
 
 // Global (addrspace 1) and constant (addrspace 4, read-only for the whole
 // launch: scalar loads when the address is uniform) views of device memory.
@@ -187,29 +243,36 @@ __device__ CT delta_one(const UnitDev& U, const uint8_t* sp, int q, int2 slice, 
   return delta_from_field<KIND, CT>(hb, own_bit);
 }
 
-// Barrier of the work-groups of one PT ladder (cooperative launches only, so
-// every member is resident).  Bounded: after ~2 s of waiting the error flag
-// is raised and all members abandon the run instead of hanging.
-// Only the exchanged energies cross work-groups, and they travel as
-// agent-scope atomics (coherent across the XCDs).  What orders them is the
-// writer waiting for its energy stores to complete (wait_stores) before its
-// work-group arrives, and the readers loading them only after the barrier
-// has been passed.  An agent-scope fence would also order them, but it writes
-// back and invalidates the whole L2 of the XCD, so every following read of
-// the coupling graph would miss; without either, a work-group can read an
-// energy of the previous step.
-__device__ bool ladder_barrier(const UnitDev& U, int64_t target) {
-  __hip_atomic_fetch_add(U.sync, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+// Energy hand-off between the work-groups of one PT ladder (cooperative
+// launches only, so every member is resident).  Step s publishes each slot's
+// energy into buffer s % 3 of the ladder; a slot holds NaN until its step's
+// value arrives, so readers poll the values themselves and no separate
+// arrival counter is needed.  After reading step s, a work-group resets its
+// own slots of buffer (s + 2) % 3 -- every work-group has finished reading
+// that buffer (step s - 1) by then, since it has published step s -- and the
+// reset is complete before the slot is published again one step later.
+// Energies travel as agent-scope atomics (coherent across the XCDs), and a
+// writer waits for its stores to complete before anything that depends on
+// them; agent-scope fences would also write back and invalidate the whole L2
+// of the XCD, which every later read of the coupling graph would pay for.
+// Bounded: after ~2 s of polling the error flag is raised and all members
+// abandon the run instead of hanging.
+__device__ bool await_energy(const UnitDev& U, const double* slot, double& out) {
   const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
-  while (__hip_atomic_load(U.sync, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+  while (true) {
+    const double v = __longlong_as_double((long long)__hip_atomic_load(
+        reinterpret_cast<const unsigned long long*>(slot), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    if (v == v) {
+      out = v;
+      return true;
+    }
     if (__hip_atomic_load(U.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return false;
     if (__builtin_amdgcn_s_memrealtime() - t0 > 200000000ull) {  // 100 MHz clock: 2 s
       __hip_atomic_store(U.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       return false;
     }
-    __builtin_amdgcn_s_sleep(2);
+    __builtin_amdgcn_s_sleep(1);
   }
-  return true;
 }
 
 // s_waitcnt vmcnt(0): every vector memory access of this lane issued so far,
@@ -228,6 +291,7 @@ __device__ __forceinline__ double load_agent(const double* p) {
 template <int KIND, bool PAIR, typename CT, int NB>
 __global__ void __launch_bounds__(kBlock)
 k_sweep(const UnitDev* __restrict__ units, SweepArgs a) {
+  using AT = typename AccT<CT>::T;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   Smem& S = *reinterpret_cast<Smem*>(smem_raw);
   const UnitDev U = units[a.unit_of_block ? a.unit_of_block[blockIdx.x] : (int)blockIdx.x];
@@ -236,24 +300,32 @@ k_sweep(const UnitDev* __restrict__ units, SweepArgs a) {
   cptr<int32_t> class_off = cnst(U.class_off);
   cptr<int32_t> slice_base = cnst(U.ell_slice_base);
   gptr<int32_t> var_of_pos = glob(U.var_of_pos);
+  gptr<double> lut_all = glob(a.lut);
   uint8_t* sp = smem_raw + sizeof(Smem);
   uint8_t* fl = sp + stride;
+  // lowest states: in LDS for the launch when they fit, else in HBM
+  uint8_t* bst = a.best_lds ? fl + stride : U.best;
   const int tid = threadIdx.x;
   const int nthreads = blockDim.x;
   const bool active = tid < TP * G;
-  const int g = active ? tid / TP : 0;
-  const int tp = tid - g * TP;
+  const int g = active ? tid >> lg : 0;
+  const int tp = tid - (g << lg);
   const int lane = tid & 63;
   const int wave_in_group = tp >> 6;
   const int nrep = (int)U.nrep;
   const int nb_g = active ? min(NB, nrep - 8 * g) : 0;
   const uint32_t rbase = U.r0 + 8u * (uint32_t)g;
 
-  for (int i = tid; i < 1024; i += nthreads) S.lut[i] = a.lut[i];
+  for (int i = tid; i < 256; i += nthreads) S.lut0[i] = lut_all[i];
   {
     const uint32_t* src = reinterpret_cast<const uint32_t*>(U.spins);
     uint32_t* dst = reinterpret_cast<uint32_t*>(sp);
     for (int i = tid; i < stride / 4; i += nthreads) dst[i] = src[i];
+    if (a.best_lds) {
+      const uint32_t* bsrc = reinterpret_cast<const uint32_t*>(U.best);
+      uint32_t* bdst = reinterpret_cast<uint32_t*>(bst);
+      for (int i = tid; i < stride / 4; i += nthreads) bdst[i] = bsrc[i];
+    }
   }
   if (tid < nrep) {
     S.rcost[tid] = U.cost[tid];
@@ -261,7 +333,10 @@ k_sweep(const UnitDev* __restrict__ units, SweepArgs a) {
     S.rbeta[tid] = U.beta[tid];
     S.racc[tid] = 0;
   }
-  if (tid == 0) S.abort_flag = 0;
+  if (tid == 0) {
+    S.abort_flag = 0;
+    S.any_new_best = 0;
+  }
   if (U.ladder > 0 && tid < U.ladder) {
     S.label[tid] = U.label[tid];
     S.lbeta[tid] = U.ladder_beta[tid];
@@ -269,6 +344,8 @@ k_sweep(const UnitDev* __restrict__ units, SweepArgs a) {
   __syncthreads();
   if (U.ladder > 0 && tid < U.ladder) S.inv[S.label[tid]] = tid;
   __syncthreads();
+
+  auto lut = [&](int row, int i) { return row == 0 ? S.lut0[i] : lut_all[row * 256 + i]; };
 
   uint32_t acc[8];
 #pragma unroll
@@ -290,11 +367,15 @@ k_sweep(const UnitDev* __restrict__ units, SweepArgs a) {
       const int nc = class_off[c + 1] - c0;
       const int K = (nc + TP - 1) >> lg;
       const int sbase = PAIR ? slice_base[c] : 0;
-      CT Ssum[8], Smin[8];
+      // per replica: sum of the accepted dE of this lane's positions, the
+      // minimum prefix of that sum and the round where it is first reached
+      AT Ssum[8], Smin[8];
+      int kmin[8];
 #pragma unroll
       for (int b = 0; b < NB; ++b) {
-        Ssum[b] = CT(0);
-        Smin[b] = CT(0);
+        Ssum[b] = AT(0);
+        Smin[b] = AT(0);
+        kmin[b] = -1;
       }
       if (active) {
         // Position k of this lane is rho = k * lanes + tp.  The rounds are
@@ -330,13 +411,16 @@ k_sweep(const UnitDev* __restrict__ units, SweepArgs a) {
               const CT dE = delta_from_field<KIND, CT>(h[b], ob);
               if (b == 0 || (r & 3u) == 0u)
                 quad = philox4x32_10(var, sweep, r >> 2, DOM_SWEEP, U.seed, U.stream);
-              const bool ok = metropolis_accept((double)dE, beta[b], pick(quad, r & 3u), S.lut, [&]() {
+              const bool ok = metropolis_accept((double)dE, beta[b], pick(quad, r & 3u), lut, [&]() {
                 return philox4x32_10(var, sweep, r, DOM_REDRAW, U.seed, U.stream);
               });
               if (ok) {
                 flips |= 1u << b;
-                Ssum[b] += dE;
-                Smin[b] = min_ct(Smin[b], Ssum[b]);
+                Ssum[b] += (AT)dE;
+                if (Ssum[b] < Smin[b]) {  // first time at the new minimum
+                  Smin[b] = Ssum[b];
+                  kmin[b] = k;
+                }
                 acc[b]++;
               }
             }
@@ -357,38 +441,28 @@ k_sweep(const UnitDev* __restrict__ units, SweepArgs a) {
           fetch(k + 2, sla, ea);
           process(k + 1, slb, eb);
         }
-      }
-      __syncthreads();
 
-      // ---- per replica: sequence minimum of the cost over this class ----
-      if (active) {
+        // ---- per replica: minimum of the cost along the class sequence ----
+        // The sequence is lane-major, so lane t's prefixes follow the sums of
+        // all earlier lanes: per wave, an exclusive scan of the sums plus each
+        // lane's minimum prefix, minimised (first lane on ties); the waves
+        // are combined in order below.
 #pragma unroll
         for (int b = 0; b < NB; ++b) {
           if (b < nb_g) {
-            CT incl = Ssum[b];
-#pragma unroll
-            for (int off = 1; off < 64; off <<= 1) {
-              const CT t = __shfl_up(incl, off, 64);
-              if (lane >= off) incl += t;
-            }
-            CT excl = __shfl_up(incl, 1, 64);
-            if (lane == 0) excl = CT(0);
-            CT mv = excl + Smin[b];
-            int ml = lane;
-#pragma unroll
-            for (int off = 32; off >= 1; off >>= 1) {
-              const CT ov = __shfl_xor(mv, off, 64);
-              const int ol = __shfl_xor(ml, off, 64);
-              if (ov < mv || (ov == mv && ol < ml)) {
-                mv = ov;
-                ml = ol;
-              }
-            }
-            const CT tot = __shfl(incl, 63, 64);
+            const AT incl = scan_add<AT>(Ssum[b]);
+            const AT excl = dpp<DPP_WAVE_SHR1, 0xf>(incl, AT(0));
+            const AT mv = excl + Smin[b];
+            const AT m = lane_value(scan_min<AT>(mv), 63);
+            const unsigned long long hit = __builtin_amdgcn_ballot_w64(mv == m);
+            const int wl = (int)__builtin_ctzll(hit);
+            const int wk = __builtin_amdgcn_readlane(kmin[b], wl);
+            const AT tot = lane_value(incl, 63);
             if (lane == 0) {
-              RedSlots<CT>::sum(S, g, b)[wave_in_group] = tot;
-              RedSlots<CT>::mn(S, g, b)[wave_in_group] = mv;
-              S.wlane[g][b][wave_in_group] = ml;
+              Slots<AT>::sum(S, g, b)[wave_in_group] = tot;
+              Slots<AT>::mn(S, g, b)[wave_in_group] = m;
+              S.wlane[g][b][wave_in_group] = wl;
+              S.wk[g][b][wave_in_group] = wk;
             }
           }
         }
@@ -397,16 +471,16 @@ k_sweep(const UnitDev* __restrict__ units, SweepArgs a) {
       if (tid < nrep) {
         const int gg = tid >> 3, bb = tid & 7;
         const int W = TP >> 6;
-        CT run = CT(0);
-        CT bestc = CT(0);
-        int bt = -1;
+        AT run = AT(0), bestc = AT(0);
+        int bt = -1, bk = -1;
         for (int w = 0; w < W; ++w) {
-          const CT cnd = run + RedSlots<CT>::mn(S, gg, bb)[w];
+          const AT cnd = run + Slots<AT>::mn(S, gg, bb)[w];
           if (bt < 0 || cnd < bestc) {
             bestc = cnd;
             bt = w * 64 + S.wlane[gg][bb][w];
+            bk = S.wk[gg][bb][w];
           }
-          run += RedSlots<CT>::sum(S, gg, bb)[w];
+          run += Slots<AT>::sum(S, gg, bb)[w];
         }
         const double before = S.rcost[tid];
         S.rcost[tid] = before + (double)run;
@@ -414,6 +488,7 @@ k_sweep(const UnitDev* __restrict__ units, SweepArgs a) {
         if (cand < S.rbest[tid]) {
           S.rbest[tid] = cand;
           S.nb_tp[tid] = bt;
+          S.kstar[tid] = bk;
           S.any_new_best = 1;
         } else {
           S.nb_tp[tid] = -1;
@@ -421,35 +496,8 @@ k_sweep(const UnitDev* __restrict__ units, SweepArgs a) {
       }
       __syncthreads();
       if (S.any_new_best) {
-        // locate k* in the owning thread, then rebuild the lowest states
-        if (active) {
-#pragma unroll
-          for (int b = 0; b < NB; ++b) {
-            if (b < nb_g && S.nb_tp[8 * g + b] == tp) {
-              CT pre = CT(0);
-              int ks = -1;  // Smin == 0: the empty prefix is the first minimum
-              if (Smin[b] < CT(0)) {
-                for (int k = 0; k < K; ++k) {
-                  const int rho = (k << lg) + tp;
-                  if (rho >= nc) break;
-                  const int q = c0 + rho;
-                  const uint32_t f = fl[q * G + g];
-                  if ((f >> b) & 1u) {
-                    const uint32_t ob = ((sp[q * G + g] ^ f) >> b) & 1u;  // pre-flip bit
-                    const int2 sl = PAIR ? load_slice(U, sbase + k) : make_int2(0, 0);
-                    pre += delta_one<KIND, PAIR, CT, NB>(U, sp, q, sl, tp, lg, G, g, b, ob);
-                    if (pre == Smin[b]) {
-                      ks = k;
-                      break;
-                    }
-                  }
-                }
-              }
-              S.kstar[8 * g + b] = ks;
-            }
-          }
-        }
-        __syncthreads();
+        // lowest state = the class's start state with the flips up to the
+        // minimum applied: every later flip of this class is undone
         if (active) {
           uint32_t newmask = 0;
 #pragma unroll
@@ -457,8 +505,7 @@ k_sweep(const UnitDev* __restrict__ units, SweepArgs a) {
             if (b < nb_g && S.nb_tp[8 * g + b] >= 0) newmask |= 1u << b;
           if (newmask) {
             for (int q = tp; q < N; q += TP) {
-              const uint32_t cur = sp[q * G + g];
-              uint32_t val = cur;
+              uint32_t val = sp[q * G + g];
               const int rho = q - c0;
               if (rho >= 0 && rho < nc) {
                 const uint32_t f = fl[q * G + g];
@@ -472,8 +519,8 @@ k_sweep(const UnitDev* __restrict__ units, SweepArgs a) {
                   }
                 }
               }
-              const uint32_t old = U.best[q * G + g];
-              U.best[q * G + g] = (uint8_t)((old & ~newmask) | (val & newmask));
+              const uint32_t old = bst[q * G + g];
+              bst[q * G + g] = (uint8_t)((old & ~newmask) | (val & newmask));
             }
           }
         }
@@ -490,16 +537,22 @@ k_sweep(const UnitDev* __restrict__ units, SweepArgs a) {
       const uint32_t step = (uint32_t)step64;
       const int L = U.ladder;
       if (U.upl > 1) {
-        double* cb = U.cbuf + (step64 & 1) * L;  // double-buffered by step parity
+        double* cb = U.cbuf + (step64 % 3) * L;
         if (tid < nrep) {
+          wait_stores();  // this slot's reset (one step ago) is complete
           store_agent(cb + U.slot0 + tid, S.rcost[tid]);
-          wait_stores();  // visible to the ladder before this work-group arrives
+        }
+        if (tid < L) {
+          double e = 0.0;
+          if (!await_energy(U, cb + tid, e)) S.abort_flag = 1;
+          S.lcost[tid] = e;
         }
         __syncthreads();
-        if (tid == 0 && !ladder_barrier(U, (step64 + 1) * U.upl)) S.abort_flag = 1;
-        __syncthreads();
         if (S.abort_flag) break;
-        if (tid < L) S.lcost[tid] = load_agent(cb + tid);
+        if (tid < nrep) {
+          const double nan = __longlong_as_double(0x7ff8000000000000ll);
+          store_agent(U.cbuf + ((step64 + 2) % 3) * L + U.slot0 + tid, nan);
+        }
       } else if (tid < L) {
         S.lcost[tid] = S.rcost[tid];
       }
@@ -534,6 +587,11 @@ k_sweep(const UnitDev* __restrict__ units, SweepArgs a) {
     const uint32_t* src = reinterpret_cast<const uint32_t*>(sp);
     uint32_t* dst = reinterpret_cast<uint32_t*>(U.spins);
     for (int i = tid; i < stride / 4; i += nthreads) dst[i] = src[i];
+    if (a.best_lds) {
+      const uint32_t* bsrc = reinterpret_cast<const uint32_t*>(bst);
+      uint32_t* bdst = reinterpret_cast<uint32_t*>(U.best);
+      for (int i = tid; i < stride / 4; i += nthreads) bdst[i] = bsrc[i];
+    }
   }
   __syncthreads();
   if (tid < nrep) {
@@ -616,7 +674,8 @@ k_sweep_ordered(const UnitDev* __restrict__ units, SweepArgs a) {
       bool ok = false;
       if (act) {
         const uint4 quad = philox4x32_10((uint32_t)i, sweep, r >> 2, DOM_SWEEP, U.seed, U.stream);
-        ok = metropolis_accept((double)dE, beta, pick(quad, r & 3u), lut, [&]() {
+        ok = metropolis_accept((double)dE, beta, pick(quad, r & 3u),
+                               [&](int row, int i) { return lut[row * 256 + i]; }, [&]() {
           return philox4x32_10((uint32_t)i, sweep, r, DOM_REDRAW, U.seed, U.stream);
         });
       }
@@ -922,8 +981,14 @@ __global__ void k_philox(int n, const uint32_t* __restrict__ ctr, uint32_t k0, u
 
 // ---------------------------------------------------------------------------
 // launch helpers (host side of this translation unit)
-size_t sweep_smem_bytes(int max_stride, int ordered) {
+size_t sweep_smem_bytes(int max_stride, int ordered, int* best_lds) {
+  if (best_lds) *best_lds = 0;
   if (ordered) return 8192 + 2 * (size_t)max_stride;
+  const size_t with_best = sizeof(Smem) + 3 * (size_t)max_stride;
+  if (with_best <= 160 * 1024) {
+    if (best_lds) *best_lds = 1;
+    return with_best;
+  }
   return sizeof(Smem) + 2 * (size_t)max_stride;
 }
 
