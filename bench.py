@@ -5,8 +5,10 @@ Default workload: parallel tempering on a 16^3 periodic cubic spin glass
 (N = 4096 spins, 12288 couplings J ~ N(0,1) fp32, h = 0), 32 replicas at
 geometric temperatures 0.2 -> 2.0, replica exchange after every sweep, 5000
 sweeps, 16 independent restarts (seeds s..s+15).  One step = the whole job:
-16 solves fused into one device launch, with the problem already resident in
-HBM.  Metric: spin-flip attempts/s (= restarts x replicas x sweeps x N / s).
+16 solves configured, fused into one device launch and read back (best cost
+and best configuration of each, as arrays), with the problem already
+resident in HBM.  Metric: spin-flip attempts/s (= restarts x replicas x
+sweeps x N / s of wall time).
 
 The roofline object prices the dominant kernel (the coloured sweep) with the
 algorithmic bytes per attempt B = d*(4 + s_c + 1) + 2 (d = 6 incident terms,
@@ -66,21 +68,20 @@ def run_c2(args):
     attempts_per_step = restarts * L * sweeps * N
 
     def step():
-        res, ms = eng.solve_batch("paralleltempering.qiotoolkit", [model] * restarts, params)
-        return res, ms
+        # results as arrays: best cost and best configuration of every restart
+        return eng.solve_batch_arrays("paralleltempering.qiotoolkit", [model] * restarts, params)
 
     for _ in range(args.warmup):
         step()
     t0 = time.perf_counter()
     dev_ms, best = [], None
     for _ in range(args.steps):
-        res, ms = step()
-        dev_ms.append(ms)
-        best = [r["solutions"]["cost"] for r in res]
+        costs, configs, tm = step()
+        dev_ms.append(tm["device_ms"])
+        best = list(costs)
     t = time.perf_counter() - t0
-    b0 = res[0]["benchmark"]
     log("phases ms: init %.1f run %.1f finalize %.1f device %.1f" % (
-        b0["batch_init_ms"], b0["batch_run_ms"], b0["batch_finalize_ms"], b0["batch_device_ms"]))
+        tm["init_ms"], tm["run_ms"], tm["finalize_ms"], tm["device_ms"]))
     value = attempts_per_step * args.steps / t
     avg_ms = float(np.mean(dev_ms))
     B = 6 * (4 + 4 + 1) + 2

@@ -40,7 +40,7 @@ template <typename CT> struct AccT { using T = int32_t; };
 template <> struct AccT<float> { using T = double; };
 
 struct Smem {
-  double lut0[256];         // row 0 of the ln table (rows 1..3 are read from HBM)
+  double lut[1024];         // the 4 x 256 ln table
   double rcost[kMaxRep];
   double rbest[kMaxRep];
   double rbeta[kMaxRep];
@@ -300,7 +300,6 @@ k_sweep(const UnitDev* __restrict__ units, SweepArgs a) {
   cptr<int32_t> class_off = cnst(U.class_off);
   cptr<int32_t> slice_base = cnst(U.ell_slice_base);
   gptr<int32_t> var_of_pos = glob(U.var_of_pos);
-  gptr<double> lut_all = glob(a.lut);
   uint8_t* sp = smem_raw + sizeof(Smem);
   uint8_t* fl = sp + stride;
   // lowest states: in LDS for the launch when they fit, else in HBM
@@ -316,7 +315,10 @@ k_sweep(const UnitDev* __restrict__ units, SweepArgs a) {
   const int nb_g = active ? min(NB, nrep - 8 * g) : 0;
   const uint32_t rbase = U.r0 + 8u * (uint32_t)g;
 
-  for (int i = tid; i < 256; i += nthreads) S.lut0[i] = lut_all[i];
+  {
+    gptr<double> lut_g = glob(a.lut);
+    for (int i = tid; i < 1024; i += nthreads) S.lut[i] = lut_g[i];
+  }
   {
     const uint32_t* src = reinterpret_cast<const uint32_t*>(U.spins);
     uint32_t* dst = reinterpret_cast<uint32_t*>(sp);
@@ -345,7 +347,7 @@ k_sweep(const UnitDev* __restrict__ units, SweepArgs a) {
   if (U.ladder > 0 && tid < U.ladder) S.inv[S.label[tid]] = tid;
   __syncthreads();
 
-  auto lut = [&](int row, int i) { return row == 0 ? S.lut0[i] : lut_all[row * 256 + i]; };
+  auto lut = [&](int row, int i) { return S.lut[row * 256 + i]; };
 
   uint32_t acc[8];
 #pragma unroll
@@ -391,39 +393,58 @@ k_sweep(const UnitDev* __restrict__ units, SweepArgs a) {
         };
         auto process = [&](int k, const int2& sl, const EllRow& e) {
           const int rho = (k << lg) + tp;
-          const int q = c0 + rho;
-          // Pairwise rows: the field is formed on every lane (past the class
-          // end the row is ghost padding), so the row's loads are consumed on
-          // all paths.  The term lists of general problems end at N - 1.
+          // The term lists of general problems end at N - 1; pairwise rows
+          // past the class end are ghost padding, so those lanes run the
+          // same branch-free path on the ghost position (always 0, its flips
+          // forced to 0) and the row's loads are consumed on every path.
           if (!PAIR && rho >= nc) return;
+          const bool valid = rho < nc;
+          const int q = valid ? c0 + rho : N;
           CT h[8];
           local_field<KIND, PAIR, CT, NB>(U, sp, q, e, sl, tp, lg, G, g, h);
-          if (rho >= nc) return;
           const uint32_t var = e.var;
           const uint32_t own = sp[q * G + g];
-          uint32_t flips = 0;
+          CT dE[8];
+          uint32_t w0[8];
           uint4 quad = make_uint4(0, 0, 0, 0);
+          uint32_t flips = 0, redraw = 0;
 #pragma unroll
           for (int b = 0; b < NB; ++b) {
-            if (b < nb_g) {
-              const uint32_t r = rbase + (uint32_t)b;
-              const uint32_t ob = (own >> b) & 1u;
-              const CT dE = delta_from_field<KIND, CT>(h[b], ob);
-              if (b == 0 || (r & 3u) == 0u)
-                quad = philox4x32_10(var, sweep, r >> 2, DOM_SWEEP, U.seed, U.stream);
-              const bool ok = metropolis_accept((double)dE, beta[b], pick(quad, r & 3u), lut, [&]() {
-                return philox4x32_10(var, sweep, r, DOM_REDRAW, U.seed, U.stream);
-              });
-              if (ok) {
-                flips |= 1u << b;
-                Ssum[b] += (AT)dE;
-                if (Ssum[b] < Smin[b]) {  // first time at the new minimum
-                  Smin[b] = Ssum[b];
-                  kmin[b] = k;
-                }
-                acc[b]++;
-              }
+            const uint32_t r = rbase + (uint32_t)b;
+            dE[b] = delta_from_field<KIND, CT>(h[b], (own >> b) & 1u);
+            if (b == 0 || (r & 3u) == 0u)
+              quad = philox4x32_10(var, sweep, r >> 2, DOM_SWEEP, U.seed, U.stream);
+            w0[b] = pick(quad, r & 3u);
+            const bool live = valid && b < nb_g;
+            const bool ok = accept_row0((double)dE[b], beta[b], w0[b], lut);
+            const bool zero = (w0[b] & 255u) == 0u && dE[b] > CT(0);
+            flips |= (live && ok && !zero) ? 1u << b : 0u;
+            redraw |= (live && zero) ? 1u << b : 0u;
+          }
+          // bucket 0 of an uphill draw (1/256): the reference's row walk, one
+          // replica at a time, with the redraw counter of that replica
+          while (redraw) {
+            const int b = __builtin_ctz(redraw);
+            redraw &= redraw - 1;
+            const uint32_t r = rbase + (uint32_t)b;
+            CT d = dE[0];
+            double bb = beta[0];
+#pragma unroll
+            for (int j = 1; j < NB; ++j) {
+              d = j == b ? dE[j] : d;
+              bb = j == b ? beta[j] : bb;
             }
+            const uint4 w = philox4x32_10(var, sweep, r, DOM_REDRAW, U.seed, U.stream);
+            if (metropolis_accept((double)d, bb, 0u, lut, [&]() { return w; })) flips |= 1u << b;
+          }
+#pragma unroll
+          for (int b = 0; b < NB; ++b) {
+            const bool ok = (flips >> b) & 1u;
+            Ssum[b] += ok ? (AT)dE[b] : AT(0);
+            const bool lower = Ssum[b] < Smin[b];  // first time at the new minimum
+            Smin[b] = lower ? Ssum[b] : Smin[b];
+            kmin[b] = lower ? k : kmin[b];
+            acc[b] += ok ? 1u : 0u;
           }
           sp[q * G + g] = (uint8_t)(own ^ flips);
           fl[q * G + g] = (uint8_t)flips;
