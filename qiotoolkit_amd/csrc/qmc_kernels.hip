@@ -133,6 +133,14 @@ template <class T> using cptr = const __attribute__((address_space(4))) T*;
 template <class T> __device__ __forceinline__ gptr<T> glob(const T* p) { return (gptr<T>)p; }
 template <class T> __device__ __forceinline__ cptr<T> cnst(const T* p) { return (cptr<T>)p; }
 __device__ __forceinline__ int uni(int x) { return __builtin_amdgcn_readfirstlane(x); }
+// A wave-uniform value held in a vector register: the Philox round keys
+// derived from it are then vector values too, instead of twenty scalar
+// registers live across the whole sweep.
+__device__ __forceinline__ uint32_t in_vgpr(uint32_t x) {
+  uint32_t v;
+  asm volatile("v_mov_b32 %0, %1" : "=v"(v) : "s"(x));
+  return v;
+}
 
 template <typename CT> __device__ __forceinline__ CT coef_of(uint32_t bits);
 template <> __device__ __forceinline__ float coef_of<float>(uint32_t bits) { return __uint_as_float(bits); }
@@ -275,6 +283,15 @@ __device__ bool await_energy(const UnitDev& U, const double* slot, double& out) 
   }
 }
 
+// Work-group barrier that orders LDS traffic only: vector memory loads in
+// flight (the next graph rows) stay in flight across it, where
+// __syncthreads() would wait for them.
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // s_waitcnt vmcnt(0): every vector memory access of this lane issued so far,
 // stores included, has completed.
 __device__ __forceinline__ void wait_stores() { __builtin_amdgcn_s_waitcnt(0x0F70); }
@@ -290,11 +307,13 @@ __device__ __forceinline__ double load_agent(const double* p) {
 
 template <int KIND, bool PAIR, typename CT, int NB>
 __global__ void __launch_bounds__(kBlock)
-k_sweep(const UnitDev* __restrict__ units, SweepArgs a) {
+k_sweep(const UnitDev* units, SweepArgs a) {
   using AT = typename AccT<CT>::T;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   Smem& S = *reinterpret_cast<Smem*>(smem_raw);
-  const UnitDev U = units[a.unit_of_block ? a.unit_of_block[blockIdx.x] : (int)blockIdx.x];
+  // The descriptor stays in memory (scalar loads at each use): fields needed
+  // only at the start and end are not held in registers across the sweeps.
+  const UnitDev& U = units[a.unit_of_block ? a.unit_of_block[blockIdx.x] : (int)blockIdx.x];
   const int N = U.N, G = U.G, TP = U.lanes, lg = U.lanes_log2;
   const int stride = U.stride;
   cptr<int32_t> class_off = cnst(U.class_off);
@@ -314,6 +333,7 @@ k_sweep(const UnitDev* __restrict__ units, SweepArgs a) {
   const int nrep = (int)U.nrep;
   const int nb_g = active ? min(NB, nrep - 8 * g) : 0;
   const uint32_t rbase = U.r0 + 8u * (uint32_t)g;
+  const uint32_t key0 = in_vgpr(U.seed), key1 = in_vgpr(U.stream);
 
   {
     gptr<double> lut_g = glob(a.lut);
@@ -355,20 +375,49 @@ k_sweep(const UnitDev* __restrict__ units, SweepArgs a) {
 
   const int64_t sweep_begin = U.progress[0];
   const int n_run = (int)min((int64_t)a.n_sweeps, U.progress[1]);
+  const double* sched = U.sched;
+  const int64_t sched_base = U.sched_base;
+  const int n_classes = U.n_classes;
+  // beta of the next sweep, loaded one sweep ahead
+  double beta_next = (sched && tid < nrep && n_run > 0) ? sched[sweep_begin - sched_base] : 0.0;
+
+  // The graph rows stream through two buffers across class and sweep
+  // boundaries: round j of class c lives in `ra` when the class starts, and
+  // rounds past a class's end are the next class's first rounds (the last
+  // class is followed by class 0 of the next sweep), so the first row of a
+  // class is already in flight while the previous class finishes.
+  EllRow ra, rb;
+  int2 sla = make_int2(0, 0), slb = make_int2(0, 0);
+  auto fetch_row = [&](int cc0, int csb, int k, int2& sl, EllRow& e) {
+    const int rho = (k << lg) + tp;
+    if (PAIR) {
+      sl = load_slice(U, csb + k);
+      ell_fetch(U, sl, tp, lg, e);
+    }
+    e.var = (uint32_t)var_of_pos[cc0 + rho];  // padded past position N - 1
+  };
+  if (active) fetch_row(class_off[0], PAIR ? slice_base[0] : 0, 0, sla, ra);
+
   for (int s = 0; s < n_run; ++s) {
     const int64_t sweep64 = sweep_begin + s;
     const uint32_t sweep = (uint32_t)sweep64;
-    if (U.sched && tid < nrep) S.rbeta[tid] = U.sched[sweep64 - U.sched_base];
-    __syncthreads();
+    if (sched && tid < nrep) {
+      S.rbeta[tid] = beta_next;
+      if (s + 1 < n_run) beta_next = sched[sweep64 + 1 - sched_base];
+    }
+    lds_barrier();
     double beta[8];
 #pragma unroll
     for (int b = 0; b < NB; ++b) beta[b] = (b < nb_g) ? S.rbeta[8 * g + b] : 0.0;
 
-    for (int c = 0; c < U.n_classes; ++c) {
+    for (int c = 0; c < n_classes; ++c) {
       const int c0 = class_off[c];
       const int nc = class_off[c + 1] - c0;
       const int K = (nc + TP - 1) >> lg;
       const int sbase = PAIR ? slice_base[c] : 0;
+      const int cn = c + 1 < n_classes ? c + 1 : 0;  // the class after this one
+      const int c0n = class_off[cn];
+      const int sbasen = PAIR ? slice_base[cn] : 0;
       // per replica: sum of the accepted dE of this lane's positions, the
       // minimum prefix of that sum and the round where it is first reached
       AT Ssum[8], Smin[8];
@@ -383,13 +432,11 @@ k_sweep(const UnitDev* __restrict__ units, SweepArgs a) {
         // Position k of this lane is rho = k * lanes + tp.  The rounds are
         // uniform; the graph row of round k + 1 is fetched into the other
         // buffer while round k is processed.
+        // round k of this class, or round k - K of the next (no branch: the
+        // loads stay in flight across the processing of the other buffer)
         auto fetch = [&](int k, int2& sl, EllRow& e) {
-          const int rho = (k << lg) + tp;
-          if (PAIR) {
-            sl = load_slice(U, sbase + k);
-            ell_fetch(U, sl, tp, lg, e);
-          }
-          e.var = (uint32_t)var_of_pos[c0 + rho];  // padded past position N - 1
+          const bool here = k < K;
+          fetch_row(here ? c0 : c0n, here ? sbase : sbasen, here ? k : k - K, sl, e);
         };
         auto process = [&](int k, const int2& sl, const EllRow& e) {
           const int rho = (k << lg) + tp;
@@ -413,7 +460,7 @@ k_sweep(const UnitDev* __restrict__ units, SweepArgs a) {
             const uint32_t r = rbase + (uint32_t)b;
             dE[b] = delta_from_field<KIND, CT>(h[b], (own >> b) & 1u);
             if (b == 0 || (r & 3u) == 0u)
-              quad = philox4x32_10(var, sweep, r >> 2, DOM_SWEEP, U.seed, U.stream);
+              quad = philox4x32_10(var, sweep, r >> 2, DOM_SWEEP, key0, key1);
             w0[b] = pick(quad, r & 3u);
             const bool live = valid && b < nb_g;
             const bool ok = accept_row0((double)dE[b], beta[b], w0[b], lut);
@@ -434,7 +481,7 @@ k_sweep(const UnitDev* __restrict__ units, SweepArgs a) {
               d = j == b ? dE[j] : d;
               bb = j == b ? beta[j] : bb;
             }
-            const uint4 w = philox4x32_10(var, sweep, r, DOM_REDRAW, U.seed, U.stream);
+            const uint4 w = philox4x32_10(var, sweep, r, DOM_REDRAW, key0, key1);
             if (metropolis_accept((double)d, bb, 0u, lut, [&]() { return w; })) flips |= 1u << b;
           }
 #pragma unroll
@@ -452,16 +499,17 @@ k_sweep(const UnitDev* __restrict__ units, SweepArgs a) {
         // Fetches past the last round read sentinel slices and padding (the
         // tables are padded for it), so no fetch sits behind a branch and
         // the loads stay in flight while the other buffer is processed.
-        EllRow ea, eb;
-        int2 sla = make_int2(0, 0), slb = make_int2(0, 0);
-        fetch(0, sla, ea);
         for (int k = 0; k < K; k += 2) {
-          fetch(k + 1, slb, eb);
-          process(k, sla, ea);
+          fetch(k + 1, slb, rb);
+          process(k, sla, ra);
           if (k + 1 >= K) break;
-          fetch(k + 2, sla, ea);
-          process(k + 1, slb, eb);
+          fetch(k + 2, sla, ra);
+          process(k + 1, slb, rb);
         }
+        // an even K leaves the next class's first row in `ra`; an odd K left
+        // it in `rb`, so fetch it again into `ra` (it stays in flight during
+        // the end of this class)
+        if (K & 1 || K == 0) fetch(K, sla, ra);
 
         // ---- per replica: minimum of the cost along the class sequence ----
         // The sequence is lane-major, so lane t's prefixes follow the sums of
@@ -488,7 +536,7 @@ k_sweep(const UnitDev* __restrict__ units, SweepArgs a) {
           }
         }
       }
-      __syncthreads();
+      lds_barrier();
       if (tid < nrep) {
         const int gg = tid >> 3, bb = tid & 7;
         const int W = TP >> 6;
@@ -515,7 +563,7 @@ k_sweep(const UnitDev* __restrict__ units, SweepArgs a) {
           S.nb_tp[tid] = -1;
         }
       }
-      __syncthreads();
+      lds_barrier();
       if (S.any_new_best) {
         // lowest state = the class's start state with the flips up to the
         // minimum applied: every later flip of this class is undone
@@ -545,7 +593,7 @@ k_sweep(const UnitDev* __restrict__ units, SweepArgs a) {
             }
           }
         }
-        __syncthreads();
+        lds_barrier();
         if (tid == 0) S.any_new_best = 0;
       }
     }  // classes
@@ -568,7 +616,7 @@ k_sweep(const UnitDev* __restrict__ units, SweepArgs a) {
           if (!await_energy(U, cb + tid, e)) S.abort_flag = 1;
           S.lcost[tid] = e;
         }
-        __syncthreads();
+        lds_barrier();
         if (S.abort_flag) break;
         if (tid < nrep) {
           const double nan = __longlong_as_double(0x7ff8000000000000ll);
@@ -577,7 +625,7 @@ k_sweep(const UnitDev* __restrict__ units, SweepArgs a) {
       } else if (tid < L) {
         S.lcost[tid] = S.rcost[tid];
       }
-      __syncthreads();
+      lds_barrier();
       const int i = (int)(step & 1u) + 2 * tid;
       if (i < L - 1) {
         const int sa = S.label[i], sb = S.label[i + 1];
@@ -592,10 +640,10 @@ k_sweep(const UnitDev* __restrict__ units, SweepArgs a) {
           S.inv[sb] = i;
         }
       }
-      __syncthreads();
+      lds_barrier();
       if (tid < nrep) S.rbeta[tid] = S.lbeta[S.inv[U.slot0 + tid]];
     }
-    __syncthreads();
+    lds_barrier();
   }  // sweeps
 
   // ---- write back ----
