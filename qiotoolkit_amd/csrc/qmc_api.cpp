@@ -122,7 +122,44 @@ struct qmc_ctx_s {
   // (created on first use, default priority), joined back by events
   hipStream_t aux[kAuxStreams] = {};
   hipEvent_t fork_ev = nullptr, join_ev[kAuxStreams] = {};
+  // pinned staging ring for small host-to-device uploads on `stream`: setting
+  // up a solve enqueues its copies instead of waiting for each one
+  std::mutex stage_mu;
+  uint8_t* stage = nullptr;
+  size_t stage_off = 0;
 };
+
+constexpr size_t kStageBytes = 32u << 20;
+
+// Copies `bytes` from host `src` to device `dst` in order on the context's
+// stream.  `src` may be reused as soon as this returns: small uploads are
+// copied into the pinned ring first (the ring wraps after draining the
+// stream), large ones wait for their own copy.
+static hipError_t upload_async(qmc_ctx c, void* dst, const void* src, size_t bytes) {
+  if (bytes == 0) return hipSuccess;
+  std::lock_guard<std::mutex> lk(c->stage_mu);
+  if (!c->stage) {
+    hipError_t e = hipHostMalloc((void**)&c->stage, kStageBytes, hipHostMallocDefault);
+    if (e != hipSuccess) {
+      c->stage = nullptr;
+      return e;
+    }
+  }
+  if (bytes > kStageBytes / 4) {
+    hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream);
+    return e != hipSuccess ? e : hipStreamSynchronize(c->stream);
+  }
+  size_t at = (c->stage_off + 255) & ~(size_t)255;
+  if (at + bytes > kStageBytes) {
+    // every earlier copy out of the ring was enqueued on `stream`
+    hipError_t e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return e;
+    at = 0;
+  }
+  std::memcpy(c->stage + at, src, bytes);
+  c->stage_off = at + bytes;
+  return hipMemcpyAsync(dst, c->stage + at, bytes, hipMemcpyHostToDevice, c->stream);
+}
 
 struct EllLayout {
   DevBuf<int32_t> slice_base;
@@ -206,6 +243,7 @@ struct qmc_walkers_s {
   int64_t sched_base = 0;
   int64_t planned = 0;       // sweeps of the last plan
   uint64_t version = 0;      // bumped whenever `units` changes
+  bool units_stale = true;   // `d_units` behind `units` (uploaded before device use)
   float last_ms = 0.f;
 };
 
@@ -224,6 +262,9 @@ struct qmc_batch_s {
     DevBuf<UnitDev> d_units;
   };
   std::vector<std::unique_ptr<Group>> groups;
+  // where each member's descriptors sit in its group (valid while the
+  // member's version equals seen_version)
+  std::vector<std::pair<Group*, size_t>> member_at;
   float last_ms = 0.f;
   // chunked runs: launches of `graph_spl` sweeps captured once and replayed
   hipGraph_t graph = nullptr;
@@ -285,6 +326,7 @@ static void ctx_release(qmc_ctx c) {
     if (c->aux[i]) (void)hipStreamDestroy(c->aux[i]);
   }
   if (c->stream) (void)hipStreamDestroy(c->stream);
+  if (c->stage) (void)hipHostFree(c->stage);
   delete c;
 }
 
@@ -604,11 +646,23 @@ static void build_units(qmc_walkers w) {
   }
 }
 
+// Rebuilds the host descriptors; the device copy follows lazily
+// (device_units), since batched runs launch from their own copies.
 static int upload_units(qmc_walkers w) {
   build_units(w);
   w->version++;
-  HIP_TRY(hipMemcpy(w->d_units.p, w->units.data(), w->units.size() * sizeof(UnitDev),
-                    hipMemcpyHostToDevice));
+  w->units_stale = true;
+  return QMC_OK;
+}
+
+// The walkers' unit descriptors on the device, brought up to date in order
+// on the context stream.  Called before every launch that reads them.
+static int device_units(qmc_walkers w, const UnitDev** out) {
+  if (w->units_stale) {
+    HIP_TRY(upload_async(w->prob->ctx, w->d_units.p, w->units.data(), w->units.size() * sizeof(UnitDev)));
+    w->units_stale = false;
+  }
+  *out = w->d_units.p;
   return QMC_OK;
 }
 
@@ -619,7 +673,7 @@ static int set_progress(qmc_walkers w, int64_t sweep0, int64_t n) {
     pr[2 * u] = sweep0;
     pr[2 * u + 1] = n;
   }
-  HIP_TRY(hipMemcpy(w->progress.p, pr.data(), pr.size() * sizeof(int64_t), hipMemcpyHostToDevice));
+  HIP_TRY(upload_async(w->prob->ctx, w->progress.p, pr.data(), pr.size() * sizeof(int64_t)));
   w->planned = n;
   return QMC_OK;
 }
@@ -654,8 +708,6 @@ int qmc_walkers_create(qmc_problem p, int32_t n_replicas, int32_t replicas_per_g
   HIP_TRY(hipSetDevice(p->ctx->device));
   HIP_TRY(w->spins.alloc((size_t)w->n_units * w->stride));
   HIP_TRY(w->best.alloc((size_t)w->n_units * w->stride));
-  HIP_TRY(hipMemset(w->spins.p, 0, w->spins.n));
-  HIP_TRY(hipMemset(w->best.p, 0, w->best.n));
   HIP_TRY(w->cost.alloc(w->R));
   HIP_TRY(w->best_cost.alloc(w->R));
   HIP_TRY(w->beta.alloc(w->R));
@@ -663,15 +715,25 @@ int qmc_walkers_create(qmc_problem p, int32_t n_replicas, int32_t replicas_per_g
   HIP_TRY(w->label.alloc(w->R));
   HIP_TRY(w->ladder_beta.alloc(32));
   HIP_TRY(w->err.alloc(1));
-  HIP_TRY(hipMemset(w->err.p, 0, sizeof(int32_t)));
-  HIP_TRY(hipMemset(w->cost.p, 0, w->R * sizeof(double)));
-  HIP_TRY(hipMemset(w->best_cost.p, 0, w->R * sizeof(double)));
-  HIP_TRY(hipMemset(w->beta.p, 0, w->R * sizeof(double)));
-  HIP_TRY(hipMemset(w->accepted.p, 0, w->R * sizeof(unsigned long long)));
-  HIP_TRY(hipMemset(w->label.p, 0, w->R * sizeof(int32_t)));
   HIP_TRY(w->d_units.alloc(w->n_units));
   HIP_TRY(w->progress.alloc(2 * (size_t)w->n_units));
-  HIP_TRY(hipMemset(w->progress.p, 0, w->progress.n * sizeof(int64_t)));
+  // everything starts at zero, cleared in one launch on the context stream
+  ZeroSpans z{};
+  int nz = 0;
+  auto zero = [&](void* ptr, size_t bytes) {
+    z.p[nz] = ptr;
+    z.n[nz++] = bytes;
+  };
+  zero(w->spins.p, w->spins.n);
+  zero(w->best.p, w->best.n);
+  zero(w->cost.p, w->R * sizeof(double));
+  zero(w->best_cost.p, w->R * sizeof(double));
+  zero(w->beta.p, w->R * sizeof(double));
+  zero(w->accepted.p, w->R * sizeof(unsigned long long));
+  zero(w->label.p, w->R * sizeof(int32_t));
+  zero(w->err.p, sizeof(int32_t));
+  zero(w->progress.p, w->progress.n * sizeof(int64_t));
+  HIP_TRY(launch_zero(z, nz, p->ctx->stream));
   int rc = upload_units(w.get());
   if (rc) return rc;
   p->refs++;
@@ -725,8 +787,10 @@ int qmc_walkers_resample(qmc_walkers w, const int32_t* src, int32_t new_R) {
   HIP_TRY(c1.alloc(new_R));
   HIP_TRY(c2.alloc(new_R));
   HIP_TRY(a0.alloc(new_R));
-  HIP_TRY(launch_unpack(w->d_units.p, w->B, 0, w->R, p->N, 0, w->scratch_states.p, st));
-  HIP_TRY(launch_unpack(w->d_units.p, w->B, 0, w->R, p->N, 1, low.p, st));
+  const UnitDev* du = nullptr;
+  if ((rc = device_units(w, &du))) return rc;
+  HIP_TRY(launch_unpack(du, w->B, 0, w->R, p->N, 0, w->scratch_states.p, st));
+  HIP_TRY(launch_unpack(du, w->B, 0, w->R, p->N, 1, low.p, st));
   HIP_TRY(launch_gather_replicas(new_R, dsrc.p, w->cost.p, w->best_cost.p, w->beta.p, w->accepted.p,
                                  c0.p, c1.p, c2.p, a0.p, st));
   HIP_TRY(hipMemcpyAsync(w->cost.p, c0.p, new_R * sizeof(double), hipMemcpyDeviceToDevice, st));
@@ -736,8 +800,8 @@ int qmc_walkers_resample(qmc_walkers w, const int32_t* src, int32_t new_R) {
   HIP_TRY(hipStreamSynchronize(st));
   w->R = new_R;
   w->n_units = (new_R + w->B - 1) / w->B;
-  if ((rc = upload_units(w))) return rc;
-  HIP_TRY(launch_pack_gather(w->d_units.p, w->n_units, w->B, new_R, p->N, dsrc.p, w->scratch_states.p,
+  if ((rc = upload_units(w)) || (rc = device_units(w, &du))) return rc;
+  HIP_TRY(launch_pack_gather(du, w->n_units, w->B, new_R, p->N, dsrc.p, w->scratch_states.p,
                              low.p, st));
   HIP_TRY(hipStreamSynchronize(st));
   return QMC_OK;
@@ -753,13 +817,12 @@ int qmc_walkers_init(qmc_walkers w, uint32_t epoch) {
   if (!w) return fail(QMC_INVALID_ARGUMENT, "null handle");
   qmc_problem p = w->prob;
   hipStream_t st = p->ctx->stream;
-  HIP_TRY(launch_init(w->d_units.p, w->n_units, p->N, epoch, st));
-  int rc = ensure_scratch(w);
-  if (rc) return rc;
-  HIP_TRY(launch_unpack(w->d_units.p, w->B, 0, w->R, p->N, 0, w->scratch_states.p, st));
-  HIP_TRY(launch_energy(p->dev(), w->scratch_states.p, w->R, w->cost.p, st));
-  HIP_TRY(hipMemcpyAsync(w->best_cost.p, w->cost.p, w->R * sizeof(double), hipMemcpyDeviceToDevice, st));
-  HIP_TRY(hipStreamSynchronize(st));
+  const UnitDev* du = nullptr;
+  int rc = device_units(w, &du);
+  if (rc || (rc = ensure_scratch(w))) return rc;
+  HIP_TRY(launch_init(du, w->n_units, p->N, epoch, st));
+  HIP_TRY(launch_unpack(du, w->B, 0, w->R, p->N, 0, w->scratch_states.p, st));
+  HIP_TRY(launch_energy(p->dev(), w->scratch_states.p, w->R, w->cost.p, st, w->best_cost.p));
   return QMC_OK;
 }
 
@@ -767,22 +830,19 @@ int qmc_walkers_set_states(qmc_walkers w, const uint8_t* states) {
   if (!w || !states) return fail(QMC_INVALID_ARGUMENT, "null argument");
   qmc_problem p = w->prob;
   hipStream_t st = p->ctx->stream;
-  int rc = ensure_scratch(w);
-  if (rc) return rc;
+  const UnitDev* du = nullptr;
+  int rc = device_units(w, &du);
+  if (rc || (rc = ensure_scratch(w))) return rc;
   const size_t n = (size_t)w->R * p->N;
-  HIP_TRY(hipMemcpyAsync(w->scratch_states.p, states, n, hipMemcpyHostToDevice, st));
-  HIP_TRY(launch_pack(w->d_units.p, w->n_units, w->B, w->R, p->N, w->scratch_states.p, 2, st));
-  HIP_TRY(launch_energy(p->dev(), w->scratch_states.p, w->R, w->cost.p, st));
-  HIP_TRY(hipMemcpyAsync(w->best_cost.p, w->cost.p, w->R * sizeof(double), hipMemcpyDeviceToDevice, st));
-  HIP_TRY(hipStreamSynchronize(st));
+  HIP_TRY(upload_async(p->ctx, w->scratch_states.p, states, n));
+  HIP_TRY(launch_pack(du, w->n_units, w->B, w->R, p->N, w->scratch_states.p, 2, st));
+  HIP_TRY(launch_energy(p->dev(), w->scratch_states.p, w->R, w->cost.p, st, w->best_cost.p));
   return QMC_OK;
 }
 
 int qmc_walkers_set_betas(qmc_walkers w, const double* betas) {
   if (!w || !betas) return fail(QMC_INVALID_ARGUMENT, "null argument");
-  HIP_TRY(hipMemcpyAsync(w->beta.p, betas, w->R * sizeof(double), hipMemcpyHostToDevice,
-                         w->prob->ctx->stream));
-  HIP_TRY(hipStreamSynchronize(w->prob->ctx->stream));
+  HIP_TRY(upload_async(w->prob->ctx, w->beta.p, betas, w->R * sizeof(double)));
   return QMC_OK;
 }
 
@@ -800,7 +860,7 @@ static int launch_units(qmc_problem p, int nb, const UnitDev* d_units, int n_uni
 
 static int upload_schedule(qmc_walkers w, int64_t sweep0, int32_t n, const double* schedule) {
   if (w->sched.n < (size_t)n) HIP_TRY(w->sched.alloc(n));
-  HIP_TRY(hipMemcpy(w->sched.p, schedule, n * sizeof(double), hipMemcpyHostToDevice));
+  HIP_TRY(upload_async(w->prob->ctx, w->sched.p, schedule, n * sizeof(double)));
   w->sched_base = sweep0;
   return QMC_OK;
 }
@@ -834,10 +894,10 @@ static int set_ladder(qmc_walkers w, int32_t ladder, const double* ladder_betas,
       lab[r] = r % ladder;
       be[r] = ladder_betas[r % ladder];
     }
-    HIP_TRY(hipMemcpy(w->label.p, lab.data(), w->R * sizeof(int32_t), hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(w->beta.p, be.data(), w->R * sizeof(double), hipMemcpyHostToDevice));
+    HIP_TRY(upload_async(w->prob->ctx, w->label.p, lab.data(), w->R * sizeof(int32_t)));
+    HIP_TRY(upload_async(w->prob->ctx, w->beta.p, be.data(), w->R * sizeof(double)));
   }
-  HIP_TRY(hipMemcpy(w->ladder_beta.p, ladder_betas, ladder * sizeof(double), hipMemcpyHostToDevice));
+  HIP_TRY(upload_async(w->prob->ctx, w->ladder_beta.p, ladder_betas, ladder * sizeof(double)));
   return QMC_OK;
 }
 
@@ -859,10 +919,11 @@ int qmc_walkers_plan_pt(qmc_walkers w, int32_t ladder, const double* ladder_beta
     if (w->cbuf.n < (size_t)n_ladders * 3 * ladder) HIP_TRY(w->cbuf.alloc((size_t)n_ladders * 3 * ladder));
     // every energy slot starts "not yet published"
     std::vector<double> nan((size_t)n_ladders * 3 * ladder, std::numeric_limits<double>::quiet_NaN());
-    HIP_TRY(hipMemcpy(w->cbuf.p, nan.data(), nan.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIP_TRY(upload_async(w->prob->ctx, w->cbuf.p, nan.data(), nan.size() * sizeof(double)));
     std::vector<int32_t> ctr(n_ladders, (int32_t)(step0 * upl));
-    HIP_TRY(hipMemcpy(w->sync.p, ctr.data(), n_ladders * sizeof(int32_t), hipMemcpyHostToDevice));
-    HIP_TRY(hipMemset(w->err.p, 0, sizeof(int32_t)));
+    HIP_TRY(upload_async(w->prob->ctx, w->sync.p, ctr.data(), n_ladders * sizeof(int32_t)));
+    const int32_t zero = 0;
+    HIP_TRY(upload_async(w->prob->ctx, w->err.p, &zero, sizeof zero));
   }
   w->ladder = ladder;
   w->spr = sweeps_per_replica;
@@ -886,8 +947,10 @@ int qmc_walkers_sweep(qmc_walkers w, int64_t sweep0, int32_t n_sweeps, int beta_
   int rc = qmc_walkers_plan(w, sweep0, n_sweeps, beta_mode == QMC_BETA_SCHEDULE ? schedule : nullptr);
   if (rc) return rc;
   qmc_ctx c = w->prob->ctx;
+  const UnitDev* du = nullptr;
+  if ((rc = device_units(w, &du))) return rc;
   HIP_TRY(hipEventRecord(c->ev0, c->stream));
-  rc = launch_units(w->prob, w->nb, w->d_units.p, w->n_units, ordered ? 64 : w->lanes * w->G, w->stride,
+  rc = launch_units(w->prob, w->nb, du, w->n_units, ordered ? 64 : w->lanes * w->G, w->stride,
                     n_sweeps, ordered, c->stream);
   if (rc) return rc;
   HIP_TRY(hipEventRecord(c->ev1, c->stream));
@@ -910,8 +973,10 @@ int qmc_walkers_pt(qmc_walkers w, int32_t ladder, const double* ladder_betas, in
       w->n_units <= max_coresident_blocks(p->kind, p->pairwise, p->coupling, w->nb, w->lanes * w->G, smem);
   if (w->B == ladder || (can_coop && !w->force_graph)) {
     if ((rc = qmc_walkers_plan_pt(w, ladder, ladder_betas, step0, n_steps, sweeps_per_replica))) return rc;
+    const UnitDev* du = nullptr;
+    if ((rc = device_units(w, &du))) return rc;
     HIP_TRY(hipEventRecord(c->ev0, st));
-    if ((rc = launch_units(p, w->nb, w->d_units.p, w->n_units, w->lanes * w->G, w->stride,
+    if ((rc = launch_units(p, w->nb, du, w->n_units, w->lanes * w->G, w->stride,
                            n_steps * sweeps_per_replica, 0, st, w->coop)))
       return rc;
     HIP_TRY(hipEventRecord(c->ev1, st));
@@ -931,8 +996,11 @@ int qmc_walkers_pt(qmc_walkers w, int32_t ladder, const double* ladder_betas, in
     if (w->last_step.n < (size_t)n_ladders) HIP_TRY(w->last_step.alloc(n_ladders));
     {
       std::vector<int64_t> ls(n_ladders, fresh ? -1 : step0 - 1);
-      HIP_TRY(hipMemcpy(w->last_step.p, ls.data(), n_ladders * sizeof(int64_t), hipMemcpyHostToDevice));
+      HIP_TRY(upload_async(c, w->last_step.p, ls.data(), n_ladders * sizeof(int64_t)));
     }
+    // nothing is uploaded while the stream is being captured
+    const UnitDev* du = nullptr;
+    if ((rc = device_units(w, &du))) return rc;
     const int upl = ladder / w->B;
     const uint32_t seed0 = w->seed + (uint32_t)(w->first_replica / ladder);
     // Steps are captured once as a graph of K (sweep, exchange) pairs and
@@ -944,7 +1012,7 @@ int qmc_walkers_pt(qmc_walkers w, int32_t ladder, const double* ladder_betas, in
       drop_graph(w);
       HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
       for (int k = 0; k < K; ++k) {
-        rc = launch_units(p, w->nb, w->d_units.p, w->n_units, w->lanes * w->G, w->stride, sweeps_per_replica, 0, st);
+        rc = launch_units(p, w->nb, du, w->n_units, w->lanes * w->G, w->stride, sweeps_per_replica, 0, st);
         if (rc) {
           hipGraph_t g;
           (void)hipStreamEndCapture(st, &g);
@@ -1041,11 +1109,15 @@ static int batch_rebuild(qmc_batch b) {
   b->groups.clear();
   b->rebuilds++;
   // Largest problems first, so the longest work-groups start earliest.
-  std::vector<qmc_walkers> order(b->members.begin(), b->members.end());
-  std::stable_sort(order.begin(), order.end(), [](qmc_walkers x, qmc_walkers y) {
+  std::vector<size_t> order(b->members.size());
+  for (size_t i = 0; i < order.size(); ++i) order[i] = i;
+  std::stable_sort(order.begin(), order.end(), [&](size_t i, size_t j) {
+    const qmc_walkers x = b->members[i], y = b->members[j];
     return (int64_t)x->prob->n_inc * x->B > (int64_t)y->prob->n_inc * y->B;
   });
-  for (qmc_walkers w : order) {
+  b->member_at.assign(b->members.size(), {nullptr, 0});
+  for (size_t i : order) {
+    qmc_walkers w = b->members[i];
     qmc_problem p = w->prob;
     // one group per kernel instantiation and work-group size, so no launch
     // carries idle threads for its smaller members
@@ -1067,6 +1139,7 @@ static int batch_rebuild(qmc_batch b) {
     }
     g->stride = std::max(g->stride, w->stride);
     g->work += (double)p->N * w->R * (double)w->planned;
+    b->member_at[i] = {g, g->units.size()};
     g->units.insert(g->units.end(), w->units.begin(), w->units.end());
   }
   for (auto& g : b->groups) {
@@ -1228,10 +1301,16 @@ int qmc_batch_read(qmc_batch b, double* lowest_energy, uint64_t* accepted, uint8
   std::vector<int32_t> rep_member;
   desc.reserve(b->members.size());
   int64_t roff = 0, soff = 0;
-  for (qmc_walkers w : b->members) {
+  for (size_t i = 0; i < b->members.size(); ++i) {
+    qmc_walkers w = b->members[i];
     qmc_problem p = w->prob;
     ReadDesc d{};
-    d.units = w->d_units.p;
+    if (i < b->member_at.size() && b->member_at[i].first && b->seen_version[i] == w->version) {
+      d.units = b->member_at[i].first->d_units.p + b->member_at[i].second;
+    } else {
+      int rc = device_units(w, &d.units);
+      if (rc) return rc;
+    }
     d.soff = soff;
     d.roff = roff;
     d.B = w->B;
@@ -1300,16 +1379,17 @@ int qmc_walkers_read(qmc_walkers w, double* cost, double* lowest, double* beta, 
   if (accepted)
     HIP_TRY(hipMemcpy(accepted, w->accepted.p, w->R * sizeof(uint64_t), hipMemcpyDeviceToHost));
   if (states || lowest_states) {
-    int rc = ensure_scratch(w);
-    if (rc) return rc;
+    const UnitDev* du = nullptr;
+    int rc = device_units(w, &du);
+    if (rc || (rc = ensure_scratch(w))) return rc;
     const size_t n = (size_t)w->R * p->N;
     if (states) {
-      HIP_TRY(launch_unpack(w->d_units.p, w->B, 0, w->R, p->N, 0, w->scratch_states.p, st));
+      HIP_TRY(launch_unpack(du, w->B, 0, w->R, p->N, 0, w->scratch_states.p, st));
       HIP_TRY(hipMemcpyAsync(states, w->scratch_states.p, n, hipMemcpyDeviceToHost, st));
       HIP_TRY(hipStreamSynchronize(st));
     }
     if (lowest_states) {
-      HIP_TRY(launch_unpack(w->d_units.p, w->B, 0, w->R, p->N, 1, w->scratch_states.p, st));
+      HIP_TRY(launch_unpack(du, w->B, 0, w->R, p->N, 1, w->scratch_states.p, st));
       HIP_TRY(hipMemcpyAsync(lowest_states, w->scratch_states.p, n, hipMemcpyDeviceToHost, st));
       HIP_TRY(hipStreamSynchronize(st));
     }
@@ -1322,12 +1402,13 @@ int qmc_walkers_read_replica(qmc_walkers w, int32_t replica, uint8_t* state, uin
   if (replica < 0 || replica >= w->R) return fail(QMC_INVALID_ARGUMENT, "replica out of range");
   qmc_problem p = w->prob;
   hipStream_t st = p->ctx->stream;
-  int rc = ensure_scratch(w);
-  if (rc) return rc;
+  const UnitDev* du = nullptr;
+  int rc = device_units(w, &du);
+  if (rc || (rc = ensure_scratch(w))) return rc;
   for (int which = 0; which < 2; ++which) {
     uint8_t* out = which ? lowest_state : state;
     if (!out) continue;
-    HIP_TRY(launch_unpack(w->d_units.p, w->B, replica, 1, p->N, which, w->scratch_states.p, st));
+    HIP_TRY(launch_unpack(du, w->B, replica, 1, p->N, which, w->scratch_states.p, st));
     HIP_TRY(hipMemcpyAsync(out, w->scratch_states.p, p->N, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
   }
@@ -1338,9 +1419,10 @@ int qmc_walkers_energy(qmc_walkers w, int lowest, double* energies) {
   if (!w || !energies) return fail(QMC_INVALID_ARGUMENT, "null argument");
   qmc_problem p = w->prob;
   hipStream_t st = p->ctx->stream;
-  int rc = ensure_scratch(w);
-  if (rc) return rc;
-  HIP_TRY(launch_unpack(w->d_units.p, w->B, 0, w->R, p->N, lowest ? 1 : 0, w->scratch_states.p, st));
+  const UnitDev* du = nullptr;
+  int rc = device_units(w, &du);
+  if (rc || (rc = ensure_scratch(w))) return rc;
+  HIP_TRY(launch_unpack(du, w->B, 0, w->R, p->N, lowest ? 1 : 0, w->scratch_states.p, st));
   HIP_TRY(launch_energy(p->dev(), w->scratch_states.p, w->R, w->energies.p, st));
   HIP_TRY(hipMemcpyAsync(energies, w->energies.p, w->R * sizeof(double), hipMemcpyDeviceToHost, st));
   HIP_TRY(hipStreamSynchronize(st));

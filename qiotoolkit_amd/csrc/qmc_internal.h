@@ -104,8 +104,17 @@ hipError_t launch_pack_gather(const UnitDev* units, int n_units, int B, int R, i
 hipError_t launch_gather_replicas(int n, const int32_t* src, const double* c0, const double* c1,
                                   const double* c2, const unsigned long long* a0, double* o0,
                                   double* o1, double* o2, unsigned long long* oa, hipStream_t st);
+// energies of n plain states into out[] (and out2[] when given)
 hipError_t launch_energy(const ProblemDev& P, const uint8_t* states, int n, double* out,
-                         hipStream_t st);
+                         hipStream_t st, double* out2 = nullptr);
+// Zeroes up to kZeroSpans device byte ranges in one launch; each pointer is
+// 16-byte aligned.
+constexpr int kZeroSpans = 12;
+struct ZeroSpans {
+  void* p[kZeroSpans];
+  size_t n[kZeroSpans];
+};
+hipError_t launch_zero(const ZeroSpans& z, int count, hipStream_t st);
 hipError_t launch_delta_all(const ProblemDev& P, int n_states, const uint8_t* states, double* out,
                             hipStream_t st);
 

@@ -22,6 +22,8 @@
 // pre-class state and the flips recorded in LDS.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "device_common.h"
 #include "qmc_internal.h"
 
@@ -979,11 +981,26 @@ __device__ double state_energy_256(const int64_t* __restrict__ term_off, const i
 __global__ void __launch_bounds__(256)
 k_energy(const int64_t* __restrict__ term_off, const int32_t* __restrict__ term_var,
          const double* __restrict__ term_c, int64_t M, int kind, int N,
-         const uint8_t* __restrict__ states, double* __restrict__ out) {
+         const uint8_t* __restrict__ states, double* __restrict__ out, double* __restrict__ out2) {
   __shared__ double part[256];
   const uint8_t* st = states + (int64_t)blockIdx.x * N;
   const double e = state_energy_256(term_off, term_var, term_c, M, kind, st, part);
-  if (threadIdx.x == 0) out[blockIdx.x] = e;
+  if (threadIdx.x == 0) {
+    out[blockIdx.x] = e;
+    if (out2) out2[blockIdx.x] = e;
+  }
+}
+
+// Work-group row y zeroes span y: 16-byte stores over the aligned body,
+// byte stores over the tail.
+__global__ void __launch_bounds__(256) k_zero_spans(ZeroSpans z) {
+  uint8_t* p = (uint8_t*)z.p[blockIdx.y];
+  const size_t n = z.n[blockIdx.y];
+  const size_t n16 = n >> 4;
+  const size_t step = (size_t)gridDim.x * blockDim.x;
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  for (size_t i = t; i < n16; i += step) ((uint4*)p)[i] = make_uint4(0u, 0u, 0u, 0u);
+  for (size_t i = (n16 << 4) + t; i < n; i += step) p[i] = 0;
 }
 
 // Batched read-back: work-group = one global replica.  Unpacks its lowest
@@ -1199,10 +1216,25 @@ hipError_t launch_gather_replicas(int n, const int32_t* src, const double* c0, c
   return hipGetLastError();
 }
 
-hipError_t launch_energy(const ProblemDev& P, const uint8_t* states, int n, double* out, hipStream_t st) {
+hipError_t launch_energy(const ProblemDev& P, const uint8_t* states, int n, double* out, hipStream_t st,
+                         double* out2) {
   if (n <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_energy, dim3(n), dim3(256), 0, st, P.term_off, P.term_var, P.term_c, P.M,
-                     P.kind, P.N, states, out);
+                     P.kind, P.N, states, out, out2);
+  return hipGetLastError();
+}
+
+hipError_t launch_zero(const ZeroSpans& z, int count, hipStream_t st) {
+  if (count <= 0) return hipSuccess;
+  if (count > kZeroSpans) return hipErrorInvalidValue;
+  size_t most = 0;
+  for (int i = 0; i < count; ++i) {
+    if ((uintptr_t)z.p[i] & 15u) return hipErrorInvalidValue;
+    most = std::max(most, z.n[i]);
+  }
+  if (most == 0) return hipSuccess;
+  const size_t blocks = std::min<size_t>((most / 16 + 255) / 256 + 1, 1024);
+  hipLaunchKernelGGL(k_zero_spans, dim3((unsigned)blocks, count), dim3(256), 0, st, z);
   return hipGetLastError();
 }
 
