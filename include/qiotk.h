@@ -61,12 +61,15 @@ int qiotk_solve_batch_arrays(qiotk_engine e, const char* target, int32_t n, cons
  * schedule compressed accordingly) of candidate launch shapes -- replicas per
  * work-group x lane depth (positions each lane covers per colour class,
  * which sets the LDS tile of a class round) -- on a sample of the solves (at
- * most 512), timed with HIP events; the fastest is returned as
- * {"params": {"replicas_per_group": B, "positions_per_lane": k}, "trials":
- * [...]} to merge into every solve's params.  Sweeps per launch need no
- * tuning: a batch runs every planned sweep in one persistent launch.
- * replicas_per_group never changes results (the RNG counters do not see the
- * grouping); positions_per_lane changes the colour-class visit order. */
+ * most 512), timed with HIP events; then, for the fastest shape, one
+ * persistent launch against graph-replayed launches of 1/4 and 1/2 of the
+ * trial sweeps.  Returned as {"params": {"replicas_per_group": B,
+ * "positions_per_lane": k, "sweeps_per_launch": s (0 = persistent),
+ * "sweeps_per_launch_fraction": s / trial_sweeps}, "trials": [...]}; merge
+ * the first two into every solve's params and scale the fraction by its
+ * sweeps.  replicas_per_group and sweeps_per_launch never change results (the
+ * RNG counters see neither); positions_per_lane changes the colour-class
+ * visit order. */
 int qiotk_tune_batch(qiotk_engine e, const char* target, int32_t n, const qiotk_model* models,
                      const char* const* params_json, int32_t trial_sweeps, char** response);
 /* Variable count and the original variable names in internal-id order

@@ -2,8 +2,8 @@
 //
 // Data layout (per problem, in "position space"): variables are renumbered so
 // that every colour class is a contiguous range of positions; inside a class
-// the positions follow ascending variable id.  Position N is a ghost that is
-// always 0, so one-variable terms need no special case.
+// the positions follow descending degree, then variable id.  Position N is a
+// ghost that is always 0, so one-variable terms need no special case.
 //
 // Per walker group ("unit"): up to 32 replicas of one problem share a
 // work-group.  Spins are packed bits, byte g of position q holding replicas
@@ -17,9 +17,10 @@
 // term, so the parallel update equals that sequential order exactly.  The
 // reference's per-flip best tracking (markov/walker.h:277-280) is kept exact:
 // each thread accumulates the prefix sum S and minimum prefix m of its
-// accepted dE; a wave scan + min-reduction gives the minimum over the class's
-// sequence and where it is attained, and the lowest state is rebuilt from the
-// pre-class state and the flips recorded in LDS.
+// accepted dE (and the round k of its first minimum); DPP wavefront scans and
+// a cross-wave min-reduction give the minimum over the class's sequence and
+// where it is attained, and the lowest state is rebuilt from the pre-class
+// state and the flips recorded in LDS.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>

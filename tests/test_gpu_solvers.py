@@ -149,6 +149,38 @@ def test_best_energy_distribution_matches_reference(engine, kind):
     assert abs(gpu.mean() - cpu.mean()) <= 0.005 * abs(cpu.mean())
 
 
+def quality_cases():
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("make_quality", os.path.join(GOLD, "make_quality.py"))
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod.cases()
+
+
+@pytest.mark.parametrize("name", ["sa_lattice2d_32", "pt_lattice3d_16", "sa_pubo_2000"])
+def test_best_energy_distribution_at_workload_size(engine, name):
+    """Full-size workloads (32x32 SA 64 x 1000, 16^3 PT 32 replicas x 5000,
+    PUBO 2000-var SA 256 x 2000): best energies over seeds 100..199 against
+    the oracle's distribution in tests/golden/quality_<name>.json."""
+    path = os.path.join(GOLD, "quality_%s.json" % name)
+    if not os.path.exists(path):
+        pytest.skip("fixture %s not generated" % os.path.basename(path))
+    fx = json.load(open(path))
+    case = quality_cases()[name]
+    target = {"sa": "simulatedannealing.qiotoolkit", "pt": "paralleltempering.qiotoolkit"}[case["solver"]]
+    doc = {"cost_function": {"type": case["kind"], "terms": case["terms"]()}}
+    model = engine.model(doc)
+    seeds = fx["seeds"]
+    print("quality %s: %d solves" % (name, len(seeds)), flush=True)
+    res, _ = engine.solve_batch(target, [model] * len(seeds),
+                                [{"params": dict(case["params"], seed=s)} for s in seeds])
+    gpu = np.array([r["solutions"]["cost"] for r in res])
+    cpu = np.array(fx["best"])
+    p = ks_2samp(gpu, cpu)
+    assert p > 0.05, (p, gpu.mean(), cpu.mean())
+    assert abs(gpu.mean() - cpu.mean()) <= 0.005 * abs(cpu.mean())
+
+
 @pytest.mark.parametrize("target", ["simulatedannealing.qiotoolkit", "paralleltempering.qiotoolkit"])
 def test_batch_arrays_match_json_and_single_solves(engine, target):
     """Batched solves of uneven instances (arrays and JSON results) equal the
