@@ -131,6 +131,42 @@ int qmc_walkers_plan_pt(qmc_walkers w, int32_t ladder, const double* ladder_beta
 int qmc_walkers_resample(qmc_walkers w, const int32_t* src, int32_t new_R);
 /* Current number of replicas. */
 int qmc_walkers_count(qmc_walkers w, int32_t* n_replicas);
+
+/* Population annealing with the population on the device
+ * (solver/population_annealing.h:262-616; replaces the host loop of
+ * make_step + resample, Poisson branch).  The walkers, created with the
+ * capacity as their replica count, hold a population of `population`
+ * citizens (replicas 0..R-1, R kept in device memory; the rest idle).
+ * beta_table[k] = the schedule's beta k steps after the last restart,
+ * k = 0..n_table-1 (n_table > the steps to run).  Every citizen starts at
+ * beta_table[0]; initialise the states with qmc_walkers_init afterwards. */
+int qmc_walkers_pa_begin(qmc_walkers w, int32_t population, double alpha, double scale,
+                         const double* beta_table, int32_t n_table, uint32_t seed,
+                         int32_t sweeps_per_step);
+/* Enqueue one step after the step's sweeps, without waiting: global lowest
+ * update, cost statistics, weights exp(-dbeta (E - E_min) scale), Poisson
+ * offspring (counter RNG (citizen, step, k, 4), key (seed, 0)) with mean
+ * population * w / sum(w), the copies, the family statistic and the restart
+ * rule (R / rho < alpha or zero variance: fresh citizens, population topped
+ * up to `population`). */
+int qmc_walkers_pa_step(qmc_walkers w);
+typedef struct {
+  int32_t population;  /* live citizens */
+  int32_t restarts;
+  int32_t overflow;    /* offspring exceeded the capacity (population clamped) */
+  int32_t have_best;
+  int64_t steps;
+  int64_t attempts;    /* spin-flip attempts of the steps so far */
+  double best;         /* lowest cost any citizen reached */
+  double beta;         /* current beta */
+  double rho;          /* family statistic of the last resampling */
+} qmc_pa_info;
+/* Waits for the stream; best_state (n_vars bytes, may be NULL) receives the
+ * state of `best`. */
+int qmc_walkers_pa_info(qmc_walkers w, qmc_pa_info* info, uint8_t* best_state);
+/* Ends device population mode: the walkers hold the live citizens as their
+ * replicas (qmc_walkers_count, _read, _energy see them). */
+int qmc_walkers_pa_end(qmc_walkers w);
 /* Largest number of planned sweeps not yet run by any work-group. */
 int qmc_walkers_remaining(qmc_walkers w, int64_t* sweeps_left);
 

@@ -93,7 +93,7 @@ void philox4x32_10(const uint32_t in_ctr[4], const uint32_t in_key[2], uint32_t 
 }
 
 // Counter domains of the GPU RNG contract.
-enum : uint32_t { DOM_SWEEP = 0, DOM_REDRAW = 1, DOM_INIT = 2, DOM_EXCHANGE = 3 };
+enum : uint32_t { DOM_SWEEP = 0, DOM_REDRAW = 1, DOM_INIT = 2, DOM_EXCHANGE = 3, DOM_POISSON = 4, DOM_PICK = 5 };
 
 inline uint32_t philox_word(uint32_t k0, uint32_t k1, uint32_t c0, uint32_t c1, uint32_t c2,
                             uint32_t c3, int w) {
@@ -109,6 +109,50 @@ inline double canonical2(uint32_t w0, uint32_t w1) {
   double r = s / 18446744073709551616.0;
   if (r >= 1.0) r = std::nextafter(1.0, 0.0);
   return r;
+}
+
+// Poisson variate with mean lam in counter mode: the uniforms are
+// u(k) = canonical2(Philox(i, step, k, DOM_POISSON) words 0, 1) under the key
+// (seed, 0).  Inversion by sequential search for lam < 30; above, the PTRS
+// transformed rejection of W. Hormann, Insurance: Mathematics and Economics
+// 12 (1993) 39-45.  (The reference draws std::poisson_distribution from its
+// master pcg32, random_generator.cc:54-58.)
+uint32_t poisson_counter(double lam, uint32_t i, uint32_t step, uint32_t seed) {
+  auto u = [&](uint32_t k) {
+    uint32_t ctr[4] = {i, step, k, DOM_POISSON}, key[2] = {seed, 0u}, w[4];
+    philox4x32_10(ctr, key, w);
+    return canonical2(w[0], w[1]);
+  };
+  if (!(lam > 0.0)) return 0;
+  if (lam < 30.0) {
+    const double x = u(0);
+    double p = std::exp(-lam), F = p;
+    uint32_t k = 0;
+    while (x > F && k < 10000) {
+      ++k;
+      p *= lam / (double)k;
+      F += p;
+      if (p < 1e-300 && (double)k > lam) break;
+    }
+    return k;
+  }
+  const double slam = std::sqrt(lam), loglam = std::log(lam);
+  const double b = 0.931 + 2.53 * slam;
+  const double a = -0.059 + 0.02483 * b;
+  const double invalpha = 1.1239 + 1.1328 / (b - 3.4);
+  const double vr = 0.9277 - 3.6224 / (b - 2.0);
+  for (uint32_t it = 0; it < (1u << 20); it += 2) {
+    const double U = u(it) - 0.5;
+    const double V = u(it + 1);
+    const double us = 0.5 - std::fabs(U);
+    const double kd = std::floor((2.0 * a / us + b) * U + lam + 0.43);
+    if (us >= 0.07 && V <= vr) return (uint32_t)kd;
+    if (kd < 0 || (us < 0.013 && V > us)) continue;
+    if (std::log(V) + std::log(invalpha) - std::log(a / (us * us) + b) <=
+        -lam + kd * loglam - std::lgamma(kd + 1.0))
+      return (uint32_t)kd;
+  }
+  return (uint32_t)std::floor(lam);
 }
 
 // ---------------------------------------------------------------------------
@@ -750,6 +794,108 @@ int orc_pt(void* pp, const orc_opts* o, const double* values, int n_values, int 
 // `omp parallel for schedule(static,1)`: citizen i draws from thread i % T,
 // threads visiting their citizens in increasing order, which this sequential
 // loop reproduces exactly for T = opts.threads (default 1).
+void orc_poisson_counter(double lam, uint32_t seed, uint32_t step, int n, uint32_t* out) {
+  for (int i = 0; i < n; ++i) out[i] = poisson_counter(lam, (uint32_t)i, step, seed);
+}
+
+// One population annealing step in counter mode, after the step's sweeps:
+// the decisions of orc_pa's step loop below (population_annealing.h:262-488,
+// resample :490-616, Poisson branch) for the costs and lowest costs the
+// sweeps left, with offspring counts from poisson_counter(citizen, step) and
+// the pick of an emptied population from Philox (0, step, 0, DOM_PICK).
+// Offspring of citizen i fill consecutive slots in citizen order (copies of
+// one citizen are identical, so this is orc_pa's order); after a restart a
+// population below the target is topped up with copies of its last citizen
+// (fresh states follow for everyone, as init_population draws them).
+// Outputs: src[0..R_final) = the citizen each new slot copies, fam_out =
+// families after the step; iout = {R_final, R_resampled, moved, restart,
+// best_src (-1: no new global lowest)}; dout = {rho, dbeta, new best}.
+int orc_pa_step_counter(int R, const double* cost, const double* lowest, const int32_t* family,
+                        int64_t step, int64_t base_step, const double* beta_table, int n_table, int target,
+                        int cap, double alpha, double scale, uint32_t seed, int have_best, double best,
+                        int32_t* src, int32_t* fam_out, int32_t* iout, double* dout) {
+  if (R < 1 || n_table < 2 || cap < target) return -1;
+  // update_lowest_cost over the citizens (first minimum)
+  int barg = 0;
+  for (int i = 1; i < R; ++i)
+    if (lowest[i] < lowest[barg]) barg = i;
+  const bool improved = !have_best || lowest[barg] < best;
+  double avg = 0, mn = std::numeric_limits<double>::max();
+  for (int i = 0; i < R; ++i) {
+    avg += cost[i];
+    mn = cost[i] < mn ? cost[i] : mn;
+  }
+  avg /= (double)R;
+  double var = 0;
+  for (int i = 0; i < R; ++i) var += (cost[i] - avg) * (cost[i] - avg);
+  var /= (double)R;
+  const int64_t k = step - base_step, last = n_table - 1;
+  const double db = beta_table[std::min(k + 1, last)] - beta_table[std::min(k, last)];
+  double rho = 1.0;
+  int Rn = R;
+  bool moved = false;
+  std::vector<int32_t> fam(family, family + R);
+  if (db > 0) {
+    std::vector<double> w(R);
+    double total = 0;
+    for (int i = 0; i < R; ++i) {
+      w[i] = std::exp(-db * (cost[i] - mn) * scale);
+      total += w[i];
+    }
+    const double rrq = (double)target / total;
+    std::vector<int32_t> s2;
+    std::vector<int32_t> f2;
+    for (int i = 0; i < R; ++i) {
+      const double mean = rrq * w[i];
+      const uint32_t c = mean > 0.0 ? poisson_counter(mean, (uint32_t)i, (uint32_t)step, seed) : 0;
+      for (uint32_t j = 0; j < c; ++j) {
+        s2.push_back(i);
+        f2.push_back(family[i]);
+      }
+    }
+    if (s2.empty()) {
+      uint32_t ctr[4] = {0u, (uint32_t)step, 0u, DOM_PICK}, key[2] = {seed, 0u}, wd[4];
+      philox4x32_10(ctr, key, wd);
+      const int i = (int)std::floor(canonical2(wd[0], wd[1]) * (double)R);
+      s2.push_back(i);
+      f2.push_back(family[i]);
+    }
+    if ((int)s2.size() > cap) return -2;  // the device clamps and flags; not restated
+    std::map<int32_t, int64_t> census;
+    for (int32_t f : f2) census[f]++;
+    double sq = 0;
+    for (auto& kv : census) sq += (double)kv.second * (double)kv.second;
+    rho = sq / (double)R;
+    Rn = (int)s2.size();
+    for (int j = 0; j < Rn; ++j) src[j] = s2[j];
+    fam = f2;
+    moved = true;
+  } else {
+    for (int j = 0; j < R; ++j) src[j] = j;
+  }
+  const bool restart = (double)R / rho < alpha || var == 0.0;
+  int Rf = Rn;
+  if (restart) {
+    if (Rn < target) {
+      for (int j = Rn; j < target; ++j) src[j] = src[Rn - 1];
+      fam.resize(target);
+      Rf = target;
+    }
+    for (int j = 0; j < Rf && j < target; ++j) fam[j] = j;
+    moved = true;
+  }
+  for (int j = 0; j < Rf; ++j) fam_out[j] = fam[j];
+  iout[0] = Rf;
+  iout[1] = Rn;
+  iout[2] = moved ? 1 : 0;
+  iout[3] = restart ? 1 : 0;
+  iout[4] = improved ? barg : -1;
+  dout[0] = rho;
+  dout[1] = db;
+  dout[2] = improved ? lowest[barg] : best;
+  return 0;
+}
+
 int orc_pa(void* pp, const orc_opts* o, int geometric, double beta_start, double beta_stop,
            int64_t step_limit, int population, int sweeps_per_replica, double alpha,
            int constant_population, double* best_cost, uint8_t* best_state,

@@ -103,6 +103,11 @@ int orc_pt(void* p, const orc_opts* o, const double* values, int n_values, int u
 /* PopulationAnnealing (solver/population_annealing.h), linear/geometric
  * schedule, Poisson (variable population) or constant_population resampling.
  * Reference mode only.  Returns best cost, best state, final population size. */
+void orc_poisson_counter(double lam, uint32_t seed, uint32_t step, int n, uint32_t* out);
+int orc_pa_step_counter(int R, const double* cost, const double* lowest, const int32_t* family,
+                        int64_t step, int64_t base_step, const double* beta_table, int n_table, int target,
+                        int cap, double alpha, double scale, uint32_t seed, int have_best, double best,
+                        int32_t* src, int32_t* fam_out, int32_t* iout, double* dout);
 int orc_pa(void* p, const orc_opts* o, int geometric, double beta_start, double beta_stop,
            int64_t step_limit, int population, int sweeps_per_replica, double alpha,
            int constant_population, double* best_cost, uint8_t* best_state,

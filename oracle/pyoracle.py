@@ -51,6 +51,11 @@ def lib(variant=""):
     L.orc_pcg_uniform.argtypes = [C.c_int, C.c_uint32, C.c_int, P(C.c_double)]
     L.orc_pcg_fork_uniform.argtypes = [C.c_uint32, C.c_int, C.c_int, P(C.c_double)]
     L.orc_pcg_poisson.argtypes = [C.c_uint32, C.c_double, C.c_int, P(C.c_uint32)]
+    L.orc_poisson_counter.argtypes = [C.c_double, C.c_uint32, C.c_uint32, C.c_int, P(C.c_uint32)]
+    L.orc_pa_step_counter.argtypes = [C.c_int, P(C.c_double), P(C.c_double), P(C.c_int32), C.c_int64,
+                                      C.c_int64, P(C.c_double), C.c_int, C.c_int, C.c_int, C.c_double,
+                                      C.c_double, C.c_uint32, C.c_int, C.c_double, P(C.c_int32),
+                                      P(C.c_int32), P(C.c_int32), P(C.c_double)]
     L.orc_philox4x32_10.argtypes = [P(C.c_uint32), P(C.c_uint32), P(C.c_uint32)]
     L.orc_uniform_ln.argtypes = [P(C.c_double)]
     L.orc_accept.argtypes = [C.c_double, C.c_double, P(C.c_uint32), P(C.c_int)]
@@ -246,6 +251,37 @@ def pcg_fork_uniform(seed, n_forks, n):
     out = np.zeros((n_forks + 1) * n)
     lib().orc_pcg_fork_uniform(seed, n_forks, n, _p(out, C.c_double))
     return out.reshape(n_forks + 1, n)
+
+
+def poisson_counter(lam, seed, step, n):
+    """Counter-mode Poisson draws for citizens 0..n-1 of a step."""
+    out = np.zeros(n, dtype=np.uint32)
+    lib().orc_poisson_counter(lam, seed & 0xFFFFFFFF, step, n, _p(out, C.c_uint32))
+    return out
+
+
+def pa_step_counter(cost, lowest, family, step, base_step, beta_table, target, cap, alpha, scale, seed,
+                    have_best, best):
+    """One counter-mode population annealing step's decisions (orc_pa_step_counter)."""
+    cost = np.ascontiguousarray(cost, dtype=np.float64)
+    lowest = np.ascontiguousarray(lowest, dtype=np.float64)
+    family = np.ascontiguousarray(family, dtype=np.int32)
+    table = np.ascontiguousarray(beta_table, dtype=np.float64)
+    R = len(cost)
+    src = np.zeros(max(cap, target), dtype=np.int32)
+    fam = np.zeros(max(cap, target), dtype=np.int32)
+    iout = np.zeros(5, dtype=np.int32)
+    dout = np.zeros(3)
+    rc = lib().orc_pa_step_counter(R, _p(cost, C.c_double), _p(lowest, C.c_double), _p(family, C.c_int32),
+                                   step, base_step, _p(table, C.c_double), len(table), target, cap, alpha,
+                                   scale, seed & 0xFFFFFFFF, int(have_best), best, _p(src, C.c_int32),
+                                   _p(fam, C.c_int32), _p(iout, C.c_int32), _p(dout, C.c_double))
+    if rc != 0:
+        raise ValueError("orc_pa_step_counter failed (rc=%d)" % rc)
+    Rf = int(iout[0])
+    return dict(src=src[:Rf], family=fam[:Rf], R=Rf, R_resampled=int(iout[1]), moved=bool(iout[2]),
+                restart=bool(iout[3]), best_src=int(iout[4]), rho=float(dout[0]), dbeta=float(dout[1]),
+                best=float(dout[2]))
 
 
 def philox(ctr, key):

@@ -26,7 +26,8 @@ EXPORTS = [
     "qmc_walkers_plan", "qmc_walkers_plan_pt", "qmc_walkers_remaining", "qmc_batch_create",
     "qmc_batch_destroy", "qmc_batch_add", "qmc_batch_remove", "qmc_batch_clear", "qmc_batch_run",
     "qmc_batch_wait", "qmc_walkers_read_replica", "qmc_batch_read", "qmc_batch_run_chunked", "qmc_walkers_resample", "qmc_walkers_count", "qmc_walkers_set_pt_mode",
-    "qmc_walkers_set_lanes", "qmc_walkers_get_lanes",
+    "qmc_walkers_set_lanes", "qmc_walkers_get_lanes", "qmc_walkers_pa_begin", "qmc_walkers_pa_step",
+    "qmc_walkers_pa_info", "qmc_walkers_pa_end",
 ]
 
 
@@ -42,6 +43,14 @@ class ProblemInfo(C.Structure):
         ("n_classes", C.c_int32), ("lanes", C.c_int32), ("coupling", C.c_int32),
         ("pairwise", C.c_int32), ("max_degree", C.c_int32), ("mean_degree", C.c_double),
         ("max_class", C.c_int32), ("pad_", C.c_int32),
+    ]
+
+
+class PaInfo(C.Structure):
+    _fields_ = [
+        ("population", C.c_int32), ("restarts", C.c_int32), ("overflow", C.c_int32),
+        ("have_best", C.c_int32), ("steps", C.c_int64), ("attempts", C.c_int64),
+        ("best", C.c_double), ("beta", C.c_double), ("rho", C.c_double),
     ]
 
 
@@ -105,6 +114,11 @@ def lib():
     L.qmc_walkers_set_pt_mode.argtypes = [vp, C.c_int]
     L.qmc_walkers_set_lanes.argtypes = [vp, C.c_int32]
     L.qmc_walkers_get_lanes.argtypes = [vp, P(C.c_int32)]
+    L.qmc_walkers_pa_begin.argtypes = [vp, C.c_int32, C.c_double, C.c_double, P(C.c_double), C.c_int32,
+                                       C.c_uint32, C.c_int32]
+    L.qmc_walkers_pa_step.argtypes = [vp]
+    L.qmc_walkers_pa_info.argtypes = [vp, P(PaInfo), vp]
+    L.qmc_walkers_pa_end.argtypes = [vp]
     _lib = L
     return L
 
@@ -300,6 +314,30 @@ class Walkers:
         v = C.c_int64()
         _check(lib().qmc_walkers_remaining(self.h, C.byref(v)))
         return v.value
+
+    # population annealing with the population on the device
+    def pa_begin(self, population, alpha, scale, beta_table, seed, sweeps_per_step):
+        t = np.ascontiguousarray(beta_table, dtype=np.float64)
+        self._pa_table = t
+        _check(lib().qmc_walkers_pa_begin(self.h, population, alpha, scale, _p(t, C.c_double), len(t),
+                                          seed & 0xFFFFFFFF, sweeps_per_step))
+
+    def pa_step(self):
+        _check(lib().qmc_walkers_pa_step(self.h))
+
+    def pa_info(self):
+        info = PaInfo()
+        best = np.zeros(max(self.n, 1), dtype=np.uint8)
+        _check(lib().qmc_walkers_pa_info(self.h, C.byref(info), _p(best, C.c_uint8)))
+        out = {f: getattr(info, f) for f, _ in PaInfo._fields_}
+        out["best_state"] = best[:self.n]
+        return out
+
+    def pa_end(self):
+        _check(lib().qmc_walkers_pa_end(self.h))
+        n = C.c_int32()
+        _check(lib().qmc_walkers_count(self.h, C.byref(n)))
+        self.R = n.value
 
 
 class Batch:

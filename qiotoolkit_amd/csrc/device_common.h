@@ -7,7 +7,7 @@
 namespace qmc {
 
 // Counter domains: ctr = (c0, c1, c2, domain).
-enum : uint32_t { DOM_SWEEP = 0, DOM_REDRAW = 1, DOM_INIT = 2, DOM_EXCHANGE = 3 };
+enum : uint32_t { DOM_SWEEP = 0, DOM_REDRAW = 1, DOM_INIT = 2, DOM_EXCHANGE = 3, DOM_POISSON = 4, DOM_PICK = 5 };
 
 // Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11).  The 32x32->64 products
 // map to v_mad_u64_u32; the key schedule is wave-uniform when the key is.

@@ -1,15 +1,16 @@
 // PopulationAnnealing (solver/population_annealing.h:102-616) over the device
-// engine: each step sweeps every citizen on the device, the host computes the
-// Boltzmann weights and Poisson offspring counts, and the device copies the
-// resampled population (Population::resample, solver/population.h:171-218).
-// Random numbers follow the counter contract: citizens' sweeps are keyed by
-// their slot and the global sweep index; offspring counts by (citizen, step).
+// engine: each step sweeps every citizen, then computes the Boltzmann
+// weights, Poisson offspring counts, the resampled population
+// (Population::resample, solver/population.h:171-218) and restarts on the
+// device, with the population size in device memory -- the host enqueues
+// steps and reads the outcome once at the end.  Random numbers follow the
+// counter contract: citizens' sweeps are keyed by their slot and the global
+// sweep index; offspring counts by (citizen, step).
 #include <algorithm>
 #include <cmath>
 #include <map>
 #include <numeric>
 
-#include "rng.h"
 #include "solver.h"
 
 namespace qiotk {
@@ -77,6 +78,7 @@ void PopulationAnnealing::configure(const Json& doc) {
   beta_start_ = beta_.get_value(0);
   if (p.has("sweeps_per_replica")) sweeps_per_replica_ = (int)p.at("sweeps_per_replica").as_int();
   if (sweeps_per_replica_ <= 0) throw Error(kValueError, "parameter `sweeps_per_replica` must be greater than 0");
+  if (p.has("replicas_per_group")) group_ = (int)p.at("replicas_per_group").as_int();
   set_output("population", Json(target_population_));
   set_output("alpha", alpha_);
   set_output("sweeps_per_replica", Json((int64_t)sweeps_per_replica_));
@@ -90,16 +92,19 @@ void PopulationAnnealing::init() {
   // Poisson resampling keeps the population near the target; room for
   // fluctuations is reserved up front.
   st.cap = (int32_t)std::max<int64_t>(4 * target_population_, target_population_ + 256);
-  device_check(qmc_walkers_create(dmodel_->handle(), st.cap, 8, seed_, 0, 0, st.cap, &st.w),
+  const int B = group_ > 0 ? group_ : default_group(target_population_);
+  device_check(qmc_walkers_create(dmodel_->handle(), st.cap, B, seed_, 0, 0, st.cap, &st.w),
                "qmc_walkers_create");
-  std::vector<int32_t> ident(target_population_);
-  std::iota(ident.begin(), ident.end(), 0);
-  device_check(qmc_walkers_resample(st.w, ident.data(), (int32_t)target_population_), "resample");
+  apply_lanes(st.w, B);
+  // beta k steps after the last restart, k = 0..steps (get_value clamps past
+  // the schedule's end)
+  const int64_t steps = steps_to_run();
+  std::vector<double> table((size_t)steps + 2);
+  for (size_t k = 0; k < table.size(); ++k) table[k] = beta_.get_value((double)k);
+  device_check(qmc_walkers_pa_begin(st.w, (int32_t)target_population_, alpha_, model_->scale_factor(),
+                                    table.data(), (int32_t)table.size(), seed_, sweeps_per_replica_),
+               "qmc_walkers_pa_begin");
   device_check(qmc_walkers_init(st.w, 0), "qmc_walkers_init");
-  std::vector<double> b(target_population_, beta_start_);
-  device_check(qmc_walkers_set_betas(st.w, b.data()), "qmc_walkers_set_betas");
-  family_.resize(target_population_);
-  std::iota(family_.begin(), family_.end(), (size_t)0);
   cur_beta_ = beta_start_;
 }
 
@@ -139,130 +144,28 @@ qmc_walkers PopulationAnnealing::walkers() const {
   return impl_ ? static_cast<PaState*>(impl_.get())->w : nullptr;
 }
 
-// The host part of one step after its sweeps (population_annealing.h:262-488):
-// lowest-cost update, weights, Poisson offspring, resampling and restarts.
+// The rest of a step after its sweeps (population_annealing.h:262-488):
+// enqueued on the device, nothing waits for it.
 void PopulationAnnealing::finish_step() {
   PaState& st = pa_state(impl_);
-  const int64_t step = step_;
-  const int64_t N = model_->n_vars;
-  const double scale = model_->scale_factor();
-  int32_t R = 0;
-  qmc_walkers_count(st.w, &R);
-  const double Rd = (double)R;
+  device_check(qmc_walkers_pa_step(st.w), "qmc_walkers_pa_step");
   sweep_ += sweeps_per_replica_;
-  attempts_ += (int64_t)R * sweeps_per_replica_ * N;
-  std::vector<double> after(R, 0.0), lowest(R, 0.0);
-  device_check(qmc_walkers_read(st.w, after.data(), lowest.data(), nullptr, nullptr, nullptr, nullptr),
-               "qmc_walkers_read");
-  // update_lowest_cost over the citizens (:301-308)
-  const int64_t kbest = std::min_element(lowest.begin(), lowest.end()) - lowest.begin();
-  if (!have_lowest_ || lowest[kbest] < lowest_cost_) {
-    std::vector<uint8_t> ls((size_t)N);
-    device_check(qmc_walkers_read_replica(st.w, (int32_t)kbest, nullptr, ls.data()), "qmc_walkers_read_replica");
-    record(lowest[kbest], ls);
-  }
-  double avg_after = 0, min_cost = std::numeric_limits<double>::max();
-  for (double c : after) {
-    avg_after += c;
-    min_cost = c < min_cost ? c : min_cost;
-  }
-  avg_after /= Rd;
-  double var_after = 0;
-  for (double c : after) var_after += (c - avg_after) * (c - avg_after);
-  var_after /= Rd;
-  for (double& c : after) c -= min_cost;
-  const double delta_beta = beta_.get_value((double)(step + 1 - base_step_)) -
-                            beta_.get_value((double)(step - base_step_));
-  double rho_t = 1.0;
-  if (delta_beta > 0) {
-    // resample (:490-616), Poisson branch
-    std::vector<double> weight(R);
-    double total = 0;
-    for (int32_t i = 0; i < R; ++i) {
-      weight[i] = std::exp(-delta_beta * after[i] * scale);
-      total += weight[i];
-    }
-    const double RRQ = (double)target_population_ / total;
-    std::vector<uint32_t> count(R, 1);
-    int64_t current = R;
-    for (int32_t i = 0; i < R; ++i) {
-      const double mean = RRQ * weight[i];
-      uint32_t nc = 0;
-      if (mean > 0.0) {
-        nc = poisson_draw(mean, [&](uint32_t k) {
-          Words4 w = philox4x32_10((uint32_t)i, (uint32_t)step, k, DOM_POISSON, seed_, 0u);
-          return canonical2(w.w[0], w.w[1]);
-        });
-      }
-      if (nc == 0) {
-        current -= count[i];
-        count[i] = 0;
-      } else if (nc > 1) {
-        current += nc - 1;
-        count[i] += nc - 1;
-      }
-    }
-    if (current == 0) {  // keep the population alive
-      Words4 w = philox4x32_10(0u, (uint32_t)step, 0u, DOM_PICK, seed_, 0u);
-      const int32_t i = (int32_t)std::floor(canonical2(w.w[0], w.w[1]) * Rd);
-      count[i] += 1;
-      current = 1;
-    }
-    if (current > st.cap)
-      throw Error(kMemoryLimited, "population grew beyond the reserved capacity");
-    std::vector<int32_t> src;
-    std::vector<size_t> fam;
-    src.reserve(current);
-    for (int32_t i = 0; i < R; ++i) {
-      for (uint32_t k = 0; k < count[i]; ++k) {
-        src.push_back(i);
-        fam.push_back(family_[i]);
-      }
-    }
-    device_check(qmc_walkers_resample(st.w, src.data(), (int32_t)src.size()), "qmc_walkers_resample");
-    family_.swap(fam);
-    std::map<size_t, size_t> census;
-    for (size_t f : family_) census[f]++;
-    rho_t = 0;
-    for (auto& kv : census) rho_t += (double)kv.second * (double)kv.second;
-    rho_t /= Rd;
-  }
-  if (Rd / rho_t < alpha_ || var_after == 0) {
-    // restart (:461-475): fresh random citizens; existing ones keep their
-    // beta until the next step sets it, as init_population does
-    cur_beta_ = beta_start_;
-    base_step_ = step;
-    int32_t Rnow = 0;
-    qmc_walkers_count(st.w, &Rnow);
-    if (Rnow < target_population_) {
-      std::vector<double> b(Rnow);
-      device_check(qmc_walkers_read(st.w, nullptr, nullptr, b.data(), nullptr, nullptr, nullptr), "read");
-      std::vector<int32_t> src(target_population_);
-      for (int64_t i = 0; i < target_population_; ++i) src[i] = (int32_t)std::min<int64_t>(i, Rnow - 1);
-      device_check(qmc_walkers_resample(st.w, src.data(), (int32_t)target_population_), "resample");
-      b.resize(target_population_, beta_start_);
-      device_check(qmc_walkers_set_betas(st.w, b.data()), "qmc_walkers_set_betas");
-      Rnow = (int32_t)target_population_;
-    }
-    device_check(qmc_walkers_init(st.w, ++epoch_), "qmc_walkers_init");
-    family_.resize(Rnow);
-    for (int64_t i = 0; i < Rnow; ++i)
-      if (i < target_population_) family_[i] = (size_t)i;
-    restarts_++;
-  } else {
-    cur_beta_ += delta_beta;
-    int32_t Rnow = 0;
-    qmc_walkers_count(st.w, &Rnow);
-    std::vector<double> b(Rnow, cur_beta_);
-    device_check(qmc_walkers_set_betas(st.w, b.data()), "qmc_walkers_set_betas");
-  }
-  step_ = step + 1;
+  step_ += 1;
   steps_done_ = step_;
 }
 
 void PopulationAnnealing::finalize() {
   if (model_->empty()) return;
   PaState& st = pa_state(impl_);
+  qmc_pa_info info{};
+  std::vector<uint8_t> best_state((size_t)model_->n_vars);
+  device_check(qmc_walkers_pa_info(st.w, &info, best_state.data()), "qmc_walkers_pa_info");
+  if (info.overflow) throw Error(kMemoryLimited, "population grew beyond the reserved capacity");
+  restarts_ = info.restarts;
+  attempts_ = info.attempts;
+  cur_beta_ = info.beta;
+  if (info.have_best) record(info.best, best_state);
+  device_check(qmc_walkers_pa_end(st.w), "qmc_walkers_pa_end");
   int32_t R = 0;
   qmc_walkers_count(st.w, &R);
   const int64_t N = model_->n_vars;

@@ -89,8 +89,10 @@ int64_t footprint(const Model& m, const std::string& target, int64_t replicas) {
     cap = std::max<int64_t>(4 * replicas, replicas + 256);
     staging = 2 * cap * N;
   }
-  const int64_t spins = 2 * cap * ((N + 8) / 8 + 4);  // 1 bit per replica, 2 copies, padding
-  return problem + 2 * spins + staging + 64 * cap;
+  // current + lowest states: a work-group of B replicas holds ceil(B/8)
+  // bytes per position, so at most one byte per replica and position
+  const int64_t spins = 2 * cap * (N + 4);
+  return problem + spins + staging + 64 * cap;
 }
 
 }  // namespace

@@ -6,7 +6,6 @@
 #include <limits>
 #include <numeric>
 
-#include "rng.h"
 
 namespace qiotk {
 
@@ -32,13 +31,13 @@ double beta_from(double v, bool use_inverse) {
 // Replicas per work-group when the caller does not pick one: enough
 // work-groups to spread over the CUs, up to 8 replicas sharing each
 // neighbour byte.
+}  // namespace
+
 int default_group(int64_t R) {
   int B = 1;
   while (B < 8 && R / (2 * B) >= 128) B *= 2;
   return (int)std::min<int64_t>(B, R);
 }
-
-}  // namespace
 
 void Solver::configure(const Json& doc) {
   const Json& p = params_of(doc);

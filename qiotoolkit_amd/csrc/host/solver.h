@@ -168,8 +168,8 @@ class ParallelTempering : public Solver {
   std::vector<uint8_t> lowest_states_flat_;
 };
 
-// population_annealing.h: device sweeps, host resampling (Poisson), linear
-// or geometric beta schedule.
+// population_annealing.h: device sweeps and device resampling (Poisson),
+// linear or geometric beta schedule.
 class PopulationAnnealing : public Solver {
  public:
   std::string target() const override { return "populationannealing.cpu"; }
@@ -185,8 +185,7 @@ class PopulationAnnealing : public Solver {
 
  private:
   void finish_step();
-  int64_t step_ = 0, sweep_ = 0, base_step_ = 0;
-  uint32_t epoch_ = 0;
+  int64_t step_ = 0, sweep_ = 0;
   bool in_batch_ = false;
   double alpha_ = 2.0;
   int64_t target_population_ = 64;
@@ -197,9 +196,13 @@ class PopulationAnnealing : public Solver {
   int sweeps_per_replica_ = 1;
   double cur_beta_ = 0.0;
   int64_t restarts_ = 0;
-  std::vector<size_t> family_;
+  int group_ = 0;
   std::shared_ptr<void> impl_;
 };
+
+// Replicas per work-group when the params do not say: small groups while
+// that still leaves at least 128 work-groups, so a launch fills the device.
+int default_group(int64_t replicas);
 
 std::unique_ptr<Solver> create_solver(const std::string& target);
 

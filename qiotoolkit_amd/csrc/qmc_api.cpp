@@ -244,6 +244,14 @@ struct qmc_walkers_s {
   int64_t planned = 0;       // sweeps of the last plan
   uint64_t version = 0;      // bumped whenever `units` changes
   bool units_stale = true;   // `d_units` behind `units` (uploaded before device use)
+  // population annealing with the population size on the device
+  int pa_on = 0;
+  PaArgs pa_args{};
+  DevBuf<PaDev> pa;
+  DevBuf<double> pa_table, pa_weight, pa_tcost, pa_tbest, pa_tbeta;
+  DevBuf<int32_t> pa_family, pa_family_next, pa_src, pa_hist, pa_count;
+  DevBuf<unsigned long long> pa_tacc;
+  DevBuf<uint8_t> pa_plane_cur, pa_plane_low, pa_best_state;
   float last_ms = 0.f;
 };
 
@@ -643,13 +651,20 @@ static void build_units(qmc_walkers w) {
     U.seed = w->seed + (uint32_t)(gr / w->per_seed);
     U.r0 = (uint32_t)(gr % w->per_seed);
     U.stream = w->stream;
+    U.rbase = r;
+    U.live = w->pa_on ? &w->pa.p->R : nullptr;
   }
 }
 
 // Rebuilds the host descriptors; the device copy follows lazily
 // (device_units), since batched runs launch from their own copies.
 static int upload_units(qmc_walkers w) {
+  std::vector<UnitDev> before;
+  before.swap(w->units);
   build_units(w);
+  if (before.size() == w->units.size() &&
+      std::memcmp(before.data(), w->units.data(), before.size() * sizeof(UnitDev)) == 0)
+    return QMC_OK;  // unchanged: batches keep their descriptors
   w->version++;
   w->units_stale = true;
   return QMC_OK;
@@ -768,6 +783,7 @@ static int ensure_scratch(qmc_walkers w) {
 
 int qmc_walkers_resample(qmc_walkers w, const int32_t* src, int32_t new_R) {
   if (!w || (!src && new_R > 0)) return fail(QMC_INVALID_ARGUMENT, "null argument");
+  if (w->pa_on) return fail(QMC_INVALID_ARGUMENT, "the population is on the device (qmc_walkers_pa_end first)");
   if (new_R < 1 || new_R > w->cap)
     return fail(QMC_MEMORY_LIMITED, "population exceeds the walker capacity");
   for (int32_t r = 0; r < new_R; ++r)
@@ -811,6 +827,129 @@ int qmc_walkers_count(qmc_walkers w, int32_t* n_replicas) {
   if (!w || !n_replicas) return fail(QMC_INVALID_ARGUMENT, "null argument");
   *n_replicas = w->R;
   return QMC_OK;
+}
+
+int qmc_walkers_pa_begin(qmc_walkers w, int32_t population, double alpha, double scale,
+                         const double* beta_table, int32_t n_table, uint32_t seed,
+                         int32_t sweeps_per_step) {
+  if (!w || !beta_table) return fail(QMC_INVALID_ARGUMENT, "null argument");
+  if (w->pa_on) return fail(QMC_INVALID_ARGUMENT, "population mode already on");
+  if (w->R != w->cap) return fail(QMC_INVALID_ARGUMENT, "walkers must hold their full capacity");
+  if (population < 1 || population > w->cap)
+    return fail(QMC_INVALID_ARGUMENT, "population must be in 1..capacity");
+  if (n_table < 2) return fail(QMC_INVALID_ARGUMENT, "beta_table needs at least 2 values");
+  if (sweeps_per_step < 1) return fail(QMC_INVALID_ARGUMENT, "sweeps_per_step must be >= 1");
+  qmc_problem p = w->prob;
+  qmc_ctx c = p->ctx;
+  const int cap = w->cap;
+  const size_t N = (size_t)p->N;
+  HIP_TRY(w->pa.alloc(1));
+  HIP_TRY(w->pa_table.alloc(n_table));
+  HIP_TRY(w->pa_weight.alloc(cap));
+  HIP_TRY(w->pa_tcost.alloc(cap));
+  HIP_TRY(w->pa_tbest.alloc(cap));
+  HIP_TRY(w->pa_tbeta.alloc(cap));
+  HIP_TRY(w->pa_tacc.alloc(cap));
+  HIP_TRY(w->pa_family.alloc(cap));
+  HIP_TRY(w->pa_family_next.alloc(cap));
+  HIP_TRY(w->pa_src.alloc(cap));
+  HIP_TRY(w->pa_count.alloc(cap));
+  HIP_TRY(w->pa_hist.alloc(population));
+  HIP_TRY(w->pa_plane_cur.alloc(std::max<size_t>((size_t)cap * N, 1)));
+  HIP_TRY(w->pa_plane_low.alloc(std::max<size_t>((size_t)cap * N, 1)));
+  HIP_TRY(w->pa_best_state.alloc(std::max<size_t>(N, 1)));
+  PaDev st{};
+  st.R = population;
+  st.best_src = -1;
+  st.cur_beta = beta_table[0];
+  HIP_TRY(upload_async(c, w->pa.p, &st, sizeof st));
+  HIP_TRY(upload_async(c, w->pa_table.p, beta_table, (size_t)n_table * sizeof(double)));
+  // family of citizen i = i; every family id stays below the population
+  std::vector<int32_t> fam(cap, 0);
+  for (int i = 0; i < population; ++i) fam[i] = i;
+  HIP_TRY(upload_async(c, w->pa_family.p, fam.data(), fam.size() * sizeof(int32_t)));
+  std::vector<double> b(cap, beta_table[0]);
+  HIP_TRY(upload_async(c, w->beta.p, b.data(), b.size() * sizeof(double)));
+  PaArgs& a = w->pa_args;
+  a = PaArgs{};
+  a.pa = w->pa.p;
+  a.beta_table = w->pa_table.p;
+  a.n_table = n_table;
+  a.target = population;
+  a.cap = cap;
+  a.n_vars = p->N;
+  a.sweeps_per_step = sweeps_per_step;
+  a.alpha = alpha;
+  a.scale = scale;
+  a.beta_start = beta_table[0];
+  a.seed = seed;
+  a.cost = w->cost.p;
+  a.best_cost = w->best_cost.p;
+  a.beta = w->beta.p;
+  a.accepted = w->accepted.p;
+  a.family = w->pa_family.p;
+  a.family_next = w->pa_family_next.p;
+  a.src = w->pa_src.p;
+  a.hist = w->pa_hist.p;
+  a.weight = w->pa_weight.p;
+  a.count = w->pa_count.p;
+  a.tcost = w->pa_tcost.p;
+  a.tbest = w->pa_tbest.p;
+  a.tbeta = w->pa_tbeta.p;
+  a.tacc = w->pa_tacc.p;
+  a.plane_cur = w->pa_plane_cur.p;
+  a.plane_low = w->pa_plane_low.p;
+  a.best_state = w->pa_best_state.p;
+  a.B = w->B;
+  w->pa_on = 1;
+  return upload_units(w);
+}
+
+int qmc_walkers_pa_step(qmc_walkers w) {
+  if (!w) return fail(QMC_INVALID_ARGUMENT, "null handle");
+  if (!w->pa_on) return fail(QMC_INVALID_ARGUMENT, "population mode is off");
+  const UnitDev* du = nullptr;
+  int rc = device_units(w, &du);
+  if (rc) return rc;
+  w->pa_args.units = du;
+  w->pa_args.n_units = w->n_units;
+  HIP_TRY(launch_pa_step(w->pa_args, w->prob->dev(), w->prob->ctx->stream));
+  return QMC_OK;
+}
+
+int qmc_walkers_pa_info(qmc_walkers w, qmc_pa_info* info, uint8_t* best_state) {
+  if (!w || !info) return fail(QMC_INVALID_ARGUMENT, "null argument");
+  if (!w->pa.p) return fail(QMC_INVALID_ARGUMENT, "population mode was never started");
+  hipStream_t st = w->prob->ctx->stream;
+  HIP_TRY(hipStreamSynchronize(st));
+  PaDev d{};
+  HIP_TRY(hipMemcpy(&d, w->pa.p, sizeof d, hipMemcpyDeviceToHost));
+  info->population = d.R;
+  info->restarts = d.restarts;
+  info->overflow = d.err;
+  info->have_best = d.have_best;
+  info->steps = d.step;
+  info->attempts = d.attempts;
+  info->best = d.best;
+  info->beta = d.cur_beta;
+  info->rho = d.rho;
+  if (best_state && w->prob->N > 0)
+    HIP_TRY(hipMemcpy(best_state, w->pa_best_state.p, (size_t)w->prob->N, hipMemcpyDeviceToHost));
+  return QMC_OK;
+}
+
+int qmc_walkers_pa_end(qmc_walkers w) {
+  if (!w) return fail(QMC_INVALID_ARGUMENT, "null handle");
+  if (!w->pa_on) return QMC_OK;
+  HIP_TRY(hipStreamSynchronize(w->prob->ctx->stream));
+  int32_t R = 0;
+  HIP_TRY(hipMemcpy(&R, &w->pa.p->R, sizeof R, hipMemcpyDeviceToHost));
+  if (R < 1 || R > w->cap) return fail(QMC_RUNTIME_ERROR, "device population size out of range");
+  w->pa_on = 0;
+  w->R = R;
+  w->n_units = (R + w->B - 1) / w->B;
+  drop_graph(w);
+  return upload_units(w);
 }
 
 int qmc_walkers_init(qmc_walkers w, uint32_t epoch) {

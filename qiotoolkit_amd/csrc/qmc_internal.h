@@ -69,11 +69,80 @@ struct UnitDev {
   int64_t sched_base;
   int64_t* progress;     // [0] next sweep index, [1] sweeps left (updated by the kernel)
   int32_t spr;           // PT: sweeps per exchange step
-  int32_t pad0;
+  int32_t rbase;         // index of the unit's first replica in its walker set
   uint32_t r0;      // replica id of the unit's first replica (RNG counter)
   uint32_t nrep;
   uint32_t seed, stream;
+  // population on the device: replicas rbase + b >= *live are idle (null: all
+  // nrep replicas are live)
+  const int32_t* live;
 };
+
+// Replicas of the unit that take part: nrep, or fewer when the population
+// size lives on the device.
+__host__ __device__ inline int unit_nrep(const UnitDev& U) {
+  int n = (int)U.nrep;
+  if (U.live) {
+    const int left = *U.live - U.rbase;
+    n = left < 0 ? 0 : (left < n ? left : n);
+  }
+  return n;
+}
+
+// Population annealing state kept on the device (one per walker set); the
+// step kernel reads and updates it, the host reads it at the end.
+struct PaDev {
+  int32_t R;          // live population: replicas 0..R-1 of the walker set
+  int32_t R_prev;     // population the last step started from
+  int32_t R_resampled;// population after its resampling, before a restart's top-up
+  int32_t moved;      // the last step copied citizens (resampling or restart)
+  int32_t restart;    // the last step restarted the population
+  int32_t err;        // the population outgrew the capacity (clamped)
+  int32_t restarts;   // restarts so far
+  int32_t best_src;   // replica holding a new global lowest after the last step, -1 none
+  int32_t have_best;
+  uint32_t epoch;     // initial-state epoch of the last restart
+  int64_t step;       // next step
+  int64_t base_step;  // step of the last restart
+  int64_t attempts;   // spin-flip attempts of the steps so far
+  double cur_beta;
+  double best;        // lowest cost any citizen has reached
+  double rho;         // family statistic of the last resampling
+};
+
+struct PaArgs {
+  PaDev* pa;
+  const double* beta_table;  // beta schedule value at k = step - base_step, k = 0..n_table-1
+  int32_t n_table;
+  int32_t target, cap, n_vars;
+  int32_t sweeps_per_step, pad;
+  double alpha, scale, beta_start;
+  uint32_t seed;
+  double* cost;
+  double* best_cost;
+  double* beta;
+  unsigned long long* accepted;
+  int32_t* family;
+  int32_t* family_next;
+  int32_t* src;
+  int32_t* hist;             // target entries
+  double* weight;            // cap entries
+  int32_t* count;            // cap entries
+  double* tcost;             // gathered copies (cap entries each)
+  double* tbest;
+  double* tbeta;
+  unsigned long long* tacc;
+  uint8_t* plane_cur;        // plain states, cap x n_vars
+  uint8_t* plane_low;
+  uint8_t* best_state;       // n_vars
+  const UnitDev* units;
+  int32_t n_units, B;
+};
+// One population annealing step after the step's sweeps
+// (population_annealing.h:262-488): statistics, the global lowest, weights,
+// Poisson offspring and their scan, the resampling copies and a restart's
+// fresh citizens -- all on `st`, reading the population size on the device.
+hipError_t launch_pa_step(const PaArgs& a, const ProblemDev& P, hipStream_t st);
 
 struct SweepArgs {
   int32_t n_sweeps;      // at most this many sweeps per unit in this launch

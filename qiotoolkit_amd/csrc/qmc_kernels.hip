@@ -333,7 +333,8 @@ k_sweep(const UnitDev* units, SweepArgs a) {
   const int tp = tid - (g << lg);
   const int lane = tid & 63;
   const int wave_in_group = tp >> 6;
-  const int nrep = (int)U.nrep;
+  const int nrep = unit_nrep(U);
+  if (nrep == 0) return;  // idle unit of a device-sized population (uniform)
   const int nb_g = active ? min(NB, nrep - 8 * g) : 0;
   const uint32_t rbase = U.r0 + 8u * (uint32_t)g;
   const uint32_t key0 = in_vgpr(U.seed), key1 = in_vgpr(U.stream);
@@ -695,7 +696,8 @@ k_sweep_ordered(const UnitDev* __restrict__ units, SweepArgs a) {
   const int G = U.G, N = U.N, stride = U.stride;
   uint8_t* start = sp + stride;             // start-of-sweep copy
   const int lane = threadIdx.x;
-  const int nrep = (int)U.nrep;
+  const int nrep = unit_nrep(U);
+  if (nrep == 0) return;  // idle unit of a device-sized population (uniform)
   for (int i = lane; i < 1024; i += 64) lut[i] = a.lut[i];
   for (int i = lane; i < stride / 4; i += 64)
     reinterpret_cast<uint32_t*>(sp)[i] = reinterpret_cast<const uint32_t*>(U.spins)[i];
@@ -862,7 +864,7 @@ __global__ void k_init_states(const UnitDev* __restrict__ units, uint32_t epoch)
       const uint32_t wi = i >> 5;
       for (int b = 0; b < 8; ++b) {
         const int rr = 8 * g + b;
-        if (rr < (int)U.nrep) {
+        if (rr < unit_nrep(U)) {
           const uint32_t r = U.r0 + (uint32_t)rr;
           const uint4 w = philox4x32_10(wi >> 2, r, epoch, DOM_INIT, U.seed, U.stream);
           const uint32_t word = pick(w, wi & 3u);
@@ -902,7 +904,7 @@ __global__ void k_pack(const UnitDev* __restrict__ units, int B, int R, const ui
       for (int b = 0; b < 8; ++b) {
         const int rr = 8 * g + b;
         const int r = u * B + rr;
-        if (rr < (int)U.nrep && r < R) byte |= ((uint32_t)(in[(int64_t)r * U.N + v] & 1u)) << b;
+        if (rr < unit_nrep(U) && r < R) byte |= ((uint32_t)(in[(int64_t)r * U.N + v] & 1u)) << b;
       }
     }
     if (which == 0 || which == 2) U.spins[q * U.G + g] = (uint8_t)byte;
@@ -926,7 +928,7 @@ __global__ void k_pack_gather(const UnitDev* __restrict__ units, int B, int R,
       for (int b = 0; b < 8; ++b) {
         const int rr = 8 * g + b;
         const int r = u * B + rr;
-        if (rr < (int)U.nrep && r < R) {
+        if (rr < unit_nrep(U) && r < R) {
           const int64_t s = (int64_t)src[r] * U.N + v;
           bc |= ((uint32_t)(cur[s] & 1u)) << b;
           bl |= ((uint32_t)(low[s] & 1u)) << b;
@@ -1064,6 +1066,381 @@ __global__ void k_philox(int n, const uint32_t* __restrict__ ctr, uint32_t k0, u
   out[4 * t + 1] = w.y;
   out[4 * t + 2] = w.z;
   out[4 * t + 3] = w.w;
+}
+
+// ---------------------------------------------------------------------------
+// Population annealing on the device (solver/population_annealing.h:262-616).
+// The population size lives in PaDev::R; units past it are idle.  One step
+// runs after the step's sweeps as a chain of launches on one stream, each
+// reading the step's decisions from PaDev: statistics, weights, Poisson
+// offspring and their scan in one 1024-thread work-group (k_pa_step), then
+// the copies of the resampled citizens and, on a restart, fresh ones.
+
+// Poisson variate with mean `lam` from the counter uniforms
+// u(k) = canonical2(Philox(i, step, k, DOM_POISSON) words 0, 1) under the key
+// (seed, 0): inversion by sequential search below 30, else the PTRS
+// transformed rejection of W. Hormann, "The transformed rejection method for
+// generating Poisson random variables", Insurance: Mathematics and Economics
+// 12 (1993) 39-45.
+__device__ uint32_t pa_poisson(double lam, uint32_t i, uint32_t step, uint32_t seed) {
+  auto u = [&](uint32_t k) {
+    const uint4 w = philox4x32_10(i, step, k, DOM_POISSON, seed, 0u);
+    return canonical2(w.x, w.y);
+  };
+  if (!(lam > 0.0)) return 0;
+  if (lam < 30.0) {
+    const double x = u(0);
+    double p = exp(-lam), F = p;
+    uint32_t k = 0;
+    while (x > F && k < 10000) {
+      ++k;
+      p *= lam / (double)k;
+      F += p;
+      if (p < 1e-300 && (double)k > lam) break;
+    }
+    return k;
+  }
+  const double slam = sqrt(lam), loglam = log(lam);
+  const double b = 0.931 + 2.53 * slam;
+  const double a = -0.059 + 0.02483 * b;
+  const double invalpha = 1.1239 + 1.1328 / (b - 3.4);
+  const double vr = 0.9277 - 3.6224 / (b - 2.0);
+  for (uint32_t it = 0; it < (1u << 20); it += 2) {
+    const double U = u(it) - 0.5;
+    const double V = u(it + 1);
+    const double us = 0.5 - fabs(U);
+    const double kd = floor((2.0 * a / us + b) * U + lam + 0.43);
+    if (us >= 0.07 && V <= vr) return (uint32_t)kd;
+    if (kd < 0 || (us < 0.013 && V > us)) continue;
+    if (log(V) + log(invalpha) - log(a / (us * us) + b) <= -lam + kd * loglam - lgamma(kd + 1.0))
+      return (uint32_t)kd;
+  }
+  return (uint32_t)floor(lam);  // not reached in practice (acceptance > 0.9 per pair)
+}
+
+// Work-group reductions of a 1024-thread block: wavefront shuffles, then the
+// wave results combined in a fixed order (deterministic).
+template <class T, class Op>
+__device__ T pa_reduce(T v, Op op, T* lds) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = op(v, __shfl_xor(v, off));
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  __syncthreads();
+  if (lane == 0) lds[wv] = v;
+  __syncthreads();
+  T r = lds[0];
+  for (int k = 1; k < nw; ++k) r = op(r, lds[k]);
+  return r;
+}
+
+// Exclusive prefix sum over the work-group (wavefront shuffle scan, then the
+// wave totals); *total = the sum over the work-group.
+__device__ int pa_exclusive_scan(int v, int* lds, int* total) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  int x = v;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const int y = __shfl_up(x, off);
+    if (lane >= off) x += y;
+  }
+  __syncthreads();
+  if (lane == 63) lds[wv] = x;
+  __syncthreads();
+  int before = 0, all = 0;
+  for (int k = 0; k < nw; ++k) {
+    before += k < wv ? lds[k] : 0;
+    all += lds[k];
+  }
+  *total = all;
+  return before + x - v;
+}
+
+struct MinOp {
+  template <class T> __device__ T operator()(T a, T b) const { return b < a ? b : a; }
+};
+struct AddOp {
+  template <class T> __device__ T operator()(T a, T b) const { return a + b; }
+};
+
+// The step's decisions (population_annealing.h:262-488 with resample
+// :490-616, Poisson branch): lowest-cost update, cost statistics, weights
+// exp(-dbeta (E - E_min) scale), Poisson offspring with mean
+// target * w / sum(w), their scan into the copy map src, the family census
+// rho = sum(family size^2) / R, and the restart rule R / rho < alpha or zero
+// cost variance.  One work-group of 1024 threads.
+__global__ void __launch_bounds__(1024) k_pa_step(PaArgs A) {
+  __shared__ double rd[16];
+  __shared__ int ri[16];
+  __shared__ long long rl[16];
+  PaDev* P = A.pa;
+  const int tid = threadIdx.x, bd = blockDim.x;
+  const int R = P->R;
+  const int64_t step = P->step;
+  const double inf = __longlong_as_double(0x7ff0000000000000ll);
+
+  double mn = inf, sum = 0.0, bmin_t = inf;
+  int barg_t = 0x7fffffff;
+  for (int i = tid; i < R; i += bd) {
+    const double c = A.cost[i];
+    mn = c < mn ? c : mn;
+    sum += c;
+    const double b = A.best_cost[i];
+    if (b < bmin_t) {
+      bmin_t = b;
+      barg_t = i;
+    }
+  }
+  mn = pa_reduce(mn, MinOp(), rd);
+  sum = pa_reduce(sum, AddOp(), rd);
+  const double bmin = pa_reduce(bmin_t, MinOp(), rd);
+  const int barg = pa_reduce(bmin_t == bmin ? barg_t : 0x7fffffff, MinOp(), ri);
+  const double avg = sum / (double)R;
+  double vs = 0.0;
+  for (int i = tid; i < R; i += bd) {
+    const double d = A.cost[i] - avg;
+    vs += d * d;
+  }
+  const double var = pa_reduce(vs, AddOp(), rd) / (double)R;
+
+  const int64_t k = step - P->base_step;
+  const int64_t last = A.n_table - 1;
+  const double db = A.beta_table[k + 1 < last ? k + 1 : last] - A.beta_table[k < last ? k : last];
+  double rho = 1.0;
+  int Rn = R;
+  bool moved = false;
+  if (db > 0.0) {
+    double ws = 0.0;
+    for (int i = tid; i < R; i += bd) {
+      const double w = exp(-db * (A.cost[i] - mn) * A.scale);
+      A.weight[i] = w;
+      ws += w;
+    }
+    const double total = pa_reduce(ws, AddOp(), rd);
+    const double rrq = (double)A.target / total;
+    for (int i = tid; i < R; i += bd) {
+      const double mean = rrq * A.weight[i];
+      A.count[i] = mean > 0.0 ? (int)pa_poisson(mean, (uint32_t)i, (uint32_t)step, A.seed) : 0;
+    }
+    // offspring of citizen i fill src[off_i .. off_i + count_i)
+    int carry = 0;
+    for (int base = 0; base < R; base += bd) {
+      const int i = base + tid;
+      const int c = i < R ? A.count[i] : 0;
+      int tot = 0;
+      const int off = carry + pa_exclusive_scan(c, ri, &tot);
+      for (int j = 0; j < c; ++j) {
+        const int d = off + j;
+        if (d < A.cap) {
+          A.src[d] = i;
+          A.family_next[d] = A.family[i];
+        }
+      }
+      carry += tot;
+    }
+    int current = carry;
+    if (current == 0) {  // keep the population alive
+      if (tid == 0) {
+        const uint4 w = philox4x32_10(0u, (uint32_t)step, 0u, DOM_PICK, A.seed, 0u);
+        const int i = (int)floor(canonical2(w.x, w.y) * (double)R);
+        A.src[0] = i;
+        A.family_next[0] = A.family[i];
+      }
+      current = 1;
+    }
+    if (current > A.cap) {
+      if (tid == 0) P->err = 1;
+      current = A.cap;
+    }
+    for (int f = tid; f < A.target; f += bd) A.hist[f] = 0;
+    __syncthreads();
+    for (int d = tid; d < current; d += bd) {
+      const int f = A.family_next[d];
+      if ((unsigned)f < (unsigned)A.target) atomicAdd(&A.hist[f], 1);
+    }
+    __syncthreads();
+    long long sq = 0;
+    for (int f = tid; f < A.target; f += bd) {
+      const long long h = A.hist[f];
+      sq += h * h;
+    }
+    sq = pa_reduce(sq, AddOp(), rl);
+    rho = (double)sq / (double)R;
+    Rn = current;
+    moved = true;
+  }
+
+  const bool restart = (double)R / rho < A.alpha || var == 0.0;
+  int Rf = Rn;
+  if (restart) {
+    if (!moved)
+      for (int i = tid; i < Rn; i += bd) {
+        A.src[i] = i;
+        A.family_next[i] = A.family[i];
+      }
+    __syncthreads();
+    if (Rn < A.target) {  // top up with copies of the last citizen
+      const int lastsrc = A.src[Rn - 1];
+      for (int i = Rn + tid; i < A.target; i += bd) A.src[i] = lastsrc;
+      Rf = A.target;
+    }
+    for (int i = tid; i < Rf && i < A.target; i += bd) A.family_next[i] = i;
+    moved = true;
+  }
+  __syncthreads();
+  if (moved)
+    for (int i = tid; i < Rf; i += bd) A.family[i] = A.family_next[i];
+  __syncthreads();
+  if (tid == 0) {
+    const bool imp = !P->have_best || bmin < P->best;
+    P->best_src = imp ? barg : -1;
+    if (imp) {
+      P->best = bmin;
+      P->have_best = 1;
+    }
+    P->attempts += (int64_t)R * A.sweeps_per_step * A.n_vars;
+    P->R_prev = R;
+    P->R_resampled = Rn;
+    P->R = Rf;
+    P->moved = moved ? 1 : 0;
+    P->restart = restart ? 1 : 0;
+    P->rho = rho;
+    P->step = step + 1;
+    if (restart) {
+      P->cur_beta = A.beta_start;
+      P->base_step = step;
+      P->epoch += 1;
+      P->restarts += 1;
+    } else {
+      P->cur_beta += db;
+    }
+  }
+}
+
+// Lowest state of a new global lowest (replica best_src of the population
+// the step started from) into best_state, variable order.
+__global__ void k_pa_copy_best(PaArgs A) {
+  const int r = A.pa->best_src;
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < 0 || v >= A.n_vars) return;
+  const UnitDev& U = A.units[r / A.B];
+  const int b = r % A.B;
+  const int q = U.pos_of_var[v];
+  A.best_state[v] = (U.best[q * U.G + (b >> 3)] >> (b & 7)) & 1u;
+}
+
+// Plain copies of the current (and lowest) states of replicas 0..n-1, n =
+// R_prev (resampling) or R (a restart's fresh states).
+__global__ void k_pa_unpack(PaArgs A, int fresh) {
+  const PaDev* P = A.pa;
+  if (fresh ? !P->restart : !P->moved) return;
+  const int R = fresh ? P->R : P->R_prev;
+  const int64_t n = (int64_t)R * A.n_vars;
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
+    const int r = (int)(t / A.n_vars), v = (int)(t - (int64_t)r * A.n_vars);
+    const UnitDev& U = A.units[r / A.B];
+    const int b = r % A.B;
+    const int at = U.pos_of_var[v] * U.G + (b >> 3);
+    A.plane_cur[t] = (U.spins[at] >> (b & 7)) & 1u;
+    if (!fresh) A.plane_low[t] = (U.best[at] >> (b & 7)) & 1u;
+  }
+}
+
+// Per-replica values of the copies, read before any is overwritten.
+__global__ void k_pa_gather(PaArgs A) {
+  const PaDev* P = A.pa;
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= P->R) return;
+  if (!P->moved) {
+    A.beta[r] = P->cur_beta;
+    return;
+  }
+  const int s = A.src[r];
+  A.tcost[r] = A.cost[s];
+  A.tbest[r] = A.best_cost[s];
+  A.tbeta[r] = A.beta[s];
+  A.tacc[r] = A.accepted[s];
+}
+
+__global__ void k_pa_scatter(PaArgs A) {
+  const PaDev* P = A.pa;
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (!P->moved || r >= P->R) return;
+  A.cost[r] = A.tcost[r];
+  A.best_cost[r] = A.tbest[r];
+  A.accepted[r] = A.tacc[r];
+  // after a restart the citizens keep their beta until the next step sets
+  // it; the top-up starts at beta_start
+  A.beta[r] = P->restart ? (r < P->R_resampled ? A.tbeta[r] : A.beta_start) : P->cur_beta;
+}
+
+// Unit bytes of the new population from the plain copies (k_pack_gather
+// with the copy map and population size on the device).
+__global__ void k_pa_pack(PaArgs A) {
+  const PaDev* P = A.pa;
+  const UnitDev& U = A.units[blockIdx.y];
+  if (!P->moved || unit_nrep(U) == 0) return;
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q > U.N) return;
+  const int u = blockIdx.y, nrep = unit_nrep(U);
+  for (int g = 0; g < U.G; ++g) {
+    uint32_t bc = 0, bl = 0;
+    if (q < U.N) {
+      const int v = U.var_of_pos[q];
+      for (int b = 0; b < 8; ++b) {
+        const int rr = 8 * g + b;
+        if (rr < nrep) {
+          const int64_t s = (int64_t)A.src[u * A.B + rr] * U.N + v;
+          bc |= ((uint32_t)(A.plane_cur[s] & 1u)) << b;
+          bl |= ((uint32_t)(A.plane_low[s] & 1u)) << b;
+        }
+      }
+    }
+    U.spins[q * U.G + g] = (uint8_t)bc;
+    U.best[q * U.G + g] = (uint8_t)bl;
+  }
+}
+
+// A restart's fresh random citizens (model/ising.h:23-41 draws, epoch from
+// the device state), as k_init_states.
+__global__ void k_pa_init(PaArgs A) {
+  const PaDev* P = A.pa;
+  const UnitDev& U = A.units[blockIdx.y];
+  if (!P->restart || unit_nrep(U) == 0) return;
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q > U.N) return;
+  const uint32_t epoch = P->epoch;
+  const int nrep = unit_nrep(U);
+  for (int g = 0; g < U.G; ++g) {
+    uint32_t byte = 0;
+    if (q < U.N) {
+      const uint32_t i = (uint32_t)U.var_of_pos[q];
+      const uint32_t wi = i >> 5;
+      for (int b = 0; b < 8; ++b) {
+        const int rr = 8 * g + b;
+        if (rr < nrep) {
+          const uint4 w = philox4x32_10(wi >> 2, U.r0 + (uint32_t)rr, epoch, DOM_INIT, U.seed, U.stream);
+          byte |= ((pick(w, wi & 3u) >> (i & 31u)) & 1u) << b;
+        }
+      }
+    }
+    U.spins[q * U.G + g] = (uint8_t)byte;
+    U.best[q * U.G + g] = (uint8_t)byte;
+  }
+}
+
+// Energies of a restart's fresh citizens: cost = lowest cost = E(state).
+__global__ void __launch_bounds__(256)
+k_pa_energy(PaArgs A, const int64_t* __restrict__ term_off, const int32_t* __restrict__ term_var,
+            const double* __restrict__ term_c, int64_t M, int kind) {
+  __shared__ double part[256];
+  const PaDev* P = A.pa;
+  if (!P->restart || (int)blockIdx.x >= P->R) return;
+  const double e = state_energy_256(term_off, term_var, term_c, M, kind,
+                                    A.plane_cur + (int64_t)blockIdx.x * A.n_vars, part);
+  if (threadIdx.x == 0) {
+    A.cost[blockIdx.x] = e;
+    A.best_cost[blockIdx.x] = e;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1222,6 +1599,24 @@ hipError_t launch_energy(const ProblemDev& P, const uint8_t* states, int n, doub
   if (n <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_energy, dim3(n), dim3(256), 0, st, P.term_off, P.term_var, P.term_c, P.M,
                      P.kind, P.N, states, out, out2);
+  return hipGetLastError();
+}
+
+hipError_t launch_pa_step(const PaArgs& a, const ProblemDev& P, hipStream_t st) {
+  const int N = a.n_vars;
+  hipLaunchKernelGGL(k_pa_step, dim3(1), dim3(1024), 0, st, a);
+  hipLaunchKernelGGL(k_pa_copy_best, dim3((N + 255) / 256), dim3(256), 0, st, a);
+  const int64_t planes = (int64_t)a.cap * N;
+  const unsigned ublocks = (unsigned)std::min<int64_t>((planes + 255) / 256, 8192);
+  hipLaunchKernelGGL(k_pa_unpack, dim3(ublocks), dim3(256), 0, st, a, 0);
+  hipLaunchKernelGGL(k_pa_gather, dim3((a.cap + 255) / 256), dim3(256), 0, st, a);
+  hipLaunchKernelGGL(k_pa_scatter, dim3((a.cap + 255) / 256), dim3(256), 0, st, a);
+  const dim3 ugrid((N + 1 + 127) / 128, a.n_units);
+  hipLaunchKernelGGL(k_pa_pack, ugrid, dim3(128), 0, st, a);
+  hipLaunchKernelGGL(k_pa_init, ugrid, dim3(128), 0, st, a);
+  hipLaunchKernelGGL(k_pa_unpack, dim3(ublocks), dim3(256), 0, st, a, 1);
+  hipLaunchKernelGGL(k_pa_energy, dim3(a.cap), dim3(256), 0, st, a, P.term_off, P.term_var, P.term_c, P.M,
+                     P.kind);
   return hipGetLastError();
 }
 

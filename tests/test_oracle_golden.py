@@ -319,3 +319,134 @@ def test_steps_per_tick_minimum():
     assert r["steps"] == 10
     r = p.sa(po.make_opts(seed=1), 0.1, 1.0, 37, 2)
     assert r["steps"] == 37
+
+
+# ------------------------------------------------- counter-mode population step
+
+
+def _canonical2(w0, w1):
+    return min((float(w0) + float(w1) * 4294967296.0) / 18446744073709551616.0, 1.0 - 2.0 ** -53)
+
+
+@pytest.mark.parametrize("lam", [0.05, 0.7, 1.0, 3.3, 12.0, 29.5])
+def test_poisson_counter_is_inverse_cdf(lam):
+    """Below 30 the counter-mode sampler is the inverse Poisson CDF of its
+    first uniform: the smallest k with F(k) >= u (scipy's ppf), checked
+    draw by draw away from the CDF steps."""
+    from scipy.stats import poisson
+    seed, step, n = 17, 5, 400
+    got = po.poisson_counter(lam, seed, step, n)
+    checked = 0
+    for i in range(n):
+        w = po.philox([i, step, 0, 4], [seed, 0])
+        u = _canonical2(w[0], w[1])
+        k = int(poisson.ppf(u, lam))
+        if abs(poisson.cdf(k, lam) - u) > 1e-12 and (k == 0 or abs(poisson.cdf(k - 1, lam) - u) > 1e-12):
+            assert got[i] == k, (i, u, got[i], k)
+            checked += 1
+    assert checked > n - 5
+
+
+@pytest.mark.parametrize("lam", [30.0, 45.5, 200.0, 5000.0])
+def test_poisson_counter_ptrs_matches_pmf(lam):
+    """From 30 up (transformed rejection) the draws follow the Poisson pmf:
+    chi-square over bins with >= 50 expected counts."""
+    from scipy.stats import chisquare, poisson
+    n = 200000
+    x = po.poisson_counter(lam, 23, 9, n).astype(np.int64)
+    lo, hi = int(poisson.ppf(1e-4, lam)), int(poisson.ppf(1 - 1e-4, lam))
+    edges = [lo]
+    while edges[-1] < hi:
+        k = edges[-1] + 1
+        while k < hi and n * (poisson.cdf(k - 1, lam) - poisson.cdf(edges[-1] - 1, lam)) < 50:
+            k += 1
+        edges.append(k)
+    obs, exp = [], []
+    prev_cdf = 0.0
+    bounds = [-1] + edges[1:-1] + [10 ** 9]
+    for a, b in zip(bounds[:-1], bounds[1:]):
+        obs.append(int(((x > a) & (x <= b)).sum()))
+        exp.append(n * (poisson.cdf(b, lam) - poisson.cdf(a, lam)))
+    exp = np.array(exp) * (sum(obs) / sum(exp))
+    assert chisquare(obs, exp).pvalue > 1e-4
+
+
+def _pa_inputs(R, seed=3, spread=4.0):
+    rng = np.random.default_rng(seed)
+    cost = np.round(rng.normal(0, spread, R)) - 50
+    lowest = cost - np.abs(np.round(rng.normal(0, 2, R)))
+    fam = rng.integers(0, R, R).astype(np.int32)
+    return cost, lowest, fam
+
+
+def test_pa_step_counter_offspring_follow_the_boltzmann_weights():
+    """Mean offspring of citizen i over many steps = target * w_i / sum(w),
+    w_i = exp(-dbeta (E_i - E_min)) (population_annealing.h:490-616)."""
+    R, target = 40, 40
+    cost, lowest, fam = _pa_inputs(R)
+    table = np.linspace(0.0, 2.0, 2001)  # dbeta = 0.001 at every step
+    counts = np.zeros(R)
+    n = 1500
+    for step in range(n):
+        r = po.pa_step_counter(cost, lowest, fam, step, step, table, target, 4 * target, 1e-9, 1.0, 11,
+                               True, -1e9)
+        assert not r["restart"] and r["moved"]
+        counts += np.bincount(r["src"], minlength=R)
+        assert np.all(np.diff(r["src"]) >= 0)  # offspring in citizen order
+        assert np.array_equal(r["family"], fam[r["src"]])
+    w = np.exp(-0.001 * (cost - cost.min()))
+    want = target * w / w.sum()
+    assert np.allclose(counts / n, want, atol=4 * np.sqrt(want / n).max())
+
+
+def test_pa_step_counter_family_statistic_and_restart_rule():
+    R, target = 30, 30
+    cost, lowest, _ = _pa_inputs(R, seed=5)
+    table = np.linspace(0.0, 3.0, 11)
+    fam = np.zeros(R, dtype=np.int32)  # one family: rho = R'^2 / R
+    r = po.pa_step_counter(cost, lowest, fam, 2, 0, table, target, 200, 1.5, 1.0, 7, False, 0.0)
+    assert r["rho"] == r["R_resampled"] ** 2 / R
+    assert r["restart"] == (R / r["rho"] < 1.5)
+    if r["restart"]:
+        assert list(r["family"][:target]) == list(range(min(target, r["R"])))
+        assert r["R"] >= target
+    # zero cost variance restarts; a restart tops the population up to the
+    # target with copies of the last resampled citizen and resets families
+    flat = np.full(R, -7.0)
+    r = po.pa_step_counter(flat, flat, np.arange(R, dtype=np.int32), 4, 0, np.ones(6), 64, 256, 2.0, 1.0, 7,
+                           True, -7.0)
+    assert r["restart"] and r["R"] == 64 and r["moved"] and r["R_resampled"] == R
+    assert list(r["src"][:R]) == list(range(R)) and np.all(r["src"][R:] == R - 1)
+    assert list(r["family"]) == list(range(64))
+    assert r["best_src"] == -1 and r["best"] == -7.0  # not lower than the best so far
+
+
+def test_pa_step_counter_identity_without_beta_change_and_global_lowest():
+    R = 20
+    cost, lowest, fam = _pa_inputs(R, seed=8)
+    table = np.array([1.0, 1.0, 1.0])
+    r = po.pa_step_counter(cost, lowest, fam, 0, 0, table, R, 100, 1.5, 1.0, 3, True, -1e9)
+    assert not r["moved"] and not r["restart"] and r["rho"] == 1.0
+    assert list(r["src"]) == list(range(R)) and np.array_equal(r["family"], fam)
+    assert r["best_src"] == -1
+    r = po.pa_step_counter(cost, lowest, fam, 0, 0, table, R, 100, 1.5, 1.0, 3, False, 0.0)
+    k = int(np.argmin(lowest))
+    assert r["best_src"] == k and r["best"] == lowest[k]
+
+
+def test_pa_step_counter_keeps_an_emptied_population_alive():
+    """All offspring counts zero: one copy of the citizen picked by Philox
+    (0, step, 0, 5) (population_annealing.h:566-572)."""
+    cost = np.array([-3.0])
+    table = np.linspace(0, 1, 4001)
+    seen = 0
+    for step in range(200):
+        r = po.pa_step_counter(cost, cost, np.zeros(1, np.int32), step, step, table, 1, 16, 1e-9, 1.0, 5,
+                               True, -10.0)
+        cnt = po.poisson_counter(1.0, 5, step, 1)[0]
+        if cnt == 0:
+            seen += 1
+            assert r["R"] == 1 and list(r["src"]) == [0]
+        else:
+            assert r["R"] == cnt
+    assert seen > 40  # P(0) = 1/e
