@@ -344,8 +344,11 @@ inline void attempt(const RunCtx& ctx, Walker& w, int64_t i, uint32_t sweep) {
   } else {
     const uint32_t k0 = ctx.o->seed, k1 = ctx.o->stream, r = (uint32_t)w.replica;
     acc = metropolis_accept(d, w.beta, [&](int row) -> uint32_t {
-      if (row == 0) return philox_word(k0, k1, (uint32_t)i, sweep, r >> 2, DOM_SWEEP, r & 3);
-      return philox_word(k0, k1, (uint32_t)i, sweep, r, DOM_REDRAW, row - 1);
+      // replica pair r >> 1: words r & 1 = row-0 draws, 2 + (r & 1) = row-1
+      // draws; rows 2 and 3 from the replica's own redraw counter
+      if (row == 0) return philox_word(k0, k1, (uint32_t)i, sweep, r >> 1, DOM_SWEEP, r & 1);
+      if (row == 1) return philox_word(k0, k1, (uint32_t)i, sweep, r >> 1, DOM_SWEEP, 2 + (r & 1));
+      return philox_word(k0, k1, (uint32_t)i, sweep, r, DOM_REDRAW, row - 2);
     });
   }
   if (acc) {

@@ -93,18 +93,18 @@ __device__ __forceinline__ bool metropolis_accept(double cost_diff, double beta,
   return value < t;
 }
 
-// The same decision without branches for a row-0 draw (bucket != 0): the
-// caller resolves bucket-0 draws (probability 1/256) with
-// metropolis_accept's row walk.  Bucket 1 reads y0 = ln(0) = -inf and the
-// interpolation yields NaN, which never accepts; t below row 0's range
+// The same decision without branches for a draw `r` of table row 0 or 1
+// (the caller resolves the 2^-16 of draws whose row-1 bucket is 0 as well
+// with metropolis_accept's row walk).  Bucket 1 reads y0 = ln(0) = -inf and
+// the interpolation yields NaN, which never accepts; t below row 0's range
 // (ln(1/256)) cannot accept either, so no early exits are needed.
 template <class Lut>
-__device__ __forceinline__ bool accept_row0(double cost_diff, double beta, uint32_t r, Lut&& lut) {
+__device__ __forceinline__ bool accept_row(double cost_diff, double beta, uint32_t r, int row, Lut&& lut) {
   const uint32_t bucket = r & 255u;
   const int i1 = (int)bucket, i0 = i1 > 0 ? i1 - 1 : 0;
   const double t = __dmul_rn(cost_diff, -beta);
-  const double y0 = lut(0, i0);
-  const double y1 = lut(0, i1);
+  const double y0 = lut(row, i0);
+  const double y1 = lut(row, i1);
   const double distance = div_ffffe(r >> 8);
   const double value = __dadd_rn(y0, __dmul_rn(distance, __dsub_rn(y1, y0)));
   return cost_diff <= 0.0 || value < t;

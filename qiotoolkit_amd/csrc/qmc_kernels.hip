@@ -456,27 +456,31 @@ k_sweep(const UnitDev* units, SweepArgs a) {
           const uint32_t var = e.var;
           const uint32_t own = sp[q * G + g];
           CT dE[8];
-          uint32_t w0[8];
           uint4 quad = make_uint4(0, 0, 0, 0);
-          uint32_t flips = 0, redraw = 0;
+          uint32_t flips = 0, deep = 0;
 #pragma unroll
           for (int b = 0; b < NB; ++b) {
             const uint32_t r = rbase + (uint32_t)b;
             dE[b] = delta_from_field<KIND, CT>(h[b], (own >> b) & 1u);
-            if (b == 0 || (r & 3u) == 0u)
-              quad = philox4x32_10(var, sweep, r >> 2, DOM_SWEEP, key0, key1);
-            w0[b] = pick(quad, r & 3u);
+            // one draw serves a replica pair: words 0, 1 = their row-0
+            // draws, words 2, 3 = their row-1 draws (used after bucket 0)
+            if (b == 0 || (r & 1u) == 0u)
+              quad = philox4x32_10(var, sweep, r >> 1, DOM_SWEEP, key0, key1);
+            const uint32_t w0 = (r & 1u) ? quad.y : quad.x;
+            const uint32_t w1 = (r & 1u) ? quad.w : quad.z;
+            const bool z0 = (w0 & 255u) == 0u;
             const bool live = valid && b < nb_g;
-            const bool ok = accept_row0((double)dE[b], beta[b], w0[b], lut);
-            const bool zero = (w0[b] & 255u) == 0u && dE[b] > CT(0);
-            flips |= (live && ok && !zero) ? 1u << b : 0u;
-            redraw |= (live && zero) ? 1u << b : 0u;
+            const bool ok = accept_row((double)dE[b], beta[b], z0 ? w1 : w0, z0 ? 1 : 0, lut);
+            const bool dp = z0 && (w1 & 255u) == 0u && dE[b] > CT(0);
+            flips |= (live && ok && !dp) ? 1u << b : 0u;
+            deep |= (live && dp) ? 1u << b : 0u;
           }
-          // bucket 0 of an uphill draw (1/256): the reference's row walk, one
-          // replica at a time, with the redraw counter of that replica
-          while (redraw) {
-            const int b = __builtin_ctz(redraw);
-            redraw &= redraw - 1;
+          // bucket 0 in rows 0 and 1 of an uphill draw (2^-16): the rest of
+          // the reference's row walk, one replica at a time, with that
+          // replica's redraw counter for rows 2 and 3
+          while (deep) {
+            const int b = __builtin_ctz(deep);
+            deep &= deep - 1;
             const uint32_t r = rbase + (uint32_t)b;
             CT d = dE[0];
             double bb = beta[0];
@@ -486,7 +490,8 @@ k_sweep(const UnitDev* units, SweepArgs a) {
               bb = j == b ? beta[j] : bb;
             }
             const uint4 w = philox4x32_10(var, sweep, r, DOM_REDRAW, key0, key1);
-            if (metropolis_accept((double)d, bb, 0u, lut, [&]() { return w; })) flips |= 1u << b;
+            if (metropolis_accept((double)d, bb, 0u, lut, [&]() { return make_uint4(0u, w.x, w.y, 0u); }))
+              flips |= 1u << b;
           }
 #pragma unroll
           for (int b = 0; b < NB; ++b) {
@@ -747,10 +752,11 @@ k_sweep_ordered(const UnitDev* __restrict__ units, SweepArgs a) {
       const CT dE = delta_from_field<KIND, CT>(h, own);
       bool ok = false;
       if (act) {
-        const uint4 quad = philox4x32_10((uint32_t)i, sweep, r >> 2, DOM_SWEEP, U.seed, U.stream);
-        ok = metropolis_accept((double)dE, beta, pick(quad, r & 3u),
-                               [&](int row, int i) { return lut[row * 256 + i]; }, [&]() {
-          return philox4x32_10((uint32_t)i, sweep, r, DOM_REDRAW, U.seed, U.stream);
+        const uint4 quad = philox4x32_10((uint32_t)i, sweep, r >> 1, DOM_SWEEP, U.seed, U.stream);
+        const uint32_t w0 = (r & 1u) ? quad.y : quad.x, w1 = (r & 1u) ? quad.w : quad.z;
+        ok = metropolis_accept((double)dE, beta, w0, [&](int row, int i) { return lut[row * 256 + i]; }, [&]() {
+          const uint4 w = philox4x32_10((uint32_t)i, sweep, r, DOM_REDRAW, U.seed, U.stream);
+          return make_uint4(w1, w.x, w.y, 0u);
         });
       }
       // one writer per byte: lanes 8g..8g+7 combine their flips
