@@ -262,7 +262,7 @@ struct qmc_batch_s {
   bool dirty = true;
   // one launch group per kernel instantiation
   struct Group {
-    int kind, pairwise, coupling, nb, coop;
+    int kind, pairwise, coupling, nb, coop, ladder;
     int block = 64;
     int stride = 0;
     double work = 0;  // attempts planned (for spreading groups over streams)
@@ -985,15 +985,16 @@ int qmc_walkers_set_betas(qmc_walkers w, const double* betas) {
   return QMC_OK;
 }
 
+// ladder: the units hold parallel tempering ladders (exchange code needed)
 static int launch_units(qmc_problem p, int nb, const UnitDev* d_units, int n_units, int block, int stride,
-                        int32_t max_sweeps, int ordered, hipStream_t st, int coop = 0) {
+                        int32_t max_sweeps, int ordered, hipStream_t st, int coop, int ladder) {
   SweepArgs a{};
   a.n_sweeps = max_sweeps;
   a.lut = p->ctx->lut.p;
   a.unit_of_block = nullptr;
   const size_t smem = sweep_smem_bytes(stride, ordered, &a.best_lds);
   HIP_TRY(launch_sweep(p->kind, p->pairwise, p->coupling, nb, d_units, n_units, block, smem, a, ordered,
-                       coop, st));
+                       coop, ladder, st));
   return QMC_OK;
 }
 
@@ -1090,7 +1091,7 @@ int qmc_walkers_sweep(qmc_walkers w, int64_t sweep0, int32_t n_sweeps, int beta_
   if ((rc = device_units(w, &du))) return rc;
   HIP_TRY(hipEventRecord(c->ev0, c->stream));
   rc = launch_units(w->prob, w->nb, du, w->n_units, ordered ? 64 : w->lanes * w->G, w->stride,
-                    n_sweeps, ordered, c->stream);
+                    n_sweeps, ordered, c->stream, 0, 0);
   if (rc) return rc;
   HIP_TRY(hipEventRecord(c->ev1, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
@@ -1116,7 +1117,7 @@ int qmc_walkers_pt(qmc_walkers w, int32_t ladder, const double* ladder_betas, in
     if ((rc = device_units(w, &du))) return rc;
     HIP_TRY(hipEventRecord(c->ev0, st));
     if ((rc = launch_units(p, w->nb, du, w->n_units, w->lanes * w->G, w->stride,
-                           n_steps * sweeps_per_replica, 0, st, w->coop)))
+                           n_steps * sweeps_per_replica, 0, st, w->coop, 1)))
       return rc;
     HIP_TRY(hipEventRecord(c->ev1, st));
     HIP_TRY(hipStreamSynchronize(st));
@@ -1151,7 +1152,7 @@ int qmc_walkers_pt(qmc_walkers w, int32_t ladder, const double* ladder_betas, in
       drop_graph(w);
       HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
       for (int k = 0; k < K; ++k) {
-        rc = launch_units(p, w->nb, du, w->n_units, w->lanes * w->G, w->stride, sweeps_per_replica, 0, st);
+        rc = launch_units(p, w->nb, du, w->n_units, w->lanes * w->G, w->stride, sweeps_per_replica, 0, st, 0, 0);
         if (rc) {
           hipGraph_t g;
           (void)hipStreamEndCapture(st, &g);
@@ -1264,7 +1265,7 @@ static int batch_rebuild(qmc_batch b) {
     qmc_batch_s::Group* g = nullptr;
     for (auto& gg : b->groups)
       if (gg->kind == p->kind && gg->pairwise == p->pairwise && gg->coupling == p->coupling &&
-          gg->nb == w->nb && gg->coop == w->coop && gg->block == block)
+          gg->nb == w->nb && gg->coop == w->coop && gg->ladder == (w->ladder > 0) && gg->block == block)
         g = gg.get();
     if (!g) {
       b->groups.push_back(std::make_unique<qmc_batch_s::Group>());
@@ -1274,6 +1275,7 @@ static int batch_rebuild(qmc_batch b) {
       g->coupling = p->coupling;
       g->nb = w->nb;
       g->coop = w->coop;
+      g->ladder = w->ladder > 0;
       g->block = block;
     }
     g->stride = std::max(g->stride, w->stride);
@@ -1308,7 +1310,8 @@ static int launch_group(qmc_batch_s::Group& g, int32_t max_sweeps, const double*
   if (smem > 160 * 1024) return fail(QMC_MEMORY_LIMITED, "batch LDS footprint exceeds 160 KiB");
   const int n = (int)g.units.size();
   if (!g.coop) {
-    HIP_TRY(launch_sweep(g.kind, g.pairwise, g.coupling, g.nb, g.d_units.p, n, g.block, smem, a, 0, 0, st));
+    HIP_TRY(launch_sweep(g.kind, g.pairwise, g.coupling, g.nb, g.d_units.p, n, g.block, smem, a, 0, 0, g.ladder,
+                         st));
     return QMC_OK;
   }
   // Cooperative ladders: launches of at most the co-resident number of
@@ -1324,7 +1327,7 @@ static int launch_group(qmc_batch_s::Group& g, int32_t max_sweeps, const double*
     }
     if (end == start) return fail(QMC_MEMORY_LIMITED, "a ladder exceeds the co-resident work-group limit");
     HIP_TRY(launch_sweep(g.kind, g.pairwise, g.coupling, g.nb, g.d_units.p + start, end - start, g.block,
-                         smem, a, 0, 1, st));
+                         smem, a, 0, 1, 1, st));
     start = end;
   }
   return QMC_OK;

@@ -155,7 +155,7 @@ struct SweepArgs {
 // bytes; *best_lds (optional) tells whether the lowest states fit in it too.
 size_t sweep_smem_bytes(int max_stride, int ordered, int* best_lds = nullptr);
 hipError_t launch_sweep(int kind, int pairwise, int coupling, int nb, const UnitDev* units, int n_units,
-                        int block, size_t smem, const SweepArgs& a, int ordered, int cooperative,
+                        int block, size_t smem, const SweepArgs& a, int ordered, int cooperative, int ladder,
                         hipStream_t st);
 // largest grid of cooperative sweep work-groups that can be co-resident
 int max_coresident_blocks(int kind, int pairwise, int coupling, int nb, int block, size_t smem);

@@ -308,7 +308,10 @@ __device__ __forceinline__ double load_agent(const double* p) {
       reinterpret_cast<const unsigned long long*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
 }
 
-template <int KIND, bool PAIR, typename CT, int NB>
+// LADDER: the launch may hold parallel tempering ladders (exchanges in the
+// kernel); without it the exchange code, and the registers it needs, are
+// compiled out.
+template <int KIND, bool PAIR, typename CT, int NB, bool LADDER>
 __global__ void __launch_bounds__(kBlock)
 k_sweep(const UnitDev* units, SweepArgs a) {
   using AT = typename AccT<CT>::T;
@@ -363,12 +366,12 @@ k_sweep(const UnitDev* units, SweepArgs a) {
     S.abort_flag = 0;
     S.any_new_best = 0;
   }
-  if (U.ladder > 0 && tid < U.ladder) {
+  if (LADDER && U.ladder > 0 && tid < U.ladder) {
     S.label[tid] = U.label[tid];
     S.lbeta[tid] = U.ladder_beta[tid];
   }
   __syncthreads();
-  if (U.ladder > 0 && tid < U.ladder) S.inv[S.label[tid]] = tid;
+  if (LADDER && U.ladder > 0 && tid < U.ladder) S.inv[S.label[tid]] = tid;
   __syncthreads();
 
   auto lut = [&](int row, int i) { return S.lut[row * 256 + i]; };
@@ -610,7 +613,7 @@ k_sweep(const UnitDev* units, SweepArgs a) {
     // ---- parallel tempering exchange (parallel_tempering.h:126-189) ----
     // Every work-group of the ladder computes the same decisions from the
     // same energies, so labels stay identical without another barrier.
-    if (U.ladder > 0 && ((sweep64 + 1) % U.spr) == 0) {
+    if (LADDER && U.ladder > 0 && ((sweep64 + 1) % U.spr) == 0) {
       const int64_t step64 = sweep64 / U.spr;
       const uint32_t step = (uint32_t)step64;
       const int L = U.ladder;
@@ -678,7 +681,7 @@ k_sweep(const UnitDev* units, SweepArgs a) {
     U.beta[tid] = S.rbeta[tid];
     U.accepted[tid] += S.racc[tid];
   }
-  if (U.ladder > 0 && U.slot0 == 0 && tid < U.ladder) U.label[tid] = S.label[tid];
+  if (LADDER && U.ladder > 0 && U.slot0 == 0 && tid < U.ladder) U.label[tid] = S.label[tid];
   if (tid == 0) {
     U.progress[0] = sweep_begin + n_run;
     U.progress[1] -= n_run;
@@ -1473,7 +1476,7 @@ static hipError_t raise_lds_limit(F fn) {
 
 template <int KIND, bool PAIR, typename CT, int NB>
 static hipError_t launch_sweep_t(const UnitDev* units, int n_units, int block, size_t smem,
-                                 const SweepArgs& a, int ordered, int cooperative, hipStream_t st) {
+                                 const SweepArgs& a, int ordered, int cooperative, int ladder, hipStream_t st) {
   if (ordered) {
     auto fn = k_sweep_ordered<KIND, PAIR, CT>;
     hipError_t e = raise_lds_limit(fn);
@@ -1481,8 +1484,9 @@ static hipError_t launch_sweep_t(const UnitDev* units, int n_units, int block, s
     hipLaunchKernelGGL(fn, dim3(n_units), dim3(64), smem, st, units, a);
     return hipGetLastError();
   }
-  auto fn = k_sweep<KIND, PAIR, CT, NB>;
-  hipError_t e = raise_lds_limit(fn);
+  auto fn = ladder ? k_sweep<KIND, PAIR, CT, NB, true> : k_sweep<KIND, PAIR, CT, NB, false>;
+  hipError_t e = ladder ? raise_lds_limit(k_sweep<KIND, PAIR, CT, NB, true>)
+                        : raise_lds_limit(k_sweep<KIND, PAIR, CT, NB, false>);
   if (e != hipSuccess) return e;
   if (cooperative) {
     SweepArgs args = a;
@@ -1496,35 +1500,36 @@ static hipError_t launch_sweep_t(const UnitDev* units, int n_units, int block, s
 
 template <int KIND, bool PAIR, typename CT>
 static hipError_t launch_nb(int nb, const UnitDev* units, int n_units, int block, size_t smem,
-                            const SweepArgs& a, int ordered, int coop, hipStream_t st) {
-  if (nb <= 1) return launch_sweep_t<KIND, PAIR, CT, 1>(units, n_units, block, smem, a, ordered, coop, st);
-  if (nb <= 2) return launch_sweep_t<KIND, PAIR, CT, 2>(units, n_units, block, smem, a, ordered, coop, st);
-  if (nb <= 4) return launch_sweep_t<KIND, PAIR, CT, 4>(units, n_units, block, smem, a, ordered, coop, st);
-  return launch_sweep_t<KIND, PAIR, CT, 8>(units, n_units, block, smem, a, ordered, coop, st);
+                            const SweepArgs& a, int ordered, int coop, int ladder, hipStream_t st) {
+  if (nb <= 1) return launch_sweep_t<KIND, PAIR, CT, 1>(units, n_units, block, smem, a, ordered, coop, ladder, st);
+  if (nb <= 2) return launch_sweep_t<KIND, PAIR, CT, 2>(units, n_units, block, smem, a, ordered, coop, ladder, st);
+  if (nb <= 4) return launch_sweep_t<KIND, PAIR, CT, 4>(units, n_units, block, smem, a, ordered, coop, ladder, st);
+  return launch_sweep_t<KIND, PAIR, CT, 8>(units, n_units, block, smem, a, ordered, coop, ladder, st);
 }
 
 hipError_t launch_sweep(int kind, int pairwise, int coupling, int nb, const UnitDev* units, int n_units,
-                        int block, size_t smem, const SweepArgs& a, int ordered, int coop,
+                        int block, size_t smem, const SweepArgs& a, int ordered, int coop, int ladder,
                         hipStream_t st) {
   const bool i32 = coupling == QMC_COUPLING_I32;
   if (kind == QMC_ISING) {
     if (pairwise)
-      return i32 ? launch_nb<QMC_ISING, true, int32_t>(nb, units, n_units, block, smem, a, ordered, coop, st)
-                 : launch_nb<QMC_ISING, true, float>(nb, units, n_units, block, smem, a, ordered, coop, st);
-    return i32 ? launch_nb<QMC_ISING, false, int32_t>(nb, units, n_units, block, smem, a, ordered, coop, st)
-               : launch_nb<QMC_ISING, false, float>(nb, units, n_units, block, smem, a, ordered, coop, st);
+      return i32 ? launch_nb<QMC_ISING, true, int32_t>(nb, units, n_units, block, smem, a, ordered, coop, ladder, st)
+                 : launch_nb<QMC_ISING, true, float>(nb, units, n_units, block, smem, a, ordered, coop, ladder, st);
+    return i32 ? launch_nb<QMC_ISING, false, int32_t>(nb, units, n_units, block, smem, a, ordered, coop, ladder, st)
+               : launch_nb<QMC_ISING, false, float>(nb, units, n_units, block, smem, a, ordered, coop, ladder, st);
   }
   if (pairwise)
-    return i32 ? launch_nb<QMC_PUBO, true, int32_t>(nb, units, n_units, block, smem, a, ordered, coop, st)
-               : launch_nb<QMC_PUBO, true, float>(nb, units, n_units, block, smem, a, ordered, coop, st);
-  return i32 ? launch_nb<QMC_PUBO, false, int32_t>(nb, units, n_units, block, smem, a, ordered, coop, st)
-             : launch_nb<QMC_PUBO, false, float>(nb, units, n_units, block, smem, a, ordered, coop, st);
+    return i32 ? launch_nb<QMC_PUBO, true, int32_t>(nb, units, n_units, block, smem, a, ordered, coop, ladder, st)
+               : launch_nb<QMC_PUBO, true, float>(nb, units, n_units, block, smem, a, ordered, coop, ladder, st);
+  return i32 ? launch_nb<QMC_PUBO, false, int32_t>(nb, units, n_units, block, smem, a, ordered, coop, ladder, st)
+             : launch_nb<QMC_PUBO, false, float>(nb, units, n_units, block, smem, a, ordered, coop, ladder, st);
 }
 
 template <int KIND, bool PAIR, typename CT, int NB>
 static int coresident_t(int block, size_t smem) {
   int per_cu = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_sweep<KIND, PAIR, CT, NB>, block,
+  // cooperative launches carry ladders
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_sweep<KIND, PAIR, CT, NB, true>, block,
                                                    smem) != hipSuccess)
     return 0;
   int dev = 0, cus = 0;
