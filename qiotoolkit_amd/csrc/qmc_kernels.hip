@@ -405,6 +405,16 @@ k_sweep(const UnitDev* units, SweepArgs a) {
   };
   if (active) fetch_row(class_off[0], PAIR ? slice_base[0] : 0, 0, sla, ra);
 
+  // parallel tempering: exchange step of the current sweep, sweeps run into
+  // it and its energy buffer (step mod 3), advanced per sweep
+  int64_t xstep = 0;
+  int xphase = 0, xbuf = 0;
+  if (LADDER && U.ladder > 0) {
+    xstep = sweep_begin / U.spr;
+    xphase = (int)(sweep_begin - xstep * U.spr);
+    xbuf = (int)(xstep % 3);
+  }
+
   for (int s = 0; s < n_run; ++s) {
     const int64_t sweep64 = sweep_begin + s;
     const uint32_t sweep = (uint32_t)sweep64;
@@ -613,12 +623,18 @@ k_sweep(const UnitDev* units, SweepArgs a) {
     // ---- parallel tempering exchange (parallel_tempering.h:126-189) ----
     // Every work-group of the ladder computes the same decisions from the
     // same energies, so labels stay identical without another barrier.
-    if (LADDER && U.ladder > 0 && ((sweep64 + 1) % U.spr) == 0) {
-      const int64_t step64 = sweep64 / U.spr;
-      const uint32_t step = (uint32_t)step64;
+    if (LADDER && U.ladder > 0 && xphase == U.spr - 1) {
+      const uint32_t step = (uint32_t)xstep;
       const int L = U.ladder;
+      // this thread's pair and its uniform, drawn before the energies arrive
+      const int i = (int)(step & 1u) + 2 * tid;
+      double u = 0.0;
+      if (i < L - 1) {
+        const uint4 w = philox4x32_10((uint32_t)i, step, 0u, DOM_EXCHANGE, U.seed, U.stream);
+        u = canonical2(w.x, w.y);
+      }
       if (U.upl > 1) {
-        double* cb = U.cbuf + (step64 % 3) * L;
+        double* cb = U.cbuf + xbuf * L;
         if (tid < nrep) {
           wait_stores();  // this slot's reset (one step ago) is complete
           store_agent(cb + U.slot0 + tid, S.rcost[tid]);
@@ -632,17 +648,14 @@ k_sweep(const UnitDev* units, SweepArgs a) {
         if (S.abort_flag) break;
         if (tid < nrep) {
           const double nan = __longlong_as_double(0x7ff8000000000000ll);
-          store_agent(U.cbuf + ((step64 + 2) % 3) * L + U.slot0 + tid, nan);
+          store_agent(U.cbuf + (xbuf == 0 ? 2 : xbuf - 1) * L + U.slot0 + tid, nan);
         }
       } else if (tid < L) {
         S.lcost[tid] = S.rcost[tid];
       }
       lds_barrier();
-      const int i = (int)(step & 1u) + 2 * tid;
       if (i < L - 1) {
         const int sa = S.label[i], sb = S.label[i + 1];
-        const uint4 w = philox4x32_10((uint32_t)i, step, 0u, DOM_EXCHANGE, U.seed, U.stream);
-        const double u = canonical2(w.x, w.y);
         const double bi = S.lbeta[i], bj = S.lbeta[i + 1];
         const double arg = __dmul_rn(__dsub_rn(S.lcost[sa], S.lcost[sb]), __dsub_rn(bi, bj));
         if (u < exp(arg)) {
@@ -654,6 +667,11 @@ k_sweep(const UnitDev* units, SweepArgs a) {
       }
       lds_barrier();
       if (tid < nrep) S.rbeta[tid] = S.lbeta[S.inv[U.slot0 + tid]];
+    }
+    if (LADDER && U.ladder > 0 && ++xphase == U.spr) {
+      xphase = 0;
+      ++xstep;
+      xbuf = xbuf == 2 ? 0 : xbuf + 1;
     }
     lds_barrier();
   }  // sweeps
