@@ -35,8 +35,8 @@ LDS_PEAK_GBS = 256 * 128 * 2.4  # 256 CUs x 128 B/clk x 2.4 GHz
 # automatic grouping and lanes), from rocprofv3 --pmc FETCH_SIZE and
 # WRITE_SIZE in separate runs, scaled by what the same counters report for a
 # plain 512 MiB copy (FETCH_SIZE x 2, WRITE_SIZE x 1):
-# 242682.5 KiB x 2 + 82106 KiB.  Summary: profiles/k_sweep_pt_16cube_hbm_traffic.md
-MEASURED_TRAFFIC_BYTES = {"c2": int((242682.5 * 2 + 82106) * 1024)}
+# 205462.1 KiB x 2 + 82106 KiB.  Summary: profiles/k_sweep_pt_16cube_hbm_traffic.md
+MEASURED_TRAFFIC_BYTES = {"c2": int((205462.1 * 2 + 82106) * 1024)}
 
 
 def log(*a):
