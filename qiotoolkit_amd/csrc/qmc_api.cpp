@@ -127,7 +127,22 @@ struct qmc_ctx_s {
   std::mutex stage_mu;
   uint8_t* stage = nullptr;
   size_t stage_off = 0;
+  // beta schedules on the device, shared by walker sets that plan the same
+  // values (many solves of one schedule upload it once)
+  struct SchedEntry {
+    std::vector<double> values;
+    std::shared_ptr<DevBuf<double>> dev;
+    uint64_t used = 0;
+  };
+  std::mutex sched_mu;
+  std::vector<SchedEntry> sched_cache;
+  uint64_t sched_clock = 0;
+  // set-up descriptors of the last set-up launch (reused in stream order)
+  DevBuf<SetupDesc> setup_descs;
 };
+
+constexpr size_t kSchedCacheEntries = 16;
+constexpr size_t kSchedCacheMaxValues = 1 << 17;
 
 constexpr size_t kStageBytes = 32u << 20;
 
@@ -222,7 +237,8 @@ struct qmc_walkers_s {
   uint32_t seed = 0, stream = 0;
   int32_t first_replica = 0;
   DevBuf<uint8_t> spins, best, scratch_states;
-  DevBuf<double> cost, best_cost, beta, sched, ladder_beta, energies;
+  DevBuf<double> cost, best_cost, beta, ladder_beta, energies;
+  std::shared_ptr<DevBuf<double>> sched;  // beta schedule (possibly shared, never rewritten)
   DevBuf<unsigned long long> accepted;
   DevBuf<int32_t> label;
   DevBuf<int64_t> progress;  // 2 per unit: next sweep, sweeps left
@@ -244,6 +260,13 @@ struct qmc_walkers_s {
   int64_t planned = 0;       // sweeps of the last plan
   uint64_t version = 0;      // bumped whenever `units` changes
   bool units_stale = true;   // `d_units` behind `units` (uploaded before device use)
+  bool pend_zero = true;     // device arrays not cleared yet (the first set-up clears them)
+  // qmc_walkers_init and a plan record what to do; the set-up launch (alone,
+  // or for every pending member of a batch at once) performs it
+  bool pend_init = false;
+  uint32_t pend_epoch = 0;
+  bool pend_prog = false;
+  int64_t pend_sweep0 = 0, pend_n = 0;
   // population annealing with the population size on the device
   int pa_on = 0;
   PaArgs pa_args{};
@@ -642,7 +665,7 @@ static void build_units(qmc_walkers w) {
       }
     }
     U.spr = w->spr;
-    U.sched = w->sched.p;
+    U.sched = w->sched ? w->sched->p : nullptr;
     U.sched_base = w->sched_base;
     U.progress = w->progress.p + 2 * (size_t)u;
     U.nrep = (uint32_t)std::min(w->B, w->R - r);
@@ -732,7 +755,20 @@ int qmc_walkers_create(qmc_problem p, int32_t n_replicas, int32_t replicas_per_g
   HIP_TRY(w->err.alloc(1));
   HIP_TRY(w->d_units.alloc(w->n_units));
   HIP_TRY(w->progress.alloc(2 * (size_t)w->n_units));
-  // everything starts at zero, cleared in one launch on the context stream
+  // everything starts at zero: cleared by the first qmc_walkers_init, or by
+  // ensure_cleared before any other device use
+  int rc = upload_units(w.get());
+  if (rc) return rc;
+  p->refs++;
+  *out = w.release();
+  return QMC_OK;
+}
+
+// Clears a new walker set's device arrays in one launch unless that already
+// happened; every entry point that touches walker memory on the device calls
+// this first (qmc_walkers_init folds it into its set-up launch).
+static int ensure_cleared(qmc_walkers w) {
+  if (!w->pend_zero) return QMC_OK;
   ZeroSpans z{};
   int nz = 0;
   auto zero = [&](void* ptr, size_t bytes) {
@@ -741,19 +777,83 @@ int qmc_walkers_create(qmc_problem p, int32_t n_replicas, int32_t replicas_per_g
   };
   zero(w->spins.p, w->spins.n);
   zero(w->best.p, w->best.n);
-  zero(w->cost.p, w->R * sizeof(double));
-  zero(w->best_cost.p, w->R * sizeof(double));
-  zero(w->beta.p, w->R * sizeof(double));
-  zero(w->accepted.p, w->R * sizeof(unsigned long long));
-  zero(w->label.p, w->R * sizeof(int32_t));
+  zero(w->cost.p, w->cap * sizeof(double));
+  zero(w->best_cost.p, w->cap * sizeof(double));
+  zero(w->beta.p, w->cap * sizeof(double));
+  zero(w->accepted.p, w->cap * sizeof(unsigned long long));
+  zero(w->label.p, w->cap * sizeof(int32_t));
   zero(w->err.p, sizeof(int32_t));
   zero(w->progress.p, w->progress.n * sizeof(int64_t));
-  HIP_TRY(launch_zero(z, nz, p->ctx->stream));
-  int rc = upload_units(w.get());
-  if (rc) return rc;
-  p->refs++;
-  *out = w.release();
+  HIP_TRY(launch_zero(z, nz, w->prob->ctx->stream));
+  w->pend_zero = false;
   return QMC_OK;
+}
+
+static void add_setup_descs(qmc_walkers w, const UnitDev* du, std::vector<SetupDesc>& out) {
+  qmc_problem p = w->prob;
+  for (int u = 0; u < w->n_units; ++u) {
+    SetupDesc d{};
+    d.unit = du + u;
+    d.term_off = p->d_term_off.p;
+    d.term_var = p->d_term_var.p;
+    d.term_c = p->d_term_c.p;
+    d.M = p->M;
+    d.kind = p->kind;
+    d.epoch = w->pend_epoch;
+    d.clear = w->pend_zero ? 1 : 0;
+    d.set_progress = w->pend_prog ? 1 : 0;
+    d.sweep0 = w->pend_sweep0;
+    d.n_sweeps = w->pend_n;
+    out.push_back(d);
+  }
+}
+
+static void setup_done(qmc_walkers w) {
+  w->pend_init = false;
+  w->pend_zero = false;
+  w->pend_prog = false;
+}
+
+// One set-up launch for the recorded work of several walker sets.
+static int launch_setups(qmc_ctx c, const std::vector<SetupDesc>& d, int max_stride) {
+  if (d.empty()) return QMC_OK;
+  if (c->setup_descs.n < d.size()) {
+    HIP_TRY(hipStreamSynchronize(c->stream));  // a launch may still read the old array
+    HIP_TRY(c->setup_descs.alloc(d.size()));
+  }
+  HIP_TRY(upload_async(c, c->setup_descs.p, d.data(), d.size() * sizeof(SetupDesc)));
+  HIP_TRY(launch_setup(c->setup_descs.p, (int)d.size(), max_stride, c->stream));
+  return QMC_OK;
+}
+
+static int ensure_scratch(qmc_walkers w);
+
+// Performs what qmc_walkers_init and a plan recorded for this walker set
+// (or only the clearing); every entry point that touches walker memory on
+// the device calls this first.
+static int materialize(qmc_walkers w) {
+  if (!w->pend_init) return ensure_cleared(w);
+  qmc_problem p = w->prob;
+  hipStream_t st = p->ctx->stream;
+  const UnitDev* du = nullptr;
+  int rc = device_units(w, &du);
+  if (rc) return rc;
+  if (setup_fits(w->stride)) {
+    std::vector<SetupDesc> d;
+    add_setup_descs(w, du, d);
+    if ((rc = launch_setups(p->ctx, d, w->stride))) return rc;
+    setup_done(w);
+    return QMC_OK;
+  }
+  // states too large for the one-launch set-up
+  if ((rc = ensure_cleared(w)) || (rc = ensure_scratch(w))) return rc;
+  HIP_TRY(launch_init(du, w->n_units, p->N, w->pend_epoch, st));
+  HIP_TRY(launch_unpack(du, w->B, 0, w->R, p->N, 0, w->scratch_states.p, st));
+  HIP_TRY(launch_energy(p->dev(), w->scratch_states.p, w->R, w->cost.p, st, w->best_cost.p));
+  const bool prog = w->pend_prog;
+  const int64_t s0 = w->pend_sweep0, n = w->pend_n;
+  setup_done(w);
+  return prog ? set_progress(w, s0, n) : QMC_OK;
 }
 
 static void drop_graph(qmc_walkers w) {
@@ -784,6 +884,7 @@ static int ensure_scratch(qmc_walkers w) {
 int qmc_walkers_resample(qmc_walkers w, const int32_t* src, int32_t new_R) {
   if (!w || (!src && new_R > 0)) return fail(QMC_INVALID_ARGUMENT, "null argument");
   if (w->pa_on) return fail(QMC_INVALID_ARGUMENT, "the population is on the device (qmc_walkers_pa_end first)");
+  if (int rc0 = materialize(w)) return rc0;
   if (new_R < 1 || new_R > w->cap)
     return fail(QMC_MEMORY_LIMITED, "population exceeds the walker capacity");
   for (int32_t r = 0; r < new_R; ++r)
@@ -833,6 +934,7 @@ int qmc_walkers_pa_begin(qmc_walkers w, int32_t population, double alpha, double
                          const double* beta_table, int32_t n_table, uint32_t seed,
                          int32_t sweeps_per_step) {
   if (!w || !beta_table) return fail(QMC_INVALID_ARGUMENT, "null argument");
+  if (int rc0 = materialize(w)) return rc0;
   if (w->pa_on) return fail(QMC_INVALID_ARGUMENT, "population mode already on");
   if (w->R != w->cap) return fail(QMC_INVALID_ARGUMENT, "walkers must hold their full capacity");
   if (population < 1 || population > w->cap)
@@ -954,19 +1056,14 @@ int qmc_walkers_pa_end(qmc_walkers w) {
 
 int qmc_walkers_init(qmc_walkers w, uint32_t epoch) {
   if (!w) return fail(QMC_INVALID_ARGUMENT, "null handle");
-  qmc_problem p = w->prob;
-  hipStream_t st = p->ctx->stream;
-  const UnitDev* du = nullptr;
-  int rc = device_units(w, &du);
-  if (rc || (rc = ensure_scratch(w))) return rc;
-  HIP_TRY(launch_init(du, w->n_units, p->N, epoch, st));
-  HIP_TRY(launch_unpack(du, w->B, 0, w->R, p->N, 0, w->scratch_states.p, st));
-  HIP_TRY(launch_energy(p->dev(), w->scratch_states.p, w->R, w->cost.p, st, w->best_cost.p));
+  // recorded; performed by the next launch that needs the state
+  w->pend_init = true;
+  w->pend_epoch = epoch;
   return QMC_OK;
 }
-
 int qmc_walkers_set_states(qmc_walkers w, const uint8_t* states) {
   if (!w || !states) return fail(QMC_INVALID_ARGUMENT, "null argument");
+  if (int rc0 = materialize(w)) return rc0;
   qmc_problem p = w->prob;
   hipStream_t st = p->ctx->stream;
   const UnitDev* du = nullptr;
@@ -981,6 +1078,7 @@ int qmc_walkers_set_states(qmc_walkers w, const uint8_t* states) {
 
 int qmc_walkers_set_betas(qmc_walkers w, const double* betas) {
   if (!w || !betas) return fail(QMC_INVALID_ARGUMENT, "null argument");
+  if (int rc0 = materialize(w)) return rc0;
   HIP_TRY(upload_async(w->prob->ctx, w->beta.p, betas, w->R * sizeof(double)));
   return QMC_OK;
 }
@@ -999,8 +1097,34 @@ static int launch_units(qmc_problem p, int nb, const UnitDev* d_units, int n_uni
 }
 
 static int upload_schedule(qmc_walkers w, int64_t sweep0, int32_t n, const double* schedule) {
-  if (w->sched.n < (size_t)n) HIP_TRY(w->sched.alloc(n));
-  HIP_TRY(upload_async(w->prob->ctx, w->sched.p, schedule, n * sizeof(double)));
+  qmc_ctx c = w->prob->ctx;
+  std::shared_ptr<DevBuf<double>> buf;
+  const bool cacheable = n > 0 && (size_t)n <= kSchedCacheMaxValues;
+  if (cacheable) {
+    std::lock_guard<std::mutex> lk(c->sched_mu);
+    for (auto& e : c->sched_cache)
+      if (e.values.size() == (size_t)n && std::memcmp(e.values.data(), schedule, n * sizeof(double)) == 0) {
+        e.used = ++c->sched_clock;
+        buf = e.dev;
+        break;
+      }
+  }
+  if (!buf) {
+    // a fresh buffer: one a launch may still read is never rewritten
+    buf = std::make_shared<DevBuf<double>>();
+    HIP_TRY(buf->alloc(std::max<int32_t>(n, 1)));
+    HIP_TRY(upload_async(c, buf->p, schedule, (size_t)n * sizeof(double)));
+    if (cacheable) {
+      std::lock_guard<std::mutex> lk(c->sched_mu);
+      if (c->sched_cache.size() >= kSchedCacheEntries) {
+        auto old = std::min_element(c->sched_cache.begin(), c->sched_cache.end(),
+                                    [](const auto& x, const auto& y) { return x.used < y.used; });
+        c->sched_cache.erase(old);
+      }
+      c->sched_cache.push_back({std::vector<double>(schedule, schedule + n), buf, ++c->sched_clock});
+    }
+  }
+  w->sched = buf;
   w->sched_base = sweep0;
   return QMC_OK;
 }
@@ -1008,6 +1132,9 @@ static int upload_schedule(qmc_walkers w, int64_t sweep0, int32_t n, const doubl
 int qmc_walkers_plan(qmc_walkers w, int64_t sweep0, int32_t n_sweeps, const double* schedule) {
   if (!w) return fail(QMC_INVALID_ARGUMENT, "null handle");
   if (n_sweeps < 0) return fail(QMC_INVALID_ARGUMENT, "n_sweeps must be >= 0");
+  // a pending set-up stays pending: it sets the progress recorded here
+  if (!w->pend_init)
+    if (int rc0 = materialize(w)) return rc0;
   int rc;
   w->ladder = 0;
   w->coop = 0;
@@ -1015,9 +1142,16 @@ int qmc_walkers_plan(qmc_walkers w, int64_t sweep0, int32_t n_sweeps, const doub
   if (schedule) {
     if ((rc = upload_schedule(w, sweep0, n_sweeps, schedule))) return rc;
   } else {
-    w->sched.release();
+    w->sched.reset();
   }
   if ((rc = upload_units(w))) return rc;
+  if (w->pend_init) {
+    w->pend_prog = true;
+    w->pend_sweep0 = sweep0;
+    w->pend_n = n_sweeps;
+    w->planned = n_sweeps;
+    return QMC_OK;
+  }
   return set_progress(w, sweep0, n_sweeps);
 }
 
@@ -1044,6 +1178,7 @@ static int set_ladder(qmc_walkers w, int32_t ladder, const double* ladder_betas,
 int qmc_walkers_plan_pt(qmc_walkers w, int32_t ladder, const double* ladder_betas, int64_t step0,
                         int32_t n_steps, int32_t sweeps_per_replica) {
   if (!w || !ladder_betas) return fail(QMC_INVALID_ARGUMENT, "null argument");
+  if (int rc0 = materialize(w)) return rc0;
   if (ladder < 1 || ladder % w->B != 0)
     return fail(QMC_INVALID_ARGUMENT, "the ladder must be a multiple of replicas_per_group");
   if (sweeps_per_replica < 1) return fail(QMC_INVALID_ARGUMENT, "sweeps_per_replica must be >= 1");
@@ -1067,7 +1202,7 @@ int qmc_walkers_plan_pt(qmc_walkers w, int32_t ladder, const double* ladder_beta
   }
   w->ladder = ladder;
   w->spr = sweeps_per_replica;
-  w->sched.release();
+  w->sched.reset();
   if ((rc = upload_units(w))) return rc;
   return set_progress(w, step0 * sweeps_per_replica, (int64_t)n_steps * sweeps_per_replica);
 }
@@ -1082,6 +1217,7 @@ static int check_barrier_error(qmc_walkers w) {
 int qmc_walkers_sweep(qmc_walkers w, int64_t sweep0, int32_t n_sweeps, int beta_mode,
                       const double* schedule, int ordered) {
   if (!w) return fail(QMC_INVALID_ARGUMENT, "null handle");
+  if (int rc0 = materialize(w)) return rc0;
   if (n_sweeps <= 0) return QMC_OK;
   if (beta_mode == QMC_BETA_SCHEDULE && !schedule) return fail(QMC_INVALID_ARGUMENT, "schedule required");
   int rc = qmc_walkers_plan(w, sweep0, n_sweeps, beta_mode == QMC_BETA_SCHEDULE ? schedule : nullptr);
@@ -1102,6 +1238,7 @@ int qmc_walkers_sweep(qmc_walkers w, int64_t sweep0, int32_t n_sweeps, int beta_
 int qmc_walkers_pt(qmc_walkers w, int32_t ladder, const double* ladder_betas, int64_t step0,
                    int32_t n_steps, int32_t sweeps_per_replica) {
   if (!w || !ladder_betas) return fail(QMC_INVALID_ARGUMENT, "null argument");
+  if (int rc0 = materialize(w)) return rc0;
   if (sweeps_per_replica < 1) return fail(QMC_INVALID_ARGUMENT, "sweeps_per_replica must be >= 1");
   qmc_problem p = w->prob;
   qmc_ctx c = p->ctx;
@@ -1129,7 +1266,7 @@ int qmc_walkers_pt(qmc_walkers w, int32_t ladder, const double* ladder_betas, in
     w->coop = 0;
     w->ladder = 0;  // exchanges run as their own launches
     w->spr = 1;
-    w->sched.release();
+    w->sched.reset();
     if ((rc = upload_units(w))) return rc;
     w->ladder = ladder;
     if ((rc = set_progress(w, step0 * sweeps_per_replica, (int64_t)n_steps * sweeps_per_replica))) return rc;
@@ -1177,6 +1314,7 @@ int qmc_walkers_pt(qmc_walkers w, int32_t ladder, const double* ladder_betas, in
 
 int qmc_walkers_remaining(qmc_walkers w, int64_t* sweeps_left) {
   if (!w || !sweeps_left) return fail(QMC_INVALID_ARGUMENT, "null argument");
+  if (int rc0 = materialize(w)) return rc0;
   std::vector<int64_t> pr(w->progress.n);
   HIP_TRY(hipStreamSynchronize(w->prob->ctx->stream));
   HIP_TRY(hipMemcpy(pr.data(), w->progress.p, pr.size() * sizeof(int64_t), hipMemcpyDeviceToHost));
@@ -1208,6 +1346,9 @@ void qmc_batch_destroy(qmc_batch b) {
 int qmc_batch_add(qmc_batch b, qmc_walkers w) {
   if (!b || !w) return fail(QMC_INVALID_ARGUMENT, "null handle");
   if (w->prob->ctx != b->ctx) return fail(QMC_INVALID_ARGUMENT, "walkers belong to another context");
+  // pending set-ups of members run in one launch with the batch
+  if (!w->pend_init)
+    if (int rc0 = materialize(w)) return rc0;
   for (auto* m : b->members)
     if (m == w) return QMC_OK;
   b->members.push_back(w);
@@ -1367,10 +1508,36 @@ static int enqueue_groups(qmc_batch b, int32_t max_sweeps) {
   return QMC_OK;
 }
 
+// The recorded set-ups of all members in one launch on the main stream,
+// through the batch's descriptor copies (current after batch_rebuild).
+static int batch_materialize(qmc_batch b) {
+  std::vector<SetupDesc> d;
+  std::vector<qmc_walkers> done;
+  int max_stride = 0;
+  for (size_t i = 0; i < b->members.size(); ++i) {
+    qmc_walkers w = b->members[i];
+    if (!w->pend_init) {
+      if (int rc = ensure_cleared(w)) return rc;
+      continue;
+    }
+    const auto& at = b->member_at[i];
+    if (!setup_fits(w->stride) || !at.first || b->seen_version[i] != w->version) {
+      if (int rc = materialize(w)) return rc;
+      continue;
+    }
+    add_setup_descs(w, at.first->d_units.p + at.second, d);
+    max_stride = std::max(max_stride, w->stride);
+    done.push_back(w);
+  }
+  if (int rc = launch_setups(b->ctx, d, max_stride)) return rc;
+  for (qmc_walkers w : done) setup_done(w);
+  return QMC_OK;
+}
+
 int qmc_batch_run(qmc_batch b, int32_t max_sweeps) {
   if (!b) return fail(QMC_INVALID_ARGUMENT, "null handle");
   int rc = batch_rebuild(b);
-  if (rc) return rc;
+  if (rc || (rc = batch_materialize(b))) return rc;
   qmc_ctx c = b->ctx;
   if ((rc = ensure_aux_streams(c))) return rc;
   HIP_TRY(hipEventRecord(c->ev0, c->stream));
@@ -1389,7 +1556,7 @@ static void drop_batch_graph(qmc_batch b) {
 int qmc_batch_run_chunked(qmc_batch b, int32_t sweeps_per_launch) {
   if (!b || sweeps_per_launch < 1) return fail(QMC_INVALID_ARGUMENT, "bad argument");
   int rc = batch_rebuild(b);
-  if (rc) return rc;
+  if (rc || (rc = batch_materialize(b))) return rc;
   qmc_ctx c = b->ctx;
   if ((rc = ensure_aux_streams(c))) return rc;
   int64_t longest = 0;
@@ -1438,6 +1605,8 @@ int qmc_batch_wait(qmc_batch b, double* ms) {
 
 int qmc_batch_read(qmc_batch b, double* lowest_energy, uint64_t* accepted, uint8_t* lowest_states) {
   if (!b) return fail(QMC_INVALID_ARGUMENT, "null handle");
+  for (qmc_walkers w : b->members)
+    if (int rc = materialize(w)) return rc;
   qmc_ctx c = b->ctx;
   std::vector<ReadDesc> desc;
   std::vector<int32_t> rep_member;
@@ -1512,6 +1681,7 @@ int qmc_walkers_set_pt_mode(qmc_walkers w, int mode) {
 int qmc_walkers_read(qmc_walkers w, double* cost, double* lowest, double* beta, uint64_t* accepted,
                      uint8_t* states, uint8_t* lowest_states) {
   if (!w) return fail(QMC_INVALID_ARGUMENT, "null handle");
+  if (int rc0 = materialize(w)) return rc0;
   qmc_problem p = w->prob;
   hipStream_t st = p->ctx->stream;
   HIP_TRY(hipStreamSynchronize(st));
@@ -1542,6 +1712,7 @@ int qmc_walkers_read(qmc_walkers w, double* cost, double* lowest, double* beta, 
 int qmc_walkers_read_replica(qmc_walkers w, int32_t replica, uint8_t* state, uint8_t* lowest_state) {
   if (!w) return fail(QMC_INVALID_ARGUMENT, "null handle");
   if (replica < 0 || replica >= w->R) return fail(QMC_INVALID_ARGUMENT, "replica out of range");
+  if (int rc0 = materialize(w)) return rc0;
   qmc_problem p = w->prob;
   hipStream_t st = p->ctx->stream;
   const UnitDev* du = nullptr;
@@ -1559,6 +1730,7 @@ int qmc_walkers_read_replica(qmc_walkers w, int32_t replica, uint8_t* state, uin
 
 int qmc_walkers_energy(qmc_walkers w, int lowest, double* energies) {
   if (!w || !energies) return fail(QMC_INVALID_ARGUMENT, "null argument");
+  if (int rc0 = materialize(w)) return rc0;
   qmc_problem p = w->prob;
   hipStream_t st = p->ctx->stream;
   const UnitDev* du = nullptr;
@@ -1573,6 +1745,7 @@ int qmc_walkers_energy(qmc_walkers w, int lowest, double* energies) {
 
 int qmc_walkers_labels(qmc_walkers w, int32_t* labels) {
   if (!w || !labels) return fail(QMC_INVALID_ARGUMENT, "null argument");
+  if (int rc0 = materialize(w)) return rc0;
   HIP_TRY(hipStreamSynchronize(w->prob->ctx->stream));
   HIP_TRY(hipMemcpy(labels, w->label.p, w->R * sizeof(int32_t), hipMemcpyDeviceToHost));
   return QMC_OK;

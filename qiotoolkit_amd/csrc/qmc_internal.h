@@ -164,6 +164,27 @@ hipError_t launch_exchange(double* cost, double* beta, int32_t* label, const dou
                            int spr, int64_t* last_step, uint32_t seed, uint32_t stream,
                            hipStream_t st);
 hipError_t launch_init(const UnitDev* units, int n_units, int N, uint32_t epoch, hipStream_t st);
+// Set-up of one freshly created unit (one work-group per entry): optionally
+// clear its counters (accepted, beta, label, the walker set's error flag),
+// set its progress, draw the initial states of `epoch` (spins = lowest) and
+// store every replica's energy as its cost and lowest cost, summed exactly
+// as launch_energy sums.
+struct SetupDesc {
+  const UnitDev* unit;
+  const int64_t* term_off;
+  const int32_t* term_var;
+  const double* term_c;
+  int64_t M;
+  int64_t sweep0, n_sweeps;  // progress to set (set_progress != 0)
+  int32_t kind;
+  uint32_t epoch;
+  int32_t clear;
+  int32_t set_progress;
+};
+// max_stride: largest spin array among the units; fails with
+// hipErrorInvalidValue when it does not fit in LDS (setup_fits says so).
+bool setup_fits(int max_stride);
+hipError_t launch_setup(const SetupDesc* descs, int n, int max_stride, hipStream_t st);
 hipError_t launch_unpack(const UnitDev* units, int B, int r0, int R, int N, int which, uint8_t* out,
                          hipStream_t st);
 hipError_t launch_pack(const UnitDev* units, int n_units, int B, int R, int N, const uint8_t* in,

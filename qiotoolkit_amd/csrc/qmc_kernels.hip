@@ -904,6 +904,97 @@ __global__ void k_init_states(const UnitDev* __restrict__ units, uint32_t epoch)
   }
 }
 
+// One work-group of 256 threads per set-up entry: counters, progress,
+// initial states (as k_init_states, staged in LDS) and energies.  Eight
+// replicas of a byte lane are summed together: thread t adds terms t,
+// t + 256, ... in order and the 256 partial sums meet in the fixed tree of
+// state_energy_256, so each replica's energy equals launch_energy's bit for
+// bit.
+__global__ void __launch_bounds__(256) k_setup_units(const SetupDesc* __restrict__ descs) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  double* part = reinterpret_cast<double*>(smem_raw);  // [8][256]
+  uint8_t* sp = smem_raw + 8 * 256 * sizeof(double);
+  const SetupDesc& D = descs[blockIdx.x];
+  const UnitDev& U = *D.unit;
+  const int tid = threadIdx.x;
+  const int N = U.N, G = U.G;
+  const int nrep = unit_nrep(U);
+  const uint32_t epoch = D.epoch;
+  if (D.clear) {
+    for (int r = tid; r < (int)U.nrep; r += 256) {
+      U.accepted[r] = 0ull;
+      U.beta[r] = 0.0;
+      U.label[r] = 0;
+    }
+    if (U.rbase == 0 && tid == 0) *U.err = 0;
+    if (!D.set_progress && tid < 2) U.progress[tid] = 0;
+  }
+  if (D.set_progress && tid == 0) {
+    U.progress[0] = D.sweep0;
+    U.progress[1] = D.n_sweeps;
+  }
+  for (int q = tid; q <= N; q += 256) {
+    for (int g = 0; g < G; ++g) {
+      uint32_t byte = 0;
+      if (q < N) {
+        const uint32_t i = (uint32_t)U.var_of_pos[q];
+        const uint32_t wi = i >> 5;
+        for (int b = 0; b < 8; ++b) {
+          const int rr = 8 * g + b;
+          if (rr < nrep) {
+            const uint4 w = philox4x32_10(wi >> 2, U.r0 + (uint32_t)rr, epoch, DOM_INIT, U.seed, U.stream);
+            byte |= ((pick(w, wi & 3u) >> (i & 31u)) & 1u) << b;
+          }
+        }
+      }
+      U.spins[q * G + g] = (uint8_t)byte;
+      U.best[q * G + g] = (uint8_t)byte;
+      sp[q * G + g] = (uint8_t)byte;
+    }
+  }
+  __syncthreads();
+  const int64_t* __restrict__ term_off = D.term_off;
+  const int32_t* __restrict__ term_var = D.term_var;
+  const double* __restrict__ term_c = D.term_c;
+  const int64_t M = D.M;
+  const int kind = D.kind;
+  for (int g = 0; g * 8 < nrep; ++g) {
+    double acc[8];
+#pragma unroll
+    for (int b = 0; b < 8; ++b) acc[b] = 0.0;
+    for (int64_t j = tid; j < M; j += 256) {
+      uint32_t x = 0;
+      for (int64_t k = term_off[j]; k < term_off[j + 1]; ++k) {
+        const uint32_t byte = sp[U.pos_of_var[term_var[k]] * G + g];
+        x = (kind == QMC_ISING) ? (x ^ byte) : (x | byte);
+      }
+      const double c = term_c[j];
+#pragma unroll
+      for (int b = 0; b < 8; ++b) {
+        const uint32_t bit = (x >> b) & 1u;
+        if (kind == QMC_ISING) acc[b] += bit ? -c : c;
+        else if (!bit) acc[b] += c;
+      }
+    }
+#pragma unroll
+    for (int b = 0; b < 8; ++b) part[b * 256 + tid] = acc[b];
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+      if (tid < w) {
+#pragma unroll
+        for (int b = 0; b < 8; ++b) part[b * 256 + tid] += part[b * 256 + tid + w];
+      }
+      __syncthreads();
+    }
+    if (tid < 8 && 8 * g + tid < nrep) {
+      const double e = part[tid * 256];
+      U.cost[8 * g + tid] = e;
+      U.best_cost[8 * g + tid] = e;
+    }
+    __syncthreads();
+  }
+}
+
 // plain states [r][var] <-> unit bytes
 __global__ void k_unpack(const UnitDev* __restrict__ units, int B, int r0, int R, int N, int which,
                          uint8_t* __restrict__ out) {
@@ -1590,6 +1681,19 @@ hipError_t launch_exchange(double* cost, double* beta, int32_t* label, const dou
 
 hipError_t launch_init(const UnitDev* units, int n_units, int N, uint32_t epoch, hipStream_t st) {
   hipLaunchKernelGGL(k_init_states, dim3((N + 1 + 127) / 128, n_units), dim3(128), 0, st, units, epoch);
+  return hipGetLastError();
+}
+
+bool setup_fits(int max_stride) { return 8 * 256 * sizeof(double) + (size_t)max_stride <= 160 * 1024; }
+
+hipError_t launch_setup(const SetupDesc* descs, int n, int max_stride, hipStream_t st) {
+  if (!setup_fits(max_stride)) return hipErrorInvalidValue;
+  if (n <= 0) return hipSuccess;
+  static hipError_t once = hipFuncSetAttribute((const void*)k_setup_units,
+                                               hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+  if (once != hipSuccess) return once;
+  const size_t smem = 8 * 256 * sizeof(double) + (size_t)max_stride;
+  hipLaunchKernelGGL(k_setup_units, dim3(n), dim3(256), smem, st, descs);
   return hipGetLastError();
 }
 
