@@ -179,7 +179,8 @@ static hipError_t upload_async(qmc_ctx c, void* dst, const void* src, size_t byt
 struct EllLayout {
   DevBuf<int32_t> slice_base;
   DevBuf<int2> slice;
-  DevBuf<uint2> ent;
+  DevBuf<uint2> ent;   // pairwise problems
+  DevBuf<uint4> ent4;  // general problems with at most 3 other members per term
 };
 
 struct qmc_problem_s {
@@ -188,6 +189,8 @@ struct qmc_problem_s {
   std::map<int, std::unique_ptr<EllLayout>> ell;  // per lane count (pairwise problems)
   std::vector<int32_t> h_inc_off, h_nbr;          // host copies to build layouts
   std::vector<double> h_coef;
+  std::vector<int32_t> h_oth_off, h_oth_pos;      // general problems
+  int max_others = 0;                             // most other members of one term
   int kind = QMC_ISING;
   int32_t N = 0;
   int64_t M = 0;
@@ -479,6 +482,11 @@ int qmc_problem_create(qmc_ctx ctx, int kind, int32_t n_vars, int64_t n_terms,
   p->n_inc = (int64_t)nbr.size();
   p->h_inc_off = inc_off;
   p->h_nbr = nbr;
+  for (size_t e = 0; e + 1 < oth_off.size(); ++e) p->max_others = std::max(p->max_others, oth_off[e + 1] - oth_off[e]);
+  if (!p->pairwise) {
+    p->h_oth_off = oth_off;
+    p->h_oth_pos = oth_pos;
+  }
   p->h_coef.resize(nbr.size());
   for (size_t e = 0; e < nbr.size(); ++e) p->h_coef[e] = coef_f[e];
   if (p->coupling == QMC_COUPLING_I32)
@@ -574,6 +582,54 @@ static EllLayout* get_layout(qmc_problem p, int lanes) {
   std::vector<int32_t> sbase(nclass + 1, 0);
   std::vector<int2> slice;
   std::vector<uint2> ent;
+  if (!p->pairwise) {
+    // general terms with at most 3 other members: 16-byte entries
+    if (p->max_others > 3) return nullptr;
+    const uint32_t ghost = (uint32_t)p->N;
+    std::vector<uint4> ent4;
+    for (int c = 0; c < nclass; ++c) {
+      const int32_t c0 = p->class_off[c], nc = p->class_off[c + 1] - c0;
+      sbase[c] = (int32_t)slice.size();
+      for (int32_t k = 0; k * lanes < nc; ++k) {
+        int32_t w = 0;
+        for (int32_t t = 0; t < lanes && k * lanes + t < nc; ++t) {
+          const int32_t q = c0 + k * lanes + t;
+          w = std::max(w, p->h_inc_off[q + 1] - p->h_inc_off[q]);
+        }
+        slice.push_back(make_int2((int32_t)ent4.size(), w));
+        for (int32_t e = 0; e < w; ++e)
+          for (int32_t t = 0; t < lanes; ++t) {
+            const int32_t rho = k * lanes + t;
+            uint4 x = make_uint4(bits(0.0), ghost, ghost, ghost);
+            if (rho < nc) {
+              const int32_t q = c0 + rho;
+              const int32_t e0 = p->h_inc_off[q], deg = p->h_inc_off[q + 1] - e0;
+              if (e < deg) {
+                const int32_t i = e0 + e;
+                uint32_t o[3] = {ghost, ghost, ghost};
+                for (int32_t m = p->h_oth_off[i]; m < p->h_oth_off[i + 1]; ++m)
+                  o[m - p->h_oth_off[i]] = (uint32_t)p->h_oth_pos[m];
+                x = make_uint4(bits(p->h_coef[i]), o[0], o[1], o[2]);
+              }
+            }
+            ent4.push_back(x);
+          }
+      }
+    }
+    sbase[nclass] = (int32_t)slice.size();
+    // two sentinel slices: the sweep reads slice descriptors up to two
+    // rounds past a class end
+    slice.push_back(make_int2((int32_t)ent4.size(), 0));
+    slice.push_back(make_int2((int32_t)ent4.size(), 0));
+    ent4.push_back(make_uint4(bits(0.0), ghost, ghost, ghost));
+    const bool ok = L->slice_base.upload(sbase.data(), sbase.size()) == hipSuccess &&
+                    L->slice.upload(slice.data(), slice.size()) == hipSuccess &&
+                    L->ent4.upload(ent4.data(), ent4.size()) == hipSuccess;
+    if (!ok) return nullptr;
+    EllLayout* raw = L.get();
+    p->ell[lanes] = std::move(L);
+    return raw;
+  }
   for (int c = 0; c < nclass; ++c) {
     const int32_t c0 = p->class_off[c], nc = p->class_off[c + 1] - c0;
     sbase[c] = (int32_t)slice.size();
@@ -615,13 +671,14 @@ static EllLayout* get_layout(qmc_problem p, int lanes) {
 static void build_units(qmc_walkers w) {
   qmc_problem p = w->prob;
   w->units.assign(w->n_units, UnitDev{});
-  EllLayout* ell = p->pairwise ? get_layout(p, w->lanes) : nullptr;
+  EllLayout* ell = (p->pairwise || p->max_others <= 3) ? get_layout(p, w->lanes) : nullptr;
   for (int u = 0; u < w->n_units; ++u) {
     UnitDev& U = w->units[u];
     if (ell) {
       U.ell_slice_base = ell->slice_base.p;
       U.ell_slice = ell->slice.p;
-      U.ell_ent = ell->ent.p;
+      U.ell_ent = p->pairwise ? ell->ent.p : nullptr;
+      U.ell4_ent = p->pairwise ? nullptr : ell->ent4.p;
     }
     U.class_off = p->d_class_off.p;
     U.var_of_pos = p->d_var_of_pos.p;

@@ -39,6 +39,10 @@ struct UnitDev {
   const int32_t* ell_slice_base;
   const int2* ell_slice;
   const uint2* ell_ent;
+  // general problems whose terms have at most 4 variables: the same slices
+  // over 16-byte entries {coefficient bits, the other members' positions,
+  // padded with the ghost position N} (null: incidence CSR below)
+  const uint4* ell4_ent;
   const int32_t* class_off;
   const int32_t* var_of_pos;
   const int32_t* pos_of_var;

@@ -216,6 +216,22 @@ __device__ __forceinline__ void local_field(const UnitDev& U, const uint8_t* sp,
     }
     return;
   }
+  if (U.ell4_ent) {
+    // sliced entries {coefficient, up to 3 other members} in term order;
+    // the ghost position pads missing members (its spin is always 0)
+    gptr<uint64_t> p = glob(reinterpret_cast<const uint64_t*>(U.ell4_ent)) + 2 * (slice.x + tp);
+    const int step = 2 << lg;
+#pragma unroll 4
+    for (int j = 0; j < slice.y; ++j) {
+      const uint64_t lo = p[j * step], hi = p[j * step + 1];
+      const uint32_t w0 = sp[(int)(uint32_t)(lo >> 32) * G + g];
+      const uint32_t w1 = sp[(int)(uint32_t)hi * G + g];
+      const uint32_t w2 = sp[(int)(uint32_t)(hi >> 32) * G + g];
+      const uint32_t w = (KIND == QMC_ISING) ? (w0 ^ w1 ^ w2) : (w0 | w1 | w2);
+      accumulate<KIND, CT, NB>(w, coef_of<CT>((uint32_t)lo), h);
+    }
+    return;
+  }
   gptr<CT> coef = glob(static_cast<const CT*>(U.coef));
   gptr<int32_t> inc = glob(U.inc_off);
   gptr<int32_t> ooff = glob(U.oth_off);
@@ -237,21 +253,6 @@ template <int KIND, typename CT>
 __device__ __forceinline__ CT delta_from_field(CT h, uint32_t own_bit) {
   if (KIND == QMC_ISING) return own_bit ? CT(2) * h : CT(-2) * h;
   return own_bit ? h : -h;
-}
-
-// Single-replica dE recomputation (rare path: locating the lowest state).
-template <int KIND, bool PAIR, typename CT, int NB>
-__device__ CT delta_one(const UnitDev& U, const uint8_t* sp, int q, int2 slice, int tp, int lg, int G, int g,
-                        int b, uint32_t own_bit) {
-  EllRow e;
-  if (PAIR) ell_fetch(U, slice, tp, lg, e);
-  CT h[8];
-  local_field<KIND, PAIR, CT, NB>(U, sp, q, e, slice, tp, lg, G, g, h);
-  CT hb = h[0];
-#pragma unroll
-  for (int k = 1; k < NB; ++k)
-    if (k == b) hb = h[k];
-  return delta_from_field<KIND, CT>(hb, own_bit);
 }
 
 // Energy hand-off between the work-groups of one PT ladder (cooperative
@@ -324,6 +325,9 @@ k_sweep(const UnitDev* units, SweepArgs a) {
   const int stride = U.stride;
   cptr<int32_t> class_off = cnst(U.class_off);
   cptr<int32_t> slice_base = cnst(U.ell_slice_base);
+  // general problems read sliced entries when their terms are at most 4-local
+  const bool gen_ell = !PAIR && U.ell4_ent != nullptr;
+  const bool slices = PAIR || gen_ell;
   gptr<int32_t> var_of_pos = glob(U.var_of_pos);
   uint8_t* sp = smem_raw + sizeof(Smem);
   uint8_t* fl = sp + stride;
@@ -400,10 +404,12 @@ k_sweep(const UnitDev* units, SweepArgs a) {
     if (PAIR) {
       sl = load_slice(U, csb + k);
       ell_fetch(U, sl, tp, lg, e);
+    } else if (gen_ell) {
+      sl = load_slice(U, csb + k);
     }
     e.var = (uint32_t)var_of_pos[cc0 + rho];  // padded past position N - 1
   };
-  if (active) fetch_row(class_off[0], PAIR ? slice_base[0] : 0, 0, sla, ra);
+  if (active) fetch_row(class_off[0], slices ? slice_base[0] : 0, 0, sla, ra);
 
   // parallel tempering: exchange step of the current sweep, sweeps run into
   // it and its energy buffer (step mod 3), advanced per sweep
@@ -431,10 +437,10 @@ k_sweep(const UnitDev* units, SweepArgs a) {
       const int c0 = class_off[c];
       const int nc = class_off[c + 1] - c0;
       const int K = (nc + TP - 1) >> lg;
-      const int sbase = PAIR ? slice_base[c] : 0;
+      const int sbase = slices ? slice_base[c] : 0;
       const int cn = c + 1 < n_classes ? c + 1 : 0;  // the class after this one
       const int c0n = class_off[cn];
-      const int sbasen = PAIR ? slice_base[cn] : 0;
+      const int sbasen = slices ? slice_base[cn] : 0;
       // per replica: sum of the accepted dE of this lane's positions, the
       // minimum prefix of that sum and the round where it is first reached
       AT Ssum[8], Smin[8];
