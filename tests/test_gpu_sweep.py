@@ -34,6 +34,16 @@ def test_philox_device_matches_kat_and_oracle(ctx):
     assert np.array_equal(got, want)
 
 
+def _klocal(n, m, kmax, seed):
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(m):
+        ids = sorted(rng.choice(n, size=1 + k % kmax, replace=False).tolist())
+        c = int(rng.integers(1, 8)) * (1 if rng.random() < 0.5 else -1)
+        out.append(dict(c=c, ids=ids))
+    return out
+
+
 PROBLEMS = {
     "lattice2d_pm1": ("ising", lambda: wl.lattice2d(8, seed=3)),
     "er_int": ("ising", lambda: wl.er_ising(150, 6.0, seed=4)),
@@ -43,6 +53,10 @@ PROBLEMS = {
         [sorted(np.random.default_rng(100 + k).choice(40, size=1 + k % 4, replace=False).tolist())
          for k in range(300)]) if c != 0]),
     "lattice3d_f32": ("ising", lambda: wl.lattice3d(6, seed=7)),
+    # terms of up to 6 variables: the incidence-CSR path (more than 3 other
+    # members per term do not fit the 16-byte sliced entries)
+    "pubo_6local": ("pubo", lambda: _klocal(60, 240, 6, seed=12)),
+    "ising_6local": ("ising", lambda: _klocal(50, 200, 6, seed=13)),
 }
 
 
@@ -94,7 +108,8 @@ def _run_pair(ctx, kind, terms, R, B, betas, seed=7, ordered=False, lanes=0):
                                accepted=acc)
 
 
-@pytest.mark.parametrize("name", ["lattice2d_pm1", "er_int", "pubo_int", "ising_4local"])
+@pytest.mark.parametrize("name", ["lattice2d_pm1", "er_int", "pubo_int", "ising_4local", "pubo_6local",
+                                  "ising_6local"])
 @pytest.mark.parametrize("B", [1, 5, 8, 32])
 def test_sa_sweeps_bit_exact_int(ctx, name, B):
     kind, gen = PROBLEMS[name]
@@ -122,7 +137,19 @@ def test_sa_sweeps_lanes_and_slices(ctx, lanes, B):
         assert np.array_equal(got[k], want[k]), k
 
 
-@pytest.mark.parametrize("name", ["lattice2d_pm1", "er_int", "pubo_int"])
+@pytest.mark.parametrize("lanes", [64, 256])
+@pytest.mark.parametrize("B", [2, 8])
+def test_pubo_sliced_entries_lanes(ctx, lanes, B):
+    """4-local PUBO through the sliced 16-byte entries at several lane counts."""
+    terms = wl.random_pubo(700, 5000, seed=9)
+    betas = np.linspace(0.01, 5.0, 10)
+    op, qp, w, got, want = _run_pair(ctx, "pubo", terms, R=11, B=B, betas=betas, lanes=lanes)
+    assert qp.info.pairwise == 0 and w.lanes() == lanes
+    for k in ("states", "cost", "lowest", "lowest_states", "accepted"):
+        assert np.array_equal(got[k], want[k]), k
+
+
+@pytest.mark.parametrize("name", ["lattice2d_pm1", "er_int", "pubo_int", "pubo_6local"])
 def test_ordered_sweeps_bit_exact_int(ctx, name):
     kind, gen = PROBLEMS[name]
     betas = np.linspace(0.1, 3.0, 12)
