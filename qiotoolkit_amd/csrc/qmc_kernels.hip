@@ -221,7 +221,7 @@ __device__ __forceinline__ void local_field(const UnitDev& U, const uint8_t* sp,
     // the ghost position pads missing members (its spin is always 0)
     gptr<uint64_t> p = glob(reinterpret_cast<const uint64_t*>(U.ell4_ent)) + 2 * (slice.x + tp);
     const int step = 2 << lg;
-#pragma unroll 4
+#pragma unroll 8  // eight entries' loads in flight (deeper unrolling runs out of registers)
     for (int j = 0; j < slice.y; ++j) {
       const uint64_t lo = p[j * step], hi = p[j * step + 1];
       const uint32_t w0 = sp[(int)(uint32_t)(lo >> 32) * G + g];
