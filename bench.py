@@ -29,7 +29,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-LDS_PEAK_GBS = 256 * 128 * 2.4  # 256 CUs x 128 B/clk x 2.4 GHz
+# LDS: 256 CUs x 256 B/clk (CDNA4 LDS reads) x 2.4 GHz.  The 128 B/clk of
+# earlier CDNA parts is exceeded on the device: a streaming 16-byte LDS read
+# kernel (--lds-probe) measures 106-109 TB/s.
+LDS_PEAK_GBS = 256 * 256 * 2.4
 
 # HBM bytes per sweep-kernel launch of the default workload (5000 sweeps,
 # automatic grouping and lanes), from rocprofv3 --pmc FETCH_SIZE and
@@ -384,7 +387,16 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (calibrates the HBM byte counters)")
+    ap.add_argument("--lds-probe", action="store_true",
+                    help="only measure the aggregate LDS read bandwidth (the LDS peak of the roofline)")
     args = ap.parse_args()
+    if args.lds_probe:
+        from qiotoolkit_amd import _qmc
+        ctx = _qmc.Context(0)
+        bw = [ctx.lds_bandwidth() for _ in range(3)]
+        print(json.dumps({"lds_read_bytes_per_s": max(bw), "runs": bw, "assumed_peak": LDS_PEAK_GBS * 1e9}))
+        ctx.close()
+        return
     if args.copy_probe:
         import torch
         n = args.copy_probe * (1 << 20) // 4

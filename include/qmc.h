@@ -97,7 +97,9 @@ int qmc_walkers_create(qmc_problem p, int32_t n_replicas, int32_t replicas_per_g
                        uint32_t seed, uint32_t stream, int32_t first_replica,
                        int32_t replicas_per_seed, qmc_walkers* out);
 void qmc_walkers_destroy(qmc_walkers w);
-/* random initial states (counter contract, INIT domain), cost, lowest = cost */
+/* random initial states (counter contract, INIT domain), cost, lowest = cost.
+ * Recorded here and performed by the next call that needs the state (for a
+ * batch: by qmc_batch_run, one launch for every pending member). */
 int qmc_walkers_init(qmc_walkers w, uint32_t epoch);
 /* set states (n_replicas * n_vars bytes, 1 = the reference's `true` bit) */
 int qmc_walkers_set_states(qmc_walkers w, const uint8_t* states);
@@ -227,6 +229,11 @@ int qmc_delta_all(qmc_problem p, int32_t n_states, const uint8_t* states, double
 
 /* Philox4x32-10 words computed on the device: out[4*n] for counters ctr[4*n]. */
 int qmc_philox(qmc_ctx ctx, int32_t n, const uint32_t* ctr, const uint32_t key[2], uint32_t* out);
+/* Aggregate LDS read bandwidth of the device in bytes/s, measured by a
+ * kernel that streams 16-byte LDS reads on every CU (no counterpart in the
+ * reference: it calibrates the LDS peak the roofline figures are quoted
+ * against). */
+int qmc_lds_bandwidth(qmc_ctx ctx, double* bytes_per_s);
 
 /* Timing of the last sweep launch on this walker set (HIP events on the
  * stream it ran on), milliseconds. */

@@ -27,7 +27,7 @@ EXPORTS = [
     "qmc_batch_destroy", "qmc_batch_add", "qmc_batch_remove", "qmc_batch_clear", "qmc_batch_run",
     "qmc_batch_wait", "qmc_walkers_read_replica", "qmc_batch_read", "qmc_batch_run_chunked", "qmc_walkers_resample", "qmc_walkers_count", "qmc_walkers_set_pt_mode",
     "qmc_walkers_set_lanes", "qmc_walkers_get_lanes", "qmc_walkers_pa_begin", "qmc_walkers_pa_step",
-    "qmc_walkers_pa_info", "qmc_walkers_pa_end",
+    "qmc_walkers_pa_info", "qmc_walkers_pa_end", "qmc_lds_bandwidth",
 ]
 
 
@@ -119,6 +119,7 @@ def lib():
     L.qmc_walkers_pa_step.argtypes = [vp]
     L.qmc_walkers_pa_info.argtypes = [vp, P(PaInfo), vp]
     L.qmc_walkers_pa_end.argtypes = [vp]
+    L.qmc_lds_bandwidth.argtypes = [vp, P(C.c_double)]
     _lib = L
     return L
 
@@ -150,6 +151,12 @@ class Context:
 
     def sync(self):
         _check(lib().qmc_sync(self.h))
+
+    def lds_bandwidth(self):
+        """Measured aggregate LDS read bandwidth in bytes/s."""
+        v = C.c_double()
+        _check(lib().qmc_lds_bandwidth(self.h, C.byref(v)))
+        return v.value
 
     def philox(self, ctrs, key):
         ctrs = np.ascontiguousarray(ctrs, dtype=np.uint32).reshape(-1, 4)

@@ -1842,6 +1842,26 @@ int qmc_delta_all(qmc_problem p, int32_t n_states, const uint8_t* states, double
   return QMC_OK;
 }
 
+int qmc_lds_bandwidth(qmc_ctx ctx, double* bytes_per_s) {
+  if (!ctx || !bytes_per_s) return fail(QMC_INVALID_ARGUMENT, "null argument");
+  int dev = 0, cus = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  const int blocks = 2 * cus, iters = 1 << 15;
+  DevBuf<uint32_t> out;
+  HIP_TRY(out.alloc((size_t)blocks * 1024));
+  (void)launch_lds_probe(blocks, 64, out.p, ctx->stream);  // warm-up
+  HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
+  const double bytes = launch_lds_probe(blocks, iters, out.p, ctx->stream);
+  HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  if (bytes < 0) return fail(QMC_DEVICE_ERROR, "LDS probe launch failed");
+  float ms = 0.f;
+  HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+  *bytes_per_s = bytes / (ms * 1e-3);
+  return QMC_OK;
+}
+
 int qmc_philox(qmc_ctx ctx, int32_t n, const uint32_t* ctr, const uint32_t key[2], uint32_t* out) {
   if (!ctx || !ctr || !key || !out || n < 0) return fail(QMC_INVALID_ARGUMENT, "bad argument");
   if (n == 0) return QMC_OK;

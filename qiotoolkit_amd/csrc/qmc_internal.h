@@ -230,6 +230,9 @@ struct ReadDesc {
 hipError_t launch_batch_read(const ReadDesc* desc, const int32_t* rep_member, int64_t n_replicas,
                              uint8_t* states, double* energy, unsigned long long* accepted,
                              hipStream_t st);
+// LDS streaming probe: `blocks` work-groups of 1024 threads, each thread
+// reading `iters` 16-byte LDS words; returns the bytes read.
+double launch_lds_probe(int blocks, int iters, uint32_t* out, hipStream_t st);
 hipError_t launch_philox(int n, const uint32_t* ctr, uint32_t k0, uint32_t k1, uint32_t* out,
                          hipStream_t st);
 

@@ -1567,6 +1567,27 @@ k_pa_energy(PaArgs A, const int64_t* __restrict__ term_off, const int32_t* __res
   }
 }
 
+// LDS read bandwidth probe: consecutive lanes read consecutive 16-byte words
+// (no bank conflicts); the accumulated words keep the loads alive.
+__global__ void __launch_bounds__(1024) k_lds_probe(int iters, uint32_t* __restrict__ out) {
+  __shared__ uint4 buf[4096];
+  for (int i = threadIdx.x; i < 4096; i += blockDim.x)
+    buf[i] = make_uint4((uint32_t)i, (uint32_t)i * 3u, (uint32_t)i * 5u, (uint32_t)i * 7u);
+  __syncthreads();
+  uint4 acc = make_uint4(0u, 0u, 0u, 0u);
+  int idx = threadIdx.x;
+#pragma unroll 8
+  for (int it = 0; it < iters; ++it) {
+    const uint4 v = buf[idx];
+    acc.x ^= v.x;
+    acc.y += v.y;
+    acc.z ^= v.z;
+    acc.w += v.w;
+    idx = (idx + 64) & 4095;
+  }
+  out[blockIdx.x * blockDim.x + threadIdx.x] = acc.x ^ acc.y ^ acc.z ^ acc.w;
+}
+
 // ---------------------------------------------------------------------------
 // launch helpers (host side of this translation unit)
 size_t sweep_smem_bytes(int max_stride, int ordered, int* best_lds) {
@@ -1796,6 +1817,11 @@ hipError_t launch_delta_all(const ProblemDev& P, int n_states, const uint8_t* st
     else hipLaunchKernelGGL((k_delta_all<QMC_PUBO, float>), grid, dim3(256), 0, st, P, n_states, states, out);
   }
   return hipGetLastError();
+}
+
+double launch_lds_probe(int blocks, int iters, uint32_t* out, hipStream_t st) {
+  hipLaunchKernelGGL(k_lds_probe, dim3(blocks), dim3(1024), 0, st, iters, out);
+  return hipGetLastError() == hipSuccess ? (double)blocks * 1024.0 * iters * 16.0 : -1.0;
 }
 
 hipError_t launch_philox(int n, const uint32_t* ctr, uint32_t k0, uint32_t k1, uint32_t* out,

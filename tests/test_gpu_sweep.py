@@ -348,3 +348,10 @@ def test_handles_destroyed_in_any_order():
     w.close()
     assert b.read()[0]["lowest"].shape == (8,)
     b.close()
+
+
+def test_lds_bandwidth_probe(ctx):
+    """The LDS streaming probe reports a plausible aggregate bandwidth
+    (MI355X: 256 CUs x 128 B/clk x 2.4 GHz = 78.6 TB/s nominal)."""
+    bw = ctx.lds_bandwidth()
+    assert 1e13 < bw < 2e14, bw
