@@ -86,7 +86,10 @@ int qiotk_model_vars(qiotk_model m, int32_t* n_vars, int32_t* names);
  * benchmark.serving {arrival_ms, admitted_ms, finished_ms, latency_ms,
  * queue_ms, rounds}; a result does not depend on what else was in flight.
  * config_json: {"hbm_budget_bytes": 68719476736, "max_active": 64,
- * "sweeps_per_round": 10} (all optional). */
+ * "sweeps_per_round": 10, "round_target_ms": 2.0, "max_rounds_per_wait": 16}
+ * (all optional): rounds are enqueued back to back until about
+ * round_target_ms of device work is in flight (at most max_rounds_per_wait)
+ * before the loop waits for the device, admits and completes solves. */
 typedef struct qiotk_server_s* qiotk_server;
 int qiotk_server_create(int device, const char* config_json, qiotk_server* out);
 /* Enqueue a solve (model as a term CSR, kind 0 ising / 1 pubo, as in

@@ -187,7 +187,9 @@ int qmc_batch_run(qmc_batch b, int32_t max_sweeps);
  * replayed; the units' progress lives on the device, so replays take no
  * argument updates.  Not for split PT ladders. */
 int qmc_batch_run_chunked(qmc_batch b, int32_t sweeps_per_launch);
-/* Wait for the last qmc_batch_run; ms = its device time (HIP events). */
+/* Wait for the runs since the previous wait; ms = their device time span
+ * (HIP events from the first run's start to the last run's end, including
+ * any work enqueued between the runs). */
 int qmc_batch_wait(qmc_batch b, double* ms);
 /* Read back every member in the order added, in one launch and one copy per
  * array: member k owns the next R_k entries of lowest_energy (FP64 energy of

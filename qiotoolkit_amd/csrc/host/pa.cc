@@ -131,14 +131,17 @@ bool PopulationAnnealing::serve_plan(qmc_batch b) {
     device_check(qmc_batch_add(b, st.w), "qmc_batch_add");
     in_batch_ = true;
   }
+  step_planned_ = true;
   return true;
 }
 
-bool PopulationAnnealing::serve_after() {
-  if (model_->empty()) return true;
+void PopulationAnnealing::serve_advance() {
+  if (!step_planned_) return;
+  step_planned_ = false;
   finish_step();
-  return step_ >= steps_to_run();
 }
+
+bool PopulationAnnealing::serve_done() { return model_->empty() || step_ >= steps_to_run(); }
 
 qmc_walkers PopulationAnnealing::walkers() const {
   return impl_ ? static_cast<PaState*>(impl_.get())->w : nullptr;

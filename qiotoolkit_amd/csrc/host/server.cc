@@ -103,6 +103,11 @@ struct qiotk_server_s {
   int64_t budget = 64ll << 30;
   int max_active = 64;
   int sweeps_per_round = 10;
+  // rounds enqueued per wait for the device: adapted so that about
+  // round_target_ms of device work is in flight, at most max_rounds_per_wait
+  double round_target_ms = 2.0;
+  int max_rounds_per_wait = 16;
+  int rounds_per_wait = 1;
   double t0 = 0;
 
   std::mutex mu;
@@ -229,6 +234,15 @@ struct qiotk_server_s {
     jp.reset();
   }
 
+  // A solve that failed mid-flight: out of the batch, its error as the result.
+  void drop(std::unique_ptr<Job>& jp, int code, const std::string& msg) {
+    Job& j = *jp;
+    if (j.in_batch && j.solver && j.solver->walkers()) qmc_batch_remove(batch, j.solver->walkers());
+    finish(j, error_doc(code, msg), code);
+    used -= j.footprint;
+    jp.reset();
+  }
+
   void run_loop() {
     while (true) {
       {
@@ -245,64 +259,71 @@ struct qiotk_server_s {
       }
       admit();
       if (active.empty()) continue;
-      // plan this round's device work of every active solve
-      for (auto& jp : active) {
-        Job& j = *jp;
-        try {
-          const bool added = j.solver->serve_plan(batch);
-          if (added) j.in_batch = true;
-          if (!added && !j.in_batch) {
-            // not expressible as rounds (time or cost limits): solve now
-            j.solver->run();
-            complete(jp);
+      // Enqueue rounds -- plan every active solve, run the batch, enqueue
+      // what follows each solve's round -- and wait for the device once.
+      int rc = QMC_OK;
+      int sub = 0;
+      while (sub < rounds_per_wait && rc == QMC_OK) {
+        for (auto& jp : active) {
+          Job& j = *jp;
+          try {
+            const bool added = j.solver->serve_plan(batch);
+            if (added) j.in_batch = true;
+            if (!added && !j.in_batch) {
+              // not expressible as rounds (time or cost limits): solve now
+              j.solver->run();
+              complete(jp);
+            }
+          } catch (const Error& e) {
+            drop(jp, e.code(), e.what());
+          } catch (const std::exception& e) {
+            drop(jp, kRuntimeError, e.what());
           }
-        } catch (const Error& e) {
-          if (j.in_batch && j.solver->walkers()) qmc_batch_remove(batch, j.solver->walkers());
-          finish(j, error_doc(e.code(), e.what()), e.code());
-          used -= j.footprint;
-          jp.reset();
-        } catch (const std::exception& e) {
-          if (j.in_batch && j.solver->walkers()) qmc_batch_remove(batch, j.solver->walkers());
-          finish(j, error_doc(kRuntimeError, e.what()), kRuntimeError);
-          used -= j.footprint;
-          jp.reset();
         }
+        active.erase(std::remove(active.begin(), active.end(), nullptr), active.end());
+        if (active.empty()) break;
+        rc = qmc_batch_run(batch, sweeps_per_round);
+        if (rc != QMC_OK) break;
+        ++sub;
+        for (auto& jp : active) {
+          try {
+            jp->solver->serve_advance();
+          } catch (const Error& e) {
+            drop(jp, e.code(), e.what());
+          } catch (const std::exception& e) {
+            drop(jp, kRuntimeError, e.what());
+          }
+        }
+        active.erase(std::remove(active.begin(), active.end(), nullptr), active.end());
       }
-      active.erase(std::remove(active.begin(), active.end(), nullptr), active.end());
-      if (active.empty()) continue;
+      if (sub == 0 && rc == QMC_OK) continue;
       double ms = 0;
-      int rc = qmc_batch_run(batch, sweeps_per_round);
       if (rc == QMC_OK) rc = qmc_batch_wait(batch, &ms);
       {
         std::lock_guard<std::mutex> lk(mu);
-        rounds++;
+        rounds += sub;
         device_ms += ms;
-        active_sum += (double)active.size();
+        active_sum += (double)active.size() * sub;
+      }
+      if (rc == QMC_OK && sub > 0 && ms > 0) {
+        const double per = ms / sub;
+        rounds_per_wait = std::max(1, std::min(max_rounds_per_wait, (int)(round_target_ms / per)));
       }
       for (auto& jp : active) {
         Job& j = *jp;
-        j.rounds++;
+        j.rounds += sub;
         if (rc != QMC_OK) {
-          if (j.in_batch && j.solver->walkers()) qmc_batch_remove(batch, j.solver->walkers());
-          finish(j, error_doc(rc, qmc_last_error()), rc);
-          used -= j.footprint;
-          jp.reset();
+          drop(jp, rc, qmc_last_error());
           continue;
         }
         bool finished = false;
         try {
-          finished = j.solver->serve_after();
+          finished = j.solver->serve_done();
         } catch (const Error& e) {
-          if (j.in_batch && j.solver->walkers()) qmc_batch_remove(batch, j.solver->walkers());
-          finish(j, error_doc(e.code(), e.what()), e.code());
-          used -= j.footprint;
-          jp.reset();
+          drop(jp, e.code(), e.what());
           continue;
         } catch (const std::exception& e) {
-          if (j.in_batch && j.solver->walkers()) qmc_batch_remove(batch, j.solver->walkers());
-          finish(j, error_doc(kRuntimeError, e.what()), kRuntimeError);
-          used -= j.footprint;
-          jp.reset();
+          drop(jp, kRuntimeError, e.what());
           continue;
         }
         if (finished) complete(jp);
@@ -322,8 +343,13 @@ int qiotk_server_create(int device, const char* config_json, qiotk_server* out) 
       if (c.has("hbm_budget_bytes")) s->budget = c.at("hbm_budget_bytes").as_int();
       if (c.has("max_active")) s->max_active = (int)c.at("max_active").as_int();
       if (c.has("sweeps_per_round")) s->sweeps_per_round = (int)c.at("sweeps_per_round").as_int();
-      if (s->budget <= 0 || s->max_active <= 0 || s->sweeps_per_round <= 0)
-        throw Error(kValueError, "hbm_budget_bytes, max_active and sweeps_per_round must be positive");
+      if (c.has("round_target_ms")) s->round_target_ms = c.at("round_target_ms").as_double();
+      if (c.has("max_rounds_per_wait")) s->max_rounds_per_wait = (int)c.at("max_rounds_per_wait").as_int();
+      if (s->budget <= 0 || s->max_active <= 0 || s->sweeps_per_round <= 0 || s->max_rounds_per_wait <= 0 ||
+          !(s->round_target_ms > 0))
+        throw Error(kValueError,
+                    "hbm_budget_bytes, max_active, sweeps_per_round, round_target_ms and max_rounds_per_wait "
+                    "must be positive");
     }
     device_check(qmc_ctx_create(device, &s->ctx), "qmc_ctx_create");
     device_check(qmc_batch_create(s->ctx, &s->batch), "qmc_batch_create");
@@ -438,6 +464,7 @@ int qiotk_server_stats(qiotk_server s, char** response) {
   j["peak_reserved_bytes"] = Json(s->peak_used);
   j["hbm_budget_bytes"] = Json(s->budget);
   j["sweeps_per_round"] = Json((int64_t)s->sweeps_per_round);
+  j["rounds_per_wait"] = Json((int64_t)s->rounds_per_wait);
   *response = dup(j.dump());
   return 0;
 }

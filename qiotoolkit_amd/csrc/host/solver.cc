@@ -145,7 +145,7 @@ bool Solver::serve_plan(qmc_batch b) {
   return batch_plan(b);
 }
 
-bool Solver::serve_after() {
+bool Solver::serve_done() {
   qmc_walkers w = walkers();
   if (!w) return true;
   int64_t left = 0;

@@ -47,10 +47,13 @@ class Solver {
   virtual int64_t run_replicas() const { return 1; }
   // Serving form: every round the server calls serve_plan() to add this
   // solve's next device work to the shared batch (false: nothing to add),
-  // runs the batch, then serve_after(), which does the host part of the
-  // step and returns true once the solve is complete (finalize() next).
+  // runs the batch, then serve_advance(), which enqueues what follows the
+  // round on the device without waiting.  Several rounds may be enqueued
+  // before the server waits for the device and asks serve_done() whether
+  // the solve is complete (finalize() next).
   virtual bool serve_plan(qmc_batch b);
-  virtual bool serve_after();
+  virtual void serve_advance() {}
+  virtual bool serve_done();
   // the solve's walker set on the device (null before init or when empty)
   virtual qmc_walkers walkers() const { return nullptr; }
   // Sweeps per device launch when fused into a batch (0: one persistent launch).
@@ -178,7 +181,8 @@ class PopulationAnnealing : public Solver {
   void run() override;
   void finalize() override;
   bool serve_plan(qmc_batch b) override;
-  bool serve_after() override;
+  void serve_advance() override;
+  bool serve_done() override;
   qmc_walkers walkers() const override;
   int64_t run_replicas() const override { return target_population_; }
   ~PopulationAnnealing() override;
@@ -186,6 +190,7 @@ class PopulationAnnealing : public Solver {
  private:
   void finish_step();
   int64_t step_ = 0, sweep_ = 0;
+  bool step_planned_ = false;  // serving: this round's sweeps are planned, its step not enqueued
   bool in_batch_ = false;
   double alpha_ = 2.0;
   int64_t target_population_ = 64;

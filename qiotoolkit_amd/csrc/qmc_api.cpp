@@ -305,9 +305,15 @@ struct qmc_batch_s {
   hipGraphExec_t graph_exec = nullptr;
   int64_t graph_key[3] = {0, 0, 0};  // sweeps per launch, launches, rebuild count
   int64_t rebuilds = 0;
+  // device time of the runs since the last wait: ev0 at the first run's
+  // start, ev1 at every run's end
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  bool span_open = false;
   ~qmc_batch_s() {
     if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
     if (graph) (void)hipGraphDestroy(graph);
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
   }
   // batched read-back
   DevBuf<ReadDesc> d_desc;
@@ -1386,6 +1392,10 @@ int qmc_batch_create(qmc_ctx ctx, qmc_batch* out) {
   if (!ctx || !out) return fail(QMC_INVALID_ARGUMENT, "null handle");
   auto* b = new qmc_batch_s();
   b->ctx = ctx;
+  if (hipEventCreate(&b->ev0) != hipSuccess || hipEventCreate(&b->ev1) != hipSuccess) {
+    delete b;
+    return fail(QMC_DEVICE_ERROR, "hipEventCreate failed");
+  }
   ctx->refs++;
   *out = b;
   return QMC_OK;
@@ -1597,9 +1607,10 @@ int qmc_batch_run(qmc_batch b, int32_t max_sweeps) {
   if (rc || (rc = batch_materialize(b))) return rc;
   qmc_ctx c = b->ctx;
   if ((rc = ensure_aux_streams(c))) return rc;
-  HIP_TRY(hipEventRecord(c->ev0, c->stream));
+  if (!b->span_open) HIP_TRY(hipEventRecord(b->ev0, c->stream));
+  b->span_open = true;
   if ((rc = enqueue_groups(b, max_sweeps))) return rc;
-  HIP_TRY(hipEventRecord(c->ev1, c->stream));
+  HIP_TRY(hipEventRecord(b->ev1, c->stream));
   return QMC_OK;
 }
 
@@ -1641,16 +1652,18 @@ int qmc_batch_run_chunked(qmc_batch b, int32_t sweeps_per_launch) {
     HIP_TRY(hipGraphInstantiate(&b->graph_exec, b->graph, nullptr, nullptr, 0));
     std::copy(key, key + 3, b->graph_key);
   }
-  HIP_TRY(hipEventRecord(c->ev0, c->stream));
+  if (!b->span_open) HIP_TRY(hipEventRecord(b->ev0, c->stream));
+  b->span_open = true;
   HIP_TRY(hipGraphLaunch(b->graph_exec, c->stream));
-  HIP_TRY(hipEventRecord(c->ev1, c->stream));
+  HIP_TRY(hipEventRecord(b->ev1, c->stream));
   return QMC_OK;
 }
 
 int qmc_batch_wait(qmc_batch b, double* ms) {
   if (!b) return fail(QMC_INVALID_ARGUMENT, "null handle");
   HIP_TRY(hipStreamSynchronize(b->ctx->stream));
-  HIP_TRY(hipEventElapsedTime(&b->last_ms, b->ctx->ev0, b->ctx->ev1));
+  if (b->span_open) HIP_TRY(hipEventElapsedTime(&b->last_ms, b->ev0, b->ev1));
+  b->span_open = false;
   if (ms) *ms = b->last_ms;
   for (qmc_walkers w : b->members)
     if (w->coop) {
