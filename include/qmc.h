@@ -34,14 +34,14 @@
 extern "C" {
 #endif
 
-/* status codes mirror utils::Error (utils/error_handling.h:25-52) */
+/* status codes: utils::Error values (utils/error_handling.h:25-52) */
 enum {
   QMC_OK = 0,
-  QMC_MEMORY_LIMITED = 1,       /* admission / allocation refused */
-  QMC_INVALID_ARGUMENT = 101,   /* ConfigurationException family */
-  QMC_DUPLICATED_VARIABLE = 105,
-  QMC_RUNTIME_ERROR = 201,      /* RuntimeException family */
-  QMC_DEVICE_ERROR = 202
+  QMC_MEMORY_LIMITED = 1,        /* MemoryLimited: admission / allocation refused */
+  QMC_DUPLICATED_VARIABLE = 101, /* DuplicatedVariable */
+  QMC_INVALID_ARGUMENT = 107,    /* ValueError */
+  QMC_RUNTIME_ERROR = 206,       /* UnmetPrecondition (e.g. a ladder barrier timed out) */
+  QMC_DEVICE_ERROR = 401         /* no counterpart: the device or the HIP runtime failed */
 };
 
 enum { QMC_ISING = 0, QMC_PUBO = 1 };

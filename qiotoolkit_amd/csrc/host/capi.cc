@@ -74,7 +74,9 @@ int guarded(char** response, F&& f) {
   } catch (const Error& e) {
     return report(e, response);
   } catch (const JsonError& e) {
-    return report(Error(kValueError, e.what()), response);
+    // malformed text is a parsing error, a value of the wrong kind a type error
+    const bool syntax = std::string(e.what()).rfind("JSON parse error", 0) == 0;
+    return report(Error(syntax ? kParsingError : kInvalidTypes, e.what()), response);
   } catch (const std::exception& e) {
     return report(Error(kRuntimeError, e.what()), response);
   }

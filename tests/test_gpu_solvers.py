@@ -82,7 +82,7 @@ def test_solver_errors(engine):
     doc = load("ising1.json")
     with pytest.raises(SolverError) as e:
         engine.solve("simulatedannealing.qiotoolkit", {"params": {"sweeps": 1}}, doc)
-    assert e.value.code == 103
+    assert e.value.code == 107  # utils::Error::ValueError
     with pytest.raises(SolverError):
         engine.solve("simulatedannealing.qiotoolkit", {}, doc)
 

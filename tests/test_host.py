@@ -78,7 +78,7 @@ def test_normalisation_errors():
     from qiotoolkit_amd import solver as S
     with pytest.raises(S.SolverError) as e:
         S.parse_model({"cost_function": {"type": "ising", "terms": [{"c": 1, "ids": [3, 3]}]}})
-    assert e.value.code == 105 and "Duplicated" in e.value.message
+    assert e.value.code == 101 and "Duplicated" in e.value.message  # utils::Error::DuplicatedVariable
     m = S.parse_model({"cost_function": {"type": "pubo", "terms": [{"c": 1, "ids": [3, 3, 4]}]}})
     assert m["term_vars"] == [0, 1]
     with pytest.raises(S.SolverError, match="greater than 0"):
@@ -138,3 +138,28 @@ def test_pt_plan_ladder():
     assert d["betas"] == [1.0 / x for x in sorted(t)]
     d = S.describe("paralleltempering.qiotoolkit", {"params": {"sweeps": 100}})
     assert d["betas"] == [4, 2, 1, 0.5, 0.1]  # default all_betas, descending
+
+
+def test_error_codes_are_the_references():
+    """Codes of utils::Error (utils/error_handling.h:25-52) for the
+    reference's own messages: DuplicatedVariable 101 (graph/graph.h:130),
+    MissingInput 102 (model/base_model.cc:8-14), ValueError 107
+    (model/ising.h:86-87), ParsingError 105 for malformed text."""
+    import ctypes as C
+    from qiotoolkit_amd import solver as S
+    cases = [
+        ({"cost_function": {"type": "ising", "terms": [{"c": 1, "ids": [3, 3]}]}}, 101),
+        ({"not_a_cost_function": {}}, 102),
+        ({"cost_function": {"type": "ising", "version": "9.9", "terms": [{"c": 1, "ids": [0, 1]}]}}, 107),
+    ]
+    for doc, code in cases:
+        with pytest.raises(S.SolverError) as e:
+            S.parse_model(doc)
+        assert e.value.code == code, (doc, e.value.code, e.value.message)
+    out = C.c_void_p()
+    rc = S.lib().qiotk_parse_model(b'{"cost_function": {"type": "ising", "terms": [', C.byref(out))
+    assert rc == 105
+    S.lib().qiotk_free(out)
+    with pytest.raises(S.SolverError) as e:
+        S.describe("simulatedannealing.qiotoolkit", {"params": {"sweeps": 1}})
+    assert e.value.code == 107
