@@ -42,6 +42,18 @@ int qiotk_model_create_terms(qiotk_engine e, int kind, int64_t n_terms, const in
 void qiotk_model_destroy(qiotk_model m);
 int qiotk_solve_model(qiotk_model m, const char* target, const char* params_json, char** response);
 
+/* Input files as Runner::configure reads them (app/runner.cc:215-260): a
+ * JSON document, or a folder of protobuf problem messages
+ * <folder>/<name>_0.pb, <name>_1.pb, ... (name = the folder's last path
+ * component; utils/proto_reader.h:38-145, utils/problem.proto:1-30), read
+ * until the next file is missing -- the first message gives the type and
+ * version, every message adds terms.  Errors: the reference's messages with
+ * FileReadError 301 (first file missing), MissingInput 102 (no cost
+ * function), ValueError 107 (type, version, no terms). */
+int qiotk_model_create_file(qiotk_engine e, const char* path, qiotk_model* out);
+int qiotk_solve_file(qiotk_engine e, const char* target, const char* params_json, const char* input_path,
+                     char** response);
+
 /* Many independent solves of one target on resident models, fused into
  * shared device launches (simulated annealing and parallel tempering):
  * models[k] with params_json[k].  responses[k] as for qiotk_solve; *device_ms
@@ -112,6 +124,8 @@ const char* qiotk_server_last_error(void);
 /* Device-free host logic, for tooling and tests. */
 /* graph::normalize_edges result as JSON: names, term CSR, const_cost, ... */
 int qiotk_parse_model(const char* input_json, char** response);
+/* the same for an input file (JSON document or protobuf problem folder) */
+int qiotk_parse_model_file(const char* path, char** response);
 /* Schedule::get_value at n points; *discretized = {"start","stop","count","values"} */
 int qiotk_schedule_eval(const char* schedule_json, int32_t n, const double* x, double* out,
                         char** discretized);

@@ -9,6 +9,7 @@
 
 #include "../../../include/qiotk.h"
 #include "model.h"
+#include "proto.h"
 #include "solver.h"
 
 using namespace qiotk;
@@ -189,6 +190,18 @@ int qiotk_model_create(qiotk_engine e, const char* input_json, qiotk_model* out)
   });
 }
 
+int qiotk_model_create_file(qiotk_engine e, const char* path, qiotk_model* out) {
+  return guarded(nullptr, [&] {
+    if (!path) throw Error(kMissingInput, "no input file");
+    auto m = std::make_unique<qiotk_model_s>();
+    m->e = e;
+    m->model = parse_model(read_input_file(path));
+    if (!m->model.empty()) m->dev = std::make_unique<DeviceModel>(e->ctx, m->model);
+    *out = m.release();
+    return 0;
+  });
+}
+
 int qiotk_model_create_terms(qiotk_engine e, int kind, int64_t n_terms, const int64_t* offsets,
                              const int32_t* ids, const double* coeffs, qiotk_model* out) {
   return guarded(nullptr, [&] {
@@ -225,6 +238,26 @@ int qiotk_solve(qiotk_engine e, const char* target, const char* params_json, con
     const double t0 = now_ms();
     qiotk_model m = nullptr;
     int rc = qiotk_model_create(e, input_json, &m);
+    if (rc) throw Error(rc, g_err);
+    std::unique_ptr<qiotk_model_s> hold(m);
+    auto s = create_solver(target);
+    s->set_model(&m->model, m->dev.get(), e->ctx);
+    s->configure(Json::parse(params_json));
+    const double t1 = now_ms();
+    Json r = run_one(*s);
+    r["benchmark"]["configure_ms"] = t1 - t0;
+    r["benchmark"]["end2end_time_ms"] = now_ms() - t0;
+    *response = dup(r.dump());
+    return 0;
+  });
+}
+
+int qiotk_solve_file(qiotk_engine e, const char* target, const char* params_json, const char* input_path,
+                     char** response) {
+  return guarded(response, [&] {
+    const double t0 = now_ms();
+    qiotk_model m = nullptr;
+    int rc = qiotk_model_create_file(e, input_path, &m);
     if (rc) throw Error(rc, g_err);
     std::unique_ptr<qiotk_model_s> hold(m);
     auto s = create_solver(target);
@@ -403,35 +436,6 @@ int qiotk_model_vars(qiotk_model m, int32_t* n_vars, int32_t* names) {
   });
 }
 
-int qiotk_parse_model(const char* input_json, char** response) {
-  return guarded(response, [&] {
-    Model m = parse_model(Json::parse(input_json));
-    Json j = Json::object();
-    j["kind"] = m.kind == QMC_ISING ? "ising" : "pubo";
-    j["n_vars"] = Json((int64_t)m.n_vars);
-    j["const_cost"] = m.const_cost;
-    Json names = Json::array(), off = Json::array(), vars = Json::array(), cs = Json::array();
-    for (int32_t n : m.names) names.push_back(Json((int64_t)n));
-    for (int64_t o : m.term_off) off.push_back(Json(o));
-    for (int32_t v : m.term_var) vars.push_back(Json((int64_t)v));
-    for (double c : m.coeff) cs.push_back(Json(c));
-    j["names"] = names;
-    j["term_offsets"] = off;
-    j["term_vars"] = vars;
-    j["coeffs"] = cs;
-    j["scale_factor"] = m.scale_factor();
-    Json ic = Json::array();
-    for (auto& kv : m.initial) {
-      Json e = Json::array();
-      e.push_back(Json((int64_t)kv.first));
-      e.push_back(Json((int64_t)kv.second));
-      ic.push_back(e);
-    }
-    j["initial"] = ic;
-    *response = dup(j.dump());
-    return 0;
-  });
-}
 
 int qiotk_schedule_eval(const char* schedule_json, int32_t n, const double* x, double* out,
                         char** discretized) {
@@ -452,6 +456,47 @@ int qiotk_schedule_eval(const char* schedule_json, int32_t n, const double* x, d
       j["values"] = vals;
       *discretized = dup(j.dump());
     }
+    return 0;
+  });
+}
+
+static Json describe_model(const Model& m) {
+  Json j = Json::object();
+  j["kind"] = m.kind == QMC_ISING ? "ising" : "pubo";
+  j["n_vars"] = Json((int64_t)m.n_vars);
+  j["const_cost"] = m.const_cost;
+  Json names = Json::array(), off = Json::array(), vars = Json::array(), cs = Json::array();
+  for (int32_t n : m.names) names.push_back(Json((int64_t)n));
+  for (int64_t o : m.term_off) off.push_back(Json(o));
+  for (int32_t v : m.term_var) vars.push_back(Json((int64_t)v));
+  for (double c : m.coeff) cs.push_back(Json(c));
+  j["names"] = names;
+  j["term_offsets"] = off;
+  j["term_vars"] = vars;
+  j["coeffs"] = cs;
+  j["scale_factor"] = m.scale_factor();
+  Json ic = Json::array();
+  for (auto& kv : m.initial) {
+    Json e = Json::array();
+    e.push_back(Json((int64_t)kv.first));
+    e.push_back(Json((int64_t)kv.second));
+    ic.push_back(e);
+  }
+  j["initial"] = ic;
+  return j;
+}
+
+int qiotk_parse_model(const char* input_json, char** response) {
+  return guarded(response, [&] {
+    *response = dup(describe_model(parse_model(Json::parse(input_json))).dump());
+    return 0;
+  });
+}
+
+int qiotk_parse_model_file(const char* path, char** response) {
+  return guarded(response, [&] {
+    if (!path) throw Error(kMissingInput, "no input file");
+    *response = dup(describe_model(parse_model(read_input_file(path))).dump());
     return 0;
   });
 }

@@ -22,7 +22,8 @@ EXPORTS = ["qiotk_engine_create", "qiotk_engine_destroy", "qiotk_solve", "qiotk_
            "qiotk_last_error", "qiotk_parse_model", "qiotk_schedule_eval", "qiotk_describe",
            "qiotk_model_create_terms", "qiotk_solve_batch_arrays", "qiotk_model_vars",
            "qiotk_tune_batch", "qiotk_server_create", "qiotk_server_submit", "qiotk_server_submit_json",
-           "qiotk_server_wait", "qiotk_server_stats", "qiotk_server_destroy", "qiotk_server_last_error"]
+           "qiotk_server_wait", "qiotk_server_stats", "qiotk_server_destroy", "qiotk_server_last_error",
+           "qiotk_model_create_file", "qiotk_solve_file", "qiotk_parse_model_file"]
 
 
 def _check_rc(rc):
@@ -63,6 +64,9 @@ def lib():
     L.qiotk_free.restype = None
     L.qiotk_last_error.restype = C.c_char_p
     L.qiotk_parse_model.argtypes = [C.c_char_p, P(vp)]
+    L.qiotk_parse_model_file.argtypes = [C.c_char_p, P(vp)]
+    L.qiotk_model_create_file.argtypes = [vp, C.c_char_p, P(vp)]
+    L.qiotk_solve_file.argtypes = [vp, C.c_char_p, C.c_char_p, C.c_char_p, P(vp)]
     L.qiotk_schedule_eval.argtypes = [C.c_char_p, C.c_int32, P(C.c_double), P(C.c_double), P(vp)]
     L.qiotk_describe.argtypes = [C.c_char_p, C.c_char_p, P(vp)]
     L.qiotk_model_create_terms.argtypes = [vp, C.c_int, C.c_int64, P(C.c_int64), P(C.c_int32),
@@ -127,6 +131,12 @@ class Engine:
     def solve(self, target, params, input_data):
         out = C.c_void_p()
         lib().qiotk_solve(self.h, target.encode(), _enc(params), _enc(input_data), C.byref(out))
+        return _raise_from(_take(out))
+
+    def solve_file(self, target, params, path):
+        """Solve the input file `path`: a JSON document or a protobuf problem folder."""
+        out = C.c_void_p()
+        lib().qiotk_solve_file(self.h, target.encode(), _enc(params), os.fspath(path).encode(), C.byref(out))
         return _raise_from(_take(out))
 
     def model(self, input_data):
@@ -295,6 +305,14 @@ def parse_model(input_data):
     """graph::normalize_edges of a cost function (no device needed)."""
     out = C.c_void_p()
     lib().qiotk_parse_model(_enc(input_data), C.byref(out))
+    return _raise_from(_take(out))
+
+
+def parse_model_file(path):
+    """The normalised model of an input file: a JSON document or a protobuf
+    problem folder (no device needed)."""
+    out = C.c_void_p()
+    lib().qiotk_parse_model_file(os.fspath(path).encode(), C.byref(out))
     return _raise_from(_take(out))
 
 

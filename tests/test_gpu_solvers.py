@@ -210,3 +210,18 @@ def test_batch_arrays_match_json_and_single_solves(engine, target):
         doc = {"cost_function": {"type": "ising", "terms": terms}}
         assert oracle_cost(doc, want) == cost[k]
     assert tm["device_ms"] > 0
+
+
+def test_solve_from_protobuf_folder(engine):
+    """The reference's ring fixture (data/models_input_problem2_pb) solved
+    from its protobuf folder and from the equivalent JSON document."""
+    folder = os.path.join(GOLD, "proto", "models_input_problem2_pb")
+    c = [-9.5 if i == 9 else (-(i + 1.0) if i % 2 == 0 else i + 1.0) for i in range(10)]
+    doc = {"cost_function": {"type": "ising", "version": "1.0",
+                             "terms": [{"c": c[i], "ids": [i, (i + 1) % 10]} for i in range(10)]}}
+    p = {"params": {"beta_start": 0.1, "beta_stop": 3.0, "sweeps": 50, "restarts": 8, "seed": 3}}
+    for target in ("simulatedannealing.qiotoolkit", "populationannealing.cpu"):
+        a = engine.solve_file(target, p, folder)
+        b = engine.solve(target, p, doc)
+        assert a["solutions"] == b["solutions"]
+        assert oracle_cost(doc, a["solutions"]["configuration"]) == a["solutions"]["cost"]
