@@ -81,6 +81,16 @@ int qmc_sync(qmc_ctx ctx);
 int qmc_problem_create(qmc_ctx ctx, int kind, int32_t n_vars, int64_t n_terms,
                        const int64_t* term_offsets, const int32_t* term_vars,
                        const double* coeffs, int32_t lanes, qmc_problem* out);
+/* The same with a choice of visit order: QMC_VISIT_COLOR_CLASSES (what
+ * qmc_problem_create does: greedy colouring, fewest classes) or
+ * QMC_VISIT_INDEX_LEVELS (class of variable v = 1 + the largest class of a
+ * term-sharing neighbour u < v: sweeps make the decisions of an index-order
+ * sweep, markov/walker.h:144-153, and end in its states; the lowest state is
+ * tracked along the class order). */
+enum { QMC_VISIT_COLOR_CLASSES = 0, QMC_VISIT_INDEX_LEVELS = 1 };
+int qmc_problem_create_visit(qmc_ctx ctx, int kind, int32_t n_vars, int64_t n_terms,
+                             const int64_t* term_offsets, const int32_t* term_vars, const double* coeffs,
+                             int32_t lanes, int32_t visit, qmc_problem* out);
 void qmc_problem_destroy(qmc_problem p);
 int qmc_problem_get_info(qmc_problem p, qmc_problem_info* info);
 /* Sweep visit order (variable ids) for `lanes` positions per class round
@@ -111,6 +121,13 @@ int qmc_walkers_set_betas(qmc_walkers w, const double* betas /* n_replicas */);
  * the coloured order of qmc_problem_order. */
 int qmc_walkers_sweep(qmc_walkers w, int64_t sweep0, int32_t n_sweeps, int beta_mode,
                       const double* schedule, int ordered);
+/* Every later sweep of this walker set -- qmc_walkers_sweep and batch
+ * launches -- visits variables in index order (ordered != 0; at most 64
+ * replicas per group, no replica-exchange ladder) or in colour-class order
+ * (0, the default).  The index order is the reference's sweep
+ * (markov/walker.h:144-153) step for step; the colour classes run their
+ * variables in parallel. */
+int qmc_walkers_set_sweep_order(qmc_walkers w, int32_t ordered);
 /* Parallel tempering over ladders of `ladder` consecutive replicas with
  * betas ladder_betas[0..ladder) (descending).  Runs n_steps steps of
  * sweeps_per_replica sweeps, exchanging pairs (i,i+1), i = step mod 2, after
@@ -152,6 +169,26 @@ int qmc_walkers_pa_begin(qmc_walkers w, int32_t population, double alpha, double
  * rule (R / rho < alpha or zero variance: fresh citizens, population topped
  * up to `population`). */
 int qmc_walkers_pa_step(qmc_walkers w);
+/* Stepping size and resampling method, set after qmc_walkers_pa_begin and
+ * before the first step (population_annealing.h:420-457, 540-578,
+ * 649-751).  strategy: QMC_PA_SCHEDULE (default: beta_table differences),
+ * QMC_PA_FRICTION_TENSOR (dbeta = constant / sqrt(zeta), zeta = sigma *
+ * sweeps_per_step / (ln sigma - ln |gamma|) from the costs before and after
+ * the step's sweeps, sigma = sqrt(var_before var_after), gamma their
+ * covariance), QMC_PA_ENERGY_VARIANCE (dbeta = constant / sqrt(sigma)) or
+ * QMC_PA_CONSTANT_CULLING (dbeta = find_delta_beta of the culling estimate
+ * with culling fraction `constant`); the adaptive ones need a beta_table of
+ * {0, 0}.  constant_population != 0: offspring floor(population * w / sum w)
+ * plus a fill to the live size drawn without replacement in proportion to
+ * the residuals (utils::RandomSelector; pick j's uniform from the counter
+ * (j, step, 1, 5)) instead of Poisson offspring. */
+enum {
+  QMC_PA_SCHEDULE = 0,
+  QMC_PA_FRICTION_TENSOR = 1,
+  QMC_PA_ENERGY_VARIANCE = 2,
+  QMC_PA_CONSTANT_CULLING = 3
+};
+int qmc_walkers_pa_strategy(qmc_walkers w, int32_t strategy, double constant, int32_t constant_population);
 typedef struct {
   int32_t population;  /* live citizens */
   int32_t restarts;
@@ -162,6 +199,7 @@ typedef struct {
   double best;         /* lowest cost any citizen reached */
   double beta;         /* current beta */
   double rho;          /* family statistic of the last resampling */
+  double delta_beta;   /* stepping size of the last step */
 } qmc_pa_info;
 /* Waits for the stream; best_state (n_vars bytes, may be NULL) receives the
  * state of `best`. */

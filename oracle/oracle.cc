@@ -442,6 +442,186 @@ void copy_spins(const std::vector<uint8_t>& v, uint8_t* out) {
   if (out) std::memcpy(out, v.data(), v.size());
 }
 
+// ---------------------------------------------------------------------------
+// utils::RandomSelector (utils/random_selector.h:9-47, random_selector.cc):
+// ids drawn with probability proportional to their weights, without
+// replacement.  Binary indexed tree over insertion positions: node q >= 1
+// holds the weight of positions [q - lowbit(q), q); the root is node P, the
+// power of two the tree has grown to (nodes 0..P).  Growing copies the old
+// root into the new one (random_selector.cc:52-65).
+struct Selector {
+  std::vector<size_t> ids;
+  std::vector<double> node{0.0, 0.0};
+  double total = 0.0;
+
+  size_t root() const { return node.size() - 1; }
+  // weight of positions below `position` (get_cumulative_weight, :22-35)
+  double below(size_t position) const {
+    size_t at = 0;
+    double w = 0.0;
+    for (size_t k = root(); k > 0; k >>= 1)
+      if (position & k) {
+        at |= k;
+        w += node[at];
+      }
+    return w;
+  }
+  // add_weight (:37-50)
+  void add(size_t position, double w) {
+    for (size_t k = 1; k < node.size(); k <<= 1) {
+      if (position & k) position ^= k;
+      else node[position + k] += w;
+    }
+  }
+  void insert(size_t id, double w) {
+    const size_t position = ids.size();
+    ids.push_back(id);
+    total += w;
+    if (position + 1 >= node.size()) {
+      const double all = node.back();
+      node.resize(2 * node.size() - 1);
+      node.back() = all;
+    }
+    add(position, w);
+  }
+  // select_position (:67-94); a descent past the root (only when rounding
+  // leaves the root below u * total) stops at the root, where the reference
+  // would read past its array
+  size_t select_position(double u) const {
+    if (ids.size() == 1) return 0;
+    double x = u * total;
+    size_t position = 0;
+    for (size_t k = root(); k > 0; k >>= 1)
+      if (position + k <= root() && x >= node[position + k]) {
+        position += k;
+        x -= node[position];
+      }
+    return position >= ids.size() ? ids.size() - 1 : position;
+  }
+  // select_and_remove with remove_position (:96-120)
+  size_t select_and_remove(double u) {
+    const size_t position = select_position(u);
+    const size_t id = ids[position];
+    ids[position] = ids.back();
+    ids.pop_back();
+    const size_t last = ids.size();
+    const double before = below(position + 1) - below(position);
+    const double after = below(last + 1) - below(last);
+    total -= before;
+    add(position, after - before);
+    add(last, -after);
+    return id;
+  }
+};
+
+// CullingEstimator (solver/population_annealing.h:175-212) over costs
+// already shifted to >= 0: expected culled fraction at dbeta minus epsilon0.
+double culling_estimate(const std::vector<double>& shifted, double epsilon0, size_t target, double scale,
+                        double db) {
+  const size_t R = shifted.size();
+  std::vector<double> w(R);
+  double q = 0.0;
+  for (size_t i = 0; i < R; ++i) {
+    w[i] = std::exp(-db * shifted[i] * scale);
+    q += w[i];
+  }
+  const double rrq = (double)target / q;
+  double culled = 0.0;
+  for (size_t i = 0; i < R; ++i) {
+    const double v = 1.0 - w[i] * rrq;
+    if (v > 0) culled += v;
+  }
+  return culled / (double)R - epsilon0;
+}
+
+// find_delta_beta (population_annealing.h:218-260): widen [lower, 2^e) from
+// e = -20 until the estimate changes sign (at most e = 19), then refine the
+// lower end by halving steps down to 2^-20.
+double find_delta_beta(const std::vector<double>& shifted, double epsilon0, size_t target, double scale) {
+  double lower = 0.0, lower_value = culling_estimate(shifted, epsilon0, target, scale, 0.0);
+  int e;
+  for (e = -20; e < 20; ++e) {
+    const double upper = std::ldexp(1.0, e);
+    const double v = culling_estimate(shifted, epsilon0, target, scale, upper);
+    if ((lower_value < 0) != (v < 0)) break;
+    lower = upper;
+    lower_value = v;
+  }
+  for (e -= 2; e >= -20; --e) {
+    const double upper = lower + std::ldexp(1.0, e);
+    const double v = culling_estimate(shifted, epsilon0, target, scale, upper);
+    if ((lower_value < 0) == (v < 0)) {
+      lower = upper;
+      lower_value = v;
+    }
+  }
+  return lower;
+}
+
+// The adaptive stepping sizes (population_annealing.h:347-373, 420-457)
+// from the costs before and after a step's sweeps.  strategy 1 friction
+// tensor, 2 energy variance, 3 constant culling; c is the strategy's
+// constant.
+double adaptive_delta_beta(int strategy, double c, const double* before, const double* after, size_t R,
+                           size_t target, double scale, int sweeps) {
+  const double Rd = (double)R;
+  double avg_b = 0, avg_a = 0, mn = std::numeric_limits<double>::max();
+  for (size_t i = 0; i < R; ++i) {
+    avg_b += before[i];
+    avg_a += after[i];
+    mn = after[i] < mn ? after[i] : mn;
+  }
+  avg_b /= Rd;
+  avg_a /= Rd;
+  if (strategy == 3) {
+    std::vector<double> shifted(after, after + R);
+    for (double& x : shifted) x -= mn;
+    return find_delta_beta(shifted, c, target, scale);
+  }
+  double var_b = 0, var_a = 0, gamma = 0;
+  for (size_t i = 0; i < R; ++i) {
+    const double db = before[i] - avg_b, da = after[i] - avg_a;
+    var_b += db * db;
+    var_a += da * da;
+    gamma += db * da;
+  }
+  var_b /= Rd;
+  var_a /= Rd;
+  gamma /= Rd;
+  const double sigma = std::sqrt(var_b * var_a);
+  if (strategy == 1) {
+    double zeta = 0;
+    if (sigma > 0 && gamma != 0) {
+      const double delta = std::log(sigma) - std::log(std::abs(gamma));
+      if (std::abs(delta) > 1e-11) zeta = sigma * ((double)sweeps / delta);
+    }
+    return zeta > 0 ? c / std::sqrt(zeta) : 0.0;
+  }
+  return sigma > 0 ? c / std::sqrt(sigma) : 0.0;
+}
+
+// Offspring counts of the constant-population branch (population_annealing.h
+// :540-578): floor(target w_i / sum w), then picks without replacement in
+// proportion to the remainders until the counts sum to R; `uniform(j)` is
+// pick j's uniform.
+template <class U>
+void residual_counts(const std::vector<double>& weight, double total, size_t target, std::vector<size_t>& count,
+                     U uniform) {
+  const size_t R = weight.size();
+  count.assign(R, 0);
+  std::vector<double> res(R);
+  size_t n = 0;
+  for (size_t i = 0; i < R; ++i) {
+    const double rw = (double)target * weight[i] / total;
+    count[i] = (size_t)std::floor(rw);
+    res[i] = rw - (double)count[i];
+    n += count[i];
+  }
+  Selector sel;
+  for (size_t i = 0; i < R; ++i) sel.insert(i, res[i]);
+  for (size_t j = 0; n < R; ++j, ++n) count[sel.select_and_remove(uniform(j))] += 1;
+}
+
 }  // namespace
 
 // ===========================================================================
@@ -792,20 +972,48 @@ int orc_pt(void* pp, const orc_opts* o, const double* values, int n_values, int 
 }
 
 // PopulationAnnealing (population_annealing.h:102-616) with the linear or
-// geometric beta schedule and Poisson resampling (Population::resample,
+// geometric beta schedule or an adaptive stepping size (strategy 1 friction
+// tensor, 2 energy variance, 3 constant culling, with its constant), and
+// Poisson or constant-population resampling (Population::resample,
 // population.h:171-218).  Citizens use per-thread RNGs under
 // `omp parallel for schedule(static,1)`: citizen i draws from thread i % T,
 // threads visiting their citizens in increasing order, which this sequential
 // loop reproduces exactly for T = opts.threads (default 1).
+// utils::RandomSelector: n weights inserted in order, then k picks without
+// replacement with the given uniforms; out[j] = pick j's id (k <= n).
+int orc_selector_draw(int n, const double* weights, int k, const double* uniforms, int64_t* out) {
+  if (n < 1 || k < 0 || k > n) return -1;
+  Selector sel;
+  for (int i = 0; i < n; ++i) sel.insert((size_t)i, weights[i]);
+  for (int j = 0; j < k; ++j) out[j] = (int64_t)sel.select_and_remove(uniforms[j]);
+  return 0;
+}
+
+// The adaptive stepping size of one step (strategy 1 friction tensor, 2
+// energy variance, 3 constant culling) from the costs before and after its
+// sweeps.
+double orc_adaptive_delta_beta(int strategy, double c, int R, const double* before, const double* after,
+                               int target, double scale, int sweeps) {
+  return adaptive_delta_beta(strategy, c, before, after, (size_t)R, (size_t)target, scale, sweeps);
+}
+
+// The culling estimate over costs already shifted to >= 0.
+double orc_culling_estimate(int R, const double* shifted, double epsilon0, int target, double scale,
+                            double db) {
+  return culling_estimate(std::vector<double>(shifted, shifted + R), epsilon0, (size_t)target, scale, db);
+}
+
 void orc_poisson_counter(double lam, uint32_t seed, uint32_t step, int n, uint32_t* out) {
   for (int i = 0; i < n; ++i) out[i] = poisson_counter(lam, (uint32_t)i, step, seed);
 }
 
 // One population annealing step in counter mode, after the step's sweeps:
 // the decisions of orc_pa's step loop below (population_annealing.h:262-488,
-// resample :490-616, Poisson branch) for the costs and lowest costs the
-// sweeps left, with offspring counts from poisson_counter(citizen, step) and
-// the pick of an emptied population from Philox (0, step, 0, DOM_PICK).
+// resample :490-616) for the costs and lowest costs the sweeps left (and,
+// for strategies 1 and 2, the costs before them), with offspring counts from
+// poisson_counter(citizen, step) or, with constant_population, residual picks
+// whose uniforms come from Philox (j, step, 1, DOM_PICK), and the pick of an
+// emptied population from Philox (0, step, 0, DOM_PICK).
 // Offspring of citizen i fill consecutive slots in citizen order (copies of
 // one citizen are identical, so this is orc_pa's order); after a restart a
 // population below the target is topped up with copies of its last citizen
@@ -816,8 +1024,10 @@ void orc_poisson_counter(double lam, uint32_t seed, uint32_t step, int n, uint32
 int orc_pa_step_counter(int R, const double* cost, const double* lowest, const int32_t* family,
                         int64_t step, int64_t base_step, const double* beta_table, int n_table, int target,
                         int cap, double alpha, double scale, uint32_t seed, int have_best, double best,
-                        int32_t* src, int32_t* fam_out, int32_t* iout, double* dout) {
-  if (R < 1 || n_table < 2 || cap < target) return -1;
+                        int32_t* src, int32_t* fam_out, int32_t* iout, double* dout, const double* cost_before,
+                        int strategy, double strategy_const, int constant_population, int sweeps_per_step) {
+  if (R < 1 || n_table < 2 || cap < target || strategy < 0 || strategy > 3) return -1;
+  if ((strategy == 1 || strategy == 2) && !cost_before) return -1;
   // update_lowest_cost over the citizens (first minimum)
   int barg = 0;
   for (int i = 1; i < R; ++i)
@@ -833,7 +1043,9 @@ int orc_pa_step_counter(int R, const double* cost, const double* lowest, const i
   for (int i = 0; i < R; ++i) var += (cost[i] - avg) * (cost[i] - avg);
   var /= (double)R;
   const int64_t k = step - base_step, last = n_table - 1;
-  const double db = beta_table[std::min(k + 1, last)] - beta_table[std::min(k, last)];
+  const double db = strategy == 0 ? beta_table[std::min(k + 1, last)] - beta_table[std::min(k, last)]
+                                  : adaptive_delta_beta(strategy, strategy_const, cost_before, cost, (size_t)R,
+                                                        (size_t)target, scale, sweeps_per_step);
   double rho = 1.0;
   int Rn = R;
   bool moved = false;
@@ -845,13 +1057,24 @@ int orc_pa_step_counter(int R, const double* cost, const double* lowest, const i
       w[i] = std::exp(-db * (cost[i] - mn) * scale);
       total += w[i];
     }
-    const double rrq = (double)target / total;
+    std::vector<size_t> count(R);
+    if (constant_population) {
+      residual_counts(w, total, (size_t)target, count, [&](size_t j) {
+        uint32_t ctr[4] = {(uint32_t)j, (uint32_t)step, 1u, DOM_PICK}, key[2] = {seed, 0u}, wd[4];
+        philox4x32_10(ctr, key, wd);
+        return canonical2(wd[0], wd[1]);
+      });
+    } else {
+      const double rrq = (double)target / total;
+      for (int i = 0; i < R; ++i) {
+        const double mean = rrq * w[i];
+        count[i] = mean > 0.0 ? poisson_counter(mean, (uint32_t)i, (uint32_t)step, seed) : 0;
+      }
+    }
     std::vector<int32_t> s2;
     std::vector<int32_t> f2;
     for (int i = 0; i < R; ++i) {
-      const double mean = rrq * w[i];
-      const uint32_t c = mean > 0.0 ? poisson_counter(mean, (uint32_t)i, (uint32_t)step, seed) : 0;
-      for (uint32_t j = 0; j < c; ++j) {
+      for (size_t j = 0; j < count[i]; ++j) {
         s2.push_back(i);
         f2.push_back(family[i]);
       }
@@ -902,10 +1125,14 @@ int orc_pa_step_counter(int R, const double* cost, const double* lowest, const i
 int orc_pa(void* pp, const orc_opts* o, int geometric, double beta_start, double beta_stop,
            int64_t step_limit, int population, int sweeps_per_replica, double alpha,
            int constant_population, double* best_cost, uint8_t* best_state,
-           int64_t* final_population, int64_t* restarts_out) {
+           int64_t* final_population, int64_t* restarts_out, int strategy, double strategy_const) {
   auto* p = static_cast<Problem*>(pp);
   if (o->rng_mode != ORC_RNG_REFERENCE || population <= 0 || step_limit <= 1) return -1;
-  if (constant_population) return -2;  // RandomSelector residual resampling: not restated
+  if (strategy < 0 || strategy > 3 || (strategy > 0 && !(strategy_const > 0))) return -1;
+  if (strategy > 0) {  // adaptive: beta starts at 0 (:733-735)
+    beta_start = beta_stop = 0.0;
+    geometric = 0;
+  }
   RunCtx ctx = make_ctx(p, o);
   const int T = o->threads > 0 ? o->threads : 1;
   Sched beta;
@@ -982,33 +1209,43 @@ int orc_pa(void* pp, const orc_opts* o, int geometric, double beta_start, double
       var_after += d * d;
     }
     var_after /= Rd;
+    const double delta_beta =
+        strategy == 0
+            ? beta.value((double)(step + 1 - base_step)) - beta.value((double)(step - base_step))
+            : adaptive_delta_beta(strategy, strategy_const, before.data(), after.data(), R, target, scale,
+                                  sweeps_per_replica);
     for (size_t i = 0; i < R; ++i) after[i] -= min_cost;
-    double delta_beta = beta.value((double)(step + 1 - base_step)) - beta.value((double)(step - base_step));
     double rho_t = 1.0;
     if (delta_beta > 0) {
-      // resample (:490-616), Poisson branch
+      // resample (:490-616)
       std::vector<double> weight(R);
       double total = 0;
       for (size_t i = 0; i < R; ++i) {
         weight[i] = std::exp(-delta_beta * after[i] * scale);
         total += weight[i];
       }
-      const double RRQ = (double)target / total;
       std::vector<size_t> count(R, 1);
       uint64_t current = R;
-      for (size_t i = 0; i < R; ++i) {
-        size_t nc = (RRQ * weight[i]) > 0.0 ? (size_t)std::floor((double)master.poisson(RRQ * weight[i])) : 0;
-        if (nc == 0) {
-          current -= count[i];
-          count[i] = 0;
-        } else if (nc > 1) {
-          current += nc - 1;
-          count[i] += nc - 1;
+      if (constant_population) {
+        residual_counts(weight, total, target, count, [&](size_t) { return master.uniform(); });
+        current = 0;
+        for (size_t c : count) current += c;
+      } else {
+        const double RRQ = (double)target / total;
+        for (size_t i = 0; i < R; ++i) {
+          size_t nc = (RRQ * weight[i]) > 0.0 ? (size_t)std::floor((double)master.poisson(RRQ * weight[i])) : 0;
+          if (nc == 0) {
+            current -= count[i];
+            count[i] = 0;
+          } else if (nc > 1) {
+            current += nc - 1;
+            count[i] += nc - 1;
+          }
         }
-      }
-      if (current == 0) {
-        size_t i = (size_t)std::floor(master.uniform() * (double)R);
-        count[i] += 1;
+        if (current == 0) {
+          size_t i = (size_t)std::floor(master.uniform() * (double)R);
+          count[i] += 1;
+        }
       }
       // Population::resample: for each surviving citizen, (count-1) copies
       // then the original.

@@ -107,11 +107,17 @@ void orc_poisson_counter(double lam, uint32_t seed, uint32_t step, int n, uint32
 int orc_pa_step_counter(int R, const double* cost, const double* lowest, const int32_t* family,
                         int64_t step, int64_t base_step, const double* beta_table, int n_table, int target,
                         int cap, double alpha, double scale, uint32_t seed, int have_best, double best,
-                        int32_t* src, int32_t* fam_out, int32_t* iout, double* dout);
+                        int32_t* src, int32_t* fam_out, int32_t* iout, double* dout, const double* cost_before,
+                        int strategy, double strategy_const, int constant_population, int sweeps_per_step);
+int orc_selector_draw(int n, const double* weights, int k, const double* uniforms, int64_t* out);
+double orc_adaptive_delta_beta(int strategy, double c, int R, const double* before, const double* after,
+                               int target, double scale, int sweeps);
+double orc_culling_estimate(int R, const double* shifted, double epsilon0, int target, double scale,
+                            double db);
 int orc_pa(void* p, const orc_opts* o, int geometric, double beta_start, double beta_stop,
            int64_t step_limit, int population, int sweeps_per_replica, double alpha,
            int constant_population, double* best_cost, uint8_t* best_state,
-           int64_t* final_population, int64_t* restarts);
+           int64_t* final_population, int64_t* restarts, int strategy, double strategy_const);
 
 #ifdef __cplusplus
 }

@@ -55,7 +55,8 @@ def lib(variant=""):
     L.orc_pa_step_counter.argtypes = [C.c_int, P(C.c_double), P(C.c_double), P(C.c_int32), C.c_int64,
                                       C.c_int64, P(C.c_double), C.c_int, C.c_int, C.c_int, C.c_double,
                                       C.c_double, C.c_uint32, C.c_int, C.c_double, P(C.c_int32),
-                                      P(C.c_int32), P(C.c_int32), P(C.c_double)]
+                                      P(C.c_int32), P(C.c_int32), P(C.c_double), P(C.c_double), C.c_int,
+                                      C.c_double, C.c_int, C.c_int]
     L.orc_philox4x32_10.argtypes = [P(C.c_uint32), P(C.c_uint32), P(C.c_uint32)]
     L.orc_uniform_ln.argtypes = [P(C.c_double)]
     L.orc_accept.argtypes = [C.c_double, C.c_double, P(C.c_uint32), P(C.c_int)]
@@ -89,7 +90,13 @@ def lib(variant=""):
                          P(C.c_double), P(C.c_int64)]
     L.orc_pa.argtypes = [C.c_void_p, P(Opts), C.c_int, C.c_double, C.c_double, C.c_int64, C.c_int,
                          C.c_int, C.c_double, C.c_int, P(C.c_double), P(C.c_uint8), P(C.c_int64),
-                         P(C.c_int64)]
+                         P(C.c_int64), C.c_int, C.c_double]
+    L.orc_selector_draw.argtypes = [C.c_int, P(C.c_double), C.c_int, P(C.c_double), P(C.c_int64)]
+    L.orc_adaptive_delta_beta.argtypes = [C.c_int, C.c_double, C.c_int, P(C.c_double), P(C.c_double),
+                                          C.c_int, C.c_double, C.c_int]
+    L.orc_adaptive_delta_beta.restype = C.c_double
+    L.orc_culling_estimate.argtypes = [C.c_int, P(C.c_double), C.c_double, C.c_int, C.c_double, C.c_double]
+    L.orc_culling_estimate.restype = C.c_double
     _libs[variant] = L
     return L
 
@@ -229,13 +236,16 @@ class Problem:
                     final_states=fs, swap_rate=sr, steps=steps.value)
 
     def pa(self, opts, beta_start, beta_stop, sweeps, population, sweeps_per_replica=1, alpha=2.0,
-           geometric=False, constant_population=False):
+           geometric=False, constant_population=False, strategy=0, strategy_const=0.0):
+        """strategy: 0 schedule, 1 friction tensor, 2 energy variance, 3 constant
+        culling (strategy_const = its constant)."""
         bc = C.c_double()
         bs = np.zeros(self.n, dtype=np.uint8)
         fp, nr = C.c_int64(), C.c_int64()
         rc = self.L.orc_pa(self.h, C.byref(opts), int(geometric), beta_start, beta_stop, sweeps,
                            population, sweeps_per_replica, alpha, int(constant_population),
-                           C.byref(bc), _p(bs, C.c_uint8), C.byref(fp), C.byref(nr))
+                           C.byref(bc), _p(bs, C.c_uint8), C.byref(fp), C.byref(nr), int(strategy),
+                           float(strategy_const))
         if rc != 0:
             raise ValueError("orc_pa rejected the parameters (rc=%d)" % rc)
         return dict(best=bc.value, best_state=bs, final_population=fp.value, restarts=nr.value)
@@ -261,8 +271,10 @@ def poisson_counter(lam, seed, step, n):
 
 
 def pa_step_counter(cost, lowest, family, step, base_step, beta_table, target, cap, alpha, scale, seed,
-                    have_best, best):
+                    have_best, best, cost_before=None, strategy=0, strategy_const=0.0,
+                    constant_population=False, sweeps_per_step=1):
     """One counter-mode population annealing step's decisions (orc_pa_step_counter)."""
+    cb = None if cost_before is None else np.ascontiguousarray(cost_before, dtype=np.float64)
     cost = np.ascontiguousarray(cost, dtype=np.float64)
     lowest = np.ascontiguousarray(lowest, dtype=np.float64)
     family = np.ascontiguousarray(family, dtype=np.int32)
@@ -275,13 +287,37 @@ def pa_step_counter(cost, lowest, family, step, base_step, beta_table, target, c
     rc = lib().orc_pa_step_counter(R, _p(cost, C.c_double), _p(lowest, C.c_double), _p(family, C.c_int32),
                                    step, base_step, _p(table, C.c_double), len(table), target, cap, alpha,
                                    scale, seed & 0xFFFFFFFF, int(have_best), best, _p(src, C.c_int32),
-                                   _p(fam, C.c_int32), _p(iout, C.c_int32), _p(dout, C.c_double))
+                                   _p(fam, C.c_int32), _p(iout, C.c_int32), _p(dout, C.c_double),
+                                   None if cb is None else _p(cb, C.c_double), int(strategy),
+                                   float(strategy_const), int(constant_population), int(sweeps_per_step))
     if rc != 0:
         raise ValueError("orc_pa_step_counter failed (rc=%d)" % rc)
     Rf = int(iout[0])
     return dict(src=src[:Rf], family=fam[:Rf], R=Rf, R_resampled=int(iout[1]), moved=bool(iout[2]),
                 restart=bool(iout[3]), best_src=int(iout[4]), rho=float(dout[0]), dbeta=float(dout[1]),
                 best=float(dout[2]))
+
+
+def selector_draw(weights, uniforms):
+    """utils::RandomSelector picks without replacement (orc_selector_draw)."""
+    w = np.ascontiguousarray(weights, dtype=np.float64)
+    u = np.ascontiguousarray(uniforms, dtype=np.float64)
+    out = np.zeros(len(u), dtype=np.int64)
+    if lib().orc_selector_draw(len(w), _p(w, C.c_double), len(u), _p(u, C.c_double), _p(out, C.c_int64)) != 0:
+        raise ValueError("orc_selector_draw rejected the sizes")
+    return out
+
+
+def adaptive_delta_beta(strategy, c, before, after, target, scale=1.0, sweeps=1):
+    b = np.ascontiguousarray(before, dtype=np.float64)
+    a = np.ascontiguousarray(after, dtype=np.float64)
+    return lib().orc_adaptive_delta_beta(strategy, c, len(a), _p(b, C.c_double), _p(a, C.c_double), target,
+                                         scale, sweeps)
+
+
+def culling_estimate(shifted, epsilon0, target, scale, db):
+    x = np.ascontiguousarray(shifted, dtype=np.float64)
+    return lib().orc_culling_estimate(len(x), _p(x, C.c_double), epsilon0, target, scale, db)
 
 
 def philox(ctr, key):

@@ -27,7 +27,8 @@ EXPORTS = [
     "qmc_batch_destroy", "qmc_batch_add", "qmc_batch_remove", "qmc_batch_clear", "qmc_batch_run",
     "qmc_batch_wait", "qmc_walkers_read_replica", "qmc_batch_read", "qmc_batch_run_chunked", "qmc_walkers_resample", "qmc_walkers_count", "qmc_walkers_set_pt_mode",
     "qmc_walkers_set_lanes", "qmc_walkers_get_lanes", "qmc_walkers_pa_begin", "qmc_walkers_pa_step",
-    "qmc_walkers_pa_info", "qmc_walkers_pa_end", "qmc_lds_bandwidth",
+    "qmc_walkers_pa_info", "qmc_walkers_pa_end", "qmc_lds_bandwidth", "qmc_walkers_pa_strategy",
+    "qmc_walkers_set_sweep_order", "qmc_problem_create_visit",
 ]
 
 
@@ -50,8 +51,15 @@ class PaInfo(C.Structure):
     _fields_ = [
         ("population", C.c_int32), ("restarts", C.c_int32), ("overflow", C.c_int32),
         ("have_best", C.c_int32), ("steps", C.c_int64), ("attempts", C.c_int64),
-        ("best", C.c_double), ("beta", C.c_double), ("rho", C.c_double),
+        ("best", C.c_double), ("beta", C.c_double), ("rho", C.c_double), ("delta_beta", C.c_double),
     ]
+
+
+# problem visit orders (qmc_problem_create_visit)
+VISIT_COLOR_CLASSES, VISIT_INDEX_LEVELS = 0, 1
+
+# population annealing stepping sizes (qmc_walkers_pa_strategy)
+PA_SCHEDULE, PA_FRICTION_TENSOR, PA_ENERGY_VARIANCE, PA_CONSTANT_CULLING = 0, 1, 2, 3
 
 
 _lib = None
@@ -117,6 +125,10 @@ def lib():
     L.qmc_walkers_pa_begin.argtypes = [vp, C.c_int32, C.c_double, C.c_double, P(C.c_double), C.c_int32,
                                        C.c_uint32, C.c_int32]
     L.qmc_walkers_pa_step.argtypes = [vp]
+    L.qmc_walkers_pa_strategy.argtypes = [vp, C.c_int32, C.c_double, C.c_int32]
+    L.qmc_walkers_set_sweep_order.argtypes = [vp, C.c_int32]
+    L.qmc_problem_create_visit.argtypes = [vp, C.c_int, C.c_int32, C.c_int64, P(C.c_int64), P(C.c_int32),
+                                           P(C.c_double), C.c_int32, C.c_int32, P(vp)]
     L.qmc_walkers_pa_info.argtypes = [vp, P(PaInfo), vp]
     L.qmc_walkers_pa_end.argtypes = [vp]
     L.qmc_lds_bandwidth.argtypes = [vp, P(C.c_double)]
@@ -170,16 +182,17 @@ class Context:
 class Problem:
     """A problem in internal ids (terms with sorted unique ids, no constant terms)."""
 
-    def __init__(self, ctx, kind, n_vars, term_offsets, term_vars, coeffs, lanes=0):
+    def __init__(self, ctx, kind, n_vars, term_offsets, term_vars, coeffs, lanes=0, visit=0):
+        """visit: VISIT_COLOR_CLASSES (0) or VISIT_INDEX_LEVELS (1)."""
         self.ctx = ctx
         self.kind = kind
         off = np.ascontiguousarray(term_offsets, dtype=np.int64)
         ids = np.ascontiguousarray(term_vars, dtype=np.int32)
         c = np.ascontiguousarray(coeffs, dtype=np.float64)
         self.h = C.c_void_p()
-        _check(lib().qmc_problem_create(ctx.h, kind, n_vars, len(c), _p(off, C.c_int64),
-                                        _p(ids, C.c_int32), _p(c, C.c_double), lanes,
-                                        C.byref(self.h)))
+        _check(lib().qmc_problem_create_visit(ctx.h, kind, n_vars, len(c), _p(off, C.c_int64),
+                                              _p(ids, C.c_int32), _p(c, C.c_double), lanes, visit,
+                                              C.byref(self.h)))
         self.info = ProblemInfo()
         _check(lib().qmc_problem_get_info(self.h, C.byref(self.info)))
         self.n = self.info.n_vars
@@ -328,6 +341,12 @@ class Walkers:
         self._pa_table = t
         _check(lib().qmc_walkers_pa_begin(self.h, population, alpha, scale, _p(t, C.c_double), len(t),
                                           seed & 0xFFFFFFFF, sweeps_per_step))
+
+    def pa_strategy(self, strategy, constant=0.0, constant_population=False):
+        _check(lib().qmc_walkers_pa_strategy(self.h, int(strategy), float(constant), int(bool(constant_population))))
+
+    def set_sweep_order(self, ordered):
+        _check(lib().qmc_walkers_set_sweep_order(self.h, int(bool(ordered))))
 
     def pa_step(self):
         _check(lib().qmc_walkers_pa_step(self.h))

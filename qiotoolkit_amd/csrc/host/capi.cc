@@ -184,7 +184,10 @@ int qiotk_model_create(qiotk_engine e, const char* input_json, qiotk_model* out)
     auto m = std::make_unique<qiotk_model_s>();
     m->e = e;
     m->model = parse_model(Json::parse(input_json));
-    if (!m->model.empty()) m->dev = std::make_unique<DeviceModel>(e->ctx, m->model);
+    if (!m->model.empty()) {
+      m->dev = std::make_unique<DeviceModel>(e->ctx, m->model);
+      m->dev->handle(QMC_VISIT_COLOR_CLASSES);  // resident: solves exclude the upload
+    }
     *out = m.release();
     return 0;
   });
@@ -196,7 +199,10 @@ int qiotk_model_create_file(qiotk_engine e, const char* path, qiotk_model* out) 
     auto m = std::make_unique<qiotk_model_s>();
     m->e = e;
     m->model = parse_model(read_input_file(path));
-    if (!m->model.empty()) m->dev = std::make_unique<DeviceModel>(e->ctx, m->model);
+    if (!m->model.empty()) {
+      m->dev = std::make_unique<DeviceModel>(e->ctx, m->model);
+      m->dev->handle(QMC_VISIT_COLOR_CLASSES);  // resident: solves exclude the upload
+    }
     *out = m.release();
     return 0;
   });
@@ -213,7 +219,10 @@ int qiotk_model_create_terms(qiotk_engine e, int kind, int64_t n_terms, const in
     auto m = std::make_unique<qiotk_model_s>();
     m->e = e;
     m->model = make_model(kind == 0 ? QMC_ISING : QMC_PUBO, terms);
-    if (!m->model.empty()) m->dev = std::make_unique<DeviceModel>(e->ctx, m->model);
+    if (!m->model.empty()) {
+      m->dev = std::make_unique<DeviceModel>(e->ctx, m->model);
+      m->dev->handle(QMC_VISIT_COLOR_CLASSES);  // resident: solves exclude the upload
+    }
     *out = m.release();
     return 0;
   });

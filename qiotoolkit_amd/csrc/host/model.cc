@@ -135,13 +135,26 @@ double Model::cost(const uint8_t* bits) const {
   return total;
 }
 
-DeviceModel::DeviceModel(qmc_ctx ctx, const Model& m, int lanes) {
-  device_check(qmc_problem_create(ctx, m.kind, m.n_vars, m.n_terms(), m.term_off.data(),
-                                  m.term_var.data(), m.coeff.data(), lanes, &p_),
-               "qmc_problem_create");
-  device_check(qmc_problem_get_info(p_, &info_), "qmc_problem_get_info");
+qmc_problem DeviceModel::handle(int visit) {
+  const int k = visit == QMC_VISIT_INDEX_LEVELS ? 1 : 0;
+  if (!p_[k]) {
+    const Model& m = *m_;
+    device_check(qmc_problem_create_visit(ctx_, m.kind, m.n_vars, m.n_terms(), m.term_off.data(),
+                                          m.term_var.data(), m.coeff.data(), lanes_, visit, &p_[k]),
+                 "qmc_problem_create_visit");
+    device_check(qmc_problem_get_info(p_[k], &info_[k]), "qmc_problem_get_info");
+  }
+  return p_[k];
 }
 
-DeviceModel::~DeviceModel() { qmc_problem_destroy(p_); }
+const qmc_problem_info& DeviceModel::info(int visit) {
+  handle(visit);
+  return info_[visit == QMC_VISIT_INDEX_LEVELS ? 1 : 0];
+}
+
+DeviceModel::~DeviceModel() {
+  for (qmc_problem p : p_)
+    if (p) qmc_problem_destroy(p);
+}
 
 }  // namespace qiotk

@@ -55,18 +55,24 @@ Model parse_model(const Json& doc);
 Model make_model(int kind, const std::vector<Term>& terms);
 
 // Device copy of a model, owned.
+// The model on the device, once per visit order (QMC_VISIT_COLOR_CLASSES,
+// QMC_VISIT_INDEX_LEVELS), each uploaded when first asked for.  `m` must
+// outlive it.
 class DeviceModel {
  public:
-  DeviceModel(qmc_ctx ctx, const Model& m, int lanes = 0);
+  DeviceModel(qmc_ctx ctx, const Model& m, int lanes = 0) : ctx_(ctx), m_(&m), lanes_(lanes) {}
   ~DeviceModel();
   DeviceModel(const DeviceModel&) = delete;
   DeviceModel& operator=(const DeviceModel&) = delete;
-  qmc_problem handle() const { return p_; }
-  const qmc_problem_info& info() const { return info_; }
+  qmc_problem handle(int visit = QMC_VISIT_COLOR_CLASSES);
+  const qmc_problem_info& info(int visit = QMC_VISIT_COLOR_CLASSES);
 
  private:
-  qmc_problem p_ = nullptr;
-  qmc_problem_info info_{};
+  qmc_ctx ctx_;
+  const Model* m_;
+  int lanes_;
+  qmc_problem p_[2] = {nullptr, nullptr};
+  qmc_problem_info info_[2]{};
 };
 
 }  // namespace qiotk

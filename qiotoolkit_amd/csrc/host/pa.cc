@@ -1,6 +1,8 @@
 // PopulationAnnealing (solver/population_annealing.h:102-616) over the device
-// engine: each step sweeps every citizen, then computes the Boltzmann
-// weights, Poisson offspring counts, the resampled population
+// engine: each step sweeps every citizen, then computes the stepping size
+// (beta schedule, friction tensor, energy variance or constant culling), the
+// Boltzmann weights, offspring counts (Poisson, or residual resampling for a
+// constant population), the resampled population
 // (Population::resample, solver/population.h:171-218) and restarts on the
 // device, with the population size in device memory -- the host enqueues
 // steps and reads the outcome once at the end.  Random numbers follow the
@@ -10,12 +12,22 @@
 #include <cmath>
 #include <map>
 #include <numeric>
+#include <sstream>
 
 #include "solver.h"
 
 namespace qiotk {
 
 namespace {
+// the reference's matcher wording (matcher/greater-than.h:30-40): values as
+// an ostream prints them
+template <class T>
+[[noreturn]] void not_greater(const char* name, T bound, T found) {
+  std::ostringstream o;
+  o << "parameter `" << name << "`: must be greater than " << bound << ", found " << found;
+  throw Error(kValueError, o.str());
+}
+
 const Json& params_of(const Json& doc) {
   if (!doc.is_object() || !doc.has("params"))
     throw Error(kMissingInput, "Input field `params` is required.");
@@ -40,16 +52,45 @@ void PopulationAnnealing::configure(const Json& doc) {
   Solver::configure(doc);
   const Json& p = params_of(doc);
   if (p.has("alpha")) alpha_ = p.at("alpha").as_double();
-  if (!(alpha_ > 1.0)) throw Error(kValueError, "parameter `alpha` must be greater than 1");
+  if (!(alpha_ > 1.0)) not_greater("alpha", 1.0, alpha_);
   target_population_ = threads_ > 0 ? threads_ : 64;
   if (p.has("population")) target_population_ = p.at("population").as_int();
-  if (target_population_ <= 0) throw Error(kValueError, "parameter `population` must be greater than 0");
+  if (target_population_ <= 0) not_greater<int64_t>("population", 0, target_population_);
   if (p.has("constant_population")) constant_population_ = p.at("constant_population").as_bool();
-  if (constant_population_)
-    throw Error(kNotImplemented, "constant_population resampling is not available on this engine");
   if (p.has("resampling_strategy")) strategy_ = p.at("resampling_strategy").as_string();
-  if (strategy_ != "linear_schedule" && strategy_ != "geometric_schedule")
-    throw Error(kNotImplemented, "resampling_strategy `" + strategy_ + "` is not available on this engine");
+  if (p.has("sweeps_per_replica")) sweeps_per_replica_ = (int)p.at("sweeps_per_replica").as_int();
+  if (sweeps_per_replica_ <= 0) not_greater<int64_t>("sweeps_per_replica", 0, sweeps_per_replica_);
+  if (p.has("replicas_per_group")) group_ = (int)p.at("replicas_per_group").as_int();
+  set_output("population", Json(target_population_));
+  set_output("alpha", alpha_);
+  set_output("constant_population", Json(constant_population_));
+  set_output("sweeps_per_replica", Json((int64_t)sweeps_per_replica_));
+  if (strategy_ != "linear_schedule" && strategy_ != "geometric_schedule") {
+    // the adaptive stepping sizes start at beta 0 (population_annealing.h:730-774)
+    auto positive = [&](const char* name, double dflt) {
+      const double v = p.has(name) ? p.at(name).as_double() : dflt;
+      if (!(v > 0.0)) not_greater(name, 0.0, v);
+      set_output(name, v);
+      return v;
+    };
+    if (strategy_ == "friction_tensor") {
+      strategy_id_ = QMC_PA_FRICTION_TENSOR;
+      strategy_const_ = positive("friction_tensor_constant", 1.0);
+    } else if (strategy_ == "energy_variance") {
+      strategy_id_ = QMC_PA_ENERGY_VARIANCE;
+      strategy_const_ = positive("initial_culling_fraction", 0.5);
+    } else if (strategy_ == "constant_culling") {
+      strategy_id_ = QMC_PA_CONSTANT_CULLING;
+      strategy_const_ = positive("culling_fraction", 0.1);
+    } else {
+      throw Error(kInitialConfig,
+                  "`resampling strategy` must be one of ['linear_schedule',  'geometric_schedule', "
+                  "'friction_tensor', 'energy_variance', 'constant_culling'].");
+    }
+    beta_start_ = 0.0;
+    return;
+  }
+  strategy_id_ = QMC_PA_SCHEDULE;
   const bool geometric = strategy_ == "geometric_schedule";
   if (p.has("beta")) {
     beta_.configure(p.at("beta"));
@@ -76,12 +117,6 @@ void PopulationAnnealing::configure(const Json& doc) {
   if (beta_.get_initial() >= beta_.get_final())
     throw Error(kValueError, "the schedule for beta must be increasing (i.e., beta.initial < beta.final).");
   beta_start_ = beta_.get_value(0);
-  if (p.has("sweeps_per_replica")) sweeps_per_replica_ = (int)p.at("sweeps_per_replica").as_int();
-  if (sweeps_per_replica_ <= 0) throw Error(kValueError, "parameter `sweeps_per_replica` must be greater than 0");
-  if (p.has("replicas_per_group")) group_ = (int)p.at("replicas_per_group").as_int();
-  set_output("population", Json(target_population_));
-  set_output("alpha", alpha_);
-  set_output("sweeps_per_replica", Json((int64_t)sweeps_per_replica_));
 }
 
 PopulationAnnealing::~PopulationAnnealing() = default;
@@ -92,18 +127,25 @@ void PopulationAnnealing::init() {
   // Poisson resampling keeps the population near the target; room for
   // fluctuations is reserved up front.
   st.cap = (int32_t)std::max<int64_t>(4 * target_population_, target_population_ + 256);
-  const int B = group_ > 0 ? group_ : default_group(target_population_);
-  device_check(qmc_walkers_create(dmodel_->handle(), st.cap, B, seed_, 0, 0, st.cap, &st.w),
+  const bool seq = sequential();
+  const int B = group_ > 0 ? group_ : seq ? 32 : default_group(target_population_);
+  device_check(qmc_walkers_create(dmodel_->handle(visit()), st.cap, B, seed_, 0, 0, st.cap, &st.w),
                "qmc_walkers_create");
   apply_lanes(st.w, B);
+  if (seq) device_check(qmc_walkers_set_sweep_order(st.w, 1), "qmc_walkers_set_sweep_order");
   // beta k steps after the last restart, k = 0..steps (get_value clamps past
-  // the schedule's end)
-  const int64_t steps = steps_to_run();
-  std::vector<double> table((size_t)steps + 2);
-  for (size_t k = 0; k < table.size(); ++k) table[k] = beta_.get_value((double)k);
+  // the schedule's end); the adaptive stepping sizes start from beta 0
+  std::vector<double> table(2, 0.0);
+  if (strategy_id_ == QMC_PA_SCHEDULE) {
+    table.resize((size_t)steps_to_run() + 2);
+    for (size_t k = 0; k < table.size(); ++k) table[k] = beta_.get_value((double)k);
+  }
   device_check(qmc_walkers_pa_begin(st.w, (int32_t)target_population_, alpha_, model_->scale_factor(),
                                     table.data(), (int32_t)table.size(), seed_, sweeps_per_replica_),
                "qmc_walkers_pa_begin");
+  if (strategy_id_ != QMC_PA_SCHEDULE || constant_population_)
+    device_check(qmc_walkers_pa_strategy(st.w, strategy_id_, strategy_const_, constant_population_ ? 1 : 0),
+                 "qmc_walkers_pa_strategy");
   device_check(qmc_walkers_init(st.w, 0), "qmc_walkers_init");
   cur_beta_ = beta_start_;
 }

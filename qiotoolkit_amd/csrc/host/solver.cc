@@ -52,6 +52,13 @@ void Solver::configure(const Json& doc) {
     positions_per_lane_ = (int)p.at("positions_per_lane").as_int();
     if (positions_per_lane_ < 0) throw Error(kValueError, "`positions_per_lane` must be >= 0");
   }
+  if (p.has("sweep_order")) {
+    const std::string o = p.at("sweep_order").as_string();
+    if (o == "colored") sweep_order_ = 0;
+    else if (o == "index") sweep_order_ = 1;
+    else if (o == "sequential") sweep_order_ = 2;
+    else throw Error(kValueError, "`sweep_order` must be 'colored', 'index' or 'sequential'");
+  }
   if (p.has("time_limit")) time_limit_ = p.at("time_limit").as_double();
   else if (p.has("timeout")) time_limit_ = p.at("timeout").as_double();
   if (p.has("cost_limit")) cost_limit_ = p.at("cost_limit").as_double();
@@ -130,7 +137,7 @@ int Solver::lanes_for(int replicas_per_group) const {
   const int G = (replicas_per_group + 7) / 8;
   const int cap = std::max(64, 512 / G / 64 * 64);
   int l = 64;
-  while (l < cap && (int64_t)l * positions_per_lane_ < dmodel_->info().max_class) l *= 2;
+  while (l < cap && (int64_t)l * positions_per_lane_ < dmodel_->info(visit()).max_class) l *= 2;
   return std::min(l, cap);
 }
 
@@ -228,8 +235,8 @@ Json Solver::get_result() const {
   dev["launches"] = Json(timing_.launches);
   dev["attempts_per_s"] = timing_.device_ms > 0 ? (double)attempts_ / (timing_.device_ms * 1e-3) : 0.0;
   if (dmodel_) {
-    dev["colour_classes"] = Json((int64_t)dmodel_->info().n_classes);
-    dev["coupling"] = dmodel_->info().coupling == QMC_COUPLING_I32 ? "int32" : "fp32";
+    dev["colour_classes"] = Json((int64_t)dmodel_->info(visit()).n_classes);
+    dev["coupling"] = dmodel_->info(visit()).coupling == QMC_COUPLING_I32 ? "int32" : "fp32";
   }
   Json c3 = Json::object();
   c3["name"] = "device";
@@ -284,10 +291,12 @@ SimulatedAnnealing::~SimulatedAnnealing() {
 
 void SimulatedAnnealing::init() {
   if (model_->empty()) return;
-  const int B = group_ > 0 ? group_ : default_group(restarts_);
-  device_check(qmc_walkers_create(dmodel_->handle(), restarts_, B, seed_, 0, 0, restarts_, &w_),
+  const bool seq = sequential();
+  const int B = group_ > 0 ? group_ : seq ? std::min(restarts_, 32) : default_group(restarts_);
+  device_check(qmc_walkers_create(dmodel_->handle(visit()), restarts_, B, seed_, 0, 0, restarts_, &w_),
                "qmc_walkers_create");
   apply_lanes(w_, B);
+  if (seq) device_check(qmc_walkers_set_sweep_order(w_, 1), "qmc_walkers_set_sweep_order");
   if (model_->has_initial) {
     std::vector<uint8_t> st((size_t)restarts_ * model_->n_vars, 0);
     for (int r = 0; r < restarts_; ++r)
@@ -385,6 +394,8 @@ void SimulatedAnnealing::finalize() {
 void ParallelTempering::configure(const Json& doc) {
   Solver::configure(doc);
   const Json& p = params_of(doc);
+  if (sequential())
+    throw Error(kNotImplemented, "parallel tempering takes `sweep_order` 'colored' or 'index'");
   if (p.has("sweeps_per_replica")) {
     sweeps_per_replica_ = (int)p.at("sweeps_per_replica").as_int();
     if (sweeps_per_replica_ <= 0) throw Error(kValueError, "`sweeps_per_replica` must be greater than 0");
@@ -437,7 +448,7 @@ void ParallelTempering::init() {
   // ladder barrier (the best split measured on the 16^3 spin glass).
   int B = group_ > 0 ? group_ : ((L >= 4 && L % 2 == 0) ? 2 : L);
   if (L % B != 0) B = L;
-  device_check(qmc_walkers_create(dmodel_->handle(), L, B, seed_, 0, 0, L, &w_), "qmc_walkers_create");
+  device_check(qmc_walkers_create(dmodel_->handle(visit()), L, B, seed_, 0, 0, L, &w_), "qmc_walkers_create");
   apply_lanes(w_, B);
   if (model_->has_initial) {
     std::vector<uint8_t> st((size_t)L * model_->n_vars, 0);

@@ -274,8 +274,10 @@ struct qmc_walkers_s {
   int pa_on = 0;
   PaArgs pa_args{};
   DevBuf<PaDev> pa;
-  DevBuf<double> pa_table, pa_weight, pa_tcost, pa_tbest, pa_tbeta;
-  DevBuf<int32_t> pa_family, pa_family_next, pa_src, pa_hist, pa_count;
+  DevBuf<double> pa_table, pa_weight, pa_tcost, pa_tbest, pa_tbeta, pa_cost0, pa_tree;
+  DevBuf<int32_t> pa_family, pa_family_next, pa_src, pa_hist, pa_count, pa_ids;
+  bool pa_stepped = false;  // a step was enqueued since qmc_walkers_pa_begin
+  int ordered = 0;           // sweeps visit variables in index order (k_sweep_ordered)
   DevBuf<unsigned long long> pa_tacc;
   DevBuf<uint8_t> pa_plane_cur, pa_plane_low, pa_best_state;
   float last_ms = 0.f;
@@ -288,7 +290,7 @@ struct qmc_batch_s {
   bool dirty = true;
   // one launch group per kernel instantiation
   struct Group {
-    int kind, pairwise, coupling, nb, coop, ladder;
+    int kind, pairwise, coupling, nb, coop, ladder, ordered;
     int block = 64;
     int stride = 0;
     double work = 0;  // attempts planned (for spreading groups over streams)
@@ -381,7 +383,16 @@ int qmc_sync(qmc_ctx c) {
 int qmc_problem_create(qmc_ctx ctx, int kind, int32_t n_vars, int64_t n_terms,
                        const int64_t* term_offsets, const int32_t* term_vars, const double* coeffs,
                        int32_t lanes, qmc_problem* out) {
+  return qmc_problem_create_visit(ctx, kind, n_vars, n_terms, term_offsets, term_vars, coeffs, lanes,
+                                  QMC_VISIT_COLOR_CLASSES, out);
+}
+
+int qmc_problem_create_visit(qmc_ctx ctx, int kind, int32_t n_vars, int64_t n_terms,
+                             const int64_t* term_offsets, const int32_t* term_vars, const double* coeffs,
+                             int32_t lanes, int32_t visit, qmc_problem* out) {
   if (!ctx || !out) return fail(QMC_INVALID_ARGUMENT, "null handle");
+  if (visit != QMC_VISIT_COLOR_CLASSES && visit != QMC_VISIT_INDEX_LEVELS)
+    return fail(QMC_INVALID_ARGUMENT, "unknown visit order");
   if (kind != QMC_ISING && kind != QMC_PUBO) return fail(QMC_INVALID_ARGUMENT, "unknown model kind");
   if (n_vars <= 0 || n_terms <= 0) return fail(QMC_INVALID_ARGUMENT, "empty problem");
   if (lanes != 0 && lanes != 64 && lanes != 128 && lanes != 256 && lanes != 512)
@@ -417,22 +428,32 @@ int qmc_problem_create(qmc_ctx ctx, int kind, int32_t n_vars, int64_t n_terms,
   p->coupling = (all_int && 2.0 * abs_sum < 2147483647.0) ? QMC_COUPLING_I32 : QMC_COUPLING_F32;
   p->pairwise = max_term <= 2;
 
-  // Greedy colouring in variable order over "shares a term": members of a
-  // class can be updated simultaneously.
+  // Classes over "shares a term": members of a class can be updated
+  // simultaneously.  Colour classes: greedy colouring in variable order (few
+  // classes).  Index levels: class of v = 1 + the largest class of a
+  // neighbour u < v (0 without one), so running the classes in turn gives
+  // every variable the neighbour values it sees in an index-order sweep --
+  // the same decisions and end-of-sweep states.
   std::vector<int32_t> color(N, -1), stamp;
   int n_colors = 0;
   for (int32_t v = 0; v < N; ++v) {
+    int level = 0;
     for (int32_t j : node_terms[v])
       for (int64_t k = term_offsets[j]; k < term_offsets[j + 1]; ++k) {
         const int32_t u = term_vars[k];
         if (u < v) {
           const int32_t cu = color[u];
+          level = std::max(level, cu + 1);
           if ((int)stamp.size() <= cu) stamp.resize(cu + 1, -1);
           stamp[cu] = v;
         }
       }
     int cv = 0;
-    while (cv < (int)stamp.size() && stamp[cv] == v) ++cv;
+    if (visit == QMC_VISIT_INDEX_LEVELS) {
+      cv = level;
+    } else {
+      while (cv < (int)stamp.size() && stamp[cv] == v) ++cv;
+    }
     color[v] = cv;
     n_colors = std::max(n_colors, cv + 1);
   }
@@ -871,10 +892,16 @@ static void add_setup_descs(qmc_walkers w, const UnitDev* du, std::vector<SetupD
   }
 }
 
-static void setup_done(qmc_walkers w) {
+// The costs a set-up just computed are also the first population-annealing
+// step's costs before its sweeps.
+static int setup_done(qmc_walkers w) {
   w->pend_init = false;
   w->pend_zero = false;
   w->pend_prog = false;
+  if (w->pa_on && w->pa_args.cost0)
+    HIP_TRY(hipMemcpyAsync(w->pa_args.cost0, w->cost.p, (size_t)w->cap * sizeof(double),
+                           hipMemcpyDeviceToDevice, w->prob->ctx->stream));
+  return QMC_OK;
 }
 
 // One set-up launch for the recorded work of several walker sets.
@@ -905,8 +932,7 @@ static int materialize(qmc_walkers w) {
     std::vector<SetupDesc> d;
     add_setup_descs(w, du, d);
     if ((rc = launch_setups(p->ctx, d, w->stride))) return rc;
-    setup_done(w);
-    return QMC_OK;
+    return setup_done(w);
   }
   // states too large for the one-launch set-up
   if ((rc = ensure_cleared(w)) || (rc = ensure_scratch(w))) return rc;
@@ -915,7 +941,7 @@ static int materialize(qmc_walkers w) {
   HIP_TRY(launch_energy(p->dev(), w->scratch_states.p, w->R, w->cost.p, st, w->best_cost.p));
   const bool prog = w->pend_prog;
   const int64_t s0 = w->pend_sweep0, n = w->pend_n;
-  setup_done(w);
+  if ((rc = setup_done(w))) return rc;
   return prog ? set_progress(w, s0, n) : QMC_OK;
 }
 
@@ -1066,8 +1092,47 @@ int qmc_walkers_pa_begin(qmc_walkers w, int32_t population, double alpha, double
   a.plane_low = w->pa_plane_low.p;
   a.best_state = w->pa_best_state.p;
   a.B = w->B;
+  a.strategy = QMC_PA_SCHEDULE;
   w->pa_on = 1;
+  w->pa_stepped = false;
   return upload_units(w);
+}
+
+int qmc_walkers_pa_strategy(qmc_walkers w, int32_t strategy, double constant, int32_t constant_population) {
+  if (!w) return fail(QMC_INVALID_ARGUMENT, "null handle");
+  if (!w->pa_on) return fail(QMC_INVALID_ARGUMENT, "population mode is off");
+  if (w->pa_stepped) return fail(QMC_INVALID_ARGUMENT, "the strategy is set before the first step");
+  if (strategy < QMC_PA_SCHEDULE || strategy > QMC_PA_CONSTANT_CULLING)
+    return fail(QMC_INVALID_ARGUMENT, "unknown resampling strategy");
+  if (strategy != QMC_PA_SCHEDULE && !(constant > 0.0))
+    return fail(QMC_INVALID_ARGUMENT, "the strategy constant must be greater than 0");
+  PaArgs& a = w->pa_args;
+  const int cap = w->cap;
+  if (strategy == QMC_PA_FRICTION_TENSOR || strategy == QMC_PA_ENERGY_VARIANCE) {
+    if (!w->pa_cost0.p) {
+      HIP_TRY(w->pa_cost0.alloc(cap));
+      // costs before the first step's sweeps: now, or when a pending set-up
+      // computes them
+      if (!w->pend_init)
+        HIP_TRY(hipMemcpyAsync(w->pa_cost0.p, w->cost.p, (size_t)cap * sizeof(double), hipMemcpyDeviceToDevice,
+                               w->prob->ctx->stream));
+    }
+    a.cost0 = w->pa_cost0.p;
+  } else {
+    a.cost0 = nullptr;
+  }
+  int pow2 = 1;
+  while (pow2 < cap) pow2 <<= 1;
+  if (constant_population && pow2 > kPaTreeLds && !w->pa_tree.p) {
+    HIP_TRY(w->pa_tree.alloc((size_t)pow2 + 1));
+    HIP_TRY(w->pa_ids.alloc(cap));
+  }
+  a.tree = w->pa_tree.p;
+  a.ids = w->pa_ids.p;
+  a.strategy = strategy;
+  a.strategy_const = strategy == QMC_PA_SCHEDULE ? 0.0 : constant;
+  a.constant_population = constant_population ? 1 : 0;
+  return QMC_OK;
 }
 
 int qmc_walkers_pa_step(qmc_walkers w) {
@@ -1079,6 +1144,7 @@ int qmc_walkers_pa_step(qmc_walkers w) {
   w->pa_args.units = du;
   w->pa_args.n_units = w->n_units;
   HIP_TRY(launch_pa_step(w->pa_args, w->prob->dev(), w->prob->ctx->stream));
+  w->pa_stepped = true;
   return QMC_OK;
 }
 
@@ -1098,6 +1164,7 @@ int qmc_walkers_pa_info(qmc_walkers w, qmc_pa_info* info, uint8_t* best_state) {
   info->best = d.best;
   info->beta = d.cur_beta;
   info->rho = d.rho;
+  info->delta_beta = d.delta_beta;
   if (best_state && w->prob->N > 0)
     HIP_TRY(hipMemcpy(best_state, w->pa_best_state.p, (size_t)w->prob->N, hipMemcpyDeviceToHost));
   return QMC_OK;
@@ -1289,6 +1356,7 @@ int qmc_walkers_sweep(qmc_walkers w, int64_t sweep0, int32_t n_sweeps, int beta_
   const UnitDev* du = nullptr;
   if ((rc = device_units(w, &du))) return rc;
   HIP_TRY(hipEventRecord(c->ev0, c->stream));
+  ordered = ordered || w->ordered;
   rc = launch_units(w->prob, w->nb, du, w->n_units, ordered ? 64 : w->lanes * w->G, w->stride,
                     n_sweeps, ordered, c->stream, 0, 0);
   if (rc) return rc;
@@ -1469,11 +1537,12 @@ static int batch_rebuild(qmc_batch b) {
     qmc_problem p = w->prob;
     // one group per kernel instantiation and work-group size, so no launch
     // carries idle threads for its smaller members
-    const int block = w->lanes * w->G;
+    const int block = w->ordered ? 64 : w->lanes * w->G;
     qmc_batch_s::Group* g = nullptr;
     for (auto& gg : b->groups)
       if (gg->kind == p->kind && gg->pairwise == p->pairwise && gg->coupling == p->coupling &&
-          gg->nb == w->nb && gg->coop == w->coop && gg->ladder == (w->ladder > 0) && gg->block == block)
+          gg->nb == w->nb && gg->coop == w->coop && gg->ladder == (w->ladder > 0) && gg->block == block &&
+          gg->ordered == w->ordered)
         g = gg.get();
     if (!g) {
       b->groups.push_back(std::make_unique<qmc_batch_s::Group>());
@@ -1484,6 +1553,7 @@ static int batch_rebuild(qmc_batch b) {
       g->nb = w->nb;
       g->coop = w->coop;
       g->ladder = w->ladder > 0;
+      g->ordered = w->ordered;
       g->block = block;
     }
     g->stride = std::max(g->stride, w->stride);
@@ -1514,12 +1584,12 @@ static int launch_group(qmc_batch_s::Group& g, int32_t max_sweeps, const double*
   a.n_sweeps = max_sweeps;
   a.lut = lut;
   a.unit_of_block = nullptr;
-  const size_t smem = sweep_smem_bytes(g.stride, 0, &a.best_lds);
+  const size_t smem = sweep_smem_bytes(g.stride, g.ordered, &a.best_lds);
   if (smem > 160 * 1024) return fail(QMC_MEMORY_LIMITED, "batch LDS footprint exceeds 160 KiB");
   const int n = (int)g.units.size();
   if (!g.coop) {
-    HIP_TRY(launch_sweep(g.kind, g.pairwise, g.coupling, g.nb, g.d_units.p, n, g.block, smem, a, 0, 0, g.ladder,
-                         st));
+    HIP_TRY(launch_sweep(g.kind, g.pairwise, g.coupling, g.nb, g.d_units.p, n, g.block, smem, a, g.ordered, 0,
+                         g.ladder, st));
     return QMC_OK;
   }
   // Cooperative ladders: launches of at most the co-resident number of
@@ -1597,7 +1667,8 @@ static int batch_materialize(qmc_batch b) {
     done.push_back(w);
   }
   if (int rc = launch_setups(b->ctx, d, max_stride)) return rc;
-  for (qmc_walkers w : done) setup_done(w);
+  for (qmc_walkers w : done)
+    if (int rc = setup_done(w)) return rc;
   return QMC_OK;
 }
 
@@ -1732,6 +1803,17 @@ int qmc_walkers_set_lanes(qmc_walkers w, int32_t lanes) {
   if ((lanes != 64 && lanes != 128 && lanes != 256 && lanes != 512) || lanes * w->G > 512)
     return fail(QMC_INVALID_ARGUMENT, "lanes must be 64, 128, 256 or 512 with lanes * ceil(B/8) <= 512");
   w->lanes = lanes;
+  drop_graph(w);
+  return upload_units(w);
+}
+
+int qmc_walkers_set_sweep_order(qmc_walkers w, int32_t ordered) {
+  if (!w) return fail(QMC_INVALID_ARGUMENT, "null handle");
+  if (ordered && w->B > 64)
+    return fail(QMC_INVALID_ARGUMENT, "index-order sweeps take at most 64 replicas per group");
+  if (ordered && (w->ladder > 0 || w->coop))
+    return fail(QMC_INVALID_ARGUMENT, "index-order sweeps take no replica-exchange ladder");
+  w->ordered = ordered ? 1 : 0;
   drop_graph(w);
   return upload_units(w);
 }

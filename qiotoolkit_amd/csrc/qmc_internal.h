@@ -112,7 +112,12 @@ struct PaDev {
   double cur_beta;
   double best;        // lowest cost any citizen has reached
   double rho;         // family statistic of the last resampling
+  double delta_beta;  // stepping size of the last step
 };
+
+// Residual resampling keeps its selection tree in LDS up to this many
+// citizens (a power of two), in PaArgs::tree above it.
+constexpr int kPaTreeLds = 4096;
 
 struct PaArgs {
   PaDev* pa;
@@ -122,6 +127,12 @@ struct PaArgs {
   int32_t sweeps_per_step, pad;
   double alpha, scale, beta_start;
   uint32_t seed;
+  int32_t strategy;           // QMC_PA_SCHEDULE ... QMC_PA_CONSTANT_CULLING
+  int32_t constant_population;// residual resampling instead of Poisson offspring
+  double strategy_const;      // friction constant, culling fraction (initial or constant)
+  double* cost0;              // costs before the step's sweeps (friction tensor, energy variance)
+  double* tree;               // selection tree above kPaTreeLds citizens (pow2(cap) + 1 entries)
+  int32_t* ids;               // its ids (cap entries)
   double* cost;
   double* best_cost;
   double* beta;

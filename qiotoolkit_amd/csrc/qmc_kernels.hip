@@ -1286,16 +1286,167 @@ struct AddOp {
   template <class T> __device__ T operator()(T a, T b) const { return a + b; }
 };
 
+// Sum over the 64 lanes of a wavefront (butterfly: every lane ends with the
+// same bits, as x + y == y + x).
+__device__ __forceinline__ double pa_wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  return v;
+}
+
+// CullingEstimator (population_annealing.h:175-212) at dbeta, by one
+// wavefront: weights w_i = exp(-dbeta (E_i - E_min) scale), Q = sum w,
+// result = sum_i max(0, 1 - w_i target / Q) / R - epsilon0.
+__device__ double pa_culling_wave(const PaArgs& A, int R, double mn, double db) {
+  const int lane = threadIdx.x & 63;
+  double q = 0.0;
+  for (int i = lane; i < R; i += 64) q += exp(-db * (A.cost[i] - mn) * A.scale);
+  q = pa_wave_sum(q);
+  const double rrq = (double)A.target / q;
+  double s = 0.0;
+  for (int i = lane; i < R; i += 64) {
+    const double v = 1.0 - exp(-db * (A.cost[i] - mn) * A.scale) * rrq;
+    if (v > 0.0) s += v;
+  }
+  s = pa_wave_sum(s);
+  return s / (double)R - A.strategy_const;
+}
+
+// find_delta_beta (population_annealing.h:218-260): the first e in
+// [-20, 20) where the estimate at 2^e changes sign against the estimate at
+// 0 brackets the root (all 40 estimates at once, one per wavefront and
+// round), then bisection by halving steps down to 2^-20 from the lower end.
+// Every thread returns the same value.
+__device__ double pa_find_delta_beta(const PaArgs& A, int R, double mn, double* ce, double* bcast) {
+  const int wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  for (int j = wv; j <= 40; j += nw) {
+    const double v = pa_culling_wave(A, R, mn, j < 40 ? ldexp(1.0, j - 20) : 0.0);
+    if ((threadIdx.x & 63) == 0) ce[j] = v;
+  }
+  __syncthreads();
+  if (wv == 0) {
+    double lower = 0.0, lower_value = ce[40];
+    int e;
+    for (e = -20; e < 20; ++e) {
+      const double uv = ce[e + 20];
+      if ((lower_value < 0) == (uv < 0)) {
+        lower = ldexp(1.0, e);
+        lower_value = uv;
+      } else {
+        break;
+      }
+    }
+    for (e -= 2; e >= -20; --e) {
+      const double upper = lower + ldexp(1.0, e);
+      const double uv = pa_culling_wave(A, R, mn, upper);
+      if ((lower_value < 0) == (uv < 0)) {
+        lower = upper;
+        lower_value = uv;
+      }
+    }
+    if ((threadIdx.x & 63) == 0) *bcast = lower;
+  }
+  __syncthreads();
+  return *bcast;
+}
+
+// Prefix sum of a selection tree (RandomSelector::get_cumulative_weight,
+// random_selector.cc:22-35): weight of the positions below `position`.
+__device__ __forceinline__ double pa_tree_cum(const double* tree, int P, int position) {
+  int p = 0;
+  double w = 0.0;
+  for (int k = P; k > 0; k >>= 1)
+    if (position & k) {
+      p |= k;
+      w += tree[p];
+    }
+  return w;
+}
+
+// RandomSelector::add_weight (random_selector.cc:37-50).
+__device__ __forceinline__ void pa_tree_add(double* tree, int P, int position, double w) {
+  for (int k = 1; k <= P; k <<= 1) {
+    if (position & k) position ^= k;
+    else tree[position + k] += w;
+  }
+}
+
+// Residual resampling (population_annealing.h:540-578): `fills` citizens
+// drawn without replacement with probability proportional to their residual
+// weights res[0..R), each adding one offspring to count[].  The selection
+// tree is utils::RandomSelector's (random_selector.cc): node q (1..P, P the
+// power of two >= R) holds the residuals of positions [q - lowbit(q), q)
+// summed in position order, exactly as R sequential insertions leave it;
+// pick j's uniform is canonical2 of Philox(j, step, 1, DOM_PICK) under
+// (seed, 0).  The tree is built by all threads, the picks -- a sequential
+// process -- by thread 0.
+__device__ void pa_residual_fill(const PaArgs& A, int R, int fills, uint32_t step, double* tree_lds,
+                                 int32_t* ids_lds) {
+  int P = 1;
+  while (P < R) P <<= 1;
+  double* tree = P <= kPaTreeLds ? tree_lds : A.tree;
+  int32_t* ids = P <= kPaTreeLds ? ids_lds : A.ids;
+  const double* res = A.weight;
+  for (int q = threadIdx.x; q <= P; q += blockDim.x) {
+    double s = 0.0;
+    if (q > 0) {
+      const int lo = q - (q & -q), hi = q < R ? q : R;
+      for (int j = lo; j < hi; ++j) s += res[j];
+    }
+    tree[q] = s;
+  }
+  for (int i = threadIdx.x; i < R; i += blockDim.x) ids[i] = i;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double total = 0.0;
+    for (int j = 0; j < R; ++j) total += res[j];
+    int n = R;
+    for (int j = 0; j < fills && n > 0; ++j) {
+      const uint4 w = philox4x32_10((uint32_t)j, step, 1u, DOM_PICK, A.seed, 0u);
+      const double u = canonical2(w.x, w.y);
+      // select_position (random_selector.cc:67-94); a descent past the last
+      // node (possible only when rounding leaves the root below u * total)
+      // stops there
+      int pos = 0;
+      if (n > 1) {
+        double x = u * total;
+        for (int k = P; k > 0; k >>= 1)
+          if (pos + k <= P && x >= tree[pos + k]) {
+            pos += k;
+            x -= tree[pos];
+          }
+        if (pos >= n) pos = n - 1;
+      }
+      const int id = ids[pos];
+      // remove_position (random_selector.cc:96-107)
+      ids[pos] = ids[n - 1];
+      n -= 1;
+      const double before = pa_tree_cum(tree, P, pos + 1) - pa_tree_cum(tree, P, pos);
+      const double after = pa_tree_cum(tree, P, n + 1) - pa_tree_cum(tree, P, n);
+      total -= before;
+      pa_tree_add(tree, P, pos, after - before);
+      pa_tree_add(tree, P, n, -after);
+      A.count[id] += 1;
+    }
+  }
+  __syncthreads();
+}
+
 // The step's decisions (population_annealing.h:262-488 with resample
-// :490-616, Poisson branch): lowest-cost update, cost statistics, weights
-// exp(-dbeta (E - E_min) scale), Poisson offspring with mean
-// target * w / sum(w), their scan into the copy map src, the family census
+// :490-616): lowest-cost update, cost statistics, the stepping size (schedule
+// difference, friction tensor, energy variance or constant culling), weights
+// exp(-dbeta (E - E_min) scale), offspring counts -- Poisson with mean
+// target * w / sum(w), or floor(target * w / sum(w)) plus a residual fill to
+// the population size -- their scan into the copy map src, the family census
 // rho = sum(family size^2) / R, and the restart rule R / rho < alpha or zero
 // cost variance.  One work-group of 1024 threads.
 __global__ void __launch_bounds__(1024) k_pa_step(PaArgs A) {
   __shared__ double rd[16];
   __shared__ int ri[16];
   __shared__ long long rl[16];
+  __shared__ double ce[41];
+  __shared__ double s_tree[kPaTreeLds + 1];
+  __shared__ int32_t s_ids[kPaTreeLds];
   PaDev* P = A.pa;
   const int tid = threadIdx.x, bd = blockDim.x;
   const int R = P->R;
@@ -1326,9 +1477,38 @@ __global__ void __launch_bounds__(1024) k_pa_step(PaArgs A) {
   }
   const double var = pa_reduce(vs, AddOp(), rd) / (double)R;
 
-  const int64_t k = step - P->base_step;
-  const int64_t last = A.n_table - 1;
-  const double db = A.beta_table[k + 1 < last ? k + 1 : last] - A.beta_table[k < last ? k : last];
+  double db = 0.0;
+  if (A.strategy == QMC_PA_SCHEDULE) {
+    const int64_t k = step - P->base_step;
+    const int64_t last = A.n_table - 1;
+    db = A.beta_table[k + 1 < last ? k + 1 : last] - A.beta_table[k < last ? k : last];
+  } else if (A.strategy == QMC_PA_CONSTANT_CULLING) {
+    db = pa_find_delta_beta(A, R, mn, ce, &rd[0]);
+  } else {
+    // sigma = sqrt(var_before var_after), gamma = cov(before, after)
+    double sb = 0.0;
+    for (int i = tid; i < R; i += bd) sb += A.cost0[i];
+    const double avg_b = pa_reduce(sb, AddOp(), rd) / (double)R;
+    double vb = 0.0, g = 0.0;
+    for (int i = tid; i < R; i += bd) {
+      const double d0 = A.cost0[i] - avg_b, d1 = A.cost[i] - avg;
+      vb += d0 * d0;
+      g += d0 * d1;
+    }
+    const double var_b = pa_reduce(vb, AddOp(), rd) / (double)R;
+    const double gamma = pa_reduce(g, AddOp(), rd) / (double)R;
+    const double sigma = sqrt(var_b * var);
+    if (A.strategy == QMC_PA_FRICTION_TENSOR) {
+      double zeta = 0.0;
+      if (sigma > 0.0 && gamma != 0.0) {
+        const double delta = log(sigma) - log(fabs(gamma));
+        if (fabs(delta) > 1e-11) zeta = sigma * ((double)A.sweeps_per_step / delta);
+      }
+      if (zeta > 0.0) db = A.strategy_const / sqrt(zeta);
+    } else if (sigma > 0.0) {
+      db = A.strategy_const / sqrt(sigma);
+    }
+  }
   double rho = 1.0;
   int Rn = R;
   bool moved = false;
@@ -1340,10 +1520,23 @@ __global__ void __launch_bounds__(1024) k_pa_step(PaArgs A) {
       ws += w;
     }
     const double total = pa_reduce(ws, AddOp(), rd);
-    const double rrq = (double)A.target / total;
-    for (int i = tid; i < R; i += bd) {
-      const double mean = rrq * A.weight[i];
-      A.count[i] = mean > 0.0 ? (int)pa_poisson(mean, (uint32_t)i, (uint32_t)step, A.seed) : 0;
+    if (A.constant_population) {
+      int fl = 0;
+      for (int i = tid; i < R; i += bd) {
+        const double rw = (double)A.target * A.weight[i] / total;
+        const int c = (int)floor(rw);
+        A.count[i] = c;
+        A.weight[i] = rw - (double)c;
+        fl += c;
+      }
+      const int floors = pa_reduce(fl, AddOp(), ri);
+      if (floors < R) pa_residual_fill(A, R, R - floors, (uint32_t)step, s_tree, s_ids);
+    } else {
+      const double rrq = (double)A.target / total;
+      for (int i = tid; i < R; i += bd) {
+        const double mean = rrq * A.weight[i];
+        A.count[i] = mean > 0.0 ? (int)pa_poisson(mean, (uint32_t)i, (uint32_t)step, A.seed) : 0;
+      }
     }
     // offspring of citizen i fill src[off_i .. off_i + count_i)
     int carry = 0;
@@ -1428,6 +1621,7 @@ __global__ void __launch_bounds__(1024) k_pa_step(PaArgs A) {
     P->moved = moved ? 1 : 0;
     P->restart = restart ? 1 : 0;
     P->rho = rho;
+    P->delta_beta = db;
     P->step = step + 1;
     if (restart) {
       P->cur_beta = A.beta_start;
@@ -1552,18 +1746,26 @@ __global__ void k_pa_init(PaArgs A) {
   }
 }
 
-// Energies of a restart's fresh citizens: cost = lowest cost = E(state).
+// Energies of a restart's fresh citizens: cost = lowest cost = E(state);
+// with cost0, every citizen's cost is kept as the next step's cost before
+// its sweeps.
 __global__ void __launch_bounds__(256)
 k_pa_energy(PaArgs A, const int64_t* __restrict__ term_off, const int32_t* __restrict__ term_var,
             const double* __restrict__ term_c, int64_t M, int kind) {
   __shared__ double part[256];
   const PaDev* P = A.pa;
-  if (!P->restart || (int)blockIdx.x >= P->R) return;
+  if ((int)blockIdx.x >= P->R) return;
+  if (!P->restart) {
+    // the next step's costs before its sweeps
+    if (A.cost0 && threadIdx.x == 0) A.cost0[blockIdx.x] = A.cost[blockIdx.x];
+    return;
+  }
   const double e = state_energy_256(term_off, term_var, term_c, M, kind,
                                     A.plane_cur + (int64_t)blockIdx.x * A.n_vars, part);
   if (threadIdx.x == 0) {
     A.cost[blockIdx.x] = e;
     A.best_cost[blockIdx.x] = e;
+    if (A.cost0) A.cost0[blockIdx.x] = e;
   }
 }
 

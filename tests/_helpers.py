@@ -5,12 +5,12 @@ from oracle import pyoracle as po
 from qiotoolkit_amd import _qmc
 
 
-def pair(ctx, kind, terms, lanes=0):
+def pair(ctx, kind, terms, lanes=0, visit=0):
     """(oracle problem, device problem) with identical internal ids."""
     op = po.Problem(kind, terms)
     off, ids, c = op.edges()
     code = _qmc.QMC_ISING if kind == "ising" else _qmc.QMC_PUBO
-    qp = _qmc.Problem(ctx, code, op.n, off, ids, c, lanes=lanes)
+    qp = _qmc.Problem(ctx, code, op.n, off, ids, c, lanes=lanes, visit=visit)
     return op, qp
 
 

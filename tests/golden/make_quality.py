@@ -10,6 +10,9 @@ single-problem workloads, one best energy per seed for seeds 100..199:
   sa_pubo_2000      SA, random PUBO 2000 vars / 20000 terms (degrees 1-4 at
                     10/40/30/20 %, c in [-10,10]\\{0}, seed 1), 256 restarts x 2000
                     sweeps, linear beta 0.01 -> 5.0; best over the restarts
+  pa_lattice2d_12_* population annealing on the 12x12 periodic +-1 lattice
+                    (seed 41) with each stepping size and resampling method
+                    (params below); the global lowest of the run
 
 The oracle is built with the reference's Release flags (liboracle_fast.so).
 The GPU quality test compares the device solver's distribution over the same
@@ -45,7 +48,30 @@ def cases():
         "sa_pubo_2000": dict(
             kind="pubo", terms=lambda: wl.random_pubo(2000, 20000, seed=1), solver="sa",
             params={"beta_start": 0.01, "beta_stop": 5.0, "restarts": 256, "sweeps": 2000}),
+        **pa_cases(),
     }
+
+
+def pa_cases():
+    lat = lambda: wl.lattice2d(12, seed=41)  # noqa: E731
+    sched = {"beta_start": 0.0, "beta_stop": 3.0}
+    out = {
+        "pa_lattice2d_12_schedule": dict(population=128, sweeps=200, **sched),
+        "pa_lattice2d_12_friction": dict(population=128, sweeps=200, resampling_strategy="friction_tensor",
+                                         friction_tensor_constant=1.0),
+        "pa_lattice2d_12_energy_variance": dict(population=64, sweeps=100, resampling_strategy="energy_variance",
+                                                initial_culling_fraction=1.0),
+        "pa_lattice2d_12_culling": dict(population=64, sweeps=200, resampling_strategy="constant_culling",
+                                        culling_fraction=0.2),
+        "pa_lattice2d_12_schedule_constant": dict(population=32, sweeps=60, constant_population=True, **sched),
+        "pa_lattice2d_12_culling_constant": dict(population=32, sweeps=60, resampling_strategy="constant_culling",
+                                                 culling_fraction=0.2, constant_population=True),
+    }
+    return {k: dict(kind="ising", terms=lat, solver="pa", params=v) for k, v in out.items()}
+
+
+PA_STRATEGIES = {"linear_schedule": (0, None), "friction_tensor": (1, "friction_tensor_constant"),
+                 "energy_variance": (2, "initial_culling_fraction"), "constant_culling": (3, "culling_fraction")}
 
 
 def best_of(op, case, seed, threads):
@@ -53,6 +79,13 @@ def best_of(op, case, seed, threads):
     if case["solver"] == "sa":
         o = po.make_opts(po.RNG_REFERENCE, seed=seed, threads=threads)
         r = op.sa(o, p["beta_start"], p["beta_stop"], p["sweeps"], p["restarts"])
+    elif case["solver"] == "pa":
+        strategy, key = PA_STRATEGIES[p.get("resampling_strategy", "linear_schedule")]
+        o = po.make_opts(po.RNG_REFERENCE, seed=seed, threads=1)
+        r = op.pa(o, p.get("beta_start", 0.0), p.get("beta_stop", 5.0), p["sweeps"], p["population"],
+                  constant_population=p.get("constant_population", False), strategy=strategy,
+                  strategy_const=p[key] if key else 0.0)
+        return float(r["best"] + op.const_cost)
     else:
         t = p["temperatures"]
         temps = wl.geometric_temperatures(t["initial"], t["final"], t["count"])

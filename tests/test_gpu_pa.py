@@ -1,6 +1,7 @@
 """Population annealing on the device (qmc_walkers_pa_*) against the oracle's
-counter-mode step, step by step: after each step's sweeps the device's
-population is read back, the oracle computes the step's decisions from those
+counter-mode step, step by step: each step's sweeps of the live citizens
+equal the oracle's counter-mode sweeps in the same visit order (colour
+classes or index order), then the device's population is read back, the oracle computes the step's decisions from those
 costs (orc_pa_step_counter: weights, Poisson offspring, copy map, family
 statistic, restart rule, global lowest), and the device's population after
 its own step must equal the oracle's prediction bit for bit -- states, lowest
@@ -15,20 +16,28 @@ from oracle import pyoracle as po
 from qiotoolkit_amd import _qmc
 from qiotoolkit_amd import workloads as wl
 
-from _helpers import pair
+from _helpers import arith_of, pair
 
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("alpha,pop,B", [(2.0, 40, 2), (1e9, 24, 1), (2.0, 70, 8)])
-def test_device_pa_steps_match_oracle(ctx, alpha, pop, B):
-    print("pa steps alpha=%g pop=%d B=%d" % (alpha, pop, B), flush=True)
-    op, qp = pair(ctx, "ising", wl.lattice2d(8, seed=4))
+def _steps_match_oracle(ctx, alpha, pop, B, strategy=0, const=0.0, constant_population=False, S=25, side=8,
+                        ordered=False, visit=_qmc.VISIT_INDEX_LEVELS):
+    op, qp = pair(ctx, "ising", wl.lattice2d(side, seed=4), visit=visit)
     cap = max(4 * pop, pop + 256)
-    seed, S = 13, 25
-    table = np.linspace(0.0, 3.0, S + 2)
+    seed = 13
+    adaptive = strategy != _qmc.PA_SCHEDULE
+    table = np.zeros(2) if adaptive else np.linspace(0.0, 3.0, S + 2)
     w = _qmc.Walkers(qp, cap, B, seed=seed, replicas_per_seed=cap)
+    if ordered:
+        w.set_sweep_order(True)
+    # the sweeps of the live citizens against the oracle's counter-mode
+    # sweeps in the same visit order
+    sweep_opts = po.make_opts(po.RNG_COUNTER, seed=seed, arith=arith_of(qp),
+                              order=None if ordered else qp.order(w.lanes())[0])
     w.pa_begin(pop, alpha, 1.0, table, seed, 1)
+    if adaptive or constant_population:
+        w.pa_strategy(strategy, const, constant_population)
     w.init(0)
     opts = po.make_opts(po.RNG_COUNTER, seed=seed)
     fam = np.arange(pop, dtype=np.int32)
@@ -36,15 +45,31 @@ def test_device_pa_steps_match_oracle(ctx, alpha, pop, B):
     have_best, best, best_state = False, 0.0, None
     cur_beta = table[0]
     for step in range(S):
+        pre = w.read()
         w.sweep(step, None, n_sweeps=1)
         before = w.read()
+        want = {k: pre[k][:R].copy() for k in ("states", "cost", "lowest", "lowest_states")}
+        op.run_sweeps(sweep_opts, want["states"], want["cost"], want["lowest"], want["lowest_states"],
+                      np.arange(R), step, pre["beta"][:R])
+        for k, v in want.items():
+            assert np.array_equal(before[k][:R], v), (step, k)
         o = po.pa_step_counter(before["cost"][:R], before["lowest"][:R], fam, step, base, table, pop, cap,
-                               alpha, 1.0, seed, have_best, best)
+                               alpha, 1.0, seed, have_best, best, cost_before=pre["cost"][:R], strategy=strategy,
+                               strategy_const=const, constant_population=constant_population)
         w.pa_step()
         info = w.pa_info()
         after = w.read()
         Rf, src = o["R"], o["src"]
+        # adaptive steps: device reductions run in another order than the
+        # oracle's sequential sums (constant culling lands on the same
+        # multiple of 2^-20)
+        if strategy == _qmc.PA_CONSTANT_CULLING or not adaptive:
+            assert info["delta_beta"] == o["dbeta"], step
+        else:
+            assert abs(info["delta_beta"] - o["dbeta"]) <= 1e-12 * abs(o["dbeta"]), step
         assert info["population"] == Rf, step
+        if constant_population and not o["restart"]:
+            assert Rf == R
         if o["best_src"] >= 0:
             have_best, best = True, o["best"]
             best_state = before["lowest_states"][o["best_src"]].copy()
@@ -71,7 +96,7 @@ def test_device_pa_steps_match_oracle(ctx, alpha, pop, B):
             assert np.array_equal(after["lowest_states"][:Rf], before["lowest_states"][src]), step
             assert np.array_equal(after["cost"][:Rf], before["cost"][src]), step
             assert np.array_equal(after["lowest"][:Rf], before["lowest"][src]), step
-            cur_beta = cur_beta + o["dbeta"]
+            cur_beta = cur_beta + info["delta_beta"]
             want_beta = np.full(Rf, cur_beta)
         assert np.array_equal(after["beta"][:Rf], want_beta), step
         assert info["restarts"] == restarts
@@ -82,6 +107,35 @@ def test_device_pa_steps_match_oracle(ctx, alpha, pop, B):
     end = w.read()
     assert np.array_equal(end["cost"], after["cost"][:R])
     w.close()
+    return restarts
+
+
+@pytest.mark.parametrize("alpha,pop,B,ordered,visit", [(2.0, 40, 2, False, 1), (1e9, 24, 1, False, 1),
+                                                       (2.0, 70, 8, False, 0), (2.0, 40, 32, True, 0),
+                                                       (2.0, 70, 16, True, 1)])
+def test_device_pa_steps_match_oracle(ctx, alpha, pop, B, ordered, visit):
+    """visit 1: index-level classes (the default of the population annealing
+    solver), 0: colour classes; ordered: one lane per replica."""
+    print("pa steps alpha=%g pop=%d B=%d ordered=%d visit=%d" % (alpha, pop, B, ordered, visit), flush=True)
+    _steps_match_oracle(ctx, alpha, pop, B, ordered=ordered, visit=visit)
+
+
+@pytest.mark.parametrize("strategy,const,constant_population", [
+    (_qmc.PA_FRICTION_TENSOR, 1.0, False), (_qmc.PA_ENERGY_VARIANCE, 0.5, False),
+    (_qmc.PA_CONSTANT_CULLING, 0.1, False), (_qmc.PA_SCHEDULE, 0.0, True),
+    (_qmc.PA_CONSTANT_CULLING, 0.2, True), (_qmc.PA_FRICTION_TENSOR, 1.0, True)])
+def test_device_pa_strategies_match_oracle(ctx, strategy, const, constant_population):
+    """The adaptive stepping sizes and residual resampling
+    (qmc_walkers_pa_strategy) step by step against orc_pa_step_counter."""
+    print("pa strategy=%d const=%g constant_population=%d" % (strategy, const, constant_population), flush=True)
+    _steps_match_oracle(ctx, 2.0, 48, 4, strategy, const, constant_population)
+
+
+def test_device_pa_constant_population_beyond_the_lds_tree(ctx):
+    """Residual resampling of 5000 citizens: the selection tree (8193 nodes)
+    lives in device memory instead of LDS."""
+    print("pa constant population 5000", flush=True)
+    _steps_match_oracle(ctx, 2.0, 5000, 8, _qmc.PA_SCHEDULE, 0.0, True, S=3, side=4)
 
 
 def test_device_pa_solve_is_deterministic():

@@ -181,6 +181,52 @@ def test_best_energy_distribution_at_workload_size(engine, name):
     assert abs(gpu.mean() - cpu.mean()) <= 0.005 * abs(cpu.mean())
 
 
+PA_QUALITY = ["pa_lattice2d_12_schedule", "pa_lattice2d_12_friction", "pa_lattice2d_12_energy_variance",
+              "pa_lattice2d_12_culling", "pa_lattice2d_12_schedule_constant", "pa_lattice2d_12_culling_constant"]
+
+
+@pytest.mark.parametrize("name", PA_QUALITY)
+def test_pa_best_energy_distribution_matches_reference(engine, name):
+    """Population annealing with each stepping size and resampling method on
+    the 12x12 lattice: best energies over seeds 100..199 against the oracle's
+    reference-mode runs in tests/golden/quality_<name>.json.  Two-sample KS
+    p > 0.05; means within 0.5 %, or within four standard errors of their
+    difference where the spread of the runs (restarts) makes 0.5 % smaller
+    than the sampling noise."""
+    fx = json.load(open(os.path.join(GOLD, "quality_%s.json" % name)))
+    case = quality_cases()[name]
+    doc = {"cost_function": {"type": case["kind"], "terms": case["terms"]()}}
+    model = engine.model(doc)
+    print("pa quality %s: %d solves" % (name, len(fx["seeds"])), flush=True)
+    gpu = np.array([model.solve("populationannealing.cpu", {"params": dict(case["params"], seed=s)})
+                    ["solutions"]["cost"] for s in fx["seeds"]])
+    cpu = np.array(fx["best"])
+    p = ks_2samp(gpu, cpu)
+    assert p > 0.05, (p, gpu.mean(), cpu.mean())
+    se = np.sqrt(gpu.var() / len(gpu) + cpu.var() / len(cpu))
+    assert abs(gpu.mean() - cpu.mean()) <= max(0.005 * abs(cpu.mean()), 4 * se), (gpu.mean(), cpu.mean(), se)
+
+
+def test_pa_strategies_find_planted_optima(engine):
+    """Every stepping size, with Poisson and constant-population resampling,
+    finds the planted optima of the N = 16 quality instances (the device
+    counterpart of population_annealing_test.cc:66-152)."""
+    variants = [{}, {"resampling_strategy": "friction_tensor"}, {"resampling_strategy": "energy_variance"},
+                {"resampling_strategy": "constant_culling", "culling_fraction": 0.2},
+                {"constant_population": True},
+                {"resampling_strategy": "constant_culling", "culling_fraction": 0.2, "constant_population": True}]
+    for family in ("wishart_planting_N_16_alpha_0.81_qualitytest", "combined_4_local_N_16_qualitytest"):
+        cases = json.load(open(os.path.join(GOLD, "e2e", family, "energies.json")))["cases"][:2]
+        for case in cases:
+            doc = json.load(open(os.path.join(GOLD, "e2e", family, case["file"])))
+            for v in variants:
+                p = dict({"population": 64, "sweeps": 100, "seed": 42, "beta_start": 0.0, "beta_stop": 5.0}, **v)
+                res = engine.solve("populationannealing.cpu", {"params": p}, doc)
+                assert res["solutions"]["cost"] == case["energy"], (family, case["file"], v)
+                assert oracle_cost(doc, res["solutions"]["configuration"]) == case["energy"]
+                assert res["solutions"]["parameters"]["constant_population"] == bool(v.get("constant_population"))
+
+
 @pytest.mark.parametrize("target", ["simulatedannealing.qiotoolkit", "paralleltempering.qiotoolkit"])
 def test_batch_arrays_match_json_and_single_solves(engine, target):
     """Batched solves of uneven instances (arrays and JSON results) equal the

@@ -81,8 +81,8 @@ def test_energy_and_delta_match_oracle(ctx, name):
     np.testing.assert_allclose(d_dev, d64, rtol=1e-5, atol=1e-5 * np.abs(d64).max())
 
 
-def _run_pair(ctx, kind, terms, R, B, betas, seed=7, ordered=False, lanes=0):
-    op, qp = pair(ctx, kind, terms)
+def _run_pair(ctx, kind, terms, R, B, betas, seed=7, ordered=False, lanes=0, visit=0):
+    op, qp = pair(ctx, kind, terms, visit=visit)
     w = _qmc.Walkers(qp, R, B, seed=seed)
     if lanes:
         w.set_lanes(lanes)
@@ -123,6 +123,39 @@ def test_sa_sweeps_bit_exact_int(ctx, name, B):
     assert np.array_equal(got["accepted"], want["accepted"])
     # every returned configuration's energy equals the oracle cost function
     assert np.array_equal(w.energy(lowest=True), np.array([op.cost(s) for s in got["lowest_states"]]))
+
+
+@pytest.mark.parametrize("name", ["lattice2d_pm1", "er_int", "pubo_int", "ising_4local", "pubo_6local"])
+@pytest.mark.parametrize("B", [1, 8])
+def test_index_levels_make_the_index_order_sweep(ctx, name, B):
+    """Classes = dependency levels of the index order (qmc_problem_create_visit
+    INDEX_LEVELS): every class is independent, each variable's term-sharing
+    neighbours of lower index sit in earlier classes, and the sweeps end in
+    the states, costs and accepted counts of the oracle's index-order sweeps
+    (markov/walker.h:144-153); the lowest states follow the class order."""
+    kind, gen = PROBLEMS[name]
+    betas = np.linspace(0.1, 3.0, 24)
+    op, qp, w, got, want = _run_pair(ctx, kind, gen(), R=37, B=B, betas=betas, visit=_qmc.VISIT_INDEX_LEVELS)
+    order, co = qp.order(w.lanes())
+    cls = np.zeros(op.n, dtype=int)
+    for c in range(len(co) - 1):
+        cls[order[co[c]:co[c + 1]]] = c
+    off, ids, _ = op.edges()
+    for j in range(len(off) - 1):
+        m = sorted(ids[off[j]:off[j + 1]])
+        for a, b in zip(m, m[1:]):
+            assert cls[a] < cls[b]
+    assert np.array_equal(got["lowest"], want["lowest"])
+    assert np.array_equal(got["lowest_states"], want["lowest_states"])
+    # the same sweeps in plain index order
+    opts = po.make_opts(po.RNG_COUNTER, seed=7, arith=arith_of(qp))
+    states = oracle_initial(op, opts, 37)
+    costs = np.array([op.cost(s) for s in states])
+    acc = op.run_sweeps(opts, states, costs, costs.copy(), states.copy(), np.arange(37), 0,
+                        np.repeat(betas, 37))
+    assert np.array_equal(got["states"], states)
+    assert np.array_equal(got["cost"], costs)
+    assert np.array_equal(got["accepted"], acc)
 
 
 @pytest.mark.parametrize("lanes", [64, 128, 256, 512])

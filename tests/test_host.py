@@ -165,6 +165,36 @@ def test_error_codes_are_the_references():
     assert e.value.code == 107
 
 
+def test_pa_configure_strategies_and_errors():
+    """PopulationAnnealing::configure (population_annealing.h:622-751):
+    every resampling_strategy is accepted; an unknown one is InitialConfig
+    104; the constants and the population must be positive (ValueError 107,
+    the matcher's wording, population_annealing_test.cc:213-230)."""
+    from qiotoolkit_amd import solver as S
+    for strat in ("linear_schedule", "geometric_schedule", "friction_tensor", "energy_variance",
+                  "constant_culling"):
+        d = S.describe("populationannealing.cpu",
+                       {"params": {"resampling_strategy": strat, "sweeps": 50, "constant_population": True}})
+        assert d["steps"] == 50
+    cases = [
+        ({"resampling_strategy": "magic"}, 104, "resampling strategy` must be one of"),
+        ({"resampling_strategy": "constant_culling", "culling_fraction": 0}, 107,
+         "parameter `culling_fraction`: must be greater than 0, found 0"),
+        ({"resampling_strategy": "friction_tensor", "friction_tensor_constant": -1}, 107,
+         "friction_tensor_constant"),
+        ({"resampling_strategy": "energy_variance", "initial_culling_fraction": 0}, 107,
+         "initial_culling_fraction"),
+        ({"population": 0, "resampling_strategy": "constant_culling", "culling_fraction": 0.000001}, 107,
+         "parameter `population`: must be greater than 0, found 0"),
+        ({"beta_start": 4, "beta_stop": 2}, 107, "beta"),
+        ({"beta_start": 0, "beta_stop": 2, "resampling_strategy": "geometric_schedule"}, 107, "positive"),
+    ]
+    for params, code, text in cases:
+        with pytest.raises(S.SolverError) as e:
+            S.describe("populationannealing.cpu", {"params": dict(params, sweeps=50)})
+        assert e.value.code == code and text in e.value.message, (params, e.value.code, e.value.message)
+
+
 # ------------------------------------------------- protobuf problem folders
 
 PROTO = os.path.join(os.path.dirname(__file__), "golden", "proto")

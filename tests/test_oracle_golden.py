@@ -450,3 +450,143 @@ def test_pa_step_counter_keeps_an_emptied_population_alive():
         else:
             assert r["R"] == cnt
     assert seen > 40  # P(0) = 1/e
+
+
+def _selector_by_scan(weights, uniforms):
+    """utils::RandomSelector's draws by a linear scan: ids kept in insertion
+    slots, a removed slot refilled by the last one; the pick is the last slot
+    whose prefix weight is <= u * total."""
+    ids = list(range(len(weights)))
+    w = list(weights)
+    out = []
+    for u in uniforms:
+        x = u * sum(w)
+        pos = max(p for p in range(len(w)) if sum(w[:p]) <= x)
+        out.append(ids[pos])
+        ids[pos], w[pos] = ids[-1], w[-1]
+        ids.pop()
+        w.pop()
+    return out
+
+
+@pytest.mark.parametrize("n,seed", [(1, 0), (2, 1), (5, 2), (16, 3), (37, 4), (300, 5)])
+def test_selector_matches_a_linear_scan(n, seed):
+    """The selection tree (random_selector.cc) against a linear scan, with
+    dyadic weights so every sum is exact: same picks, including zero weights
+    and removals of the last slot."""
+    rng = np.random.default_rng(seed)
+    weights = rng.integers(0, 64, n) / 64.0
+    weights[rng.random(n) < 0.2] = 0.0
+    weights[0] = max(weights[0], 1 / 64)
+    k = int(np.count_nonzero(weights))
+    uniforms = rng.integers(0, 2 ** 20, k) / 2.0 ** 20
+    got = po.selector_draw(weights, uniforms)
+    assert list(got) == _selector_by_scan(weights, uniforms)
+    assert len(set(got)) == k and all(weights[i] > 0 for i in got)
+
+
+def test_selector_single_pick_is_the_inverse_cdf():
+    """One pick with u on a fine grid: id i is returned for exactly the
+    fraction w_i / sum(w) of the grid."""
+    w = np.array([3.0, 0.0, 1.0, 4.0, 0.5, 7.5])
+    M = 1 << 12
+    hits = np.zeros(len(w))
+    for j in range(M):
+        hits[po.selector_draw(w, [(j + 0.5) / M])[0]] += 1
+    assert np.array_equal(hits / M, w / w.sum())
+
+
+@pytest.mark.parametrize("eps0,E,scale", [(0.1, 1.0, 1.0), (0.2, 3.0, 0.5), (0.05, 12.0, 2.0), (0.3, 0.25, 1.0)])
+def test_culling_estimate_and_find_delta_beta_closed_form(eps0, E, scale):
+    """Half the population at cost 0, half at E, population = target: the
+    culling estimate is tanh(x/2)/2 - eps0 with x = dbeta E scale, so the root
+    is 2 atanh(2 eps0) / (E scale); find_delta_beta returns the lower end of
+    a 2^-20 bracket of it (population_annealing.h:175-260)."""
+    R = 64
+    shifted = np.array([0.0] * (R // 2) + [E] * (R // 2))
+    for db in (0.0, 0.01, 0.3, 2.0):
+        x = db * E * scale
+        assert abs(po.culling_estimate(shifted, eps0, R, scale, db) - (np.tanh(x / 2) / 2 - eps0)) < 1e-14
+    root = 2 * np.arctanh(2 * eps0) / (E * scale)
+    before = np.zeros(R)
+    got = po.adaptive_delta_beta(3, eps0, before, shifted - 5.0, R, scale)  # unshifted costs
+    assert got <= root < got + 2.0 ** -20 + 1e-12
+    assert (got * 2 ** 20) == int(got * 2 ** 20)  # a multiple of 2^-20
+
+
+def test_friction_tensor_and_energy_variance_stepping():
+    """Costs before (0,1,2,3) and after (0,2,1,3) the sweeps: variances 1.25,
+    covariance 1, sigma 1.25, so zeta = 1.25 s / ln 1.25 for s sweeps
+    (population_annealing.h:347-373, 420-439)."""
+    b = np.array([0.0, 1.0, 2.0, 3.0])
+    a = np.array([0.0, 2.0, 1.0, 3.0])
+    for s in (1, 10):
+        ft = po.adaptive_delta_beta(1, 0.7, b, a, 4, 1.0, s)
+        assert abs(ft - 0.7 * np.sqrt(np.log(1.25) / (1.25 * s))) < 1e-15
+    assert abs(po.adaptive_delta_beta(2, 0.5, b, a, 4) - 0.5 / np.sqrt(1.25)) < 1e-15
+    # sigma = |gamma| (delta = 0), or no correlation (gamma = 0): no step
+    assert po.adaptive_delta_beta(1, 1.0, b, 2 * b + 1, 4) == 0.0
+    assert po.adaptive_delta_beta(1, 1.0, b, np.array([1.0, 0.0, 0.0, 1.0]), 4) == 0.0
+    # zero variance: no step
+    assert po.adaptive_delta_beta(2, 0.5, b, np.full(4, 3.0), 4) == 0.0
+
+
+def test_pa_step_counter_constant_population():
+    """Residual resampling (population_annealing.h:540-578): every count is
+    floor(target w / sum w) or one more, the counts sum to the population
+    size, and the fill picks follow the selection tree with uniforms from
+    Philox (j, step, 1, 5)."""
+    R = target = 48
+    cost, lowest, fam = _pa_inputs(R, seed=12)
+    table = np.linspace(0.0, 0.2, 40)
+    for step in range(30):
+        r = po.pa_step_counter(cost, lowest, fam, step, 0, table, target, 4 * target, 1e-9, 1.0, 21, True, -1e9,
+                               constant_population=True)
+        assert r["R_resampled"] == R and not r["restart"]
+        cnt = np.bincount(r["src"], minlength=R)
+        db = table[step + 1] - table[step]
+        w = np.exp(-db * (cost - cost.min()))
+        rw = target * w / w.sum()
+        fl = np.floor(rw)
+        assert np.all((cnt == fl) | (cnt == fl + 1))
+        fills = R - int(fl.sum())
+        extra = np.flatnonzero(cnt == fl + 1)
+        assert len(extra) == fills
+        u = [float(pcanon(po.philox([j, step, 1, 5], [21, 0]))) for j in range(fills)]
+        assert sorted(po.selector_draw(rw - fl, u)) == sorted(extra)
+
+
+def pcanon(words):
+    return (int(words[0]) + int(words[1]) * 2.0 ** 32) / 2.0 ** 64
+
+
+def test_pa_step_counter_adaptive_strategies_use_their_stepping():
+    R = target = 40
+    cost, lowest, fam = _pa_inputs(R, seed=3)
+    before = cost + np.round(np.random.default_rng(4).normal(0, 3, R))
+    z = np.zeros(2)
+    for strategy, c in ((1, 1.0), (2, 0.5), (3, 0.1)):
+        r = po.pa_step_counter(cost, lowest, fam, 5, 0, z, target, 4 * target, 1e-9, 1.0, 9, True, -1e9,
+                               cost_before=before, strategy=strategy, strategy_const=c, sweeps_per_step=3)
+        assert r["dbeta"] == po.adaptive_delta_beta(strategy, c, before, cost, target, 1.0, 3)
+        assert r["dbeta"] > 0 and r["moved"]
+
+
+@pytest.mark.parametrize("strategy,const", [(0, 0.0), (1, 1.0), (2, 0.5), (3, 0.2)])
+@pytest.mark.parametrize("constant_population", [False, True])
+def test_pa_strategies_reach_planted_optima(strategy, const, constant_population):
+    """Population annealing with every stepping size, with Poisson or
+    constant-population resampling, finds the planted (exhaustively checked)
+    optima of the N = 16 quality instances -- the counterpart of
+    population_annealing_test.cc:66-152 on the reference's toy model."""
+    for family in ("wishart_planting_N_16_alpha_0.81_qualitytest", "combined_4_local_N_16_qualitytest"):
+        cases = json.load(open(os.path.join(GOLD, "e2e", family, "energies.json")))["cases"][:2]
+        for case in cases:
+            cf = json.load(open(os.path.join(GOLD, "e2e", family, case["file"])))["cost_function"]
+            p = po.Problem(cf["type"], cf["terms"])
+            o = po.make_opts(po.RNG_REFERENCE, seed=42, threads=1)
+            r = p.pa(o, 0.0, 5.0, 100, 64, constant_population=constant_population, strategy=strategy,
+                     strategy_const=const)
+            assert r["best"] + p.const_cost == case["energy"], (family, case["file"])
+            if constant_population:
+                assert r["final_population"] == 64
