@@ -54,7 +54,8 @@ struct Smem {
   int32_t abort_flag;
   int32_t any_new_best;
   uint32_t racc[kMaxRep];
-  int32_t nb_tp[kMaxRep];   // lane of the new lowest cost in the class sequence, -1 none
+  int32_t nb_c[kMaxRep];    // class of the sweep's new lowest cost, -1 none
+  int32_t nb_tp[kMaxRep];   // its lane in the class sequence
   int32_t kstar[kMaxRep];   // its round (-1: before the lane's first flip)
   // per-wave partials [byte group][replica in group][wave]
   union {
@@ -365,6 +366,7 @@ k_sweep(const UnitDev* units, SweepArgs a) {
     S.rbest[tid] = U.best_cost[tid];
     S.rbeta[tid] = U.beta[tid];
     S.racc[tid] = 0;
+    S.nb_c[tid] = -1;
   }
   if (tid == 0) {
     S.abort_flag = 0;
@@ -583,48 +585,55 @@ k_sweep(const UnitDev* units, SweepArgs a) {
         S.rcost[tid] = before + (double)run;
         const double cand = before + (double)bestc;
         if (cand < S.rbest[tid]) {
+          // the state there is rebuilt once, at the end of the sweep
           S.rbest[tid] = cand;
+          S.nb_c[tid] = c;
           S.nb_tp[tid] = bt;
           S.kstar[tid] = bk;
           S.any_new_best = 1;
-        } else {
-          S.nb_tp[tid] = -1;
         }
       }
       lds_barrier();
-      if (S.any_new_best) {
-        // lowest state = the class's start state with the flips up to the
-        // minimum applied: every later flip of this class is undone
-        if (active) {
-          uint32_t newmask = 0;
+    }  // classes
+
+    // ---- lowest states reached during this sweep ----
+    // Every position is visited once per sweep and fl holds its flips of
+    // this sweep, so the state at a replica's new minimum (class c*, lane
+    // t*, round k*) has the end-of-sweep bits of the positions visited up to
+    // that point and the start-of-sweep bits (end ^ flips) of the rest.
+    if (S.any_new_best) {
+      if (active) {
+        uint32_t newmask = 0;
 #pragma unroll
-          for (int b = 0; b < NB; ++b)
-            if (b < nb_g && S.nb_tp[8 * g + b] >= 0) newmask |= 1u << b;
-          if (newmask) {
-            for (int q = tp; q < N; q += TP) {
-              uint32_t val = sp[q * G + g];
-              const int rho = q - c0;
-              if (rho >= 0 && rho < nc) {
-                const uint32_t f = fl[q * G + g];
-                const int tq = rho & (TP - 1), kq = rho >> lg;
+        for (int b = 0; b < NB; ++b)
+          if (b < nb_g && S.nb_c[8 * g + b] >= 0) newmask |= 1u << b;
+        if (newmask) {
+          int c = 0;
+          for (int q = tp; q < N; q += TP) {
+            while (q >= class_off[c + 1]) ++c;
+            const int rho = q - class_off[c];
+            const int tq = rho & (TP - 1), kq = rho >> lg;
+            const uint32_t cur = sp[q * G + g];
+            const uint32_t f = fl[q * G + g];
+            uint32_t val = cur;
 #pragma unroll
-                for (int b = 0; b < NB; ++b) {
-                  if ((newmask >> b) & 1u) {
-                    const int ts = S.nb_tp[8 * g + b], ks = S.kstar[8 * g + b];
-                    const bool after = tq > ts || (tq == ts && kq > ks);
-                    if (after) val ^= f & (1u << b);
-                  }
-                }
+            for (int b = 0; b < NB; ++b) {
+              if ((newmask >> b) & 1u) {
+                const int cs = S.nb_c[8 * g + b], ts = S.nb_tp[8 * g + b], ks = S.kstar[8 * g + b];
+                const bool after = c > cs || (c == cs && (tq > ts || (tq == ts && kq > ks)));
+                if (after) val ^= f & (1u << b);
               }
-              const uint32_t old = bst[q * G + g];
-              bst[q * G + g] = (uint8_t)((old & ~newmask) | (val & newmask));
             }
+            const uint32_t old = bst[q * G + g];
+            bst[q * G + g] = (uint8_t)((old & ~newmask) | (val & newmask));
           }
         }
-        lds_barrier();
-        if (tid == 0) S.any_new_best = 0;
       }
-    }  // classes
+      lds_barrier();
+      if (tid < nrep) S.nb_c[tid] = -1;
+      if (tid == 0) S.any_new_best = 0;
+      lds_barrier();
+    }
 
     // ---- parallel tempering exchange (parallel_tempering.h:126-189) ----
     // Every work-group of the ladder computes the same decisions from the
