@@ -57,23 +57,25 @@ struct Smem {
   int32_t nb_c[kMaxRep];    // class of the sweep's new lowest cost, -1 none
   int32_t nb_tp[kMaxRep];   // its lane in the class sequence
   int32_t kstar[kMaxRep];   // its round (-1: before the lane's first flip)
-  // per-wave partials [byte group][replica in group][wave]
+  // per-wave partials [class parity][byte group][replica in group][wave]:
+  // a class's combine reads its parity's slots while the next class fills
+  // the other ones
   union {
-    double d[4][8][kMaxWavesPerGroup];
-    int32_t i[4][8][kMaxWavesPerGroup];
+    double d[2][4][8][kMaxWavesPerGroup];
+    int32_t i[2][4][8][kMaxWavesPerGroup];
   } wsum, wmin;
-  int32_t wlane[4][8][kMaxWavesPerGroup];
-  int32_t wk[4][8][kMaxWavesPerGroup];
+  int32_t wlane[2][4][8][kMaxWavesPerGroup];
+  int32_t wk[2][4][8][kMaxWavesPerGroup];
 };
 
 template <typename AT> struct Slots;
 template <> struct Slots<double> {
-  __device__ static double* sum(Smem& s, int g, int b) { return s.wsum.d[g][b]; }
-  __device__ static double* mn(Smem& s, int g, int b) { return s.wmin.d[g][b]; }
+  __device__ static double* sum(Smem& s, int p, int g, int b) { return s.wsum.d[p][g][b]; }
+  __device__ static double* mn(Smem& s, int p, int g, int b) { return s.wmin.d[p][g][b]; }
 };
 template <> struct Slots<int32_t> {
-  __device__ static int32_t* sum(Smem& s, int g, int b) { return s.wsum.i[g][b]; }
-  __device__ static int32_t* mn(Smem& s, int g, int b) { return s.wmin.i[g][b]; }
+  __device__ static int32_t* sum(Smem& s, int p, int g, int b) { return s.wsum.i[p][g][b]; }
+  __device__ static int32_t* mn(Smem& s, int p, int g, int b) { return s.wmin.i[p][g][b]; }
 };
 
 // ---------------------------------------------------------------------------
@@ -439,6 +441,7 @@ k_sweep(const UnitDev* units, SweepArgs a) {
       const int c0 = class_off[c];
       const int nc = class_off[c + 1] - c0;
       const int K = (nc + TP - 1) >> lg;
+      const int par = c & 1;
       const int sbase = slices ? slice_base[c] : 0;
       const int cn = c + 1 < n_classes ? c + 1 : 0;  // the class after this one
       const int c0n = class_off[cn];
@@ -558,10 +561,10 @@ k_sweep(const UnitDev* units, SweepArgs a) {
             const int wk = __builtin_amdgcn_readlane(kmin[b], wl);
             const AT tot = lane_value(incl, 63);
             if (lane == 0) {
-              Slots<AT>::sum(S, g, b)[wave_in_group] = tot;
-              Slots<AT>::mn(S, g, b)[wave_in_group] = m;
-              S.wlane[g][b][wave_in_group] = wl;
-              S.wk[g][b][wave_in_group] = wk;
+              Slots<AT>::sum(S, par, g, b)[wave_in_group] = tot;
+              Slots<AT>::mn(S, par, g, b)[wave_in_group] = m;
+              S.wlane[par][g][b][wave_in_group] = wl;
+              S.wk[par][g][b][wave_in_group] = wk;
             }
           }
         }
@@ -573,13 +576,13 @@ k_sweep(const UnitDev* units, SweepArgs a) {
         AT run = AT(0), bestc = AT(0);
         int bt = -1, bk = -1;
         for (int w = 0; w < W; ++w) {
-          const AT cnd = run + Slots<AT>::mn(S, gg, bb)[w];
+          const AT cnd = run + Slots<AT>::mn(S, par, gg, bb)[w];
           if (bt < 0 || cnd < bestc) {
             bestc = cnd;
-            bt = w * 64 + S.wlane[gg][bb][w];
-            bk = S.wk[gg][bb][w];
+            bt = w * 64 + S.wlane[par][gg][bb][w];
+            bk = S.wk[par][gg][bb][w];
           }
-          run += Slots<AT>::sum(S, gg, bb)[w];
+          run += Slots<AT>::sum(S, par, gg, bb)[w];
         }
         const double before = S.rcost[tid];
         S.rcost[tid] = before + (double)run;
@@ -593,8 +596,10 @@ k_sweep(const UnitDev* units, SweepArgs a) {
           S.any_new_best = 1;
         }
       }
-      lds_barrier();
+      // no barrier: the next class writes the other parity's slots, and the
+      // one after it passes the next class's barrier first
     }  // classes
+    lds_barrier();
 
     // ---- lowest states reached during this sweep ----
     // Every position is visited once per sweep and fl holds its flips of
