@@ -39,7 +39,7 @@ LDS_PEAK_GBS = 256 * 256 * 2.4
 # WRITE_SIZE in separate runs, scaled by what the same counters report for a
 # plain 512 MiB copy (FETCH_SIZE x 2, WRITE_SIZE x 1):
 # 205462.1 KiB x 2 + 82106 KiB.  Summary: profiles/k_sweep_pt_16cube_hbm_traffic.md
-MEASURED_TRAFFIC_BYTES = {"c2": int((205462.1 * 2 + 82106) * 1024)}
+MEASURED_TRAFFIC_BYTES = {"c2": int((198183.25 * 2 + 82106) * 1024)}
 
 
 def log(*a):
