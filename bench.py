@@ -233,7 +233,7 @@ def run_c4(args):
     return out
 
 
-def c5_requests(rate, duration, seed=5005):
+def c5_requests(rate, duration, seed=5005, pa_extra=None):
     """Poisson arrivals at `rate` per second over `duration` seconds.  70 %
     Ising ER (N log-uniform in [100, 50000], mean degree 8, J ~ N(0,1) fp32),
     30 % random PUBO (V log-uniform in [500, 20000] vars, 10 V terms of degree
@@ -260,7 +260,7 @@ def c5_requests(rate, duration, seed=5005):
         if pa:
             target = "populationannealing.cpu"
             params = {"population": 512, "beta_start": 0.0, "beta_stop": 3.0, "sweeps": 200,
-                      "sweeps_per_replica": 10, "seed": 9000 + k}
+                      "sweeps_per_replica": 10, "seed": 9000 + k, **(pa_extra or {})}
             attempts = 512 * 200 * 10 * n  # nominal: the population fluctuates around 512
         else:
             target = "simulatedannealing.qiotoolkit"
@@ -276,7 +276,12 @@ def run_c5(args):
     the serving loop; latency = finish - scheduled arrival."""
     from qiotoolkit_amd.solver import Server
     t_gen = time.perf_counter()
-    reqs = c5_requests(args.rate, args.duration)
+    pa_extra = {}
+    if args.pa_group:
+        pa_extra["replicas_per_group"] = args.pa_group
+    if args.pa_order:
+        pa_extra["sweep_order"] = args.pa_order
+    reqs = c5_requests(args.rate, args.duration, pa_extra=pa_extra)
     log("c5: %d requests generated in %.1f s (%d ising, %d pubo, %d PA, %d SA)" % (
         len(reqs), time.perf_counter() - t_gen, sum(r["kind"] == "ising" for r in reqs),
         sum(r["kind"] == "pubo" for r in reqs), sum(r["target"].startswith("pop") for r in reqs),
@@ -326,7 +331,7 @@ def run_c5(args):
                                "sweeps, beta 0->3) or SA (64 x 1000 sweeps, beta 0.1->3) 50/50"
                                % (args.rate, args.duration),
                    "requests": len(reqs), "offered_rate_per_s": args.rate, "server": cfg,
-                   "parallelism": "none (1 GPU)"},
+                   "pa_params": pa_extra, "parallelism": "none (1 GPU)"},
         "latency_ms": {"p50": float(np.percentile(lat, 50)) if len(lat) else None,
                        "p99": float(np.percentile(lat, 99)) if len(lat) else None,
                        "max": float(lat.max()) if len(lat) else None},
@@ -381,6 +386,8 @@ def main():
     ap.add_argument("--max-active", type=int, default=64, help="c5: concurrently admitted requests")
     ap.add_argument("--sweeps-per-round", type=int, default=10, help="c5: sweeps per shared launch")
     ap.add_argument("--hbm-budget-gib", type=float, default=0.0, help="c5: device memory for requests")
+    ap.add_argument("--pa-group", type=int, default=0, help="c5: replicas per work-group of PA requests (0 = solver default)")
+    ap.add_argument("--pa-order", default="", help="c5: sweep_order of PA requests (default: the solver's, index)")
     ap.add_argument("--plain-sample", type=int, default=128,
                     help="c4: instances run one per launch for the plain-form rate")
     ap.add_argument("--no-cpu", action="store_true")

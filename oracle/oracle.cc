@@ -979,6 +979,68 @@ int orc_pt(void* pp, const orc_opts* o, const double* values, int n_values, int 
 // `omp parallel for schedule(static,1)`: citizen i draws from thread i % T,
 // threads visiting their citizens in increasing order, which this sequential
 // loop reproduces exactly for T = opts.threads (default 1).
+// ---------------------------------------------------------------------------
+// The reference's test-only toy model (solver/test/test_model.cc:3-38, its
+// default constants): the state is a double v, cost |v - 42|, a random state
+// uniform() * 100, a random transition v + uniform() * 10 - 5; a sweep is 10
+// random transitions (markov/walker.h:168-188, get_sweep_size() = 10), each
+// attempted and, if accepted, applied with cost += cost difference and the
+// strict-< lowest check (walker.h:209-229, 277-280).  Used only to check the
+// oracle's SA and PT loops against the reference's toy-model golden values.
+struct ToyWalker {
+  Pcg32 rng;
+  double v = 0, cost = 0, lowest = 0, beta = 0, T = 0;
+  uint64_t evals = 0, accepted = 0;
+  void init() {
+    v = rng.uniform() * 100.0;
+    cost = std::fabs(v - 42.0);
+    lowest = cost;
+  }
+  void set_temperature(double t) {  // metropolis.h:35-61
+    T = t > 0 ? t : 0.0;
+    beta = t > 0 ? 1.0 / t : std::numeric_limits<double>::infinity();
+  }
+  void set_beta(double b) {
+    beta = b > 0 ? b : 0.0;
+    T = b > 0 ? 1.0 / b : std::numeric_limits<double>::infinity();
+  }
+  void sweep() {
+    for (int t = 0; t < 10; ++t) {
+      const double nv = v + rng.uniform() * 10.0 - 5.0;
+      const double d = std::fabs(nv - 42.0) - std::fabs(v - 42.0);
+      ++evals;
+      if (metropolis_accept(d, beta, [&](int) { return rng(); })) {
+        v = nv;
+        ++accepted;
+        cost += d;
+        if (cost < lowest) lowest = cost;
+      }
+    }
+  }
+};
+
+// Milestone (observe/observable.cc:279-305) of the lowest cost after every
+// step: (steps so far, lowest) whenever the value changes.
+struct Milestones {
+  std::vector<double> steps, values;
+  double total = 0;
+  void record(double value) {
+    total += 1.0;
+    if (values.empty() || value != values.back()) {
+      steps.push_back(total);
+      values.push_back(value);
+    }
+  }
+  int write(double* st, double* val, int cap) const {
+    const int n = (int)values.size();
+    for (int i = 0; i < n && i < cap; ++i) {
+      st[i] = steps[i];
+      val[i] = values[i];
+    }
+    return n;
+  }
+};
+
 // utils::RandomSelector: n weights inserted in order, then k picks without
 // replacement with the given uniforms; out[j] = pick j's id (k <= n).
 int orc_selector_draw(int n, const double* weights, int k, const double* uniforms, int64_t* out) {
@@ -987,6 +1049,101 @@ int orc_selector_draw(int n, const double* weights, int k, const double* uniform
   for (int i = 0; i < n; ++i) sel.insert((size_t)i, weights[i]);
   for (int j = 0; j < k; ++j) out[j] = (int64_t)sel.select_and_remove(uniforms[j]);
   return 0;
+}
+
+// SimulatedAnnealing on the toy model (simulated_annealing.h:45-110 with
+// SteppingSolver::run, stepping_solver.h:112-160): restarts forked from
+// PCG(seed) in order, β linear from beta_start to beta_stop stretched to
+// step_limit - 1 steps, one sweep per step, the global lowest updated every
+// step (a cost_limit is set) and its milestones recorded.  Returns the
+// number of milestones.
+int orc_toy_sa(uint32_t seed, int64_t step_limit, int restarts, double beta_start, double beta_stop,
+               double* ms_steps, double* ms_values, int cap) {
+  if (restarts < 1 || step_limit < 2) return -1;
+  Pcg32 master(seed);
+  std::vector<ToyWalker> w(restarts);
+  for (auto& r : w) r.rng = master.fork();
+  for (auto& r : w) r.init();
+  double lowest = w[0].cost;
+  Sched beta;
+  beta.type = 0;
+  beta.initial = beta_start;
+  beta.final_ = beta_stop;
+  beta.start = 0;
+  beta.stop = (double)std::max<int64_t>(1, step_limit - 1);
+  Milestones ms;
+  for (int64_t step = 0; step < step_limit; ++step) {
+    const double b = beta.value((double)step);
+    for (auto& r : w) {
+      r.set_beta(b);
+      r.sweep();
+    }
+    for (auto& r : w)
+      if (r.lowest < lowest) lowest = r.lowest;
+    ms.record(lowest);
+  }
+  return ms.write(ms_steps, ms_values, cap);
+}
+
+// ParallelTempering on the toy model (parallel_tempering.h:68-189): n
+// temperatures sorted ascending, replica i forked i-th from PCG(seed), each
+// step sweeps_per_replica sweeps per replica, then per replica the observed
+// acceptance rate of the step, its cost and its direction label (UP 0 for the
+// replica at the lowest temperature, DOWN 1 at the highest, carried along by
+// swaps), then swaps of pairs (i, i+1), i = step mod 2, with probability
+// min(1, exp((E_i - E_i+1)(β_i - β_i+1))) from the master's uniform.  Outputs
+// the step averages the reference's observer reports per replica and the
+// lowest-cost milestones; returns the number of milestones.
+int orc_toy_pt(uint32_t seed, int64_t step_limit, int n, const double* temps, int sweeps_per_replica,
+               double* T_out, double* acc_rate, double* avg_cost, double* avg_dir, double* swap_rate,
+               double* ms_steps, double* ms_values, int cap) {
+  if (n < 2 || step_limit < 1) return -1;
+  std::vector<double> t(temps, temps + n);
+  std::sort(t.begin(), t.end());
+  Pcg32 master(seed);
+  std::vector<ToyWalker> w(n);
+  for (auto& r : w) r.rng = master.fork();
+  std::vector<int> dir(n, 0);
+  for (int i = 0; i < n; ++i) {
+    w[i].set_temperature(t[i]);
+    w[i].init();
+  }
+  double lowest = w[0].cost;
+  std::vector<double> acc(n, 0), cst(n, 0), dsum(n, 0), sw(n, 0), swn(n, 0);
+  Milestones ms;
+  for (int64_t step = 0; step < step_limit; ++step) {
+    for (auto& r : w)
+      for (int k = 0; k < sweeps_per_replica; ++k) r.sweep();
+    for (int i = 0; i < n; ++i) {
+      acc[i] += (double)w[i].accepted / (double)w[i].evals;
+      cst[i] += w[i].cost;
+      if (w[i].lowest < lowest) lowest = w[i].lowest;
+      w[i].accepted = w[i].evals = 0;
+      dsum[i] += (double)dir[i];
+    }
+    for (int i = (int)(step & 1); i < n - 1; i += 2) {
+      ToyWalker &a = w[i], &b = w[i + 1];
+      const bool swap = master.uniform() < std::exp((a.cost - b.cost) * (a.beta - b.beta));
+      if (swap) {
+        std::swap(a.v, b.v);
+        std::swap(a.cost, b.cost);
+        std::swap(dir[i], dir[i + 1]);
+      }
+      sw[i] += swap ? 1.0 : 0.0;
+      swn[i] += 1.0;
+    }
+    dir[0] = 0;
+    dir[n - 1] = 1;
+    ms.record(lowest);
+  }
+  for (int i = 0; i < n; ++i) {
+    T_out[i] = w[i].T;
+    acc_rate[i] = acc[i] / (double)step_limit;
+    avg_cost[i] = cst[i] / (double)step_limit;
+    avg_dir[i] = dsum[i] / (double)step_limit;
+    swap_rate[i] = swn[i] > 0 ? sw[i] / swn[i] : -1.0;
+  }
+  return ms.write(ms_steps, ms_values, cap);
 }
 
 // The adaptive stepping size of one step (strategy 1 friction tensor, 2

@@ -109,6 +109,11 @@ int orc_pa_step_counter(int R, const double* cost, const double* lowest, const i
                         int cap, double alpha, double scale, uint32_t seed, int have_best, double best,
                         int32_t* src, int32_t* fam_out, int32_t* iout, double* dout, const double* cost_before,
                         int strategy, double strategy_const, int constant_population, int sweeps_per_step);
+int orc_toy_sa(uint32_t seed, int64_t step_limit, int restarts, double beta_start, double beta_stop,
+               double* ms_steps, double* ms_values, int cap);
+int orc_toy_pt(uint32_t seed, int64_t step_limit, int n, const double* temps, int sweeps_per_replica,
+               double* T_out, double* acc_rate, double* avg_cost, double* avg_dir, double* swap_rate,
+               double* ms_steps, double* ms_values, int cap);
 int orc_selector_draw(int n, const double* weights, int k, const double* uniforms, int64_t* out);
 double orc_adaptive_delta_beta(int strategy, double c, int R, const double* before, const double* after,
                                int target, double scale, int sweeps);

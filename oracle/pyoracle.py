@@ -91,6 +91,9 @@ def lib(variant=""):
     L.orc_pa.argtypes = [C.c_void_p, P(Opts), C.c_int, C.c_double, C.c_double, C.c_int64, C.c_int,
                          C.c_int, C.c_double, C.c_int, P(C.c_double), P(C.c_uint8), P(C.c_int64),
                          P(C.c_int64), C.c_int, C.c_double]
+    L.orc_toy_sa.argtypes = [C.c_uint32, C.c_int64, C.c_int, C.c_double, C.c_double, P(C.c_double),
+                             P(C.c_double), C.c_int]
+    L.orc_toy_pt.argtypes = [C.c_uint32, C.c_int64, C.c_int, P(C.c_double), C.c_int] + [P(C.c_double)] * 7 + [C.c_int]
     L.orc_selector_draw.argtypes = [C.c_int, P(C.c_double), C.c_int, P(C.c_double), P(C.c_int64)]
     L.orc_adaptive_delta_beta.argtypes = [C.c_int, C.c_double, C.c_int, P(C.c_double), P(C.c_double),
                                           C.c_int, C.c_double, C.c_int]
@@ -296,6 +299,30 @@ def pa_step_counter(cost, lowest, family, step, base_step, beta_table, target, c
     return dict(src=src[:Rf], family=fam[:Rf], R=Rf, R_resampled=int(iout[1]), moved=bool(iout[2]),
                 restart=bool(iout[3]), best_src=int(iout[4]), rho=float(dout[0]), dbeta=float(dout[1]),
                 best=float(dout[2]))
+
+
+def toy_sa(seed, step_limit, restarts, beta_start, beta_stop):
+    """SA on the reference's toy model: the lowest-cost milestones [(step, cost)]."""
+    st, va = np.zeros(1000), np.zeros(1000)
+    n = lib().orc_toy_sa(seed, step_limit, restarts, beta_start, beta_stop, _p(st, C.c_double),
+                         _p(va, C.c_double), 1000)
+    if n < 0:
+        raise ValueError("orc_toy_sa rejected the parameters")
+    return list(zip(st[:n], va[:n]))
+
+
+def toy_pt(seed, step_limit, temps, sweeps_per_replica=1):
+    """PT on the reference's toy model: per-replica step averages and milestones."""
+    n = len(temps)
+    t = np.ascontiguousarray(temps, dtype=np.float64)
+    outs = [np.zeros(n) for _ in range(5)]
+    st, va = np.zeros(1000), np.zeros(1000)
+    k = lib().orc_toy_pt(seed, step_limit, n, _p(t, C.c_double), sweeps_per_replica,
+                         *[_p(o, C.c_double) for o in outs], _p(st, C.c_double), _p(va, C.c_double), 1000)
+    if k < 0:
+        raise ValueError("orc_toy_pt rejected the parameters")
+    names = ("T", "acc_rate", "avg_cost", "avg_dir", "swap_rate")
+    return dict(zip(names, outs)), list(zip(st[:k], va[:k]))
 
 
 def selector_draw(weights, uniforms):

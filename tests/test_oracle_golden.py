@@ -590,3 +590,38 @@ def test_pa_strategies_reach_planted_optima(strategy, const, constant_population
             assert r["best"] + p.const_cost == case["energy"], (family, case["file"])
             if constant_population:
                 assert r["final_population"] == 64
+
+
+def test_toy_model_sa_milestones():
+    """The reference's SA on its toy model (simulated_annealing_test.cc:143-180:
+    seed 42, 4 restarts, 100 steps, beta 0.5 -> 10, cost_limit 0): the
+    lowest-cost milestones it prints.  Pins pcg32 seeding and fork, the
+    two-draw uniform, the ln-table accept on continuous cost differences, the
+    stretched linear schedule and the strict lowest tracking."""
+    want = [(1, 9.464013), (2, 0.794104), (3, 0.245903), (4, 0.132852), (6, 0.097936), (7, 0.000760),
+            (84, 0.000191)]
+    got = po.toy_sa(42, 100, 4, 0.5, 10.0)
+    assert [int(s) for s, _ in got] == [s for s, _ in want]
+    for (_, c), (_, w) in zip(got, want):
+        assert abs(c - w) <= 5e-7
+
+
+def test_toy_model_pt_replica_statistics():
+    """The reference's PT on its toy model (parallel_tempering_test.cc:113-190:
+    seed 42, 200 steps, temperatures linear 0.5 -> 0.1, 5 replicas): the
+    per-replica step averages its observer reports and the lowest-cost
+    milestones.  Pins the replica forks, the exchange rule
+    u < exp((E_i - E_i+1)(beta_i - beta_i+1)) on pairs i = step mod 2 with the
+    master's uniforms, the direction labels and the averaging."""
+    stats, ms = po.toy_pt(42, 200, np.linspace(0.5, 0.1, 5))
+    want = {"T": [0.1, 0.2, 0.3, 0.4, 0.5],
+            "acc_rate": [0.053, 0.0765, 0.11, 0.159, 0.208],
+            "avg_cost": [0.334950, 0.448586, 0.426404, 0.460988, 0.596713],
+            "avg_dir": [0.0, 0.515, 0.565, 0.705, 0.995],
+            "swap_rate": [0.63, 0.85, 0.80, 0.88]}
+    for k, v in want.items():
+        assert np.allclose(stats[k][:len(v)], v, rtol=0, atol=5e-7), (k, stats[k])
+    want_ms = [(1, 9.464013), (2, 0.145333), (3, 0.076090), (5, 0.020307), (12, 0.003722), (88, 0.000020)]
+    assert [int(s) for s, _ in ms] == [s for s, _ in want_ms]
+    for (_, c), (_, w) in zip(ms, want_ms):
+        assert abs(c - w) <= 5e-7
