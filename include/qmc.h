@@ -181,14 +181,18 @@ int qmc_walkers_pa_step(qmc_walkers w);
  * {0, 0}.  constant_population != 0: offspring floor(population * w / sum w)
  * plus a fill to the live size drawn without replacement in proportion to
  * the residuals (utils::RandomSelector; pick j's uniform from the counter
- * (j, step, 1, 5)) instead of Poisson offspring. */
+ * (j, step, 1, 5)) instead of Poisson offspring.  exp_repopulation != 0: a
+ * restart doubles the target population first (the reference's feature
+ * switch 2, FEATURE_PA_EXP_REPOPULATION, population_annealing.h:463-470);
+ * past the capacity the target is clamped and `overflow` is set. */
 enum {
   QMC_PA_SCHEDULE = 0,
   QMC_PA_FRICTION_TENSOR = 1,
   QMC_PA_ENERGY_VARIANCE = 2,
   QMC_PA_CONSTANT_CULLING = 3
 };
-int qmc_walkers_pa_strategy(qmc_walkers w, int32_t strategy, double constant, int32_t constant_population);
+int qmc_walkers_pa_strategy(qmc_walkers w, int32_t strategy, double constant, int32_t constant_population,
+                            int32_t exp_repopulation);
 typedef struct {
   int32_t population;  /* live citizens */
   int32_t restarts;
@@ -200,6 +204,8 @@ typedef struct {
   double beta;         /* current beta */
   double rho;          /* family statistic of the last resampling */
   double delta_beta;   /* stepping size of the last step */
+  int32_t target;      /* target population (doubled by restarts under exp_repopulation) */
+  int32_t pad_;
 } qmc_pa_info;
 /* Waits for the stream; best_state (n_vars bytes, may be NULL) receives the
  * state of `best`. */

@@ -1177,12 +1177,14 @@ void orc_poisson_counter(double lam, uint32_t seed, uint32_t step, int n, uint32
 // (fresh states follow for everyone, as init_population draws them).
 // Outputs: src[0..R_final) = the citizen each new slot copies, fam_out =
 // families after the step; iout = {R_final, R_resampled, moved, restart,
-// best_src (-1: no new global lowest)}; dout = {rho, dbeta, new best}.
+// best_src (-1: no new global lowest), target after the step}; dout = {rho,
+// dbeta, new best}.  exp_repopulation: a restart doubles the target first.
 int orc_pa_step_counter(int R, const double* cost, const double* lowest, const int32_t* family,
                         int64_t step, int64_t base_step, const double* beta_table, int n_table, int target,
                         int cap, double alpha, double scale, uint32_t seed, int have_best, double best,
                         int32_t* src, int32_t* fam_out, int32_t* iout, double* dout, const double* cost_before,
-                        int strategy, double strategy_const, int constant_population, int sweeps_per_step) {
+                        int strategy, double strategy_const, int constant_population, int sweeps_per_step,
+                        int exp_repopulation) {
   if (R < 1 || n_table < 2 || cap < target || strategy < 0 || strategy > 3) return -1;
   if ((strategy == 1 || strategy == 2) && !cost_before) return -1;
   // update_lowest_cost over the citizens (first minimum)
@@ -1258,13 +1260,18 @@ int orc_pa_step_counter(int R, const double* cost, const double* lowest, const i
   }
   const bool restart = (double)R / rho < alpha || var == 0.0;
   int Rf = Rn;
+  int tnew = target;
+  if (restart && exp_repopulation) {  // population_annealing.h:463-470
+    tnew = 2 * target;
+    if (tnew > cap) return -2;  // the device clamps and flags; not restated
+  }
   if (restart) {
-    if (Rn < target) {
-      for (int j = Rn; j < target; ++j) src[j] = src[Rn - 1];
-      fam.resize(target);
-      Rf = target;
+    if (Rn < tnew) {
+      for (int j = Rn; j < tnew; ++j) src[j] = src[Rn - 1];
+      fam.resize(tnew);
+      Rf = tnew;
     }
-    for (int j = 0; j < Rf && j < target; ++j) fam[j] = j;
+    for (int j = 0; j < Rf && j < tnew; ++j) fam[j] = j;
     moved = true;
   }
   for (int j = 0; j < Rf; ++j) fam_out[j] = fam[j];
@@ -1273,6 +1280,7 @@ int orc_pa_step_counter(int R, const double* cost, const double* lowest, const i
   iout[2] = moved ? 1 : 0;
   iout[3] = restart ? 1 : 0;
   iout[4] = improved ? barg : -1;
+  iout[5] = tnew;
   dout[0] = rho;
   dout[1] = db;
   dout[2] = improved ? lowest[barg] : best;
@@ -1282,7 +1290,8 @@ int orc_pa_step_counter(int R, const double* cost, const double* lowest, const i
 int orc_pa(void* pp, const orc_opts* o, int geometric, double beta_start, double beta_stop,
            int64_t step_limit, int population, int sweeps_per_replica, double alpha,
            int constant_population, double* best_cost, uint8_t* best_state,
-           int64_t* final_population, int64_t* restarts_out, int strategy, double strategy_const) {
+           int64_t* final_population, int64_t* restarts_out, int strategy, double strategy_const,
+           int exp_repopulation) {
   auto* p = static_cast<Problem*>(pp);
   if (o->rng_mode != ORC_RNG_REFERENCE || population <= 0 || step_limit <= 1) return -1;
   if (strategy < 0 || strategy > 3 || (strategy > 0 && !(strategy_const > 0))) return -1;
@@ -1423,6 +1432,7 @@ int orc_pa(void* pp, const orc_opts* o, int geometric, double beta_start, double
     if (Rd / rho_t < alpha || var_after == 0) {
       cur_beta = b0;
       base_step = step;
+      if (exp_repopulation) target *= 2;  // population_annealing.h:463-470
       init_population();
       n_restarts++;
     } else {

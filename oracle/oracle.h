@@ -108,7 +108,8 @@ int orc_pa_step_counter(int R, const double* cost, const double* lowest, const i
                         int64_t step, int64_t base_step, const double* beta_table, int n_table, int target,
                         int cap, double alpha, double scale, uint32_t seed, int have_best, double best,
                         int32_t* src, int32_t* fam_out, int32_t* iout, double* dout, const double* cost_before,
-                        int strategy, double strategy_const, int constant_population, int sweeps_per_step);
+                        int strategy, double strategy_const, int constant_population, int sweeps_per_step,
+                        int exp_repopulation);
 int orc_toy_sa(uint32_t seed, int64_t step_limit, int restarts, double beta_start, double beta_stop,
                double* ms_steps, double* ms_values, int cap);
 int orc_toy_pt(uint32_t seed, int64_t step_limit, int n, const double* temps, int sweeps_per_replica,
@@ -122,7 +123,8 @@ double orc_culling_estimate(int R, const double* shifted, double epsilon0, int t
 int orc_pa(void* p, const orc_opts* o, int geometric, double beta_start, double beta_stop,
            int64_t step_limit, int population, int sweeps_per_replica, double alpha,
            int constant_population, double* best_cost, uint8_t* best_state,
-           int64_t* final_population, int64_t* restarts, int strategy, double strategy_const);
+           int64_t* final_population, int64_t* restarts, int strategy, double strategy_const,
+           int exp_repopulation);
 
 #ifdef __cplusplus
 }

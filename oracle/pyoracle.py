@@ -56,7 +56,7 @@ def lib(variant=""):
                                       C.c_int64, P(C.c_double), C.c_int, C.c_int, C.c_int, C.c_double,
                                       C.c_double, C.c_uint32, C.c_int, C.c_double, P(C.c_int32),
                                       P(C.c_int32), P(C.c_int32), P(C.c_double), P(C.c_double), C.c_int,
-                                      C.c_double, C.c_int, C.c_int]
+                                      C.c_double, C.c_int, C.c_int, C.c_int]
     L.orc_philox4x32_10.argtypes = [P(C.c_uint32), P(C.c_uint32), P(C.c_uint32)]
     L.orc_uniform_ln.argtypes = [P(C.c_double)]
     L.orc_accept.argtypes = [C.c_double, C.c_double, P(C.c_uint32), P(C.c_int)]
@@ -90,7 +90,7 @@ def lib(variant=""):
                          P(C.c_double), P(C.c_int64)]
     L.orc_pa.argtypes = [C.c_void_p, P(Opts), C.c_int, C.c_double, C.c_double, C.c_int64, C.c_int,
                          C.c_int, C.c_double, C.c_int, P(C.c_double), P(C.c_uint8), P(C.c_int64),
-                         P(C.c_int64), C.c_int, C.c_double]
+                         P(C.c_int64), C.c_int, C.c_double, C.c_int]
     L.orc_toy_sa.argtypes = [C.c_uint32, C.c_int64, C.c_int, C.c_double, C.c_double, P(C.c_double),
                              P(C.c_double), C.c_int]
     L.orc_toy_pt.argtypes = [C.c_uint32, C.c_int64, C.c_int, P(C.c_double), C.c_int] + [P(C.c_double)] * 7 + [C.c_int]
@@ -239,7 +239,7 @@ class Problem:
                     final_states=fs, swap_rate=sr, steps=steps.value)
 
     def pa(self, opts, beta_start, beta_stop, sweeps, population, sweeps_per_replica=1, alpha=2.0,
-           geometric=False, constant_population=False, strategy=0, strategy_const=0.0):
+           geometric=False, constant_population=False, strategy=0, strategy_const=0.0, exp_repopulation=False):
         """strategy: 0 schedule, 1 friction tensor, 2 energy variance, 3 constant
         culling (strategy_const = its constant)."""
         bc = C.c_double()
@@ -248,7 +248,7 @@ class Problem:
         rc = self.L.orc_pa(self.h, C.byref(opts), int(geometric), beta_start, beta_stop, sweeps,
                            population, sweeps_per_replica, alpha, int(constant_population),
                            C.byref(bc), _p(bs, C.c_uint8), C.byref(fp), C.byref(nr), int(strategy),
-                           float(strategy_const))
+                           float(strategy_const), int(exp_repopulation))
         if rc != 0:
             raise ValueError("orc_pa rejected the parameters (rc=%d)" % rc)
         return dict(best=bc.value, best_state=bs, final_population=fp.value, restarts=nr.value)
@@ -275,7 +275,7 @@ def poisson_counter(lam, seed, step, n):
 
 def pa_step_counter(cost, lowest, family, step, base_step, beta_table, target, cap, alpha, scale, seed,
                     have_best, best, cost_before=None, strategy=0, strategy_const=0.0,
-                    constant_population=False, sweeps_per_step=1):
+                    constant_population=False, sweeps_per_step=1, exp_repopulation=False):
     """One counter-mode population annealing step's decisions (orc_pa_step_counter)."""
     cb = None if cost_before is None else np.ascontiguousarray(cost_before, dtype=np.float64)
     cost = np.ascontiguousarray(cost, dtype=np.float64)
@@ -285,19 +285,21 @@ def pa_step_counter(cost, lowest, family, step, base_step, beta_table, target, c
     R = len(cost)
     src = np.zeros(max(cap, target), dtype=np.int32)
     fam = np.zeros(max(cap, target), dtype=np.int32)
-    iout = np.zeros(5, dtype=np.int32)
+    iout = np.zeros(6, dtype=np.int32)
     dout = np.zeros(3)
     rc = lib().orc_pa_step_counter(R, _p(cost, C.c_double), _p(lowest, C.c_double), _p(family, C.c_int32),
                                    step, base_step, _p(table, C.c_double), len(table), target, cap, alpha,
                                    scale, seed & 0xFFFFFFFF, int(have_best), best, _p(src, C.c_int32),
                                    _p(fam, C.c_int32), _p(iout, C.c_int32), _p(dout, C.c_double),
                                    None if cb is None else _p(cb, C.c_double), int(strategy),
-                                   float(strategy_const), int(constant_population), int(sweeps_per_step))
+                                   float(strategy_const), int(constant_population), int(sweeps_per_step),
+                                   int(exp_repopulation))
     if rc != 0:
         raise ValueError("orc_pa_step_counter failed (rc=%d)" % rc)
     Rf = int(iout[0])
     return dict(src=src[:Rf], family=fam[:Rf], R=Rf, R_resampled=int(iout[1]), moved=bool(iout[2]),
-                restart=bool(iout[3]), best_src=int(iout[4]), rho=float(dout[0]), dbeta=float(dout[1]),
+                restart=bool(iout[3]), best_src=int(iout[4]), target=int(iout[5]), rho=float(dout[0]),
+                dbeta=float(dout[1]),
                 best=float(dout[2]))
 
 

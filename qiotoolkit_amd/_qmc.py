@@ -52,6 +52,7 @@ class PaInfo(C.Structure):
         ("population", C.c_int32), ("restarts", C.c_int32), ("overflow", C.c_int32),
         ("have_best", C.c_int32), ("steps", C.c_int64), ("attempts", C.c_int64),
         ("best", C.c_double), ("beta", C.c_double), ("rho", C.c_double), ("delta_beta", C.c_double),
+        ("target", C.c_int32), ("pad_", C.c_int32),
     ]
 
 
@@ -125,7 +126,7 @@ def lib():
     L.qmc_walkers_pa_begin.argtypes = [vp, C.c_int32, C.c_double, C.c_double, P(C.c_double), C.c_int32,
                                        C.c_uint32, C.c_int32]
     L.qmc_walkers_pa_step.argtypes = [vp]
-    L.qmc_walkers_pa_strategy.argtypes = [vp, C.c_int32, C.c_double, C.c_int32]
+    L.qmc_walkers_pa_strategy.argtypes = [vp, C.c_int32, C.c_double, C.c_int32, C.c_int32]
     L.qmc_walkers_set_sweep_order.argtypes = [vp, C.c_int32]
     L.qmc_problem_create_visit.argtypes = [vp, C.c_int, C.c_int32, C.c_int64, P(C.c_int64), P(C.c_int32),
                                            P(C.c_double), C.c_int32, C.c_int32, P(vp)]
@@ -342,8 +343,9 @@ class Walkers:
         _check(lib().qmc_walkers_pa_begin(self.h, population, alpha, scale, _p(t, C.c_double), len(t),
                                           seed & 0xFFFFFFFF, sweeps_per_step))
 
-    def pa_strategy(self, strategy, constant=0.0, constant_population=False):
-        _check(lib().qmc_walkers_pa_strategy(self.h, int(strategy), float(constant), int(bool(constant_population))))
+    def pa_strategy(self, strategy, constant=0.0, constant_population=False, exp_repopulation=False):
+        _check(lib().qmc_walkers_pa_strategy(self.h, int(strategy), float(constant), int(bool(constant_population)),
+                                             int(bool(exp_repopulation))))
 
     def set_sweep_order(self, ordered):
         _check(lib().qmc_walkers_set_sweep_order(self.h, int(bool(ordered))))

@@ -121,12 +121,19 @@ void PopulationAnnealing::configure(const Json& doc) {
 
 PopulationAnnealing::~PopulationAnnealing() = default;
 
+// Poisson resampling keeps the population near the target; exponential
+// repopulation (feature 2) doubles the target at restarts, so it gets more
+// room (past it the solve ends with MemoryLimited).
+int64_t PopulationAnnealing::device_replicas() const {
+  return feature_enabled(2) ? std::max<int64_t>(64 * target_population_, target_population_ + 4096)
+                            : std::max<int64_t>(4 * target_population_, target_population_ + 256);
+}
+
 void PopulationAnnealing::init() {
   if (model_->empty()) return;
   PaState& st = pa_state(impl_);
-  // Poisson resampling keeps the population near the target; room for
-  // fluctuations is reserved up front.
-  st.cap = (int32_t)std::max<int64_t>(4 * target_population_, target_population_ + 256);
+  const bool exp_repop = feature_enabled(2);
+  st.cap = (int32_t)device_replicas();
   const bool seq = sequential();
   const int B = group_ > 0 ? group_ : seq ? 32 : default_group(target_population_);
   device_check(qmc_walkers_create(dmodel_->handle(visit()), st.cap, B, seed_, 0, 0, st.cap, &st.w),
@@ -143,8 +150,9 @@ void PopulationAnnealing::init() {
   device_check(qmc_walkers_pa_begin(st.w, (int32_t)target_population_, alpha_, model_->scale_factor(),
                                     table.data(), (int32_t)table.size(), seed_, sweeps_per_replica_),
                "qmc_walkers_pa_begin");
-  if (strategy_id_ != QMC_PA_SCHEDULE || constant_population_)
-    device_check(qmc_walkers_pa_strategy(st.w, strategy_id_, strategy_const_, constant_population_ ? 1 : 0),
+  if (strategy_id_ != QMC_PA_SCHEDULE || constant_population_ || exp_repop)
+    device_check(qmc_walkers_pa_strategy(st.w, strategy_id_, strategy_const_, constant_population_ ? 1 : 0,
+                                         exp_repop ? 1 : 0),
                  "qmc_walkers_pa_strategy");
   device_check(qmc_walkers_init(st.w, 0), "qmc_walkers_init");
   cur_beta_ = beta_start_;

@@ -78,17 +78,12 @@ struct Done {
 // Device bytes a solve holds while admitted: the problem's term graph and
 // sweep layouts, the walkers' packed spins (current + lowest), per-replica
 // arrays and the plain-state staging used by reads and resampling.
-int64_t footprint(const Model& m, const std::string& target, int64_t replicas) {
+int64_t footprint(const Model& m, const std::string& target, int64_t cap) {
   if (m.empty()) return 0;
   const int64_t N = m.n_vars, M = m.n_terms();
   const int64_t inc = (int64_t)m.term_var.size();
   const int64_t problem = 48 * inc + 24 * M + 24 * N + (1 << 16);
-  int64_t cap = replicas;
-  int64_t staging = replicas * N;
-  if (target.rfind("populationannealing", 0) == 0) {
-    cap = std::max<int64_t>(4 * replicas, replicas + 256);
-    staging = 2 * cap * N;
-  }
+  const int64_t staging = (target.rfind("populationannealing", 0) == 0 ? 2 : 1) * cap * N;
   // current + lowest states: a work-group of B replicas holds ceil(B/8)
   // bytes per position, so at most one byte per replica and position
   const int64_t spins = 2 * cap * (N + 4);
@@ -173,7 +168,7 @@ struct qiotk_server_s {
       try {
         s = create_solver(r.target);
         s->configure(r.params);
-        fp = footprint(r.model, r.target, s->run_replicas());
+        fp = footprint(r.model, r.target, s->device_replicas());
       } catch (const Error& e) {
         auto rq = std::move(pending.front());
         pending.pop_front();

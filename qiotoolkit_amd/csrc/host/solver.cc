@@ -52,6 +52,17 @@ void Solver::configure(const Json& doc) {
     positions_per_lane_ = (int)p.at("positions_per_lane").as_int();
     if (positions_per_lane_ < 0) throw Error(kValueError, "`positions_per_lane` must be >= 0");
   }
+  // Solver::configure (solver.h:205-217): feature lists; disabling wins
+  auto features = [&](const char* key) {
+    std::set<int> out;
+    if (!p.has(key)) return out;
+    const Json& a = p.at(key);
+    if (!a.is_array()) throw Error(kInvalidTypes, std::string("parameter `") + key + "` must be an array");
+    for (size_t i = 0; i < a.size(); ++i) out.insert((int)a[i].as_int());
+    return out;
+  };
+  enabled_features_ = features("enabled_features");
+  for (int f : features("disabled_features")) enabled_features_.erase(f);
   if (p.has("sweep_order")) {
     const std::string o = p.at("sweep_order").as_string();
     if (o == "colored") sweep_order_ = 0;

@@ -8,6 +8,7 @@
 #include <limits>
 #include <memory>
 #include <optional>
+#include <set>
 #include <string>
 #include <vector>
 
@@ -45,6 +46,9 @@ class Solver {
   // Replicas of one independent run as configured (SA restarts, PT ladder
   // length, PA population); known after configure().
   virtual int64_t run_replicas() const { return 1; }
+  // replicas the solve allocates on the device (population annealing
+  // reserves room for its population to fluctuate or grow)
+  virtual int64_t device_replicas() const { return run_replicas(); }
   // Serving form: every round the server calls serve_plan() to add this
   // solve's next device work to the shared batch (false: nothing to add),
   // runs the batch, then serve_advance(), which enqueues what follows the
@@ -105,6 +109,11 @@ class Solver {
   // index-order sweep, run class by class), "sequential" 2 (one lane per
   // replica walking the index order), absent -1 (the solver's default)
   int sweep_order_ = -1;
+  // feature switches of utils/config.h:50-62 named by "enabled_features" /
+  // "disabled_features" (solver.h:205-217); only 2, FEATURE_PA_EXP_REPOPULATION,
+  // changes what this engine does
+  std::set<int> enabled_features_;
+  bool feature_enabled(int f) const { return enabled_features_.count(f) > 0; }
   virtual bool index_by_default() const { return false; }
   int visit() const {
     return sweep_order_ == 1 || (sweep_order_ < 0 && index_by_default()) ? QMC_VISIT_INDEX_LEVELS
@@ -196,6 +205,7 @@ class PopulationAnnealing : public Solver {
   bool serve_done() override;
   qmc_walkers walkers() const override;
   int64_t run_replicas() const override { return target_population_; }
+  int64_t device_replicas() const override;
   ~PopulationAnnealing() override;
 
  protected:

@@ -1045,12 +1045,13 @@ int qmc_walkers_pa_begin(qmc_walkers w, int32_t population, double alpha, double
   HIP_TRY(w->pa_family_next.alloc(cap));
   HIP_TRY(w->pa_src.alloc(cap));
   HIP_TRY(w->pa_count.alloc(cap));
-  HIP_TRY(w->pa_hist.alloc(population));
+  HIP_TRY(w->pa_hist.alloc(cap));
   HIP_TRY(w->pa_plane_cur.alloc(std::max<size_t>((size_t)cap * N, 1)));
   HIP_TRY(w->pa_plane_low.alloc(std::max<size_t>((size_t)cap * N, 1)));
   HIP_TRY(w->pa_best_state.alloc(std::max<size_t>(N, 1)));
   PaDev st{};
   st.R = population;
+  st.target = population;
   st.best_src = -1;
   st.cur_beta = beta_table[0];
   HIP_TRY(upload_async(c, w->pa.p, &st, sizeof st));
@@ -1098,7 +1099,8 @@ int qmc_walkers_pa_begin(qmc_walkers w, int32_t population, double alpha, double
   return upload_units(w);
 }
 
-int qmc_walkers_pa_strategy(qmc_walkers w, int32_t strategy, double constant, int32_t constant_population) {
+int qmc_walkers_pa_strategy(qmc_walkers w, int32_t strategy, double constant, int32_t constant_population,
+                            int32_t exp_repopulation) {
   if (!w) return fail(QMC_INVALID_ARGUMENT, "null handle");
   if (!w->pa_on) return fail(QMC_INVALID_ARGUMENT, "population mode is off");
   if (w->pa_stepped) return fail(QMC_INVALID_ARGUMENT, "the strategy is set before the first step");
@@ -1132,6 +1134,7 @@ int qmc_walkers_pa_strategy(qmc_walkers w, int32_t strategy, double constant, in
   a.strategy = strategy;
   a.strategy_const = strategy == QMC_PA_SCHEDULE ? 0.0 : constant;
   a.constant_population = constant_population ? 1 : 0;
+  a.exp_repopulation = exp_repopulation ? 1 : 0;
   return QMC_OK;
 }
 
@@ -1165,6 +1168,7 @@ int qmc_walkers_pa_info(qmc_walkers w, qmc_pa_info* info, uint8_t* best_state) {
   info->beta = d.cur_beta;
   info->rho = d.rho;
   info->delta_beta = d.delta_beta;
+  info->target = d.target;
   if (best_state && w->prob->N > 0)
     HIP_TRY(hipMemcpy(best_state, w->pa_best_state.p, (size_t)w->prob->N, hipMemcpyDeviceToHost));
   return QMC_OK;

@@ -113,6 +113,8 @@ struct PaDev {
   double best;        // lowest cost any citizen has reached
   double rho;         // family statistic of the last resampling
   double delta_beta;  // stepping size of the last step
+  int32_t target;     // target population (doubles at restarts with exp_repopulation)
+  int32_t pad;
 };
 
 // Residual resampling keeps its selection tree in LDS up to this many
@@ -129,6 +131,7 @@ struct PaArgs {
   uint32_t seed;
   int32_t strategy;           // QMC_PA_SCHEDULE ... QMC_PA_CONSTANT_CULLING
   int32_t constant_population;// residual resampling instead of Poisson offspring
+  int32_t exp_repopulation;   // a restart doubles the target population
   double strategy_const;      // friction constant, culling fraction (initial or constant)
   double* cost0;              // costs before the step's sweeps (friction tensor, energy variance)
   double* tree;               // selection tree above kPaTreeLds citizens (pow2(cap) + 1 entries)
@@ -140,7 +143,7 @@ struct PaArgs {
   int32_t* family;
   int32_t* family_next;
   int32_t* src;
-  int32_t* hist;             // target entries
+  int32_t* hist;             // cap entries (families below the target)
   double* weight;            // cap entries
   int32_t* count;            // cap entries
   double* tcost;             // gathered copies (cap entries each)

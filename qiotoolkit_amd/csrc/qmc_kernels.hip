@@ -1311,12 +1311,12 @@ __device__ __forceinline__ double pa_wave_sum(double v) {
 // CullingEstimator (population_annealing.h:175-212) at dbeta, by one
 // wavefront: weights w_i = exp(-dbeta (E_i - E_min) scale), Q = sum w,
 // result = sum_i max(0, 1 - w_i target / Q) / R - epsilon0.
-__device__ double pa_culling_wave(const PaArgs& A, int R, double mn, double db) {
+__device__ double pa_culling_wave(const PaArgs& A, int R, int target, double mn, double db) {
   const int lane = threadIdx.x & 63;
   double q = 0.0;
   for (int i = lane; i < R; i += 64) q += exp(-db * (A.cost[i] - mn) * A.scale);
   q = pa_wave_sum(q);
-  const double rrq = (double)A.target / q;
+  const double rrq = (double)target / q;
   double s = 0.0;
   for (int i = lane; i < R; i += 64) {
     const double v = 1.0 - exp(-db * (A.cost[i] - mn) * A.scale) * rrq;
@@ -1331,10 +1331,10 @@ __device__ double pa_culling_wave(const PaArgs& A, int R, double mn, double db) 
 // 0 brackets the root (all 40 estimates at once, one per wavefront and
 // round), then bisection by halving steps down to 2^-20 from the lower end.
 // Every thread returns the same value.
-__device__ double pa_find_delta_beta(const PaArgs& A, int R, double mn, double* ce, double* bcast) {
+__device__ double pa_find_delta_beta(const PaArgs& A, int R, int target, double mn, double* ce, double* bcast) {
   const int wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
   for (int j = wv; j <= 40; j += nw) {
-    const double v = pa_culling_wave(A, R, mn, j < 40 ? ldexp(1.0, j - 20) : 0.0);
+    const double v = pa_culling_wave(A, R, target, mn, j < 40 ? ldexp(1.0, j - 20) : 0.0);
     if ((threadIdx.x & 63) == 0) ce[j] = v;
   }
   __syncthreads();
@@ -1352,7 +1352,7 @@ __device__ double pa_find_delta_beta(const PaArgs& A, int R, double mn, double* 
     }
     for (e -= 2; e >= -20; --e) {
       const double upper = lower + ldexp(1.0, e);
-      const double uv = pa_culling_wave(A, R, mn, upper);
+      const double uv = pa_culling_wave(A, R, target, mn, upper);
       if ((lower_value < 0) == (uv < 0)) {
         lower = upper;
         lower_value = uv;
@@ -1464,6 +1464,7 @@ __global__ void __launch_bounds__(1024) k_pa_step(PaArgs A) {
   PaDev* P = A.pa;
   const int tid = threadIdx.x, bd = blockDim.x;
   const int R = P->R;
+  const int target = P->target;
   const int64_t step = P->step;
   const double inf = __longlong_as_double(0x7ff0000000000000ll);
 
@@ -1497,7 +1498,7 @@ __global__ void __launch_bounds__(1024) k_pa_step(PaArgs A) {
     const int64_t last = A.n_table - 1;
     db = A.beta_table[k + 1 < last ? k + 1 : last] - A.beta_table[k < last ? k : last];
   } else if (A.strategy == QMC_PA_CONSTANT_CULLING) {
-    db = pa_find_delta_beta(A, R, mn, ce, &rd[0]);
+    db = pa_find_delta_beta(A, R, target, mn, ce, &rd[0]);
   } else {
     // sigma = sqrt(var_before var_after), gamma = cov(before, after)
     double sb = 0.0;
@@ -1537,7 +1538,7 @@ __global__ void __launch_bounds__(1024) k_pa_step(PaArgs A) {
     if (A.constant_population) {
       int fl = 0;
       for (int i = tid; i < R; i += bd) {
-        const double rw = (double)A.target * A.weight[i] / total;
+        const double rw = (double)target * A.weight[i] / total;
         const int c = (int)floor(rw);
         A.count[i] = c;
         A.weight[i] = rw - (double)c;
@@ -1546,7 +1547,7 @@ __global__ void __launch_bounds__(1024) k_pa_step(PaArgs A) {
       const int floors = pa_reduce(fl, AddOp(), ri);
       if (floors < R) pa_residual_fill(A, R, R - floors, (uint32_t)step, s_tree, s_ids);
     } else {
-      const double rrq = (double)A.target / total;
+      const double rrq = (double)target / total;
       for (int i = tid; i < R; i += bd) {
         const double mean = rrq * A.weight[i];
         A.count[i] = mean > 0.0 ? (int)pa_poisson(mean, (uint32_t)i, (uint32_t)step, A.seed) : 0;
@@ -1582,15 +1583,15 @@ __global__ void __launch_bounds__(1024) k_pa_step(PaArgs A) {
       if (tid == 0) P->err = 1;
       current = A.cap;
     }
-    for (int f = tid; f < A.target; f += bd) A.hist[f] = 0;
+    for (int f = tid; f < target; f += bd) A.hist[f] = 0;
     __syncthreads();
     for (int d = tid; d < current; d += bd) {
       const int f = A.family_next[d];
-      if ((unsigned)f < (unsigned)A.target) atomicAdd(&A.hist[f], 1);
+      if ((unsigned)f < (unsigned)target) atomicAdd(&A.hist[f], 1);
     }
     __syncthreads();
     long long sq = 0;
-    for (int f = tid; f < A.target; f += bd) {
+    for (int f = tid; f < target; f += bd) {
       const long long h = A.hist[f];
       sq += h * h;
     }
@@ -1602,6 +1603,16 @@ __global__ void __launch_bounds__(1024) k_pa_step(PaArgs A) {
 
   const bool restart = (double)R / rho < A.alpha || var == 0.0;
   int Rf = Rn;
+  // a restart with exp_repopulation doubles the target first
+  // (population_annealing.h:463-470), within the capacity
+  int tnew = target;
+  if (restart && A.exp_repopulation) {
+    tnew = 2 * target;
+    if (tnew > A.cap) {
+      if (tid == 0) P->err = 1;
+      tnew = A.cap;
+    }
+  }
   if (restart) {
     if (!moved)
       for (int i = tid; i < Rn; i += bd) {
@@ -1609,12 +1620,12 @@ __global__ void __launch_bounds__(1024) k_pa_step(PaArgs A) {
         A.family_next[i] = A.family[i];
       }
     __syncthreads();
-    if (Rn < A.target) {  // top up with copies of the last citizen
+    if (Rn < tnew) {  // top up with copies of the last citizen
       const int lastsrc = A.src[Rn - 1];
-      for (int i = Rn + tid; i < A.target; i += bd) A.src[i] = lastsrc;
-      Rf = A.target;
+      for (int i = Rn + tid; i < tnew; i += bd) A.src[i] = lastsrc;
+      Rf = tnew;
     }
-    for (int i = tid; i < Rf && i < A.target; i += bd) A.family_next[i] = i;
+    for (int i = tid; i < Rf && i < tnew; i += bd) A.family_next[i] = i;
     moved = true;
   }
   __syncthreads();
@@ -1636,6 +1647,7 @@ __global__ void __launch_bounds__(1024) k_pa_step(PaArgs A) {
     P->restart = restart ? 1 : 0;
     P->rho = rho;
     P->delta_beta = db;
+    P->target = tnew;
     P->step = step + 1;
     if (restart) {
       P->cur_beta = A.beta_start;

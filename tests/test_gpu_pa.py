@@ -22,9 +22,9 @@ pytestmark = pytest.mark.gpu
 
 
 def _steps_match_oracle(ctx, alpha, pop, B, strategy=0, const=0.0, constant_population=False, S=25, side=8,
-                        ordered=False, visit=_qmc.VISIT_INDEX_LEVELS):
+                        ordered=False, visit=_qmc.VISIT_INDEX_LEVELS, exp_repopulation=False, cap=0):
     op, qp = pair(ctx, "ising", wl.lattice2d(side, seed=4), visit=visit)
-    cap = max(4 * pop, pop + 256)
+    cap = cap or max(4 * pop, pop + 256)
     seed = 13
     adaptive = strategy != _qmc.PA_SCHEDULE
     table = np.zeros(2) if adaptive else np.linspace(0.0, 3.0, S + 2)
@@ -36,14 +36,15 @@ def _steps_match_oracle(ctx, alpha, pop, B, strategy=0, const=0.0, constant_popu
     sweep_opts = po.make_opts(po.RNG_COUNTER, seed=seed, arith=arith_of(qp),
                               order=None if ordered else qp.order(w.lanes())[0])
     w.pa_begin(pop, alpha, 1.0, table, seed, 1)
-    if adaptive or constant_population:
-        w.pa_strategy(strategy, const, constant_population)
+    if adaptive or constant_population or exp_repopulation:
+        w.pa_strategy(strategy, const, constant_population, exp_repopulation)
     w.init(0)
     opts = po.make_opts(po.RNG_COUNTER, seed=seed)
     fam = np.arange(pop, dtype=np.int32)
     R, base, epoch, restarts = pop, 0, 0, 0
     have_best, best, best_state = False, 0.0, None
     cur_beta = table[0]
+    target = pop
     for step in range(S):
         pre = w.read()
         w.sweep(step, None, n_sweeps=1)
@@ -53,9 +54,10 @@ def _steps_match_oracle(ctx, alpha, pop, B, strategy=0, const=0.0, constant_popu
                       np.arange(R), step, pre["beta"][:R])
         for k, v in want.items():
             assert np.array_equal(before[k][:R], v), (step, k)
-        o = po.pa_step_counter(before["cost"][:R], before["lowest"][:R], fam, step, base, table, pop, cap,
+        o = po.pa_step_counter(before["cost"][:R], before["lowest"][:R], fam, step, base, table, target, cap,
                                alpha, 1.0, seed, have_best, best, cost_before=pre["cost"][:R], strategy=strategy,
-                               strategy_const=const, constant_population=constant_population)
+                               strategy_const=const, constant_population=constant_population,
+                               exp_repopulation=exp_repopulation)
         w.pa_step()
         info = w.pa_info()
         after = w.read()
@@ -68,6 +70,8 @@ def _steps_match_oracle(ctx, alpha, pop, B, strategy=0, const=0.0, constant_popu
         else:
             assert abs(info["delta_beta"] - o["dbeta"]) <= 1e-12 * abs(o["dbeta"]), step
         assert info["population"] == Rf, step
+        assert info["target"] == o["target"], step
+        target = o["target"]
         if constant_population and not o["restart"]:
             assert Rf == R
         if o["best_src"] >= 0:
@@ -129,6 +133,14 @@ def test_device_pa_strategies_match_oracle(ctx, strategy, const, constant_popula
     (qmc_walkers_pa_strategy) step by step against orc_pa_step_counter."""
     print("pa strategy=%d const=%g constant_population=%d" % (strategy, const, constant_population), flush=True)
     _steps_match_oracle(ctx, 2.0, 48, 4, strategy, const, constant_population)
+
+
+def test_device_pa_exponential_repopulation(ctx):
+    """Feature switch 2 (FEATURE_PA_EXP_REPOPULATION): every restart doubles
+    the target population before the fresh citizens are drawn."""
+    print("pa exp repopulation", flush=True)
+    restarts = _steps_match_oracle(ctx, 2.0, 16, 4, exp_repopulation=True, cap=4096, S=25)
+    assert restarts >= 1
 
 
 def test_device_pa_constant_population_beyond_the_lds_tree(ctx):

@@ -214,7 +214,10 @@ def test_pa_strategies_find_planted_optima(engine):
     variants = [{}, {"resampling_strategy": "friction_tensor"}, {"resampling_strategy": "energy_variance"},
                 {"resampling_strategy": "constant_culling", "culling_fraction": 0.2},
                 {"constant_population": True},
-                {"resampling_strategy": "constant_culling", "culling_fraction": 0.2, "constant_population": True}]
+                {"resampling_strategy": "constant_culling", "culling_fraction": 0.2, "constant_population": True},
+                # feature switch 2 as population_annealing_test.cc:66-101 enable it
+                {"resampling_strategy": "friction_tensor", "enabled_features": [2]},
+                {"resampling_strategy": "energy_variance", "enabled_features": [2]}]
     for family in ("wishart_planting_N_16_alpha_0.81_qualitytest", "combined_4_local_N_16_qualitytest"):
         cases = json.load(open(os.path.join(GOLD, "e2e", family, "energies.json")))["cases"][:2]
         for case in cases:

@@ -625,3 +625,25 @@ def test_toy_model_pt_replica_statistics():
     assert [int(s) for s, _ in ms] == [s for s, _ in want_ms]
     for (_, c), (_, w) in zip(ms, want_ms):
         assert abs(c - w) <= 5e-7
+
+
+def test_pa_exponential_repopulation():
+    """Feature switch 2 (population_annealing.h:463-470): each restart
+    doubles the target, so the final population is the initial one times a
+    power of two at least 2^restarts / (population fluctuation), and the
+    counter-mode step reports the doubled target."""
+    fam = os.path.join(GOLD, "e2e", "wishart_planting_N_16_alpha_0.81_qualitytest")
+    first = json.load(open(os.path.join(fam, "energies.json")))["cases"][0]["file"]
+    cf = json.load(open(os.path.join(fam, first)))["cost_function"]
+    p = po.Problem(cf["type"], cf["terms"])
+    o = po.make_opts(po.RNG_REFERENCE, seed=42, threads=1)
+    r = p.pa(o, 0.0, 5.0, 60, 4, exp_repopulation=True)
+    plain = p.pa(po.make_opts(po.RNG_REFERENCE, seed=42, threads=1), 0.0, 5.0, 60, 4)
+    assert r["restarts"] >= 1
+    assert r["final_population"] >= 4 * 2 ** r["restarts"] // 2
+    assert r["final_population"] > plain["final_population"] or r["restarts"] == 0
+    flat = np.full(6, -3.0)
+    st = po.pa_step_counter(flat, flat, np.arange(6, dtype=np.int32), 2, 0, np.linspace(0, 1, 5), 6, 64, 2.0, 1.0,
+                            3, True, -3.0, exp_repopulation=True)
+    assert st["restart"] and st["target"] == 12 and st["R"] == 12
+    assert list(st["family"]) == list(range(12))
