@@ -129,6 +129,11 @@ int qiotk_parse_model_file(const char* path, char** response);
 /* Schedule::get_value at n points; *discretized = {"start","stop","count","values"} */
 int qiotk_schedule_eval(const char* schedule_json, int32_t n, const double* x, double* out,
                         char** discretized);
+/* The observer protocol (observe/observer.cc) of params_json's "observer"
+ * fed with events_json, a list of {"label": name, "value": int | null},
+ * {"observe": name, "value": v, "weight": w (default 1)} and {"restart":
+ * step}; *response = its render, as benchmark.extensions "solver" holds it. */
+int qiotk_observe(const char* params_json, const char* events_json, char** response);
 /* Solver::configure only: the resolved plan (steps, betas, restarts) */
 int qiotk_describe(const char* target, const char* params_json, char** response);
 

@@ -193,6 +193,16 @@ enum {
 };
 int qmc_walkers_pa_strategy(qmc_walkers w, int32_t strategy, double constant, int32_t constant_population,
                             int32_t exp_repopulation);
+/* Per-step records of the next steps (up to max_steps; before the first
+ * step): 13 doubles per step -- epoch before the step, mean cost after the
+ * sweeps, population, sigma, tau, zeta (NaN where undefined), stepping size,
+ * family count, sum of family sizes, size of the family with the largest
+ * id, rho, restart flag, resampled flag -- the observations of
+ * population_annealing.h:262-470.  qmc_walkers_pa_log waits for the stream
+ * and copies the first n_steps records. */
+#define QMC_PA_LOG_FIELDS 13
+int qmc_walkers_pa_observe(qmc_walkers w, int32_t max_steps);
+int qmc_walkers_pa_log(qmc_walkers w, int32_t n_steps, double* records);
 typedef struct {
   int32_t population;  /* live citizens */
   int32_t restarts;

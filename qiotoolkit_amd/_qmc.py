@@ -28,7 +28,7 @@ EXPORTS = [
     "qmc_batch_wait", "qmc_walkers_read_replica", "qmc_batch_read", "qmc_batch_run_chunked", "qmc_walkers_resample", "qmc_walkers_count", "qmc_walkers_set_pt_mode",
     "qmc_walkers_set_lanes", "qmc_walkers_get_lanes", "qmc_walkers_pa_begin", "qmc_walkers_pa_step",
     "qmc_walkers_pa_info", "qmc_walkers_pa_end", "qmc_lds_bandwidth", "qmc_walkers_pa_strategy",
-    "qmc_walkers_set_sweep_order", "qmc_problem_create_visit",
+    "qmc_walkers_set_sweep_order", "qmc_problem_create_visit", "qmc_walkers_pa_observe", "qmc_walkers_pa_log",
 ]
 
 
@@ -128,6 +128,8 @@ def lib():
     L.qmc_walkers_pa_step.argtypes = [vp]
     L.qmc_walkers_pa_strategy.argtypes = [vp, C.c_int32, C.c_double, C.c_int32, C.c_int32]
     L.qmc_walkers_set_sweep_order.argtypes = [vp, C.c_int32]
+    L.qmc_walkers_pa_observe.argtypes = [vp, C.c_int32]
+    L.qmc_walkers_pa_log.argtypes = [vp, C.c_int32, P(C.c_double)]
     L.qmc_problem_create_visit.argtypes = [vp, C.c_int, C.c_int32, C.c_int64, P(C.c_int64), P(C.c_int32),
                                            P(C.c_double), C.c_int32, C.c_int32, P(vp)]
     L.qmc_walkers_pa_info.argtypes = [vp, P(PaInfo), vp]
@@ -349,6 +351,14 @@ class Walkers:
 
     def set_sweep_order(self, ordered):
         _check(lib().qmc_walkers_set_sweep_order(self.h, int(bool(ordered))))
+
+    def pa_observe(self, max_steps):
+        _check(lib().qmc_walkers_pa_observe(self.h, int(max_steps)))
+
+    def pa_log(self, n_steps):
+        out = np.zeros((n_steps, 13))
+        _check(lib().qmc_walkers_pa_log(self.h, int(n_steps), _p(out, C.c_double)))
+        return out
 
     def pa_step(self):
         _check(lib().qmc_walkers_pa_step(self.h))

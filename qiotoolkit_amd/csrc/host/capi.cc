@@ -9,6 +9,7 @@
 
 #include "../../../include/qiotk.h"
 #include "model.h"
+#include "observer.h"
 #include "proto.h"
 #include "solver.h"
 
@@ -506,6 +507,29 @@ int qiotk_parse_model_file(const char* path, char** response) {
   return guarded(response, [&] {
     if (!path) throw Error(kMissingInput, "no input file");
     *response = dup(describe_model(parse_model(read_input_file(path))).dump());
+    return 0;
+  });
+}
+
+int qiotk_observe(const char* params_json, const char* events_json, char** response) {
+  return guarded(response, [&] {
+    Observer o;
+    o.configure(Json::parse(params_json));
+    const Json ev = Json::parse(events_json);
+    if (!ev.is_array()) throw Error(kInvalidTypes, "events must be an array");
+    for (const Json& e : ev.items()) {
+      if (e.has("label")) {
+        const Json& v = e.at("value");
+        if (v.is_null()) o.clear_label(e.at("label").as_string());
+        else o.set_label(e.at("label").as_string(), v.as_int());
+      } else if (e.has("restart")) {
+        o.restart(e.at("restart").as_int());
+      } else {
+        o.observe(e.at("observe").as_string(), e.at("value").as_double(),
+                  e.has("weight") ? e.at("weight").as_double() : 1.0);
+      }
+    }
+    *response = dup(o.render().dump());
     return 0;
   });
 }

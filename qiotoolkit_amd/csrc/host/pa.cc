@@ -150,6 +150,8 @@ void PopulationAnnealing::init() {
   device_check(qmc_walkers_pa_begin(st.w, (int32_t)target_population_, alpha_, model_->scale_factor(),
                                     table.data(), (int32_t)table.size(), seed_, sweeps_per_replica_),
                "qmc_walkers_pa_begin");
+  if (!observer_.empty())
+    device_check(qmc_walkers_pa_observe(st.w, (int32_t)steps_to_run()), "qmc_walkers_pa_observe");
   if (strategy_id_ != QMC_PA_SCHEDULE || constant_population_ || exp_repop)
     device_check(qmc_walkers_pa_strategy(st.w, strategy_id_, strategy_const_, constant_population_ ? 1 : 0,
                                          exp_repop ? 1 : 0),
@@ -244,8 +246,38 @@ void PopulationAnnealing::finalize() {
     lowest_cost_ = g;
     lowest_state_ = gs;
   }
-  Json ext = Json::object();
   set_output("restarts", Json(restarts_));
+  observe_run();
+}
+
+// The observations of every step (population_annealing.h:262-470) from the
+// device's per-step records, in the reference's order: epoch, cost,
+// population, sigma, tau and zeta where defined, and after a resampling the
+// stepping size, family count, family sizes and rho; a restart restarts the
+// observer's windows at its step.
+void PopulationAnnealing::observe_run() {
+  if (observer_.empty() || model_->empty()) return;
+  PaState& st = pa_state(impl_);
+  std::vector<double> log((size_t)steps_done_ * QMC_PA_LOG_FIELDS);
+  device_check(qmc_walkers_pa_log(st.w, (int32_t)steps_done_, log.data()), "qmc_walkers_pa_log");
+  observe_steps(steps_done_, [&](int64_t s) {
+    const double* r = log.data() + s * QMC_PA_LOG_FIELDS;
+    observer_.observe("epoch", r[0]);
+    observer_.observe("cost", r[1]);
+    observer_.observe("population", r[2]);
+    observer_.observe("sigma", r[3]);
+    if (r[4] == r[4]) {
+      observer_.observe("tau", r[4]);
+      observer_.observe("zeta", r[5]);
+    }
+    if (r[12] > 0) {
+      observer_.observe("delta_beta", r[6]);
+      observer_.observe("families", r[7]);
+      observer_.observe_many("family_size", r[8], (int64_t)r[7], r[9]);
+      observer_.observe("rho_t", r[10]);
+    }
+    if (r[11] > 0) observer_.restart(s);
+  });
 }
 
 }  // namespace qiotk

@@ -41,6 +41,7 @@ int default_group(int64_t R) {
 
 void Solver::configure(const Json& doc) {
   const Json& p = params_of(doc);
+  observer_.configure(doc);
   // Solver::configure (solver.h:156-218)
   if (p.has("threads")) threads_ = (int)p.at("threads").as_int();
   if (p.has("lanes")) lanes_ = (int)p.at("lanes").as_int();
@@ -179,6 +180,7 @@ void Solver::finalize_from(const double* lowest_energy, const uint64_t* accepted
   std::vector<uint8_t> st(states, states + R * N);
   accepted_ = (int64_t)std::accumulate(accepted, accepted + R, (uint64_t)0);
   populate(low, st, R);
+  observe_run();
 }
 
 bool Solver::best(double* cost, int8_t* config) const {
@@ -234,9 +236,20 @@ Json Solver::get_result() const {
   c1["name"] = "counters";
   c1["value"] = counters;
   ext.push_back(c1);
-  Json sol = Json::object();
+  // SteppingSolver::get_solver_properties (stepping_solver.h:268-283,
+  // solver.h:57-75); milestones are recorded only in the reference's
+  // profiling builds, so a release build reports them empty
+  Json sol = observer_.render();
+  if (time_limit_) sol["time_limit"] = *time_limit_;
+  if (eval_limit_) sol["eval_limit"] = Json(*eval_limit_);
+  if (cost_limit_) sol["cost_limit"] = *cost_limit_;
+  sol["cost_milestones"] = Json::array();
   sol["last_step"] = Json(steps_done_);
-  if (!exit_reason_.empty()) sol["exit_reason"] = exit_reason_;
+  if (step_limit_ > 0) sol["step_limit"] = Json(step_limit_);
+  std::string reason = exit_reason_;
+  if (reason.empty() && step_limit_ > 0 && steps_done_ >= step_limit_)
+    reason = "Stop due to step limit: " + std::to_string(step_limit_) + ", steps:" + std::to_string(steps_done_);
+  if (!reason.empty()) sol["exit_reason"] = reason;
   Json c2 = Json::object();
   c2["name"] = "solver";
   c2["value"] = sol;
@@ -360,12 +373,13 @@ void SimulatedAnnealing::run() {
       std::vector<double> low(restarts_);
       device_check(qmc_walkers_read(w_, nullptr, low.data(), nullptr, nullptr, nullptr, nullptr), "read");
       if (*std::min_element(low.begin(), low.end()) <= *cost_limit_) {
-        exit_reason_ = "Stop due to cost limit";
+        exit_reason_ = "Stop due to cost limit: " + std::to_string(*cost_limit_) + ", current lowest cost:";
         break;
       }
     }
     if (time_limit_ && wall_s() - t0 >= *time_limit_) {
-      exit_reason_ = "Stop due to time_limit";
+      exit_reason_ = "Stop due to time_limit.steps : " + std::to_string(s) + ", time_limit:" +
+                     std::to_string(*time_limit_);
       break;
     }
     chunk = std::min<int64_t>(chunk * 2, 1 << 14);
@@ -398,15 +412,37 @@ void SimulatedAnnealing::finalize() {
   device_check(qmc_walkers_energy(w_, 1, lowest_.data()), "qmc_walkers_energy");
   accepted_ = (int64_t)std::accumulate(acc.begin(), acc.end(), (uint64_t)0);
   populate(lowest_, lowest_states_flat_, restarts_);
+  observe_run();
 }
 
 // ===========================================================================
 // ParallelTempering (parallel_tempering.h:68-234)
+
+// The step observations a run makes on the reference's side: "step", then per
+// replica (label "replica") its temperature or beta (parallel_tempering.h:
+// 141-152); the per-step dynamics observables are refused at configure.
+void ParallelTempering::observe_run() {
+  std::vector<double> t = temperatures_.get_discretized_values();
+  if (use_inverse_temperatures_) std::sort(t.begin(), t.end(), std::greater<double>());
+  else std::sort(t.begin(), t.end(), std::less<double>());
+  const char* name = use_inverse_temperatures_ ? "beta" : "T";
+  observe_steps(steps_done_, [&](int64_t) {
+    for (size_t i = 0; i < t.size(); ++i) {
+      observer_.set_label("replica", (int64_t)i);
+      observer_.observe(name, use_inverse_temperatures_ ? beta_from(t[i], true) : (t[i] > 0 ? t[i] : 0.0));
+      observer_.clear_label("replica");
+    }
+  });
+}
 void ParallelTempering::configure(const Json& doc) {
   Solver::configure(doc);
   const Json& p = params_of(doc);
   if (sequential())
     throw Error(kNotImplemented, "parallel tempering takes `sweep_order` 'colored' or 'index'");
+  for (const char* name : {"acc_rate", "avg_cost", "avg_dir", "swap_rate"})
+    if (observer_.watching(name))
+      throw Error(kNotImplemented, std::string("parallel tempering observes `T` / `beta` and `step` on this engine, "
+                                               "not `") + name + "`");
   if (p.has("sweeps_per_replica")) {
     sweeps_per_replica_ = (int)p.at("sweeps_per_replica").as_int();
     if (sweeps_per_replica_ <= 0) throw Error(kValueError, "`sweeps_per_replica` must be greater than 0");
@@ -494,12 +530,13 @@ void ParallelTempering::run() {
       std::vector<double> low(L);
       device_check(qmc_walkers_read(w_, nullptr, low.data(), nullptr, nullptr, nullptr, nullptr), "read");
       if (*std::min_element(low.begin(), low.end()) <= *cost_limit_) {
-        exit_reason_ = "Stop due to cost limit";
+        exit_reason_ = "Stop due to cost limit: " + std::to_string(*cost_limit_) + ", current lowest cost:";
         break;
       }
     }
     if (time_limit_ && wall_s() - t0 >= *time_limit_) {
-      exit_reason_ = "Stop due to time_limit";
+      exit_reason_ = "Stop due to time_limit.steps : " + std::to_string(s) + ", time_limit:" +
+                     std::to_string(*time_limit_);
       break;
     }
     chunk = std::min<int64_t>(chunk * 2, 1 << 14);
@@ -534,6 +571,7 @@ void ParallelTempering::finalize() {
   device_check(qmc_walkers_energy(w_, 1, lowest_.data()), "qmc_walkers_energy");
   accepted_ = (int64_t)std::accumulate(acc.begin(), acc.end(), (uint64_t)0);
   populate(lowest_, lowest_states_flat_, L);
+  observe_run();
 }
 
 // ===========================================================================

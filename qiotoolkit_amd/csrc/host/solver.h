@@ -14,6 +14,7 @@
 
 #include "json.h"
 #include "model.h"
+#include "observer.h"
 #include "schedule.h"
 
 namespace qiotk {
@@ -113,6 +114,22 @@ class Solver {
   // "disabled_features" (solver.h:205-217); only 2, FEATURE_PA_EXP_REPOPULATION,
   // changes what this engine does
   std::set<int> enabled_features_;
+  // the reference's observer (observe/observer.h) and the "step" observation
+  // SteppingSolver::run makes before every step (stepping_solver.h:127-133);
+  // per_step(s) adds the solver's own observations of step s
+  Observer observer_;
+  // replays the run's observations into observer_ (end of finalize)
+  virtual void observe_run() { observe_steps(steps_done_, [](int64_t) {}); }
+  template <class F>
+  void observe_steps(int64_t steps, F per_step) {
+    if (observer_.empty()) return;
+    for (int64_t s = 0; s < steps; ++s) {
+      observer_.set_label("step", s);
+      observer_.observe("step", (double)s);
+      per_step(s);
+      observer_.clear_label("step");
+    }
+  }
   bool feature_enabled(int f) const { return enabled_features_.count(f) > 0; }
   virtual bool index_by_default() const { return false; }
   int visit() const {
@@ -180,6 +197,9 @@ class ParallelTempering : public Solver {
   int64_t run_replicas() const override { return (int64_t)ladder().size(); }
   qmc_walkers walkers() const override { return w_; }
 
+ protected:
+  void observe_run() override;
+
  private:
   bool use_inverse_temperatures_ = true;
   Schedule temperatures_;
@@ -209,6 +229,7 @@ class PopulationAnnealing : public Solver {
   ~PopulationAnnealing() override;
 
  protected:
+  void observe_run() override;
   // under colour classes the best energies of population annealing are
   // distributed differently from the reference's (a 12x12 lattice at
   // population 64 finds lower energies with fewer restarts)

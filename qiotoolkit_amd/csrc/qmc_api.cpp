@@ -274,7 +274,7 @@ struct qmc_walkers_s {
   int pa_on = 0;
   PaArgs pa_args{};
   DevBuf<PaDev> pa;
-  DevBuf<double> pa_table, pa_weight, pa_tcost, pa_tbest, pa_tbeta, pa_cost0, pa_tree;
+  DevBuf<double> pa_table, pa_weight, pa_tcost, pa_tbest, pa_tbeta, pa_cost0, pa_tree, pa_log;
   DevBuf<int32_t> pa_family, pa_family_next, pa_src, pa_hist, pa_count, pa_ids;
   bool pa_stepped = false;  // a step was enqueued since qmc_walkers_pa_begin
   int ordered = 0;           // sweeps visit variables in index order (k_sweep_ordered)
@@ -1099,6 +1099,43 @@ int qmc_walkers_pa_begin(qmc_walkers w, int32_t population, double alpha, double
   return upload_units(w);
 }
 
+// The costs before the next step's sweeps kept on the device: now, or when
+// a pending set-up computes them.
+static int pa_keep_costs_before(qmc_walkers w) {
+  if (!w->pa_cost0.p) HIP_TRY(w->pa_cost0.alloc(w->cap));
+  if (!w->pend_init)
+    HIP_TRY(hipMemcpyAsync(w->pa_cost0.p, w->cost.p, (size_t)w->cap * sizeof(double), hipMemcpyDeviceToDevice,
+                           w->prob->ctx->stream));
+  w->pa_args.cost0 = w->pa_cost0.p;
+  return QMC_OK;
+}
+
+int qmc_walkers_pa_observe(qmc_walkers w, int32_t max_steps) {
+  if (!w) return fail(QMC_INVALID_ARGUMENT, "null handle");
+  if (!w->pa_on) return fail(QMC_INVALID_ARGUMENT, "population mode is off");
+  if (w->pa_stepped) return fail(QMC_INVALID_ARGUMENT, "observation starts before the first step");
+  if (max_steps < 1) return fail(QMC_INVALID_ARGUMENT, "max_steps must be >= 1");
+  if (int rc = pa_keep_costs_before(w)) return rc;
+  HIP_TRY(w->pa_log.alloc((size_t)max_steps * kPaLogFields));
+  ZeroSpans z{};
+  z.p[0] = w->pa_log.p;
+  z.n[0] = (size_t)max_steps * kPaLogFields * sizeof(double);
+  HIP_TRY(launch_zero(z, 1, w->prob->ctx->stream));
+  w->pa_args.log = w->pa_log.p;
+  w->pa_args.log_steps = max_steps;
+  return QMC_OK;
+}
+
+int qmc_walkers_pa_log(qmc_walkers w, int32_t n_steps, double* records) {
+  if (!w || (!records && n_steps > 0)) return fail(QMC_INVALID_ARGUMENT, "null argument");
+  if (!w->pa_log.p) return fail(QMC_INVALID_ARGUMENT, "observation was not started");
+  if (n_steps < 0 || n_steps > w->pa_args.log_steps) return fail(QMC_INVALID_ARGUMENT, "n_steps out of range");
+  HIP_TRY(hipStreamSynchronize(w->prob->ctx->stream));
+  if (n_steps > 0)
+    HIP_TRY(hipMemcpy(records, w->pa_log.p, (size_t)n_steps * kPaLogFields * sizeof(double), hipMemcpyDeviceToHost));
+  return QMC_OK;
+}
+
 int qmc_walkers_pa_strategy(qmc_walkers w, int32_t strategy, double constant, int32_t constant_population,
                             int32_t exp_repopulation) {
   if (!w) return fail(QMC_INVALID_ARGUMENT, "null handle");
@@ -1110,19 +1147,8 @@ int qmc_walkers_pa_strategy(qmc_walkers w, int32_t strategy, double constant, in
     return fail(QMC_INVALID_ARGUMENT, "the strategy constant must be greater than 0");
   PaArgs& a = w->pa_args;
   const int cap = w->cap;
-  if (strategy == QMC_PA_FRICTION_TENSOR || strategy == QMC_PA_ENERGY_VARIANCE) {
-    if (!w->pa_cost0.p) {
-      HIP_TRY(w->pa_cost0.alloc(cap));
-      // costs before the first step's sweeps: now, or when a pending set-up
-      // computes them
-      if (!w->pend_init)
-        HIP_TRY(hipMemcpyAsync(w->pa_cost0.p, w->cost.p, (size_t)cap * sizeof(double), hipMemcpyDeviceToDevice,
-                               w->prob->ctx->stream));
-    }
-    a.cost0 = w->pa_cost0.p;
-  } else {
-    a.cost0 = nullptr;
-  }
+  if (strategy == QMC_PA_FRICTION_TENSOR || strategy == QMC_PA_ENERGY_VARIANCE)
+    if (int rc = pa_keep_costs_before(w)) return rc;
   int pow2 = 1;
   while (pow2 < cap) pow2 <<= 1;
   if (constant_population && pow2 > kPaTreeLds && !w->pa_tree.p) {

@@ -120,6 +120,12 @@ struct PaDev {
 // Residual resampling keeps its selection tree in LDS up to this many
 // citizens (a power of two), in PaArgs::tree above it.
 constexpr int kPaTreeLds = 4096;
+// Per-step record of a population annealing step (the observations of
+// population_annealing.h:262-470): epoch before the step, mean cost after the
+// sweeps, population, sigma, tau, zeta (NaN when not defined), the stepping
+// size, family count, sum of family sizes, size of the family with the
+// largest id, rho, restart flag, resampled flag.
+constexpr int kPaLogFields = QMC_PA_LOG_FIELDS;
 
 struct PaArgs {
   PaDev* pa;
@@ -136,6 +142,8 @@ struct PaArgs {
   double* cost0;              // costs before the step's sweeps (friction tensor, energy variance)
   double* tree;               // selection tree above kPaTreeLds citizens (pow2(cap) + 1 entries)
   int32_t* ids;               // its ids (cap entries)
+  double* log;                // per-step records (kPaLogFields doubles each), or null
+  int32_t log_steps;          // records the log holds
   double* cost;
   double* best_cost;
   double* beta;

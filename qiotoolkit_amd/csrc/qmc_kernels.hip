@@ -1299,6 +1299,9 @@ struct MinOp {
 struct AddOp {
   template <class T> __device__ T operator()(T a, T b) const { return a + b; }
 };
+struct MaxOp {
+  template <class T> __device__ T operator()(T a, T b) const { return b > a ? b : a; }
+};
 
 // Sum over the 64 lanes of a wavefront (butterfly: every lane ends with the
 // same bits, as x + y == y + x).
@@ -1492,15 +1495,12 @@ __global__ void __launch_bounds__(1024) k_pa_step(PaArgs A) {
   }
   const double var = pa_reduce(vs, AddOp(), rd) / (double)R;
 
-  double db = 0.0;
-  if (A.strategy == QMC_PA_SCHEDULE) {
-    const int64_t k = step - P->base_step;
-    const int64_t last = A.n_table - 1;
-    db = A.beta_table[k + 1 < last ? k + 1 : last] - A.beta_table[k < last ? k : last];
-  } else if (A.strategy == QMC_PA_CONSTANT_CULLING) {
-    db = pa_find_delta_beta(A, R, target, mn, ce, &rd[0]);
-  } else {
-    // sigma = sqrt(var_before var_after), gamma = cov(before, after)
+  // sigma = sqrt(var_before var_after), gamma = cov(before, after), and
+  // tau = sweeps / (ln sigma - ln |gamma|), zeta = sigma tau where defined
+  // (population_annealing.h:347-373), from the costs before the sweeps
+  const double nan = __longlong_as_double(0x7ff8000000000000ll);
+  double sigma = nan, tau = nan, zeta = nan;
+  if (A.cost0) {
     double sb = 0.0;
     for (int i = tid; i < R; i += bd) sb += A.cost0[i];
     const double avg_b = pa_reduce(sb, AddOp(), rd) / (double)R;
@@ -1512,17 +1512,26 @@ __global__ void __launch_bounds__(1024) k_pa_step(PaArgs A) {
     }
     const double var_b = pa_reduce(vb, AddOp(), rd) / (double)R;
     const double gamma = pa_reduce(g, AddOp(), rd) / (double)R;
-    const double sigma = sqrt(var_b * var);
-    if (A.strategy == QMC_PA_FRICTION_TENSOR) {
-      double zeta = 0.0;
-      if (sigma > 0.0 && gamma != 0.0) {
-        const double delta = log(sigma) - log(fabs(gamma));
-        if (fabs(delta) > 1e-11) zeta = sigma * ((double)A.sweeps_per_step / delta);
+    sigma = sqrt(var_b * var);
+    if (sigma > 0.0 && gamma != 0.0) {
+      const double delta = log(sigma) - log(fabs(gamma));
+      if (fabs(delta) > 1e-11) {
+        tau = (double)A.sweeps_per_step / delta;
+        zeta = sigma * tau;
       }
-      if (zeta > 0.0) db = A.strategy_const / sqrt(zeta);
-    } else if (sigma > 0.0) {
-      db = A.strategy_const / sqrt(sigma);
     }
+  }
+  double db = 0.0;
+  if (A.strategy == QMC_PA_SCHEDULE) {
+    const int64_t k = step - P->base_step;
+    const int64_t last = A.n_table - 1;
+    db = A.beta_table[k + 1 < last ? k + 1 : last] - A.beta_table[k < last ? k : last];
+  } else if (A.strategy == QMC_PA_CONSTANT_CULLING) {
+    db = pa_find_delta_beta(A, R, target, mn, ce, &rd[0]);
+  } else if (A.strategy == QMC_PA_FRICTION_TENSOR) {
+    if (zeta > 0.0) db = A.strategy_const / sqrt(zeta);
+  } else if (sigma > 0.0) {
+    db = A.strategy_const / sqrt(sigma);
   }
   double rho = 1.0;
   int Rn = R;
@@ -1591,12 +1600,24 @@ __global__ void __launch_bounds__(1024) k_pa_step(PaArgs A) {
     }
     __syncthreads();
     long long sq = 0;
+    int nfam = 0, fmax = -1;
     for (int f = tid; f < target; f += bd) {
       const long long h = A.hist[f];
       sq += h * h;
+      nfam += h > 0 ? 1 : 0;
+      fmax = h > 0 ? f : fmax;
     }
     sq = pa_reduce(sq, AddOp(), rl);
     rho = (double)sq / (double)R;
+    if (A.log) {
+      nfam = pa_reduce(nfam, AddOp(), ri);
+      fmax = pa_reduce(fmax, MaxOp(), ri);
+      if (tid == 0 && step < A.log_steps) {
+        double* rec = A.log + step * kPaLogFields;
+        rec[7] = (double)nfam;
+        rec[9] = fmax >= 0 ? (double)A.hist[fmax] : 0.0;
+      }
+    }
     Rn = current;
     moved = true;
   }
@@ -1648,6 +1669,21 @@ __global__ void __launch_bounds__(1024) k_pa_step(PaArgs A) {
     P->rho = rho;
     P->delta_beta = db;
     P->target = tnew;
+    if (A.log && step < A.log_steps) {
+      double* rec = A.log + step * kPaLogFields;
+      rec[0] = (double)P->epoch;
+      rec[1] = avg;
+      rec[2] = (double)R;
+      rec[3] = sigma;
+      rec[4] = tau;
+      rec[5] = zeta;
+      rec[6] = db;
+      if (!(db > 0.0)) rec[7] = rec[9] = 0.0;
+      rec[8] = (double)Rn;
+      rec[10] = rho;
+      rec[11] = restart ? 1.0 : 0.0;
+      rec[12] = db > 0.0 ? 1.0 : 0.0;
+    }
     P->step = step + 1;
     if (restart) {
       P->cur_beta = A.beta_start;

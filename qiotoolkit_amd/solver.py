@@ -23,7 +23,7 @@ EXPORTS = ["qiotk_engine_create", "qiotk_engine_destroy", "qiotk_solve", "qiotk_
            "qiotk_model_create_terms", "qiotk_solve_batch_arrays", "qiotk_model_vars",
            "qiotk_tune_batch", "qiotk_server_create", "qiotk_server_submit", "qiotk_server_submit_json",
            "qiotk_server_wait", "qiotk_server_stats", "qiotk_server_destroy", "qiotk_server_last_error",
-           "qiotk_model_create_file", "qiotk_solve_file", "qiotk_parse_model_file"]
+           "qiotk_model_create_file", "qiotk_solve_file", "qiotk_parse_model_file", "qiotk_observe"]
 
 
 def _check_rc(rc):
@@ -69,6 +69,7 @@ def lib():
     L.qiotk_solve_file.argtypes = [vp, C.c_char_p, C.c_char_p, C.c_char_p, P(vp)]
     L.qiotk_schedule_eval.argtypes = [C.c_char_p, C.c_int32, P(C.c_double), P(C.c_double), P(vp)]
     L.qiotk_describe.argtypes = [C.c_char_p, C.c_char_p, P(vp)]
+    L.qiotk_observe.argtypes = [C.c_char_p, C.c_char_p, P(vp)]
     L.qiotk_model_create_terms.argtypes = [vp, C.c_int, C.c_int64, P(C.c_int64), P(C.c_int32),
                                            P(C.c_double), P(vp)]
     _lib = L
@@ -326,6 +327,14 @@ def schedule_eval(schedule, xs):
                               y.ctypes.data_as(C.POINTER(C.c_double)), C.byref(out))
     meta = _raise_from(_take(out))
     return y, meta
+
+
+def observe(params, events):
+    """The observer protocol of params["observer"] fed with events (see
+    qiotk_observe); returns its render."""
+    out = C.c_void_p()
+    lib().qiotk_observe(_enc(params), _enc(events), C.byref(out))
+    return _raise_from(_take(out))
 
 
 def describe(target, params):

@@ -22,7 +22,8 @@ pytestmark = pytest.mark.gpu
 
 
 def _steps_match_oracle(ctx, alpha, pop, B, strategy=0, const=0.0, constant_population=False, S=25, side=8,
-                        ordered=False, visit=_qmc.VISIT_INDEX_LEVELS, exp_repopulation=False, cap=0):
+                        ordered=False, visit=_qmc.VISIT_INDEX_LEVELS, exp_repopulation=False, cap=0,
+                        observe=False):
     op, qp = pair(ctx, "ising", wl.lattice2d(side, seed=4), visit=visit)
     cap = cap or max(4 * pop, pop + 256)
     seed = 13
@@ -38,6 +39,9 @@ def _steps_match_oracle(ctx, alpha, pop, B, strategy=0, const=0.0, constant_popu
     w.pa_begin(pop, alpha, 1.0, table, seed, 1)
     if adaptive or constant_population or exp_repopulation:
         w.pa_strategy(strategy, const, constant_population, exp_repopulation)
+    if observe:
+        w.pa_observe(S)
+    records = []
     w.init(0)
     opts = po.make_opts(po.RNG_COUNTER, seed=seed)
     fam = np.arange(pop, dtype=np.int32)
@@ -62,6 +66,23 @@ def _steps_match_oracle(ctx, alpha, pop, B, strategy=0, const=0.0, constant_popu
         info = w.pa_info()
         after = w.read()
         Rf, src = o["R"], o["src"]
+        if observe:
+            # the step's observations (population_annealing.h:262-470)
+            b0, b1 = pre["cost"][:R], before["cost"][:R]
+            vb, va = np.mean((b0 - b0.mean()) ** 2), np.mean((b1 - b1.mean()) ** 2)
+            gam = np.mean((b0 - b0.mean()) * (b1 - b1.mean()))
+            sig = np.sqrt(vb * va)
+            tau = zeta = np.nan
+            if sig > 0 and gam != 0 and abs(np.log(sig) - np.log(abs(gam))) > 1e-11:
+                tau = 1.0 / (np.log(sig) - np.log(abs(gam)))
+                zeta = sig * tau
+            moved = o["dbeta"] > 0
+            nf = fl = 0
+            if moved:
+                fams = fam[src[:o["R_resampled"]]]
+                nf, fl = len(set(fams.tolist())), int(np.sum(fams == fams.max()))
+            records.append((epoch, b1.mean(), R, sig, tau, zeta, nf, o["R_resampled"] if moved else None, fl,
+                            o["rho"], o["restart"], moved))
         # adaptive steps: device reductions run in another order than the
         # oracle's sequential sums (constant culling lands on the same
         # multiple of 2^-20)
@@ -106,6 +127,16 @@ def _steps_match_oracle(ctx, alpha, pop, B, strategy=0, const=0.0, constant_popu
         assert info["restarts"] == restarts
         assert info["beta"] == cur_beta
         fam, R = o["family"], Rf
+    if observe:
+        log = w.pa_log(S)
+        for s_, (ep, mean, Rs, sig, tau, zeta, nf, rn, fl, rho, rst, moved) in enumerate(records):
+            r = log[s_]
+            assert r[0] == ep and r[2] == Rs and r[11] == rst and r[12] == moved, s_
+            assert abs(r[1] - mean) <= 1e-12 * max(1.0, abs(mean)), s_
+            for got, want in ((r[3], sig), (r[4], tau), (r[5], zeta)):
+                assert (np.isnan(got) and np.isnan(want)) or abs(got - want) <= 1e-9 * max(1.0, abs(want)), s_
+            if moved:
+                assert r[7] == nf and r[8] == rn and r[9] == fl and r[10] == rho, s_
     w.pa_end()
     assert w.R == R
     end = w.read()
@@ -130,9 +161,10 @@ def test_device_pa_steps_match_oracle(ctx, alpha, pop, B, ordered, visit):
     (_qmc.PA_CONSTANT_CULLING, 0.2, True), (_qmc.PA_FRICTION_TENSOR, 1.0, True)])
 def test_device_pa_strategies_match_oracle(ctx, strategy, const, constant_population):
     """The adaptive stepping sizes and residual resampling
-    (qmc_walkers_pa_strategy) step by step against orc_pa_step_counter."""
+    (qmc_walkers_pa_strategy) step by step against orc_pa_step_counter, with
+    the per-step observation records (qmc_walkers_pa_observe)."""
     print("pa strategy=%d const=%g constant_population=%d" % (strategy, const, constant_population), flush=True)
-    _steps_match_oracle(ctx, 2.0, 48, 4, strategy, const, constant_population)
+    _steps_match_oracle(ctx, 2.0, 48, 4, strategy, const, constant_population, observe=True)
 
 
 def test_device_pa_exponential_repopulation(ctx):

@@ -274,3 +274,50 @@ def test_solve_from_protobuf_folder(engine):
         b = engine.solve(target, p, doc)
         assert a["solutions"] == b["solutions"]
         assert oracle_cost(doc, a["solutions"]["configuration"]) == a["solutions"]["cost"]
+
+
+def test_benchmark_solver_properties(engine):
+    """benchmark.extensions "solver" as SteppingSolver::get_solver_properties
+    renders it (stepping_solver.h:268-283, solver.h:57-75), in the shape the
+    reference's SA toy test prints (simulated_annealing_test.cc:159-177, a
+    release build: no milestones recorded)."""
+    doc = load("ising1.json")
+    p = {"params": {"step_limit": 100, "restarts": 4, "seed": 42, "beta_start": 0.5, "cost_limit": -1000,
+                    "beta_stop": 10.0}}
+    res = engine.solve("simulatedannealing.qiotoolkit", p, doc)
+    ext = {e["name"]: e["value"] for e in res["benchmark"]["extensions"]}
+    sol = ext["solver"]
+    assert sol["cost_limit"] == -1000
+    assert sol["cost_milestones"] == []
+    assert sol["last_step"] == 100 and sol["step_limit"] == 100
+    assert sol["exit_reason"] == "Stop due to step limit: 100, steps:100"
+    # reaching the cost limit ends the run early with the reference's message
+    p["params"]["cost_limit"] = 0
+    res = engine.solve("simulatedannealing.qiotoolkit", p, doc)
+    sol = {e["name"]: e["value"] for e in res["benchmark"]["extensions"]}["solver"]
+    assert sol["exit_reason"] == "Stop due to cost limit: 0.000000, current lowest cost:"
+    assert sol["last_step"] < 100
+
+
+def test_pa_observer_render(engine):
+    """A population annealing solve with the reference's observer protocol
+    (observe/observer.h:80-92): step windows of 10, averages of the step
+    observations, constants of the last step and epoch of each window, in
+    benchmark.extensions "solver"."""
+    doc = {"cost_function": {"type": "ising", "terms": wl.lattice2d(10, seed=2)}}
+    p = {"params": {"population": 64, "beta_start": 0, "beta_stop": 3, "sweeps": 40, "seed": 5},
+         "observer": {"average_over": {"step": 10}, "constant": ["step", "epoch"],
+                      "average": ["cost", "population", "delta_beta", "rho_t", "families", "family_size",
+                                  "sigma"]}}
+    res = engine.solve("populationannealing.cpu", p, doc)
+    sol = {e["name"]: e["value"] for e in res["benchmark"]["extensions"]}["solver"]
+    groups = sol["step"]
+    assert len(groups) >= 4
+    assert groups[-1]["step"] == 39 and sol["last_step"] == 40
+    for g in groups:
+        assert 32 <= g["population"] <= 128
+        assert g["sigma"] >= 0
+        assert g["epoch"] >= 0
+    # delta_beta is 3/39 per step of the schedule wherever it was observed
+    seen = [g["delta_beta"] for g in groups if "delta_beta" in g]
+    assert seen and all(abs(d - 3.0 / 39) < 1e-12 for d in seen)

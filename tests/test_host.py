@@ -277,3 +277,76 @@ def test_json_input_file(tmp_path):
     p = tmp_path / "input.json"
     p.write_text(json.dumps(doc))
     assert S.parse_model_file(str(p)) == S.parse_model(doc)
+
+
+def _step_events(steps, per_step, restart_at=()):
+    ev = []
+    for s in range(steps):
+        ev += [{"label": "step", "value": s}, {"observe": "step", "value": s}]
+        ev += per_step(s)
+        ev.append({"label": "step", "value": None})
+        if s in restart_at:
+            ev.append({"restart": s})
+    return ev
+
+
+def test_observer_windows_groups_and_kinds():
+    """observe/observer.cc: average_over windows start at the last restart
+    (v - (v - restart) % w), group_by labels nest inside them in label
+    order, Constant keeps the last value of positive weight, Average the
+    weighted mean; groups render as arrays in label-value order."""
+    from qiotoolkit_amd import solver as S
+
+    def per_step(s):
+        ev = []
+        for r in range(2):
+            ev += [{"label": "replica", "value": r}, {"observe": "T", "value": 0.1 * (r + 1)},
+                   {"observe": "avg_cost", "value": s + r}, {"label": "replica", "value": None}]
+        return ev
+    cfg = {"observer": {"average_over": {"step": 2}, "group_by": ["replica"], "constant": ["T"],
+                        "average": ["avg_cost", "step"]}}
+    out = S.observe(cfg, _step_events(7, per_step, restart_at=(3,)))
+    # windows: {0,1} {2,3} | restart at 3 | {4} {5,6}
+    assert [g["step"] for g in out["step"]] == [0.5, 2.5, 4, 5.5]
+    assert [[r["avg_cost"] for r in g["replica"]] for g in out["step"]] == [[0.5, 1.5], [2.5, 3.5], [4, 5],
+                                                                            [5.5, 6.5]]
+    assert all([r["T"] for r in g["replica"]] == [0.1, 0.2] for g in out["step"])
+    # weights, zero-weight constants, and a protocol list
+    ev = [{"observe": "a", "value": 1.0, "weight": 3}, {"observe": "a", "value": 5.0},
+          {"observe": "c", "value": 2.0}, {"observe": "c", "value": 9.0, "weight": 0},
+          {"observe": "u", "value": 4.0}]
+    out = S.observe({"observer": [{"average": ["a"]}, {"constant": ["c"]}]}, ev)
+    assert out == {"a": 2.0, "c": 2.0}
+
+
+def test_observer_pt_render_shape_and_errors():
+    """The PT observation sequence of parallel_tempering.h:141-186 renders as
+    the reference's toy test prints it (parallel_tempering_test.cc:153-190):
+    one entry per replica with T, acc_rate, avg_cost, avg_dir, and
+    swap_rate for every replica but the last."""
+    from qiotoolkit_amd import solver as S
+    L = 5
+
+    def per_step(s):
+        ev = []
+        for i in range(L):
+            ev += [{"label": "replica", "value": i}, {"observe": "T", "value": 0.1 * (i + 1)},
+                   {"observe": "acc_rate", "value": 0.5}, {"observe": "avg_cost", "value": 1.0},
+                   {"observe": "avg_dir", "value": i % 2}, {"label": "replica", "value": None}]
+        for i in range(s & 1, L - 1, 2):
+            ev += [{"label": "replica", "value": i}, {"observe": "swap_rate", "value": (i + s) % 2},
+                   {"label": "replica", "value": None}]
+        return ev
+    cfg = {"observer": {"group_by": ["replica"], "constant": ["T"],
+                        "average": ["avg_cost", "acc_rate", "avg_dir", "swap_rate"]}}
+    out = S.observe(cfg, _step_events(200, per_step))
+    reps = out["replica"]
+    assert len(reps) == L and [r["T"] for r in reps] == [0.1, 0.2, 0.30000000000000004, 0.4, 0.5]
+    assert all(set(r) == {"T", "acc_rate", "avg_cost", "avg_dir", "swap_rate"} for r in reps[:-1])
+    assert set(reps[-1]) == {"T", "acc_rate", "avg_cost", "avg_dir"}
+    with pytest.raises(S.SolverError) as e:
+        S.observe({"observer": [3]}, [])
+    assert e.value.code == 103
+    with pytest.raises(S.SolverError):
+        S.describe("paralleltempering.qiotoolkit", {"params": {"sweeps": 10}, "observer": {"average": ["acc_rate"]}})
+    S.describe("paralleltempering.qiotoolkit", {"params": {"sweeps": 10}, "observer": {"constant": ["beta"]}})
