@@ -348,9 +348,17 @@ Json SimulatedAnnealing::describe() const {
   return d;
 }
 
+// evaluations after s steps: one cost evaluation per restart at init, one
+// difference evaluation per variable and restart per step
+// (simulated_annealing.h:65-100)
+int64_t SimulatedAnnealing::planned_steps() {
+  const int64_t R = restarts_, N = model_->n_vars;
+  return eval_limited_steps([&](int64_t s) { return R + s * R * N; });
+}
+
 void SimulatedAnnealing::run() {
   if (model_->empty()) return;
-  const int64_t steps = steps_to_run();
+  const int64_t steps = planned_steps();
   const int64_t N = model_->n_vars;
   const double t0 = wall_s();
   // Without time or cost limits all steps are one launch; otherwise ticks of
@@ -390,7 +398,7 @@ void SimulatedAnnealing::run() {
 bool SimulatedAnnealing::batch_plan(qmc_batch b) {
   if (time_limit_ || cost_limit_) return false;
   if (model_->empty()) return true;
-  const int64_t steps = steps_to_run();
+  const int64_t steps = planned_steps();
   device_check(qmc_walkers_plan(w_, 0, (int32_t)steps, betas_.data()), "qmc_walkers_plan");
   device_check(qmc_batch_add(b, w_), "qmc_batch_add");
   batched_replicas_ = restarts_;
@@ -507,9 +515,16 @@ void ParallelTempering::init() {
   }
 }
 
+// evaluations after s steps: one per replica at init, then every replica's
+// difference evaluations (parallel_tempering.h:100-152)
+int64_t ParallelTempering::planned_steps() {
+  const int64_t L = (int64_t)betas_.size(), N = model_->n_vars;
+  return eval_limited_steps([&](int64_t s) { return L + s * L * N * sweeps_per_replica_; });
+}
+
 void ParallelTempering::run() {
   if (model_->empty()) return;
-  const int64_t steps = steps_to_run();
+  const int64_t steps = planned_steps();
   const int L = (int)betas_.size();
   const int64_t N = model_->n_vars;
   const double t0 = wall_s();
@@ -548,7 +563,7 @@ bool ParallelTempering::batch_plan(qmc_batch b) {
   if (time_limit_ || cost_limit_) return false;
   if (model_->empty()) return true;
   const int L = (int)betas_.size();
-  const int64_t steps = steps_to_run();
+  const int64_t steps = planned_steps();
   if (qmc_walkers_plan_pt(w_, L, betas_.data(), 0, (int32_t)steps, sweeps_per_replica_) != QMC_OK)
     return false;  // ladder split over work-groups: runs on its own
   device_check(qmc_batch_add(b, w_), "qmc_batch_add");

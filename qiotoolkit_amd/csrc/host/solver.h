@@ -85,6 +85,31 @@ class Solver {
   // SteppingSolver (stepping_solver.h:112-268): fewer than steps_per_tick steps
   // are never run, because the step limit is checked after each tick.
   int64_t steps_to_run() const;
+  // Steps a run with an evaluation limit takes (SteppingSolver::run,
+  // stepping_solver.h:119-245): ticks of steps_per_tick steps, growing to
+  // max(n + 1, 1.2 n) after every tick that takes under 0.95 tick_every (all
+  // of them here), the last clamped to the step limit; after a tick the step
+  // limit is checked, then evaluations(steps so far) against eval_limit,
+  // which stops the run with the reference's message.  Without a limit:
+  // steps_to_run().
+  template <class F>
+  int64_t eval_limited_steps(F evaluations) {
+    const int64_t total = steps_to_run();
+    if (!eval_limit_) return total;
+    int64_t s = 0, tick = steps_per_tick_, next = tick;
+    while (true) {
+      s += next;
+      tick = (int64_t)std::max((double)tick + 1.0, (double)tick * 1.2);
+      if (s >= total) return total;
+      next = std::min(total - s, tick);
+      const int64_t e = evaluations(s);
+      if (e >= *eval_limit_) {
+        exit_reason_ = "Stop due to Evaluation limit: " + std::to_string(*eval_limit_) +
+                       ", evaluations:" + std::to_string(e);
+        return s;
+      }
+    }
+  }
   bool has_time_limit() const { return (bool)time_limit_; }
   void record(double cost, const std::vector<uint8_t>& state);
   // ModelSolver::populate_solutions: best `number_of_solutions` walkers
@@ -167,6 +192,8 @@ class SimulatedAnnealing : public Solver {
   int64_t run_replicas() const override { return restarts_; }
   qmc_walkers walkers() const override { return w_; }
 
+  int64_t planned_steps();
+
  private:
   bool use_inverse_temperature_ = false;
   int restarts_ = 64;
@@ -199,6 +226,8 @@ class ParallelTempering : public Solver {
 
  protected:
   void observe_run() override;
+
+  int64_t planned_steps();
 
  private:
   bool use_inverse_temperatures_ = true;

@@ -321,3 +321,24 @@ def test_pa_observer_render(engine):
     # delta_beta is 3/39 per step of the schedule wherever it was observed
     seen = [g["delta_beta"] for g in groups if "delta_beta" in g]
     assert seen and all(abs(d - 3.0 / 39) < 1e-12 for d in seen)
+
+
+def test_eval_limit_stops_at_the_tick(engine):
+    """eval_limit (stepping_solver.h:235-244): checked after each tick of
+    10, 12, 14, 16, ... steps against cost evaluations (one per replica at
+    init) plus difference evaluations (replicas x variables per step).
+    ising1 has 3 variables: SA with 4 restarts counts 4 + 12 s, so a limit of
+    500 stops after the tick ending at 52 steps (628 evaluations); PT with
+    the default 5 betas counts 5 + 15 s and stops at 36 (545)."""
+    doc = load("ising1.json")
+    res = engine.solve("simulatedannealing.qiotoolkit",
+                       {"params": {"restarts": 4, "seed": 3, "sweeps": 1000, "eval_limit": 500}}, doc)
+    sol = {e["name"]: e["value"] for e in res["benchmark"]["extensions"]}["solver"]
+    assert sol["last_step"] == 52 and sol["eval_limit"] == 500
+    assert sol["exit_reason"] == "Stop due to Evaluation limit: 500, evaluations:628"
+    res = engine.solve("paralleltempering.qiotoolkit", {"params": {"seed": 3, "sweeps": 1000, "eval_limit": 500}},
+                       doc)
+    sol = {e["name"]: e["value"] for e in res["benchmark"]["extensions"]}["solver"]
+    assert sol["last_step"] == 36
+    assert sol["exit_reason"] == "Stop due to Evaluation limit: 500, evaluations:545"
+    assert res["solutions"]["cost"] == -10
