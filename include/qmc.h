@@ -216,6 +216,7 @@ typedef struct {
   double delta_beta;   /* stepping size of the last step */
   int32_t target;      /* target population (doubled by restarts under exp_repopulation) */
   int32_t pad_;
+  int64_t evaluations; /* cost + difference evaluations as the reference counts them */
 } qmc_pa_info;
 /* Waits for the stream; best_state (n_vars bytes, may be NULL) receives the
  * state of `best`. */

@@ -52,7 +52,7 @@ class PaInfo(C.Structure):
         ("population", C.c_int32), ("restarts", C.c_int32), ("overflow", C.c_int32),
         ("have_best", C.c_int32), ("steps", C.c_int64), ("attempts", C.c_int64),
         ("best", C.c_double), ("beta", C.c_double), ("rho", C.c_double), ("delta_beta", C.c_double),
-        ("target", C.c_int32), ("pad_", C.c_int32),
+        ("target", C.c_int32), ("pad_", C.c_int32), ("evaluations", C.c_int64),
     ]
 
 

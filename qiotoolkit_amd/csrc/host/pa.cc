@@ -163,16 +163,25 @@ void PopulationAnnealing::init() {
 void PopulationAnnealing::run() {
   if (model_->empty()) return;
   PaState& st = pa_state(impl_);
-  const int64_t steps = steps_to_run();
-  while (step_ < steps) {
-    device_check(qmc_walkers_sweep(st.w, sweep_, sweeps_per_replica_, QMC_BETA_PER_REPLICA, nullptr, 0),
-                 "qmc_walkers_sweep");
-    double ms = 0;
-    qmc_walkers_last_ms(st.w, &ms);
-    timing_.device_ms += ms;
-    timing_.launches++;
-    finish_step();
-  }
+  auto run_to = [&](int64_t s) {
+    while (step_ < s) {
+      device_check(qmc_walkers_sweep(st.w, sweep_, sweeps_per_replica_, QMC_BETA_PER_REPLICA, nullptr, 0),
+                   "qmc_walkers_sweep");
+      double ms = 0;
+      qmc_walkers_last_ms(st.w, &ms);
+      timing_.device_ms += ms;
+      timing_.launches++;
+      finish_step();
+    }
+  };
+  // with an evaluation limit the population's count is read at each tick
+  const int64_t steps = eval_limited_steps([&](int64_t s) {
+    run_to(s);
+    qmc_pa_info info{};
+    device_check(qmc_walkers_pa_info(st.w, &info, nullptr), "qmc_walkers_pa_info");
+    return (int64_t)info.evaluations;
+  });
+  run_to(steps);
 }
 
 bool PopulationAnnealing::serve_plan(qmc_batch b) {

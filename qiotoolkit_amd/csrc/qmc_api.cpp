@@ -1052,6 +1052,7 @@ int qmc_walkers_pa_begin(qmc_walkers w, int32_t population, double alpha, double
   PaDev st{};
   st.R = population;
   st.target = population;
+  st.pending_inits = population;
   st.best_src = -1;
   st.cur_beta = beta_table[0];
   HIP_TRY(upload_async(c, w->pa.p, &st, sizeof st));
@@ -1195,6 +1196,7 @@ int qmc_walkers_pa_info(qmc_walkers w, qmc_pa_info* info, uint8_t* best_state) {
   info->rho = d.rho;
   info->delta_beta = d.delta_beta;
   info->target = d.target;
+  info->evaluations = d.evals;
   if (best_state && w->prob->N > 0)
     HIP_TRY(hipMemcpy(best_state, w->pa_best_state.p, (size_t)w->prob->N, hipMemcpyDeviceToHost));
   return QMC_OK;

@@ -115,6 +115,11 @@ struct PaDev {
   double delta_beta;  // stepping size of the last step
   int32_t target;     // target population (doubles at restarts with exp_repopulation)
   int32_t pad;
+  // cost + difference evaluations as make_step accumulates its citizens'
+  // counters (population_annealing.h:300-308): the step's sweeps plus the
+  // (re)initialisations made before them
+  int64_t evals;
+  int64_t pending_inits;
 };
 
 // Residual resampling keeps its selection tree in LDS up to this many

@@ -1661,6 +1661,8 @@ __global__ void __launch_bounds__(1024) k_pa_step(PaArgs A) {
       P->have_best = 1;
     }
     P->attempts += (int64_t)R * A.sweeps_per_step * A.n_vars;
+    P->evals += (int64_t)R * A.sweeps_per_step * A.n_vars + P->pending_inits;
+    P->pending_inits = restart ? Rf : 0;
     P->R_prev = R;
     P->R_resampled = Rn;
     P->R = Rf;

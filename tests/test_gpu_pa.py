@@ -49,6 +49,7 @@ def _steps_match_oracle(ctx, alpha, pop, B, strategy=0, const=0.0, constant_popu
     have_best, best, best_state = False, 0.0, None
     cur_beta = table[0]
     target = pop
+    evals, pending = 0, pop
     for step in range(S):
         pre = w.read()
         w.sweep(step, None, n_sweeps=1)
@@ -92,6 +93,10 @@ def _steps_match_oracle(ctx, alpha, pop, B, strategy=0, const=0.0, constant_popu
             assert abs(info["delta_beta"] - o["dbeta"]) <= 1e-12 * abs(o["dbeta"]), step
         assert info["population"] == Rf, step
         assert info["target"] == o["target"], step
+        # make_step's evaluation count: the sweeps plus the inits before them
+        evals += R * op.n + pending
+        pending = Rf if o["restart"] else 0
+        assert info["evaluations"] == evals, step
         target = o["target"]
         if constant_population and not o["restart"]:
             assert Rf == R

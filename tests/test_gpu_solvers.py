@@ -342,3 +342,22 @@ def test_eval_limit_stops_at_the_tick(engine):
     assert sol["last_step"] == 36
     assert sol["exit_reason"] == "Stop due to Evaluation limit: 500, evaluations:545"
     assert res["solutions"]["cost"] == -10
+
+
+def test_pa_eval_limit(engine):
+    """Population annealing stops at the first tick end (10, 22, 36, 52, ...
+    steps) where its evaluations reach eval_limit."""
+    doc = {"cost_function": {"type": "ising", "terms": wl.lattice2d(6, seed=2)}}
+    p = {"params": {"population": 16, "beta_start": 0, "beta_stop": 3, "sweeps": 500, "seed": 5,
+                    "eval_limit": 20000}}
+    res = engine.solve("populationannealing.cpu", p, doc)
+    sol = {e["name"]: e["value"] for e in res["benchmark"]["extensions"]}["solver"]
+    ends, s, tick = [], 0, 10
+    while s < 500:
+        s += tick
+        ends.append(s)
+        tick = int(max(tick + 1.0, tick * 1.2))
+    assert sol["last_step"] in ends and sol["last_step"] < 500
+    reason = sol["exit_reason"]
+    assert reason.startswith("Stop due to Evaluation limit: 20000, evaluations:")
+    assert int(reason.rsplit(":", 1)[1]) >= 20000
