@@ -128,6 +128,17 @@ int qmc_walkers_sweep(qmc_walkers w, int64_t sweep0, int32_t n_sweeps, int beta_
  * (markov/walker.h:144-153) step for step; the colour classes run their
  * variables in parallel. */
 int qmc_walkers_set_sweep_order(qmc_walkers w, int32_t ordered);
+/* Observe the parallel tempering steps that follow (steps 0..max_steps-1
+ * of their global numbering; they then run as graph-replayed sweep and
+ * exchange launches): per ladder, step and ladder position k, 4 doubles --
+ * the cost after the step's sweeps, the transitions accepted during them,
+ * the direction label before the swaps (0 up, 1 down: it travels with the
+ * states and is reset at both ends) and the swap of pair (k, k+1) (-1: not
+ * attempted), the observations of parallel_tempering.h:141-188.
+ * qmc_walkers_pt_log waits for the stream and copies the first n_values
+ * values (layout [ladder][step][position][4]). */
+int qmc_walkers_pt_observe(qmc_walkers w, int32_t ladder, int64_t max_steps);
+int qmc_walkers_pt_log(qmc_walkers w, int64_t n_values, double* out);
 /* Parallel tempering over ladders of `ladder` consecutive replicas with
  * betas ladder_betas[0..ladder) (descending).  Runs n_steps steps of
  * sweeps_per_replica sweeps, exchanging pairs (i,i+1), i = step mod 2, after

@@ -899,7 +899,7 @@ int orc_sa(void* pp, const orc_opts* o, int sched_type, double initial, double f
 int orc_pt(void* pp, const orc_opts* o, const double* values, int n_values, int use_inverse,
            int64_t step_limit, int sweeps_per_replica, double* slot_beta, double* lowest,
            uint8_t* lowest_states, double* final_costs, uint8_t* final_states,
-           double* swap_rate, int64_t* steps_run) {
+           double* swap_rate, int64_t* steps_run, double* acc_rate, double* avg_cost, double* avg_dir) {
   auto* p = static_cast<Problem*>(pp);
   if (n_values <= 0 || sweeps_per_replica <= 0) return -1;
   RunCtx ctx = make_ctx(p, o);
@@ -927,11 +927,26 @@ int orc_pt(void* pp, const orc_opts* o, const double* values, int n_values, int 
     beta_of_index[k] = ws[k].beta;
   }
   std::vector<double> swaps(R, 0.0), tries(R, 0.0);
+  // observations per temperature position (parallel_tempering.h:141-188):
+  // acceptance rate of the step, cost after the sweeps, direction label
+  std::vector<double> o_acc(R, 0.0), o_cost(R, 0.0), o_dir(R, 0.0);
+  std::vector<int> dir(R, 0);  // UP 0, DOWN 1, per position
+  std::vector<uint64_t> acc0(R, 0);
+  const double evals = (double)p->n * sweeps_per_replica;
   for (int64_t s = 0; s < steps; ++s) {
+    for (int r = 0; r < R; ++r) acc0[r] = ws[r].accepted;
 #pragma omp parallel for schedule(dynamic, 1)
     for (int r = 0; r < R; ++r)
       for (int k = 0; k < sweeps_per_replica; ++k)
         make_sweep(ctx, ws[r], (uint32_t)(s * sweeps_per_replica + k));
+    for (int i = 0; i < R; ++i) {
+      // the walker at position i: itself in reference mode (states move),
+      // the slot holding beta index i in counter mode (labels move)
+      const int r = o->rng_mode == ORC_RNG_REFERENCE ? i : label[i];
+      o_acc[i] += (double)(ws[r].accepted - acc0[r]) / evals;
+      o_cost[i] += ws[r].cost_;
+      o_dir[i] += (double)dir[i];
+    }
     for (int i = (int)(s & 1); i < R - 1; i += 2) {
       tries[i] += 1;
       if (o->rng_mode == ORC_RNG_REFERENCE) {
@@ -940,6 +955,7 @@ int orc_pt(void* pp, const orc_opts* o, const double* values, int n_values, int 
         if (master.uniform() < std::exp((a.cost_ - b.cost_) * (a.beta - b.beta))) {
           std::swap(a.state, b.state);  // Walker::swap_state (walker.h:256-260)
           std::swap(a.cost_, b.cost_);
+          std::swap(dir[i], dir[i + 1]);
           swaps[i] += 1;
         }
       } else {
@@ -953,10 +969,13 @@ int orc_pt(void* pp, const orc_opts* o, const double* values, int n_values, int 
           std::swap(label[i], label[i + 1]);
           a.beta = beta_of_index[i + 1];
           b.beta = beta_of_index[i];
+          std::swap(dir[i], dir[i + 1]);
           swaps[i] += 1;
         }
       }
     }
+    dir[0] = 0;
+    dir[R - 1] = 1;
   }
   const int64_t N = p->n;
   for (int r = 0; r < R; ++r) {
@@ -966,6 +985,9 @@ int orc_pt(void* pp, const orc_opts* o, const double* values, int n_values, int 
     if (final_costs) final_costs[r] = ws[r].cost_;
     if (final_states) copy_spins(ws[r].state.spins, final_states + (size_t)r * N);
     if (swap_rate) swap_rate[r] = tries[r] > 0 ? swaps[r] / tries[r] : 0.0;
+    if (acc_rate) acc_rate[r] = o_acc[r] / (double)steps;
+    if (avg_cost) avg_cost[r] = o_cost[r] / (double)steps;
+    if (avg_dir) avg_dir[r] = o_dir[r] / (double)steps;
   }
   if (steps_run) *steps_run = steps;
   return 0;

@@ -98,7 +98,7 @@ int orc_sa(void* p, const orc_opts* o, int sched_type, double initial, double fi
 int orc_pt(void* p, const orc_opts* o, const double* values, int n_values, int use_inverse,
            int64_t step_limit, int sweeps_per_replica, double* slot_beta, double* lowest,
            uint8_t* lowest_states, double* final_costs, uint8_t* final_states,
-           double* swap_rate, int64_t* steps_run);
+           double* swap_rate, int64_t* steps_run, double* acc_rate, double* avg_cost, double* avg_dir);
 
 /* PopulationAnnealing (solver/population_annealing.h), linear/geometric
  * schedule, Poisson (variable population) or constant_population resampling.

@@ -87,7 +87,7 @@ def lib(variant=""):
                          P(C.c_int64)]
     L.orc_pt.argtypes = [C.c_void_p, P(Opts), P(C.c_double), C.c_int, C.c_int, C.c_int64, C.c_int,
                          P(C.c_double), P(C.c_double), P(C.c_uint8), P(C.c_double), P(C.c_uint8),
-                         P(C.c_double), P(C.c_int64)]
+                         P(C.c_double), P(C.c_int64), P(C.c_double), P(C.c_double), P(C.c_double)]
     L.orc_pa.argtypes = [C.c_void_p, P(Opts), C.c_int, C.c_double, C.c_double, C.c_int64, C.c_int,
                          C.c_int, C.c_double, C.c_int, P(C.c_double), P(C.c_uint8), P(C.c_int64),
                          P(C.c_int64), C.c_int, C.c_double, C.c_int]
@@ -228,15 +228,17 @@ class Problem:
         fc = np.zeros(R)
         fs = np.zeros((R, N), dtype=np.uint8)
         sr = np.zeros(R)
+        ar, ac, ad = np.zeros(R), np.zeros(R), np.zeros(R)
         steps = C.c_int64()
         rc = self.L.orc_pt(self.h, C.byref(opts), _p(vals, C.c_double), R, int(use_inverse), sweeps,
                            sweeps_per_replica, _p(sb, C.c_double), _p(lowest, C.c_double),
                            _p(lstates, C.c_uint8), _p(fc, C.c_double), _p(fs, C.c_uint8),
-                           _p(sr, C.c_double), C.byref(steps))
+                           _p(sr, C.c_double), C.byref(steps), _p(ar, C.c_double), _p(ac, C.c_double),
+                           _p(ad, C.c_double))
         if rc != 0:
             raise ValueError("orc_pt rejected the parameters")
         return dict(slot_beta=sb, lowest=lowest, lowest_states=lstates, final_costs=fc,
-                    final_states=fs, swap_rate=sr, steps=steps.value)
+                    final_states=fs, swap_rate=sr, steps=steps.value, acc_rate=ar, avg_cost=ac, avg_dir=ad)
 
     def pa(self, opts, beta_start, beta_stop, sweeps, population, sweeps_per_replica=1, alpha=2.0,
            geometric=False, constant_population=False, strategy=0, strategy_const=0.0, exp_repopulation=False):

@@ -29,6 +29,7 @@ EXPORTS = [
     "qmc_walkers_set_lanes", "qmc_walkers_get_lanes", "qmc_walkers_pa_begin", "qmc_walkers_pa_step",
     "qmc_walkers_pa_info", "qmc_walkers_pa_end", "qmc_lds_bandwidth", "qmc_walkers_pa_strategy",
     "qmc_walkers_set_sweep_order", "qmc_problem_create_visit", "qmc_walkers_pa_observe", "qmc_walkers_pa_log",
+    "qmc_walkers_pt_observe", "qmc_walkers_pt_log",
 ]
 
 
@@ -130,6 +131,8 @@ def lib():
     L.qmc_walkers_set_sweep_order.argtypes = [vp, C.c_int32]
     L.qmc_walkers_pa_observe.argtypes = [vp, C.c_int32]
     L.qmc_walkers_pa_log.argtypes = [vp, C.c_int32, P(C.c_double)]
+    L.qmc_walkers_pt_observe.argtypes = [vp, C.c_int32, C.c_int64]
+    L.qmc_walkers_pt_log.argtypes = [vp, C.c_int64, P(C.c_double)]
     L.qmc_problem_create_visit.argtypes = [vp, C.c_int, C.c_int32, C.c_int64, P(C.c_int64), P(C.c_int32),
                                            P(C.c_double), C.c_int32, C.c_int32, P(vp)]
     L.qmc_walkers_pa_info.argtypes = [vp, P(PaInfo), vp]
@@ -358,6 +361,17 @@ class Walkers:
     def pa_log(self, n_steps):
         out = np.zeros((n_steps, 13))
         _check(lib().qmc_walkers_pa_log(self.h, int(n_steps), _p(out, C.c_double)))
+        return out
+
+    def pt_observe(self, ladder, max_steps):
+        _check(lib().qmc_walkers_pt_observe(self.h, int(ladder), int(max_steps)))
+        self._pt_obs = (ladder, max_steps)
+
+    def pt_log(self):
+        """[ladder][step][position][cost, accepted, direction, swap]"""
+        ladder, steps = self._pt_obs
+        out = np.zeros((self.R // ladder, steps, ladder, 4))
+        _check(lib().qmc_walkers_pt_log(self.h, out.size, _p(out, C.c_double)))
         return out
 
     def pa_step(self):

@@ -426,20 +426,42 @@ void SimulatedAnnealing::finalize() {
 // ===========================================================================
 // ParallelTempering (parallel_tempering.h:68-234)
 
-// The step observations a run makes on the reference's side: "step", then per
-// replica (label "replica") its temperature or beta (parallel_tempering.h:
-// 141-152); the per-step dynamics observables are refused at configure.
+// The step observations of parallel_tempering.h:141-188: "step", then per
+// replica (label "replica", the ladder position) its temperature or beta,
+// and -- from the device's per-step records of an observed run -- its
+// acceptance rate, cost and direction label, then each attempted pair's
+// swap under the lower position's label.
 void ParallelTempering::observe_run() {
+  if (observer_.empty()) return;
   std::vector<double> t = temperatures_.get_discretized_values();
   if (use_inverse_temperatures_) std::sort(t.begin(), t.end(), std::greater<double>());
   else std::sort(t.begin(), t.end(), std::less<double>());
   const char* name = use_inverse_temperatures_ ? "beta" : "T";
-  observe_steps(steps_done_, [&](int64_t) {
-    for (size_t i = 0; i < t.size(); ++i) {
+  const int L = (int)t.size();
+  std::vector<double> log;
+  if (observe_dynamics_ && w_) {
+    log.resize((size_t)steps_done_ * L * 4);
+    device_check(qmc_walkers_pt_log(w_, (int64_t)log.size(), log.data()), "qmc_walkers_pt_log");
+  }
+  const double evals = (double)model_->n_vars * sweeps_per_replica_;
+  observe_steps(steps_done_, [&](int64_t s) {
+    const double* rec = log.empty() ? nullptr : log.data() + (size_t)s * L * 4;
+    for (int i = 0; i < L; ++i) {
       observer_.set_label("replica", (int64_t)i);
       observer_.observe(name, use_inverse_temperatures_ ? beta_from(t[i], true) : (t[i] > 0 ? t[i] : 0.0));
+      if (rec) {
+        observer_.observe("acc_rate", rec[4 * i + 1] / evals);
+        observer_.observe("avg_cost", rec[4 * i + 0]);
+        observer_.observe("avg_dir", rec[4 * i + 2]);
+      }
       observer_.clear_label("replica");
     }
+    if (rec)
+      for (int i = (int)(s & 1); i < L - 1; i += 2) {
+        observer_.set_label("replica", (int64_t)i);
+        observer_.observe("swap_rate", rec[4 * i + 3]);
+        observer_.clear_label("replica");
+      }
   });
 }
 void ParallelTempering::configure(const Json& doc) {
@@ -447,10 +469,9 @@ void ParallelTempering::configure(const Json& doc) {
   const Json& p = params_of(doc);
   if (sequential())
     throw Error(kNotImplemented, "parallel tempering takes `sweep_order` 'colored' or 'index'");
+  observe_dynamics_ = false;
   for (const char* name : {"acc_rate", "avg_cost", "avg_dir", "swap_rate"})
-    if (observer_.watching(name))
-      throw Error(kNotImplemented, std::string("parallel tempering observes `T` / `beta` and `step` on this engine, "
-                                               "not `") + name + "`");
+    observe_dynamics_ = observe_dynamics_ || observer_.watching(name);
   if (p.has("sweeps_per_replica")) {
     sweeps_per_replica_ = (int)p.at("sweeps_per_replica").as_int();
     if (sweeps_per_replica_ <= 0) throw Error(kValueError, "`sweeps_per_replica` must be greater than 0");
@@ -505,6 +526,8 @@ void ParallelTempering::init() {
   if (L % B != 0) B = L;
   device_check(qmc_walkers_create(dmodel_->handle(visit()), L, B, seed_, 0, 0, L, &w_), "qmc_walkers_create");
   apply_lanes(w_, B);
+  if (observe_dynamics_)
+    device_check(qmc_walkers_pt_observe(w_, L, planned_steps()), "qmc_walkers_pt_observe");
   if (model_->has_initial) {
     std::vector<uint8_t> st((size_t)L * model_->n_vars, 0);
     for (int r = 0; r < L; ++r)
@@ -560,7 +583,7 @@ void ParallelTempering::run() {
 }
 
 bool ParallelTempering::batch_plan(qmc_batch b) {
-  if (time_limit_ || cost_limit_) return false;
+  if (time_limit_ || cost_limit_ || observe_dynamics_) return false;
   if (model_->empty()) return true;
   const int L = (int)betas_.size();
   const int64_t steps = planned_steps();

@@ -230,6 +230,7 @@ class ParallelTempering : public Solver {
   int64_t planned_steps();
 
  private:
+  bool observe_dynamics_ = false;  // the observer watches acc_rate, avg_cost, avg_dir or swap_rate
   bool use_inverse_temperatures_ = true;
   Schedule temperatures_;
   int sweeps_per_replica_ = 1;

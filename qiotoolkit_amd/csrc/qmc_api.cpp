@@ -278,6 +278,11 @@ struct qmc_walkers_s {
   DevBuf<int32_t> pa_family, pa_family_next, pa_src, pa_hist, pa_count, pa_ids;
   bool pa_stepped = false;  // a step was enqueued since qmc_walkers_pa_begin
   int ordered = 0;           // sweeps visit variables in index order (k_sweep_ordered)
+  // parallel tempering observations (graph-replayed steps only)
+  DevBuf<double> pt_log;
+  DevBuf<unsigned long long> pt_acc_prev;
+  DevBuf<int32_t> pt_dir;
+  PtObs pt_obs{};
   DevBuf<unsigned long long> pa_tacc;
   DevBuf<uint8_t> pa_plane_cur, pa_plane_low, pa_best_state;
   float last_ms = 0.f;
@@ -1411,7 +1416,7 @@ int qmc_walkers_pt(qmc_walkers w, int32_t ladder, const double* ladder_betas, in
   const size_t smem = sweep_smem_bytes(w->stride, 0);
   const bool can_coop = w->B != ladder && ladder % w->B == 0 &&
       w->n_units <= max_coresident_blocks(p->kind, p->pairwise, p->coupling, w->nb, w->lanes * w->G, smem);
-  if (w->B == ladder || (can_coop && !w->force_graph)) {
+  if (!w->pt_obs.log && (w->B == ladder || (can_coop && !w->force_graph))) {
     if ((rc = qmc_walkers_plan_pt(w, ladder, ladder_betas, step0, n_steps, sweeps_per_replica))) return rc;
     const UnitDev* du = nullptr;
     if ((rc = device_units(w, &du))) return rc;
@@ -1447,7 +1452,7 @@ int qmc_walkers_pt(qmc_walkers w, int32_t ladder, const double* ladder_betas, in
     // replayed; the kernels read their step from the device-side progress,
     // and replays past the planned steps do nothing.
     const int K = std::min<int32_t>(n_steps, 64);
-    const int key[4] = {K, sweeps_per_replica, w->n_units, ladder};
+    const int key[4] = {K, sweeps_per_replica, w->n_units, ladder};  // observation drops the graph
     if (!w->graph_exec || std::memcmp(key, w->graph_key, sizeof key) != 0) {
       drop_graph(w);
       HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
@@ -1460,7 +1465,7 @@ int qmc_walkers_pt(qmc_walkers w, int32_t ladder, const double* ladder_betas, in
         }
         HIP_TRY(launch_exchange(w->cost.p, w->beta.p, w->label.p, w->ladder_beta.p, n_ladders, ladder,
                                 w->progress.p, upl, sweeps_per_replica, w->last_step.p, seed0,
-                                w->stream, st));
+                                w->stream, w->pt_obs, st));
       }
       HIP_TRY(hipStreamEndCapture(st, &w->graph));
       HIP_TRY(hipGraphInstantiate(&w->graph_exec, w->graph, nullptr, nullptr, 0));
@@ -1848,6 +1853,44 @@ int qmc_walkers_set_sweep_order(qmc_walkers w, int32_t ordered) {
   w->ordered = ordered ? 1 : 0;
   drop_graph(w);
   return upload_units(w);
+}
+
+int qmc_walkers_pt_observe(qmc_walkers w, int32_t ladder, int64_t max_steps) {
+  if (!w) return fail(QMC_INVALID_ARGUMENT, "null handle");
+  if (ladder < 2 || ladder > 32 || w->R % ladder != 0)
+    return fail(QMC_INVALID_ARGUMENT, "ladder must be 2..32 and divide the replica count");
+  if (max_steps < 1) return fail(QMC_INVALID_ARGUMENT, "max_steps must be >= 1");
+  if (int rc0 = materialize(w)) return rc0;
+  const int n_ladders = w->R / ladder;
+  const size_t n = (size_t)n_ladders * (size_t)max_steps * ladder * 4;
+  HIP_TRY(w->pt_log.alloc(n));
+  HIP_TRY(w->pt_acc_prev.alloc(w->cap));
+  HIP_TRY(w->pt_dir.alloc((size_t)n_ladders * ladder));
+  // accepted counts so far, direction labels up (0) at the start
+  HIP_TRY(hipMemcpyAsync(w->pt_acc_prev.p, w->accepted.p, (size_t)w->cap * sizeof(unsigned long long),
+                         hipMemcpyDeviceToDevice, w->prob->ctx->stream));
+  ZeroSpans z{};
+  z.p[0] = w->pt_dir.p;
+  z.n[0] = (size_t)n_ladders * ladder * sizeof(int32_t);
+  z.p[1] = w->pt_log.p;
+  z.n[1] = n * sizeof(double);
+  HIP_TRY(launch_zero(z, 2, w->prob->ctx->stream));
+  w->pt_obs.log = w->pt_log.p;
+  w->pt_obs.log_steps = max_steps;
+  w->pt_obs.accepted = w->accepted.p;
+  w->pt_obs.acc_prev = w->pt_acc_prev.p;
+  w->pt_obs.dir = w->pt_dir.p;
+  drop_graph(w);
+  return QMC_OK;
+}
+
+int qmc_walkers_pt_log(qmc_walkers w, int64_t n_values, double* out) {
+  if (!w || (!out && n_values > 0)) return fail(QMC_INVALID_ARGUMENT, "null argument");
+  if (!w->pt_log.p) return fail(QMC_INVALID_ARGUMENT, "observation was not started");
+  if (n_values < 0 || (size_t)n_values > w->pt_log.n) return fail(QMC_INVALID_ARGUMENT, "n_values out of range");
+  HIP_TRY(hipStreamSynchronize(w->prob->ctx->stream));
+  if (n_values > 0) HIP_TRY(hipMemcpy(out, w->pt_log.p, (size_t)n_values * sizeof(double), hipMemcpyDeviceToHost));
+  return QMC_OK;
 }
 
 int qmc_walkers_get_lanes(qmc_walkers w, int32_t* lanes) {

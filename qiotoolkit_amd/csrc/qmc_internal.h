@@ -190,10 +190,22 @@ hipError_t launch_sweep(int kind, int pairwise, int coupling, int nb, const Unit
                         hipStream_t st);
 // largest grid of cooperative sweep work-groups that can be co-resident
 int max_coresident_blocks(int kind, int pairwise, int coupling, int nb, int block, size_t smem);
+// Per-step observations of graph-replayed parallel tempering
+// (parallel_tempering.h:141-188): for ladder ld, step s and ladder position
+// k, log[((ld * log_steps + s) * L + k) * 4 + {0,1,2,3}] = cost after the
+// sweeps, transitions accepted during the step, direction label (0 up, 1
+// down) before the swaps, swap of pair (k, k+1) (-1: not attempted).
+struct PtObs {
+  double* log;
+  int64_t log_steps;
+  const unsigned long long* accepted;  // per replica, cumulative
+  unsigned long long* acc_prev;        // per replica, at the last step
+  int32_t* dir;                        // per ladder position
+};
 hipError_t launch_exchange(double* cost, double* beta, int32_t* label, const double* lbeta,
                            int n_ladders, int L, const int64_t* progress, int units_per_ladder,
                            int spr, int64_t* last_step, uint32_t seed, uint32_t stream,
-                           hipStream_t st);
+                           const PtObs& obs, hipStream_t st);
 hipError_t launch_init(const UnitDev* units, int n_units, int N, uint32_t epoch, hipStream_t st);
 // Set-up of one freshly created unit (one work-group per entry): optionally
 // clear its counters (accepted, beta, label, the walker set's error flag),

@@ -866,7 +866,7 @@ __global__ void k_exchange(double* __restrict__ cost, double* __restrict__ beta,
                            int32_t* __restrict__ label, const double* __restrict__ lbeta,
                            int n_ladders, int L, const int64_t* __restrict__ progress,
                            int units_per_ladder, int spr, int64_t* __restrict__ last_step,
-                           uint32_t seed0, uint32_t stream) {
+                           uint32_t seed0, uint32_t stream, PtObs obs) {
   const int ld = blockIdx.x;
   if (ld >= n_ladders) return;
   const int64_t next = progress[2 * (int64_t)ld * units_per_ladder];
@@ -878,6 +878,20 @@ __global__ void k_exchange(double* __restrict__ cost, double* __restrict__ beta,
   const uint32_t step = (uint32_t)step64;
   const int p = threadIdx.x;
   const int i = (int)(step & 1u) + 2 * p;
+  // observations of the step, before its swaps
+  const bool ob = obs.log != nullptr && step64 < obs.log_steps;
+  double* rec = ob ? obs.log + ((int64_t)ld * obs.log_steps + step64) * L * 4 : nullptr;
+  int32_t* dir = ob ? obs.dir + ld * L : nullptr;
+  if (ob && p < L) {
+    const int slot = ld * L + label[ld * L + p];
+    const unsigned long long a = obs.accepted[slot];
+    rec[4 * p + 0] = cost[slot];
+    rec[4 * p + 1] = (double)(a - obs.acc_prev[slot]);
+    obs.acc_prev[slot] = a;
+    rec[4 * p + 2] = (double)dir[p];
+    rec[4 * p + 3] = -1.0;
+  }
+  __syncthreads();
   if (i < L - 1) {
     int32_t* lab = label + ld * L;
     const int sa = ld * L + lab[i], sb = ld * L + lab[i + 1];
@@ -886,14 +900,27 @@ __global__ void k_exchange(double* __restrict__ cost, double* __restrict__ beta,
     const double u = canonical2(w.x, w.y);
     const double bi = lbeta[i], bj = lbeta[i + 1];
     const double arg = __dmul_rn(__dsub_rn(cost[sa], cost[sb]), __dsub_rn(bi, bj));
-    if (u < exp(arg)) {
+    const bool swap = u < exp(arg);
+    if (swap) {
       lab[i] = sb - ld * L;
       lab[i + 1] = sa - ld * L;
       beta[sa] = bj;
       beta[sb] = bi;
     }
+    if (ob) {
+      rec[4 * i + 3] = swap ? 1.0 : 0.0;
+      if (swap) {  // the direction labels travel with the states
+        const int32_t t = dir[i];
+        dir[i] = dir[i + 1];
+        dir[i + 1] = t;
+      }
+    }
   }
   __syncthreads();
+  if (ob && threadIdx.x == 0) {
+    dir[0] = 0;
+    dir[L - 1] = 1;
+  }
   if (threadIdx.x == 0) last_step[ld] = step64;
 }
 
@@ -1965,10 +1992,11 @@ int max_coresident_blocks(int kind, int pairwise, int coupling, int nb, int bloc
 hipError_t launch_exchange(double* cost, double* beta, int32_t* label, const double* lbeta,
                            int n_ladders, int L, const int64_t* progress, int units_per_ladder,
                            int spr, int64_t* last_step, uint32_t seed, uint32_t stream,
-                           hipStream_t st) {
+                           const PtObs& obs, hipStream_t st) {
   if (n_ladders <= 0 || L < 2) return hipSuccess;
+  if (obs.log && L > 32) return hipErrorInvalidValue;  // one thread per ladder position
   hipLaunchKernelGGL(k_exchange, dim3(n_ladders), dim3(32), 0, st, cost, beta, label, lbeta,
-                     n_ladders, L, progress, units_per_ladder, spr, last_step, seed, stream);
+                     n_ladders, L, progress, units_per_ladder, spr, last_step, seed, stream, obs);
   return hipGetLastError();
 }
 

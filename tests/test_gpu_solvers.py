@@ -361,3 +361,26 @@ def test_pa_eval_limit(engine):
     reason = sol["exit_reason"]
     assert reason.startswith("Stop due to Evaluation limit: 20000, evaluations:")
     assert int(reason.rsplit(":", 1)[1]) >= 20000
+
+
+def test_pt_observer_render(engine):
+    """The reference's PT observer configuration
+    (parallel_tempering_test.cc:113-131) on a device solve: one entry per
+    replica with T, acc_rate, avg_cost, avg_dir and (all but the last)
+    swap_rate; the lowest temperature's direction is always up (0), the
+    highest's average is near 1, rates lie in [0, 1]."""
+    doc = {"cost_function": {"type": "ising", "terms": wl.lattice2d(8, seed=3)}}
+    p = {"params": {"seed": 42, "step_limit": 200,
+                    "temperatures": {"type": "linear", "initial": 3.0, "final": 0.5, "count": 5}},
+         "observer": {"group_by": ["replica"], "constant": ["T"],
+                      "average": ["avg_cost", "acc_rate", "avg_dir", "swap_rate"]}}
+    res = engine.solve("paralleltempering.qiotoolkit", p, doc)
+    sol = {e["name"]: e["value"] for e in res["benchmark"]["extensions"]}["solver"]
+    reps = sol["replica"]
+    assert [r["T"] for r in reps] == [0.5, 1.125, 1.75, 2.375, 3.0]
+    assert all(set(r) == {"T", "acc_rate", "avg_cost", "avg_dir", "swap_rate"} for r in reps[:-1])
+    assert "swap_rate" not in reps[-1]
+    assert reps[0]["avg_dir"] == 0 and reps[-1]["avg_dir"] > 0.9
+    assert all(0 <= r["acc_rate"] <= 1 for r in reps)
+    assert reps[0]["avg_cost"] < reps[-1]["avg_cost"]
+    assert res["solutions"]["cost"] <= reps[0]["avg_cost"]

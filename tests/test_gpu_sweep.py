@@ -237,6 +237,42 @@ def test_pt_matches_oracle(ctx, name, B, L, mode):
         assert np.allclose(got["beta"][sl], want["slot_beta"])
 
 
+@pytest.mark.parametrize("B", [1, 4, 8])
+def test_pt_observations_match_oracle(ctx, B):
+    """Observed parallel tempering (qmc_walkers_pt_observe: graph-replayed
+    steps whose exchange launch records each ladder position's cost,
+    acceptances, direction label and pair swap) against the oracle's
+    counter-mode run of the same ladders: per-position step averages of
+    acc_rate, avg_cost, avg_dir and swap_rate as parallel_tempering.h:141-188
+    observes them -- exact for integer couplings -- and the same final
+    states as an unobserved run."""
+    kind, gen = PROBLEMS["er_int"]
+    op, qp = pair(ctx, kind, gen())
+    L, n_ladders, steps = 8, 2, 30
+    temps = wl.geometric_temperatures(0.3, 3.0, L)
+    betas = sorted([1.0 / t for t in temps], reverse=True)
+    w = _qmc.Walkers(qp, L * n_ladders, B, seed=21, replicas_per_seed=L)
+    order, _ = qp.order(w.lanes())
+    w.init()
+    w.pt_observe(L, steps)
+    w.pt(betas, 0, steps)
+    got = w.read()
+    log = w.pt_log()
+    for ld in range(n_ladders):
+        opts = po.make_opts(po.RNG_COUNTER, seed=21 + ld, arith=arith_of(qp), order=order, steps_per_tick=1)
+        want = op.pt(opts, temps, False, steps)
+        sl = slice(ld * L, (ld + 1) * L)
+        assert np.array_equal(got["states"][sl], want["final_states"])
+        rec = log[ld]
+        assert np.array_equal(rec[:, :, 0].mean(axis=0), want["avg_cost"])
+        assert np.allclose(rec[:, :, 1].mean(axis=0) / op.n, want["acc_rate"], rtol=1e-14, atol=0)
+        assert np.array_equal(rec[:, :, 2].mean(axis=0), want["avg_dir"])
+        sw = rec[:, :L - 1, 3]
+        tried = sw >= 0
+        assert np.array_equal(tried.sum(axis=0), [steps // 2] * (L - 1))
+        assert np.allclose(np.where(tried, sw, 0).sum(axis=0) / tried.sum(axis=0), want["swap_rate"][:L - 1])
+
+
 def test_batch_mixed_problems_same_as_alone(ctx):
     """Work-groups of different problems, sizes and kernel variants launched
     together, in chunks, give exactly the results of each run alone."""
