@@ -197,7 +197,8 @@ __device__ __forceinline__ void accumulate(uint32_t wv, CT cc, CT (&h)[8]) {
 // prefetched ELL row (entries beyond the first 8 are loaded here).
 template <int KIND, bool PAIR, typename CT, int NB>
 __device__ __forceinline__ void local_field(const UnitDev& U, const uint8_t* sp, int q, const EllRow& e,
-                                            int2 slice, int tp, int lg, int G, int g, CT (&h)[8]) {
+                                            int2 slice, int tp, int lg, int G, int g, CT (&h)[8], int sub = 0,
+                                            int lgs = 0) {
 #pragma unroll
   for (int b = 0; b < NB; ++b) h[b] = CT(0);
   if (PAIR) {
@@ -221,17 +222,25 @@ __device__ __forceinline__ void local_field(const UnitDev& U, const uint8_t* sp,
   }
   if (U.ell4_ent) {
     // sliced entries {coefficient, up to 3 other members} in term order;
-    // the ghost position pads missing members (its spin is always 0)
+    // the ghost position pads missing members (its spin is always 0).  With
+    // 2^lgs lanes per position (tp = the position's lane), this lane takes
+    // entries sub, sub + 2^lgs, ...
     gptr<uint64_t> p = glob(reinterpret_cast<const uint64_t*>(U.ell4_ent)) + 2 * (slice.x + tp);
     const int step = 2 << lg;
-#pragma unroll 8  // eight entries' loads in flight (deeper unrolling runs out of registers)
-    for (int j = 0; j < slice.y; ++j) {
+    auto entry = [&](int j) {
       const uint64_t lo = p[j * step], hi = p[j * step + 1];
       const uint32_t w0 = sp[(int)(uint32_t)(lo >> 32) * G + g];
       const uint32_t w1 = sp[(int)(uint32_t)hi * G + g];
       const uint32_t w2 = sp[(int)(uint32_t)(hi >> 32) * G + g];
       const uint32_t w = (KIND == QMC_ISING) ? (w0 ^ w1 ^ w2) : (w0 | w1 | w2);
       accumulate<KIND, CT, NB>(w, coef_of<CT>((uint32_t)lo), h);
+    };
+    if (lgs == 0) {
+#pragma unroll 8  // eight entries' loads in flight (deeper unrolling runs out of registers)
+      for (int j = 0; j < slice.y; ++j) entry(j);
+    } else {
+#pragma unroll 4
+      for (int j = sub; j < slice.y; j += 1 << lgs) entry(j);
     }
     return;
   }
@@ -403,8 +412,20 @@ k_sweep(const UnitDev* units, SweepArgs a) {
   // class is already in flight while the previous class finishes.
   EllRow ra, rb;
   int2 sla = make_int2(0, 0), slb = make_int2(0, 0);
-  auto fetch_row = [&](int cc0, int csb, int k, int2& sl, EllRow& e) {
-    const int rho = (k << lg) + tp;
+  // Small classes of general problems with integer couplings share each
+  // position among 2^lgs lanes (at most 8, while positions x lanes per
+  // position fit the lanes): each sums every 2^lgs-th incident term and a
+  // butterfly adds the partial fields (integer sums: the order does not
+  // matter), so a round waits for a fraction of the position's loads.
+  constexpr bool kSplit = !PAIR && std::is_same<CT, int32_t>::value;
+  auto split_of = [&](int ncls) {
+    int l = 0;
+    if (kSplit && gen_ell)
+      while (l < 3 && (ncls << (l + 1)) <= TP) ++l;
+    return l;
+  };
+  auto fetch_row = [&](int cc0, int csb, int k, int2& sl, EllRow& e, int lgs) {
+    const int rho = (k << lg) + (tp >> lgs);
     if (PAIR) {
       sl = load_slice(U, csb + k);
       ell_fetch(U, sl, tp, lg, e);
@@ -413,7 +434,7 @@ k_sweep(const UnitDev* units, SweepArgs a) {
     }
     e.var = (uint32_t)var_of_pos[cc0 + rho];  // padded past position N - 1
   };
-  if (active) fetch_row(class_off[0], slices ? slice_base[0] : 0, 0, sla, ra);
+  if (active) fetch_row(class_off[0], slices ? slice_base[0] : 0, 0, sla, ra, split_of(class_off[1] - class_off[0]));
 
   // parallel tempering: exchange step of the current sweep, sweeps run into
   // it and its energy buffer (step mod 3), advanced per sweep
@@ -446,6 +467,7 @@ k_sweep(const UnitDev* units, SweepArgs a) {
       const int cn = c + 1 < n_classes ? c + 1 : 0;  // the class after this one
       const int c0n = class_off[cn];
       const int sbasen = slices ? slice_base[cn] : 0;
+      const int lgs = split_of(nc), lgsn = split_of(class_off[cn + 1] - c0n);
       // per replica: sum of the accepted dE of this lane's positions, the
       // minimum prefix of that sum and the round where it is first reached
       AT Ssum[8], Smin[8];
@@ -464,10 +486,11 @@ k_sweep(const UnitDev* units, SweepArgs a) {
         // loads stay in flight across the processing of the other buffer)
         auto fetch = [&](int k, int2& sl, EllRow& e) {
           const bool here = k < K;
-          fetch_row(here ? c0 : c0n, here ? sbase : sbasen, here ? k : k - K, sl, e);
+          fetch_row(here ? c0 : c0n, here ? sbase : sbasen, here ? k : k - K, sl, e, here ? lgs : lgsn);
         };
         auto process = [&](int k, const int2& sl, const EllRow& e) {
-          const int rho = (k << lg) + tp;
+          const int sub = tp & ((1 << lgs) - 1);
+          const int rho = (k << lg) + (tp >> lgs);
           // The term lists of general problems end at N - 1; pairwise rows
           // past the class end are ghost padding, so those lanes run the
           // same branch-free path on the ghost position (always 0, its flips
@@ -476,7 +499,14 @@ k_sweep(const UnitDev* units, SweepArgs a) {
           const bool valid = rho < nc;
           const int q = valid ? c0 + rho : N;
           CT h[8];
-          local_field<KIND, PAIR, CT, NB>(U, sp, q, e, sl, tp, lg, G, g, h);
+          local_field<KIND, PAIR, CT, NB>(U, sp, q, e, sl, tp >> lgs, lg, G, g, h, sub, lgs);
+          if (kSplit && lgs) {
+            // every lane of a position is active here (they share rho)
+#pragma unroll
+            for (int b = 0; b < NB; ++b)
+              for (int o = 1; o < (1 << lgs); o <<= 1) h[b] += __shfl_xor(h[b], o);
+            if (sub) return;
+          }
           const uint32_t var = e.var;
           const uint32_t own = sp[q * G + g];
           CT dE[8];
@@ -579,7 +609,8 @@ k_sweep(const UnitDev* units, SweepArgs a) {
           const AT cnd = run + Slots<AT>::mn(S, par, gg, bb)[w];
           if (bt < 0 || cnd < bestc) {
             bestc = cnd;
-            bt = w * 64 + S.wlane[par][gg][bb][w];
+            // the lane's position (lanes share positions in split classes)
+            bt = (w * 64 + S.wlane[par][gg][bb][w]) >> lgs;
             bk = S.wk[par][gg][bb][w];
           }
           run += Slots<AT>::sum(S, par, gg, bb)[w];
