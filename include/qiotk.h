@@ -8,7 +8,10 @@
  * (utils/error_handling.h:25-52) with a message.
  *
  * Targets: simulatedannealing.qiotoolkit, paralleltempering.qiotoolkit,
- * populationannealing.cpu (app/runner.cc:137-148).
+ * populationannealing.cpu (app/runner.cc:137-148), and the parameter-free
+ * simulatedannealing-parameterfree.qiotoolkit and
+ * paralleltempering-parameterfree.qiotoolkit (app/runner.cc:172-181; not
+ * taken by the serving loop).
  */
 #ifndef QIOTK_H
 #define QIOTK_H

@@ -21,7 +21,8 @@ HIP_FLAGS = [
 
 DEVICE_SOURCES = ["qmc_kernels.hip", "qmc_api.cpp"]
 HOST_SOURCES = ["host/json.cc", "host/model.cc", "host/schedule.cc", "host/solver.cc", "host/pa.cc",
-                "host/capi.cc", "host/server.cc", "host/proto.cc", "host/observer.cc"]
+                "host/capi.cc", "host/server.cc", "host/proto.cc", "host/observer.cc",
+                "host/param_free.cc"]
 
 
 def _run(cmd):

@@ -485,6 +485,10 @@ static Json describe_model(const Model& m) {
   j["term_vars"] = vars;
   j["coeffs"] = cs;
   j["scale_factor"] = m.scale_factor();
+  if (!m.empty()) {
+    j["estimate_max_cost_diff"] = m.estimate_max_cost_diff();
+    j["estimate_min_cost_diff"] = m.estimate_min_cost_diff();
+  }
   Json ic = Json::array();
   for (auto& kv : m.initial) {
     Json e = Json::array();

@@ -2,6 +2,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <iterator>
+#include <limits>
 #include <set>
 
 namespace qiotk {
@@ -155,6 +157,87 @@ const qmc_problem_info& DeviceModel::info(int visit) {
 DeviceModel::~DeviceModel() {
   for (qmc_problem p : p_)
     if (p) qmc_problem_destroy(p);
+}
+
+namespace {
+
+// Largest differencing (Karmarkar-Karp) over one multiset, as
+// utils::least_diff_method (utils/utils.cc:43-53): replace the two largest
+// values by their difference until one value is left.
+double largest_differencing(std::multiset<double> s) {
+  if (s.empty()) return 0.0;
+  while (s.size() > 1) {
+    auto a = std::prev(s.end());
+    const double x = *a;
+    s.erase(a);
+    auto b = std::prev(s.end());
+    const double y = *b;
+    s.erase(b);
+    s.insert(x - y);
+  }
+  return *s.begin();
+}
+
+// The two-set form (utils/utils.cc:55-100): the largest of each set are
+// paired off, the remainder returns to the set it came from (a tie returns
+// one of them to the smaller set), and the result is the least nonzero
+// remainder seen; with one set empty, the other's least value.
+double largest_differencing(std::multiset<double> a, std::multiset<double> b) {
+  double least = std::numeric_limits<double>::max();
+  if (a.empty() && !b.empty()) return *b.begin();
+  if (!a.empty() && b.empty()) return *a.begin();
+  while (!a.empty() && !b.empty()) {
+    auto ia = std::prev(a.end());
+    const double x = *ia;
+    a.erase(ia);
+    auto ib = std::prev(b.end());
+    const double y = *ib;
+    b.erase(ib);
+    double d = x - y;
+    if (d > 0) {
+      a.insert(d);
+    } else if (d < 0) {
+      d = -d;
+      b.insert(d);
+    } else if (a.size() < b.size()) {
+      a.insert(x);
+    } else {
+      b.insert(y);
+    }
+    if (least > d && d != 0) least = d;
+  }
+  return least;
+}
+
+}  // namespace
+
+double Model::estimate_max_cost_diff() const {
+  std::vector<double> sum(n_vars, 0.0);
+  for (int64_t t = 0; t < n_terms(); ++t)
+    for (int64_t k = term_off[t]; k < term_off[t + 1]; ++k) sum[term_var[k]] += std::fabs(coeff[t]);
+  double mx = 0.0;
+  for (double v : sum) mx = std::max(mx, v);
+  return kind == QMC_ISING ? 2.0 * mx : mx;
+}
+
+double Model::estimate_min_cost_diff() const {
+  const double none = std::numeric_limits<double>::max();
+  std::vector<std::multiset<double>> pos(n_vars), neg(kind == QMC_ISING ? 0 : n_vars);
+  for (int64_t t = 0; t < n_terms(); ++t)
+    for (int64_t k = term_off[t]; k < term_off[t + 1]; ++k) {
+      const int32_t v = term_var[k];
+      if (kind == QMC_ISING) pos[v].insert(std::fabs(coeff[t]));
+      else if (coeff[t] > 0) pos[v].insert(coeff[t]);
+      else if (coeff[t] < 0) neg[v].insert(-coeff[t]);
+    }
+  double least = none;
+  for (int32_t v = 0; v < n_vars; ++v) {
+    const double d = kind == QMC_ISING ? largest_differencing(pos[v]) : largest_differencing(pos[v], neg[v]);
+    if (d == 0) continue;
+    least = std::min(least, d);
+  }
+  if (least == none) least = 0.1;  // MIN_DELTA_DEFAULT (model/graph_model.h:16)
+  return kind == QMC_ISING ? 2.0 * least : least;
 }
 
 }  // namespace qiotk

@@ -48,6 +48,14 @@ struct Model {
   Json render(const uint8_t* bits) const;
   // FP64 cost of a state (host-side, used for the const-cost-free checks)
   double cost(const uint8_t* bits) const;
+  // Bounds on the cost change of one flip that set the parameter-free beta
+  // range: the largest sum of |c| over a variable's terms (graph/graph.h:453-487),
+  // and the least difference among a variable's coefficients by largest
+  // differencing (utils/utils.cc:43-100), minimised over variables, 0.1 when
+  // every variable's is 0; Ising doubles both (model/ising.h:176-224), PUBO
+  // splits positive and negative coefficients (model/pubo.h:127-185).
+  double estimate_max_cost_diff() const;
+  double estimate_min_cost_diff() const;
 };
 
 // Parse {"cost_function": {...}} (or the inner object) and normalise it.

@@ -167,6 +167,8 @@ struct qiotk_server_s {
       std::unique_ptr<Solver> s;
       try {
         s = create_solver(r.target);
+        if (!s->servable())
+          throw Error(kNotImplemented, "the serving loop does not run target " + r.target);
         s->configure(r.params);
         fp = footprint(r.model, r.target, s->device_replicas());
       } catch (const Error& e) {

@@ -619,6 +619,8 @@ std::unique_ptr<Solver> create_solver(const std::string& target) {
   if (target == "paralleltempering.qiotoolkit") return std::make_unique<ParallelTempering>();
   if (target == "populationannealing.cpu" || target == "populationannealing.qiotoolkit")
     return std::make_unique<PopulationAnnealing>();
+  if (target == "simulatedannealing-parameterfree.qiotoolkit") return std::make_unique<ParameterFree>(false);
+  if (target == "paralleltempering-parameterfree.qiotoolkit") return std::make_unique<ParameterFree>(true);
   throw Error(kValueError, "Unknown solver: " + target);
 }
 

@@ -72,8 +72,10 @@ class Solver {
   bool best(double* cost, int8_t* config) const;
   // Resolved run plan (steps, betas), computable without a device.
   virtual Json describe() const;
-  Json get_result() const;
+  virtual Json get_result() const;
   Json get_solutions() const;  // ModelSolver::get_solutions (model_solver.h:111-150)
+  // the serving loop can advance this solve round by round
+  virtual bool servable() const { return true; }
 
   int64_t attempts() const { return attempts_; }
   int64_t accepted() const { return accepted_; }
@@ -176,6 +178,7 @@ class Solver {
   std::string exit_reason_;
   int64_t batched_replicas_ = 0;
   bool served_ = false;
+  friend class ParameterFree;  // reads its workers' results
 };
 
 // simulated_annealing.h
@@ -283,6 +286,52 @@ class PopulationAnnealing : public Solver {
   int64_t restarts_ = 0;
   int group_ = 0;
   std::shared_ptr<void> impl_;
+};
+
+// Parameter-free simulated annealing or parallel tempering
+// (solver/parameter_free_solver.h:11-257 with the linear search of
+// parameter_free_linear_solver.h:8-30 and strategy/linear_search.cc:13-122;
+// adapters sa_parameter_free.h:18-175, pt_parameter_free.h:18-190): repeated
+// worker runs whose sweeps walk a set of sample points that stretches (best
+// at the last point) or shrinks (best at the first) by 1.5 after every pass,
+// until 8 runs in a row bring no lower cost or the time limit (5 s unless
+// given) is spent.  Every pass over the sample points is planned before its
+// first run, so its runs are fused into one device batch; their results are
+// then taken in order with the reference's bookkeeping, and the runs after a
+// stop inside a pass are discarded.
+class ParameterFree : public Solver {
+ public:
+  explicit ParameterFree(bool tempering) : tempering_(tempering) {}
+  std::string target() const override;
+  void configure(const Json& doc) override;
+  void init() override {}
+  void run() override;
+  void finalize() override;
+  Json describe() const override;
+  Json get_result() const override;
+  bool servable() const override { return false; }
+  int64_t run_replicas() const override { return min_num_; }
+
+ private:
+  std::unique_ptr<Solver> worker(const std::vector<double>& params, uint32_t seed) const;
+  Json worker_params(const std::vector<double>& params, uint32_t seed) const;
+  Json output_for(const std::vector<double>& params) const;
+  double execution_cost(const std::vector<double>& params) const;
+  std::vector<double> betas() const;  // PTParameterFree::determine_all_betas_
+
+  bool tempering_;
+  Json base_ = Json::object();  // the request's params, forwarded to the workers where they apply
+  double max_prob_ = 0.5, min_prob_ = 0.0001, adjustment_ = 0.001;
+  double beta_start_ = 0.01, beta_stop_ = 10.0;
+  double max_diff_ = 0.0, min_diff_ = 0.0;
+  std::vector<int> points_{10, 20, 30, 50};
+  int64_t min_num_ = 64;  // min_num_restarts (SA) or min_num_betas (PT)
+  int nonimprovement_limit_ = 8;
+  double change_factor_ = 1.5;
+  std::vector<double> best_params_;
+  std::unique_ptr<Solver> last_;  // the last worker taken, source of the extra solutions
+  Json training_ = Json::object();
+  int64_t runs_ = 0, passes_ = 0, discarded_ = 0;
 };
 
 // Replicas per work-group when the params do not say: small groups while

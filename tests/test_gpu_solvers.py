@@ -384,3 +384,52 @@ def test_pt_observer_render(engine):
     assert all(0 <= r["acc_rate"] <= 1 for r in reps)
     assert reps[0]["avg_cost"] < reps[-1]["avg_cost"]
     assert res["solutions"]["cost"] <= reps[0]["avg_cost"]
+
+
+PF_TARGETS = ["simulatedannealing-parameterfree.qiotoolkit", "paralleltempering-parameterfree.qiotoolkit"]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("fname,cost,config", FIXTURES)
+@pytest.mark.parametrize("target", PF_TARGETS)
+def test_parameter_free_reference_fixtures(engine, fname, cost, config, target):
+    """sa_pf_test.cc / pt_pf_test.cc: the reference's fixtures with pf5s.json
+    ({"timeout": 5}) reach the same costs and configurations."""
+    doc = load(fname)
+    res = engine.solve(target, {"params": {"timeout": 5, "seed": 11}}, doc)
+    sol = res["solutions"]
+    assert sol["cost"] == cost
+    for k, v in (config or {}).items():
+        assert sol["configuration"][k] == v
+    if "configuration" in sol:
+        assert oracle_cost(doc, sol["configuration"]) == sol["cost"]
+        for s in sol["solutions"]:
+            assert oracle_cost(doc, s["configuration"]) == s["cost"]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("target", PF_TARGETS)
+def test_parameter_free_search(engine, target):
+    """The search stops after 8 runs without a lower cost, well inside the
+    time limit; the best run's parameters are reported; the planted optimum
+    of a 16-variable Wishart instance is found."""
+    family = "wishart_planting_N_16_alpha_0.81_qualitytest"
+    case = json.load(open(os.path.join(GOLD, "e2e", family, "energies.json")))["cases"][0]
+    doc = json.load(open(os.path.join(GOLD, "e2e", family, case["file"])))
+    res = engine.solve(target, {"params": {"timeout": 30, "seed": 3}}, doc)
+    assert res["solutions"]["cost"] == case["energy"]
+    sol = [e for e in res["benchmark"]["extensions"] if e["name"] == "solver"][0]["value"]
+    tr = sol["training"]
+    assert tr["runs"] >= 8 and tr["search_ms"] < 30e3
+    assert "without a lower cost" in sol["search_exit_reason"]
+    par = res["solutions"]["parameters"]
+    assert par["sweeps"] >= 2
+    if target.startswith("simulated"):
+        assert par["restarts"] >= 64 and par["beta_stop"] > par["beta_start"] > 0
+    else:
+        assert len(par["all_betas"]) == 32
+    # same seed, same search
+    again = engine.solve(target, {"params": {"timeout": 30, "seed": 3}}, doc)
+    assert again["solutions"]["parameters"] == par
+    assert [e for e in again["benchmark"]["extensions"] if e["name"] == "solver"][0]["value"]["training"]["runs"] \
+        == tr["runs"]
