@@ -141,13 +141,12 @@ void ParameterFree::configure(const Json& doc) {
   if (!model_->empty()) {
     max_diff_ = model_->estimate_max_cost_diff();
     min_diff_ = model_->estimate_min_cost_diff();
-    auto inconsistent = [](const std::string& m) { throw Error(kInconsistentValue, m); };
     std::ostringstream o;
-    if (max_diff_ <= 0) o << "max cost difference should be > 0, found :" << max_diff_, inconsistent(o.str());
-    if (min_diff_ <= 0) o << "min cost difference should be > 0, found :" << min_diff_, inconsistent(o.str());
-    if (min_diff_ > max_diff_)
-      o << "min cost difference should be <= max cost difference, found :" << min_diff_ << " > " << max_diff_,
-          inconsistent(o.str());
+    if (max_diff_ <= 0) o << "max cost difference should be > 0, found :" << max_diff_;
+    else if (min_diff_ <= 0) o << "min cost difference should be > 0, found :" << min_diff_;
+    else if (min_diff_ > max_diff_)
+      o << "min cost difference should be <= max cost difference, found :" << min_diff_ << " > " << max_diff_;
+    if (!o.str().empty()) throw Error(kInconsistentValue, o.str());
     beta_start_ = -std::log(max_prob_) / max_diff_;
     beta_stop_ = -std::log(min_prob_) / (min_diff_ + adjustment_ * (max_diff_ - min_diff_));
   }
@@ -184,7 +183,8 @@ std::vector<double> ParameterFree::betas() const {
 // (sa_parameter_free.h:127-144, pt_parameter_free.h:117-137): sweeps =
 // max(2, point); SA restarts = min_num_restarts * last point / point.
 Json ParameterFree::output_for(const std::vector<double>& params) const {
-  const int64_t s = std::lround(params[0]), back = std::lround(params[1]);
+  // a shrinking pass can take a small point of an unsorted list to 0
+  const int64_t s = std::max<long>(1, std::lround(params[0])), back = std::lround(params[1]);
   Json o = Json::object();
   if (tempering_) {
     Json a = Json::array();
