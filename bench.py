@@ -362,10 +362,18 @@ def cpu_baseline_c2(inp, temps, args):
     p.pt(o, tvals, False, sweeps)
     dt = time.perf_counter() - t0
     attempts = 32 * 4096 * sweeps
+    # the same ladder on one thread, over a fifth of the sweeps (bounded)
+    o1 = po.make_opts(po.RNG_REFERENCE, seed=7, threads=1, steps_per_tick=1)
+    sweeps1 = max(20, sweeps // 5)
+    t1 = time.perf_counter()
+    p.pt(o1, tvals, False, sweeps1)
+    dt1 = time.perf_counter() - t1
     return {"value": attempts / dt, "unit": "spin-flip attempts/s", "cores": threads,
             "kind": "port",
+            "single_thread": 32 * 4096 * sweeps1 / dt1,
             "sample": "one 32-replica ladder of the same workload, %d of 5000 sweeps, %.1f s, "
-                      "pcg32 + ln-table accept, OpenMP over replicas" % (sweeps, dt)}
+                      "pcg32 + ln-table accept, OpenMP over replicas; single_thread: %d sweeps on 1 "
+                      "thread, %.1f s" % (sweeps, dt, sweeps1, dt1)}
 
 
 def main():
