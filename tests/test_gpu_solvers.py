@@ -433,3 +433,15 @@ def test_parameter_free_search(engine, target):
     assert again["solutions"]["parameters"] == par
     assert [e for e in again["benchmark"]["extensions"] if e["name"] == "solver"][0]["value"]["training"]["runs"] \
         == tr["runs"]
+
+
+@pytest.mark.gpu
+def test_parameter_free_time_limit(engine):
+    """A spent time limit ends the search after its first pass
+    (parameter_free_solver.h:101-108): the four runs of [10, 20, 30, 50]."""
+    doc = load("ising1.json")
+    res = engine.solve("simulatedannealing-parameterfree.qiotoolkit", {"params": {"timeout": 1e-9, "seed": 2}}, doc)
+    assert res["solutions"]["cost"] == -10
+    sol = [e for e in res["benchmark"]["extensions"] if e["name"] == "solver"][0]["value"]
+    assert sol["training"]["runs"] == 4 and sol["training"]["passes"] == 1
+    assert sol["search_exit_reason"].startswith("Stop due to time limit")
