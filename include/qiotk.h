@@ -125,7 +125,10 @@ void qiotk_server_destroy(qiotk_server s);
 const char* qiotk_server_last_error(void);
 
 /* Device-free host logic, for tooling and tests. */
-/* graph::normalize_edges result as JSON: names, term CSR, const_cost, ... */
+/* graph::normalize_edges result as JSON: names, term CSR, const_cost, ...,
+ * and for a non-empty model estimate_max_cost_diff / estimate_min_cost_diff
+ * (model/ising.h:176-224, model/pubo.h:127-185), which set the
+ * parameter-free beta range */
 int qiotk_parse_model(const char* input_json, char** response);
 /* the same for an input file (JSON document or protobuf problem folder) */
 int qiotk_parse_model_file(const char* path, char** response);
