@@ -237,6 +237,7 @@ void ParameterFree::run() {
   int cur = -1;
   int stable = 0;
   double exec_best = 0.0;
+  double rate = 0.0;  // flip attempts per second of the last pass
   bool stop = false;
   have_lowest_ = false;
   while (!stop) {
@@ -262,9 +263,28 @@ void ParameterFree::run() {
     // the rest of the pass: parameters {point, last point} (linear_search.cc:53-93)
     std::vector<std::vector<double>> plan;
     for (int i = cur + 1; i < (int)points.size(); ++i) plan.push_back({(double)points[i], (double)points.back()});
+    // The reference hands each run the time left as its own limit and cuts
+    // it there; fused runs are not cut, so a pass that would not end in the
+    // time left, at the rate the last pass ran, is not started.
+    double planned = 0.0;
+    for (const auto& pr : plan) {
+      const Json o = output_for(pr);
+      const double reps = tempering_ ? (double)o.at("all_betas").size() * (base_.has("sweeps_per_replica")
+                                                                               ? (double)base_.at("sweeps_per_replica").as_int()
+                                                                               : 1.0)
+                                     : (double)o.at("restarts").as_int();
+      planned += (double)o.at("sweeps").as_int() * reps * (double)model_->n_vars;
+    }
+    if (have_lowest_ && rate > 0 && wall_s() - t0 + planned / rate > *time_limit_) {
+      exit_reason_ = "Stop due to time limit: " + std::to_string(*time_limit_) +
+                     " (the next pass would not end within it)";
+      break;
+    }
     std::vector<std::unique_ptr<Solver>> ws;
     for (size_t k = 0; k < plan.size(); ++k) ws.push_back(worker(plan[k], run_seed(seed_, runs_ + (int64_t)k)));
+    const double tp = wall_s();
     const std::vector<double> share = run_workers(ctx_, *model_, ws);
+    rate = planned / std::max(1e-6, wall_s() - tp);
     ++passes_;
     for (size_t k = 0; k < ws.size(); ++k) {
       Solver& w = *ws[k];
