@@ -74,15 +74,19 @@ def run_c2(args):
         # results as arrays: best cost and best configuration of every restart
         return eng.solve_batch_arrays("paralleltempering.qiotoolkit", [model] * restarts, params)
 
-    for _ in range(args.warmup):
-        step()
-    t0 = time.perf_counter()
-    dev_ms, best = [], None
-    for _ in range(args.steps):
-        costs, configs, tm = step()
-        dev_ms.append(tm["device_ms"])
-        best = list(costs)
-    t = time.perf_counter() - t0
+    try:
+        for _ in range(args.warmup):
+            step()
+        t0 = time.perf_counter()
+        dev_ms, best = [], None
+        for _ in range(args.steps):
+            costs, configs, tm = step()
+            dev_ms.append(tm["device_ms"])
+            best = list(costs)
+        t = time.perf_counter() - t0
+    finally:
+        model.close()
+        eng.close()
     log("phases ms: init %.1f run %.1f finalize %.1f device %.1f" % (
         tm["init_ms"], tm["run_ms"], tm["finalize_ms"], tm["device_ms"]))
     value = attempts_per_step * args.steps / t
@@ -202,6 +206,9 @@ def run_c4(args):
             same &= bool(c1[0] == costs[k])
         plain[label] = plain_att / plain_t
     plain_best = max(plain.values())
+    for m in models:
+        m.close()
+    eng.close()
     out = {
         "metric": "spin-flip attempts/s at equal final-energy quality; p50/p99 solve latency",
         "value": value,

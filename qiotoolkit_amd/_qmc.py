@@ -9,6 +9,8 @@ import os
 
 import numpy as np
 
+from . import _handles
+
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libqmc.so")
 
@@ -65,6 +67,16 @@ PA_SCHEDULE, PA_FRICTION_TENSOR, PA_ENERGY_VARIANCE, PA_CONSTANT_CULLING = 0, 1,
 
 
 _lib = None
+
+
+def _unload():
+    """Unload libqmc.so at exit (after every handle is closed), so that its
+    code-object unregistration runs while the HIP runtime is intact."""
+    global _lib
+    if _lib is not None:
+        import _ctypes
+        h, _lib = _lib._handle, None
+        _ctypes.dlclose(h)
 
 
 def lib():
@@ -139,6 +151,7 @@ def lib():
     L.qmc_walkers_pa_end.argtypes = [vp]
     L.qmc_lds_bandwidth.argtypes = [vp, P(C.c_double)]
     _lib = L
+    _handles.on_unload(_unload)
     return L
 
 
@@ -155,13 +168,17 @@ class Context:
     def __init__(self, device=0):
         self.h = C.c_void_p()
         _check(lib().qmc_ctx_create(device, C.byref(self.h)))
+        _handles.track(self)
 
     def close(self):
         if self.h:
+            _handles.untrack(self)
             lib().qmc_ctx_destroy(self.h)
             self.h = C.c_void_p()
 
     def __del__(self):
+        if _handles.finalizing():
+            return
         try:
             self.close()
         except Exception:
@@ -199,16 +216,20 @@ class Problem:
         _check(lib().qmc_problem_create_visit(ctx.h, kind, n_vars, len(c), _p(off, C.c_int64),
                                               _p(ids, C.c_int32), _p(c, C.c_double), lanes, visit,
                                               C.byref(self.h)))
+        _handles.track(self)
         self.info = ProblemInfo()
         _check(lib().qmc_problem_get_info(self.h, C.byref(self.info)))
         self.n = self.info.n_vars
 
     def close(self):
         if self.h:
+            _handles.untrack(self)
             lib().qmc_problem_destroy(self.h)
             self.h = C.c_void_p()
 
     def __del__(self):
+        if _handles.finalizing():
+            return
         try:
             self.close()
         except Exception:
@@ -243,13 +264,17 @@ class Walkers:
         self.h = C.c_void_p()
         _check(lib().qmc_walkers_create(problem.h, n_replicas, replicas_per_group, seed & 0xFFFFFFFF,
                                         stream, first_replica, rps, C.byref(self.h)))
+        _handles.track(self)
 
     def close(self):
         if self.h:
+            _handles.untrack(self)
             lib().qmc_walkers_destroy(self.h)
             self.h = C.c_void_p()
 
     def __del__(self):
+        if _handles.finalizing():
+            return
         try:
             self.close()
         except Exception:
@@ -400,13 +425,17 @@ class Batch:
         self.h = C.c_void_p()
         self.members = []
         _check(lib().qmc_batch_create(ctx.h, C.byref(self.h)))
+        _handles.track(self)
 
     def close(self):
         if self.h:
+            _handles.untrack(self)
             lib().qmc_batch_destroy(self.h)
             self.h = C.c_void_p()
 
     def __del__(self):
+        if _handles.finalizing():
+            return
         try:
             self.close()
         except Exception:

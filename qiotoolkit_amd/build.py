@@ -48,7 +48,11 @@ def build_host():
         return None
     out = os.path.join(HERE, "libqiotk.so")
     srcs = [os.path.join(CSRC, s) for s in HOST_SOURCES]
-    _run(["g++", "-std=c++17", "-O2", "-fPIC", "-shared", "-Wall", "-pthread", "-I", os.path.join(ROOT, "include"),
+    # -fno-gnu-unique: unique symbols would make the library impossible to
+    # unload (the Python binding unloads it at exit, before the C runtime's
+    # exit handlers)
+    _run(["g++", "-std=c++17", "-O2", "-fPIC", "-shared", "-Wall", "-pthread", "-fno-gnu-unique",
+          "-I", os.path.join(ROOT, "include"),
           "-o", out] + srcs + [os.path.join(HERE, "libqmc.so"), "-Wl,-rpath,$ORIGIN"])
     return out
 

@@ -7,6 +7,7 @@
 #include <memory>
 #include <string>
 
+#include "../err_buf.h"
 #include "../../../include/qiotk.h"
 #include "model.h"
 #include "observer.h"
@@ -27,7 +28,7 @@ struct qiotk_model_s {
 
 namespace {
 
-thread_local std::string g_err;
+thread_local qmc::ErrBuf g_err;
 
 char* dup(const std::string& s) {
   char* p = (char*)std::malloc(s.size() + 1);

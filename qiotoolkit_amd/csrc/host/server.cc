@@ -22,6 +22,7 @@
 #include <thread>
 #include <vector>
 
+#include "../err_buf.h"
 #include "../../../include/qiotk.h"
 #include "model.h"
 #include "solver.h"
@@ -41,7 +42,7 @@ char* dup(const std::string& s) {
   return p;
 }
 
-thread_local std::string g_err;
+thread_local qmc::ErrBuf g_err;
 
 Json error_doc(int code, const std::string& msg) {
   Json j = Json::object();

@@ -15,13 +15,14 @@
 #include <string>
 #include <vector>
 
+#include "err_buf.h"
 #include "qmc_internal.h"
 
 using namespace qmc;
 
 namespace {
 
-thread_local std::string g_err;
+thread_local qmc::ErrBuf g_err;
 
 int fail(int code, const std::string& msg) {
   g_err = msg;
@@ -330,6 +331,13 @@ struct qmc_batch_s {
   DevBuf<unsigned long long> d_acc;
 };
 
+// Set by an exit handler registered when the first context is created, i.e.
+// after the HIP runtime registered its own teardown, so it runs first: a
+// context released after that point (a finaliser during process exit) is
+// abandoned instead of calling into a runtime that is shutting down.
+static std::atomic<bool> g_exiting{false};
+static void mark_exiting() { g_exiting.store(true); }
+
 extern "C" {
 
 const char* qmc_last_error(void) { return g_err.c_str(); }
@@ -340,6 +348,8 @@ int qmc_ctx_create(int device, qmc_ctx* out) {
   HIP_TRY(hipGetDeviceCount(&n));
   if (device < 0 || device >= n) return fail(QMC_INVALID_ARGUMENT, "no such device");
   HIP_TRY(hipSetDevice(device));
+  static std::once_flag exit_hook;
+  std::call_once(exit_hook, [] { std::atexit(mark_exiting); });
   auto* c = new qmc_ctx_s();
   c->device = device;
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
@@ -361,6 +371,7 @@ int qmc_ctx_create(int device, qmc_ctx* out) {
 
 static void ctx_release(qmc_ctx c) {
   if (!c || --c->refs > 0) return;
+  if (g_exiting.load()) return;  // the process is ending: nothing left to free
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   c->lut.release();

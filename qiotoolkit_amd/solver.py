@@ -10,7 +10,7 @@ import ctypes as C
 import json
 import os
 
-from . import _qmc
+from . import _handles, _qmc
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libqiotk.so")
@@ -39,6 +39,16 @@ class SolverError(RuntimeError):
 
 
 _lib = None
+
+
+def _unload():
+    """Unload libqiotk.so at exit (after every handle is closed), so that its
+    code-object unregistration runs while the HIP runtime is intact."""
+    global _lib
+    if _lib is not None:
+        import _ctypes
+        h, _lib = _lib._handle, None
+        _ctypes.dlclose(h)
 
 
 def lib():
@@ -73,6 +83,7 @@ def lib():
     L.qiotk_model_create_terms.argtypes = [vp, C.c_int, C.c_int64, P(C.c_int64), P(C.c_int32),
                                            P(C.c_double), P(vp)]
     _lib = L
+    _handles.on_unload(_unload)
     L.qiotk_solve_batch_arrays.argtypes = [vp, C.c_char_p, C.c_int32, P(vp), P(C.c_char_p), vp, vp, vp]
     L.qiotk_model_vars.argtypes = [vp, P(C.c_int32), vp]
     L.qiotk_tune_batch.argtypes = [vp, C.c_char_p, C.c_int32, P(vp), P(C.c_char_p), C.c_int32, P(vp)]
@@ -117,13 +128,17 @@ class Engine:
         rc = lib().qiotk_engine_create(device, C.byref(self.h))
         if rc:
             raise SolverError(rc, lib().qiotk_last_error().decode())
+        _handles.track(self)
 
     def close(self):
         if self.h:
+            _handles.untrack(self)
             lib().qiotk_engine_destroy(self.h)
             self.h = C.c_void_p()
 
     def __del__(self):
+        if _handles.finalizing():
+            return
         try:
             self.close()
         except Exception:
@@ -205,13 +220,17 @@ class Server:
         rc = lib().qiotk_server_create(device, json.dumps(config).encode(), C.byref(self.h))
         if rc:
             raise SolverError(rc, lib().qiotk_server_last_error().decode())
+        _handles.track(self)
 
     def close(self):
         if self.h:
+            _handles.untrack(self)
             lib().qiotk_server_destroy(self.h)
             self.h = C.c_void_p()
 
     def __del__(self):
+        if _handles.finalizing():
+            return
         try:
             self.close()
         except Exception:
@@ -273,13 +292,17 @@ class Model:
             rc = lib().qiotk_model_create(engine.h, _enc(input_data), C.byref(self.h))
         if rc:
             raise SolverError(rc, lib().qiotk_last_error().decode())
+        _handles.track(self)
 
     def close(self):
         if self.h:
+            _handles.untrack(self)
             lib().qiotk_model_destroy(self.h)
             self.h = C.c_void_p()
 
     def __del__(self):
+        if _handles.finalizing():
+            return
         try:
             self.close()
         except Exception:
