@@ -1,25 +1,37 @@
 #!/usr/bin/env python3
-"""Benchmark of the MI355X sweep engine on the parallel-tempering workload.
+"""Benchmark of the MI355X sweep engine.
 
-Default workload: parallel tempering on a 16^3 periodic cubic spin glass
-(N = 4096 spins, 12288 couplings J ~ N(0,1) fp32, h = 0), 32 replicas at
-geometric temperatures 0.2 -> 2.0, replica exchange after every sweep, 5000
-sweeps, 16 independent restarts (seeds s..s+15).  One step = the whole job:
-16 solves configured, fused into one device launch and read back (best cost
-and best configuration of each, as arrays), with the problem already
-resident in HBM.  Metric: spin-flip attempts/s (= restarts x replicas x
-sweeps x N / s of wall time).
+The default run (``python bench.py``) prints one JSON line.  Its top level is
+parallel tempering on a 16^3 periodic cubic spin glass (N = 4096 spins, 12288
+couplings J ~ N(0,1) fp32, h = 0), 32 replicas at geometric temperatures
+0.2 -> 2.0, replica exchange after every sweep, 5000 sweeps, 16 independent
+restarts (seeds s..s+15).  One step = the whole job: 16 solves configured,
+fused into one device launch and read back (best cost and best
+configuration of each, as arrays), with the problem already resident in
+HBM.  Metric: spin-flip attempts/s (= restarts x replicas x sweeps x N / s of
+wall time).  Nested under their own keys, in the same run:
 
-The roofline object prices the dominant kernel (the coloured sweep) with the
-algorithmic bytes per attempt B = d*(4 + s_c + 1) + 2 (d = 6 incident terms,
-s_c = 4 coupling bytes) = 56 B against the 8 TB/s HBM3E peak, measured live
-from HIP events on the stream the kernel runs on.  The CPU baseline times the
-reference algorithm restated in C++ (pcg32 streams, ln-table accept, the same
-PT loop) on a bounded sample of the same workload on this host.
+  c4  the uneven batch: 4096 ER instances, 8 restarts x 1000 SA sweeps each,
+      autotuned shared launches against the plain one-instance-per-launch form
+  c5  the serving stream: Poisson arrivals of mixed Ising / PUBO requests,
+      population annealing or SA, p50 / p99 latency and the sustained rate
+  c1  SA on the 32x32 +-1 lattice, 64 restarts x 1000 sweeps
+  c3  SA on the 2000-variable random PUBO, 256 restarts x 2000 sweeps
+
+Each carries a roofline object for its dominant kernel: the algorithmic bytes
+per attempt B = d*(4 + s_c + 1) + 2 (d incident terms, s_c coupling bytes)
+over the kernel's average launch time (HIP events on the stream the kernel
+runs on), against the 8 TB/s HBM3E peak; ``bound`` names what the counters
+show limits the kernel, ``traffic`` the HBM bytes per launch the counters
+measured (profiles/).  The CPU baseline times the reference algorithm
+restated in C++ (pcg32 streams, ln-table accept, the same PT loop), built on
+this host with the reference's Release flags, on a bounded sample of the
+same workload.
 """
 import argparse
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -34,16 +46,27 @@ HBM_PEAK_GBS = 8000.0
 # kernel (--lds-probe) measures 106-109 TB/s.
 LDS_PEAK_GBS = 256 * 256 * 2.4
 
-# HBM bytes per sweep-kernel launch of the default workload (5000 sweeps,
-# automatic grouping and lanes), from rocprofv3 --pmc FETCH_SIZE and
-# WRITE_SIZE in separate runs, scaled by what the same counters report for a
-# plain 512 MiB copy (FETCH_SIZE x 2, WRITE_SIZE x 1):
-# 205462.1 KiB x 2 + 82106 KiB.  Summary: profiles/k_sweep_pt_16cube_hbm_traffic.md
+# HBM bytes per sweep-kernel launch of the default shapes, from rocprofv3
+# --pmc FETCH_SIZE and WRITE_SIZE in separate runs, scaled by what the same
+# counters report for a plain 512 MiB copy (FETCH_SIZE x 2, WRITE_SIZE x 1).
+# Summaries: profiles/k_sweep_*_hbm_traffic.md
 MEASURED_TRAFFIC_BYTES = {"c2": int((198183.25 * 2 + 82106) * 1024)}
+
+# What limits each kernel, from its SQ counters (profiles/k_sweep_*_sq_counters.md):
+# VALU issue plus stalls on the per-CU stream of graph rows from L2 and on
+# barriers; the HBM figure is the algorithmic bytes priced against the HBM peak.
+LIMITER = "valu-issue and L2->CU row stream (graph and state on chip); frac = algorithmic bytes vs HBM peak"
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
+
+
+def roofline(attempts, B, kernel_ms, kernel, traffic=None):
+    achieved = attempts * B / (kernel_ms * 1e-3) / 1e9
+    return {"bound": LIMITER, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "bytes_per_attempt": B,
+            "kernel": kernel, "kernel_ms": kernel_ms, "lds_frac": achieved / LDS_PEAK_GBS}
 
 
 def c2_workload(seed):
@@ -92,7 +115,6 @@ def run_c2(args):
     value = attempts_per_step * args.steps / t
     avg_ms = float(np.mean(dev_ms))
     B = 6 * (4 + 4 + 1) + 2
-    achieved = attempts_per_step * B / (avg_ms * 1e-3) / 1e9
     default_shape = args.sweeps == 5000 and not args.group and not args.lanes
     traffic = MEASURED_TRAFFIC_BYTES["c2"] if default_shape else None
     out = {
@@ -112,10 +134,7 @@ def run_c2(args):
                                "exchange every sweep, %d sweeps, 16 restarts" % sweeps,
                    "n_spins": N, "couplings": 12288, "replicas": L, "restarts": restarts,
                    "sweeps": sweeps, "parallelism": "none (1 GPU)"},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "bytes_per_attempt": B, "kernel": "k_sweep<ISING,pairwise,f32>",
-                     "kernel_ms": avg_ms, "lds_frac": achieved / LDS_PEAK_GBS},
+        "roofline": roofline(attempts_per_step, B, avg_ms, "k_sweep<ISING,pairwise,f32>", traffic),
         "quality": {"best_energy_mean": float(np.mean(best)), "best_energy_min": float(np.min(best)),
                     "per_spin_mean": float(np.mean(best)) / N},
     }
@@ -124,13 +143,54 @@ def run_c2(args):
     return out
 
 
-def c4_instances(n=4096, seed=4096):
-    """4096 Erdos-Renyi instances, N log-uniform in [64, 4096], mean degree 6,
-    J uniform in {-3..3}\\{0}, one field term h in {-1,0,1} per spin."""
+SA_WORKLOADS = {
+    # name: (kind, terms, params, incident terms per variable, coupling bytes, dtype, description)
+    "c1": ("ising", lambda wl: wl.lattice2d(32, seed=1),
+           {"beta_start": 0.1, "beta_stop": 3.0, "restarts": 64, "sweeps": 1000}, 4, 4, "int32",
+           "SA 2D square lattice 32x32 periodic, J in {-1,+1}, h = 0, 64 restarts x 1000 sweeps, beta 0.1->3.0"),
+    "c3": ("pubo", lambda wl: wl.random_pubo(2000, 20000, seed=1),
+           {"beta_start": 0.01, "beta_stop": 5.0, "restarts": 256, "sweeps": 2000}, 26, 4, "int32",
+           "SA random PUBO 2000 vars, 20000 terms of degree 1/2/3/4 at 10/40/30/20%, c in [-10,10]\\{0}, "
+           "256 restarts x 2000 sweeps, beta 0.01->5.0"),
+}
+
+
+def run_sa(args, name, steps, warmup):
+    """One simulated-annealing solve per step on a problem resident in HBM."""
     from qiotoolkit_amd import workloads as wl
-    sizes = wl.log_uniform_sizes(n, 64, 4096, seed=seed)
-    return [(int(N),) + wl.er_ising_arrays(int(N), 6.0, 3, seed=seed * 16 + k)
-            for k, N in enumerate(sizes)]
+    from qiotoolkit_amd.solver import Engine
+    kind, terms, params, d, sc, dtype, desc = SA_WORKLOADS[name]
+    eng = Engine(0)
+    model = eng.model({"cost_function": {"type": kind, "version": "1.0", "terms": terms(wl)}})
+    n = model.n_vars()
+    target = "simulatedannealing.qiotoolkit"
+    plist = [{"params": dict(params, seed=300)}]
+    try:
+        for _ in range(warmup):
+            eng.solve_batch_arrays(target, [model], plist, configs=False)
+        dev_ms, best = [], []
+        t0 = time.perf_counter()
+        for s in range(steps):
+            cost, _, tm = eng.solve_batch_arrays(target, [model], [{"params": dict(params, seed=300 + s)}],
+                                                 configs=True)
+            dev_ms.append(tm["device_ms"])
+            best.append(float(cost[0]))
+        t = time.perf_counter() - t0
+    finally:
+        model.close()
+        eng.close()
+    attempts = params["restarts"] * params["sweeps"] * n
+    B = d * (4 + sc + 1) + 2
+    return {"value": attempts * steps / t, "unit": "spin-flip attempts/s", "steps": steps, "warmup": warmup,
+            "ms_per_step": t * 1e3 / steps, "dtype": dtype,
+            "config": {"workload": desc, "n_vars": n, "attempts_per_step": attempts},
+            "roofline": roofline(attempts, B, float(np.mean(dev_ms)), "k_sweep (device time of the solve)"),
+            "quality": {"best_cost_mean": float(np.mean(best)), "best_cost_min": float(np.min(best))}}
+
+
+def c4_instances(n=4096, seed=4096):
+    from qiotoolkit_amd import workloads as wl
+    return wl.batched_er_instances(n, seed)
 
 
 def run_c4(args):
@@ -186,7 +246,7 @@ def run_c4(args):
     value = attempts * args.steps / t
     avg_ms = float(np.mean(dev_ms))
     B = 7 * (4 + 1 + 1) + 2   # d = 6 couplings + 1 field term, int8 coefficients
-    achieved = attempts * B / (avg_ms * 1e-3) / 1e9
+    default_shape = sweeps == 1000 and len(models) == 4096 and tune is not None
     # plain form: the same solves, one instance per launch, on a sample --
     # with the batch's parameters, and with one replica per work-group (the
     # plain form's best shape; the grouping does not change results)
@@ -227,10 +287,8 @@ def run_c4(args):
                                % (len(models), sweeps),
                    "instances": len(models), "spins": sum(i[0] for i in insts), "restarts": 8,
                    "sweeps": sweeps, "params": extra, "parallelism": "none (1 GPU)"},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "bytes_per_attempt": B,
-                     "kernel": "k_sweep<ISING,pairwise,i32>", "kernel_ms": avg_ms,
-                     "lds_frac": achieved / LDS_PEAK_GBS},
+        "roofline": roofline(attempts, B, avg_ms, "k_sweep<ISING,pairwise,i32> (all launches of a step)",
+                             MEASURED_TRAFFIC_BYTES.get("c4") if default_shape else None),
         "plain_per_instance": {"value": plain_best, "by_shape": plain, "instances_sampled": len(idx),
                                "speedup_of_batched": value / plain_best,
                                "identical_best_costs": same},
@@ -241,41 +299,8 @@ def run_c4(args):
 
 
 def c5_requests(rate, duration, seed=5005, pa_extra=None):
-    """Poisson arrivals at `rate` per second over `duration` seconds.  70 %
-    Ising ER (N log-uniform in [100, 50000], mean degree 8, J ~ N(0,1) fp32),
-    30 % random PUBO (V log-uniform in [500, 20000] vars, 10 V terms of degree
-    1/2/3/4 at 10/40/30/20 %, c in [-10, 10] \\ {0}); each request is
-    population annealing (population 512, beta 0 -> 3 over 200 steps of 10
-    sweeps) or simulated annealing (64 restarts x 1000 sweeps, beta 0.1 ->
-    3.0) by a fair coin."""
     from qiotoolkit_amd import workloads as wl
-    rng = np.random.default_rng(seed)
-    t, out = 0.0, []
-    while True:
-        t += rng.exponential(1.0 / rate)
-        if t > duration:
-            break
-        k = len(out)
-        ising = rng.random() < 0.7
-        pa = rng.random() < 0.5
-        if ising:
-            n = int(round(np.exp(rng.uniform(np.log(100), np.log(50000)))))
-            arrays = wl.er_ising_f32_arrays(n, 8.0, seed=seed * 1000 + k)
-        else:
-            n = int(round(np.exp(rng.uniform(np.log(500), np.log(20000)))))
-            arrays = wl.random_pubo_arrays(n, 10 * n, seed=seed * 1000 + k)
-        if pa:
-            target = "populationannealing.cpu"
-            params = {"population": 512, "beta_start": 0.0, "beta_stop": 3.0, "sweeps": 200,
-                      "sweeps_per_replica": 10, "seed": 9000 + k, **(pa_extra or {})}
-            attempts = 512 * 200 * 10 * n  # nominal: the population fluctuates around 512
-        else:
-            target = "simulatedannealing.qiotoolkit"
-            params = {"beta_start": 0.1, "beta_stop": 3.0, "restarts": 64, "sweeps": 1000, "seed": 9000 + k}
-            attempts = 64 * 1000 * n
-        out.append({"t": t, "kind": "ising" if ising else "pubo", "n": n, "target": target,
-                    "params": params, "arrays": arrays, "nominal_attempts": attempts})
-    return out
+    return wl.request_stream(rate, duration, seed, pa_extra)
 
 
 def run_c5(args):
@@ -343,10 +368,36 @@ def run_c5(args):
                        "p99": float(np.percentile(lat, 99)) if len(lat) else None,
                        "max": float(lat.max()) if len(lat) else None},
         "requests_per_s": len(lat) / t_end,
+        # completed requests per second between the first arrival and the
+        # last completion; equal to the offered rate when the loop keeps up
+        "sustained_rate_per_s": len(lat) / max(1e-9, t_end - (reqs[0]["t"] if reqs else 0.0)),
         "failed": fails,
         "server_stats": st,
     }
     return out
+
+
+def _host_oracle():
+    """Build the oracle for this host (-march=native, the reference's Release
+    flags) and return its variant; the portable -march=x86-64-v3 build if the
+    host build fails."""
+    try:
+        subprocess.run(["make", "-s", "-B", "-C", os.path.join(ROOT, "oracle"), "liboracle_native.so"],
+                       check=True, timeout=180, capture_output=True)
+        return "_native", "-O3 -Ofast -march=native -mtune=native -funroll-loops -fopenmp"
+    except Exception as e:  # noqa: BLE001
+        log("host-tuned oracle build failed (%s); using the portable build" % e)
+        return "_fast", "-O3 -Ofast -march=x86-64-v3 -mtune=generic -funroll-loops -fopenmp"
+
+
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def cpu_baseline_c2(inp, temps, args):
@@ -357,7 +408,8 @@ def cpu_baseline_c2(inp, temps, args):
     except AttributeError:
         cores = os.cpu_count() or 1
     threads = max(1, min(32, cores))
-    p = po.Problem("ising", inp["cost_function"]["terms"], variant="_fast")
+    variant, flags = _host_oracle()
+    p = po.Problem("ising", inp["cost_function"]["terms"], variant=variant)
     tvals = [temps["initial"] * (temps["final"] / temps["initial"]) ** (i / 31) for i in range(32)]
     # calibrate: a short run, then a sample of ~args.cpu_seconds
     o = po.make_opts(po.RNG_REFERENCE, seed=7, threads=threads, steps_per_tick=1)
@@ -375,12 +427,16 @@ def cpu_baseline_c2(inp, temps, args):
     t1 = time.perf_counter()
     p.pt(o1, tvals, False, sweeps1)
     dt1 = time.perf_counter() - t1
+    single = 32 * 4096 * sweeps1 / dt1
     return {"value": attempts / dt, "unit": "spin-flip attempts/s", "cores": threads,
             "kind": "port",
-            "single_thread": 32 * 4096 * sweeps1 / dt1,
+            "single_thread": single,
+            "thread_scaling": (attempts / dt) / single,
+            "cpu_model": _cpu_model(),
+            "build": "oracle/liboracle%s.so built on this host: %s" % (variant, flags),
             "sample": "one 32-replica ladder of the same workload, %d of 5000 sweeps, %.1f s, "
-                      "pcg32 + ln-table accept, OpenMP over replicas; single_thread: %d sweeps on 1 "
-                      "thread, %.1f s" % (sweeps, dt, sweeps1, dt1)}
+                      "pcg32 + ln-table accept, OpenMP over replicas on %d threads; single_thread: %d sweeps "
+                      "on 1 thread, %.1f s" % (sweeps, dt, threads, sweeps1, dt1)}
 
 
 def main():
@@ -388,7 +444,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="c2")
+    ap.add_argument("--workload", default="all",
+                    help="all (c2 at the top level, c4 c5 c1 c3 nested), or one of c2 c4 c5 c1 c3")
     ap.add_argument("--sweeps", type=int, default=5000)
     ap.add_argument("--group", type=int, default=0, help="replicas per work-group (0 = auto)")
     ap.add_argument("--lanes", type=int, default=0, help="positions per class round (0 = auto)")
@@ -396,7 +453,7 @@ def main():
                     help="lanes cover the largest colour class in this many rounds (0 = auto)")
     ap.add_argument("--instances", type=int, default=4096, help="c4: number of instances")
     ap.add_argument("--no-tune", action="store_true", help="c4: skip the autotuner")
-    ap.add_argument("--rate", type=float, default=1.0, help="c5: offered requests per second")
+    ap.add_argument("--rate", type=float, default=4.0, help="c5: offered requests per second")
     ap.add_argument("--duration", type=float, default=30.0, help="c5: seconds of arrivals")
     ap.add_argument("--max-active", type=int, default=64, help="c5: concurrently admitted requests")
     ap.add_argument("--sweeps-per-round", type=int, default=10, help="c5: sweeps per shared launch")
@@ -435,11 +492,31 @@ def main():
               "multi-GPU runs are not part of this build (--gpus %d refused)" % args.gpus,
               file=sys.stderr)
         sys.exit(2)
-    runs = {"c2": run_c2, "c4": run_c4, "c5": run_c5}
-    if args.workload not in runs:
+    runs = {"c2": run_c2, "c4": run_c4, "c5": run_c5,
+            "c1": lambda a: run_sa(a, "c1", a.steps, a.warmup), "c3": lambda a: run_sa(a, "c3", a.steps, a.warmup)}
+    if args.workload == "all":
+        out = run_c2(args)
+        nested = [
+            ("c4", lambda: run_c4(argparse.Namespace(**dict(vars(args), steps=min(args.steps, 3),
+                                                             warmup=min(args.warmup, 1), sweeps=1000)))),
+            ("c5", lambda: run_c5(args)),
+            ("c1", lambda: run_sa(args, "c1", args.steps, args.warmup)),
+            ("c3", lambda: run_sa(args, "c3", min(args.steps, 5), min(args.warmup, 1))),
+        ]
+        drop = ("metric", "n_gpus", "higher_is_better", "scaling", "vs_baseline")
+        for name, fn in nested:
+            t0 = time.perf_counter()
+            try:
+                res = fn()
+                out[name] = {k: v for k, v in res.items() if k not in drop}
+            except Exception as e:  # noqa: BLE001 -- the line still reports the other workloads
+                out[name] = {"error": "%s: %s" % (type(e).__name__, e)}
+            log("%s done in %.1f s" % (name, time.perf_counter() - t0))
+    elif args.workload in runs:
+        out = runs[args.workload](args)
+    else:
         print("unknown workload %s" % args.workload, file=sys.stderr)
         sys.exit(2)
-    out = runs[args.workload](args)
     print(json.dumps(out), flush=True)
 
 

@@ -39,7 +39,9 @@ namespace {
 // O'Neill, "PCG: A Family of Simple Fast Space-Efficient Statistically Good
 // Algorithms for Random Number Generation" (2014): 64-bit LCG, output
 // XSH-RR of the *previous* state; seed(s) sets state = bump(s + inc).
-struct Pcg32 {
+// One cache line each: the replicas' generators and walkers sit side by side
+// in vectors and are advanced by different OpenMP threads.
+struct alignas(64) Pcg32 {
   using result_type = uint32_t;
   static constexpr uint64_t kMult = 6364136223846793005ULL;
   static constexpr uint64_t kInc = 1442695040888963407ULL;
@@ -303,7 +305,7 @@ inline double delta(const Problem& p, const State& s, int64_t i, int arith) {
 
 // ---------------------------------------------------------------------------
 // Walker / Metropolis (markov/walker.h:35-315, metropolis.h:18-116).
-struct Walker {
+struct alignas(64) Walker {
   State state;
   double cost_ = 0;
   double lowest_cost = 0;

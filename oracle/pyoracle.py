@@ -141,9 +141,15 @@ class Problem:
     """Normalised problem in the oracle (graph::normalize_edges semantics)."""
 
     def __init__(self, kind, terms, variant=""):
+        """terms: [{'c': c, 'ids': [...]}, ...] or a term CSR (offsets, ids, coeffs)."""
         self.L = lib(variant)
         self.kind = ISING if kind in ("ising", ISING) else PUBO
-        c, off, ids = flatten_terms(terms)
+        if isinstance(terms, tuple):
+            off, ids, c = (np.ascontiguousarray(terms[0], dtype=np.int64),
+                           np.ascontiguousarray(terms[1], dtype=np.int32),
+                           np.ascontiguousarray(terms[2], dtype=np.float64))
+        else:
+            c, off, ids = flatten_terms(terms)
         err = C.create_string_buffer(256)
         h = self.L.orc_problem_new(self.kind, len(c), _p(c, C.c_double), _p(off, C.c_int64),
                                    _p(ids, C.c_int32) if len(ids) else None, err, 256)

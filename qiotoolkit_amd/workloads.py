@@ -201,3 +201,52 @@ def random_pubo_arrays(n_vars, n_terms, fractions=(0.1, 0.4, 0.3, 0.2), cmax=10,
     offsets = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
     coeffs = rng.choice(vals, size=n_terms).astype(np.float64)
     return offsets, ids, coeffs
+
+
+def batched_er_instances(n=4096, seed=4096):
+    """The uneven batch: `n` Erdos-Renyi instances with N log-uniform in
+    [64, 4096], mean degree 6, J uniform in {-3..3} \\ {0} and one field term
+    h in {-1, 0, 1} per spin.  Returns [(N, offsets, ids, coeffs), ...]."""
+    sizes = log_uniform_sizes(n, 64, 4096, seed=seed)
+    return [(int(N),) + er_ising_arrays(int(N), 6.0, 3, seed=seed * 16 + k) for k, N in enumerate(sizes)]
+
+
+def request_stream(rate, duration, seed=5005, pa_extra=None, pa_share=0.5):
+    """A serving stream: Poisson arrivals at `rate` per second over `duration`
+    seconds.  70 % Ising ER (N log-uniform in [100, 50000], mean degree 8,
+    J ~ N(0,1) fp32), 30 % random PUBO (V log-uniform in [500, 20000] vars,
+    10 V terms of degree 1/2/3/4 at 10/40/30/20 %, c in [-10, 10] \\ {0}); each
+    request is population annealing (population 512, beta 0 -> 3 over 200
+    steps of 10 sweeps) with probability `pa_share`, else simulated annealing
+    (64 restarts x 1000 sweeps, beta 0.1 -> 3.0).  Returns a list of
+    {"t", "kind", "n", "target", "params", "arrays", "nominal_attempts"}."""
+    rng = np.random.default_rng(seed)
+    t, out = 0.0, []
+    while True:
+        t += rng.exponential(1.0 / rate)
+        if t > duration:
+            break
+        k = len(out)
+        ising = rng.random() < 0.7
+        pa = rng.random() < pa_share
+        if ising:
+            n = int(round(np.exp(rng.uniform(np.log(100), np.log(50000)))))
+            arrays = er_ising_f32_arrays(n, 8.0, seed=seed * 1000 + k)
+        else:
+            n = int(round(np.exp(rng.uniform(np.log(500), np.log(20000)))))
+            arrays = random_pubo_arrays(n, 10 * n, seed=seed * 1000 + k)
+        out.append(dict(t=t, kind="ising" if ising else "pubo", n=n, arrays=arrays,
+                        **stream_request(pa, n, 9000 + k, pa_extra)))
+    return out
+
+
+def stream_request(pa, n, seed, pa_extra=None):
+    """Target, params and nominal spin-flip attempts of one request of the
+    serving stream on `n` variables."""
+    if pa:
+        params = {"population": 512, "beta_start": 0.0, "beta_stop": 3.0, "sweeps": 200,
+                  "sweeps_per_replica": 10, "seed": seed, **(pa_extra or {})}
+        # nominal: the population fluctuates around 512
+        return dict(target="populationannealing.cpu", params=params, nominal_attempts=512 * 200 * 10 * n)
+    params = {"beta_start": 0.1, "beta_stop": 3.0, "restarts": 64, "sweeps": 1000, "seed": seed}
+    return dict(target="simulatedannealing.qiotoolkit", params=params, nominal_attempts=64 * 1000 * n)

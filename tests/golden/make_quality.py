@@ -13,6 +13,9 @@ single-problem workloads, one best energy per seed for seeds 100..199:
   pa_lattice2d_12_* population annealing on the 12x12 periodic +-1 lattice
                     (seed 41) with each stepping size and resampling method
                     (params below); the global lowest of the run
+  sa_er_batch_*     SA on three members of the uneven 4096-instance ER batch
+                    (smallest, median and largest N), 8 restarts x 1000
+                    sweeps, linear beta 0.1 -> 3.0; best over the restarts
 
 The oracle is built with the reference's Release flags (liboracle_fast.so).
 The GPU quality test compares the device solver's distribution over the same
@@ -49,7 +52,26 @@ def cases():
             kind="pubo", terms=lambda: wl.random_pubo(2000, 20000, seed=1), solver="sa",
             params={"beta_start": 0.01, "beta_stop": 5.0, "restarts": 256, "sweeps": 2000}),
         **pa_cases(),
+        **batch_cases(),
     }
+
+
+# Members of the uneven 4096-instance ER batch (workloads.batched_er_instances):
+# the smallest (N = 64), the median (N = 518) and the largest (N = 4086).
+BATCH_MEMBERS = {"min": 393, "median": 1918, "max": 4075}
+
+
+def batch_member(k, seed=4096):
+    """Term CSR of member k of the uneven batch, generated alone."""
+    N = int(wl.log_uniform_sizes(4096, 64, 4096, seed=seed)[k])
+    return wl.er_ising_arrays(N, 6.0, 3, seed=seed * 16 + k)
+
+
+def batch_cases():
+    return {"sa_er_batch_%s" % name: dict(
+        kind="ising", terms=(lambda k=k: batch_member(k)), solver="sa", member=k,
+        params={"beta_start": 0.1, "beta_stop": 3.0, "restarts": 8, "sweeps": 1000})
+        for name, k in BATCH_MEMBERS.items()}
 
 
 def pa_cases():

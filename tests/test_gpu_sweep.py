@@ -237,6 +237,43 @@ def test_pt_matches_oracle(ctx, name, B, L, mode):
         assert np.allclose(got["beta"][sl], want["slot_beta"])
 
 
+@pytest.mark.parametrize("side,n_ladders,steps", [(6, 3, 40), (16, 2, 24)])
+def test_pt_f32_split_ladder_matches_oracle(ctx, side, n_ladders, steps):
+    """The benchmark's kernel instantiation -- fp32 couplings, a 32-replica
+    ladder split over 16 cooperative work-groups of 2 replicas, exchange
+    inside the kernel through the cross-work-group energy hand-off -- step
+    for step against the oracle's counter-mode parallel tempering
+    (parallel_tempering.h:134-189) with the same fp32 field sums: identical
+    states, lowest states and slot betas; costs and lowest costs within 1e-5
+    relative (the device sums a class's accepted dE per wavefront).  The
+    16^3 case is the benchmark problem at its default lane count."""
+    terms = wl.lattice3d(side, seed=7)
+    op, qp = pair(ctx, "ising", terms)
+    assert qp.info.coupling == _qmc.QMC_COUPLING_F32
+    L = 32
+    temps = wl.geometric_temperatures(0.2, 2.0, L)
+    betas = sorted([1.0 / t for t in temps], reverse=True)
+    w = _qmc.Walkers(qp, L * n_ladders, 2, seed=31, replicas_per_seed=L)
+    order, _ = qp.order(w.lanes())
+    w.init()
+    w.pt(betas, 0, steps)
+    got = w.read()
+    for ld in range(n_ladders):
+        opts = po.make_opts(po.RNG_COUNTER, seed=31 + ld, arith=po.ARITH_F32, order=order, steps_per_tick=1)
+        want = op.pt(opts, temps, False, steps)
+        sl = slice(ld * L, (ld + 1) * L)
+        assert np.array_equal(got["states"][sl], want["final_states"]), ld
+        assert np.array_equal(got["lowest_states"][sl], want["lowest_states"]), ld
+        assert np.allclose(got["beta"][sl], want["slot_beta"]), ld
+        np.testing.assert_allclose(got["cost"][sl], want["final_costs"], rtol=1e-5)
+        np.testing.assert_allclose(got["lowest"][sl], want["lowest"], rtol=1e-5)
+        # every lowest state's fp64 energy is the lowest cost within 1e-5
+        e = np.array([op.cost(s) for s in got["lowest_states"][sl]])
+        np.testing.assert_allclose(got["lowest"][sl], e, rtol=1e-5)
+    w.close()
+    qp.close()
+
+
 @pytest.mark.parametrize("B", [1, 4, 8])
 def test_pt_observations_match_oracle(ctx, B):
     """Observed parallel tempering (qmc_walkers_pt_observe: graph-replayed
