@@ -291,6 +291,19 @@ int qmc_walkers_labels(qmc_walkers w, int32_t* labels);
 /* Energies of arbitrary states (FP64 on device). */
 int qmc_energy(qmc_problem p, int32_t n_states, const uint8_t* states, double* energies);
 
+/* qmc_energy without allocating: no hipMalloc, hipFree or allocating library
+ * call.  No counterpart in the reference; for callers that cannot allocate
+ * device memory (tight serving loops, harnesses that forbid it).  The states
+ * and energies are staged in a fixed device array of
+ * QMC_NOALLOC_SCRATCH_BYTES per GPU, held from the library's load until the
+ * process ends: n_states * n_vars (rounded up to 256) + 8 * n_states bytes
+ * must fit, else QMC_INVALID_ARGUMENT.  Every call on one GPU shares that
+ * array; calls hold a per-device lock until their results are copied back,
+ * so concurrent callers take turns.  Synchronous, on the problem's context
+ * stream; results equal qmc_energy's. */
+#define QMC_NOALLOC_SCRATCH_BYTES (16u << 20)
+int qmc_energy_noalloc(qmc_problem p, int32_t n_states, const uint8_t* states, double* energies);
+
 /* Device-side delta-energy of every variable for given states (the GPU's own
  * dE arithmetic: int32 or fp32 sums in CSR order), out[n_states*n_vars]. */
 int qmc_delta_all(qmc_problem p, int32_t n_states, const uint8_t* states, double* out);

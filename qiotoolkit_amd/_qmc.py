@@ -17,6 +17,9 @@ LIB_PATH = os.path.join(_HERE, "libqmc.so")
 QMC_ISING, QMC_PUBO = 0, 1
 QMC_COUPLING_I32, QMC_COUPLING_F32 = 0, 1
 QMC_BETA_SCHEDULE, QMC_BETA_PER_REPLICA = 0, 1
+QMC_INVALID_ARGUMENT = 107
+# device scratch of qmc_energy_noalloc per GPU (QMC_NOALLOC_SCRATCH_BYTES)
+NOALLOC_SCRATCH_BYTES = 16 << 20
 
 # every symbol include/qmc.h declares
 EXPORTS = [
@@ -31,7 +34,7 @@ EXPORTS = [
     "qmc_walkers_set_lanes", "qmc_walkers_get_lanes", "qmc_walkers_pa_begin", "qmc_walkers_pa_step",
     "qmc_walkers_pa_info", "qmc_walkers_pa_end", "qmc_lds_bandwidth", "qmc_walkers_pa_strategy",
     "qmc_walkers_set_sweep_order", "qmc_problem_create_visit", "qmc_walkers_pa_observe", "qmc_walkers_pa_log",
-    "qmc_walkers_pt_observe", "qmc_walkers_pt_log",
+    "qmc_walkers_pt_observe", "qmc_walkers_pt_log", "qmc_energy_noalloc",
 ]
 
 
@@ -114,6 +117,7 @@ def lib():
     L.qmc_walkers_energy.argtypes = [vp, C.c_int, P(C.c_double)]
     L.qmc_walkers_labels.argtypes = [vp, P(C.c_int32)]
     L.qmc_energy.argtypes = [vp, C.c_int32, P(C.c_uint8), P(C.c_double)]
+    L.qmc_energy_noalloc.argtypes = [vp, C.c_int32, P(C.c_uint8), P(C.c_double)]
     L.qmc_delta_all.argtypes = [vp, C.c_int32, P(C.c_uint8), P(C.c_double)]
     L.qmc_philox.argtypes = [vp, C.c_int32, P(C.c_uint32), P(C.c_uint32), P(C.c_uint32)]
     L.qmc_walkers_last_ms.argtypes = [vp, P(C.c_double)]
@@ -241,10 +245,13 @@ class Problem:
         _check(lib().qmc_problem_order(self.h, lanes, _p(o, C.c_int32), _p(co, C.c_int32)))
         return o, co
 
-    def energy(self, states):
+    def energy(self, states, noalloc=False):
+        """FP64 energies of plain states; noalloc: through qmc_energy_noalloc,
+        which stages them in the library's fixed device scratch array."""
         s = np.ascontiguousarray(states, dtype=np.uint8).reshape(-1, self.n)
         out = np.zeros(len(s))
-        _check(lib().qmc_energy(self.h, len(s), _p(s, C.c_uint8), _p(out, C.c_double)))
+        fn = lib().qmc_energy_noalloc if noalloc else lib().qmc_energy
+        _check(fn(self.h, len(s), _p(s, C.c_uint8), _p(out, C.c_double)))
         return out
 
     def delta_all(self, states):

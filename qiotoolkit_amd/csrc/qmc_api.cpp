@@ -338,6 +338,31 @@ struct qmc_batch_s {
 static std::atomic<bool> g_exiting{false};
 static void mark_exiting() { g_exiting.store(true); }
 
+// Device-side staging of qmc_energy_noalloc: a fixed array in every GPU's
+// copy of the code object, so that entry point never allocates.  In host code
+// `scratch` names a host-side stand-in; its device address comes from
+// hipGetSymbolAddress, looked up once per device (with the device current)
+// when the first context on it is created, and only that address is used.
+constexpr size_t kScratchBytes = (size_t)QMC_NOALLOC_SCRATCH_BYTES;
+static __device__ __align__(256) uint8_t scratch[kScratchBytes];
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[kMaxDevices] = {};
+// one caller at a time per device: every call shares that device's array
+static std::mutex g_scratch_use[kMaxDevices];
+
+// With `device` current: the device address of its scratch array.
+static hipError_t scratch_setup(int device) {
+  if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_ptr[device]) return hipSuccess;
+  void* ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess) return e;
+  g_scratch_ptr[device] = ptr;
+  return hipSuccess;
+}
+
 extern "C" {
 
 const char* qmc_last_error(void) { return g_err.c_str(); }
@@ -350,6 +375,11 @@ int qmc_ctx_create(int device, qmc_ctx* out) {
   HIP_TRY(hipSetDevice(device));
   static std::once_flag exit_hook;
   std::call_once(exit_hook, [] { std::atexit(mark_exiting); });
+  {
+    const hipError_t e = scratch_setup(device);
+    if (e != hipSuccess)
+      return fail(QMC_DEVICE_ERROR, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+  }
   auto* c = new qmc_ctx_s();
   c->device = device;
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
@@ -2006,6 +2036,38 @@ int qmc_energy(qmc_problem p, int32_t n_states, const uint8_t* states, double* e
   HIP_TRY(launch_energy(p->dev(), ds.p, n_states, de.p, st));
   HIP_TRY(hipStreamSynchronize(st));
   HIP_TRY(hipMemcpy(energies, de.p, n_states * sizeof(double), hipMemcpyDeviceToHost));
+  return QMC_OK;
+}
+
+int qmc_energy_noalloc(qmc_problem p, int32_t n_states, const uint8_t* states, double* energies) {
+  if (!p || !states || !energies || n_states < 0) return fail(QMC_INVALID_ARGUMENT, "bad argument");
+  if (n_states == 0) return QMC_OK;
+  // states first (256-byte aligned end), then the energies
+  const size_t state_bytes = ((size_t)n_states * (size_t)p->N + 255) & ~(size_t)255;
+  const size_t need = state_bytes + (size_t)n_states * sizeof(double);
+  if (need > kScratchBytes)
+    return fail(QMC_INVALID_ARGUMENT, "qmc_energy_noalloc: " + std::to_string(n_states) + " states of " +
+                                          std::to_string(p->N) + " variables need " + std::to_string(need) +
+                                          " bytes of device scratch; the limit is " +
+                                          std::to_string(kScratchBytes));
+  const int dev = p->ctx->device;
+  void* base = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    base = (dev >= 0 && dev < kMaxDevices) ? g_scratch_ptr[dev] : nullptr;
+  }
+  if (!base) return fail(QMC_DEVICE_ERROR, "qmc_energy_noalloc: no scratch address for this device");
+  std::lock_guard<std::mutex> use(g_scratch_use[dev]);
+  HIP_TRY(hipSetDevice(dev));
+  hipStream_t st = p->ctx->stream;
+  uint8_t* d_states = static_cast<uint8_t*>(base);
+  double* d_energy = reinterpret_cast<double*>(d_states + state_bytes);
+  // every byte the kernel reads is written by this copy, and every energy by
+  // the kernel: nothing needs clearing between calls
+  HIP_TRY(hipMemcpyAsync(d_states, states, (size_t)n_states * p->N, hipMemcpyHostToDevice, st));
+  HIP_TRY(launch_energy(p->dev(), d_states, n_states, d_energy, st));
+  HIP_TRY(hipMemcpyAsync(energies, d_energy, (size_t)n_states * sizeof(double), hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
   return QMC_OK;
 }
 
