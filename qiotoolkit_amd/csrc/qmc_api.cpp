@@ -302,6 +302,11 @@ struct qmc_batch_s {
     double work = 0;  // attempts planned (for spreading groups over streams)
     std::vector<UnitDev> units;
     DevBuf<UnitDev> d_units;
+    // cooperative groups: launches of units [chunk[i], chunk[i+1]) (at most
+    // the co-resident work-group count, never splitting a ladder);
+    // chunk_err: a ladder exceeds the co-resident limit
+    std::vector<int> chunk;
+    bool chunk_err = false;
   };
   std::vector<std::unique_ptr<Group>> groups;
   // where each member's descriptors sit in its group (valid while the
@@ -1641,6 +1646,25 @@ static int batch_rebuild(qmc_batch b) {
   }
   for (auto& g : b->groups) {
     HIP_TRY(g->d_units.upload(g->units.data(), g->units.size()));
+    if (g->coop) {
+      // launch chunks of at most the co-resident number of work-groups,
+      // never splitting a ladder (its work-groups are adjacent)
+      const size_t smem = sweep_smem_bytes(g->stride, g->ordered);
+      const int cap = max_coresident_blocks(g->kind, g->pairwise, g->coupling, g->nb, g->block, smem);
+      const int n = (int)g->units.size();
+      g->chunk.assign(1, 0);
+      g->chunk_err = false;
+      while (g->chunk.back() < n) {
+        const int start = g->chunk.back();
+        int end = start;
+        while (end < n && end + g->units[end].upl - start <= cap) end += g->units[end].upl;
+        if (end == start) {
+          g->chunk_err = true;
+          break;
+        }
+        g->chunk.push_back(end);
+      }
+    }
   }
   for (size_t i = 0; i < b->members.size(); ++i) b->seen_version[i] = b->members[i]->version;
   b->dirty = false;
@@ -1670,21 +1694,12 @@ static int launch_group(qmc_batch_s::Group& g, int32_t max_sweeps, const double*
                          g.ladder, st));
     return QMC_OK;
   }
-  // Cooperative ladders: launches of at most the co-resident number of
-  // work-groups, never splitting a ladder (its work-groups are adjacent).
-  const int cap = max_coresident_blocks(g.kind, g.pairwise, g.coupling, g.nb, g.block, smem);
-  int start = 0;
-  while (start < n) {
-    int end = start;
-    while (end < n) {
-      const int upl = g.units[end].upl;
-      if (end + upl - start > cap) break;
-      end += upl;
-    }
-    if (end == start) return fail(QMC_MEMORY_LIMITED, "a ladder exceeds the co-resident work-group limit");
+  // Cooperative ladders: the chunks planned when the group was built.
+  if (g.chunk_err) return fail(QMC_MEMORY_LIMITED, "a ladder exceeds the co-resident work-group limit");
+  for (size_t i = 0; i + 1 < g.chunk.size(); ++i) {
+    const int start = g.chunk[i], end = g.chunk[i + 1];
     HIP_TRY(launch_sweep(g.kind, g.pairwise, g.coupling, g.nb, g.d_units.p + start, end - start, g.block,
                          smem, a, 0, 1, 1, st));
-    start = end;
   }
   return QMC_OK;
 }

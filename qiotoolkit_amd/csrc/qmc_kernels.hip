@@ -148,6 +148,18 @@ __device__ __forceinline__ uint32_t in_vgpr(uint32_t x) {
   return v;
 }
 
+// LDS bytes: index arithmetic stays 32-bit (ds_read/ds_write take a 32-bit
+// address), where generic pointers into LDS make every gather a 64-bit
+// multiply-add.
+using lds_u8 = __attribute__((address_space(3))) uint8_t;
+using lds_u32 = __attribute__((address_space(3))) uint32_t;
+// Byte index of position q in byte lane g of a unit's LDS spin array: q * G +
+// g with a full-rate 24-bit multiply-add (q < 2^24: a spin array in LDS has
+// fewer than 160 Ki bytes).
+__device__ __forceinline__ uint32_t lds_at(uint32_t q, int G, int g) {
+  return __umul24(q, (uint32_t)G) + (uint32_t)g;
+}
+
 template <typename CT> __device__ __forceinline__ CT coef_of(uint32_t bits);
 template <> __device__ __forceinline__ float coef_of<float>(uint32_t bits) { return __uint_as_float(bits); }
 template <> __device__ __forceinline__ int32_t coef_of<int32_t>(uint32_t bits) { return (int32_t)bits; }
@@ -160,17 +172,41 @@ struct EllRow {
   uint32_t var;
 };
 
-__device__ __forceinline__ int2 load_slice(const UnitDev& U, int s) {
-  cptr<int32_t> t = cnst(reinterpret_cast<const int32_t*>(U.ell_slice));
-  return make_int2(t[2 * s], t[2 * s + 1]);
+// A pointer every lane of the wave holds, moved to scalar registers.
+template <class T> __device__ __forceinline__ T* uni_ptr(T* p) {
+  const uint64_t v = reinterpret_cast<uint64_t>(p);
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+  return reinterpret_cast<T*>(((uint64_t)hi << 32) | lo);
+}
+
+// The unit's graph tables, read once from its descriptor into scalar
+// registers: re-reading the descriptor inside the round loop would be a
+// vector load whose wait (vmcnt counts in order) drains the graph row
+// prefetched for the next round.
+struct Rows {
+  cptr<int32_t> slice;   // {offset, width} per slice
+  gptr<uint64_t> ent;    // pairwise entries {position, coefficient bits}
+  gptr<uint64_t> ent4;   // general 16-byte entries as two 64-bit words (or null)
+};
+__device__ __forceinline__ Rows rows_of(const UnitDev& U) {
+  Rows r;
+  r.slice = cnst(uni_ptr(reinterpret_cast<const int32_t*>(U.ell_slice)));
+  r.ent = glob(uni_ptr(reinterpret_cast<const uint64_t*>(U.ell_ent)));
+  r.ent4 = glob(uni_ptr(reinterpret_cast<const uint64_t*>(U.ell4_ent)));
+  return r;
+}
+
+__device__ __forceinline__ int2 load_slice(const Rows& R, int s) {
+  return make_int2(R.slice[2 * s], R.slice[2 * s + 1]);
 }
 
 // slice = {offset, width} (uniform); lane tp of a slice with `lanes` = 1 << lg.
 // Always 8 loads, without branches, so the compiler can keep them in flight
 // across the processing of the previous row: entries past the slice width
 // belong to the next slice (or the tail padding) and are masked at use.
-__device__ __forceinline__ void ell_fetch(const UnitDev& U, int2 slice, int tp, int lg, EllRow& e) {
-  gptr<uint64_t> p = glob(reinterpret_cast<const uint64_t*>(U.ell_ent)) + slice.x + tp;
+__device__ __forceinline__ void ell_fetch(const Rows& R, int2 slice, int tp, int lg, EllRow& e) {
+  gptr<uint64_t> p = R.ent + slice.x + tp;
   const int step = 1 << lg;
 #pragma unroll
   for (int j = 0; j < 8; ++j) {
@@ -196,15 +232,15 @@ __device__ __forceinline__ void accumulate(uint32_t wv, CT cc, CT (&h)[8]) {
 // reference's calculate_cost_difference does.  Pairwise problems read the
 // prefetched ELL row (entries beyond the first 8 are loaded here).
 template <int KIND, bool PAIR, typename CT, int NB>
-__device__ __forceinline__ void local_field(const UnitDev& U, const uint8_t* sp, int q, const EllRow& e,
-                                            int2 slice, int tp, int lg, int G, int g, CT (&h)[8], int sub = 0,
-                                            int lgs = 0) {
+__device__ __forceinline__ void local_field(const UnitDev& U, const Rows& R, const lds_u8* sp, int q,
+                                            const EllRow& e, int2 slice, int tp, int lg, int G, int g,
+                                            CT (&h)[8], int sub = 0, int lgs = 0) {
 #pragma unroll
   for (int b = 0; b < NB; ++b) h[b] = CT(0);
   if (PAIR) {
     uint32_t wv[8];
 #pragma unroll
-    for (int j = 0; j < 8; ++j) wv[j] = sp[(int)(uint32_t)e.ent[j] * G + g];
+    for (int j = 0; j < 8; ++j) wv[j] = sp[lds_at((uint32_t)e.ent[j], G, g)];
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
       // a zero coefficient past the width adds nothing (at most the sign of a zero field)
@@ -212,26 +248,26 @@ __device__ __forceinline__ void local_field(const UnitDev& U, const uint8_t* sp,
       accumulate<KIND, CT, NB>(wv[j], coef_of<CT>(cb), h);
     }
     if (slice.y > 8) {
-      gptr<uint64_t> p = glob(reinterpret_cast<const uint64_t*>(U.ell_ent)) + slice.x + tp;
+      gptr<uint64_t> p = R.ent + slice.x + tp;
       for (int j = 8; j < slice.y; ++j) {
         const uint64_t t = p[j << lg];
-        accumulate<KIND, CT, NB>(sp[(int)(uint32_t)t * G + g], coef_of<CT>((uint32_t)(t >> 32)), h);
+        accumulate<KIND, CT, NB>(sp[lds_at((uint32_t)t, G, g)], coef_of<CT>((uint32_t)(t >> 32)), h);
       }
     }
     return;
   }
-  if (U.ell4_ent) {
+  if (R.ent4) {
     // sliced entries {coefficient, up to 3 other members} in term order;
     // the ghost position pads missing members (its spin is always 0).  With
     // 2^lgs lanes per position (tp = the position's lane), this lane takes
     // entries sub, sub + 2^lgs, ...
-    gptr<uint64_t> p = glob(reinterpret_cast<const uint64_t*>(U.ell4_ent)) + 2 * (slice.x + tp);
+    gptr<uint64_t> p = R.ent4 + 2 * (slice.x + tp);
     const int step = 2 << lg;
     auto entry = [&](int j) {
       const uint64_t lo = p[j * step], hi = p[j * step + 1];
-      const uint32_t w0 = sp[(int)(uint32_t)(lo >> 32) * G + g];
-      const uint32_t w1 = sp[(int)(uint32_t)hi * G + g];
-      const uint32_t w2 = sp[(int)(uint32_t)(hi >> 32) * G + g];
+      const uint32_t w0 = sp[lds_at((uint32_t)(lo >> 32), G, g)];
+      const uint32_t w1 = sp[lds_at((uint32_t)hi, G, g)];
+      const uint32_t w2 = sp[lds_at((uint32_t)(hi >> 32), G, g)];
       const uint32_t w = (KIND == QMC_ISING) ? (w0 ^ w1 ^ w2) : (w0 | w1 | w2);
       accumulate<KIND, CT, NB>(w, coef_of<CT>((uint32_t)lo), h);
     };
@@ -254,7 +290,7 @@ __device__ __forceinline__ void local_field(const UnitDev& U, const uint8_t* sp,
     uint32_t w = 0;
     const int o0 = ooff[e], o1 = ooff[e + 1];
     for (int o = o0; o < o1; ++o) {
-      const uint32_t x = sp[opos[o] * G + g];
+      const uint32_t x = sp[lds_at(opos[o], G, g)];
       w = (KIND == QMC_ISING) ? (w ^ x) : (w | x);
     }
     accumulate<KIND, CT, NB>(w, cc, h);
@@ -332,19 +368,20 @@ k_sweep(const UnitDev* units, SweepArgs a) {
   Smem& S = *reinterpret_cast<Smem*>(smem_raw);
   // The descriptor stays in memory (scalar loads at each use): fields needed
   // only at the start and end are not held in registers across the sweeps.
-  const UnitDev& U = units[a.unit_of_block ? a.unit_of_block[blockIdx.x] : (int)blockIdx.x];
+  const UnitDev& U = units[uni(a.unit_of_block ? a.unit_of_block[blockIdx.x] : (int)blockIdx.x)];
   const int N = U.N, G = U.G, TP = U.lanes, lg = U.lanes_log2;
   const int stride = U.stride;
   cptr<int32_t> class_off = cnst(U.class_off);
   cptr<int32_t> slice_base = cnst(U.ell_slice_base);
   // general problems read sliced entries when their terms are at most 4-local
-  const bool gen_ell = !PAIR && U.ell4_ent != nullptr;
+  const Rows rows = rows_of(U);
+  const bool gen_ell = !PAIR && rows.ent4 != nullptr;
   const bool slices = PAIR || gen_ell;
-  gptr<int32_t> var_of_pos = glob(U.var_of_pos);
-  uint8_t* sp = smem_raw + sizeof(Smem);
-  uint8_t* fl = sp + stride;
+  gptr<int32_t> var_of_pos = glob(uni_ptr(U.var_of_pos));
+  lds_u8* sp = (lds_u8*)(smem_raw + sizeof(Smem));  // generic -> LDS address space
+  lds_u8* fl = sp + stride;
   // lowest states: in LDS for the launch when they fit, else in HBM
-  uint8_t* bst = a.best_lds ? fl + stride : U.best;
+  uint8_t* bst = a.best_lds ? reinterpret_cast<uint8_t*>(smem_raw + sizeof(Smem) + 2 * stride) : U.best;
   const int tid = threadIdx.x;
   const int nthreads = blockDim.x;
   const bool active = tid < TP * G;
@@ -364,7 +401,7 @@ k_sweep(const UnitDev* units, SweepArgs a) {
   }
   {
     const uint32_t* src = reinterpret_cast<const uint32_t*>(U.spins);
-    uint32_t* dst = reinterpret_cast<uint32_t*>(sp);
+    lds_u32* dst = reinterpret_cast<lds_u32*>(sp);
     for (int i = tid; i < stride / 4; i += nthreads) dst[i] = src[i];
     if (a.best_lds) {
       const uint32_t* bsrc = reinterpret_cast<const uint32_t*>(U.best);
@@ -427,10 +464,10 @@ k_sweep(const UnitDev* units, SweepArgs a) {
   auto fetch_row = [&](int cc0, int csb, int k, int2& sl, EllRow& e, int lgs) {
     const int rho = (k << lg) + (tp >> lgs);
     if (PAIR) {
-      sl = load_slice(U, csb + k);
-      ell_fetch(U, sl, tp, lg, e);
+      sl = load_slice(rows, csb + k);
+      ell_fetch(rows, sl, tp, lg, e);
     } else if (gen_ell) {
-      sl = load_slice(U, csb + k);
+      sl = load_slice(rows, csb + k);
     }
     e.var = (uint32_t)var_of_pos[cc0 + rho];  // padded past position N - 1
   };
@@ -499,7 +536,7 @@ k_sweep(const UnitDev* units, SweepArgs a) {
           const bool valid = rho < nc;
           const int q = valid ? c0 + rho : N;
           CT h[8];
-          local_field<KIND, PAIR, CT, NB>(U, sp, q, e, sl, tp >> lgs, lg, G, g, h, sub, lgs);
+          local_field<KIND, PAIR, CT, NB>(U, rows, sp, q, e, sl, tp >> lgs, lg, G, g, h, sub, lgs);
           if (kSplit && lgs) {
             // every lane of a position is active here (they share rho)
 #pragma unroll
@@ -508,7 +545,7 @@ k_sweep(const UnitDev* units, SweepArgs a) {
             if (sub) return;
           }
           const uint32_t var = e.var;
-          const uint32_t own = sp[q * G + g];
+          const uint32_t own = sp[lds_at(q, G, g)];
           CT dE[8];
           uint4 quad = make_uint4(0, 0, 0, 0);
           uint32_t flips = 0, deep = 0;
@@ -556,8 +593,8 @@ k_sweep(const UnitDev* units, SweepArgs a) {
             kmin[b] = lower ? k : kmin[b];
             acc[b] += ok ? 1u : 0u;
           }
-          sp[q * G + g] = (uint8_t)(own ^ flips);
-          fl[q * G + g] = (uint8_t)flips;
+          sp[lds_at(q, G, g)] = (uint8_t)(own ^ flips);
+          fl[lds_at(q, G, g)] = (uint8_t)flips;
         };
         // Fetches past the last round read sentinel slices and padding (the
         // tables are padded for it), so no fetch sits behind a branch and
@@ -734,7 +771,7 @@ k_sweep(const UnitDev* units, SweepArgs a) {
       if (b < nb_g && acc[b]) atomicAdd(&S.racc[8 * g + b], acc[b]);
   }
   {
-    const uint32_t* src = reinterpret_cast<const uint32_t*>(sp);
+    const lds_u32* src = reinterpret_cast<const lds_u32*>(sp);
     uint32_t* dst = reinterpret_cast<uint32_t*>(U.spins);
     for (int i = tid; i < stride / 4; i += nthreads) dst[i] = src[i];
     if (a.best_lds) {
