@@ -55,7 +55,9 @@ MEASURED_TRAFFIC_BYTES = {"c2": int((198183.25 * 2 + 82106) * 1024)}
 # What limits each kernel, from its SQ counters (profiles/k_sweep_*_sq_counters.md):
 # VALU issue plus stalls on the per-CU stream of graph rows from L2 and on
 # barriers; the HBM figure is the algorithmic bytes priced against the HBM peak.
-LIMITER = "valu-issue and L2->CU row stream (graph and state on chip); frac = algorithmic bytes vs HBM peak"
+LIMITER = ("on-chip: VALU issue, the L2->CU graph-row stream and the cross-work-group ladder exchange "
+           "(graph and state stay in L2/LDS); frac = algorithmic bytes over the HBM peak, above 1 when "
+           "on-chip reuse serves more bytes than HBM could")
 
 
 def log(*a):

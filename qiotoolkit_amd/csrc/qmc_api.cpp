@@ -307,6 +307,7 @@ struct qmc_batch_s {
     // chunk_err: a ladder exceeds the co-resident limit
     std::vector<int> chunk;
     bool chunk_err = false;
+    int pre_rounds = 0;  // early-field rounds of its split ladders (largest)
   };
   std::vector<std::unique_ptr<Group>> groups;
   // where each member's descriptors sit in its group (valid while the
@@ -1296,14 +1297,42 @@ int qmc_walkers_set_betas(qmc_walkers w, const double* betas) {
   return QMC_OK;
 }
 
+// Dynamic LDS of a sweep launch, plus room for the early fields of split
+// ladders (pre_rounds rounds of colour class 0, see SweepArgs) when they fit
+// the 160 KiB; fills the launch arguments that depend on it.
+static size_t sweep_launch_smem(int stride, int ordered, int block, int nb, int pre_rounds, SweepArgs& a) {
+  size_t smem = sweep_smem_bytes(stride, ordered, &a.best_lds);
+  a.pre_rounds = 0;
+  a.pre_off = 0;
+  if (pre_rounds > 0 && !ordered) {
+    const size_t off = (smem + 15) & ~(size_t)15;
+    const size_t bytes = sweep_pre_bytes(pre_rounds, block, nb);
+    if (off + bytes <= 160 * 1024) {
+      a.pre_rounds = pre_rounds;
+      a.pre_off = (int32_t)off;
+      smem = off + bytes;
+    }
+  }
+  return smem;
+}
+
+// Rounds of colour class 0 of a split ladder's sweeps (0: no early fields --
+// not a split ladder, or terms of more than two variables).
+static int early_rounds(qmc_walkers w) {
+  const qmc_problem p = w->prob;
+  if (!p->pairwise || w->ladder <= 0 || w->ladder == w->B || p->class_off.size() < 3) return 0;
+  return (p->class_off[1] - p->class_off[0] + w->lanes - 1) / w->lanes;
+}
+
 // ladder: the units hold parallel tempering ladders (exchange code needed)
 static int launch_units(qmc_problem p, int nb, const UnitDev* d_units, int n_units, int block, int stride,
-                        int32_t max_sweeps, int ordered, hipStream_t st, int coop, int ladder) {
+                        int32_t max_sweeps, int ordered, hipStream_t st, int coop, int ladder,
+                        int pre_rounds = 0) {
   SweepArgs a{};
   a.n_sweeps = max_sweeps;
   a.lut = p->ctx->lut.p;
   a.unit_of_block = nullptr;
-  const size_t smem = sweep_smem_bytes(stride, ordered, &a.best_lds);
+  const size_t smem = sweep_launch_smem(stride, ordered, block, nb, coop ? pre_rounds : 0, a);
   HIP_TRY(launch_sweep(p->kind, p->pairwise, p->coupling, nb, d_units, n_units, block, smem, a, ordered,
                        coop, ladder, st));
   return QMC_OK;
@@ -1459,7 +1488,10 @@ int qmc_walkers_pt(qmc_walkers w, int32_t ladder, const double* ladder_betas, in
   hipStream_t st = c->stream;
   const int n_ladders = w->R / std::max(1, ladder);
   int rc;
-  const size_t smem = sweep_smem_bytes(w->stride, 0);
+  SweepArgs sa{};
+  const int pre = (p->pairwise && w->B != ladder && p->class_off.size() >= 3)
+                      ? (p->class_off[1] - p->class_off[0] + w->lanes - 1) / w->lanes : 0;
+  const size_t smem = sweep_launch_smem(w->stride, 0, w->lanes * w->G, w->nb, pre, sa);
   const bool can_coop = w->B != ladder && ladder % w->B == 0 &&
       w->n_units <= max_coresident_blocks(p->kind, p->pairwise, p->coupling, w->nb, w->lanes * w->G, smem);
   if (!w->pt_obs.log && (w->B == ladder || (can_coop && !w->force_graph))) {
@@ -1468,7 +1500,7 @@ int qmc_walkers_pt(qmc_walkers w, int32_t ladder, const double* ladder_betas, in
     if ((rc = device_units(w, &du))) return rc;
     HIP_TRY(hipEventRecord(c->ev0, st));
     if ((rc = launch_units(p, w->nb, du, w->n_units, w->lanes * w->G, w->stride,
-                           n_steps * sweeps_per_replica, 0, st, w->coop, 1)))
+                           n_steps * sweeps_per_replica, 0, st, w->coop, 1, early_rounds(w))))
       return rc;
     HIP_TRY(hipEventRecord(c->ev1, st));
     HIP_TRY(hipStreamSynchronize(st));
@@ -1640,6 +1672,7 @@ static int batch_rebuild(qmc_batch b) {
       g->block = block;
     }
     g->stride = std::max(g->stride, w->stride);
+    if (w->coop) g->pre_rounds = std::max(g->pre_rounds, early_rounds(w));
     g->work += (double)p->N * w->R * (double)w->planned;
     b->member_at[i] = {g, g->units.size()};
     g->units.insert(g->units.end(), w->units.begin(), w->units.end());
@@ -1649,7 +1682,8 @@ static int batch_rebuild(qmc_batch b) {
     if (g->coop) {
       // launch chunks of at most the co-resident number of work-groups,
       // never splitting a ladder (its work-groups are adjacent)
-      const size_t smem = sweep_smem_bytes(g->stride, g->ordered);
+      SweepArgs sa{};
+      const size_t smem = sweep_launch_smem(g->stride, g->ordered, g->block, g->nb, g->pre_rounds, sa);
       const int cap = max_coresident_blocks(g->kind, g->pairwise, g->coupling, g->nb, g->block, smem);
       const int n = (int)g->units.size();
       g->chunk.assign(1, 0);
@@ -1686,7 +1720,7 @@ static int launch_group(qmc_batch_s::Group& g, int32_t max_sweeps, const double*
   a.n_sweeps = max_sweeps;
   a.lut = lut;
   a.unit_of_block = nullptr;
-  const size_t smem = sweep_smem_bytes(g.stride, g.ordered, &a.best_lds);
+  const size_t smem = sweep_launch_smem(g.stride, g.ordered, g.block, g.nb, g.coop ? g.pre_rounds : 0, a);
   if (smem > 160 * 1024) return fail(QMC_MEMORY_LIMITED, "batch LDS footprint exceeds 160 KiB");
   const int n = (int)g.units.size();
   if (!g.coop) {

@@ -180,7 +180,18 @@ struct SweepArgs {
   int32_t best_lds;      // lowest states kept in LDS during the launch (else in HBM)
   const double* lut;     // device, 1024 values
   const int32_t* unit_of_block;  // optional block -> unit map (null: identity)
+  // Split parallel tempering ladders: dynamic LDS at byte pre_off holds the
+  // local fields (NB per thread) and variable ids of pre_rounds rounds of
+  // colour class 0, computed while the ladder's energies travel between
+  // work-groups (0: no room, fields computed in the class as usual)
+  int32_t pre_rounds;
+  int32_t pre_off;
 };
+// Bytes of that area for a launch of `block` threads with NB replicas per
+// byte lane.
+inline size_t sweep_pre_bytes(int rounds, int block, int nb) {
+  return (size_t)rounds * block * (nb + 1) * 4;
+}
 
 // Dynamic LDS of a sweep launch whose largest unit spin array is max_stride
 // bytes; *best_lds (optional) tells whether the lowest states fit in it too.

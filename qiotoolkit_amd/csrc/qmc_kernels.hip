@@ -473,6 +473,37 @@ k_sweep(const UnitDev* units, SweepArgs a) {
   };
   if (active) fetch_row(class_off[0], slices ? slice_base[0] : 0, 0, sla, ra, split_of(class_off[1] - class_off[0]));
 
+  // Split ladders: while the ladder's energies travel between work-groups
+  // after a sweep, the local fields of the next sweep's colour class 0 are
+  // computed into LDS (they depend only on the spins of the other classes,
+  // final at the end of a sweep, and not on beta); the class then only
+  // decides.  Same fields, same decisions: the results do not change.
+  constexpr bool kEarly = PAIR && LADDER;
+  using lds_ct = __attribute__((address_space(3))) CT;
+  lds_ct* hpre = (lds_ct*)(smem_raw + a.pre_off);  // [round][replica][thread]
+  lds_u32* vpre = (lds_u32*)(smem_raw + a.pre_off + (size_t)a.pre_rounds * NB * nthreads * 4);  // [round][thread]
+  const int nc0 = class_off[1] - class_off[0];
+  const int K0 = (nc0 + TP - 1) >> lg;
+  const bool early_ok = kEarly && U.ladder > 0 && U.upl > 1 && n_classes >= 2 && K0 <= a.pre_rounds;
+  bool early_ready = false;
+  // round k of class cls, or round k - K(cls) of the class after it
+  auto fetch_in = [&](int cls, int k, int2& sl, EllRow& e) {
+    const int cc0 = class_off[cls];
+    const int Kc = (class_off[cls + 1] - cc0 + TP - 1) >> lg;
+    const int cn = cls + 1 < n_classes ? cls + 1 : 0;
+    const bool here = k < Kc;
+    fetch_row(here ? cc0 : class_off[cn], here ? slice_base[cls] : slice_base[cn], here ? k : k - Kc, sl, e, 0);
+  };
+  auto field_early = [&](int k, const int2& sl, const EllRow& e) {
+    const int rho = (k << lg) + tp;
+    const int q = rho < nc0 ? class_off[0] + rho : N;
+    CT h[8];
+    local_field<KIND, PAIR, CT, NB>(U, rows, sp, q, e, sl, tp, lg, G, g, h);
+#pragma unroll
+    for (int b = 0; b < NB; ++b) hpre[(k * NB + b) * nthreads + tid] = h[b];
+    vpre[k * nthreads + tid] = e.var;
+  };
+
   // parallel tempering: exchange step of the current sweep, sweeps run into
   // it and its energy buffer (step mod 3), advanced per sweep
   int64_t xstep = 0;
@@ -525,26 +556,9 @@ k_sweep(const UnitDev* units, SweepArgs a) {
           const bool here = k < K;
           fetch_row(here ? c0 : c0n, here ? sbase : sbasen, here ? k : k - K, sl, e, here ? lgs : lgsn);
         };
-        auto process = [&](int k, const int2& sl, const EllRow& e) {
-          const int sub = tp & ((1 << lgs) - 1);
-          const int rho = (k << lg) + (tp >> lgs);
-          // The term lists of general problems end at N - 1; pairwise rows
-          // past the class end are ghost padding, so those lanes run the
-          // same branch-free path on the ghost position (always 0, its flips
-          // forced to 0) and the row's loads are consumed on every path.
-          if (!PAIR && rho >= nc) return;
-          const bool valid = rho < nc;
-          const int q = valid ? c0 + rho : N;
-          CT h[8];
-          local_field<KIND, PAIR, CT, NB>(U, rows, sp, q, e, sl, tp >> lgs, lg, G, g, h, sub, lgs);
-          if (kSplit && lgs) {
-            // every lane of a position is active here (they share rho)
-#pragma unroll
-            for (int b = 0; b < NB; ++b)
-              for (int o = 1; o < (1 << lgs); o <<= 1) h[b] += __shfl_xor(h[b], o);
-            if (sub) return;
-          }
-          const uint32_t var = e.var;
+        // the decision for the replicas of this lane at position q (round
+        // k) from their local fields h
+        auto decide = [&](int k, int q, bool valid, const CT (&h)[8], uint32_t var) {
           const uint32_t own = sp[lds_at(q, G, g)];
           CT dE[8];
           uint4 quad = make_uint4(0, 0, 0, 0);
@@ -596,20 +610,53 @@ k_sweep(const UnitDev* units, SweepArgs a) {
           sp[lds_at(q, G, g)] = (uint8_t)(own ^ flips);
           fl[lds_at(q, G, g)] = (uint8_t)flips;
         };
-        // Fetches past the last round read sentinel slices and padding (the
-        // tables are padded for it), so no fetch sits behind a branch and
-        // the loads stay in flight while the other buffer is processed.
-        for (int k = 0; k < K; k += 2) {
-          fetch(k + 1, slb, rb);
-          process(k, sla, ra);
-          if (k + 1 >= K) break;
-          fetch(k + 2, sla, ra);
-          process(k + 1, slb, rb);
+        auto process = [&](int k, const int2& sl, const EllRow& e) {
+          const int sub = tp & ((1 << lgs) - 1);
+          const int rho = (k << lg) + (tp >> lgs);
+          // The term lists of general problems end at N - 1; pairwise rows
+          // past the class end are ghost padding, so those lanes run the
+          // same branch-free path on the ghost position (always 0, its flips
+          // forced to 0) and the row's loads are consumed on every path.
+          if (!PAIR && rho >= nc) return;
+          const bool valid = rho < nc;
+          const int q = valid ? c0 + rho : N;
+          CT h[8];
+          local_field<KIND, PAIR, CT, NB>(U, rows, sp, q, e, sl, tp >> lgs, lg, G, g, h, sub, lgs);
+          if (kSplit && lgs) {
+            // every lane of a position is active here (they share rho)
+#pragma unroll
+            for (int b = 0; b < NB; ++b)
+              for (int o = 1; o < (1 << lgs); o <<= 1) h[b] += __shfl_xor(h[b], o);
+            if (sub) return;
+          }
+          decide(k, q, valid, h, e.var);
+        };
+        if (kEarly && c == 0 && early_ready) {
+          // fields computed during the last exchange; the row pipeline
+          // already holds class 1's first row
+          for (int k = 0; k < K; ++k) {
+            CT h[8];
+#pragma unroll
+            for (int b = 0; b < NB; ++b) h[b] = hpre[(k * NB + b) * nthreads + tid];
+            const int rho = (k << lg) + tp;
+            decide(k, rho < nc ? c0 + rho : N, rho < nc, h, vpre[k * nthreads + tid]);
+          }
+        } else {
+          // Fetches past the last round read sentinel slices and padding
+          // (the tables are padded for it), so no fetch sits behind a branch
+          // and the loads stay in flight while the other buffer is processed.
+          for (int k = 0; k < K; k += 2) {
+            fetch(k + 1, slb, rb);
+            process(k, sla, ra);
+            if (k + 1 >= K) break;
+            fetch(k + 2, sla, ra);
+            process(k + 1, slb, rb);
+          }
+          // an even K leaves the next class's first row in `ra`; an odd K
+          // left it in `rb`, so fetch it again into `ra` (it stays in flight
+          // during the end of this class)
+          if (K & 1 || K == 0) fetch(K, sla, ra);
         }
-        // an even K leaves the next class's first row in `ra`; an odd K left
-        // it in `rb`, so fetch it again into `ra` (it stays in flight during
-        // the end of this class)
-        if (K & 1 || K == 0) fetch(K, sla, ra);
 
         // ---- per replica: minimum of the cost along the class sequence ----
         // The sequence is lane-major, so lane t's prefixes follow the sums of
@@ -666,6 +713,7 @@ k_sweep(const UnitDev* units, SweepArgs a) {
       }
       // no barrier: the next class writes the other parity's slots, and the
       // one after it passes the next class's barrier first
+      if (kEarly && c == 0) early_ready = false;
     }  // classes
     lds_barrier();
 
@@ -726,6 +774,21 @@ k_sweep(const UnitDev* units, SweepArgs a) {
         if (tid < nrep) {
           wait_stores();  // this slot's reset (one step ago) is complete
           store_agent(cb + U.slot0 + tid, S.rcost[tid]);
+        }
+        // while the energies travel: the fields of class 0 of the next sweep
+        // (its row pipeline moves on to class 1's first row)
+        if (early_ok && s + 1 < n_run) {
+          if (active) {
+            for (int k = 0; k < K0; k += 2) {
+              fetch_in(0, k + 1, slb, rb);
+              field_early(k, sla, ra);
+              if (k + 1 >= K0) break;
+              fetch_in(0, k + 2, sla, ra);
+              field_early(k + 1, slb, rb);
+            }
+            if (K0 & 1 || K0 == 0) fetch_in(0, K0, sla, ra);
+          }
+          early_ready = true;
         }
         if (tid < L) {
           double e = 0.0;
