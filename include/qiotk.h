@@ -143,6 +143,16 @@ int qiotk_observe(const char* params_json, const char* events_json, char** respo
 /* Solver::configure only: the resolved plan (steps, betas, restarts) */
 int qiotk_describe(const char* target, const char* params_json, char** response);
 
+/* The _QIO_METRICS line the reference's CLI prints after a solve
+ * (Runner::metric_to_console, app/runner.cc:640-688) for a response of
+ * qiotk_solve*: "_QIO_METRICS{"common": {solver_time_ms, cpu_time_ms,
+ * max_mem_bytes, preprocessing_time_ms, postprocessing_time_ms}, "solver":
+ * {...}, "problem": {num_terms, num_variables, locality, min_locality,
+ * avg_locality, accumulated_dependent_vars, sum_coefficient_degrees_total,
+ * max_coupling_magnitude, min_coupling_magnitude, model}}_QIO_METRICS"
+ * (absent numbers as 0).  Device-free. */
+int qiotk_metrics(const char* response_json, char** out);
+
 void qiotk_free(char* p);
 const char* qiotk_last_error(void);
 

@@ -43,6 +43,15 @@ def build_device():
     return out
 
 
+def _commit():
+    try:
+        r = subprocess.run(["git", "-C", ROOT, "rev-parse", "--short", "HEAD"], capture_output=True, text=True,
+                           timeout=10)
+        return r.stdout.strip() or "unknown"
+    except (OSError, subprocess.SubprocessError):
+        return "unknown"
+
+
 def build_host():
     if not HOST_SOURCES:
         return None
@@ -52,6 +61,7 @@ def build_host():
     # unload (the Python binding unloads it at exit, before the C runtime's
     # exit handlers)
     _run(["g++", "-std=c++17", "-O2", "-fPIC", "-shared", "-Wall", "-pthread", "-fno-gnu-unique",
+          '-DQIOTK_COMMIT="%s"' % _commit(),
           "-I", os.path.join(ROOT, "include"),
           "-o", out] + srcs + [os.path.join(HERE, "libqmc.so"), "-Wl,-rpath,$ORIGIN"])
     return out

@@ -1,9 +1,16 @@
 // C ABI of the host library (include/qiotk.h): Runner-style solves over the
 // device engine.
+#include <sys/resource.h>
+#include <sys/stat.h>
+#include <unistd.h>
+
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <ctime>
+#include <dirent.h>
 #include <memory>
 #include <string>
 
@@ -54,19 +61,106 @@ double now_ms() {
   return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
 }
 
-// Runner::get_run_output (app/runner.cc:513-631): init, timed run, finalize.
-Json run_one(Solver& s) {
-  const double t0 = now_ms();
+// process CPU time (utils/timing.cc:29)
+double cpu_ms() { return 1e3 * (double)std::clock() / CLOCKS_PER_SEC; }
+
+// peak resident memory (utils/operating_system.cc:20-25)
+int64_t max_rss_bytes() {
+  struct rusage ru {};
+  getrusage(RUSAGE_SELF, &ru);
+  return (int64_t)ru.ru_maxrss * 1024;
+}
+
+// Bytes of an input file, or of every file of a protobuf problem folder.
+int64_t path_bytes(const char* path) {
+  struct stat st {};
+  if (stat(path, &st) != 0) return 0;
+  if (!S_ISDIR(st.st_mode)) return (int64_t)st.st_size;
+  int64_t n = 0;
+  if (DIR* d = opendir(path)) {
+    while (dirent* e = readdir(d)) {
+      struct stat fs {};
+      const std::string f = std::string(path) + "/" + e->d_name;
+      if (stat(f.c_str(), &fs) == 0 && S_ISREG(fs.st_mode)) n += (int64_t)fs.st_size;
+    }
+    closedir(d);
+  }
+  return n;
+}
+
+const double kStartupMs = now_ms();
+const std::time_t kStartup = std::time(nullptr);
+
+// utils/metadata.cc:142-161
+Json build_properties() {
+  Json b = Json::object();
+  b["branch"] = std::string("main");
+  b["commit_hash"] = std::string(QIOTK_COMMIT);
+  b["compiler"] = std::string("g++ " __VERSION__);
+  b["build_type"] = std::string("Release");
+  b["qiotoolkit_version"] = std::string("qiotoolkit_amd (MI355X)");
+  return b;
+}
+Json invocation_properties() {
+  Json v = Json::object();
+  char host[256] = {0};
+  gethostname(host, sizeof host - 1);
+  v["host"] = std::string(host);
+  const char* user = std::getenv("USER");
+  v["user"] = std::string(user ? user : "");
+  char when[64] = {0};
+  std::strftime(when, sizeof when, "%Y-%m-%d %H:%M:%S", std::localtime(&kStartup));
+  v["datetime"] = std::string(when);
+  char cwd[4096] = {0};
+  v["directory"] = std::string(getcwd(cwd, sizeof cwd - 1) ? cwd : "");
+  return v;
+}
+
+// Where and when a request started, and what it read.
+struct Request {
+  double t0_ms = now_ms();
+  double cpu0_ms = cpu_ms();
+  int64_t read_bytes = 0;
+};
+
+// Runner::get_run_output (app/runner.cc:513-631): init, timed run,
+// finalize, and the benchmark block under the keys of utils/config.h.
+Json run_one(Solver& s, const Request& rq) {
+  const double t0 = now_ms(), c0 = cpu_ms();
   s.init();
   const double t1 = now_ms();
   s.run();
   const double t2 = now_ms();
   s.finalize();
   Json r = s.get_result();
-  r["benchmark"]["execution_time_ms"] = t2 - t1;
-  r["benchmark"]["preprocessing_ms"] = t1 - t0;
-  r["benchmark"]["postprocessing_ms"] = now_ms() - t2;
-  r["benchmark"]["threads"] = Json((int64_t)1);
+  Json& b = r["benchmark"];
+  // the solver's thread count: the configured `threads`, else the one host
+  // thread that drives the device
+  b["threads"] = Json((int64_t)std::max(1, s.thread_count()));
+  b["solver_time_ms"] = t2 - t1;
+  b["cpu_time_ms"] = cpu_ms() - c0;
+  b["max_mem_bytes"] = Json(max_rss_bytes());
+  b["disk_io_read_bytes"] = Json(rq.read_bytes);
+  Json ext = b["extensions"];
+  Json eb = Json::object();
+  eb["name"] = "build";
+  eb["value"] = build_properties();
+  ext.push_back(eb);
+  Json ei = Json::object();
+  ei["name"] = "invocation";
+  ei["value"] = invocation_properties();
+  ext.push_back(ei);
+  b["extensions"] = ext;
+  b["end2end_time_ms"] = now_ms() - rq.t0_ms;
+  b["end2end_cputime_ms"] = cpu_ms() - rq.cpu0_ms;
+  b["preprocessing_time_ms"] = t1 - t0;
+  b["postprocessing_time_ms"] = now_ms() - t2;
+  // the response's own size, this field included (app/runner.cc:612-622)
+  const std::string key = ",\"disk_io_write_bytes\":";
+  const int64_t n0 = (int64_t)(r.dump().size() + key.size());
+  int64_t n = n0 + 1;
+  while (n != n0 + (int64_t)std::to_string(n).size()) n = n0 + (int64_t)std::to_string(n).size();
+  b["disk_io_write_bytes"] = Json(n);
   return r;
 }
 
@@ -160,6 +254,42 @@ extern "C" {
 
 const char* qiotk_last_error(void) { return g_err.c_str(); }
 
+// Runner::metric_to_console (app/runner.cc:640-688): the common, solver and
+// problem metrics of a response's benchmark block between _QIO_METRICS labels.
+int qiotk_metrics(const char* response_json, char** out) {
+  return guarded(out, [&] {
+    const Json r = Json::parse(response_json);
+    if (!r.is_object() || !r.has("benchmark")) throw Error(kMissingInput, "the response has no benchmark block");
+    const Json& b = r.at("benchmark");
+    auto copy = [&](const char* k, Json& dst) { dst[k] = b.has(k) ? b.at(k) : Json((int64_t)0); };
+    Json solver_props = Json::object();
+    std::string model;
+    if (b.has("extensions"))
+      for (size_t i = 0; i < b.at("extensions").size(); ++i) {
+        const Json& e = b.at("extensions")[i];
+        if (e.at("name").as_string() == "solver") solver_props = e.at("value");
+        if (e.at("name").as_string() == "device" && e.at("value").has("model"))
+          model = e.at("value").at("model").as_string();
+      }
+    Json metrics = Json::object();
+    Json common = Json::object();
+    for (const char* k : {"solver_time_ms", "cpu_time_ms", "max_mem_bytes", "preprocessing_time_ms",
+                          "postprocessing_time_ms"})
+      copy(k, common);
+    metrics["common"] = common;
+    metrics["solver"] = b.has("solver") ? b.at("solver") : solver_props;
+    Json problem = Json::object();
+    for (const char* k : {"num_terms", "num_variables", "locality", "min_locality", "avg_locality",
+                          "accumulated_dependent_vars", "sum_coefficient_degrees_total", "max_coupling_magnitude",
+                          "min_coupling_magnitude"})
+      copy(k, problem);
+    problem["model"] = model;
+    metrics["problem"] = problem;
+    *out = dup("_QIO_METRICS" + metrics.dump() + "_QIO_METRICS");
+    return 0;
+  });
+}
+
 void qiotk_free(char* p) { std::free(p); }
 
 int qiotk_engine_create(int device, qiotk_engine* out) {
@@ -186,6 +316,7 @@ int qiotk_model_create(qiotk_engine e, const char* input_json, qiotk_model* out)
     auto m = std::make_unique<qiotk_model_s>();
     m->e = e;
     m->model = parse_model(Json::parse(input_json));
+    m->model.source_bytes = (int64_t)std::strlen(input_json);
     if (!m->model.empty()) {
       m->dev = std::make_unique<DeviceModel>(e->ctx, m->model);
       m->dev->handle(QMC_VISIT_COLOR_CLASSES);  // resident: solves exclude the upload
@@ -201,6 +332,7 @@ int qiotk_model_create_file(qiotk_engine e, const char* path, qiotk_model* out) 
     auto m = std::make_unique<qiotk_model_s>();
     m->e = e;
     m->model = parse_model(read_input_file(path));
+    m->model.source_bytes = path_bytes(path);
     if (!m->model.empty()) {
       m->dev = std::make_unique<DeviceModel>(e->ctx, m->model);
       m->dev->handle(QMC_VISIT_COLOR_CLASSES);  // resident: solves exclude the upload
@@ -236,8 +368,10 @@ int qiotk_solve_model(qiotk_model m, const char* target, const char* params_json
   return guarded(response, [&] {
     auto s = create_solver(target);
     s->set_model(&m->model, m->dev.get(), m->e->ctx);
+    Request rq;
+    rq.read_bytes = (int64_t)std::strlen(params_json) + m->model.source_bytes;
     s->configure(Json::parse(params_json));
-    Json r = run_one(*s);
+    Json r = run_one(*s, rq);
     *response = dup(r.dump());
     return 0;
   });
@@ -246,18 +380,16 @@ int qiotk_solve_model(qiotk_model m, const char* target, const char* params_json
 int qiotk_solve(qiotk_engine e, const char* target, const char* params_json, const char* input_json,
                 char** response) {
   return guarded(response, [&] {
-    const double t0 = now_ms();
+    Request rq;  // configure time and its CPU time count towards end2end
     qiotk_model m = nullptr;
     int rc = qiotk_model_create(e, input_json, &m);
     if (rc) throw Error(rc, g_err);
     std::unique_ptr<qiotk_model_s> hold(m);
+    rq.read_bytes = (int64_t)std::strlen(params_json) + m->model.source_bytes;
     auto s = create_solver(target);
     s->set_model(&m->model, m->dev.get(), e->ctx);
     s->configure(Json::parse(params_json));
-    const double t1 = now_ms();
-    Json r = run_one(*s);
-    r["benchmark"]["configure_ms"] = t1 - t0;
-    r["benchmark"]["end2end_time_ms"] = now_ms() - t0;
+    Json r = run_one(*s, rq);
     *response = dup(r.dump());
     return 0;
   });
@@ -266,18 +398,16 @@ int qiotk_solve(qiotk_engine e, const char* target, const char* params_json, con
 int qiotk_solve_file(qiotk_engine e, const char* target, const char* params_json, const char* input_path,
                      char** response) {
   return guarded(response, [&] {
-    const double t0 = now_ms();
+    Request rq;  // configure time and its CPU time count towards end2end
     qiotk_model m = nullptr;
     int rc = qiotk_model_create_file(e, input_path, &m);
     if (rc) throw Error(rc, g_err);
     std::unique_ptr<qiotk_model_s> hold(m);
+    rq.read_bytes = (int64_t)std::strlen(params_json) + m->model.source_bytes;
     auto s = create_solver(target);
     s->set_model(&m->model, m->dev.get(), e->ctx);
     s->configure(Json::parse(params_json));
-    const double t1 = now_ms();
-    Json r = run_one(*s);
-    r["benchmark"]["configure_ms"] = t1 - t0;
-    r["benchmark"]["end2end_time_ms"] = now_ms() - t0;
+    Json r = run_one(*s, rq);
     *response = dup(r.dump());
     return 0;
   });

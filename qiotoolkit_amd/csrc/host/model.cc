@@ -40,6 +40,7 @@ Model make_model(int kind, const std::vector<Term>& terms) {
     m.min_locality = std::min(m.min_locality, loc);
     m.total_locality += loc;
     if (loc > 0) {
+      m.dependent_vars += (uint64_t)loc * (loc - 1);
       const double a = std::fabs(t.c);
       m.max_coupling = std::max(m.max_coupling, a);
       m.min_coupling = std::min(m.min_coupling, a);

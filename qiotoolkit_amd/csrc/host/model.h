@@ -35,6 +35,8 @@ struct Model {
   // graph properties (graph/graph_properties.h)
   uint32_t max_locality = 0, min_locality = UINT32_MAX;
   uint64_t total_locality = 0;
+  uint64_t dependent_vars = 0;  // sum of locality x (locality - 1) over non-constant terms
+  int64_t source_bytes = 0;     // size of the input the model was read from (0: term arrays)
   double max_coupling = 0.0, min_coupling = 1.7976931348623157e308;
   // initial_configuration, internal ids -> reference bit (true = -1 / 0)
   std::vector<std::pair<int32_t, uint8_t>> initial;

@@ -174,6 +174,29 @@ void PopulationAnnealing::run() {
       finish_step();
     }
   };
+  if (time_limit_ || cost_limit_) {
+    // Ticks of steps with the limits checked in between, as SteppingSolver::run
+    // does (stepping_solver.h:119-258): the step limit, then the evaluation
+    // limit, the cost limit against the population's global lowest, the time
+    // limit; the tick grows to max(n + 1, 1.2 n).
+    const double t0 = now_s();
+    const int64_t total = steps_to_run();
+    int64_t tick = steps_per_tick_;
+    while (step_ < total) {
+      run_to(std::min(total, step_ + tick));
+      tick = (int64_t)std::max((double)tick + 1.0, (double)tick * 1.2);
+      if (step_ >= total) break;
+      qmc_pa_info info{};
+      device_check(qmc_walkers_pa_info(st.w, &info, nullptr), "qmc_walkers_pa_info");
+      if (eval_limit_ && info.evaluations >= *eval_limit_) {
+        exit_reason_ = "Stop due to Evaluation limit: " + std::to_string(*eval_limit_) + ", evaluations:" +
+                       std::to_string(info.evaluations);
+        break;
+      }
+      if (limits_reached(step_, t0, info.have_best != 0, info.best)) break;
+    }
+    return;
+  }
   // with an evaluation limit the population's count is read at each tick
   const int64_t steps = eval_limited_steps([&](int64_t s) {
     run_to(s);
@@ -185,7 +208,11 @@ void PopulationAnnealing::run() {
 }
 
 bool PopulationAnnealing::serve_plan(qmc_batch b) {
-  if (model_->empty() || step_ >= steps_to_run()) return false;
+  if (model_->empty() || step_ >= steps_to_run() || limit_hit_) return false;
+  if (!serving_) {
+    serving_ = true;
+    serve_t0_ = now_s();
+  }
   PaState& st = pa_state(impl_);
   device_check(qmc_walkers_plan(st.w, sweep_, sweeps_per_replica_, nullptr), "qmc_walkers_plan");
   if (!in_batch_) {
@@ -202,7 +229,15 @@ void PopulationAnnealing::serve_advance() {
   finish_step();
 }
 
-bool PopulationAnnealing::serve_done() { return model_->empty() || step_ >= steps_to_run(); }
+// Served with time or cost limits: checked after every wait for the device.
+bool PopulationAnnealing::serve_done() {
+  if (model_->empty() || step_ >= steps_to_run() || limit_hit_) return true;
+  if (!(time_limit_ || cost_limit_)) return false;
+  qmc_pa_info info{};
+  device_check(qmc_walkers_pa_info(pa_state(impl_).w, &info, nullptr), "qmc_walkers_pa_info");
+  limit_hit_ = limits_reached(step_, serve_t0_, info.have_best != 0, info.best);
+  return limit_hit_;
+}
 
 qmc_walkers PopulationAnnealing::walkers() const {
   return impl_ ? static_cast<PaState*>(impl_.get())->w : nullptr;

@@ -322,7 +322,9 @@ void ParameterFree::run() {
         stable = 0;
         update_best();
       }
-      if (cost_limit_ && have_lowest_ && lowest_cost_ + model_->const_cost <= *cost_limit_) {
+      // the lowest cost without the constant terms, as SteppingSolver
+      // compares it (stepping_solver.h:151-152, model_solver.h:119)
+      if (cost_limit_ && have_lowest_ && lowest_cost_ <= *cost_limit_) {
         exit_reason_ = "Stop due to cost limit: " + std::to_string(*cost_limit_);
         stop = true;
       }

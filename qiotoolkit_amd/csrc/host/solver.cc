@@ -161,15 +161,48 @@ void Solver::apply_lanes(qmc_walkers w, int replicas_per_group) const {
 bool Solver::serve_plan(qmc_batch b) {
   if (served_) return false;
   served_ = true;
+  serving_ = true;
+  serve_t0_ = now_s();
   return batch_plan(b);
 }
 
+double Solver::now_s() { return wall_s(); }
+
+bool Solver::limits_reached(int64_t steps, double t0, bool has_low, double low) {
+  if (cost_limit_ && has_low && low <= *cost_limit_) {
+    exit_reason_ = "Stop due to cost limit: " + std::to_string(*cost_limit_) + ", current lowest cost:";
+    return true;
+  }
+  if (time_limit_ && wall_s() - t0 >= *time_limit_) {
+    exit_reason_ = "Stop due to time_limit.steps : " + std::to_string(steps) + ", time_limit:" +
+                   std::to_string(*time_limit_);
+    return true;
+  }
+  return false;
+}
+
+// Serving: the whole run is planned at once; with time or cost limits they
+// are checked after every wait for the device, and a solve past one
+// completes there (the server takes its walkers out of the batch) with the
+// steps it ran.
 bool Solver::serve_done() {
   qmc_walkers w = walkers();
   if (!w) return true;
   int64_t left = 0;
   device_check(qmc_walkers_remaining(w, &left), "qmc_walkers_remaining");
-  return left == 0;
+  if (left == 0 || !(time_limit_ || cost_limit_)) return left == 0;
+  const int64_t planned = steps_done_;
+  const int64_t done = planned - (left + sweeps_per_step_ - 1) / sweeps_per_step_;
+  double low = 0.0;
+  if (cost_limit_) {
+    std::vector<double> l((size_t)batched_replicas_);
+    device_check(qmc_walkers_read(w, nullptr, l.data(), nullptr, nullptr, nullptr, nullptr), "qmc_walkers_read");
+    low = *std::min_element(l.begin(), l.end());
+  }
+  if (!limits_reached(done, serve_t0_, cost_limit_.has_value(), low)) return false;
+  if (planned > 0) attempts_ = attempts_ / planned * done;
+  steps_done_ = done;
+  return true;
 }
 
 void Solver::finalize_from(const double* lowest_energy, const uint64_t* accepted, const uint8_t* states) {
@@ -220,13 +253,25 @@ Json Solver::get_solutions() const {
 
 Json Solver::get_result() const {
   Json b = Json::object();
+  // Solver::get_benchmark (solver/solver.h:78-146) with the graph size
+  // property of graph/properties.h:52-69 under the keys of utils/config.h
   b["cost_function_evaluation_count"] = Json((int64_t)0);
   b["cost_difference_evaluation_count"] = Json(attempts_);
-  b["max_locality"] = Json((int64_t)(model_->empty() ? 0 : model_->max_locality));
-  b["min_locality"] = Json((int64_t)(model_->empty() ? 0 : model_->min_locality));
-  b["avg_locality"] = model_->n_terms() ? (double)model_->total_locality / (double)model_->n_terms() : 0.0;
-  b["variable_count"] = Json((int64_t)model_->n_vars);
-  b["term_count"] = Json(model_->n_terms());
+  Json size = Json::object();
+  size["nodes"] = Json((int64_t)model_->n_vars);
+  size["edges"] = Json(model_->n_terms());
+  size["locality"] = Json((int64_t)model_->max_locality);
+  size["min_locality"] = Json((int64_t)model_->min_locality);
+  size["avg_locality"] = model_->n_terms() ? (double)model_->total_locality / (double)model_->n_terms() : 0.0;
+  size["sum_coefficient_degrees_total"] = Json((int64_t)model_->total_locality);
+  size["accumulated_dependent_vars"] = Json((int64_t)model_->dependent_vars);
+  size["max_coupling_magnitude"] = model_->max_coupling;
+  size["min_coupling_magnitude"] = model_->min_coupling;
+  b["num_variables"] = size["nodes"];
+  b["num_terms"] = size["edges"];
+  for (const char* k : {"locality", "min_locality", "avg_locality", "accumulated_dependent_vars",
+                        "sum_coefficient_degrees_total", "max_coupling_magnitude", "min_coupling_magnitude"})
+    b[k] = size[k];
   Json ext = Json::array();
   Json counters = Json::object();
   counters["accepted_transitions"] = Json(accepted_);
@@ -254,7 +299,17 @@ Json Solver::get_result() const {
   c2["name"] = "solver";
   c2["value"] = sol;
   ext.push_back(c2);
+  // ModelSolver::get_model_properties -> GraphModel::get_benchmark_properties
+  Json graph = Json::object();
+  graph["size"] = size;
+  Json mprops = Json::object();
+  mprops["graph"] = graph;
+  Json cm = Json::object();
+  cm["name"] = "model";
+  cm["value"] = mprops;
+  ext.push_back(cm);
   Json dev = Json::object();
+  dev["model"] = model_->kind == QMC_PUBO ? "pubo" : "ising";  // Model::get_identifier
   dev["device_time_ms"] = timing_.device_ms;
   dev["launches"] = Json(timing_.launches);
   dev["attempts_per_s"] = timing_.device_ms > 0 ? (double)attempts_ / (timing_.device_ms * 1e-3) : 0.0;
@@ -396,7 +451,9 @@ void SimulatedAnnealing::run() {
 }
 
 bool SimulatedAnnealing::batch_plan(qmc_batch b) {
-  if (time_limit_ || cost_limit_) return false;
+  // limits need checks between launches: served solves get them between
+  // rounds, fused batches cannot stop one member early
+  if ((time_limit_ || cost_limit_) && !serving_) return false;
   if (model_->empty()) return true;
   const int64_t steps = planned_steps();
   device_check(qmc_walkers_plan(w_, 0, (int32_t)steps, betas_.data()), "qmc_walkers_plan");
@@ -583,7 +640,10 @@ void ParallelTempering::run() {
 }
 
 bool ParallelTempering::batch_plan(qmc_batch b) {
-  if (time_limit_ || cost_limit_ || observe_dynamics_) return false;
+  // limits need checks between launches: served solves get them between
+  // rounds, fused batches cannot stop one member early
+  if (((time_limit_ || cost_limit_) && !serving_) || observe_dynamics_) return false;
+  sweeps_per_step_ = sweeps_per_replica_;
   if (model_->empty()) return true;
   const int L = (int)betas_.size();
   const int64_t steps = planned_steps();

@@ -82,6 +82,8 @@ class Solver {
   const Timing& timing() const { return timing_; }
   double lowest_cost() const { return lowest_cost_; }
   uint32_t seed() const { return seed_; }
+  // the `threads` parameter (0: not given)
+  int thread_count() const { return threads_; }
 
  protected:
   // SteppingSolver (stepping_solver.h:112-268): fewer than steps_per_tick steps
@@ -113,6 +115,12 @@ class Solver {
     }
   }
   bool has_time_limit() const { return (bool)time_limit_; }
+  // time_limit and cost_limit after `steps` steps of a run started at
+  // wall-clock t0 (seconds), the lowest cost so far given when has_low
+  // (stepping_solver.h:151-152, 195-258); sets exit_reason_ in the
+  // reference's words when one is reached
+  bool limits_reached(int64_t steps, double t0, bool has_low, double low);
+  static double now_s();
   void record(double cost, const std::vector<uint8_t>& state);
   // ModelSolver::populate_solutions: best `number_of_solutions` walkers
   void populate(const std::vector<double>& lowest, const std::vector<uint8_t>& states, int64_t R);
@@ -178,6 +186,11 @@ class Solver {
   std::string exit_reason_;
   int64_t batched_replicas_ = 0;
   bool served_ = false;
+  // served round by round (serve_plan): a whole-run plan with time or cost
+  // limits checked by serve_done() between waits for the device
+  bool serving_ = false;
+  double serve_t0_ = 0.0;
+  int64_t sweeps_per_step_ = 1;
   friend class ParameterFree;  // reads its workers' results
 };
 
@@ -272,6 +285,7 @@ class PopulationAnnealing : public Solver {
   void finish_step();
   int64_t step_ = 0, sweep_ = 0;
   bool step_planned_ = false;  // serving: this round's sweeps are planned, its step not enqueued
+  bool limit_hit_ = false;     // serving: a time or cost limit ended the run
   bool in_batch_ = false;
   double alpha_ = 2.0;
   int64_t target_population_ = 64;

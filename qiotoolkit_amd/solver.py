@@ -23,7 +23,8 @@ EXPORTS = ["qiotk_engine_create", "qiotk_engine_destroy", "qiotk_solve", "qiotk_
            "qiotk_model_create_terms", "qiotk_solve_batch_arrays", "qiotk_model_vars",
            "qiotk_tune_batch", "qiotk_server_create", "qiotk_server_submit", "qiotk_server_submit_json",
            "qiotk_server_wait", "qiotk_server_stats", "qiotk_server_destroy", "qiotk_server_last_error",
-           "qiotk_model_create_file", "qiotk_solve_file", "qiotk_parse_model_file", "qiotk_observe"]
+           "qiotk_model_create_file", "qiotk_solve_file", "qiotk_parse_model_file", "qiotk_observe",
+           "qiotk_metrics"]
 
 
 def _check_rc(rc):
@@ -80,6 +81,7 @@ def lib():
     L.qiotk_schedule_eval.argtypes = [C.c_char_p, C.c_int32, P(C.c_double), P(C.c_double), P(vp)]
     L.qiotk_describe.argtypes = [C.c_char_p, C.c_char_p, P(vp)]
     L.qiotk_observe.argtypes = [C.c_char_p, C.c_char_p, P(vp)]
+    L.qiotk_metrics.argtypes = [C.c_char_p, P(vp)]
     L.qiotk_model_create_terms.argtypes = [vp, C.c_int, C.c_int64, P(C.c_int64), P(C.c_int32),
                                            P(C.c_double), P(vp)]
     _lib = L
@@ -358,6 +360,17 @@ def observe(params, events):
     out = C.c_void_p()
     lib().qiotk_observe(_enc(params), _enc(events), C.byref(out))
     return _raise_from(_take(out))
+
+
+def metrics(response):
+    """The _QIO_METRICS line of a solve response (Runner::metric_to_console)."""
+    out = C.c_void_p()
+    rc = lib().qiotk_metrics(_enc(response), C.byref(out))
+    s = C.cast(out, C.c_char_p).value.decode()
+    lib().qiotk_free(out)
+    if rc:
+        _raise_from(json.loads(s))
+    return s
 
 
 def describe(target, params):

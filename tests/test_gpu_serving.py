@@ -123,3 +123,41 @@ def test_admission_by_device_budget():
     assert rc_big == 1 and doc_big["error"]["code"] == 1
     assert 0 < st["peak_reserved_bytes"] <= budget
     assert st["peak_active"] <= 2
+
+
+def test_limited_requests_run_as_rounds():
+    """Requests with a time or cost limit are advanced round by round like
+    the others: a time-limited SA request of a million sweeps completes at its
+    limit with the reference's exit reason while the unlimited requests
+    submitted with it finish long before, each with its alone result; a
+    cost-limited PA request stops at its first check."""
+    from qiotoolkit_amd.solver import Server
+    reqs = requests()[:3]
+    alone = alone_results(reqs)
+    srv = Server(0, max_active=8, sweeps_per_round=10)
+    long_id = srv.submit(SA, {"params": {"beta_start": 0.1, "beta_stop": 3.0, "restarts": 8, "sweeps": 1000000,
+                                         "timeout": 2.0, "seed": 9}},
+                         "ising", *wl.er_ising_arrays(400, 6.0, 3, seed=77))
+    cost_id = srv.submit(PA, {"params": {"population": 32, "beta_start": 0, "beta_stop": 3, "sweeps": 2000,
+                                         "cost_limit": 1e9, "seed": 4}},
+                         "ising", *wl.er_ising_arrays(300, 6.0, 3, seed=78))
+    ids = [srv.submit(t, {"params": p}, k, *a) for t, p, k, a in reqs]
+    got = [srv.wait(i, 120000) for i in ids]
+    rc_long, doc_long = srv.wait(long_id, 120000)
+    rc_cost, doc_cost = srv.wait(cost_id, 120000)
+    srv.close()
+    for a, (rc, b) in zip(alone, got):
+        assert rc == 0
+        check_same(a, b)
+        # not held up behind the long request
+        assert b["benchmark"]["serving"]["finished_ms"] < doc_long["benchmark"]["serving"]["finished_ms"] - 500
+    assert rc_long == 0
+    sol = {e["name"]: e["value"] for e in doc_long["benchmark"]["extensions"]}["solver"]
+    assert sol["exit_reason"].startswith("Stop due to time_limit.steps : ")
+    assert 0 < sol["last_step"] < 1000000
+    lat = doc_long["benchmark"]["serving"]["finished_ms"] - doc_long["benchmark"]["serving"]["admitted_ms"]
+    assert 2000 <= lat < 20000
+    assert rc_cost == 0
+    sol = {e["name"]: e["value"] for e in doc_cost["benchmark"]["extensions"]}["solver"]
+    assert sol["exit_reason"].startswith("Stop due to cost limit:")
+    assert sol["last_step"] < 2000

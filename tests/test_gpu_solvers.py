@@ -445,3 +445,84 @@ def test_parameter_free_time_limit(engine):
     sol = [e for e in res["benchmark"]["extensions"] if e["name"] == "solver"][0]["value"]
     assert sol["training"]["runs"] == 4 and sol["training"]["passes"] == 1
     assert sol["search_exit_reason"].startswith("Stop due to time limit")
+
+
+def test_pa_time_and_cost_limits(engine):
+    """Population annealing checks time_limit and cost_limit after each tick
+    of 10, 12, 14, 16, ... steps (stepping_solver.h:151-152, 195-258): a spent
+    time limit stops after the first tick, a cost limit above the reachable
+    costs stops there too, each with the reference's exit reason."""
+    doc = {"cost_function": {"type": "ising", "terms": wl.lattice2d(8, seed=2)}}
+    base = {"population": 32, "beta_start": 0, "beta_stop": 3, "sweeps": 400, "seed": 5}
+    res = engine.solve("populationannealing.cpu", {"params": dict(base, timeout=1e-9)}, doc)
+    sol = {e["name"]: e["value"] for e in res["benchmark"]["extensions"]}["solver"]
+    assert sol["last_step"] == 10
+    assert sol["exit_reason"].startswith("Stop due to time_limit.steps : 10, time_limit:")
+    res = engine.solve("populationannealing.cpu", {"params": dict(base, cost_limit=1000)}, doc)
+    sol = {e["name"]: e["value"] for e in res["benchmark"]["extensions"]}["solver"]
+    assert sol["last_step"] == 10
+    assert sol["exit_reason"].startswith("Stop due to cost limit: 1000.000000")
+    assert res["solutions"]["cost"] <= 1000
+    # without limits the run takes all its steps
+    res = engine.solve("populationannealing.cpu", {"params": base}, doc)
+    sol = {e["name"]: e["value"] for e in res["benchmark"]["extensions"]}["solver"]
+    assert sol["last_step"] == 400
+
+
+@pytest.mark.gpu
+def test_parameter_free_predicted_overrun():
+    """On the 16^3 lattice longer runs keep finding lower costs, so the
+    search stretches its sweeps every pass; with a 1 s limit it stops before
+    a pass that, at the previous pass's rate, would not end within the limit
+    -- and so never overruns it."""
+    from qiotoolkit_amd.solver import Engine
+    eng = Engine(0)
+    doc = {"cost_function": {"type": "ising", "terms": wl.lattice3d(16, seed=1)}}
+    res = eng.solve("simulatedannealing-parameterfree.qiotoolkit", {"params": {"timeout": 1.0, "seed": 4}}, doc)
+    eng.close()
+    sol = [e for e in res["benchmark"]["extensions"] if e["name"] == "solver"][0]["value"]
+    assert sol["search_exit_reason"] == ("Stop due to time limit: 1.000000 "
+                                         "(the next pass would not end within it)"), sol["search_exit_reason"]
+    assert sol["training"]["passes"] >= 2
+    assert sol["training"]["search_ms"] < 1000.0
+
+
+def test_benchmark_block_of_the_runner(engine):
+    """The benchmark block as Runner::get_run_output and Solver::get_benchmark
+    render it (app/runner.cc:541-628, solver/solver.h:78-146, keys of
+    utils/config.h): graph size properties (puboemptymixed: 4 variables, a
+    constant term -> min_locality 0), timing, CPU time, memory, input and
+    response bytes, threads from the params, build and invocation
+    extensions; and the _QIO_METRICS line of the response."""
+    from qiotoolkit_amd import solver
+    doc = load("puboemptymixed.json")
+    terms = doc["cost_function"]["terms"]
+    params = {"params": {"beta_start": 0.1, "beta_stop": 3.0, "sweeps": 50, "restarts": 8, "seed": 3,
+                         "threads": 4}}
+    res = engine.solve("simulatedannealing.qiotoolkit", params, doc)
+    b = res["benchmark"]
+    nonconst = [t for t in terms if t["ids"]]
+    names = {i for t in terms for i in t["ids"]}
+    assert b["num_variables"] == len(names) and b["num_terms"] == len(nonconst)
+    assert b["locality"] == max(len(t["ids"]) for t in terms)
+    assert b["min_locality"] == min(len(t["ids"]) for t in terms)
+    assert b["sum_coefficient_degrees_total"] == sum(len(t["ids"]) for t in terms)
+    assert b["accumulated_dependent_vars"] == sum(len(t["ids"]) * (len(t["ids"]) - 1) for t in nonconst)
+    assert b["avg_locality"] == sum(len(t["ids"]) for t in terms) / len(nonconst)
+    assert b["max_coupling_magnitude"] == max(abs(t["c"]) for t in nonconst)
+    assert b["min_coupling_magnitude"] == min(abs(t["c"]) for t in nonconst)
+    assert b["threads"] == 4
+    assert b["solver_time_ms"] > 0 and b["cpu_time_ms"] > 0 and b["max_mem_bytes"] > 1 << 20
+    assert b["end2end_time_ms"] >= b["solver_time_ms"]
+    assert b["end2end_cputime_ms"] >= 0 and b["preprocessing_time_ms"] >= 0 and b["postprocessing_time_ms"] >= 0
+    assert b["disk_io_read_bytes"] == len(json.dumps(params)) + len(json.dumps(doc))
+    assert b["disk_io_write_bytes"] > 200
+    ext = {e["name"]: e["value"] for e in b["extensions"]}
+    assert {"counters", "solver", "model", "build", "invocation", "device"} <= set(ext)
+    assert ext["model"]["graph"]["size"]["nodes"] == b["num_variables"]
+    assert ext["build"]["build_type"] == "Release" and ext["invocation"]["host"]
+    line = solver.metrics(res)
+    m = json.loads(line[len("_QIO_METRICS"):-len("_QIO_METRICS")])
+    assert m["problem"]["model"] == "pubo" and m["problem"]["num_variables"] == b["num_variables"]
+    assert m["common"]["solver_time_ms"] == b["solver_time_ms"]
+    assert m["solver"]["last_step"] == 50

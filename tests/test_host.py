@@ -444,3 +444,37 @@ def test_parameter_free_configure_errors(params, msg):
     from qiotoolkit_amd import solver as S
     with pytest.raises(S.SolverError, match=re.escape(msg)):
         S.describe("simulatedannealing-parameterfree.qiotoolkit", {"params": params})
+
+
+def test_c_program_compiles_and_links(tmp_path):
+    """tests/c_abi/solve_fixture.c is valid C99 against include/qiotk.h and
+    include/qmc.h and links against libqiotk.so and libqmc.so (-Wall
+    -Werror); the GPU test runs it."""
+    from tests import _c_abi
+    exe = _c_abi.build(str(tmp_path))
+    assert os.path.exists(exe)
+
+
+def test_qio_metrics_line():
+    """qiotk_metrics renders Runner::metric_to_console (app/runner.cc:640-688):
+    common, solver and problem metrics between _QIO_METRICS labels, absent
+    numbers as 0, the model identifier from the response."""
+    from qiotoolkit_amd import solver
+    resp = {"benchmark": {"solver_time_ms": 12.5, "cpu_time_ms": 3.0, "max_mem_bytes": 1024,
+                          "num_terms": 10, "num_variables": 10, "locality": 2, "min_locality": 2,
+                          "avg_locality": 2.0, "accumulated_dependent_vars": 20,
+                          "sum_coefficient_degrees_total": 20, "max_coupling_magnitude": 1.0,
+                          "min_coupling_magnitude": 1.0,
+                          "extensions": [{"name": "solver", "value": {"last_step": 100}},
+                                         {"name": "device", "value": {"model": "ising"}}]},
+            "solutions": {"cost": -10}}
+    line = solver.metrics(resp)
+    assert line.startswith("_QIO_METRICS{") and line.endswith("}_QIO_METRICS")
+    m = json.loads(line[len("_QIO_METRICS"):-len("_QIO_METRICS")])
+    assert m["common"] == {"solver_time_ms": 12.5, "cpu_time_ms": 3.0, "max_mem_bytes": 1024,
+                           "preprocessing_time_ms": 0, "postprocessing_time_ms": 0}
+    assert m["solver"] == {"last_step": 100}
+    assert m["problem"]["num_terms"] == 10 and m["problem"]["model"] == "ising"
+    assert m["problem"]["accumulated_dependent_vars"] == 20
+    with pytest.raises(solver.SolverError):
+        solver.metrics({"solutions": {}})
