@@ -75,8 +75,8 @@ int qiotk_solve_batch_arrays(qiotk_engine e, const char* target, int32_t n, cons
 /* Autotuner for batched solves: short trials (trial_sweeps sweeps each, the
  * schedule compressed accordingly) of candidate launch shapes -- replicas per
  * work-group x lane depth (positions each lane covers per colour class,
- * which sets the LDS tile of a class round) -- on a sample of the solves (at
- * most 512), timed with HIP events; then, for the fastest shape, one
+ * which sets the LDS tile of a class round) -- on the solves (every k-th one
+ * past 8192), timed with HIP events; then, for the fastest shape, one
  * persistent launch against graph-replayed launches of 1/4 and 1/2 of the
  * trial sweeps.  Returned as {"params": {"replicas_per_group": B,
  * "positions_per_lane": k, "sweeps_per_launch": s (0 = persistent),

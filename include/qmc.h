@@ -65,7 +65,10 @@ typedef struct {
   int32_t max_degree;      /* max incident terms per variable */
   double mean_degree;
   int32_t max_class;       /* size of the largest colour class */
-  int32_t pad_;
+  int32_t entry_bytes;     /* bytes per incidence of the sliced rows: 4 (pairwise, integer
+                              coefficients in [-128, 127], 24-bit position + int8 coefficient),
+                              8 (pairwise, position + int32 or fp32 coefficient), 16 (terms of up
+                              to 4 variables), 0 (incidence lists) */
 } qmc_problem_info;
 
 /* thread-local message of the last failing call */

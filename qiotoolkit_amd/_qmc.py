@@ -49,7 +49,7 @@ class ProblemInfo(C.Structure):
         ("n_vars", C.c_int32), ("n_terms", C.c_int64), ("n_incidences", C.c_int64),
         ("n_classes", C.c_int32), ("lanes", C.c_int32), ("coupling", C.c_int32),
         ("pairwise", C.c_int32), ("max_degree", C.c_int32), ("mean_degree", C.c_double),
-        ("max_class", C.c_int32), ("pad_", C.c_int32),
+        ("max_class", C.c_int32), ("entry_bytes", C.c_int32),
     ]
 
 

@@ -460,8 +460,10 @@ int qiotk_tune_batch(qiotk_engine e, const char* target, int32_t n, const qiotk_
   return guarded(response, [&] {
     if (n <= 0) throw Error(kValueError, "nothing to tune");
     if (trial_sweeps < 2) throw Error(kValueError, "trial_sweeps must be >= 2");
-    // sample: every k-th solve, at most 512
-    const int32_t stride = std::max<int32_t>(1, (n + 511) / 512);
+    // sample: every k-th solve, at most 8192 -- the whole batch as a rule:
+    // how its graphs share the L2 and the memory behind it decides the
+    // shape, and a small sample fits caches the full batch does not
+    const int32_t stride = std::max<int32_t>(1, (n + 8191) / 8192);
     std::vector<int32_t> idx;
     for (int32_t k = 0; k < n; k += stride) idx.push_back(k);
     std::vector<qiotk_model> ms;

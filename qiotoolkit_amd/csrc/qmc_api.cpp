@@ -110,6 +110,13 @@ struct DevBuf {
 
 constexpr int kAuxStreams = 3;
 
+// Pairwise problems whose coefficients are integers in [-128, 127] and whose
+// positions fit 24 bits hold 4-byte entries {position | int8 coefficient <<
+// 24}: half the bytes of the 8-byte entries.
+constexpr int64_t kInt8PosLimit = (int64_t)1 << 24;
+static bool int8_entries_fit(qmc_problem p);
+
+
 // Handles are reference counted: a problem keeps its context alive, a
 // walker set its problem, a batch its context and members.  *_destroy drops
 // the caller's reference, so handles may be destroyed in any order.
@@ -180,14 +187,19 @@ static hipError_t upload_async(qmc_ctx c, void* dst, const void* src, size_t byt
 struct EllLayout {
   DevBuf<int32_t> slice_base;
   DevBuf<int2> slice;
-  DevBuf<uint2> ent;   // pairwise problems
+  DevBuf<uint2> ent;   // pairwise problems, 8-byte entries
+  // pairwise problems with int8 coefficients: 4-byte entries {position |
+  // coefficient << 24} (int8 set) instead of `ent`
+  DevBuf<uint32_t> ent32;
+  bool int8 = false;
   DevBuf<uint4> ent4;  // general problems with at most 3 other members per term
 };
 
 struct qmc_problem_s {
   std::atomic<int> refs{1};
   qmc_ctx ctx = nullptr;
-  std::map<int, std::unique_ptr<EllLayout>> ell;  // per lane count (pairwise problems)
+  std::map<int, std::unique_ptr<EllLayout>> ell;       // per lane count (compact entries where they fit)
+  std::map<int, std::unique_ptr<EllLayout>> ell_wide;  // per lane count, 8-byte pairwise entries
   std::vector<int32_t> h_inc_off, h_nbr;          // host copies to build layouts
   std::vector<double> h_coef;
   std::vector<int32_t> h_oth_off, h_oth_pos;      // general problems
@@ -253,6 +265,7 @@ struct qmc_walkers_s {
   int coop = 0;              // ladders split over work-groups, exchanged in-kernel
   int force_graph = 0;       // split ladders through graph-replayed launches instead
   int nb = 8;                // compile-time replicas per byte lane (1, 2, 4, 8)
+  int pair = 0;              // kernel form: 0 general terms, 1 pairwise 8-byte, 2 pairwise int8 entries
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
   int graph_key[4] = {0, 0, 0, 0};
@@ -621,7 +634,7 @@ int qmc_problem_get_info(qmc_problem p, qmc_problem_info* info) {
   info->max_class = 0;
   for (size_t c = 0; c + 1 < p->class_off.size(); ++c)
     info->max_class = std::max(info->max_class, p->class_off[c + 1] - p->class_off[c]);
-  info->pad_ = 0;
+  info->entry_bytes = p->pairwise ? (int8_entries_fit(p) ? 4 : 8) : (p->max_others <= 3 ? 16 : 0);
   return QMC_OK;
 }
 
@@ -645,9 +658,17 @@ int qmc_problem_order(qmc_problem p, int32_t lanes, int32_t* order, int32_t* cla
 
 // Sliced ELL of the pairwise form for `lanes` lanes: slice (c, k) covers
 // positions c0 + k*lanes .. + lanes of class c; width = its max degree.
-static EllLayout* get_layout(qmc_problem p, int lanes) {
-  auto it = p->ell.find(lanes);
-  if (it != p->ell.end()) return it->second.get();
+static bool int8_entries_fit(qmc_problem p) {
+  if (!p->pairwise || p->coupling != QMC_COUPLING_I32 || (int64_t)p->N >= kInt8PosLimit) return false;
+  for (double c : p->h_coef)
+    if (c < -128.0 || c > 127.0) return false;
+  return true;
+}
+
+static EllLayout* get_layout(qmc_problem p, int lanes, bool wide = false) {
+  auto& cache = (wide && p->pairwise) ? p->ell_wide : p->ell;
+  auto it = cache.find(lanes);
+  if (it != cache.end()) return it->second.get();
   auto L = std::make_unique<EllLayout>();
   const int nclass = (int)p->class_off.size() - 1;
   const bool i32 = p->coupling == QMC_COUPLING_I32;
@@ -706,9 +727,19 @@ static EllLayout* get_layout(qmc_problem p, int lanes) {
                     L->ent4.upload(ent4.data(), ent4.size()) == hipSuccess;
     if (!ok) return nullptr;
     EllLayout* raw = L.get();
-    p->ell[lanes] = std::move(L);
+    cache[lanes] = std::move(L);
     return raw;
   }
+  const bool compact = !wide && int8_entries_fit(p);
+  std::vector<uint32_t> ent32;
+  const uint32_t ghost = (uint32_t)p->N;
+  auto put = [&](uint32_t pos, double c) {
+    if (compact)
+      ent32.push_back(pos | ((uint32_t)(uint8_t)(int8_t)(int32_t)c << 24));
+    else
+      ent.push_back(make_uint2(pos, bits(c)));
+  };
+  auto n_ent = [&]() { return (int32_t)(compact ? ent32.size() : ent.size()); };
   for (int c = 0; c < nclass; ++c) {
     const int32_t c0 = p->class_off[c], nc = p->class_off[c + 1] - c0;
     sbase[c] = (int32_t)slice.size();
@@ -718,32 +749,40 @@ static EllLayout* get_layout(qmc_problem p, int lanes) {
         const int32_t q = c0 + k * lanes + t;
         w = std::max(w, p->h_inc_off[q + 1] - p->h_inc_off[q]);
       }
-      slice.push_back(make_int2((int32_t)ent.size(), w));
+      slice.push_back(make_int2(n_ent(), w));
       for (int32_t e = 0; e < w; ++e)
         for (int32_t t = 0; t < lanes; ++t) {
           const int32_t rho = k * lanes + t;
-          uint2 x = make_uint2((uint32_t)p->N, bits(0.0));
+          bool real = false;
           if (rho < nc) {
             const int32_t q = c0 + rho;
             const int32_t e0 = p->h_inc_off[q], deg = p->h_inc_off[q + 1] - e0;
-            if (e < deg) x = make_uint2((uint32_t)p->h_nbr[e0 + e], bits(p->h_coef[e0 + e]));
+            if (e < deg) {
+              put((uint32_t)p->h_nbr[e0 + e], p->h_coef[e0 + e]);
+              real = true;
+            }
           }
-          ent.push_back(x);
+          if (!real) put(ghost, 0.0);
         }
     }
   }
   sbase[nclass] = (int32_t)slice.size();
   // two sentinel slices and 8 rows of tail padding: the sweep prefetches up
   // to two rounds past a class end and always reads 8 rows of a slice
-  slice.push_back(make_int2((int32_t)ent.size(), 0));
-  slice.push_back(make_int2((int32_t)ent.size(), 0));
-  ent.insert(ent.end(), (size_t)8 * lanes + 1, make_uint2((uint32_t)p->N, bits(0.0)));
-  const bool ok = L->slice_base.upload(sbase.data(), sbase.size()) == hipSuccess &&
-                  L->slice.upload(slice.data(), slice.size()) == hipSuccess &&
-                  L->ent.upload(ent.data(), ent.size()) == hipSuccess;
+  slice.push_back(make_int2(n_ent(), 0));
+  slice.push_back(make_int2(n_ent(), 0));
+  for (int64_t i = 0; i < (int64_t)8 * lanes + 1; ++i) put(ghost, 0.0);
+  bool ok = L->slice_base.upload(sbase.data(), sbase.size()) == hipSuccess &&
+            L->slice.upload(slice.data(), slice.size()) == hipSuccess;
+  if (compact) {
+    ok = ok && L->ent32.upload(ent32.data(), ent32.size()) == hipSuccess;
+    L->int8 = true;
+  } else {
+    ok = ok && L->ent.upload(ent.data(), ent.size()) == hipSuccess;
+  }
   if (!ok) return nullptr;
   EllLayout* raw = L.get();
-  p->ell[lanes] = std::move(L);
+  cache[lanes] = std::move(L);
   return raw;
 }
 
@@ -751,6 +790,7 @@ static void build_units(qmc_walkers w) {
   qmc_problem p = w->prob;
   w->units.assign(w->n_units, UnitDev{});
   EllLayout* ell = (p->pairwise || p->max_others <= 3) ? get_layout(p, w->lanes) : nullptr;
+  w->pair = !p->pairwise ? 0 : (ell && ell->int8 ? 2 : 1);
   for (int u = 0; u < w->n_units; ++u) {
     UnitDev& U = w->units[u];
     if (ell) {
@@ -758,6 +798,7 @@ static void build_units(qmc_walkers w) {
       U.ell_slice = ell->slice.p;
       U.ell_ent = p->pairwise ? ell->ent.p : nullptr;
       U.ell4_ent = p->pairwise ? nullptr : ell->ent4.p;
+      U.ell_ent32 = ell->ent32.p;
     }
     U.class_off = p->d_class_off.p;
     U.var_of_pos = p->d_var_of_pos.p;
@@ -1325,15 +1366,16 @@ static int early_rounds(qmc_walkers w) {
 }
 
 // ladder: the units hold parallel tempering ladders (exchange code needed)
-static int launch_units(qmc_problem p, int nb, const UnitDev* d_units, int n_units, int block, int stride,
-                        int32_t max_sweeps, int ordered, hipStream_t st, int coop, int ladder,
-                        int pre_rounds = 0) {
+static int launch_units(qmc_walkers w, const UnitDev* d_units, int block, int32_t max_sweeps, int ordered,
+                        hipStream_t st, int coop, int ladder, int pre_rounds = 0) {
+  const qmc_problem p = w->prob;
+  const int nb = w->nb, n_units = w->n_units;
   SweepArgs a{};
   a.n_sweeps = max_sweeps;
   a.lut = p->ctx->lut.p;
   a.unit_of_block = nullptr;
-  const size_t smem = sweep_launch_smem(stride, ordered, block, nb, coop ? pre_rounds : 0, a);
-  HIP_TRY(launch_sweep(p->kind, p->pairwise, p->coupling, nb, d_units, n_units, block, smem, a, ordered,
+  const size_t smem = sweep_launch_smem(w->stride, ordered, block, nb, coop ? pre_rounds : 0, a);
+  HIP_TRY(launch_sweep(p->kind, ordered ? p->pairwise : w->pair, p->coupling, nb, d_units, n_units, block, smem, a, ordered,
                        coop, ladder, st));
   return QMC_OK;
 }
@@ -1469,8 +1511,7 @@ int qmc_walkers_sweep(qmc_walkers w, int64_t sweep0, int32_t n_sweeps, int beta_
   if ((rc = device_units(w, &du))) return rc;
   HIP_TRY(hipEventRecord(c->ev0, c->stream));
   ordered = ordered || w->ordered;
-  rc = launch_units(w->prob, w->nb, du, w->n_units, ordered ? 64 : w->lanes * w->G, w->stride,
-                    n_sweeps, ordered, c->stream, 0, 0);
+  rc = launch_units(w, du, ordered ? 64 : w->lanes * w->G, n_sweeps, ordered, c->stream, 0, 0);
   if (rc) return rc;
   HIP_TRY(hipEventRecord(c->ev1, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
@@ -1493,14 +1534,14 @@ int qmc_walkers_pt(qmc_walkers w, int32_t ladder, const double* ladder_betas, in
                       ? (p->class_off[1] - p->class_off[0] + w->lanes - 1) / w->lanes : 0;
   const size_t smem = sweep_launch_smem(w->stride, 0, w->lanes * w->G, w->nb, pre, sa);
   const bool can_coop = w->B != ladder && ladder % w->B == 0 &&
-      w->n_units <= max_coresident_blocks(p->kind, p->pairwise, p->coupling, w->nb, w->lanes * w->G, smem);
+      w->n_units <= max_coresident_blocks(p->kind, w->pair, p->coupling, w->nb, w->lanes * w->G, smem);
   if (!w->pt_obs.log && (w->B == ladder || (can_coop && !w->force_graph))) {
     if ((rc = qmc_walkers_plan_pt(w, ladder, ladder_betas, step0, n_steps, sweeps_per_replica))) return rc;
     const UnitDev* du = nullptr;
     if ((rc = device_units(w, &du))) return rc;
     HIP_TRY(hipEventRecord(c->ev0, st));
-    if ((rc = launch_units(p, w->nb, du, w->n_units, w->lanes * w->G, w->stride,
-                           n_steps * sweeps_per_replica, 0, st, w->coop, 1, early_rounds(w))))
+    if ((rc = launch_units(w, du, w->lanes * w->G, n_steps * sweeps_per_replica, 0, st, w->coop, 1,
+                           early_rounds(w))))
       return rc;
     HIP_TRY(hipEventRecord(c->ev1, st));
     HIP_TRY(hipStreamSynchronize(st));
@@ -1535,7 +1576,7 @@ int qmc_walkers_pt(qmc_walkers w, int32_t ladder, const double* ladder_betas, in
       drop_graph(w);
       HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
       for (int k = 0; k < K; ++k) {
-        rc = launch_units(p, w->nb, du, w->n_units, w->lanes * w->G, w->stride, sweeps_per_replica, 0, st, 0, 0);
+        rc = launch_units(w, du, w->lanes * w->G, sweeps_per_replica, 0, st, 0, 0);
         if (rc) {
           hipGraph_t g;
           (void)hipStreamEndCapture(st, &g);
@@ -1655,7 +1696,7 @@ static int batch_rebuild(qmc_batch b) {
     const int block = w->ordered ? 64 : w->lanes * w->G;
     qmc_batch_s::Group* g = nullptr;
     for (auto& gg : b->groups)
-      if (gg->kind == p->kind && gg->pairwise == p->pairwise && gg->coupling == p->coupling &&
+      if (gg->kind == p->kind && gg->pairwise == (w->ordered ? p->pairwise : w->pair) && gg->coupling == p->coupling &&
           gg->nb == w->nb && gg->coop == w->coop && gg->ladder == (w->ladder > 0) && gg->block == block &&
           gg->ordered == w->ordered)
         g = gg.get();
@@ -1663,7 +1704,7 @@ static int batch_rebuild(qmc_batch b) {
       b->groups.push_back(std::make_unique<qmc_batch_s::Group>());
       g = b->groups.back().get();
       g->kind = p->kind;
-      g->pairwise = p->pairwise;
+      g->pairwise = w->ordered ? p->pairwise : w->pair;
       g->coupling = p->coupling;
       g->nb = w->nb;
       g->coop = w->coop;

@@ -43,6 +43,9 @@ struct UnitDev {
   // over 16-byte entries {coefficient bits, the other members' positions,
   // padded with the ghost position N} (null: incidence CSR below)
   const uint4* ell4_ent;
+  // pairwise problems with int8 coefficients (kernels with PAIR 2): the same
+  // slices over 4-byte entries {neighbour position | coefficient << 24}
+  const uint32_t* ell_ent32;
   const int32_t* class_off;
   const int32_t* var_of_pos;
   const int32_t* pos_of_var;
@@ -196,6 +199,7 @@ inline size_t sweep_pre_bytes(int rounds, int block, int nb) {
 // Dynamic LDS of a sweep launch whose largest unit spin array is max_stride
 // bytes; *best_lds (optional) tells whether the lowest states fit in it too.
 size_t sweep_smem_bytes(int max_stride, int ordered, int* best_lds = nullptr);
+// pairwise: 0 general terms, 1 pairwise 8-byte entries, 2 pairwise int8 4-byte entries
 hipError_t launch_sweep(int kind, int pairwise, int coupling, int nb, const UnitDev* units, int n_units,
                         int block, size_t smem, const SweepArgs& a, int ordered, int cooperative, int ladder,
                         hipStream_t st);

@@ -164,11 +164,17 @@ template <typename CT> __device__ __forceinline__ CT coef_of(uint32_t bits);
 template <> __device__ __forceinline__ float coef_of<float>(uint32_t bits) { return __uint_as_float(bits); }
 template <> __device__ __forceinline__ int32_t coef_of<int32_t>(uint32_t bits) { return (int32_t)bits; }
 
-// The first (up to) 8 sliced-ELL entries of one position of a lane (low
-// word: neighbour position, high word: coefficient bits) and the position's
-// variable id, loaded ahead of use.
-struct EllRow {
+// The first (up to) 8 sliced-ELL entries of one position of a lane and the
+// position's variable id, loaded ahead of use.  PAIR 1: 8-byte entries (low
+// word: neighbour position, high word: coefficient bits); PAIR 2: 4-byte
+// entries of integer problems whose coefficients fit int8 (neighbour
+// position in the low 24 bits, the coefficient as int8 in the high 8).
+template <int PAIR> struct EllRow {
   uint64_t ent[8];
+  uint32_t var;
+};
+template <> struct EllRow<2> {
+  uint32_t ent[8];
   uint32_t var;
 };
 
@@ -188,12 +194,14 @@ struct Rows {
   cptr<int32_t> slice;   // {offset, width} per slice
   gptr<uint64_t> ent;    // pairwise entries {position, coefficient bits}
   gptr<uint64_t> ent4;   // general 16-byte entries as two 64-bit words (or null)
+  gptr<uint32_t> ent32;  // 4-byte pairwise entries {position | int8 coefficient << 24}
 };
 __device__ __forceinline__ Rows rows_of(const UnitDev& U) {
   Rows r;
   r.slice = cnst(uni_ptr(reinterpret_cast<const int32_t*>(U.ell_slice)));
   r.ent = glob(uni_ptr(reinterpret_cast<const uint64_t*>(U.ell_ent)));
   r.ent4 = glob(uni_ptr(reinterpret_cast<const uint64_t*>(U.ell4_ent)));
+  r.ent32 = glob(uni_ptr(U.ell_ent32));
   return r;
 }
 
@@ -205,13 +213,23 @@ __device__ __forceinline__ int2 load_slice(const Rows& R, int s) {
 // Always 8 loads, without branches, so the compiler can keep them in flight
 // across the processing of the previous row: entries past the slice width
 // belong to the next slice (or the tail padding) and are masked at use.
-__device__ __forceinline__ void ell_fetch(const Rows& R, int2 slice, int tp, int lg, EllRow& e) {
-  gptr<uint64_t> p = R.ent + slice.x + tp;
+template <int PAIR>
+__device__ __forceinline__ void ell_fetch(const Rows& R, int2 slice, int tp, int lg, EllRow<PAIR>& e) {
   const int step = 1 << lg;
+  if constexpr (PAIR == 2) {
+    gptr<uint32_t> p = R.ent32 + slice.x + tp;
 #pragma unroll
-  for (int j = 0; j < 8; ++j) {
-    e.ent[j] = *p;
-    p += step;
+    for (int j = 0; j < 8; ++j) {
+      e.ent[j] = *p;
+      p += step;
+    }
+  } else {
+    gptr<uint64_t> p = R.ent + slice.x + tp;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      e.ent[j] = *p;
+      p += step;
+    }
   }
 }
 
@@ -231,12 +249,33 @@ __device__ __forceinline__ void accumulate(uint32_t wv, CT cc, CT (&h)[8]) {
 // Sums run over the variable's incident terms in term order, as the
 // reference's calculate_cost_difference does.  Pairwise problems read the
 // prefetched ELL row (entries beyond the first 8 are loaded here).
-template <int KIND, bool PAIR, typename CT, int NB>
+template <int KIND, int PAIR, typename CT, int NB>
 __device__ __forceinline__ void local_field(const UnitDev& U, const Rows& R, const lds_u8* sp, int q,
-                                            const EllRow& e, int2 slice, int tp, int lg, int G, int g,
+                                            const EllRow<PAIR>& e, int2 slice, int tp, int lg, int G, int g,
                                             CT (&h)[8], int sub = 0, int lgs = 0) {
 #pragma unroll
   for (int b = 0; b < NB; ++b) h[b] = CT(0);
+  if constexpr (PAIR == 2) {
+    // int8 entries: position in the low 24 bits, the coefficient in the high
+    // 8 (entries past the width belong to the next slice and are masked)
+    static_assert(std::is_same<CT, int32_t>::value, "int8 entries carry integer coefficients");
+    uint32_t wv[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) wv[j] = sp[lds_at(e.ent[j] & 0xFFFFFFu, G, g)];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int32_t c = j < slice.y ? (int32_t)e.ent[j] >> 24 : 0;
+      accumulate<KIND, CT, NB>(wv[j], c, h);
+    }
+    if (slice.y > 8) {
+      gptr<uint32_t> p = R.ent32 + slice.x + tp;
+      for (int j = 8; j < slice.y; ++j) {
+        const uint32_t t = p[j << lg];
+        accumulate<KIND, CT, NB>(sp[lds_at(t & 0xFFFFFFu, G, g)], (int32_t)t >> 24, h);
+      }
+    }
+    return;
+  }
   if (PAIR) {
     uint32_t wv[8];
 #pragma unroll
@@ -360,7 +399,7 @@ __device__ __forceinline__ double load_agent(const double* p) {
 // LADDER: the launch may hold parallel tempering ladders (exchanges in the
 // kernel); without it the exchange code, and the registers it needs, are
 // compiled out.
-template <int KIND, bool PAIR, typename CT, int NB, bool LADDER>
+template <int KIND, int PAIR, typename CT, int NB, bool LADDER>
 __global__ void __launch_bounds__(kBlock)
 k_sweep(const UnitDev* units, SweepArgs a) {
   using AT = typename AccT<CT>::T;
@@ -447,7 +486,7 @@ k_sweep(const UnitDev* units, SweepArgs a) {
   // rounds past a class's end are the next class's first rounds (the last
   // class is followed by class 0 of the next sweep), so the first row of a
   // class is already in flight while the previous class finishes.
-  EllRow ra, rb;
+  EllRow<PAIR> ra, rb;
   int2 sla = make_int2(0, 0), slb = make_int2(0, 0);
   // Small classes of general problems with integer couplings share each
   // position among 2^lgs lanes (at most 8, while positions x lanes per
@@ -461,7 +500,7 @@ k_sweep(const UnitDev* units, SweepArgs a) {
       while (l < 3 && (ncls << (l + 1)) <= TP) ++l;
     return l;
   };
-  auto fetch_row = [&](int cc0, int csb, int k, int2& sl, EllRow& e, int lgs) {
+  auto fetch_row = [&](int cc0, int csb, int k, int2& sl, EllRow<PAIR>& e, int lgs) {
     const int rho = (k << lg) + (tp >> lgs);
     if (PAIR) {
       sl = load_slice(rows, csb + k);
@@ -487,14 +526,14 @@ k_sweep(const UnitDev* units, SweepArgs a) {
   const bool early_ok = kEarly && U.ladder > 0 && U.upl > 1 && n_classes >= 2 && K0 <= a.pre_rounds;
   bool early_ready = false;
   // round k of class cls, or round k - K(cls) of the class after it
-  auto fetch_in = [&](int cls, int k, int2& sl, EllRow& e) {
+  auto fetch_in = [&](int cls, int k, int2& sl, EllRow<PAIR>& e) {
     const int cc0 = class_off[cls];
     const int Kc = (class_off[cls + 1] - cc0 + TP - 1) >> lg;
     const int cn = cls + 1 < n_classes ? cls + 1 : 0;
     const bool here = k < Kc;
     fetch_row(here ? cc0 : class_off[cn], here ? slice_base[cls] : slice_base[cn], here ? k : k - Kc, sl, e, 0);
   };
-  auto field_early = [&](int k, const int2& sl, const EllRow& e) {
+  auto field_early = [&](int k, const int2& sl, const EllRow<PAIR>& e) {
     const int rho = (k << lg) + tp;
     const int q = rho < nc0 ? class_off[0] + rho : N;
     CT h[8];
@@ -552,7 +591,7 @@ k_sweep(const UnitDev* units, SweepArgs a) {
         // buffer while round k is processed.
         // round k of this class, or round k - K of the next (no branch: the
         // loads stay in flight across the processing of the other buffer)
-        auto fetch = [&](int k, int2& sl, EllRow& e) {
+        auto fetch = [&](int k, int2& sl, EllRow<PAIR>& e) {
           const bool here = k < K;
           fetch_row(here ? c0 : c0n, here ? sbase : sbasen, here ? k : k - K, sl, e, here ? lgs : lgsn);
         };
@@ -610,7 +649,7 @@ k_sweep(const UnitDev* units, SweepArgs a) {
           sp[lds_at(q, G, g)] = (uint8_t)(own ^ flips);
           fl[lds_at(q, G, g)] = (uint8_t)flips;
         };
-        auto process = [&](int k, const int2& sl, const EllRow& e) {
+        auto process = [&](int k, const int2& sl, const EllRow<PAIR>& e) {
           const int sub = tp & ((1 << lgs) - 1);
           const int rho = (k << lg) + (tp >> lgs);
           // The term lists of general problems end at N - 1; pairwise rows
@@ -2034,11 +2073,11 @@ static hipError_t raise_lds_limit(F fn) {
   return once;
 }
 
-template <int KIND, bool PAIR, typename CT, int NB>
+template <int KIND, int PAIR, typename CT, int NB>
 static hipError_t launch_sweep_t(const UnitDev* units, int n_units, int block, size_t smem,
                                  const SweepArgs& a, int ordered, int cooperative, int ladder, hipStream_t st) {
   if (ordered) {
-    auto fn = k_sweep_ordered<KIND, PAIR, CT>;
+    auto fn = k_sweep_ordered<KIND, (PAIR != 0), CT>;
     hipError_t e = raise_lds_limit(fn);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(fn, dim3(n_units), dim3(64), smem, st, units, a);
@@ -2058,7 +2097,7 @@ static hipError_t launch_sweep_t(const UnitDev* units, int n_units, int block, s
   return hipGetLastError();
 }
 
-template <int KIND, bool PAIR, typename CT>
+template <int KIND, int PAIR, typename CT>
 static hipError_t launch_nb(int nb, const UnitDev* units, int n_units, int block, size_t smem,
                             const SweepArgs& a, int ordered, int coop, int ladder, hipStream_t st) {
   if (nb <= 1) return launch_sweep_t<KIND, PAIR, CT, 1>(units, n_units, block, smem, a, ordered, coop, ladder, st);
@@ -2071,21 +2110,23 @@ hipError_t launch_sweep(int kind, int pairwise, int coupling, int nb, const Unit
                         int block, size_t smem, const SweepArgs& a, int ordered, int coop, int ladder,
                         hipStream_t st) {
   const bool i32 = coupling == QMC_COUPLING_I32;
+#define QMC_LAUNCH(K, P)                                                                                \
+  return i32 ? launch_nb<K, P, int32_t>(nb, units, n_units, block, smem, a, ordered, coop, ladder, st) \
+             : launch_nb<K, P, float>(nb, units, n_units, block, smem, a, ordered, coop, ladder, st)
+  // int8 entries only for integer coefficients
+  if (pairwise == 2 && !i32) return hipErrorInvalidValue;
   if (kind == QMC_ISING) {
-    if (pairwise)
-      return i32 ? launch_nb<QMC_ISING, true, int32_t>(nb, units, n_units, block, smem, a, ordered, coop, ladder, st)
-                 : launch_nb<QMC_ISING, true, float>(nb, units, n_units, block, smem, a, ordered, coop, ladder, st);
-    return i32 ? launch_nb<QMC_ISING, false, int32_t>(nb, units, n_units, block, smem, a, ordered, coop, ladder, st)
-               : launch_nb<QMC_ISING, false, float>(nb, units, n_units, block, smem, a, ordered, coop, ladder, st);
+    if (pairwise == 2) return launch_nb<QMC_ISING, 2, int32_t>(nb, units, n_units, block, smem, a, ordered, coop, ladder, st);
+    if (pairwise) QMC_LAUNCH(QMC_ISING, 1);
+    QMC_LAUNCH(QMC_ISING, 0);
   }
-  if (pairwise)
-    return i32 ? launch_nb<QMC_PUBO, true, int32_t>(nb, units, n_units, block, smem, a, ordered, coop, ladder, st)
-               : launch_nb<QMC_PUBO, true, float>(nb, units, n_units, block, smem, a, ordered, coop, ladder, st);
-  return i32 ? launch_nb<QMC_PUBO, false, int32_t>(nb, units, n_units, block, smem, a, ordered, coop, ladder, st)
-             : launch_nb<QMC_PUBO, false, float>(nb, units, n_units, block, smem, a, ordered, coop, ladder, st);
+  if (pairwise == 2) return launch_nb<QMC_PUBO, 2, int32_t>(nb, units, n_units, block, smem, a, ordered, coop, ladder, st);
+  if (pairwise) QMC_LAUNCH(QMC_PUBO, 1);
+  QMC_LAUNCH(QMC_PUBO, 0);
+#undef QMC_LAUNCH
 }
 
-template <int KIND, bool PAIR, typename CT, int NB>
+template <int KIND, int PAIR, typename CT, int NB>
 static int coresident_t(int block, size_t smem) {
   int per_cu = 0;
   // cooperative launches carry ladders
@@ -2098,7 +2139,7 @@ static int coresident_t(int block, size_t smem) {
   return per_cu * cus;
 }
 
-template <int KIND, bool PAIR, typename CT>
+template <int KIND, int PAIR, typename CT>
 static int coresident_nb(int nb, int block, size_t smem) {
   if (nb <= 1) return coresident_t<KIND, PAIR, CT, 1>(block, smem);
   if (nb <= 2) return coresident_t<KIND, PAIR, CT, 2>(block, smem);
@@ -2108,16 +2149,18 @@ static int coresident_nb(int nb, int block, size_t smem) {
 
 int max_coresident_blocks(int kind, int pairwise, int coupling, int nb, int block, size_t smem) {
   const bool i32 = coupling == QMC_COUPLING_I32;
+#define QMC_CORES(K, P) \
+  return i32 ? coresident_nb<K, P, int32_t>(nb, block, smem) : coresident_nb<K, P, float>(nb, block, smem)
+  if (pairwise == 2 && !i32) return 0;
   if (kind == QMC_ISING) {
-    if (pairwise) return i32 ? coresident_nb<QMC_ISING, true, int32_t>(nb, block, smem)
-                             : coresident_nb<QMC_ISING, true, float>(nb, block, smem);
-    return i32 ? coresident_nb<QMC_ISING, false, int32_t>(nb, block, smem)
-               : coresident_nb<QMC_ISING, false, float>(nb, block, smem);
+    if (pairwise == 2) return coresident_nb<QMC_ISING, 2, int32_t>(nb, block, smem);
+    if (pairwise) QMC_CORES(QMC_ISING, 1);
+    QMC_CORES(QMC_ISING, 0);
   }
-  if (pairwise) return i32 ? coresident_nb<QMC_PUBO, true, int32_t>(nb, block, smem)
-                           : coresident_nb<QMC_PUBO, true, float>(nb, block, smem);
-  return i32 ? coresident_nb<QMC_PUBO, false, int32_t>(nb, block, smem)
-             : coresident_nb<QMC_PUBO, false, float>(nb, block, smem);
+  if (pairwise == 2) return coresident_nb<QMC_PUBO, 2, int32_t>(nb, block, smem);
+  if (pairwise) QMC_CORES(QMC_PUBO, 1);
+  QMC_CORES(QMC_PUBO, 0);
+#undef QMC_CORES
 }
 
 hipError_t launch_exchange(double* cost, double* beta, int32_t* label, const double* lbeta,

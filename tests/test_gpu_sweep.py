@@ -461,3 +461,31 @@ def test_lds_bandwidth_probe(ctx):
     (MI355X: 256 CUs x 128 B/clk x 2.4 GHz = 78.6 TB/s nominal)."""
     bw = ctx.lds_bandwidth()
     assert 1e13 < bw < 2e14, bw
+
+
+def test_row_entry_widths_and_wide_rows(ctx):
+    """Pairwise rows hold 4-byte entries (24-bit position, int8 coefficient)
+    for integer coefficients in [-128, 127] and 8-byte entries otherwise (fp32,
+    or an integer beyond int8); both forms are bit-exact against the oracle
+    (the 4-byte form through every integer pairwise test; here the 8-byte form
+    with an integer coefficient of 1000, and an fp32 ER model: identical
+    states, costs within 1e-5)."""
+    kind, gen = PROBLEMS["lattice3d_f32"]
+    _, qp = pair(ctx, kind, gen())
+    assert qp.info.entry_bytes == 8
+    _, qp = pair(ctx, "ising", wl.er_ising(150, 6.0, seed=4))
+    assert qp.info.entry_bytes == 4
+    big = wl.er_ising(150, 6.0, seed=4)
+    big[-1] = dict(big[-1], c=1000)
+    op, qp, w, got, want = _run_pair(ctx, "ising", big, R=16, B=8, betas=np.linspace(0.1, 3.0, 12))
+    assert qp.info.entry_bytes == 8 and qp.info.coupling == _qmc.QMC_COUPLING_I32
+    for k in ("states", "cost", "lowest", "lowest_states"):
+        assert np.array_equal(got[k], want[k]), k
+    _, qp = pair(ctx, "pubo", wl.random_pubo(120, 600, seed=5))
+    assert qp.info.entry_bytes == 16
+    terms = wl.er_ising_f32(5000, 8.0, seed=17)
+    op, qp, w, got, want = _run_pair(ctx, "ising", terms, R=16, B=16, betas=np.linspace(0.2, 3.0, 12))
+    assert qp.info.entry_bytes == 8 and qp.info.coupling == _qmc.QMC_COUPLING_F32
+    assert np.array_equal(got["states"], want["states"])
+    np.testing.assert_allclose(got["cost"], want["cost"], rtol=1e-5)
+    np.testing.assert_allclose(got["lowest"], want["lowest"], rtol=1e-5)
