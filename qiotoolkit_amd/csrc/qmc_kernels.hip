@@ -357,17 +357,20 @@ __device__ __forceinline__ CT delta_from_field(CT h, uint32_t own_bit) {
 // Bounded: after ~2 s of polling the error flag is raised and all members
 // abandon the run instead of hanging.
 __device__ bool await_energy(const UnitDev& U, const double* slot, double& out) {
+  using g64 = __attribute__((address_space(1))) unsigned long long;
+  using g32 = __attribute__((address_space(1))) int32_t;
+  g64* sl = (g64*)slot;
+  g32* err = (g32*)U.err;
   const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
   while (true) {
-    const double v = __longlong_as_double((long long)__hip_atomic_load(
-        reinterpret_cast<const unsigned long long*>(slot), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    const double v = __longlong_as_double((long long)__hip_atomic_load(sl, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
     if (v == v) {
       out = v;
       return true;
     }
-    if (__hip_atomic_load(U.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return false;
+    if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return false;
     if (__builtin_amdgcn_s_memrealtime() - t0 > 200000000ull) {  // 100 MHz clock: 2 s
-      __hip_atomic_store(U.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       return false;
     }
     __builtin_amdgcn_s_sleep(1);
@@ -388,12 +391,13 @@ __device__ __forceinline__ void lds_barrier() {
 __device__ __forceinline__ void wait_stores() { __builtin_amdgcn_s_waitcnt(0x0F70); }
 
 __device__ __forceinline__ void store_agent(double* p, double v) {
-  __hip_atomic_store(reinterpret_cast<unsigned long long*>(p), (unsigned long long)__double_as_longlong(v),
-                     __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  using g64 = __attribute__((address_space(1))) unsigned long long;
+  __hip_atomic_store((g64*)p, (unsigned long long)__double_as_longlong(v), __ATOMIC_RELAXED,
+                     __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ __forceinline__ double load_agent(const double* p) {
-  return __longlong_as_double((long long)__hip_atomic_load(
-      reinterpret_cast<const unsigned long long*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+  using g64 = __attribute__((address_space(1))) unsigned long long;
+  return __longlong_as_double((long long)__hip_atomic_load((g64*)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
 }
 
 // LADDER: the launch may hold parallel tempering ladders (exchanges in the
@@ -475,7 +479,11 @@ k_sweep(const UnitDev* units, SweepArgs a) {
 
   const int64_t sweep_begin = U.progress[0];
   const int n_run = (int)min((int64_t)a.n_sweeps, U.progress[1]);
-  const double* sched = U.sched;
+  // Every global access inside the sweep loop goes through a global (not
+  // generic) pointer: one flat access in the loop makes the compiler's wait
+  // counts conservative (s_waitcnt vmcnt(0)), which waits for the graph row
+  // just prefetched for the next round.
+  gptr<double> sched = glob(U.sched);
   const int64_t sched_base = U.sched_base;
   const int n_classes = U.n_classes;
   // beta of the next sweep, loaded one sweep ahead
@@ -543,13 +551,20 @@ k_sweep(const UnitDev* units, SweepArgs a) {
     vpre[k * nthreads + tid] = e.var;
   };
 
+  // ladder fields of the descriptor, read once (uniform) before the sweeps
+  const int x_ladder = LADDER ? uni(U.ladder) : 0;
+  const int x_spr = LADDER ? uni(U.spr) : 1;
+  const int x_upl = LADDER ? uni(U.upl) : 1;
+  const int x_slot0 = LADDER ? uni(U.slot0) : 0;
+  double* const x_cbuf = LADDER ? uni_ptr(U.cbuf) : nullptr;
+  const uint32_t x_seed = uni(U.seed), x_stream = uni(U.stream);
   // parallel tempering: exchange step of the current sweep, sweeps run into
   // it and its energy buffer (step mod 3), advanced per sweep
   int64_t xstep = 0;
   int xphase = 0, xbuf = 0;
-  if (LADDER && U.ladder > 0) {
-    xstep = sweep_begin / U.spr;
-    xphase = (int)(sweep_begin - xstep * U.spr);
+  if (LADDER && x_ladder > 0) {
+    xstep = sweep_begin / x_spr;
+    xphase = (int)(sweep_begin - xstep * x_spr);
     xbuf = (int)(xstep % 3);
   }
 
@@ -684,17 +699,17 @@ k_sweep(const UnitDev* units, SweepArgs a) {
           // Fetches past the last round read sentinel slices and padding
           // (the tables are padded for it), so no fetch sits behind a branch
           // and the loads stay in flight while the other buffer is processed.
+          // The loop has one exit: an odd K's last pair fetches the next
+          // class's first row (round K) into `ra` and skips the second half.
           for (int k = 0; k < K; k += 2) {
             fetch(k + 1, slb, rb);
             process(k, sla, ra);
-            if (k + 1 >= K) break;
-            fetch(k + 2, sla, ra);
-            process(k + 1, slb, rb);
+            // the last pair of an odd K: `ra` takes the next class's first
+            // row (round K) instead of round K + 1
+            fetch(min(k + 2, K), sla, ra);
+            if (k + 1 < K) process(k + 1, slb, rb);
           }
-          // an even K leaves the next class's first row in `ra`; an odd K
-          // left it in `rb`, so fetch it again into `ra` (it stays in flight
-          // during the end of this class)
-          if (K & 1 || K == 0) fetch(K, sla, ra);
+          if (K == 0) fetch(0, sla, ra);
         }
 
         // ---- per replica: minimum of the cost along the class sequence ----
@@ -768,24 +783,32 @@ k_sweep(const UnitDev* units, SweepArgs a) {
         for (int b = 0; b < NB; ++b)
           if (b < nb_g && S.nb_c[8 * g + b] >= 0) newmask |= 1u << b;
         if (newmask) {
-          int c = 0;
-          for (int q = tp; q < N; q += TP) {
-            while (q >= class_off[c + 1]) ++c;
-            const int rho = q - class_off[c];
-            const int tq = rho & (TP - 1), kq = rho >> lg;
-            const uint32_t cur = sp[q * G + g];
-            const uint32_t f = fl[q * G + g];
-            uint32_t val = cur;
+          auto rebuild = [&](auto* out) {
+            int c = 0;
+            for (int q = tp; q < N; q += TP) {
+              while (q >= class_off[c + 1]) ++c;
+              const int rho = q - class_off[c];
+              const int tq = rho & (TP - 1), kq = rho >> lg;
+              const uint32_t cur = sp[q * G + g];
+              const uint32_t f = fl[q * G + g];
+              uint32_t val = cur;
 #pragma unroll
-            for (int b = 0; b < NB; ++b) {
-              if ((newmask >> b) & 1u) {
-                const int cs = S.nb_c[8 * g + b], ts = S.nb_tp[8 * g + b], ks = S.kstar[8 * g + b];
-                const bool after = c > cs || (c == cs && (tq > ts || (tq == ts && kq > ks)));
-                if (after) val ^= f & (1u << b);
+              for (int b = 0; b < NB; ++b) {
+                if ((newmask >> b) & 1u) {
+                  const int cs = S.nb_c[8 * g + b], ts = S.nb_tp[8 * g + b], ks = S.kstar[8 * g + b];
+                  const bool after = c > cs || (c == cs && (tq > ts || (tq == ts && kq > ks)));
+                  if (after) val ^= f & (1u << b);
+                }
               }
+              const uint32_t old = out[q * G + g];
+              out[q * G + g] = (uint8_t)((old & ~newmask) | (val & newmask));
             }
-            const uint32_t old = bst[q * G + g];
-            bst[q * G + g] = (uint8_t)((old & ~newmask) | (val & newmask));
+          };
+          // explicit address spaces: LDS or global stores, never flat ones
+          if (a.best_lds) {
+            rebuild((lds_u8*)bst);
+          } else {
+            rebuild((__attribute__((address_space(1))) uint8_t*)bst);
           }
         }
       }
@@ -798,21 +821,21 @@ k_sweep(const UnitDev* units, SweepArgs a) {
     // ---- parallel tempering exchange (parallel_tempering.h:126-189) ----
     // Every work-group of the ladder computes the same decisions from the
     // same energies, so labels stay identical without another barrier.
-    if (LADDER && U.ladder > 0 && xphase == U.spr - 1) {
+    if (LADDER && x_ladder > 0 && xphase == x_spr - 1) {
       const uint32_t step = (uint32_t)xstep;
-      const int L = U.ladder;
+      const int L = x_ladder;
       // this thread's pair and its uniform, drawn before the energies arrive
       const int i = (int)(step & 1u) + 2 * tid;
       double u = 0.0;
       if (i < L - 1) {
-        const uint4 w = philox4x32_10((uint32_t)i, step, 0u, DOM_EXCHANGE, U.seed, U.stream);
+        const uint4 w = philox4x32_10((uint32_t)i, step, 0u, DOM_EXCHANGE, x_seed, x_stream);
         u = canonical2(w.x, w.y);
       }
-      if (U.upl > 1) {
-        double* cb = U.cbuf + xbuf * L;
+      if (x_upl > 1) {
+        double* cb = x_cbuf + xbuf * L;
         if (tid < nrep) {
           wait_stores();  // this slot's reset (one step ago) is complete
-          store_agent(cb + U.slot0 + tid, S.rcost[tid]);
+          store_agent(cb + x_slot0 + tid, S.rcost[tid]);
         }
         // while the energies travel: the fields of class 0 of the next sweep
         // (its row pipeline moves on to class 1's first row)
@@ -821,11 +844,10 @@ k_sweep(const UnitDev* units, SweepArgs a) {
             for (int k = 0; k < K0; k += 2) {
               fetch_in(0, k + 1, slb, rb);
               field_early(k, sla, ra);
-              if (k + 1 >= K0) break;
-              fetch_in(0, k + 2, sla, ra);
-              field_early(k + 1, slb, rb);
+              fetch_in(0, min(k + 2, K0), sla, ra);
+              if (k + 1 < K0) field_early(k + 1, slb, rb);
             }
-            if (K0 & 1 || K0 == 0) fetch_in(0, K0, sla, ra);
+            if (K0 == 0) fetch_in(0, 0, sla, ra);
           }
           early_ready = true;
         }
@@ -838,7 +860,7 @@ k_sweep(const UnitDev* units, SweepArgs a) {
         if (S.abort_flag) break;
         if (tid < nrep) {
           const double nan = __longlong_as_double(0x7ff8000000000000ll);
-          store_agent(U.cbuf + (xbuf == 0 ? 2 : xbuf - 1) * L + U.slot0 + tid, nan);
+          store_agent(x_cbuf + (xbuf == 0 ? 2 : xbuf - 1) * L + x_slot0 + tid, nan);
         }
       } else if (tid < L) {
         S.lcost[tid] = S.rcost[tid];
@@ -856,9 +878,9 @@ k_sweep(const UnitDev* units, SweepArgs a) {
         }
       }
       lds_barrier();
-      if (tid < nrep) S.rbeta[tid] = S.lbeta[S.inv[U.slot0 + tid]];
+      if (tid < nrep) S.rbeta[tid] = S.lbeta[S.inv[x_slot0 + tid]];
     }
-    if (LADDER && U.ladder > 0 && ++xphase == U.spr) {
+    if (LADDER && x_ladder > 0 && ++xphase == x_spr) {
       xphase = 0;
       ++xstep;
       xbuf = xbuf == 2 ? 0 : xbuf + 1;
