@@ -186,7 +186,7 @@ static hipError_t upload_async(qmc_ctx c, void* dst, const void* src, size_t byt
 
 struct EllLayout {
   DevBuf<int32_t> slice_base;
-  DevBuf<int2> slice;
+  DevBuf<int4> slice;  // {entry offset, width, first position, lane split}
   DevBuf<uint2> ent;   // pairwise problems, 8-byte entries
   // pairwise problems with int8 coefficients: 4-byte entries {position |
   // coefficient << 24} (int8 set) instead of `ent`
@@ -680,8 +680,16 @@ static EllLayout* get_layout(qmc_problem p, int lanes, bool wide = false) {
     return u;
   };
   std::vector<int32_t> sbase(nclass + 1, 0);
-  std::vector<int2> slice;
+  std::vector<int4> slice;
   std::vector<uint2> ent;
+  // The sweep fetches at most one round past a class's end as slice
+  // sbase[c] + K_c: the table ends with copies of its first two slices (the
+  // next sweep's first rounds), or zero-width slices when it has fewer.
+  auto close_table = [&](int32_t n_entries) {
+    const size_t real = slice.size();
+    for (size_t i = 0; i < 2; ++i)
+      slice.push_back(i < real ? slice[i] : make_int4(n_entries, 0, p->N, 0));
+  };
   if (!p->pairwise) {
     // general terms with at most 3 other members: 16-byte entries
     if (p->max_others > 3) return nullptr;
@@ -690,13 +698,18 @@ static EllLayout* get_layout(qmc_problem p, int lanes, bool wide = false) {
     for (int c = 0; c < nclass; ++c) {
       const int32_t c0 = p->class_off[c], nc = p->class_off[c + 1] - c0;
       sbase[c] = (int32_t)slice.size();
+      // small classes of integer problems share a position among 2^split
+      // lanes (at most 8, while positions x lanes per position fit the lanes)
+      int32_t split = 0;
+      if (i32)
+        while (split < 3 && ((int64_t)nc << (split + 1)) <= lanes) ++split;
       for (int32_t k = 0; k * lanes < nc; ++k) {
         int32_t w = 0;
         for (int32_t t = 0; t < lanes && k * lanes + t < nc; ++t) {
           const int32_t q = c0 + k * lanes + t;
           w = std::max(w, p->h_inc_off[q + 1] - p->h_inc_off[q]);
         }
-        slice.push_back(make_int2((int32_t)ent4.size(), w));
+        slice.push_back(make_int4((int32_t)ent4.size(), w, c0 + k * lanes, split));
         for (int32_t e = 0; e < w; ++e)
           for (int32_t t = 0; t < lanes; ++t) {
             const int32_t rho = k * lanes + t;
@@ -717,10 +730,7 @@ static EllLayout* get_layout(qmc_problem p, int lanes, bool wide = false) {
       }
     }
     sbase[nclass] = (int32_t)slice.size();
-    // two sentinel slices: the sweep reads slice descriptors up to two
-    // rounds past a class end
-    slice.push_back(make_int2((int32_t)ent4.size(), 0));
-    slice.push_back(make_int2((int32_t)ent4.size(), 0));
+    close_table((int32_t)ent4.size());
     ent4.push_back(make_uint4(bits(0.0), ghost, ghost, ghost));
     const bool ok = L->slice_base.upload(sbase.data(), sbase.size()) == hipSuccess &&
                     L->slice.upload(slice.data(), slice.size()) == hipSuccess &&
@@ -749,7 +759,7 @@ static EllLayout* get_layout(qmc_problem p, int lanes, bool wide = false) {
         const int32_t q = c0 + k * lanes + t;
         w = std::max(w, p->h_inc_off[q + 1] - p->h_inc_off[q]);
       }
-      slice.push_back(make_int2(n_ent(), w));
+      slice.push_back(make_int4(n_ent(), w, c0 + k * lanes, 0));
       for (int32_t e = 0; e < w; ++e)
         for (int32_t t = 0; t < lanes; ++t) {
           const int32_t rho = k * lanes + t;
@@ -767,10 +777,9 @@ static EllLayout* get_layout(qmc_problem p, int lanes, bool wide = false) {
     }
   }
   sbase[nclass] = (int32_t)slice.size();
-  // two sentinel slices and 8 rows of tail padding: the sweep prefetches up
-  // to two rounds past a class end and always reads 8 rows of a slice
-  slice.push_back(make_int2(n_ent(), 0));
-  slice.push_back(make_int2(n_ent(), 0));
+  // the table's closing copies, and 8 rows of tail padding: the sweep
+  // always reads 8 rows of a slice
+  close_table(n_ent());
   for (int64_t i = 0; i < (int64_t)8 * lanes + 1; ++i) put(ghost, 0.0);
   bool ok = L->slice_base.upload(sbase.data(), sbase.size()) == hipSuccess &&
             L->slice.upload(slice.data(), slice.size()) == hipSuccess;

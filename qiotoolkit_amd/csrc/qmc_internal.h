@@ -33,11 +33,15 @@ struct ProblemDev {
 struct UnitDev {
   // problem: sliced ELL of the pairwise form for this unit's lane count --
   // slice (class c, round k) = ell_slice_base[c] + k; ell_slice[slice] =
-  // {offset, width}; entry e of lane t is ell_ent[offset + e*lanes + t] =
-  // {neighbour position, coefficient bits (int32 or fp32)}, padded with the
-  // ghost position and a zero coefficient
+  // {entry offset, width, first position (class start + k*lanes), lane
+  // split (log2 lanes per position)}; the table runs through the classes in
+  // order and ends with copies of its first two slices (round k past the
+  // last class's end is round k of the next sweep's first class); entry e
+  // of lane t is ell_ent[offset + e*lanes + t] = {neighbour position,
+  // coefficient bits (int32 or fp32)}, padded with the ghost position and a
+  // zero coefficient
   const int32_t* ell_slice_base;
-  const int2* ell_slice;
+  const int4* ell_slice;
   const uint2* ell_ent;
   // general problems whose terms have at most 4 variables: the same slices
   // over 16-byte entries {coefficient bits, the other members' positions,

@@ -191,7 +191,7 @@ template <class T> __device__ __forceinline__ T* uni_ptr(T* p) {
 // vector load whose wait (vmcnt counts in order) drains the graph row
 // prefetched for the next round.
 struct Rows {
-  cptr<int32_t> slice;   // {offset, width} per slice
+  cptr<int32_t> slice;   // {entry offset, width, first position, lane split} per slice
   gptr<uint64_t> ent;    // pairwise entries {position, coefficient bits}
   gptr<uint64_t> ent4;   // general 16-byte entries as two 64-bit words (or null)
   gptr<uint32_t> ent32;  // 4-byte pairwise entries {position | int8 coefficient << 24}
@@ -205,8 +205,8 @@ __device__ __forceinline__ Rows rows_of(const UnitDev& U) {
   return r;
 }
 
-__device__ __forceinline__ int2 load_slice(const Rows& R, int s) {
-  return make_int2(R.slice[2 * s], R.slice[2 * s + 1]);
+__device__ __forceinline__ int4 load_slice(const Rows& R, int s) {
+  return make_int4(R.slice[4 * s], R.slice[4 * s + 1], R.slice[4 * s + 2], R.slice[4 * s + 3]);
 }
 
 // slice = {offset, width} (uniform); lane tp of a slice with `lanes` = 1 << lg.
@@ -214,7 +214,7 @@ __device__ __forceinline__ int2 load_slice(const Rows& R, int s) {
 // across the processing of the previous row: entries past the slice width
 // belong to the next slice (or the tail padding) and are masked at use.
 template <int PAIR>
-__device__ __forceinline__ void ell_fetch(const Rows& R, int2 slice, int tp, int lg, EllRow<PAIR>& e) {
+__device__ __forceinline__ void ell_fetch(const Rows& R, int4 slice, int tp, int lg, EllRow<PAIR>& e) {
   const int step = 1 << lg;
   if constexpr (PAIR == 2) {
     gptr<uint32_t> p = R.ent32 + slice.x + tp;
@@ -235,6 +235,19 @@ __device__ __forceinline__ void ell_fetch(const Rows& R, int2 slice, int tp, int
 
 template <int KIND, typename CT, int NB>
 __device__ __forceinline__ void accumulate(uint32_t wv, CT cc, CT (&h)[8]) {
+  if constexpr (KIND == QMC_ISING && std::is_same<CT, float>::value) {
+    // -c is c with its sign bit flipped: move replica b's bit into bit 31
+    // and XOR it into the coefficient (the same value as `bit ? -c : c`)
+    const uint32_t cb = __float_as_uint(cc);
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      const uint32_t sh = wv << (31 - b);
+      // (sh & 0x80000000) ^ cb: v_bitop3_b32 truth table 0x6a
+      const uint32_t t = b == 0 ? (sh ^ cb) : __builtin_amdgcn_bitop3_b32(sh, 0x80000000u, cb, 0x6a);
+      h[b] += __uint_as_float(t);
+    }
+    return;
+  }
 #pragma unroll
   for (int b = 0; b < NB; ++b) {
     const bool bit = (wv >> b) & 1u;
@@ -251,7 +264,7 @@ __device__ __forceinline__ void accumulate(uint32_t wv, CT cc, CT (&h)[8]) {
 // prefetched ELL row (entries beyond the first 8 are loaded here).
 template <int KIND, int PAIR, typename CT, int NB>
 __device__ __forceinline__ void local_field(const UnitDev& U, const Rows& R, const lds_u8* sp, int q,
-                                            const EllRow<PAIR>& e, int2 slice, int tp, int lg, int G, int g,
+                                            const EllRow<PAIR>& e, int4 slice, int tp, int lg, int G, int g,
                                             CT (&h)[8], int sub = 0, int lgs = 0) {
 #pragma unroll
   for (int b = 0; b < NB; ++b) h[b] = CT(0);
@@ -495,7 +508,7 @@ k_sweep(const UnitDev* units, SweepArgs a) {
   // class is followed by class 0 of the next sweep), so the first row of a
   // class is already in flight while the previous class finishes.
   EllRow<PAIR> ra, rb;
-  int2 sla = make_int2(0, 0), slb = make_int2(0, 0);
+  int4 sla = make_int4(0, 0, 0, 0), slb = make_int4(0, 0, 0, 0);
   // Small classes of general problems with integer couplings share each
   // position among 2^lgs lanes (at most 8, while positions x lanes per
   // position fit the lanes): each sums every 2^lgs-th incident term and a
@@ -508,17 +521,23 @@ k_sweep(const UnitDev* units, SweepArgs a) {
       while (l < 3 && (ncls << (l + 1)) <= TP) ++l;
     return l;
   };
-  auto fetch_row = [&](int cc0, int csb, int k, int2& sl, EllRow<PAIR>& e, int lgs) {
-    const int rho = (k << lg) + (tp >> lgs);
-    if (PAIR) {
-      sl = load_slice(rows, csb + k);
-      ell_fetch(rows, sl, tp, lg, e);
-    } else if (gen_ell) {
-      sl = load_slice(rows, csb + k);
-    }
-    e.var = (uint32_t)var_of_pos[cc0 + rho];  // padded past position N - 1
+  // Slice tables run through the classes in order and end with copies of the
+  // first two slices, so round k past a class's end is slice
+  // slice_base[c] + k whichever class follows; each slice carries its first
+  // position and lane split, so the variable of this lane's position needs
+  // no class lookup.
+  auto fetch_slice = [&](int sidx, int4& sl, EllRow<PAIR>& e) {
+    sl = load_slice(rows, sidx);
+    if (PAIR) ell_fetch(rows, sl, tp, lg, e);
+    e.var = (uint32_t)var_of_pos[sl.z + (tp >> sl.w)];  // padded past position N - 1
   };
-  if (active) fetch_row(class_off[0], slices ? slice_base[0] : 0, 0, sla, ra, split_of(class_off[1] - class_off[0]));
+  // problems without slices (terms of more than 4 variables): position of
+  // round k of the class starting at cc0
+  auto fetch_csr = [&](int cc0, int k, EllRow<PAIR>& e) { e.var = (uint32_t)var_of_pos[cc0 + (k << lg) + tp]; };
+  if (active) {
+    if (slices) fetch_slice(slice_base[0], sla, ra);
+    else fetch_csr(class_off[0], 0, ra);
+  }
 
   // Split ladders: while the ladder's energies travel between work-groups
   // after a sweep, the local fields of the next sweep's colour class 0 are
@@ -533,15 +552,9 @@ k_sweep(const UnitDev* units, SweepArgs a) {
   const int K0 = (nc0 + TP - 1) >> lg;
   const bool early_ok = kEarly && U.ladder > 0 && U.upl > 1 && n_classes >= 2 && K0 <= a.pre_rounds;
   bool early_ready = false;
-  // round k of class cls, or round k - K(cls) of the class after it
-  auto fetch_in = [&](int cls, int k, int2& sl, EllRow<PAIR>& e) {
-    const int cc0 = class_off[cls];
-    const int Kc = (class_off[cls + 1] - cc0 + TP - 1) >> lg;
-    const int cn = cls + 1 < n_classes ? cls + 1 : 0;
-    const bool here = k < Kc;
-    fetch_row(here ? cc0 : class_off[cn], here ? slice_base[cls] : slice_base[cn], here ? k : k - Kc, sl, e, 0);
-  };
-  auto field_early = [&](int k, const int2& sl, const EllRow<PAIR>& e) {
+  // round k of class 0, or round k - K0 of class 1 (pairwise problems only)
+  auto fetch_in = [&](int k, int4& sl, EllRow<PAIR>& e) { fetch_slice(slice_base[0] + k, sl, e); };
+  auto field_early = [&](int k, const int4& sl, const EllRow<PAIR>& e) {
     const int rho = (k << lg) + tp;
     const int q = rho < nc0 ? class_off[0] + rho : N;
     CT h[8];
@@ -588,8 +601,7 @@ k_sweep(const UnitDev* units, SweepArgs a) {
       const int sbase = slices ? slice_base[c] : 0;
       const int cn = c + 1 < n_classes ? c + 1 : 0;  // the class after this one
       const int c0n = class_off[cn];
-      const int sbasen = slices ? slice_base[cn] : 0;
-      const int lgs = split_of(nc), lgsn = split_of(class_off[cn + 1] - c0n);
+      const int lgs = split_of(nc);
       // per replica: sum of the accepted dE of this lane's positions, the
       // minimum prefix of that sum and the round where it is first reached
       AT Ssum[8], Smin[8];
@@ -606,9 +618,13 @@ k_sweep(const UnitDev* units, SweepArgs a) {
         // buffer while round k is processed.
         // round k of this class, or round k - K of the next (no branch: the
         // loads stay in flight across the processing of the other buffer)
-        auto fetch = [&](int k, int2& sl, EllRow<PAIR>& e) {
-          const bool here = k < K;
-          fetch_row(here ? c0 : c0n, here ? sbase : sbasen, here ? k : k - K, sl, e, here ? lgs : lgsn);
+        auto fetch = [&](int k, int4& sl, EllRow<PAIR>& e) {
+          if (slices) {
+            fetch_slice(sbase + k, sl, e);
+          } else {
+            const bool here = k < K;
+            fetch_csr(here ? c0 : c0n, here ? k : k - K, e);
+          }
         };
         // the decision for the replicas of this lane at position q (round
         // k) from their local fields h
@@ -664,7 +680,7 @@ k_sweep(const UnitDev* units, SweepArgs a) {
           sp[lds_at(q, G, g)] = (uint8_t)(own ^ flips);
           fl[lds_at(q, G, g)] = (uint8_t)flips;
         };
-        auto process = [&](int k, const int2& sl, const EllRow<PAIR>& e) {
+        auto process = [&](int k, const int4& sl, const EllRow<PAIR>& e) {
           const int sub = tp & ((1 << lgs) - 1);
           const int rho = (k << lg) + (tp >> lgs);
           // The term lists of general problems end at N - 1; pairwise rows
@@ -842,12 +858,12 @@ k_sweep(const UnitDev* units, SweepArgs a) {
         if (early_ok && s + 1 < n_run) {
           if (active) {
             for (int k = 0; k < K0; k += 2) {
-              fetch_in(0, k + 1, slb, rb);
+              fetch_in(k + 1, slb, rb);
               field_early(k, sla, ra);
-              fetch_in(0, min(k + 2, K0), sla, ra);
+              fetch_in(min(k + 2, K0), sla, ra);
               if (k + 1 < K0) field_early(k + 1, slb, rb);
             }
-            if (K0 == 0) fetch_in(0, 0, sla, ra);
+            if (K0 == 0) fetch_in(0, sla, ra);
           }
           early_ready = true;
         }
