@@ -232,8 +232,16 @@ def run_c4(args):
         extra["positions_per_lane"] = ch["positions_per_lane"]
         if ch["sweeps_per_launch_fraction"] > 0:
             extra["sweeps_per_launch"] = max(1, int(round(ch["sweeps_per_launch_fraction"] * sweeps)))
-        log("c4 autotune %.1f s: %s" % (tune["seconds"], json.dumps(extra)))
+        log("c4 autotune %.1f s: %s" % (tune["seconds"], json.dumps(ch)))
         plist = [params(k) for k in range(len(models))]
+        if ch.get("large_min_vars", 0) > 0:
+            # the largest solves spread over more work-groups
+            for k, inst in enumerate(insts):
+                if inst[0] >= ch["large_min_vars"]:
+                    plist[k]["params"]["replicas_per_group"] = ch["large_replicas_per_group"]
+                    plist[k]["params"]["positions_per_lane"] = ch["large_positions_per_lane"]
+            for key in ("large_min_vars", "large_replicas_per_group", "large_positions_per_lane"):
+                extra[key] = ch[key]
     attempts = 8 * sweeps * sum(i[0] for i in insts)
     for _ in range(args.warmup):
         eng.solve_batch_arrays(target, models, plist, configs=False)
@@ -252,8 +260,8 @@ def run_c4(args):
     # plain form: the same solves, one instance per launch, on a sample --
     # with the batch's parameters, and with one replica per work-group (the
     # plain form's best shape; the grouping does not change results)
-    stride = max(1, len(models) // args.plain_sample)
-    idx = list(range(0, len(models), stride))
+    stride = max(1, len(models) // max(1, args.plain_sample))
+    idx = list(range(0, len(models), stride)) if args.plain_sample > 0 else []
     plain = {}
     same = True
     for label, over in (("batch_params", {}), ("one_replica_per_group", {"replicas_per_group": 1})):
@@ -266,8 +274,9 @@ def run_c4(args):
             plain_t += time.perf_counter() - t1
             plain_att += 8 * sweeps * insts[k][0]
             same &= bool(c1[0] == costs[k])
-        plain[label] = plain_att / plain_t
-    plain_best = max(plain.values())
+        if plain_t > 0:
+            plain[label] = plain_att / plain_t
+    plain_best = max(plain.values()) if plain else None
     for m in models:
         m.close()
     eng.close()
@@ -292,7 +301,7 @@ def run_c4(args):
         "roofline": roofline(attempts, B, avg_ms, "k_sweep<ISING,pairwise,i32> (all launches of a step)",
                              MEASURED_TRAFFIC_BYTES.get("c4") if default_shape else None),
         "plain_per_instance": {"value": plain_best, "by_shape": plain, "instances_sampled": len(idx),
-                               "speedup_of_batched": value / plain_best,
+                               "speedup_of_batched": value / plain_best if plain_best else None,
                                "identical_best_costs": same},
         "autotune": tune,
         "quality": {"best_cost_mean": float(np.mean(costs))},
@@ -413,17 +422,24 @@ def cpu_baseline_c2(inp, temps, args):
     variant, flags = _host_oracle()
     p = po.Problem("ising", inp["cost_function"]["terms"], variant=variant)
     tvals = [temps["initial"] * (temps["final"] / temps["initial"]) ** (i / 31) for i in range(32)]
-    # calibrate: a short run, then a sample of ~args.cpu_seconds
+    # calibrate on a short run, then whole ladders (restarts of the
+    # workload, 5000 sweeps each) until the sample reaches about
+    # args.cpu_seconds, at most the workload's 16
     o = po.make_opts(po.RNG_REFERENCE, seed=7, threads=threads, steps_per_tick=1)
     t0 = time.perf_counter()
     p.pt(o, tvals, False, 20)
     dt = max(1e-3, time.perf_counter() - t0)
-    sweeps = int(max(20, min(5000, 20 * args.cpu_seconds / dt)))
+    sweeps = 5000 if args.sweeps == 5000 else args.sweeps
+    per_ladder = dt * sweeps / 20
+    ladders = int(max(1, min(16, round(args.cpu_seconds / per_ladder))))
+    if per_ladder > 2 * args.cpu_seconds:  # a slow host: one partial ladder
+        sweeps, ladders = int(max(20, sweeps * args.cpu_seconds / per_ladder)), 1
     t0 = time.perf_counter()
-    p.pt(o, tvals, False, sweeps)
+    for k in range(ladders):
+        p.pt(po.make_opts(po.RNG_REFERENCE, seed=7 + k, threads=threads, steps_per_tick=1), tvals, False, sweeps)
     dt = time.perf_counter() - t0
-    attempts = 32 * 4096 * sweeps
-    # the same ladder on one thread, over a fifth of the sweeps (bounded)
+    attempts = ladders * 32 * 4096 * sweeps
+    # one ladder on one thread, over a fifth of the sweeps (bounded)
     o1 = po.make_opts(po.RNG_REFERENCE, seed=7, threads=1, steps_per_tick=1)
     sweeps1 = max(20, sweeps // 5)
     t1 = time.perf_counter()
@@ -436,9 +452,9 @@ def cpu_baseline_c2(inp, temps, args):
             "thread_scaling": (attempts / dt) / single,
             "cpu_model": _cpu_model(),
             "build": "oracle/liboracle%s.so built on this host: %s" % (variant, flags),
-            "sample": "one 32-replica ladder of the same workload, %d of 5000 sweeps, %.1f s, "
-                      "pcg32 + ln-table accept, OpenMP over replicas on %d threads; single_thread: %d sweeps "
-                      "on 1 thread, %.1f s" % (sweeps, dt, threads, sweeps1, dt1)}
+            "sample": "%d of the 16 restarts of the same workload (32-replica ladders, %d sweeps each), %.1f s, "
+                      "pcg32 + ln-table accept, OpenMP over replicas on %d threads; single_thread: one ladder, %d "
+                      "sweeps on 1 thread, %.1f s" % (ladders, sweeps, dt, threads, sweeps1, dt1)}
 
 
 def main():
@@ -463,7 +479,7 @@ def main():
     ap.add_argument("--pa-group", type=int, default=0, help="c5: replicas per work-group of PA requests (0 = solver default)")
     ap.add_argument("--pa-order", default="", help="c5: sweep_order of PA requests (default: the solver's, index)")
     ap.add_argument("--plain-sample", type=int, default=128,
-                    help="c4: instances run one per launch for the plain-form rate")
+                    help="c4: instances run one per launch for the plain-form rate (0: none)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",

@@ -76,15 +76,21 @@ int qiotk_solve_batch_arrays(qiotk_engine e, const char* target, int32_t n, cons
  * schedule compressed accordingly) of candidate launch shapes -- replicas per
  * work-group x lane depth (positions each lane covers per colour class,
  * which sets the LDS tile of a class round) -- on the solves (every k-th one
- * past 8192), timed with HIP events; then, for the fastest shape, one
+ * past 8192), timed with HIP events; then, for the fastest shape, another
+ * shape for the largest solves (those of at least the sample's 75th, 90th
+ * or 97th percentile size): 1/2 or 1/4 of the replicas per work-group (more
+ * work-groups) or lane depth 0 or 2 (wider work-groups); then one
  * persistent launch against graph-replayed launches of 1/4 and 1/2 of the
- * trial sweeps.  Returned as {"params": {"replicas_per_group": B,
- * "positions_per_lane": k, "sweeps_per_launch": s (0 = persistent),
+ * trial sweeps.  Returned as
+ * {"params": {"replicas_per_group": B, "positions_per_lane": k,
+ * "large_min_vars": m (0 = none), "large_replicas_per_group": Bl,
+ * "large_positions_per_lane": kl, "sweeps_per_launch": s (0 = persistent),
  * "sweeps_per_launch_fraction": s / trial_sweeps}, "trials": [...]}; merge
- * the first two into every solve's params and scale the fraction by its
- * sweeps.  replicas_per_group and sweeps_per_launch never change results (the
- * RNG counters see neither); positions_per_lane changes the colour-class
- * visit order. */
+ * replicas_per_group and positions_per_lane (Bl and kl for solves of at
+ * least m variables) into every solve's params and scale the fraction by
+ * its sweeps.  replicas_per_group and sweeps_per_launch
+ * never change results (the RNG counters see neither); positions_per_lane
+ * changes the colour-class visit order. */
 int qiotk_tune_batch(qiotk_engine e, const char* target, int32_t n, const qiotk_model* models,
                      const char* const* params_json, int32_t trial_sweeps, char** response);
 /* Variable count and the original variable names in internal-id order

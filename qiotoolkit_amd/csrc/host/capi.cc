@@ -480,24 +480,20 @@ int qiotk_tune_batch(qiotk_engine e, const char* target, int32_t n, const qiotk_
       s->configure(doc);
       min_group = std::min<int64_t>(min_group, s->run_replicas());
     }
-    // candidates: replicas per work-group x lane depth (0 = library default)
-    std::vector<std::pair<int, int>> cands;
-    for (int B : {1, 2, 4, 8, 16, 32}) {
-      if (B > min_group) break;
-      if (pt && min_group % B) continue;
-      for (int depth : {0, 2, 8}) cands.emplace_back(B, depth);
-    }
+    // one trial of the sample: replicas per work-group B (B_large for solves
+    // of at least large_min variables), lane depth, sweeps per launch
     Json trials = Json::array();
-    double best_rate = -1;
-    int best_B = 0, best_depth = 0;
-    for (auto [B, depth] : cands) {
+    auto trial = [&](int B, int depth, int64_t large_min, int B_large, int depth_large, int64_t spl) {
       std::vector<std::string> ps;
-      for (Json doc : base) {
+      for (size_t i = 0; i < base.size(); ++i) {
+        Json doc = base[i];
         Json& p = doc["params"];
         p["sweeps"] = Json((int64_t)trial_sweeps);
         p.remove("step_limit");
-        p["replicas_per_group"] = Json((int64_t)B);
-        p["positions_per_lane"] = Json((int64_t)depth);
+        const bool large = large_min > 0 && (int64_t)ms[i]->model.n_vars >= large_min;
+        p["replicas_per_group"] = Json((int64_t)(large ? B_large : B));
+        p["positions_per_lane"] = Json((int64_t)(large ? depth_large : depth));
+        if (spl) p["sweeps_per_launch"] = Json(spl);
         ps.push_back(doc.dump());
       }
       std::vector<const char*> pp;
@@ -510,44 +506,73 @@ int qiotk_tune_batch(qiotk_engine e, const char* target, int32_t n, const qiotk_
       Json t = Json::object();
       t["replicas_per_group"] = Json((int64_t)B);
       t["positions_per_lane"] = Json((int64_t)depth);
+      if (large_min > 0) {
+        t["large_min_vars"] = Json(large_min);
+        t["large_replicas_per_group"] = Json((int64_t)B_large);
+        t["large_positions_per_lane"] = Json((int64_t)depth_large);
+      }
+      if (spl) t["sweeps_per_launch"] = Json(spl);
       t["device_ms"] = br.device_ms;
       t["attempts_per_s"] = rate;
       trials.push_back(t);
-      if (rate > best_rate) {
-        best_rate = rate;
-        best_B = B;
-        best_depth = depth;
+      return rate;
+    };
+    // 1. one shape for every solve: replicas per work-group x lane depth (0 =
+    //    library default)
+    double best_rate = -1;
+    int best_B = 0, best_depth = 0;
+    for (int B : {1, 2, 4, 8, 16, 32}) {
+      if (B > min_group) break;
+      if (pt && min_group % B) continue;
+      for (int depth : {0, 2, 8}) {
+        const double rate = trial(B, depth, 0, 0, 0, 0);
+        if (rate > best_rate) {
+          best_rate = rate;
+          best_B = B;
+          best_depth = depth;
+        }
       }
     }
-    // sweeps per launch: one persistent launch (0) against graph-replayed
-    // chunks, for the chosen shape (split PT ladders always run persistent)
+    // 2. the largest solves in another shape: a batch's largest solves run
+    //    longest, and with many replicas per work-group they finish as one
+    //    work-group per solve on a mostly idle device -- fewer replicas per
+    //    work-group (more work-groups) or a shallower lane depth (wider
+    //    work-groups) for the solves of at least the sample's 75th, 90th or
+    //    97th percentile size
+    int64_t best_large = 0;
+    int best_B_large = 0, best_depth_large = 0;
+    {
+      std::vector<int64_t> sizes;
+      for (auto m : ms) sizes.push_back((int64_t)m->model.n_vars);
+      std::sort(sizes.begin(), sizes.end());
+      std::vector<int64_t> thresholds;
+      for (double q : {0.75, 0.9, 0.97}) {
+        const size_t at = std::min(sizes.size() - 1, (size_t)(q * (double)sizes.size()));
+        const int64_t thr = sizes[at];
+        if (thr > sizes.front() && (thresholds.empty() || thresholds.back() != thr)) thresholds.push_back(thr);
+      }
+      std::vector<std::pair<int, int>> shapes;  // (replicas per work-group, lane depth)
+      for (int Bl = best_B / 2; Bl >= 1 && Bl >= best_B / 4; Bl /= 2)
+        if (!pt || min_group % Bl == 0) shapes.emplace_back(Bl, best_depth);
+      for (int dl : {0, 2})
+        if (dl != best_depth) shapes.emplace_back(best_B, dl);
+      for (int64_t thr : thresholds)
+        for (auto [Bl, dl] : shapes) {
+          const double rate = trial(best_B, best_depth, thr, Bl, dl, 0);
+          if (rate > best_rate) {
+            best_rate = rate;
+            best_large = thr;
+            best_B_large = Bl;
+            best_depth_large = dl;
+          }
+        }
+    }
+    // 3. sweeps per launch: one persistent launch (0) against graph-replayed
+    //    chunks, for the chosen shape (split PT ladders always run persistent)
     int64_t best_spl = 0;
     if (!pt || best_B == min_group) {
       for (int64_t spl : {(int64_t)std::max(1, trial_sweeps / 4), (int64_t)std::max(1, trial_sweeps / 2)}) {
-        std::vector<std::string> ps;
-        for (Json doc : base) {
-          Json& p = doc["params"];
-          p["sweeps"] = Json((int64_t)trial_sweeps);
-          p.remove("step_limit");
-          p["replicas_per_group"] = Json((int64_t)best_B);
-          p["positions_per_lane"] = Json((int64_t)best_depth);
-          p["sweeps_per_launch"] = Json(spl);
-          ps.push_back(doc.dump());
-        }
-        std::vector<const char*> pp;
-        for (auto& x : ps) pp.push_back(x.c_str());
-        BatchRun br;
-        run_batch(e, target, (int32_t)ms.size(), ms.data(), pp.data(), br);
-        int64_t att = 0;
-        for (auto& s : br.solvers) att += s->attempts();
-        const double rate = br.device_ms > 0 ? (double)att / (br.device_ms * 1e-3) : 0.0;
-        Json t = Json::object();
-        t["replicas_per_group"] = Json((int64_t)best_B);
-        t["positions_per_lane"] = Json((int64_t)best_depth);
-        t["sweeps_per_launch"] = Json(spl);
-        t["device_ms"] = br.device_ms;
-        t["attempts_per_s"] = rate;
-        trials.push_back(t);
+        const double rate = trial(best_B, best_depth, best_large, best_B_large, best_depth_large, spl);
         if (rate > best_rate) {
           best_rate = rate;
           best_spl = spl;
@@ -561,6 +586,11 @@ int qiotk_tune_batch(qiotk_engine e, const char* target, int32_t n, const qiotk_
     // a chunk length chosen on a short trial scales with the full run
     chosen["sweeps_per_launch"] = Json(best_spl);
     chosen["sweeps_per_launch_fraction"] = best_spl ? (double)best_spl / trial_sweeps : 0.0;
+    // solves of at least large_min_vars variables take
+    // large_replicas_per_group instead (0: one shape for all)
+    chosen["large_min_vars"] = Json(best_large);
+    chosen["large_replicas_per_group"] = Json((int64_t)best_B_large);
+    chosen["large_positions_per_lane"] = Json((int64_t)best_depth_large);
     r["params"] = chosen;
     r["sample"] = Json((int64_t)idx.size());
     r["trial_sweeps"] = Json((int64_t)trial_sweeps);

@@ -76,8 +76,13 @@ def test_uneven_batch_full_size(engine):
     extra = {"replicas_per_group": ch["replicas_per_group"], "positions_per_lane": ch["positions_per_lane"]}
     if ch["sweeps_per_launch_fraction"] > 0:
         extra["sweeps_per_launch"] = max(1, int(round(ch["sweeps_per_launch_fraction"] * 1000)))
-    print("tuned shape %s" % json.dumps(extra), flush=True)
+    print("tuned shape %s" % json.dumps(ch), flush=True)
     plist = [params(k, extra) for k in range(len(models))]
+    if ch.get("large_min_vars", 0) > 0:  # the largest solves in smaller work-groups
+        for k, inst in enumerate(insts):
+            if inst[0] >= ch["large_min_vars"]:
+                plist[k]["params"]["replicas_per_group"] = ch["large_replicas_per_group"]
+                plist[k]["params"]["positions_per_lane"] = ch["large_positions_per_lane"]
     cost, cfg, tm = engine.solve_batch_arrays(SA, models, plist, configs=True)
     print("batch solved: device %.1f ms" % tm["device_ms"], flush=True)
     # every returned configuration under the oracle's cost function
