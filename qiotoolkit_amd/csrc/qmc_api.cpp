@@ -11,6 +11,7 @@
 #include <limits>
 #include <map>
 #include <memory>
+#include <unordered_set>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -123,18 +124,27 @@ static bool int8_entries_fit(qmc_problem p);
 struct qmc_ctx_s {
   std::atomic<int> refs{1};
   int device = 0;
-  hipStream_t stream = nullptr;
+  // The context's stream.  Every use goes through stream_of() or
+  // sync_stream(), which first enqueue the uploads staged since the last use.
+  hipStream_t raw_stream = nullptr;
   DevBuf<double> lut;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   // batches fan their launch groups out over the main stream and these
   // (created on first use, default priority), joined back by events
   hipStream_t aux[kAuxStreams] = {};
   hipEvent_t fork_ev = nullptr, join_ev[kAuxStreams] = {};
-  // pinned staging ring for small host-to-device uploads on `stream`: setting
-  // up a solve enqueues its copies instead of waiting for each one
+  // pinned staging ring for small host-to-device uploads: they are copied
+  // into the ring and recorded, and the next use of the stream brings all of
+  // them to the device with one copy of the ring's span (into `dstage`, its
+  // device mirror) and one scatter launch, instead of one copy per upload
   std::mutex stage_mu;
   uint8_t* stage = nullptr;
   size_t stage_off = 0;
+  DevBuf<uint8_t> dstage;
+  std::vector<ScatterRec> pending;  // staged, not yet enqueued (src = ring offset)
+  std::unordered_set<const void*> pending_dst;
+  size_t pend_begin = 0;            // ring offset of the first pending byte
+  hipError_t sticky = hipSuccess;   // a failed flush, reported by sync_stream()
   // beta schedules on the device, shared by walker sets that plan the same
   // values (many solves of one schedule upload it once)
   struct SchedEntry {
@@ -154,10 +164,70 @@ constexpr size_t kSchedCacheMaxValues = 1 << 17;
 
 constexpr size_t kStageBytes = 32u << 20;
 
+// Enqueues the staged uploads on the stream (caller holds stage_mu): one
+// copy of the ring's span and one scatter launch, or direct copies for one
+// or two uploads (or when the copy records do not fit behind the data).
+static hipError_t flush_locked(qmc_ctx c) {
+  if (c->pending.empty()) return hipSuccess;
+  hipError_t e = hipSuccess;
+  const size_t n = c->pending.size();
+  const size_t at = (c->stage_off + 15) & ~(size_t)15;
+  const size_t rec_bytes = n * sizeof(ScatterRec);
+  if (n <= 2 || at + rec_bytes > kStageBytes) {
+    for (const ScatterRec& r : c->pending) {
+      e = hipMemcpyAsync((void*)r.dst, c->stage + r.src, r.bytes, hipMemcpyHostToDevice, c->raw_stream);
+      if (e != hipSuccess) break;
+    }
+  } else {
+    std::memcpy(c->stage + at, c->pending.data(), rec_bytes);
+    c->stage_off = at + rec_bytes;
+    e = hipMemcpyAsync(c->dstage.p + c->pend_begin, c->stage + c->pend_begin, c->stage_off - c->pend_begin,
+                       hipMemcpyHostToDevice, c->raw_stream);
+    if (e == hipSuccess)
+      e = launch_scatter(c->dstage.p, reinterpret_cast<const ScatterRec*>(c->dstage.p + at), (int)n,
+                         c->raw_stream);
+  }
+  c->pending.clear();
+  c->pending_dst.clear();
+  c->pend_begin = c->stage_off;
+  return e;
+}
+
+// The context's stream, after enqueueing the staged uploads (a failed
+// flush is kept and reported by the next sync_stream()).
+static hipStream_t stream_of(qmc_ctx c) {
+  std::lock_guard<std::mutex> lk(c->stage_mu);
+  const hipError_t e = flush_locked(c);
+  if (e != hipSuccess && c->sticky == hipSuccess) c->sticky = e;
+  return c->raw_stream;
+}
+
+// A context's stream for a function body: each use converts through
+// stream_of(), so work enqueued after an upload always follows its flush.
+struct CtxStream {
+  qmc_ctx c;
+  operator hipStream_t() const { return stream_of(c); }
+};
+
+// Launch helpers that take a raw stream check that nothing is staged (a
+// launch before the flush would read device memory the staged uploads have
+// not written yet).
+static bool uploads_pending(qmc_ctx c) {
+  std::lock_guard<std::mutex> lk(c->stage_mu);
+  return !c->pending.empty();
+}
+
+static hipError_t sync_stream(qmc_ctx c) {
+  const const CtxStream st{c};
+  if (c->sticky != hipSuccess) return c->sticky;
+  return hipStreamSynchronize(st);
+}
+
 // Copies `bytes` from host `src` to device `dst` in order on the context's
 // stream.  `src` may be reused as soon as this returns: small uploads are
-// copied into the pinned ring first (the ring wraps after draining the
-// stream), large ones wait for their own copy.
+// copied into the pinned ring and enqueued at the next use of the stream
+// (the ring wraps after draining the stream); large ones wait for their own
+// copy.
 static hipError_t upload_async(qmc_ctx c, void* dst, const void* src, size_t bytes) {
   if (bytes == 0) return hipSuccess;
   std::lock_guard<std::mutex> lk(c->stage_mu);
@@ -167,21 +237,36 @@ static hipError_t upload_async(qmc_ctx c, void* dst, const void* src, size_t byt
       c->stage = nullptr;
       return e;
     }
+    e = c->dstage.alloc(kStageBytes);
+    if (e != hipSuccess) {
+      (void)hipHostFree(c->stage);
+      c->stage = nullptr;
+      return e;
+    }
   }
+  hipError_t e = hipSuccess;
   if (bytes > kStageBytes / 4) {
-    hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream);
-    return e != hipSuccess ? e : hipStreamSynchronize(c->stream);
+    if ((e = flush_locked(c)) != hipSuccess) return e;
+    e = hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->raw_stream);
+    return e != hipSuccess ? e : hipStreamSynchronize(c->raw_stream);
   }
-  size_t at = (c->stage_off + 255) & ~(size_t)255;
+  // a second upload to the same place waits for the first to be enqueued
+  // (the scatter applies one batch's copies in no particular order)
+  if (c->pending_dst.count(dst) && (e = flush_locked(c)) != hipSuccess) return e;
+  size_t at = (c->stage_off + 15) & ~(size_t)15;
   if (at + bytes > kStageBytes) {
-    // every earlier copy out of the ring was enqueued on `stream`
-    hipError_t e = hipStreamSynchronize(c->stream);
-    if (e != hipSuccess) return e;
+    // every earlier copy out of the ring is enqueued on the stream
+    if ((e = flush_locked(c)) != hipSuccess) return e;
+    if ((e = hipStreamSynchronize(c->raw_stream)) != hipSuccess) return e;
     at = 0;
+    c->pend_begin = 0;
   }
+  if (c->pending.empty()) c->pend_begin = at;
   std::memcpy(c->stage + at, src, bytes);
   c->stage_off = at + bytes;
-  return hipMemcpyAsync(dst, c->stage + at, bytes, hipMemcpyHostToDevice, c->stream);
+  c->pending.push_back(ScatterRec{(uint64_t)at, (uint64_t)(uintptr_t)dst, (uint64_t)bytes});
+  c->pending_dst.insert(dst);
+  return hipSuccess;
 }
 
 struct EllLayout {
@@ -401,7 +486,7 @@ int qmc_ctx_create(int device, qmc_ctx* out) {
   }
   auto* c = new qmc_ctx_s();
   c->device = device;
-  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+  if (hipStreamCreateWithFlags(&c->raw_stream, hipStreamNonBlocking) != hipSuccess) {
     delete c;
     return fail(QMC_DEVICE_ERROR, "hipStreamCreate failed");
   }
@@ -422,8 +507,9 @@ static void ctx_release(qmc_ctx c) {
   if (!c || --c->refs > 0) return;
   if (g_exiting.load()) return;  // the process is ending: nothing left to free
   (void)hipSetDevice(c->device);
-  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  if (c->raw_stream) (void)hipStreamSynchronize(c->raw_stream);
   c->lut.release();
+  c->dstage.release();
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
   if (c->fork_ev) (void)hipEventDestroy(c->fork_ev);
@@ -432,7 +518,7 @@ static void ctx_release(qmc_ctx c) {
     if (c->join_ev[i]) (void)hipEventDestroy(c->join_ev[i]);
     if (c->aux[i]) (void)hipStreamDestroy(c->aux[i]);
   }
-  if (c->stream) (void)hipStreamDestroy(c->stream);
+  if (c->raw_stream) (void)hipStreamDestroy(c->raw_stream);
   if (c->stage) (void)hipHostFree(c->stage);
   delete c;
 }
@@ -441,7 +527,7 @@ void qmc_ctx_destroy(qmc_ctx c) { ctx_release(c); }
 
 int qmc_sync(qmc_ctx c) {
   if (!c) return fail(QMC_INVALID_ARGUMENT, "null context");
-  HIP_TRY(hipStreamSynchronize(c->stream));
+  HIP_TRY(sync_stream(c));
   return QMC_OK;
 }
 
@@ -970,7 +1056,7 @@ static int ensure_cleared(qmc_walkers w) {
   zero(w->label.p, w->cap * sizeof(int32_t));
   zero(w->err.p, sizeof(int32_t));
   zero(w->progress.p, w->progress.n * sizeof(int64_t));
-  HIP_TRY(launch_zero(z, nz, w->prob->ctx->stream));
+  HIP_TRY(launch_zero(z, nz, stream_of(w->prob->ctx)));
   w->pend_zero = false;
   return QMC_OK;
 }
@@ -1002,7 +1088,7 @@ static int setup_done(qmc_walkers w) {
   w->pend_prog = false;
   if (w->pa_on && w->pa_args.cost0)
     HIP_TRY(hipMemcpyAsync(w->pa_args.cost0, w->cost.p, (size_t)w->cap * sizeof(double),
-                           hipMemcpyDeviceToDevice, w->prob->ctx->stream));
+                           hipMemcpyDeviceToDevice, stream_of(w->prob->ctx)));
   return QMC_OK;
 }
 
@@ -1010,11 +1096,11 @@ static int setup_done(qmc_walkers w) {
 static int launch_setups(qmc_ctx c, const std::vector<SetupDesc>& d, int max_stride) {
   if (d.empty()) return QMC_OK;
   if (c->setup_descs.n < d.size()) {
-    HIP_TRY(hipStreamSynchronize(c->stream));  // a launch may still read the old array
+    HIP_TRY(sync_stream(c));  // a launch may still read the old array
     HIP_TRY(c->setup_descs.alloc(d.size()));
   }
   HIP_TRY(upload_async(c, c->setup_descs.p, d.data(), d.size() * sizeof(SetupDesc)));
-  HIP_TRY(launch_setup(c->setup_descs.p, (int)d.size(), max_stride, c->stream));
+  HIP_TRY(launch_setup(c->setup_descs.p, (int)d.size(), max_stride, stream_of(c)));
   return QMC_OK;
 }
 
@@ -1026,7 +1112,7 @@ static int ensure_scratch(qmc_walkers w);
 static int materialize(qmc_walkers w) {
   if (!w->pend_init) return ensure_cleared(w);
   qmc_problem p = w->prob;
-  hipStream_t st = p->ctx->stream;
+  const CtxStream st{p->ctx};
   const UnitDev* du = nullptr;
   int rc = device_units(w, &du);
   if (rc) return rc;
@@ -1057,7 +1143,7 @@ static void drop_graph(qmc_walkers w) {
 static void walkers_release(qmc_walkers w) {
   if (!w || --w->refs > 0) return;
   qmc_problem p = w->prob;
-  (void)hipStreamSynchronize(p->ctx->stream);
+  (void)sync_stream(p->ctx);
   drop_graph(w);
   delete w;
   problem_release(p);
@@ -1081,7 +1167,7 @@ int qmc_walkers_resample(qmc_walkers w, const int32_t* src, int32_t new_R) {
   for (int32_t r = 0; r < new_R; ++r)
     if (src[r] < 0 || src[r] >= w->R) return fail(QMC_INVALID_ARGUMENT, "resample source out of range");
   qmc_problem p = w->prob;
-  hipStream_t st = p->ctx->stream;
+  const CtxStream st{p->ctx};
   int rc = ensure_scratch(w);
   if (rc) return rc;
   const size_t n = (size_t)w->R * p->N;
@@ -1208,7 +1294,7 @@ static int pa_keep_costs_before(qmc_walkers w) {
   if (!w->pa_cost0.p) HIP_TRY(w->pa_cost0.alloc(w->cap));
   if (!w->pend_init)
     HIP_TRY(hipMemcpyAsync(w->pa_cost0.p, w->cost.p, (size_t)w->cap * sizeof(double), hipMemcpyDeviceToDevice,
-                           w->prob->ctx->stream));
+                           stream_of(w->prob->ctx)));
   w->pa_args.cost0 = w->pa_cost0.p;
   return QMC_OK;
 }
@@ -1223,7 +1309,7 @@ int qmc_walkers_pa_observe(qmc_walkers w, int32_t max_steps) {
   ZeroSpans z{};
   z.p[0] = w->pa_log.p;
   z.n[0] = (size_t)max_steps * kPaLogFields * sizeof(double);
-  HIP_TRY(launch_zero(z, 1, w->prob->ctx->stream));
+  HIP_TRY(launch_zero(z, 1, stream_of(w->prob->ctx)));
   w->pa_args.log = w->pa_log.p;
   w->pa_args.log_steps = max_steps;
   return QMC_OK;
@@ -1233,7 +1319,7 @@ int qmc_walkers_pa_log(qmc_walkers w, int32_t n_steps, double* records) {
   if (!w || (!records && n_steps > 0)) return fail(QMC_INVALID_ARGUMENT, "null argument");
   if (!w->pa_log.p) return fail(QMC_INVALID_ARGUMENT, "observation was not started");
   if (n_steps < 0 || n_steps > w->pa_args.log_steps) return fail(QMC_INVALID_ARGUMENT, "n_steps out of range");
-  HIP_TRY(hipStreamSynchronize(w->prob->ctx->stream));
+  HIP_TRY(sync_stream(w->prob->ctx));
   if (n_steps > 0)
     HIP_TRY(hipMemcpy(records, w->pa_log.p, (size_t)n_steps * kPaLogFields * sizeof(double), hipMemcpyDeviceToHost));
   return QMC_OK;
@@ -1275,7 +1361,7 @@ int qmc_walkers_pa_step(qmc_walkers w) {
   if (rc) return rc;
   w->pa_args.units = du;
   w->pa_args.n_units = w->n_units;
-  HIP_TRY(launch_pa_step(w->pa_args, w->prob->dev(), w->prob->ctx->stream));
+  HIP_TRY(launch_pa_step(w->pa_args, w->prob->dev(), stream_of(w->prob->ctx)));
   w->pa_stepped = true;
   return QMC_OK;
 }
@@ -1283,7 +1369,7 @@ int qmc_walkers_pa_step(qmc_walkers w) {
 int qmc_walkers_pa_info(qmc_walkers w, qmc_pa_info* info, uint8_t* best_state) {
   if (!w || !info) return fail(QMC_INVALID_ARGUMENT, "null argument");
   if (!w->pa.p) return fail(QMC_INVALID_ARGUMENT, "population mode was never started");
-  hipStream_t st = w->prob->ctx->stream;
+  const CtxStream st{w->prob->ctx};
   HIP_TRY(hipStreamSynchronize(st));
   PaDev d{};
   HIP_TRY(hipMemcpy(&d, w->pa.p, sizeof d, hipMemcpyDeviceToHost));
@@ -1307,7 +1393,7 @@ int qmc_walkers_pa_info(qmc_walkers w, qmc_pa_info* info, uint8_t* best_state) {
 int qmc_walkers_pa_end(qmc_walkers w) {
   if (!w) return fail(QMC_INVALID_ARGUMENT, "null handle");
   if (!w->pa_on) return QMC_OK;
-  HIP_TRY(hipStreamSynchronize(w->prob->ctx->stream));
+  HIP_TRY(sync_stream(w->prob->ctx));
   int32_t R = 0;
   HIP_TRY(hipMemcpy(&R, &w->pa.p->R, sizeof R, hipMemcpyDeviceToHost));
   if (R < 1 || R > w->cap) return fail(QMC_RUNTIME_ERROR, "device population size out of range");
@@ -1329,7 +1415,7 @@ int qmc_walkers_set_states(qmc_walkers w, const uint8_t* states) {
   if (!w || !states) return fail(QMC_INVALID_ARGUMENT, "null argument");
   if (int rc0 = materialize(w)) return rc0;
   qmc_problem p = w->prob;
-  hipStream_t st = p->ctx->stream;
+  const CtxStream st{p->ctx};
   const UnitDev* du = nullptr;
   int rc = device_units(w, &du);
   if (rc || (rc = ensure_scratch(w))) return rc;
@@ -1378,6 +1464,7 @@ static int early_rounds(qmc_walkers w) {
 static int launch_units(qmc_walkers w, const UnitDev* d_units, int block, int32_t max_sweeps, int ordered,
                         hipStream_t st, int coop, int ladder, int pre_rounds = 0) {
   const qmc_problem p = w->prob;
+  if (uploads_pending(p->ctx)) return fail(QMC_DEVICE_ERROR, "internal: sweep launch before its uploads");
   const int nb = w->nb, n_units = w->n_units;
   SweepArgs a{};
   a.n_sweeps = max_sweeps;
@@ -1518,12 +1605,12 @@ int qmc_walkers_sweep(qmc_walkers w, int64_t sweep0, int32_t n_sweeps, int beta_
   qmc_ctx c = w->prob->ctx;
   const UnitDev* du = nullptr;
   if ((rc = device_units(w, &du))) return rc;
-  HIP_TRY(hipEventRecord(c->ev0, c->stream));
+  HIP_TRY(hipEventRecord(c->ev0, stream_of(c)));
   ordered = ordered || w->ordered;
-  rc = launch_units(w, du, ordered ? 64 : w->lanes * w->G, n_sweeps, ordered, c->stream, 0, 0);
+  rc = launch_units(w, du, ordered ? 64 : w->lanes * w->G, n_sweeps, ordered, stream_of(c), 0, 0);
   if (rc) return rc;
-  HIP_TRY(hipEventRecord(c->ev1, c->stream));
-  HIP_TRY(hipStreamSynchronize(c->stream));
+  HIP_TRY(hipEventRecord(c->ev1, stream_of(c)));
+  HIP_TRY(sync_stream(c));
   HIP_TRY(hipEventElapsedTime(&w->last_ms, c->ev0, c->ev1));
   return QMC_OK;
 }
@@ -1535,7 +1622,7 @@ int qmc_walkers_pt(qmc_walkers w, int32_t ladder, const double* ladder_betas, in
   if (sweeps_per_replica < 1) return fail(QMC_INVALID_ARGUMENT, "sweeps_per_replica must be >= 1");
   qmc_problem p = w->prob;
   qmc_ctx c = p->ctx;
-  hipStream_t st = c->stream;
+  const CtxStream st{c};
   const int n_ladders = w->R / std::max(1, ladder);
   int rc;
   SweepArgs sa{};
@@ -1612,7 +1699,7 @@ int qmc_walkers_remaining(qmc_walkers w, int64_t* sweeps_left) {
   if (!w || !sweeps_left) return fail(QMC_INVALID_ARGUMENT, "null argument");
   if (int rc0 = materialize(w)) return rc0;
   std::vector<int64_t> pr(w->progress.n);
-  HIP_TRY(hipStreamSynchronize(w->prob->ctx->stream));
+  HIP_TRY(sync_stream(w->prob->ctx));
   HIP_TRY(hipMemcpy(pr.data(), w->progress.p, pr.size() * sizeof(int64_t), hipMemcpyDeviceToHost));
   int64_t m = 0;
   for (int u = 0; u < w->n_units; ++u) m = std::max(m, pr[2 * u + 1]);
@@ -1637,7 +1724,7 @@ int qmc_batch_create(qmc_ctx ctx, qmc_batch* out) {
 void qmc_batch_destroy(qmc_batch b) {
   if (!b) return;
   qmc_ctx c = b->ctx;
-  (void)hipStreamSynchronize(c->stream);
+  (void)sync_stream(c);
   for (qmc_walkers w : b->members) walkers_release(w);
   delete b;
   ctx_release(c);
@@ -1686,7 +1773,7 @@ static int batch_rebuild(qmc_batch b) {
     if (b->seen_version[i] != b->members[i]->version) stale = true;
   if (!stale) return QMC_OK;
   // launches still running may read the descriptors released below
-  HIP_TRY(hipStreamSynchronize(b->ctx->stream));
+  HIP_TRY(sync_stream(b->ctx));
   b->groups.clear();
   b->rebuilds++;
   // Largest problems first, so the longest work-groups start earliest.
@@ -1804,7 +1891,7 @@ static int enqueue_groups(qmc_batch b, int32_t max_sweeps) {
   for (auto* g : gs) coop_any |= g->coop != 0;
   const int n_streams = (gs.size() > 1 && !coop_any) ? 1 + kAuxStreams : 1;
   if (n_streams > 1) {
-    HIP_TRY(hipEventRecord(c->fork_ev, c->stream));
+    HIP_TRY(hipEventRecord(c->fork_ev, stream_of(c)));
     for (int i = 0; i < kAuxStreams; ++i) HIP_TRY(hipStreamWaitEvent(c->aux[i], c->fork_ev, 0));
   }
   std::stable_sort(gs.begin(), gs.end(), [](auto* x, auto* y) { return x->work > y->work; });
@@ -1812,12 +1899,14 @@ static int enqueue_groups(qmc_batch b, int32_t max_sweeps) {
   for (auto* g : gs) {
     const int s = (int)(std::min_element(load.begin(), load.end()) - load.begin());
     load[s] += g->work;
-    if ((rc = launch_group(*g, max_sweeps, c->lut.p, s == 0 ? c->stream : c->aux[s - 1]))) return rc;
+    const hipStream_t gs = s == 0 ? stream_of(c) : c->aux[s - 1];
+    if (uploads_pending(c)) return fail(QMC_DEVICE_ERROR, "internal: batch launch before its uploads");
+    if ((rc = launch_group(*g, max_sweeps, c->lut.p, gs))) return rc;
   }
   if (n_streams > 1)
     for (int i = 0; i < kAuxStreams; ++i) {
       HIP_TRY(hipEventRecord(c->join_ev[i], c->aux[i]));
-      HIP_TRY(hipStreamWaitEvent(c->stream, c->join_ev[i], 0));
+      HIP_TRY(hipStreamWaitEvent(stream_of(c), c->join_ev[i], 0));
     }
   return QMC_OK;
 }
@@ -1855,10 +1944,10 @@ int qmc_batch_run(qmc_batch b, int32_t max_sweeps) {
   if (rc || (rc = batch_materialize(b))) return rc;
   qmc_ctx c = b->ctx;
   if ((rc = ensure_aux_streams(c))) return rc;
-  if (!b->span_open) HIP_TRY(hipEventRecord(b->ev0, c->stream));
+  if (!b->span_open) HIP_TRY(hipEventRecord(b->ev0, stream_of(c)));
   b->span_open = true;
   if ((rc = enqueue_groups(b, max_sweeps))) return rc;
-  HIP_TRY(hipEventRecord(b->ev1, c->stream));
+  HIP_TRY(hipEventRecord(b->ev1, stream_of(c)));
   return QMC_OK;
 }
 
@@ -1886,30 +1975,30 @@ int qmc_batch_run_chunked(qmc_batch b, int32_t sweeps_per_launch) {
     drop_batch_graph(b);
     // The kernels read each unit's progress on the device, so one recorded
     // sequence of launches serves every replay without argument updates.
-    HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    HIP_TRY(hipStreamBeginCapture(stream_of(c), hipStreamCaptureModeThreadLocal));
     for (int64_t i = 0; i < launches; ++i) {
       rc = enqueue_groups(b, sweeps_per_launch);
       if (rc) {
         hipGraph_t g = nullptr;
-        (void)hipStreamEndCapture(c->stream, &g);
+        (void)hipStreamEndCapture(stream_of(c), &g);
         if (g) (void)hipGraphDestroy(g);
         return rc;
       }
     }
-    HIP_TRY(hipStreamEndCapture(c->stream, &b->graph));
+    HIP_TRY(hipStreamEndCapture(stream_of(c), &b->graph));
     HIP_TRY(hipGraphInstantiate(&b->graph_exec, b->graph, nullptr, nullptr, 0));
     std::copy(key, key + 3, b->graph_key);
   }
-  if (!b->span_open) HIP_TRY(hipEventRecord(b->ev0, c->stream));
+  if (!b->span_open) HIP_TRY(hipEventRecord(b->ev0, stream_of(c)));
   b->span_open = true;
-  HIP_TRY(hipGraphLaunch(b->graph_exec, c->stream));
-  HIP_TRY(hipEventRecord(b->ev1, c->stream));
+  HIP_TRY(hipGraphLaunch(b->graph_exec, stream_of(c)));
+  HIP_TRY(hipEventRecord(b->ev1, stream_of(c)));
   return QMC_OK;
 }
 
 int qmc_batch_wait(qmc_batch b, double* ms) {
   if (!b) return fail(QMC_INVALID_ARGUMENT, "null handle");
-  HIP_TRY(hipStreamSynchronize(b->ctx->stream));
+  HIP_TRY(sync_stream(b->ctx));
   if (b->span_open) HIP_TRY(hipEventElapsedTime(&b->last_ms, b->ev0, b->ev1));
   b->span_open = false;
   if (ms) *ms = b->last_ms;
@@ -1956,21 +2045,21 @@ int qmc_batch_read(qmc_batch b, double* lowest_energy, uint64_t* accepted, uint8
     soff += (int64_t)w->R * p->N;
   }
   if (roff == 0) return QMC_OK;
-  HIP_TRY(hipStreamSynchronize(c->stream));  // pooled buffers below may be reused
+  HIP_TRY(sync_stream(c));  // pooled buffers below may be reused
   HIP_TRY(b->d_desc.upload(desc.data(), desc.size()));
   HIP_TRY(b->d_rep_member.upload(rep_member.data(), rep_member.size()));
   HIP_TRY(b->d_states.reserve((size_t)std::max<int64_t>(soff, 1)));
   HIP_TRY(b->d_energy.reserve((size_t)roff));
   HIP_TRY(b->d_acc.reserve((size_t)roff));
   HIP_TRY(launch_batch_read(b->d_desc.p, b->d_rep_member.p, roff, b->d_states.p, b->d_energy.p, b->d_acc.p,
-                            c->stream));
+                            stream_of(c)));
   if (lowest_energy)
-    HIP_TRY(hipMemcpyAsync(lowest_energy, b->d_energy.p, roff * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipMemcpyAsync(lowest_energy, b->d_energy.p, roff * sizeof(double), hipMemcpyDeviceToHost, stream_of(c)));
   if (accepted)
-    HIP_TRY(hipMemcpyAsync(accepted, b->d_acc.p, roff * sizeof(uint64_t), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipMemcpyAsync(accepted, b->d_acc.p, roff * sizeof(uint64_t), hipMemcpyDeviceToHost, stream_of(c)));
   if (lowest_states && soff)
-    HIP_TRY(hipMemcpyAsync(lowest_states, b->d_states.p, (size_t)soff, hipMemcpyDeviceToHost, c->stream));
-  HIP_TRY(hipStreamSynchronize(c->stream));
+    HIP_TRY(hipMemcpyAsync(lowest_states, b->d_states.p, (size_t)soff, hipMemcpyDeviceToHost, stream_of(c)));
+  HIP_TRY(sync_stream(c));
   return QMC_OK;
 }
 
@@ -2008,13 +2097,13 @@ int qmc_walkers_pt_observe(qmc_walkers w, int32_t ladder, int64_t max_steps) {
   HIP_TRY(w->pt_dir.alloc((size_t)n_ladders * ladder));
   // accepted counts so far, direction labels up (0) at the start
   HIP_TRY(hipMemcpyAsync(w->pt_acc_prev.p, w->accepted.p, (size_t)w->cap * sizeof(unsigned long long),
-                         hipMemcpyDeviceToDevice, w->prob->ctx->stream));
+                         hipMemcpyDeviceToDevice, stream_of(w->prob->ctx)));
   ZeroSpans z{};
   z.p[0] = w->pt_dir.p;
   z.n[0] = (size_t)n_ladders * ladder * sizeof(int32_t);
   z.p[1] = w->pt_log.p;
   z.n[1] = n * sizeof(double);
-  HIP_TRY(launch_zero(z, 2, w->prob->ctx->stream));
+  HIP_TRY(launch_zero(z, 2, stream_of(w->prob->ctx)));
   w->pt_obs.log = w->pt_log.p;
   w->pt_obs.log_steps = max_steps;
   w->pt_obs.accepted = w->accepted.p;
@@ -2028,7 +2117,7 @@ int qmc_walkers_pt_log(qmc_walkers w, int64_t n_values, double* out) {
   if (!w || (!out && n_values > 0)) return fail(QMC_INVALID_ARGUMENT, "null argument");
   if (!w->pt_log.p) return fail(QMC_INVALID_ARGUMENT, "observation was not started");
   if (n_values < 0 || (size_t)n_values > w->pt_log.n) return fail(QMC_INVALID_ARGUMENT, "n_values out of range");
-  HIP_TRY(hipStreamSynchronize(w->prob->ctx->stream));
+  HIP_TRY(sync_stream(w->prob->ctx));
   if (n_values > 0) HIP_TRY(hipMemcpy(out, w->pt_log.p, (size_t)n_values * sizeof(double), hipMemcpyDeviceToHost));
   return QMC_OK;
 }
@@ -2050,7 +2139,7 @@ int qmc_walkers_read(qmc_walkers w, double* cost, double* lowest, double* beta, 
   if (!w) return fail(QMC_INVALID_ARGUMENT, "null handle");
   if (int rc0 = materialize(w)) return rc0;
   qmc_problem p = w->prob;
-  hipStream_t st = p->ctx->stream;
+  const CtxStream st{p->ctx};
   HIP_TRY(hipStreamSynchronize(st));
   if (cost) HIP_TRY(hipMemcpy(cost, w->cost.p, w->R * sizeof(double), hipMemcpyDeviceToHost));
   if (lowest) HIP_TRY(hipMemcpy(lowest, w->best_cost.p, w->R * sizeof(double), hipMemcpyDeviceToHost));
@@ -2081,7 +2170,7 @@ int qmc_walkers_read_replica(qmc_walkers w, int32_t replica, uint8_t* state, uin
   if (replica < 0 || replica >= w->R) return fail(QMC_INVALID_ARGUMENT, "replica out of range");
   if (int rc0 = materialize(w)) return rc0;
   qmc_problem p = w->prob;
-  hipStream_t st = p->ctx->stream;
+  const CtxStream st{p->ctx};
   const UnitDev* du = nullptr;
   int rc = device_units(w, &du);
   if (rc || (rc = ensure_scratch(w))) return rc;
@@ -2099,7 +2188,7 @@ int qmc_walkers_energy(qmc_walkers w, int lowest, double* energies) {
   if (!w || !energies) return fail(QMC_INVALID_ARGUMENT, "null argument");
   if (int rc0 = materialize(w)) return rc0;
   qmc_problem p = w->prob;
-  hipStream_t st = p->ctx->stream;
+  const CtxStream st{p->ctx};
   const UnitDev* du = nullptr;
   int rc = device_units(w, &du);
   if (rc || (rc = ensure_scratch(w))) return rc;
@@ -2113,7 +2202,7 @@ int qmc_walkers_energy(qmc_walkers w, int lowest, double* energies) {
 int qmc_walkers_labels(qmc_walkers w, int32_t* labels) {
   if (!w || !labels) return fail(QMC_INVALID_ARGUMENT, "null argument");
   if (int rc0 = materialize(w)) return rc0;
-  HIP_TRY(hipStreamSynchronize(w->prob->ctx->stream));
+  HIP_TRY(sync_stream(w->prob->ctx));
   HIP_TRY(hipMemcpy(labels, w->label.p, w->R * sizeof(int32_t), hipMemcpyDeviceToHost));
   return QMC_OK;
 }
@@ -2127,7 +2216,7 @@ int qmc_walkers_last_ms(qmc_walkers w, double* ms) {
 int qmc_energy(qmc_problem p, int32_t n_states, const uint8_t* states, double* energies) {
   if (!p || !states || !energies || n_states < 0) return fail(QMC_INVALID_ARGUMENT, "bad argument");
   if (n_states == 0) return QMC_OK;
-  hipStream_t st = p->ctx->stream;
+  const CtxStream st{p->ctx};
   DevBuf<uint8_t> ds;
   DevBuf<double> de;
   HIP_TRY(ds.upload(states, (size_t)n_states * p->N));
@@ -2158,7 +2247,7 @@ int qmc_energy_noalloc(qmc_problem p, int32_t n_states, const uint8_t* states, d
   if (!base) return fail(QMC_DEVICE_ERROR, "qmc_energy_noalloc: no scratch address for this device");
   std::lock_guard<std::mutex> use(g_scratch_use[dev]);
   HIP_TRY(hipSetDevice(dev));
-  hipStream_t st = p->ctx->stream;
+  const CtxStream st{p->ctx};
   uint8_t* d_states = static_cast<uint8_t*>(base);
   double* d_energy = reinterpret_cast<double*>(d_states + state_bytes);
   // every byte the kernel reads is written by this copy, and every energy by
@@ -2173,7 +2262,7 @@ int qmc_energy_noalloc(qmc_problem p, int32_t n_states, const uint8_t* states, d
 int qmc_delta_all(qmc_problem p, int32_t n_states, const uint8_t* states, double* out) {
   if (!p || !states || !out || n_states < 0) return fail(QMC_INVALID_ARGUMENT, "bad argument");
   if (n_states == 0) return QMC_OK;
-  hipStream_t st = p->ctx->stream;
+  const CtxStream st{p->ctx};
   DevBuf<uint8_t> ds;
   DevBuf<double> d;
   HIP_TRY(ds.upload(states, (size_t)n_states * p->N));
@@ -2192,11 +2281,11 @@ int qmc_lds_bandwidth(qmc_ctx ctx, double* bytes_per_s) {
   const int blocks = 2 * cus, iters = 1 << 15;
   DevBuf<uint32_t> out;
   HIP_TRY(out.alloc((size_t)blocks * 1024));
-  (void)launch_lds_probe(blocks, 64, out.p, ctx->stream);  // warm-up
-  HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
-  const double bytes = launch_lds_probe(blocks, iters, out.p, ctx->stream);
-  HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));
-  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  (void)launch_lds_probe(blocks, 64, out.p, stream_of(ctx));  // warm-up
+  HIP_TRY(hipEventRecord(ctx->ev0, stream_of(ctx)));
+  const double bytes = launch_lds_probe(blocks, iters, out.p, stream_of(ctx));
+  HIP_TRY(hipEventRecord(ctx->ev1, stream_of(ctx)));
+  HIP_TRY(sync_stream(ctx));
   if (bytes < 0) return fail(QMC_DEVICE_ERROR, "LDS probe launch failed");
   float ms = 0.f;
   HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
@@ -2210,8 +2299,8 @@ int qmc_philox(qmc_ctx ctx, int32_t n, const uint32_t* ctr, const uint32_t key[2
   DevBuf<uint32_t> dc, dout;
   HIP_TRY(dc.upload(ctr, (size_t)4 * n));
   HIP_TRY(dout.alloc((size_t)4 * n));
-  HIP_TRY(launch_philox(n, dc.p, key[0], key[1], dout.p, ctx->stream));
-  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  HIP_TRY(launch_philox(n, dc.p, key[0], key[1], dout.p, stream_of(ctx)));
+  HIP_TRY(sync_stream(ctx));
   HIP_TRY(hipMemcpy(out, dout.p, (size_t)16 * n, hipMemcpyDeviceToHost));
   return QMC_OK;
 }

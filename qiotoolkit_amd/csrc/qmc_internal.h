@@ -293,5 +293,15 @@ hipError_t launch_batch_read(const ReadDesc* desc, const int32_t* rep_member, in
 double launch_lds_probe(int blocks, int iters, uint32_t* out, hipStream_t st);
 hipError_t launch_philox(int n, const uint32_t* ctr, uint32_t k0, uint32_t k1, uint32_t* out,
                          hipStream_t st);
+// One staged host-to-device upload: `bytes` bytes from offset `src` of the
+// device copy of the staging ring to device address `dst`.
+struct ScatterRec {
+  uint64_t src;
+  uint64_t dst;
+  uint64_t bytes;
+};
+// Applies n such copies (one work-group each, in no particular order: their
+// destinations do not overlap).
+hipError_t launch_scatter(const uint8_t* stage, const ScatterRec* rec, int n, hipStream_t st);
 
 }  // namespace qmc

@@ -2089,6 +2089,22 @@ __global__ void __launch_bounds__(1024) k_lds_probe(int iters, uint32_t* __restr
   out[blockIdx.x * blockDim.x + threadIdx.x] = acc.x ^ acc.y ^ acc.z ^ acc.w;
 }
 
+// Staged uploads: record b copies rec[b].bytes bytes from the device copy of
+// the staging ring to rec[b].dst, in 4-byte words when everything is aligned.
+__global__ void __launch_bounds__(256) k_scatter(const uint8_t* __restrict__ stage,
+                                                 const ScatterRec* __restrict__ rec) {
+  const ScatterRec r = rec[blockIdx.x];
+  const uint8_t* src = stage + r.src;
+  uint8_t* dst = reinterpret_cast<uint8_t*>(r.dst);
+  if (((r.src | r.dst | r.bytes) & 3u) == 0) {
+    const uint32_t* s4 = reinterpret_cast<const uint32_t*>(src);
+    uint32_t* d4 = reinterpret_cast<uint32_t*>(dst);
+    for (uint64_t i = threadIdx.x; i < r.bytes / 4; i += blockDim.x) d4[i] = s4[i];
+  } else {
+    for (uint64_t i = threadIdx.x; i < r.bytes; i += blockDim.x) dst[i] = src[i];
+  }
+}
+
 // ---------------------------------------------------------------------------
 // launch helpers (host side of this translation unit)
 size_t sweep_smem_bytes(int max_stride, int ordered, int* best_lds) {
@@ -2328,6 +2344,12 @@ hipError_t launch_delta_all(const ProblemDev& P, int n_states, const uint8_t* st
 double launch_lds_probe(int blocks, int iters, uint32_t* out, hipStream_t st) {
   hipLaunchKernelGGL(k_lds_probe, dim3(blocks), dim3(1024), 0, st, iters, out);
   return hipGetLastError() == hipSuccess ? (double)blocks * 1024.0 * iters * 16.0 : -1.0;
+}
+
+hipError_t launch_scatter(const uint8_t* stage, const ScatterRec* rec, int n, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_scatter, dim3((unsigned)n), dim3(256), 0, st, stage, rec);
+  return hipGetLastError();
 }
 
 hipError_t launch_philox(int n, const uint32_t* ctr, uint32_t k0, uint32_t k1, uint32_t* out,
