@@ -162,6 +162,11 @@ def run_sa(args, name, steps, warmup):
     from qiotoolkit_amd import workloads as wl
     from qiotoolkit_amd.solver import Engine
     kind, terms, params, d, sc, dtype, desc = SA_WORKLOADS[name]
+    params = dict(params)
+    if args.group:
+        params["replicas_per_group"] = args.group
+    if args.lanes:
+        params["lanes"] = args.lanes
     eng = Engine(0)
     model = eng.model({"cost_function": {"type": kind, "version": "1.0", "terms": terms(wl)}})
     n = model.n_vars()

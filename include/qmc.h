@@ -77,10 +77,17 @@ const char* qmc_last_error(void);
 int qmc_ctx_create(int device, qmc_ctx* out);
 void qmc_ctx_destroy(qmc_ctx ctx);
 int qmc_sync(qmc_ctx ctx);
+/* Compute units of the context's device (no counterpart in the reference,
+ * which sizes its work by OpenMP threads): the host solvers size default
+ * launch shapes by it. */
+int qmc_ctx_compute_units(qmc_ctx ctx, int32_t* compute_units);
 
 /* Problem in internal ids 0..n_vars-1 (first-appearance order) as a term
  * CSR; ids sorted inside each term, no constant terms.  lanes = default
- * class-round width of walkers on it (0: from the largest colour class). */
+ * class-round width of walkers on it (0: each walker set picks its own from
+ * the largest colour class and its work-group count: from 64 lanes, doubled
+ * while its work-groups hold fewer than 8 waves per compute unit and one
+ * round does not cover the largest class). */
 int qmc_problem_create(qmc_ctx ctx, int kind, int32_t n_vars, int64_t n_terms,
                        const int64_t* term_offsets, const int32_t* term_vars,
                        const double* coeffs, int32_t lanes, qmc_problem* out);

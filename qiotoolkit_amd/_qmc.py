@@ -23,7 +23,7 @@ NOALLOC_SCRATCH_BYTES = 16 << 20
 
 # every symbol include/qmc.h declares
 EXPORTS = [
-    "qmc_last_error", "qmc_ctx_create", "qmc_ctx_destroy", "qmc_sync", "qmc_problem_create",
+    "qmc_last_error", "qmc_ctx_create", "qmc_ctx_destroy", "qmc_sync", "qmc_ctx_compute_units", "qmc_problem_create",
     "qmc_problem_destroy", "qmc_problem_get_info", "qmc_problem_order", "qmc_walkers_create",
     "qmc_walkers_destroy", "qmc_walkers_init", "qmc_walkers_set_states", "qmc_walkers_set_betas",
     "qmc_walkers_sweep", "qmc_walkers_pt", "qmc_walkers_read", "qmc_walkers_energy",

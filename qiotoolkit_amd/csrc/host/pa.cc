@@ -135,7 +135,7 @@ void PopulationAnnealing::init() {
   const bool exp_repop = feature_enabled(2);
   st.cap = (int32_t)device_replicas();
   const bool seq = sequential();
-  const int B = group_ > 0 ? group_ : seq ? 32 : default_group(target_population_);
+  const int B = group_ > 0 ? group_ : seq ? 32 : default_group(target_population_, compute_units());
   device_check(qmc_walkers_create(dmodel_->handle(visit()), st.cap, B, seed_, 0, 0, st.cap, &st.w),
                "qmc_walkers_create");
   apply_lanes(st.w, B);

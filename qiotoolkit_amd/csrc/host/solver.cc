@@ -28,15 +28,20 @@ double beta_from(double v, bool use_inverse) {
   return v <= 0 ? std::numeric_limits<double>::infinity() : 1.0 / v;
 }
 
-// Replicas per work-group when the caller does not pick one: enough
-// work-groups to spread over the CUs, up to 8 replicas sharing each
-// neighbour byte.
 }  // namespace
 
-int default_group(int64_t R) {
+// Replicas per work-group when the caller does not pick one: at least one
+// work-group per compute unit, up to 8 replicas sharing each neighbour byte.
+int default_group(int64_t R, int compute_units) {
   int B = 1;
-  while (B < 8 && R / (2 * B) >= 128) B *= 2;
+  while (B < 8 && R / (2 * B) >= compute_units) B *= 2;
   return (int)std::min<int64_t>(B, R);
+}
+
+int Solver::compute_units() const {
+  int32_t cus = 256;
+  if (ctx_) qmc_ctx_compute_units(ctx_, &cus);
+  return cus;
 }
 
 void Solver::configure(const Json& doc) {
@@ -371,7 +376,7 @@ SimulatedAnnealing::~SimulatedAnnealing() {
 void SimulatedAnnealing::init() {
   if (model_->empty()) return;
   const bool seq = sequential();
-  const int B = group_ > 0 ? group_ : seq ? std::min(restarts_, 32) : default_group(restarts_);
+  const int B = group_ > 0 ? group_ : seq ? std::min(restarts_, 32) : default_group(restarts_, compute_units());
   device_check(qmc_walkers_create(dmodel_->handle(visit()), restarts_, B, seed_, 0, 0, restarts_, &w_),
                "qmc_walkers_create");
   apply_lanes(w_, B);

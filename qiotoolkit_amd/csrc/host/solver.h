@@ -173,6 +173,7 @@ class Solver {
   }
   bool sequential() const { return sweep_order_ == 2; }
   int lanes_for(int replicas_per_group) const;
+  int compute_units() const;  // of the context's device (default launch shapes)
   void apply_lanes(qmc_walkers w, int replicas_per_group) const;
   int64_t steps_done_ = 0;
   int64_t attempts_ = 0, accepted_ = 0;
@@ -350,7 +351,7 @@ class ParameterFree : public Solver {
 
 // Replicas per work-group when the params do not say: small groups while
 // that still leaves at least 128 work-groups, so a launch fills the device.
-int default_group(int64_t replicas);
+int default_group(int64_t replicas, int compute_units);
 
 std::unique_ptr<Solver> create_solver(const std::string& target);
 

@@ -124,6 +124,7 @@ static bool int8_entries_fit(qmc_problem p);
 struct qmc_ctx_s {
   std::atomic<int> refs{1};
   int device = 0;
+  int n_cus = 256;  // compute units of the device (launch-shape defaults)
   // The context's stream.  Every use goes through stream_of() or
   // sync_stream(), which first enqueue the uploads staged since the last use.
   hipStream_t raw_stream = nullptr;
@@ -295,6 +296,8 @@ struct qmc_problem_s {
   int coupling = QMC_COUPLING_I32;
   int pairwise = 1;
   int lanes = 64;
+  bool lanes_given = false;  // lanes chosen by the caller (else walker sets pick their own)
+  int32_t max_class = 0;
   int max_degree = 0;
   std::vector<int32_t> class_off, var_of_pos, pos_of_var;
   DevBuf<int32_t> d_class_off, d_var_of_pos, d_pos_of_var, d_inc_off, d_nbr, d_oth_off, d_oth_pos;
@@ -486,6 +489,11 @@ int qmc_ctx_create(int device, qmc_ctx* out) {
   }
   auto* c = new qmc_ctx_s();
   c->device = device;
+  {
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cus > 0)
+      c->n_cus = cus;
+  }
   if (hipStreamCreateWithFlags(&c->raw_stream, hipStreamNonBlocking) != hipSuccess) {
     delete c;
     return fail(QMC_DEVICE_ERROR, "hipStreamCreate failed");
@@ -524,6 +532,12 @@ static void ctx_release(qmc_ctx c) {
 }
 
 void qmc_ctx_destroy(qmc_ctx c) { ctx_release(c); }
+
+int qmc_ctx_compute_units(qmc_ctx c, int32_t* compute_units) {
+  if (!c || !compute_units) return fail(QMC_INVALID_ARGUMENT, "null argument");
+  *compute_units = c->n_cus;
+  return QMC_OK;
+}
 
 int qmc_sync(qmc_ctx c) {
   if (!c) return fail(QMC_INVALID_ARGUMENT, "null context");
@@ -626,6 +640,8 @@ int qmc_problem_create_visit(qmc_ctx ctx, int kind, int32_t n_vars, int64_t n_te
     for (int32_t q = 0; q < N; ++q) p->pos_of_var[p->var_of_pos[q]] = q;
   }
   const int32_t nmax = *std::max_element(csize.begin(), csize.end());
+  p->max_class = nmax;
+  p->lanes_given = lanes != 0;
   if (lanes) {
     p->lanes = lanes;
   } else {
@@ -988,6 +1004,18 @@ static int set_progress(qmc_walkers w, int64_t sweep0, int64_t n) {
   return QMC_OK;
 }
 
+// Class-round width of a walker set on a problem created without one: from
+// 64, lanes double while the set's work-groups hold fewer than 8 waves per
+// compute unit, one round does not yet cover the largest colour class, and
+// lanes x G stays within 512.  Few work-groups get wide ones (shorter class
+// rounds, more waves to hide latency); many get narrow ones.
+static int default_lanes(qmc_problem p, int64_t n_units, int G) {
+  const int64_t want = 8 * (int64_t)p->ctx->n_cus;
+  int l = 64;
+  while (l * 2 * G <= 512 && l < p->max_class && n_units * (l * G / 64) < want) l *= 2;
+  return l;
+}
+
 int qmc_walkers_create(qmc_problem p, int32_t n_replicas, int32_t replicas_per_group,
                         uint32_t seed, uint32_t stream, int32_t first_replica,
                         int32_t replicas_per_seed, qmc_walkers* out) {
@@ -1001,10 +1029,10 @@ int qmc_walkers_create(qmc_problem p, int32_t n_replicas, int32_t replicas_per_g
   w->R = n_replicas;
   w->B = replicas_per_group;
   w->G = (w->B + 7) / 8;
-  w->lanes = p->lanes;
+  w->n_units = (w->R + w->B - 1) / w->B;
+  w->lanes = p->lanes_given ? p->lanes : default_lanes(p, w->n_units, w->G);
   while (w->lanes * w->G > 512) w->lanes /= 2;
   if (w->lanes < 64) return fail(QMC_INVALID_ARGUMENT, "replica bytes exceed the work-group");
-  w->n_units = (w->R + w->B - 1) / w->B;
   w->cap = w->R;
   w->cap_units = w->n_units;
   w->stride = (int32_t)(((int64_t)(p->N + 1) * w->G + 3) / 4 * 4);
