@@ -22,7 +22,7 @@ HIP_FLAGS = [
 DEVICE_SOURCES = ["qmc_kernels.hip", "qmc_api.cpp"]
 HOST_SOURCES = ["host/json.cc", "host/model.cc", "host/schedule.cc", "host/solver.cc", "host/pa.cc",
                 "host/capi.cc", "host/server.cc", "host/proto.cc", "host/observer.cc",
-                "host/param_free.cc"]
+                "host/param_free.cc", "host/model_stream.cc"]
 
 
 def _run(cmd):

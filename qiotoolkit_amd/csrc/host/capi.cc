@@ -315,7 +315,7 @@ int qiotk_model_create(qiotk_engine e, const char* input_json, qiotk_model* out)
   return guarded(nullptr, [&] {
     auto m = std::make_unique<qiotk_model_s>();
     m->e = e;
-    m->model = parse_model(Json::parse(input_json));
+    m->model = read_model_text(input_json);
     m->model.source_bytes = (int64_t)std::strlen(input_json);
     if (!m->model.empty()) {
       m->dev = std::make_unique<DeviceModel>(e->ctx, m->model);
@@ -331,7 +331,7 @@ int qiotk_model_create_file(qiotk_engine e, const char* path, qiotk_model* out) 
     if (!path) throw Error(kMissingInput, "no input file");
     auto m = std::make_unique<qiotk_model_s>();
     m->e = e;
-    m->model = parse_model(read_input_file(path));
+    m->model = read_model_file(path);
     m->model.source_bytes = path_bytes(path);
     if (!m->model.empty()) {
       m->dev = std::make_unique<DeviceModel>(e->ctx, m->model);
@@ -345,14 +345,9 @@ int qiotk_model_create_file(qiotk_engine e, const char* path, qiotk_model* out) 
 int qiotk_model_create_terms(qiotk_engine e, int kind, int64_t n_terms, const int64_t* offsets,
                              const int32_t* ids, const double* coeffs, qiotk_model* out) {
   return guarded(nullptr, [&] {
-    std::vector<Term> terms(n_terms);
-    for (int64_t k = 0; k < n_terms; ++k) {
-      terms[k].c = coeffs[k];
-      terms[k].ids.assign(ids + offsets[k], ids + offsets[k + 1]);
-    }
     auto m = std::make_unique<qiotk_model_s>();
     m->e = e;
-    m->model = make_model(kind == 0 ? QMC_ISING : QMC_PUBO, terms);
+    m->model = make_model_flat(kind == 0 ? QMC_ISING : QMC_PUBO, n_terms, offsets, ids, coeffs);
     if (!m->model.empty()) {
       m->dev = std::make_unique<DeviceModel>(e->ctx, m->model);
       m->dev->handle(QMC_VISIT_COLOR_CLASSES);  // resident: solves exclude the upload
@@ -648,6 +643,7 @@ static Json describe_model(const Model& m) {
   j["term_vars"] = vars;
   j["coeffs"] = cs;
   j["scale_factor"] = m.scale_factor();
+  j["input_reader"] = m.streamed ? "stream" : "tree";
   if (!m.empty()) {
     j["estimate_max_cost_diff"] = m.estimate_max_cost_diff();
     j["estimate_min_cost_diff"] = m.estimate_min_cost_diff();
@@ -665,7 +661,7 @@ static Json describe_model(const Model& m) {
 
 int qiotk_parse_model(const char* input_json, char** response) {
   return guarded(response, [&] {
-    *response = dup(describe_model(parse_model(Json::parse(input_json))).dump());
+    *response = dup(describe_model(read_model_text(input_json)).dump());
     return 0;
   });
 }
@@ -673,7 +669,7 @@ int qiotk_parse_model(const char* input_json, char** response) {
 int qiotk_parse_model_file(const char* path, char** response) {
   return guarded(response, [&] {
     if (!path) throw Error(kMissingInput, "no input file");
-    *response = dup(describe_model(parse_model(read_input_file(path))).dump());
+    *response = dup(describe_model(read_model_file(path)).dump());
     return 0;
   });
 }

@@ -8,26 +8,26 @@
 
 namespace qiotk {
 
-Model make_model(int kind, const std::vector<Term>& terms) {
+Model make_model_flat(int kind, int64_t n_terms, const int64_t* offsets, const int32_t* ids,
+                      const double* coeffs) {
   // graph::normalize_edges (graph/graph.h:74-177)
-  if (terms.empty())
+  if (n_terms <= 0)
     throw Error(kValueError, "parameter `terms`: size must be greater than 0, found 0");
   Model m;
   m.kind = kind;
+  m.term_off.reserve((size_t)n_terms + 1);
   m.term_off.push_back(0);
+  m.term_var.reserve((size_t)offsets[n_terms]);
+  m.coeff.reserve((size_t)n_terms);
+  m.name_to_id.reserve((size_t)std::min<int64_t>(offsets[n_terms], 1 << 24));
   std::vector<int32_t> members;
-  for (const Term& t : terms) {
+  for (int64_t k = 0; k < n_terms; ++k) {
     members.clear();
-    for (int32_t name : t.ids) {
-      auto it = m.name_to_id.find(name);
-      int32_t id;
-      if (it == m.name_to_id.end()) {
-        id = (int32_t)m.names.size();
-        m.name_to_id.emplace(name, id);
-        m.names.push_back(name);
-      } else {
-        id = it->second;
-      }
+    for (int64_t j = offsets[k]; j < offsets[k + 1]; ++j) {
+      const int32_t name = ids[j];
+      auto ins = m.name_to_id.emplace(name, (int32_t)m.names.size());
+      if (ins.second) m.names.push_back(name);
+      const int32_t id = ins.first->second;
       if (std::find(members.begin(), members.end(), id) == members.end()) {
         members.push_back(id);
       } else if (kind == QMC_ISING) {
@@ -35,24 +35,66 @@ Model make_model(int kind, const std::vector<Term>& terms) {
       }
     }
     std::sort(members.begin(), members.end());
+    const double c = coeffs[k];
     const uint32_t loc = (uint32_t)members.size();
     m.max_locality = std::max(m.max_locality, loc);
     m.min_locality = std::min(m.min_locality, loc);
     m.total_locality += loc;
     if (loc > 0) {
       m.dependent_vars += (uint64_t)loc * (loc - 1);
-      const double a = std::fabs(t.c);
+      const double a = std::fabs(c);
       m.max_coupling = std::max(m.max_coupling, a);
       m.min_coupling = std::min(m.min_coupling, a);
       m.term_var.insert(m.term_var.end(), members.begin(), members.end());
       m.term_off.push_back((int64_t)m.term_var.size());
-      m.coeff.push_back(t.c);
+      m.coeff.push_back(c);
     } else {
-      m.const_cost += t.c;
+      m.const_cost += c;
     }
   }
   m.n_vars = (int32_t)m.names.size();
   return m;
+}
+
+Model make_model(int kind, const std::vector<Term>& terms) {
+  std::vector<int64_t> off(1, 0);
+  std::vector<int32_t> ids;
+  std::vector<double> c;
+  for (const Term& t : terms) {
+    ids.insert(ids.end(), t.ids.begin(), t.ids.end());
+    off.push_back((int64_t)ids.size());
+    c.push_back(t.c);
+  }
+  return make_model_flat(kind, (int64_t)terms.size(), off.data(), ids.data(), c.data());
+}
+
+void apply_initial_value(Model& m, const std::string& name_text, int64_t v) {
+  // GraphModelConfiguration::map_initial_configuration (model/graph_model.h:64-95)
+  const int32_t name = (int32_t)std::stol(name_text);
+  auto it = m.name_to_id.find(name);
+  if (it == m.name_to_id.end())
+    throw Error(kInitialConfig, "parameter `initial_configuration` have node name: " + name_text +
+                                    " which is not found in model node names");
+  uint8_t bit;
+  if (m.kind == QMC_ISING) {
+    if (v != 1 && v != -1)
+      throw Error(kValueError, "parameter `initial_configuration` may have 1 and -1 values for ising models, found: " + std::to_string(v));
+    bit = v == -1;
+  } else {
+    if (v != 0 && v != 1)
+      throw Error(kValueError, "parameter `initial_configuration` may have 1 and 0 values for pubo models, found: " + std::to_string(v));
+    bit = v == 0;
+  }
+  m.initial.emplace_back(it->second, bit);
+}
+
+void finish_initial_configuration(Model& m) {
+  if (!m.initial.empty() && (int32_t)m.initial.size() != m.n_vars)
+    throw Error(kInitialConfig, "parameter `initial_configuration` have size: " +
+                                    std::to_string(m.initial.size()) +
+                                    " which is different from model nodes size: " +
+                                    std::to_string(m.n_vars));
+  m.has_initial = !m.initial.empty();
 }
 
 Model parse_model(const Json& doc) {
@@ -71,44 +113,21 @@ Model parse_model(const Json& doc) {
     throw Error(kValueError, "Expecting `version` equals 1.0 or 1.1, but found: " + version);
   if (!cf->has("terms")) throw Error(kMissingInput, "parameter `terms` is required");
   const Json& jt = cf->at("terms");
-  std::vector<Term> terms;
-  terms.reserve(jt.size());
+  std::vector<int64_t> off(1, 0);
+  std::vector<int32_t> ids;
+  std::vector<double> c;
+  off.reserve(jt.size() + 1);
+  c.reserve(jt.size());
   for (const Json& t : jt.items()) {
-    Term term;
-    term.c = t.at("c").as_double();
-    for (const Json& id : t.at("ids").items()) term.ids.push_back((int32_t)id.as_int());
-    terms.push_back(std::move(term));
+    c.push_back(t.at("c").as_double());
+    for (const Json& id : t.at("ids").items()) ids.push_back((int32_t)id.as_int());
+    off.push_back((int64_t)ids.size());
   }
-  Model m = make_model(kind, terms);
+  Model m = make_model_flat(kind, (int64_t)c.size(), off.data(), ids.data(), c.data());
   m.version = version;
   if (cf->has("initial_configuration")) {
-    // GraphModelConfiguration::map_initial_configuration (model/graph_model.h:64-95)
-    const Json& ic = cf->at("initial_configuration");
-    for (const auto& kv : ic.members()) {
-      const int32_t name = (int32_t)std::stol(kv.first);
-      auto it = m.name_to_id.find(name);
-      if (it == m.name_to_id.end())
-        throw Error(kInitialConfig, "parameter `initial_configuration` have node name: " + kv.first +
-                                        " which is not found in model node names");
-      const int64_t v = kv.second.as_int();
-      uint8_t bit;
-      if (kind == QMC_ISING) {
-        if (v != 1 && v != -1)
-          throw Error(kValueError, "parameter `initial_configuration` may have 1 and -1 values for ising models, found: " + std::to_string(v));
-        bit = v == -1;
-      } else {
-        if (v != 0 && v != 1)
-          throw Error(kValueError, "parameter `initial_configuration` may have 1 and 0 values for pubo models, found: " + std::to_string(v));
-        bit = v == 0;
-      }
-      m.initial.emplace_back(it->second, bit);
-    }
-    if (!m.initial.empty() && (int32_t)m.initial.size() != m.n_vars)
-      throw Error(kInitialConfig, "parameter `initial_configuration` have size: " +
-                                      std::to_string(m.initial.size()) +
-                                      " which is different from model nodes size: " +
-                                      std::to_string(m.n_vars));
-    m.has_initial = !m.initial.empty();
+    for (const auto& kv : cf->at("initial_configuration").members()) apply_initial_value(m, kv.first, kv.second.as_int());
+    finish_initial_configuration(m);
   }
   return m;
 }

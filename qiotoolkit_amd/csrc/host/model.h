@@ -4,6 +4,8 @@
 #pragma once
 #include <cstdint>
 #include <map>
+#include <optional>
+#include <unordered_map>
 #include <string>
 #include <vector>
 
@@ -31,12 +33,13 @@ struct Model {
   std::vector<double> coeff;
   double const_cost = 0.0;
   std::vector<int32_t> names;     // internal id -> input id
-  std::map<int32_t, int32_t> name_to_id;
+  std::unordered_map<int32_t, int32_t> name_to_id;
   // graph properties (graph/graph_properties.h)
   uint32_t max_locality = 0, min_locality = UINT32_MAX;
   uint64_t total_locality = 0;
   uint64_t dependent_vars = 0;  // sum of locality x (locality - 1) over non-constant terms
   int64_t source_bytes = 0;     // size of the input the model was read from (0: term arrays)
+  bool streamed = false;        // read by the streaming reader (else a tree parse or arrays)
   double max_coupling = 0.0, min_coupling = 1.7976931348623157e308;
   // initial_configuration, internal ids -> reference bit (true = -1 / 0)
   std::vector<std::pair<int32_t, uint8_t>> initial;
@@ -63,6 +66,28 @@ struct Model {
 // Parse {"cost_function": {...}} (or the inner object) and normalise it.
 Model parse_model(const Json& doc);
 Model make_model(int kind, const std::vector<Term>& terms);
+// The same from a term CSR (offsets[n_terms + 1] into ids, one coefficient
+// per term).
+Model make_model_flat(int kind, int64_t n_terms, const int64_t* offsets, const int32_t* ids,
+                      const double* coeffs);
+// initial_configuration of a built model: each {input id: value} member in
+// document order, then the size check
+void apply_initial_value(Model& m, const std::string& name_text, int64_t value);
+void finish_initial_configuration(Model& m);
+
+// Streaming input (the reference reads its input through a SAX reader in
+// 64 KiB file chunks, utils/stream_handler_json.h:124-182): the document is
+// read once, straight into the term CSR, with no document tree.  nullopt
+// when the document is not of the usual shape ({"cost_function": {"type",
+// "version", "terms": [{"c", "ids"}], "initial_configuration"}} plus other
+// keys, which are skipped) -- the caller then parses it as a tree, which
+// gives the reference's error for a malformed document.
+std::optional<Model> stream_model_text(const char* text, size_t n);
+std::optional<Model> stream_model_file(const std::string& path);
+// Parse a document from text or a file (a protobuf folder goes through the
+// wire-format reader), streaming when the shape allows.
+Model read_model_text(const char* text);
+Model read_model_file(const std::string& path);
 
 // Device copy of a model, owned.
 // The model on the device, once per visit order (QMC_VISIT_COLOR_CLASSES,

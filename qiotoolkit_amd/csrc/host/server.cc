@@ -387,12 +387,7 @@ int qiotk_server_submit(qiotk_server s, const char* target, const char* params_j
                         int64_t* request_id) {
   const double arrival = now_ms() - s->t0;
   try {
-    std::vector<Term> terms(n_terms);
-    for (int64_t k = 0; k < n_terms; ++k) {
-      terms[k].c = coeffs[k];
-      terms[k].ids.assign(ids + offsets[k], ids + offsets[k + 1]);
-    }
-    Model m = make_model(kind == 0 ? QMC_ISING : QMC_PUBO, terms);
+    Model m = make_model_flat(kind == 0 ? QMC_ISING : QMC_PUBO, n_terms, offsets, ids, coeffs);
     return submit(s, target, params_json, std::move(m), arrival, request_id);
   } catch (const Error& e) {
     g_err = e.what();
@@ -407,7 +402,7 @@ int qiotk_server_submit_json(qiotk_server s, const char* target, const char* par
                              const char* input_json, int64_t* request_id) {
   const double arrival = now_ms() - s->t0;
   try {
-    Model m = parse_model(Json::parse(input_json));
+    Model m = read_model_text(input_json);
     return submit(s, target, params_json, std::move(m), arrival, request_id);
   } catch (const Error& e) {
     g_err = e.what();
