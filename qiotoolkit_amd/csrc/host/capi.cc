@@ -127,7 +127,7 @@ struct Request {
 // finalize, and the benchmark block under the keys of utils/config.h.
 Json run_one(Solver& s, const Request& rq) {
   const double t0 = now_ms(), c0 = cpu_ms();
-  s.init();
+  s.init_with_retry();
   const double t1 = now_ms();
   s.run();
   const double t2 = now_ms();
@@ -199,7 +199,7 @@ void run_batch(qiotk_engine e, const char* target, int32_t n, const qiotk_model*
     auto s = create_solver(target);
     s->set_model(&models[k]->model, models[k]->dev.get(), e->ctx);
     s->configure(Json::parse(params_json[k]));
-    s->init();
+    s->init_with_retry();
     br.fused[k] = s->batch_plan(b) ? 1 : 0;
     br.solvers.push_back(std::move(s));
   }

@@ -123,10 +123,14 @@ PopulationAnnealing::~PopulationAnnealing() = default;
 
 // Poisson resampling keeps the population near the target; exponential
 // repopulation (feature 2) doubles the target at restarts, so it gets more
-// room (past it the solve ends with MemoryLimited).
+// room (past it the solve ends with MemoryLimited).  The memory-saving form
+// leaves room for half the population more (Poisson fluctuations of a
+// population of P are about sqrt(P)).
 int64_t PopulationAnnealing::device_replicas() const {
-  return feature_enabled(2) ? std::max<int64_t>(64 * target_population_, target_population_ + 4096)
-                            : std::max<int64_t>(4 * target_population_, target_population_ + 256);
+  const int64_t P = target_population_;
+  if (feature_enabled(2))
+    return memory_saving_ ? std::max<int64_t>(16 * P, P + 1024) : std::max<int64_t>(64 * P, P + 4096);
+  return memory_saving_ ? std::max<int64_t>(P + P / 2, P + 256) : std::max<int64_t>(4 * P, P + 256);
 }
 
 void PopulationAnnealing::init() {
@@ -135,7 +139,7 @@ void PopulationAnnealing::init() {
   const bool exp_repop = feature_enabled(2);
   st.cap = (int32_t)device_replicas();
   const bool seq = sequential();
-  const int B = group_ > 0 ? group_ : seq ? 32 : default_group(target_population_, compute_units());
+  const int B = saving_group(group_ > 0 ? group_ : seq ? 32 : default_group(target_population_, compute_units()));
   device_check(qmc_walkers_create(dmodel_->handle(visit()), st.cap, B, seed_, 0, 0, st.cap, &st.w),
                "qmc_walkers_create");
   apply_lanes(st.w, B);

@@ -52,7 +52,7 @@ std::vector<double> run_workers(qmc_ctx ctx, const Model& m, std::vector<std::un
   std::vector<int> fused(ws.size(), 0);
   bool any = false;
   for (size_t k = 0; k < ws.size(); ++k) {
-    ws[k]->init();
+    ws[k]->init_with_retry();
     fused[k] = ws[k]->batch_plan(b) ? 1 : 0;
     any = any || (fused[k] && ws[k]->batch_replicas() > 0);
   }

@@ -165,13 +165,25 @@ struct qiotk_server_s {
     while (!pending.empty() && (int)active.size() < max_active) {
       Request& r = *pending.front();
       int64_t fp = 0;
+      bool saving = false;
       std::unique_ptr<Solver> s;
       try {
         s = create_solver(r.target);
         if (!s->servable())
           throw Error(kNotImplemented, "the serving loop does not run target " + r.target);
         s->configure(r.params);
+        saving = s->memory_saving();
         fp = footprint(r.model, r.target, s->device_replicas());
+        if (fp > budget && !saving) {
+          // Runner::get_run_output's retry (app/runner.cc:518-537) at
+          // admission: the memory-saving form, when that one fits
+          s->set_memory_saving(true);
+          const int64_t fp_saving = footprint(r.model, r.target, s->device_replicas());
+          if (fp_saving <= budget) {
+            fp = fp_saving;
+            saving = true;
+          }
+        }
       } catch (const Error& e) {
         auto rq = std::move(pending.front());
         pending.pop_front();
@@ -200,7 +212,8 @@ struct qiotk_server_s {
         job->solver = create_solver(job->req->target);
         job->solver->set_model(&job->req->model, job->dev.get(), ctx);
         job->solver->configure(job->req->params);
-        job->solver->init();
+        if (saving) job->solver->set_memory_saving(true);
+        job->solver->init_with_retry();
       } catch (const Error& e) {
         finish(*job, error_doc(e.code(), e.what()), e.code());
         continue;

@@ -38,6 +38,30 @@ int default_group(int64_t R, int compute_units) {
   return (int)std::min<int64_t>(B, R);
 }
 
+void Solver::init_with_retry() {
+  try {
+    init();
+  } catch (const Error& e) {
+    if (e.code() != kMemoryLimited || memory_saving_) throw;
+    release_device();
+    set_memory_saving(true);
+    init();
+  }
+}
+
+void Solver::set_memory_saving(bool on) {
+  memory_saving_ = on;
+  if (on) set_output("memory_saving", Json(true));
+  else out_params_.remove("memory_saving");
+}
+
+int Solver::saving_group(int B, int divide) const {
+  if (!memory_saving_) return B;
+  int b = std::min(B, 8);
+  while (divide > 0 && b > 1 && divide % b) --b;
+  return std::max(1, b);
+}
+
 int Solver::compute_units() const {
   int32_t cus = 256;
   if (ctx_) qmc_ctx_compute_units(ctx_, &cus);
@@ -69,6 +93,7 @@ void Solver::configure(const Json& doc) {
   };
   enabled_features_ = features("enabled_features");
   for (int f : features("disabled_features")) enabled_features_.erase(f);
+  set_memory_saving(feature_enabled(3));  // FEATURE_USE_MEMORY_SAVING (utils/config.h:58)
   if (p.has("sweep_order")) {
     const std::string o = p.at("sweep_order").as_string();
     if (o == "colored") sweep_order_ = 0;
@@ -373,10 +398,16 @@ SimulatedAnnealing::~SimulatedAnnealing() {
   if (w_) qmc_walkers_destroy(w_);
 }
 
+void SimulatedAnnealing::release_device() {
+  if (w_) qmc_walkers_destroy(w_);
+  w_ = nullptr;
+}
+
 void SimulatedAnnealing::init() {
   if (model_->empty()) return;
   const bool seq = sequential();
-  const int B = group_ > 0 ? group_ : seq ? std::min(restarts_, 32) : default_group(restarts_, compute_units());
+  const int B = saving_group(group_ > 0 ? group_ : seq ? std::min(restarts_, 32)
+                                                    : default_group(restarts_, compute_units()));
   device_check(qmc_walkers_create(dmodel_->handle(visit()), restarts_, B, seed_, 0, 0, restarts_, &w_),
                "qmc_walkers_create");
   apply_lanes(w_, B);
@@ -558,6 +589,11 @@ ParallelTempering::~ParallelTempering() {
   if (w_) qmc_walkers_destroy(w_);
 }
 
+void ParallelTempering::release_device() {
+  if (w_) qmc_walkers_destroy(w_);
+  w_ = nullptr;
+}
+
 std::vector<double> ParallelTempering::ladder() const {
   // ParallelTempering::init (parallel_tempering.h:68-81): betas descending or
   // temperatures ascending, then one beta per replica
@@ -586,6 +622,7 @@ void ParallelTempering::init() {
   // ladder barrier (the best split measured on the 16^3 spin glass).
   int B = group_ > 0 ? group_ : ((L >= 4 && L % 2 == 0) ? 2 : L);
   if (L % B != 0) B = L;
+  B = saving_group(B, L);
   device_check(qmc_walkers_create(dmodel_->handle(visit()), L, B, seed_, 0, 0, L, &w_), "qmc_walkers_create");
   apply_lanes(w_, B);
   if (observe_dynamics_)

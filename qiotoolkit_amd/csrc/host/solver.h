@@ -38,6 +38,16 @@ class Solver {
   virtual void init() = 0;
   virtual void run() = 0;
   virtual void finalize() = 0;
+  // Runner::get_run_output (app/runner.cc:518-537): init(); when it fails
+  // with MemoryLimited and the memory-saving form is not on yet, switch it on
+  // (FEATURE_USE_MEMORY_SAVING, utils/config.h:58) and init again.
+  void init_with_retry();
+  // The memory-saving form: at most 8 replicas per work-group (one byte
+  // lane, the smallest LDS footprint per position) and, for population
+  // annealing, room for the population to grow by half instead of 4x.
+  // Switched on by "enabled_features": [3] or by the retry above.
+  bool memory_saving() const { return memory_saving_; }
+  void set_memory_saving(bool on);
   // Fused launches with other solves: plan the whole run on the device and
   // add the walkers to `b`; false when this solve needs host-side steps.
   virtual bool batch_plan(qmc_batch) { return false; }
@@ -166,6 +176,12 @@ class Solver {
     }
   }
   bool feature_enabled(int f) const { return enabled_features_.count(f) > 0; }
+  bool memory_saving_ = false;
+  // drops what init() created on the device (before a retry)
+  virtual void release_device() {}
+  // replicas per work-group in the memory-saving form: at most 8, and a
+  // divisor of `divide` when it is positive
+  int saving_group(int B, int divide = 0) const;
   virtual bool index_by_default() const { return false; }
   int visit() const {
     return sweep_order_ == 1 || (sweep_order_ < 0 && index_by_default()) ? QMC_VISIT_INDEX_LEVELS
@@ -211,6 +227,9 @@ class SimulatedAnnealing : public Solver {
 
   int64_t planned_steps();
 
+ protected:
+  void release_device() override;
+
  private:
   bool use_inverse_temperature_ = false;
   int restarts_ = 64;
@@ -243,6 +262,7 @@ class ParallelTempering : public Solver {
 
  protected:
   void observe_run() override;
+  void release_device() override;
 
   int64_t planned_steps();
 
@@ -281,6 +301,7 @@ class PopulationAnnealing : public Solver {
   // distributed differently from the reference's (a 12x12 lattice at
   // population 64 finds lower energies with fewer restarts)
   bool index_by_default() const override { return true; }
+  void release_device() override { impl_.reset(); }
 
  private:
   void finish_step();

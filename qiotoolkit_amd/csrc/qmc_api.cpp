@@ -34,7 +34,8 @@ int fail(int code, const std::string& msg) {
   do {                                                                                 \
     hipError_t e_ = (expr);                                                            \
     if (e_ != hipSuccess)                                                              \
-      return fail(QMC_DEVICE_ERROR, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+      return fail(e_ == hipErrorOutOfMemory ? QMC_MEMORY_LIMITED : QMC_DEVICE_ERROR,   \
+                  std::string(#expr) + ": " + hipGetErrorString(e_));                  \
   } while (0)
 
 // Device allocations are recycled through size buckets (powers of two from
@@ -59,7 +60,26 @@ class DevicePool {
         return hipSuccess;
       }
     }
-    return hipMalloc(out, b);
+    hipError_t e = hipMalloc(out, b);
+    if (e == hipErrorOutOfMemory) {
+      // give the device's cached free buffers back (hipFree waits for the
+      // device first) and try once more
+      (void)hipGetLastError();
+      trim();
+      e = hipMalloc(out, b);
+    }
+    return e;
+  }
+  // Frees the cached buffers of the current device.
+  void trim() {
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    std::lock_guard<std::mutex> lk(mu_);
+    for (auto& kv : free_) {
+      if (kv.first.first != dev) continue;
+      for (void* q : kv.second) (void)hipFree(q);
+      kv.second.clear();
+    }
   }
   void release(void* p, size_t bytes) {
     if (!p) return;
