@@ -100,15 +100,22 @@ __device__ __forceinline__ bool metropolis_accept(double cost_diff, double beta,
 // the interpolation yields NaN, which never accepts; t below row 0's range
 // (ln(1/256)) cannot accept either, so no early exits are needed.
 template <class Lut>
-__device__ __forceinline__ bool accept_row(double cost_diff, double beta, uint32_t r, int row, Lut&& lut) {
+__device__ __forceinline__ double row_value(uint32_t r, int row, Lut&& lut) {
   const uint32_t bucket = r & 255u;
   const int i1 = (int)bucket, i0 = i1 > 0 ? i1 - 1 : 0;
-  const double t = __dmul_rn(cost_diff, -beta);
   const double y0 = lut(row, i0);
   const double y1 = lut(row, i1);
   const double distance = div_ffffe(r >> 8);
-  const double value = __dadd_rn(y0, __dmul_rn(distance, __dsub_rn(y1, y0)));
-  return cost_diff <= 0.0 || value < t;
+  return __dadd_rn(y0, __dmul_rn(distance, __dsub_rn(y1, y0)));
+}
+// The threshold side of the decision, `value` = row_value(r, row, lut),
+// depends on the draw only: it can be formed before the cost difference.
+__device__ __forceinline__ bool accept_value(double cost_diff, double beta, double value) {
+  return cost_diff <= 0.0 || value < __dmul_rn(cost_diff, -beta);
+}
+template <class Lut>
+__device__ __forceinline__ bool accept_row(double cost_diff, double beta, uint32_t r, int row, Lut&& lut) {
+  return accept_value(cost_diff, beta, row_value(r, row, lut));
 }
 
 }  // namespace qmc

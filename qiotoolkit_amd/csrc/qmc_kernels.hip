@@ -96,31 +96,82 @@ __device__ __forceinline__ double dpp(double x, double ident) {
   return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
-template <typename AT>
-__device__ __forceinline__ AT scan_add(AT x) {  // inclusive prefix sum over the wave
-  x += dpp<DPP_ROW_SHR1, 0xf>(x, AT(0));
-  x += dpp<DPP_ROW_SHR2, 0xf>(x, AT(0));
-  x += dpp<DPP_ROW_SHR4, 0xf>(x, AT(0));
-  x += dpp<DPP_ROW_SHR8, 0xf>(x, AT(0));
-  x += dpp<DPP_ROW_BCAST15, 0xa>(x, AT(0));
-  x += dpp<DPP_ROW_BCAST31, 0xc>(x, AT(0));
-  return x;
+// Lanes without a source read 0 (bound_ctrl), every row written: no
+// identity has to be placed in the destination first.
+template <int CTRL>
+__device__ __forceinline__ int32_t dpp_z(int32_t x) {
+  return __builtin_amdgcn_update_dpp(0, x, CTRL, 0xf, 0xf, true);
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_z(double x) {
+  const long long xi = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_mov_dpp((int)xi, CTRL, 0xf, 0xf, true);
+  const int hi = __builtin_amdgcn_mov_dpp((int)(xi >> 32), CTRL, 0xf, 0xf, true);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+// Lanes without a source (and rows outside ROWS) keep `old`.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ int32_t dpp_keep(int32_t old, int32_t x) {
+  return __builtin_amdgcn_update_dpp(old, x, CTRL, ROWS, 0xf, false);
+}
+template <int CTRL, int ROWS>
+__device__ __forceinline__ double dpp_keep(double old, double x) {
+  const long long xi = __double_as_longlong(x), oi = __double_as_longlong(old);
+  const int lo = __builtin_amdgcn_update_dpp((int)oi, (int)xi, CTRL, ROWS, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp((int)(oi >> 32), (int)(xi >> 32), CTRL, ROWS, 0xf, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
 template <typename AT> __device__ __forceinline__ AT max_of();
 template <> __device__ __forceinline__ double max_of<double>() { return __builtin_huge_val(); }
 template <> __device__ __forceinline__ int32_t max_of<int32_t>() { return 0x7fffffff; }
 
-template <typename AT>
-__device__ __forceinline__ AT scan_min(AT x) {  // inclusive prefix minimum over the wave
-  const AT id = max_of<AT>();
-  x = min_ct(x, dpp<DPP_ROW_SHR1, 0xf>(x, id));
-  x = min_ct(x, dpp<DPP_ROW_SHR2, 0xf>(x, id));
-  x = min_ct(x, dpp<DPP_ROW_SHR4, 0xf>(x, id));
-  x = min_ct(x, dpp<DPP_ROW_SHR8, 0xf>(x, id));
-  x = min_ct(x, dpp<DPP_ROW_BCAST15, 0xa>(x, id));
-  x = min_ct(x, dpp<DPP_ROW_BCAST31, 0xc>(x, id));
-  return x;
+// Inclusive prefix sums and prefix minima over the wave of the NB values of
+// a lane's replicas, step by step, so the replicas' dependency chains
+// interleave.  Sums: the row shifts read 0 where they have no source; the
+// row broadcasts leave half the rows unwritten, which must read 0.
+template <int CTRL, typename AT, int NB>
+__device__ __forceinline__ void add_step_z(AT (&x)[8]) {
+#pragma unroll
+  for (int b = 0; b < NB; ++b) x[b] += dpp_z<CTRL>(x[b]);
+}
+template <int CTRL, int ROWS, typename AT, int NB>
+__device__ __forceinline__ void add_step(AT (&x)[8]) {
+#pragma unroll
+  for (int b = 0; b < NB; ++b) x[b] += dpp<CTRL, ROWS>(x[b], AT(0));
+}
+template <typename AT, int NB>
+__device__ __forceinline__ void scan_add_n(AT (&x)[8]) {
+  add_step_z<DPP_ROW_SHR1, AT, NB>(x);
+  add_step_z<DPP_ROW_SHR2, AT, NB>(x);
+  add_step_z<DPP_ROW_SHR4, AT, NB>(x);
+  add_step_z<DPP_ROW_SHR8, AT, NB>(x);
+  add_step<DPP_ROW_BCAST15, 0xa, AT, NB>(x);
+  add_step<DPP_ROW_BCAST31, 0xc, AT, NB>(x);
+}
+// Minima: a lane without a source (or in an unwritten row) keeps the
+// previous step's value in t, which is a minimum over lanes at or before
+// it in its own prefix, so it is never below the lane's running minimum
+// and min() leaves that unchanged; t starts as x itself.
+template <int CTRL, int ROWS, typename AT, int NB>
+__device__ __forceinline__ void min_step(AT (&x)[8], AT (&t)[8]) {
+#pragma unroll
+  for (int b = 0; b < NB; ++b) {
+    t[b] = dpp_keep<CTRL, ROWS>(t[b], x[b]);
+    x[b] = min_ct(x[b], t[b]);
+  }
+}
+template <typename AT, int NB>
+__device__ __forceinline__ void scan_min_n(AT (&x)[8]) {
+  AT t[8];
+#pragma unroll
+  for (int b = 0; b < NB; ++b) t[b] = x[b];
+  min_step<DPP_ROW_SHR1, 0xf, AT, NB>(x, t);
+  min_step<DPP_ROW_SHR2, 0xf, AT, NB>(x, t);
+  min_step<DPP_ROW_SHR4, 0xf, AT, NB>(x, t);
+  min_step<DPP_ROW_SHR8, 0xf, AT, NB>(x, t);
+  min_step<DPP_ROW_BCAST15, 0xa, AT, NB>(x, t);
+  min_step<DPP_ROW_BCAST31, 0xc, AT, NB>(x, t);
 }
 
 __device__ __forceinline__ int32_t lane_value(int32_t x, int lane) { return __builtin_amdgcn_readlane(x, lane); }
@@ -176,6 +227,15 @@ template <int PAIR> struct EllRow {
 template <> struct EllRow<2> {
   uint32_t ent[8];
   uint32_t var;
+};
+
+// The draws of one lane's replicas at one position: the interpolated
+// ln-table value of each replica's draw (row 0, or row 1 after a row-0
+// bucket 0), and the replicas whose row-1 bucket is 0 as well (the rest of
+// the row walk runs only for those, on an uphill move).
+struct Draw {
+  double val[8];
+  uint32_t deep;
 };
 
 // A pointer every lane of the wave holds, moved to scalar registers.
@@ -256,25 +316,27 @@ __device__ __forceinline__ void accumulate(uint32_t wv, CT cc, CT (&h)[8]) {
   }
 }
 
-// Local field of the replicas of byte lane g at position q:
-//   Ising: h_b = sum_e c_e * (-1)^(parity of the other members, replica b)
-//   PUBO : h_b = sum_e c_e * [all other members are 1 in replica b]
-// Sums run over the variable's incident terms in term order, as the
-// reference's calculate_cost_difference does.  Pairwise problems read the
-// prefetched ELL row (entries beyond the first 8 are loaded here).
+// Pairwise rows: the spin bytes of the first 8 entries' neighbours (LDS
+// reads issued here, consumed by pair_field).
+template <int PAIR>
+__device__ __forceinline__ void pair_gather(const lds_u8* sp, const EllRow<PAIR>& e, int G, int g,
+                                            uint32_t (&wv)[8]) {
+#pragma unroll
+  for (int j = 0; j < 8; ++j)
+    wv[j] = sp[lds_at(PAIR == 2 ? (uint32_t)(e.ent[j] & 0xFFFFFFu) : (uint32_t)e.ent[j], G, g)];
+}
+
+// The local field of a pairwise row from its gathered neighbour bytes.
 template <int KIND, int PAIR, typename CT, int NB>
-__device__ __forceinline__ void local_field(const UnitDev& U, const Rows& R, const lds_u8* sp, int q,
-                                            const EllRow<PAIR>& e, int4 slice, int tp, int lg, int G, int g,
-                                            CT (&h)[8], int sub = 0, int lgs = 0) {
+__device__ __forceinline__ void pair_field(const Rows& R, const lds_u8* sp, const EllRow<PAIR>& e,
+                                           const uint32_t (&wv)[8], int4 slice, int tp, int lg, int G, int g,
+                                           CT (&h)[8]) {
 #pragma unroll
   for (int b = 0; b < NB; ++b) h[b] = CT(0);
   if constexpr (PAIR == 2) {
     // int8 entries: position in the low 24 bits, the coefficient in the high
     // 8 (entries past the width belong to the next slice and are masked)
     static_assert(std::is_same<CT, int32_t>::value, "int8 entries carry integer coefficients");
-    uint32_t wv[8];
-#pragma unroll
-    for (int j = 0; j < 8; ++j) wv[j] = sp[lds_at(e.ent[j] & 0xFFFFFFu, G, g)];
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
       const int32_t c = j < slice.y ? (int32_t)e.ent[j] >> 24 : 0;
@@ -287,12 +349,7 @@ __device__ __forceinline__ void local_field(const UnitDev& U, const Rows& R, con
         accumulate<KIND, CT, NB>(sp[lds_at(t & 0xFFFFFFu, G, g)], (int32_t)t >> 24, h);
       }
     }
-    return;
-  }
-  if (PAIR) {
-    uint32_t wv[8];
-#pragma unroll
-    for (int j = 0; j < 8; ++j) wv[j] = sp[lds_at((uint32_t)e.ent[j], G, g)];
+  } else {
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
       // a zero coefficient past the width adds nothing (at most the sign of a zero field)
@@ -306,8 +363,27 @@ __device__ __forceinline__ void local_field(const UnitDev& U, const Rows& R, con
         accumulate<KIND, CT, NB>(sp[lds_at((uint32_t)t, G, g)], coef_of<CT>((uint32_t)(t >> 32)), h);
       }
     }
+  }
+}
+
+// Local field of the replicas of byte lane g at position q:
+//   Ising: h_b = sum_e c_e * (-1)^(parity of the other members, replica b)
+//   PUBO : h_b = sum_e c_e * [all other members are 1 in replica b]
+// Sums run over the variable's incident terms in term order, as the
+// reference's calculate_cost_difference does.  Pairwise problems read the
+// prefetched ELL row (entries beyond the first 8 are loaded here).
+template <int KIND, int PAIR, typename CT, int NB>
+__device__ __forceinline__ void local_field(const UnitDev& U, const Rows& R, const lds_u8* sp, int q,
+                                            const EllRow<PAIR>& e, int4 slice, int tp, int lg, int G, int g,
+                                            CT (&h)[8], int sub = 0, int lgs = 0) {
+  if constexpr (PAIR != 0) {
+    uint32_t wv[8];
+    pair_gather<PAIR>(sp, e, G, g, wv);
+    pair_field<KIND, PAIR, CT, NB>(R, sp, e, wv, slice, tp, lg, G, g, h);
     return;
   }
+#pragma unroll
+  for (int b = 0; b < NB; ++b) h[b] = CT(0);
   if (R.ent4) {
     // sliced entries {coefficient, up to 3 other members} in term order;
     // the ghost position pads missing members (its spin is always 0).  With
@@ -626,17 +702,15 @@ k_sweep(const UnitDev* units, SweepArgs a) {
             fetch_csr(here ? c0 : c0n, here ? k : k - K, e);
           }
         };
-        // the decision for the replicas of this lane at position q (round
-        // k) from their local fields h
-        auto decide = [&](int k, int q, bool valid, const CT (&h)[8], uint32_t var) {
-          const uint32_t own = sp[lds_at(q, G, g)];
-          CT dE[8];
+        // The draws of the replicas of this lane at the position of `var`
+        // and their interpolated ln-table values: independent of the field
+        // and of beta, so they are formed while the neighbour bytes are read.
+        auto draw = [&](uint32_t var, Draw& d) {
           uint4 quad = make_uint4(0, 0, 0, 0);
-          uint32_t flips = 0, deep = 0;
+          d.deep = 0;
 #pragma unroll
           for (int b = 0; b < NB; ++b) {
             const uint32_t r = rbase + (uint32_t)b;
-            dE[b] = delta_from_field<KIND, CT>(h[b], (own >> b) & 1u);
             // one draw serves a replica pair: words 0, 1 = their row-0
             // draws, words 2, 3 = their row-1 draws (used after bucket 0)
             if (b == 0 || (r & 1u) == 0u)
@@ -644,9 +718,22 @@ k_sweep(const UnitDev* units, SweepArgs a) {
             const uint32_t w0 = (r & 1u) ? quad.y : quad.x;
             const uint32_t w1 = (r & 1u) ? quad.w : quad.z;
             const bool z0 = (w0 & 255u) == 0u;
+            d.val[b] = row_value(z0 ? w1 : w0, z0 ? 1 : 0, lut);
+            d.deep |= (z0 && (w1 & 255u) == 0u) ? 1u << b : 0u;
+          }
+        };
+        // the decision for the replicas of this lane at position q (round
+        // k) from their local fields h and draws d
+        auto decide = [&](int k, int q, bool valid, const CT (&h)[8], const Draw& d, uint32_t var) {
+          const uint32_t own = sp[lds_at(q, G, g)];
+          CT dE[8];
+          uint32_t flips = 0, deep = 0;
+#pragma unroll
+          for (int b = 0; b < NB; ++b) {
+            dE[b] = delta_from_field<KIND, CT>(h[b], (own >> b) & 1u);
             const bool live = valid && b < nb_g;
-            const bool ok = accept_row((double)dE[b], beta[b], z0 ? w1 : w0, z0 ? 1 : 0, lut);
-            const bool dp = z0 && (w1 & 255u) == 0u && dE[b] > CT(0);
+            const bool ok = accept_value((double)dE[b], beta[b], d.val[b]);
+            const bool dp = ((d.deep >> b) & 1u) && dE[b] > CT(0);
             flips |= (live && ok && !dp) ? 1u << b : 0u;
             deep |= (live && dp) ? 1u << b : 0u;
           }
@@ -691,15 +778,24 @@ k_sweep(const UnitDev* units, SweepArgs a) {
           const bool valid = rho < nc;
           const int q = valid ? c0 + rho : N;
           CT h[8];
-          local_field<KIND, PAIR, CT, NB>(U, rows, sp, q, e, sl, tp >> lgs, lg, G, g, h, sub, lgs);
-          if (kSplit && lgs) {
-            // every lane of a position is active here (they share rho)
+          Draw d;
+          if constexpr (PAIR != 0) {
+            uint32_t wv[8];
+            pair_gather<PAIR>(sp, e, G, g, wv);
+            draw(e.var, d);
+            pair_field<KIND, PAIR, CT, NB>(rows, sp, e, wv, sl, tp, lg, G, g, h);
+          } else {
+            local_field<KIND, PAIR, CT, NB>(U, rows, sp, q, e, sl, tp >> lgs, lg, G, g, h, sub, lgs);
+            if (kSplit && lgs) {
+              // every lane of a position is active here (they share rho)
 #pragma unroll
-            for (int b = 0; b < NB; ++b)
-              for (int o = 1; o < (1 << lgs); o <<= 1) h[b] += __shfl_xor(h[b], o);
-            if (sub) return;
+              for (int b = 0; b < NB; ++b)
+                for (int o = 1; o < (1 << lgs); o <<= 1) h[b] += __shfl_xor(h[b], o);
+              if (sub) return;
+            }
+            draw(e.var, d);
           }
-          decide(k, q, valid, h, e.var);
+          decide(k, q, valid, h, d, e.var);
         };
         if (kEarly && c == 0 && early_ready) {
           // fields computed during the last exchange; the row pipeline
@@ -709,7 +805,10 @@ k_sweep(const UnitDev* units, SweepArgs a) {
 #pragma unroll
             for (int b = 0; b < NB; ++b) h[b] = hpre[(k * NB + b) * nthreads + tid];
             const int rho = (k << lg) + tp;
-            decide(k, rho < nc ? c0 + rho : N, rho < nc, h, vpre[k * nthreads + tid]);
+            const uint32_t var = vpre[k * nthreads + tid];
+            Draw d;
+            draw(var, d);
+            decide(k, rho < nc ? c0 + rho : N, rho < nc, h, d, var);
           }
         } else {
           // Fetches past the last round read sentinel slices and padding
@@ -733,23 +832,31 @@ k_sweep(const UnitDev* units, SweepArgs a) {
         // all earlier lanes: per wave, an exclusive scan of the sums plus each
         // lane's minimum prefix, minimised (first lane on ties); the waves
         // are combined in order below.
+        // (All NB replicas of the lane are scanned together, absent ones
+        // with zero sums, so the chains interleave; only present ones store.)
+        AT incl[8], mv[8], mn[8];
+#pragma unroll
+        for (int b = 0; b < NB; ++b) incl[b] = Ssum[b];
+        scan_add_n<AT, NB>(incl);
 #pragma unroll
         for (int b = 0; b < NB; ++b) {
-          if (b < nb_g) {
-            const AT incl = scan_add<AT>(Ssum[b]);
-            const AT excl = dpp<DPP_WAVE_SHR1, 0xf>(incl, AT(0));
-            const AT mv = excl + Smin[b];
-            const AT m = lane_value(scan_min<AT>(mv), 63);
-            const unsigned long long hit = __builtin_amdgcn_ballot_w64(mv == m);
-            const int wl = (int)__builtin_ctzll(hit);
-            const int wk = __builtin_amdgcn_readlane(kmin[b], wl);
-            const AT tot = lane_value(incl, 63);
-            if (lane == 0) {
-              Slots<AT>::sum(S, par, g, b)[wave_in_group] = tot;
-              Slots<AT>::mn(S, par, g, b)[wave_in_group] = m;
-              S.wlane[par][g][b][wave_in_group] = wl;
-              S.wk[par][g][b][wave_in_group] = wk;
-            }
+          const AT excl = dpp_z<DPP_WAVE_SHR1>(incl[b]);
+          mv[b] = excl + Smin[b];
+          mn[b] = mv[b];
+        }
+        scan_min_n<AT, NB>(mn);
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+          const AT m = lane_value(mn[b], 63);
+          const unsigned long long hit = __builtin_amdgcn_ballot_w64(mv[b] == m);
+          const int wl = (int)__builtin_ctzll(hit);
+          const int wk = __builtin_amdgcn_readlane(kmin[b], wl);
+          const AT tot = lane_value(incl[b], 63);
+          if (lane == 0 && b < nb_g) {
+            Slots<AT>::sum(S, par, g, b)[wave_in_group] = tot;
+            Slots<AT>::mn(S, par, g, b)[wave_in_group] = m;
+            S.wlane[par][g][b][wave_in_group] = wl;
+            S.wk[par][g][b][wave_in_group] = wk;
           }
         }
       }
@@ -757,17 +864,31 @@ k_sweep(const UnitDev* units, SweepArgs a) {
       if (tid < nrep) {
         const int gg = tid >> 3, bb = tid & 7;
         const int W = TP >> 6;
+        // every wave's slots read at once (one LDS round trip), then
+        // combined in wave order; slots of absent waves are read, not used
+        AT mnv[kMaxWavesPerGroup], smv[kMaxWavesPerGroup];
+        int wlv[kMaxWavesPerGroup], wkv[kMaxWavesPerGroup];
+#pragma unroll
+        for (int w = 0; w < kMaxWavesPerGroup; ++w) {
+          mnv[w] = Slots<AT>::mn(S, par, gg, bb)[w];
+          smv[w] = Slots<AT>::sum(S, par, gg, bb)[w];
+          wlv[w] = S.wlane[par][gg][bb][w];
+          wkv[w] = S.wk[par][gg][bb][w];
+        }
         AT run = AT(0), bestc = AT(0);
         int bt = -1, bk = -1;
-        for (int w = 0; w < W; ++w) {
-          const AT cnd = run + Slots<AT>::mn(S, par, gg, bb)[w];
-          if (bt < 0 || cnd < bestc) {
-            bestc = cnd;
-            // the lane's position (lanes share positions in split classes)
-            bt = (w * 64 + S.wlane[par][gg][bb][w]) >> lgs;
-            bk = S.wk[par][gg][bb][w];
+#pragma unroll
+        for (int w = 0; w < kMaxWavesPerGroup; ++w) {
+          if (w < W) {
+            const AT cnd = run + mnv[w];
+            if (bt < 0 || cnd < bestc) {
+              bestc = cnd;
+              // the lane's position (lanes share positions in split classes)
+              bt = (w * 64 + wlv[w]) >> lgs;
+              bk = wkv[w];
+            }
+            run += smv[w];
           }
-          run += Slots<AT>::sum(S, par, gg, bb)[w];
         }
         const double before = S.rcost[tid];
         S.rcost[tid] = before + (double)run;
