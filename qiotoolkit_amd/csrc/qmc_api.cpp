@@ -298,6 +298,9 @@ struct EllLayout {
   // coefficient << 24} (int8 set) instead of `ent`
   DevBuf<uint32_t> ent32;
   bool int8 = false;
+  // pairwise: the smallest of 4, 6, 8 entries covering every slice's width
+  // (8 when wider slices exist; their further entries are read per row)
+  int row_e = 8;
   DevBuf<uint4> ent4;  // general problems with at most 3 other members per term
 };
 
@@ -374,6 +377,7 @@ struct qmc_walkers_s {
   int force_graph = 0;       // split ladders through graph-replayed launches instead
   int nb = 8;                // compile-time replicas per byte lane (1, 2, 4, 8)
   int pair = 0;              // kernel form: 0 general terms, 1 pairwise 8-byte, 2 pairwise int8 entries
+  int row_e = 8;             // pairwise entries per row read ahead (4, 6, 8)
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
   int graph_key[4] = {0, 0, 0, 0};
@@ -417,7 +421,7 @@ struct qmc_batch_s {
   bool dirty = true;
   // one launch group per kernel instantiation
   struct Group {
-    int kind, pairwise, coupling, nb, coop, ladder, ordered;
+    int kind, pairwise, coupling, nb, row_e, coop, ladder, ordered;
     int block = 64;
     int stride = 0;
     double work = 0;  // attempts planned (for spreading groups over streams)
@@ -872,6 +876,7 @@ static EllLayout* get_layout(qmc_problem p, int lanes, bool wide = false) {
       ent.push_back(make_uint2(pos, bits(c)));
   };
   auto n_ent = [&]() { return (int32_t)(compact ? ent32.size() : ent.size()); };
+  int32_t widest = 0;
   for (int c = 0; c < nclass; ++c) {
     const int32_t c0 = p->class_off[c], nc = p->class_off[c + 1] - c0;
     sbase[c] = (int32_t)slice.size();
@@ -881,8 +886,11 @@ static EllLayout* get_layout(qmc_problem p, int lanes, bool wide = false) {
         const int32_t q = c0 + k * lanes + t;
         w = std::max(w, p->h_inc_off[q + 1] - p->h_inc_off[q]);
       }
+      widest = std::max(widest, w);
       slice.push_back(make_int4(n_ent(), w, c0 + k * lanes, 0));
-      for (int32_t e = 0; e < w; ++e)
+      // at least 8 entries stored: the kernels read their first 4, 6 or 8
+      // without masks, and the entries past the width add zero
+      for (int32_t e = 0; e < std::max(w, 8); ++e)
         for (int32_t t = 0; t < lanes; ++t) {
           const int32_t rho = k * lanes + t;
           bool real = false;
@@ -899,8 +907,9 @@ static EllLayout* get_layout(qmc_problem p, int lanes, bool wide = false) {
     }
   }
   sbase[nclass] = (int32_t)slice.size();
+  L->row_e = widest <= 4 ? 4 : (widest <= 6 ? 6 : 8);
   // the table's closing copies, and 8 rows of tail padding: the sweep
-  // always reads 8 rows of a slice
+  // reads up to 8 rows of a slice
   close_table(n_ent());
   for (int64_t i = 0; i < (int64_t)8 * lanes + 1; ++i) put(ghost, 0.0);
   bool ok = L->slice_base.upload(sbase.data(), sbase.size()) == hipSuccess &&
@@ -922,6 +931,7 @@ static void build_units(qmc_walkers w) {
   w->units.assign(w->n_units, UnitDev{});
   EllLayout* ell = (p->pairwise || p->max_others <= 3) ? get_layout(p, w->lanes) : nullptr;
   w->pair = !p->pairwise ? 0 : (ell && ell->int8 ? 2 : 1);
+  w->row_e = (p->pairwise && ell) ? ell->row_e : 8;
   for (int u = 0; u < w->n_units; ++u) {
     UnitDev& U = w->units[u];
     if (ell) {
@@ -1519,8 +1529,8 @@ static int launch_units(qmc_walkers w, const UnitDev* d_units, int block, int32_
   a.lut = p->ctx->lut.p;
   a.unit_of_block = nullptr;
   const size_t smem = sweep_launch_smem(w->stride, ordered, block, nb, coop ? pre_rounds : 0, a);
-  HIP_TRY(launch_sweep(p->kind, ordered ? p->pairwise : w->pair, p->coupling, nb, d_units, n_units, block, smem, a, ordered,
-                       coop, ladder, st));
+  HIP_TRY(launch_sweep(p->kind, ordered ? p->pairwise : w->pair, p->coupling, nb, w->row_e, d_units, n_units, block,
+                       smem, a, ordered, coop, ladder, st));
   return QMC_OK;
 }
 
@@ -1678,7 +1688,7 @@ int qmc_walkers_pt(qmc_walkers w, int32_t ladder, const double* ladder_betas, in
                       ? (p->class_off[1] - p->class_off[0] + w->lanes - 1) / w->lanes : 0;
   const size_t smem = sweep_launch_smem(w->stride, 0, w->lanes * w->G, w->nb, pre, sa);
   const bool can_coop = w->B != ladder && ladder % w->B == 0 &&
-      w->n_units <= max_coresident_blocks(p->kind, w->pair, p->coupling, w->nb, w->lanes * w->G, smem);
+      w->n_units <= max_coresident_blocks(p->kind, w->pair, p->coupling, w->nb, w->row_e, w->lanes * w->G, smem);
   if (!w->pt_obs.log && (w->B == ladder || (can_coop && !w->force_graph))) {
     if ((rc = qmc_walkers_plan_pt(w, ladder, ladder_betas, step0, n_steps, sweeps_per_replica))) return rc;
     const UnitDev* du = nullptr;
@@ -1841,8 +1851,8 @@ static int batch_rebuild(qmc_batch b) {
     qmc_batch_s::Group* g = nullptr;
     for (auto& gg : b->groups)
       if (gg->kind == p->kind && gg->pairwise == (w->ordered ? p->pairwise : w->pair) && gg->coupling == p->coupling &&
-          gg->nb == w->nb && gg->coop == w->coop && gg->ladder == (w->ladder > 0) && gg->block == block &&
-          gg->ordered == w->ordered)
+          gg->nb == w->nb && gg->row_e == w->row_e && gg->coop == w->coop && gg->ladder == (w->ladder > 0) &&
+          gg->block == block && gg->ordered == w->ordered)
         g = gg.get();
     if (!g) {
       b->groups.push_back(std::make_unique<qmc_batch_s::Group>());
@@ -1851,6 +1861,7 @@ static int batch_rebuild(qmc_batch b) {
       g->pairwise = w->ordered ? p->pairwise : w->pair;
       g->coupling = p->coupling;
       g->nb = w->nb;
+      g->row_e = w->row_e;
       g->coop = w->coop;
       g->ladder = w->ladder > 0;
       g->ordered = w->ordered;
@@ -1869,7 +1880,7 @@ static int batch_rebuild(qmc_batch b) {
       // never splitting a ladder (its work-groups are adjacent)
       SweepArgs sa{};
       const size_t smem = sweep_launch_smem(g->stride, g->ordered, g->block, g->nb, g->pre_rounds, sa);
-      const int cap = max_coresident_blocks(g->kind, g->pairwise, g->coupling, g->nb, g->block, smem);
+      const int cap = max_coresident_blocks(g->kind, g->pairwise, g->coupling, g->nb, g->row_e, g->block, smem);
       const int n = (int)g->units.size();
       g->chunk.assign(1, 0);
       g->chunk_err = false;
@@ -1909,16 +1920,16 @@ static int launch_group(qmc_batch_s::Group& g, int32_t max_sweeps, const double*
   if (smem > 160 * 1024) return fail(QMC_MEMORY_LIMITED, "batch LDS footprint exceeds 160 KiB");
   const int n = (int)g.units.size();
   if (!g.coop) {
-    HIP_TRY(launch_sweep(g.kind, g.pairwise, g.coupling, g.nb, g.d_units.p, n, g.block, smem, a, g.ordered, 0,
-                         g.ladder, st));
+    HIP_TRY(launch_sweep(g.kind, g.pairwise, g.coupling, g.nb, g.row_e, g.d_units.p, n, g.block, smem, a, g.ordered,
+                         0, g.ladder, st));
     return QMC_OK;
   }
   // Cooperative ladders: the chunks planned when the group was built.
   if (g.chunk_err) return fail(QMC_MEMORY_LIMITED, "a ladder exceeds the co-resident work-group limit");
   for (size_t i = 0; i + 1 < g.chunk.size(); ++i) {
     const int start = g.chunk[i], end = g.chunk[i + 1];
-    HIP_TRY(launch_sweep(g.kind, g.pairwise, g.coupling, g.nb, g.d_units.p + start, end - start, g.block,
-                         smem, a, 0, 1, 1, st));
+    HIP_TRY(launch_sweep(g.kind, g.pairwise, g.coupling, g.nb, g.row_e, g.d_units.p + start, end - start,
+                         g.block, smem, a, 0, 1, 1, st));
   }
   return QMC_OK;
 }

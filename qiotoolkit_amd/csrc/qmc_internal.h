@@ -39,7 +39,7 @@ struct UnitDev {
   // last class's end is round k of the next sweep's first class); entry e
   // of lane t is ell_ent[offset + e*lanes + t] = {neighbour position,
   // coefficient bits (int32 or fp32)}, padded with the ghost position and a
-  // zero coefficient
+  // zero coefficient; a slice stores max(width, 8) entries per lane
   const int32_t* ell_slice_base;
   const int4* ell_slice;
   const uint2* ell_ent;
@@ -204,11 +204,13 @@ inline size_t sweep_pre_bytes(int rounds, int block, int nb) {
 // bytes; *best_lds (optional) tells whether the lowest states fit in it too.
 size_t sweep_smem_bytes(int max_stride, int ordered, int* best_lds = nullptr);
 // pairwise: 0 general terms, 1 pairwise 8-byte entries, 2 pairwise int8 4-byte entries
-hipError_t launch_sweep(int kind, int pairwise, int coupling, int nb, const UnitDev* units, int n_units,
+// row_e: entries per pairwise row read ahead by the kernel (4, 6 or 8; any
+// value is valid for every layout, since slices store at least 8 entries)
+hipError_t launch_sweep(int kind, int pairwise, int coupling, int nb, int row_e, const UnitDev* units, int n_units,
                         int block, size_t smem, const SweepArgs& a, int ordered, int cooperative, int ladder,
                         hipStream_t st);
 // largest grid of cooperative sweep work-groups that can be co-resident
-int max_coresident_blocks(int kind, int pairwise, int coupling, int nb, int block, size_t smem);
+int max_coresident_blocks(int kind, int pairwise, int coupling, int nb, int row_e, int block, size_t smem);
 // Per-step observations of graph-replayed parallel tempering
 // (parallel_tempering.h:141-188): for ladder ld, step s and ladder position
 // k, log[((ld * log_steps + s) * L + k) * 4 + {0,1,2,3}] = cost after the

@@ -206,8 +206,11 @@ using lds_u8 = __attribute__((address_space(3))) uint8_t;
 using lds_u32 = __attribute__((address_space(3))) uint32_t;
 // Byte index of position q in byte lane g of a unit's LDS spin array: q * G +
 // g with a full-rate 24-bit multiply-add (q < 2^24: a spin array in LDS has
-// fewer than 160 Ki bytes).
+// fewer than 160 Ki bytes); units of at most 4 replicas (NB < 8) have one
+// byte lane, so the index is q itself.
+template <int NB>
 __device__ __forceinline__ uint32_t lds_at(uint32_t q, int G, int g) {
+  if constexpr (NB < 8) return q;
   return __umul24(q, (uint32_t)G) + (uint32_t)g;
 }
 
@@ -270,23 +273,25 @@ __device__ __forceinline__ int4 load_slice(const Rows& R, int s) {
 }
 
 // slice = {offset, width} (uniform); lane tp of a slice with `lanes` = 1 << lg.
-// Always 8 loads, without branches, so the compiler can keep them in flight
-// across the processing of the previous row: entries past the slice width
-// belong to the next slice (or the tail padding) and are masked at use.
-template <int PAIR>
+// Always E loads (the row width of the kernel: 4, 6 or 8), without
+// branches, so the compiler can keep them in flight across the processing of
+// the previous row.  Every slice stores at least 8 entries (zero
+// coefficients on the ghost position past its width), so the first E are
+// always the slice's own and need no mask.
+template <int PAIR, int E>
 __device__ __forceinline__ void ell_fetch(const Rows& R, int4 slice, int tp, int lg, EllRow<PAIR>& e) {
   const int step = 1 << lg;
   if constexpr (PAIR == 2) {
     gptr<uint32_t> p = R.ent32 + slice.x + tp;
 #pragma unroll
-    for (int j = 0; j < 8; ++j) {
+    for (int j = 0; j < E; ++j) {
       e.ent[j] = *p;
       p += step;
     }
   } else {
     gptr<uint64_t> p = R.ent + slice.x + tp;
 #pragma unroll
-    for (int j = 0; j < 8; ++j) {
+    for (int j = 0; j < E; ++j) {
       e.ent[j] = *p;
       p += step;
     }
@@ -316,51 +321,46 @@ __device__ __forceinline__ void accumulate(uint32_t wv, CT cc, CT (&h)[8]) {
   }
 }
 
-// Pairwise rows: the spin bytes of the first 8 entries' neighbours (LDS
+// Pairwise rows: the spin bytes of the first E entries' neighbours (LDS
 // reads issued here, consumed by pair_field).
-template <int PAIR>
+template <int PAIR, int NB, int E>
 __device__ __forceinline__ void pair_gather(const lds_u8* sp, const EllRow<PAIR>& e, int G, int g,
                                             uint32_t (&wv)[8]) {
 #pragma unroll
-  for (int j = 0; j < 8; ++j)
-    wv[j] = sp[lds_at(PAIR == 2 ? (uint32_t)(e.ent[j] & 0xFFFFFFu) : (uint32_t)e.ent[j], G, g)];
+  for (int j = 0; j < E; ++j)
+    wv[j] = sp[lds_at<NB>(PAIR == 2 ? (uint32_t)(e.ent[j] & 0xFFFFFFu) : (uint32_t)e.ent[j], G, g)];
 }
 
-// The local field of a pairwise row from its gathered neighbour bytes.
-template <int KIND, int PAIR, typename CT, int NB>
+// The local field of a pairwise row from its gathered neighbour bytes (the
+// first E entries; entries from E to the width are loaded here).  Padding
+// entries have zero coefficients: they add nothing (at most the sign of a
+// zero field, which no decision depends on).
+template <int KIND, int PAIR, typename CT, int NB, int E>
 __device__ __forceinline__ void pair_field(const Rows& R, const lds_u8* sp, const EllRow<PAIR>& e,
                                            const uint32_t (&wv)[8], int4 slice, int tp, int lg, int G, int g,
                                            CT (&h)[8]) {
 #pragma unroll
   for (int b = 0; b < NB; ++b) h[b] = CT(0);
   if constexpr (PAIR == 2) {
-    // int8 entries: position in the low 24 bits, the coefficient in the high
-    // 8 (entries past the width belong to the next slice and are masked)
+    // int8 entries: position in the low 24 bits, the coefficient in the high 8
     static_assert(std::is_same<CT, int32_t>::value, "int8 entries carry integer coefficients");
 #pragma unroll
-    for (int j = 0; j < 8; ++j) {
-      const int32_t c = j < slice.y ? (int32_t)e.ent[j] >> 24 : 0;
-      accumulate<KIND, CT, NB>(wv[j], c, h);
-    }
-    if (slice.y > 8) {
+    for (int j = 0; j < E; ++j) accumulate<KIND, CT, NB>(wv[j], (int32_t)e.ent[j] >> 24, h);
+    if (slice.y > E) {
       gptr<uint32_t> p = R.ent32 + slice.x + tp;
-      for (int j = 8; j < slice.y; ++j) {
+      for (int j = E; j < slice.y; ++j) {
         const uint32_t t = p[j << lg];
-        accumulate<KIND, CT, NB>(sp[lds_at(t & 0xFFFFFFu, G, g)], (int32_t)t >> 24, h);
+        accumulate<KIND, CT, NB>(sp[lds_at<NB>(t & 0xFFFFFFu, G, g)], (int32_t)t >> 24, h);
       }
     }
   } else {
 #pragma unroll
-    for (int j = 0; j < 8; ++j) {
-      // a zero coefficient past the width adds nothing (at most the sign of a zero field)
-      const uint32_t cb = j < slice.y ? (uint32_t)(e.ent[j] >> 32) : 0u;
-      accumulate<KIND, CT, NB>(wv[j], coef_of<CT>(cb), h);
-    }
-    if (slice.y > 8) {
+    for (int j = 0; j < E; ++j) accumulate<KIND, CT, NB>(wv[j], coef_of<CT>((uint32_t)(e.ent[j] >> 32)), h);
+    if (slice.y > E) {
       gptr<uint64_t> p = R.ent + slice.x + tp;
-      for (int j = 8; j < slice.y; ++j) {
+      for (int j = E; j < slice.y; ++j) {
         const uint64_t t = p[j << lg];
-        accumulate<KIND, CT, NB>(sp[lds_at((uint32_t)t, G, g)], coef_of<CT>((uint32_t)(t >> 32)), h);
+        accumulate<KIND, CT, NB>(sp[lds_at<NB>((uint32_t)t, G, g)], coef_of<CT>((uint32_t)(t >> 32)), h);
       }
     }
   }
@@ -372,14 +372,14 @@ __device__ __forceinline__ void pair_field(const Rows& R, const lds_u8* sp, cons
 // Sums run over the variable's incident terms in term order, as the
 // reference's calculate_cost_difference does.  Pairwise problems read the
 // prefetched ELL row (entries beyond the first 8 are loaded here).
-template <int KIND, int PAIR, typename CT, int NB>
+template <int KIND, int PAIR, typename CT, int NB, int E>
 __device__ __forceinline__ void local_field(const UnitDev& U, const Rows& R, const lds_u8* sp, int q,
                                             const EllRow<PAIR>& e, int4 slice, int tp, int lg, int G, int g,
                                             CT (&h)[8], int sub = 0, int lgs = 0) {
   if constexpr (PAIR != 0) {
     uint32_t wv[8];
-    pair_gather<PAIR>(sp, e, G, g, wv);
-    pair_field<KIND, PAIR, CT, NB>(R, sp, e, wv, slice, tp, lg, G, g, h);
+    pair_gather<PAIR, NB, E>(sp, e, G, g, wv);
+    pair_field<KIND, PAIR, CT, NB, E>(R, sp, e, wv, slice, tp, lg, G, g, h);
     return;
   }
 #pragma unroll
@@ -393,9 +393,9 @@ __device__ __forceinline__ void local_field(const UnitDev& U, const Rows& R, con
     const int step = 2 << lg;
     auto entry = [&](int j) {
       const uint64_t lo = p[j * step], hi = p[j * step + 1];
-      const uint32_t w0 = sp[lds_at((uint32_t)(lo >> 32), G, g)];
-      const uint32_t w1 = sp[lds_at((uint32_t)hi, G, g)];
-      const uint32_t w2 = sp[lds_at((uint32_t)(hi >> 32), G, g)];
+      const uint32_t w0 = sp[lds_at<NB>((uint32_t)(lo >> 32), G, g)];
+      const uint32_t w1 = sp[lds_at<NB>((uint32_t)hi, G, g)];
+      const uint32_t w2 = sp[lds_at<NB>((uint32_t)(hi >> 32), G, g)];
       const uint32_t w = (KIND == QMC_ISING) ? (w0 ^ w1 ^ w2) : (w0 | w1 | w2);
       accumulate<KIND, CT, NB>(w, coef_of<CT>((uint32_t)lo), h);
     };
@@ -418,7 +418,7 @@ __device__ __forceinline__ void local_field(const UnitDev& U, const Rows& R, con
     uint32_t w = 0;
     const int o0 = ooff[e], o1 = ooff[e + 1];
     for (int o = o0; o < o1; ++o) {
-      const uint32_t x = sp[lds_at(opos[o], G, g)];
+      const uint32_t x = sp[lds_at<NB>(opos[o], G, g)];
       w = (KIND == QMC_ISING) ? (w ^ x) : (w | x);
     }
     accumulate<KIND, CT, NB>(w, cc, h);
@@ -491,8 +491,9 @@ __device__ __forceinline__ double load_agent(const double* p) {
 
 // LADDER: the launch may hold parallel tempering ladders (exchanges in the
 // kernel); without it the exchange code, and the registers it needs, are
-// compiled out.
-template <int KIND, int PAIR, typename CT, int NB, bool LADDER>
+// compiled out.  E: entries of a pairwise row fetched ahead (4, 6 or 8; the
+// smallest that covers every slice's width, else 8 and the rest per row).
+template <int KIND, int PAIR, typename CT, int NB, bool LADDER, int E>
 __global__ void __launch_bounds__(kBlock)
 k_sweep(const UnitDev* units, SweepArgs a) {
   using AT = typename AccT<CT>::T;
@@ -501,7 +502,8 @@ k_sweep(const UnitDev* units, SweepArgs a) {
   // The descriptor stays in memory (scalar loads at each use): fields needed
   // only at the start and end are not held in registers across the sweeps.
   const UnitDev& U = units[uni(a.unit_of_block ? a.unit_of_block[blockIdx.x] : (int)blockIdx.x)];
-  const int N = U.N, G = U.G, TP = U.lanes, lg = U.lanes_log2;
+  // at most 4 replicas per unit (NB < 8) means a single byte group
+  const int N = U.N, G = NB < 8 ? 1 : U.G, TP = U.lanes, lg = U.lanes_log2;
   const int stride = U.stride;
   cptr<int32_t> class_off = cnst(U.class_off);
   cptr<int32_t> slice_base = cnst(U.ell_slice_base);
@@ -517,7 +519,7 @@ k_sweep(const UnitDev* units, SweepArgs a) {
   const int tid = threadIdx.x;
   const int nthreads = blockDim.x;
   const bool active = tid < TP * G;
-  const int g = active ? tid >> lg : 0;
+  const int g = (NB < 8 || !active) ? 0 : tid >> lg;
   const int tp = tid - (g << lg);
   const int lane = tid & 63;
   const int wave_in_group = tp >> 6;
@@ -604,7 +606,7 @@ k_sweep(const UnitDev* units, SweepArgs a) {
   // no class lookup.
   auto fetch_slice = [&](int sidx, int4& sl, EllRow<PAIR>& e) {
     sl = load_slice(rows, sidx);
-    if (PAIR) ell_fetch(rows, sl, tp, lg, e);
+    if (PAIR) ell_fetch<PAIR, E>(rows, sl, tp, lg, e);
     e.var = (uint32_t)var_of_pos[sl.z + (tp >> sl.w)];  // padded past position N - 1
   };
   // problems without slices (terms of more than 4 variables): position of
@@ -634,7 +636,7 @@ k_sweep(const UnitDev* units, SweepArgs a) {
     const int rho = (k << lg) + tp;
     const int q = rho < nc0 ? class_off[0] + rho : N;
     CT h[8];
-    local_field<KIND, PAIR, CT, NB>(U, rows, sp, q, e, sl, tp, lg, G, g, h);
+    local_field<KIND, PAIR, CT, NB, E>(U, rows, sp, q, e, sl, tp, lg, G, g, h);
 #pragma unroll
     for (int b = 0; b < NB; ++b) hpre[(k * NB + b) * nthreads + tid] = h[b];
     vpre[k * nthreads + tid] = e.var;
@@ -725,7 +727,7 @@ k_sweep(const UnitDev* units, SweepArgs a) {
         // the decision for the replicas of this lane at position q (round
         // k) from their local fields h and draws d
         auto decide = [&](int k, int q, bool valid, const CT (&h)[8], const Draw& d, uint32_t var) {
-          const uint32_t own = sp[lds_at(q, G, g)];
+          const uint32_t own = sp[lds_at<NB>(q, G, g)];
           CT dE[8];
           uint32_t flips = 0, deep = 0;
 #pragma unroll
@@ -764,8 +766,8 @@ k_sweep(const UnitDev* units, SweepArgs a) {
             kmin[b] = lower ? k : kmin[b];
             acc[b] += ok ? 1u : 0u;
           }
-          sp[lds_at(q, G, g)] = (uint8_t)(own ^ flips);
-          fl[lds_at(q, G, g)] = (uint8_t)flips;
+          sp[lds_at<NB>(q, G, g)] = (uint8_t)(own ^ flips);
+          fl[lds_at<NB>(q, G, g)] = (uint8_t)flips;
         };
         auto process = [&](int k, const int4& sl, const EllRow<PAIR>& e) {
           const int sub = tp & ((1 << lgs) - 1);
@@ -781,11 +783,11 @@ k_sweep(const UnitDev* units, SweepArgs a) {
           Draw d;
           if constexpr (PAIR != 0) {
             uint32_t wv[8];
-            pair_gather<PAIR>(sp, e, G, g, wv);
+            pair_gather<PAIR, NB, E>(sp, e, G, g, wv);
             draw(e.var, d);
-            pair_field<KIND, PAIR, CT, NB>(rows, sp, e, wv, sl, tp, lg, G, g, h);
+            pair_field<KIND, PAIR, CT, NB, E>(rows, sp, e, wv, sl, tp, lg, G, g, h);
           } else {
-            local_field<KIND, PAIR, CT, NB>(U, rows, sp, q, e, sl, tp >> lgs, lg, G, g, h, sub, lgs);
+            local_field<KIND, PAIR, CT, NB, E>(U, rows, sp, q, e, sl, tp >> lgs, lg, G, g, h, sub, lgs);
             if (kSplit && lgs) {
               // every lane of a position is active here (they share rho)
 #pragma unroll
@@ -2248,7 +2250,7 @@ static hipError_t raise_lds_limit(F fn) {
   return once;
 }
 
-template <int KIND, int PAIR, typename CT, int NB>
+template <int KIND, int PAIR, typename CT, int NB, int E>
 static hipError_t launch_sweep_t(const UnitDev* units, int n_units, int block, size_t smem,
                                  const SweepArgs& a, int ordered, int cooperative, int ladder, hipStream_t st) {
   if (ordered) {
@@ -2258,9 +2260,9 @@ static hipError_t launch_sweep_t(const UnitDev* units, int n_units, int block, s
     hipLaunchKernelGGL(fn, dim3(n_units), dim3(64), smem, st, units, a);
     return hipGetLastError();
   }
-  auto fn = ladder ? k_sweep<KIND, PAIR, CT, NB, true> : k_sweep<KIND, PAIR, CT, NB, false>;
-  hipError_t e = ladder ? raise_lds_limit(k_sweep<KIND, PAIR, CT, NB, true>)
-                        : raise_lds_limit(k_sweep<KIND, PAIR, CT, NB, false>);
+  auto fn = ladder ? k_sweep<KIND, PAIR, CT, NB, true, E> : k_sweep<KIND, PAIR, CT, NB, false, E>;
+  hipError_t e = ladder ? raise_lds_limit(k_sweep<KIND, PAIR, CT, NB, true, E>)
+                        : raise_lds_limit(k_sweep<KIND, PAIR, CT, NB, false, E>);
   if (e != hipSuccess) return e;
   if (cooperative) {
     SweepArgs args = a;
@@ -2272,40 +2274,56 @@ static hipError_t launch_sweep_t(const UnitDev* units, int n_units, int block, s
   return hipGetLastError();
 }
 
-template <int KIND, int PAIR, typename CT>
-static hipError_t launch_nb(int nb, const UnitDev* units, int n_units, int block, size_t smem,
-                            const SweepArgs& a, int ordered, int coop, int ladder, hipStream_t st) {
-  if (nb <= 1) return launch_sweep_t<KIND, PAIR, CT, 1>(units, n_units, block, smem, a, ordered, coop, ladder, st);
-  if (nb <= 2) return launch_sweep_t<KIND, PAIR, CT, 2>(units, n_units, block, smem, a, ordered, coop, ladder, st);
-  if (nb <= 4) return launch_sweep_t<KIND, PAIR, CT, 4>(units, n_units, block, smem, a, ordered, coop, ladder, st);
-  return launch_sweep_t<KIND, PAIR, CT, 8>(units, n_units, block, smem, a, ordered, coop, ladder, st);
+// Rows narrower than 8 entries get kernels of their own for up to 2
+// replicas per byte lane (the lattice shapes); the rest read 8.
+template <int KIND, int PAIR, typename CT, int NB>
+static hipError_t launch_e(int row_e, const UnitDev* units, int n_units, int block, size_t smem,
+                           const SweepArgs& a, int ordered, int coop, int ladder, hipStream_t st) {
+  if constexpr (PAIR != 0 && NB <= 2) {
+    if (row_e == 4)
+      return launch_sweep_t<KIND, PAIR, CT, NB, 4>(units, n_units, block, smem, a, ordered, coop, ladder, st);
+    if (row_e == 6)
+      return launch_sweep_t<KIND, PAIR, CT, NB, 6>(units, n_units, block, smem, a, ordered, coop, ladder, st);
+  }
+  return launch_sweep_t<KIND, PAIR, CT, NB, 8>(units, n_units, block, smem, a, ordered, coop, ladder, st);
 }
 
-hipError_t launch_sweep(int kind, int pairwise, int coupling, int nb, const UnitDev* units, int n_units,
+template <int KIND, int PAIR, typename CT>
+static hipError_t launch_nb(int nb, int row_e, const UnitDev* units, int n_units, int block, size_t smem,
+                            const SweepArgs& a, int ordered, int coop, int ladder, hipStream_t st) {
+  if (nb <= 1) return launch_e<KIND, PAIR, CT, 1>(row_e, units, n_units, block, smem, a, ordered, coop, ladder, st);
+  if (nb <= 2) return launch_e<KIND, PAIR, CT, 2>(row_e, units, n_units, block, smem, a, ordered, coop, ladder, st);
+  if (nb <= 4) return launch_e<KIND, PAIR, CT, 4>(row_e, units, n_units, block, smem, a, ordered, coop, ladder, st);
+  return launch_e<KIND, PAIR, CT, 8>(row_e, units, n_units, block, smem, a, ordered, coop, ladder, st);
+}
+
+hipError_t launch_sweep(int kind, int pairwise, int coupling, int nb, int row_e, const UnitDev* units, int n_units,
                         int block, size_t smem, const SweepArgs& a, int ordered, int coop, int ladder,
                         hipStream_t st) {
   const bool i32 = coupling == QMC_COUPLING_I32;
-#define QMC_LAUNCH(K, P)                                                                                \
-  return i32 ? launch_nb<K, P, int32_t>(nb, units, n_units, block, smem, a, ordered, coop, ladder, st) \
-             : launch_nb<K, P, float>(nb, units, n_units, block, smem, a, ordered, coop, ladder, st)
+#define QMC_LAUNCH(K, P)                                                                                       \
+  return i32 ? launch_nb<K, P, int32_t>(nb, row_e, units, n_units, block, smem, a, ordered, coop, ladder, st) \
+             : launch_nb<K, P, float>(nb, row_e, units, n_units, block, smem, a, ordered, coop, ladder, st)
   // int8 entries only for integer coefficients
   if (pairwise == 2 && !i32) return hipErrorInvalidValue;
   if (kind == QMC_ISING) {
-    if (pairwise == 2) return launch_nb<QMC_ISING, 2, int32_t>(nb, units, n_units, block, smem, a, ordered, coop, ladder, st);
+    if (pairwise == 2)
+      return launch_nb<QMC_ISING, 2, int32_t>(nb, row_e, units, n_units, block, smem, a, ordered, coop, ladder, st);
     if (pairwise) QMC_LAUNCH(QMC_ISING, 1);
     QMC_LAUNCH(QMC_ISING, 0);
   }
-  if (pairwise == 2) return launch_nb<QMC_PUBO, 2, int32_t>(nb, units, n_units, block, smem, a, ordered, coop, ladder, st);
+  if (pairwise == 2)
+    return launch_nb<QMC_PUBO, 2, int32_t>(nb, row_e, units, n_units, block, smem, a, ordered, coop, ladder, st);
   if (pairwise) QMC_LAUNCH(QMC_PUBO, 1);
   QMC_LAUNCH(QMC_PUBO, 0);
 #undef QMC_LAUNCH
 }
 
-template <int KIND, int PAIR, typename CT, int NB>
+template <int KIND, int PAIR, typename CT, int NB, int E>
 static int coresident_t(int block, size_t smem) {
   int per_cu = 0;
   // cooperative launches carry ladders
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_sweep<KIND, PAIR, CT, NB, true>, block,
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_sweep<KIND, PAIR, CT, NB, true, E>, block,
                                                    smem) != hipSuccess)
     return 0;
   int dev = 0, cus = 0;
@@ -2314,25 +2332,35 @@ static int coresident_t(int block, size_t smem) {
   return per_cu * cus;
 }
 
-template <int KIND, int PAIR, typename CT>
-static int coresident_nb(int nb, int block, size_t smem) {
-  if (nb <= 1) return coresident_t<KIND, PAIR, CT, 1>(block, smem);
-  if (nb <= 2) return coresident_t<KIND, PAIR, CT, 2>(block, smem);
-  if (nb <= 4) return coresident_t<KIND, PAIR, CT, 4>(block, smem);
-  return coresident_t<KIND, PAIR, CT, 8>(block, smem);
+template <int KIND, int PAIR, typename CT, int NB>
+static int coresident_e(int row_e, int block, size_t smem) {
+  if constexpr (PAIR != 0 && NB <= 2) {
+    if (row_e == 4) return coresident_t<KIND, PAIR, CT, NB, 4>(block, smem);
+    if (row_e == 6) return coresident_t<KIND, PAIR, CT, NB, 6>(block, smem);
+  }
+  return coresident_t<KIND, PAIR, CT, NB, 8>(block, smem);
 }
 
-int max_coresident_blocks(int kind, int pairwise, int coupling, int nb, int block, size_t smem) {
+template <int KIND, int PAIR, typename CT>
+static int coresident_nb(int nb, int row_e, int block, size_t smem) {
+  if (nb <= 1) return coresident_e<KIND, PAIR, CT, 1>(row_e, block, smem);
+  if (nb <= 2) return coresident_e<KIND, PAIR, CT, 2>(row_e, block, smem);
+  if (nb <= 4) return coresident_e<KIND, PAIR, CT, 4>(row_e, block, smem);
+  return coresident_e<KIND, PAIR, CT, 8>(row_e, block, smem);
+}
+
+int max_coresident_blocks(int kind, int pairwise, int coupling, int nb, int row_e, int block, size_t smem) {
   const bool i32 = coupling == QMC_COUPLING_I32;
-#define QMC_CORES(K, P) \
-  return i32 ? coresident_nb<K, P, int32_t>(nb, block, smem) : coresident_nb<K, P, float>(nb, block, smem)
+#define QMC_CORES(K, P)                                                \
+  return i32 ? coresident_nb<K, P, int32_t>(nb, row_e, block, smem) \
+             : coresident_nb<K, P, float>(nb, row_e, block, smem)
   if (pairwise == 2 && !i32) return 0;
   if (kind == QMC_ISING) {
-    if (pairwise == 2) return coresident_nb<QMC_ISING, 2, int32_t>(nb, block, smem);
+    if (pairwise == 2) return coresident_nb<QMC_ISING, 2, int32_t>(nb, row_e, block, smem);
     if (pairwise) QMC_CORES(QMC_ISING, 1);
     QMC_CORES(QMC_ISING, 0);
   }
-  if (pairwise == 2) return coresident_nb<QMC_PUBO, 2, int32_t>(nb, block, smem);
+  if (pairwise == 2) return coresident_nb<QMC_PUBO, 2, int32_t>(nb, row_e, block, smem);
   if (pairwise) QMC_CORES(QMC_PUBO, 1);
   QMC_CORES(QMC_PUBO, 0);
 #undef QMC_CORES

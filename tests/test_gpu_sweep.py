@@ -193,6 +193,67 @@ def test_ordered_sweeps_bit_exact_int(ctx, name):
     assert np.array_equal(got["lowest_states"], want["lowest_states"])
 
 
+def _bounded_degree(n, dmax, m, seed):
+    """A ring plus random chords while both ends have fewer than dmax
+    neighbours; integer couplings in +-[1, 3]."""
+    rng = np.random.default_rng(seed)
+    deg = np.zeros(n, dtype=int)
+    edges = set()
+    for i in range(n):
+        edges.add((min(i, (i + 1) % n), max(i, (i + 1) % n)))
+    deg += 2
+    tries = 0
+    while len(edges) < m and tries < 50 * m:
+        tries += 1
+        i, j = sorted(rng.choice(n, size=2, replace=False).tolist())
+        if (i, j) in edges or deg[i] >= dmax or deg[j] >= dmax:
+            continue
+        edges.add((i, j))
+        deg[i] += 1
+        deg[j] += 1
+    return [dict(c=int(rng.integers(1, 4)) * (1 if rng.random() < 0.5 else -1), ids=[i, j])
+            for i, j in sorted(edges)]
+
+
+ROW_CASES = {
+    # widest slice -> entries read ahead per row: 2 -> 4 (padding), 4 -> 4,
+    # 5 -> 6 (padding), 6 -> 6, > 8 -> 8 plus the rest per row
+    "ring": lambda: _bounded_degree(300, 2, 300, seed=31),
+    "deg5": lambda: _bounded_degree(400, 5, 900, seed=32),
+    "lattice2d_pm1": lambda: wl.lattice2d(8, seed=3),
+    "lattice2d_wide": lambda: [dict(t, c=t["c"] * 300) for t in wl.lattice2d(8, seed=3)],
+    "lattice3d_int": lambda: [dict(t, c=int(np.sign(t["c"]) or 1)) for t in wl.lattice3d(6, seed=7)],
+    "er_int": lambda: wl.er_ising(150, 6.0, seed=4),
+}
+
+
+@pytest.mark.parametrize("name", list(ROW_CASES))
+@pytest.mark.parametrize("B", [1, 2])
+def test_row_widths_bit_exact(ctx, name, B):
+    """Kernels reading 4, 6 or 8 entries per pairwise row ahead (chosen from
+    the widest slice; slices store at least 8 entries, zero-padded) against
+    the oracle's sweeps, int8 and 4-byte-int entries both: identical states,
+    costs, lowest costs and states, accepted counts."""
+    betas = np.linspace(0.1, 3.0, 24)
+    op, qp, w, got, want = _run_pair(ctx, "ising", ROW_CASES[name](), R=12, B=B, betas=betas)
+    assert qp.info.coupling == _qmc.QMC_COUPLING_I32
+    for k in ("states", "cost", "lowest", "lowest_states", "accepted"):
+        assert np.array_equal(got[k], want[k]), k
+
+
+def test_sa_sweeps_f32_six_entry_rows(ctx):
+    """fp32 couplings on the 3D lattice (every slice 6 wide) with 2 replicas
+    per work-group: the 6-entry kernel; identical states, costs within 1e-5."""
+    kind, gen = PROBLEMS["lattice3d_f32"]
+    betas = np.linspace(0.2, 3.0, 16)
+    op, qp, w, got, want = _run_pair(ctx, kind, gen(), R=8, B=2, betas=betas)
+    assert qp.info.coupling == _qmc.QMC_COUPLING_F32
+    assert np.array_equal(got["states"], want["states"])
+    assert np.array_equal(got["lowest_states"], want["lowest_states"])
+    np.testing.assert_allclose(got["cost"], want["cost"], rtol=1e-5)
+    np.testing.assert_allclose(got["lowest"], want["lowest"], rtol=1e-5)
+
+
 def test_sa_sweeps_f32(ctx):
     kind, gen = PROBLEMS["lattice3d_f32"]
     betas = np.linspace(0.2, 3.0, 16)
