@@ -1498,13 +1498,18 @@ static size_t sweep_launch_smem(int stride, int ordered, int block, int nb, int 
   size_t smem = sweep_smem_bytes(stride, ordered, &a.best_lds);
   a.pre_rounds = 0;
   a.pre_off = 0;
+  a.pre_draws = 0;
   if (pre_rounds > 0 && !ordered) {
     const size_t off = (smem + 15) & ~(size_t)15;
-    const size_t bytes = sweep_pre_bytes(pre_rounds, block, nb);
-    if (off + bytes <= 160 * 1024) {
-      a.pre_rounds = pre_rounds;
-      a.pre_off = (int32_t)off;
-      smem = off + bytes;
+    // fields and draws, else fields only, else neither
+    for (int draws = 1; draws >= 0; --draws) {
+      const size_t bytes = sweep_pre_bytes(pre_rounds, block, nb, draws != 0);
+      if (off + bytes <= 160 * 1024) {
+        a.pre_rounds = pre_rounds;
+        a.pre_off = (int32_t)off;
+        a.pre_draws = draws;
+        return off + bytes;
+      }
     }
   }
   return smem;

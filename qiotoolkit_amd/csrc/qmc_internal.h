@@ -190,14 +190,17 @@ struct SweepArgs {
   // Split parallel tempering ladders: dynamic LDS at byte pre_off holds the
   // local fields (NB per thread) and variable ids of pre_rounds rounds of
   // colour class 0, computed while the ladder's energies travel between
-  // work-groups (0: no room, fields computed in the class as usual)
+  // work-groups (0: no room, fields computed in the class as usual), and
+  // with pre_draws set also their draws (NB fp64 table values and a mask
+  // per thread)
   int32_t pre_rounds;
   int32_t pre_off;
+  int32_t pre_draws;
 };
 // Bytes of that area for a launch of `block` threads with NB replicas per
 // byte lane.
-inline size_t sweep_pre_bytes(int rounds, int block, int nb) {
-  return (size_t)rounds * block * (nb + 1) * 4;
+inline size_t sweep_pre_bytes(int rounds, int block, int nb, bool draws) {
+  return (size_t)rounds * block * ((nb + 1) * 4 + (draws ? nb * 8 + 4 : 0));
 }
 
 // Dynamic LDS of a sweep launch whose largest unit spin array is max_stride

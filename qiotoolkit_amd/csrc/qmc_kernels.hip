@@ -617,26 +617,69 @@ k_sweep(const UnitDev* units, SweepArgs a) {
     else fetch_csr(class_off[0], 0, ra);
   }
 
+  // The draws of the replicas of this lane at the position of `var` in
+  // sweep `sw`, and their interpolated ln-table values: independent of the
+  // field and of beta, so they are formed while the neighbour bytes are
+  // read (or during a ladder exchange, for the next sweep's first class).
+  auto draw = [&](uint32_t var, uint32_t sw, Draw& d) {
+    uint4 quad = make_uint4(0, 0, 0, 0);
+    d.deep = 0;
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      const uint32_t r = rbase + (uint32_t)b;
+      // one draw serves a replica pair: words 0, 1 = their row-0
+      // draws, words 2, 3 = their row-1 draws (used after bucket 0)
+      if (b == 0 || (r & 1u) == 0u) quad = philox4x32_10(var, sw, r >> 1, DOM_SWEEP, key0, key1);
+      const uint32_t w0 = (r & 1u) ? quad.y : quad.x;
+      const uint32_t w1 = (r & 1u) ? quad.w : quad.z;
+      const bool z0 = (w0 & 255u) == 0u;
+      d.val[b] = row_value(z0 ? w1 : w0, z0 ? 1 : 0, lut);
+      d.deep |= (z0 && (w1 & 255u) == 0u) ? 1u << b : 0u;
+    }
+  };
+
   // Split ladders: while the ladder's energies travel between work-groups
   // after a sweep, the local fields of the next sweep's colour class 0 are
   // computed into LDS (they depend only on the spins of the other classes,
-  // final at the end of a sweep, and not on beta); the class then only
-  // decides.  Same fields, same decisions: the results do not change.
+  // final at the end of a sweep, and not on beta), and with room for them
+  // the draws of that class too; the class then only decides.  Same fields,
+  // same draws, same decisions: the results do not change.
   constexpr bool kEarly = PAIR && LADDER;
   using lds_ct = __attribute__((address_space(3))) CT;
-  lds_ct* hpre = (lds_ct*)(smem_raw + a.pre_off);  // [round][replica][thread]
-  lds_u32* vpre = (lds_u32*)(smem_raw + a.pre_off + (size_t)a.pre_rounds * NB * nthreads * 4);  // [round][thread]
+  using lds_f64 = __attribute__((address_space(3))) double;
+  // [round][replica][thread] fields, [round][thread] variables, then (when
+  // a.pre_draws) [round][replica][thread] draw values, [round][thread] masks
+  lds_ct* hpre = (lds_ct*)(smem_raw + a.pre_off);
+  lds_u32* vpre = (lds_u32*)(smem_raw + a.pre_off + (size_t)a.pre_rounds * NB * nthreads * 4);
+  lds_f64* dpre = (lds_f64*)(smem_raw + a.pre_off + (size_t)a.pre_rounds * (NB + 1) * nthreads * 4);
+  lds_u32* mpre = (lds_u32*)(smem_raw + a.pre_off + (size_t)a.pre_rounds * ((NB + 1) * 4 + NB * 8) * nthreads);
+  const bool pre_draws = a.pre_draws != 0;
   const int nc0 = class_off[1] - class_off[0];
   const int K0 = (nc0 + TP - 1) >> lg;
   const bool early_ok = kEarly && U.ladder > 0 && U.upl > 1 && n_classes >= 2 && K0 <= a.pre_rounds;
   bool early_ready = false;
   // round k of class 0, or round k - K0 of class 1 (pairwise problems only)
   auto fetch_in = [&](int k, int4& sl, EllRow<PAIR>& e) { fetch_slice(slice_base[0] + k, sl, e); };
-  auto field_early = [&](int k, const int4& sl, const EllRow<PAIR>& e) {
+  auto field_early = [&](int k, const int4& sl, const EllRow<PAIR>& e, uint32_t sw) {
     const int rho = (k << lg) + tp;
     const int q = rho < nc0 ? class_off[0] + rho : N;
     CT h[8];
-    local_field<KIND, PAIR, CT, NB, E>(U, rows, sp, q, e, sl, tp, lg, G, g, h);
+    if constexpr (kEarly) {
+      if (pre_draws) {
+        uint32_t wv[8];
+        pair_gather<PAIR, NB, E>(sp, e, G, g, wv);
+        Draw d;
+        draw(e.var, sw, d);
+        pair_field<KIND, PAIR, CT, NB, E>(rows, sp, e, wv, sl, tp, lg, G, g, h);
+#pragma unroll
+        for (int b = 0; b < NB; ++b) dpre[(k * NB + b) * nthreads + tid] = d.val[b];
+        mpre[k * nthreads + tid] = d.deep;
+      } else {
+        local_field<KIND, PAIR, CT, NB, E>(U, rows, sp, q, e, sl, tp, lg, G, g, h);
+      }
+    } else {
+      local_field<KIND, PAIR, CT, NB, E>(U, rows, sp, q, e, sl, tp, lg, G, g, h);
+    }
 #pragma unroll
     for (int b = 0; b < NB; ++b) hpre[(k * NB + b) * nthreads + tid] = h[b];
     vpre[k * nthreads + tid] = e.var;
@@ -662,6 +705,10 @@ k_sweep(const UnitDev* units, SweepArgs a) {
   for (int s = 0; s < n_run; ++s) {
     const int64_t sweep64 = sweep_begin + s;
     const uint32_t sweep = (uint32_t)sweep64;
+    // Split ladders: the energy-slot resets of the last exchange complete
+    // here, before the next publish; the only other access in flight is the
+    // row fetched before that exchange, long arrived, so the wait is short.
+    if (LADDER && x_upl > 1 && tid < nrep) wait_stores();
     if (sched && tid < nrep) {
       S.rbeta[tid] = beta_next;
       if (s + 1 < n_run) beta_next = sched[sweep64 + 1 - sched_base];
@@ -702,26 +749,6 @@ k_sweep(const UnitDev* units, SweepArgs a) {
           } else {
             const bool here = k < K;
             fetch_csr(here ? c0 : c0n, here ? k : k - K, e);
-          }
-        };
-        // The draws of the replicas of this lane at the position of `var`
-        // and their interpolated ln-table values: independent of the field
-        // and of beta, so they are formed while the neighbour bytes are read.
-        auto draw = [&](uint32_t var, Draw& d) {
-          uint4 quad = make_uint4(0, 0, 0, 0);
-          d.deep = 0;
-#pragma unroll
-          for (int b = 0; b < NB; ++b) {
-            const uint32_t r = rbase + (uint32_t)b;
-            // one draw serves a replica pair: words 0, 1 = their row-0
-            // draws, words 2, 3 = their row-1 draws (used after bucket 0)
-            if (b == 0 || (r & 1u) == 0u)
-              quad = philox4x32_10(var, sweep, r >> 1, DOM_SWEEP, key0, key1);
-            const uint32_t w0 = (r & 1u) ? quad.y : quad.x;
-            const uint32_t w1 = (r & 1u) ? quad.w : quad.z;
-            const bool z0 = (w0 & 255u) == 0u;
-            d.val[b] = row_value(z0 ? w1 : w0, z0 ? 1 : 0, lut);
-            d.deep |= (z0 && (w1 & 255u) == 0u) ? 1u << b : 0u;
           }
         };
         // the decision for the replicas of this lane at position q (round
@@ -784,7 +811,7 @@ k_sweep(const UnitDev* units, SweepArgs a) {
           if constexpr (PAIR != 0) {
             uint32_t wv[8];
             pair_gather<PAIR, NB, E>(sp, e, G, g, wv);
-            draw(e.var, d);
+            draw(e.var, sweep, d);
             pair_field<KIND, PAIR, CT, NB, E>(rows, sp, e, wv, sl, tp, lg, G, g, h);
           } else {
             local_field<KIND, PAIR, CT, NB, E>(U, rows, sp, q, e, sl, tp >> lgs, lg, G, g, h, sub, lgs);
@@ -795,13 +822,13 @@ k_sweep(const UnitDev* units, SweepArgs a) {
                 for (int o = 1; o < (1 << lgs); o <<= 1) h[b] += __shfl_xor(h[b], o);
               if (sub) return;
             }
-            draw(e.var, d);
+            draw(e.var, sweep, d);
           }
           decide(k, q, valid, h, d, e.var);
         };
         if (kEarly && c == 0 && early_ready) {
-          // fields computed during the last exchange; the row pipeline
-          // already holds class 1's first row
+          // fields (and draws) computed during the last exchange; the row
+          // pipeline already holds class 1's first row
           for (int k = 0; k < K; ++k) {
             CT h[8];
 #pragma unroll
@@ -809,7 +836,13 @@ k_sweep(const UnitDev* units, SweepArgs a) {
             const int rho = (k << lg) + tp;
             const uint32_t var = vpre[k * nthreads + tid];
             Draw d;
-            draw(var, d);
+            if (pre_draws) {
+#pragma unroll
+              for (int b = 0; b < NB; ++b) d.val[b] = dpre[(k * NB + b) * nthreads + tid];
+              d.deep = mpre[k * nthreads + tid];
+            } else {
+              draw(var, sweep, d);
+            }
             decide(k, rho < nc ? c0 + rho : N, rho < nc, h, d, var);
           }
         } else {
@@ -877,6 +910,7 @@ k_sweep(const UnitDev* units, SweepArgs a) {
           wlv[w] = S.wlane[par][gg][bb][w];
           wkv[w] = S.wk[par][gg][bb][w];
         }
+        const double before = S.rcost[tid], best_before = S.rbest[tid];
         AT run = AT(0), bestc = AT(0);
         int bt = -1, bk = -1;
 #pragma unroll
@@ -892,10 +926,9 @@ k_sweep(const UnitDev* units, SweepArgs a) {
             run += smv[w];
           }
         }
-        const double before = S.rcost[tid];
         S.rcost[tid] = before + (double)run;
         const double cand = before + (double)bestc;
-        if (cand < S.rbest[tid]) {
+        if (cand < best_before) {
           // the state there is rebuilt once, at the end of the sweep
           S.rbest[tid] = cand;
           S.nb_c[tid] = c;
@@ -973,18 +1006,20 @@ k_sweep(const UnitDev* units, SweepArgs a) {
       if (x_upl > 1) {
         double* cb = x_cbuf + xbuf * L;
         if (tid < nrep) {
-          wait_stores();  // this slot's reset (one step ago) is complete
+          // this slot's reset (one step ago) completed at the start of the
+          // sweep
           store_agent(cb + x_slot0 + tid, S.rcost[tid]);
         }
         // while the energies travel: the fields of class 0 of the next sweep
         // (its row pipeline moves on to class 1's first row)
         if (early_ok && s + 1 < n_run) {
           if (active) {
+            const uint32_t next = (uint32_t)(sweep64 + 1);
             for (int k = 0; k < K0; k += 2) {
               fetch_in(k + 1, slb, rb);
-              field_early(k, sla, ra);
+              field_early(k, sla, ra, next);
               fetch_in(min(k + 2, K0), sla, ra);
-              if (k + 1 < K0) field_early(k + 1, slb, rb);
+              if (k + 1 < K0) field_early(k + 1, slb, rb, next);
             }
             if (K0 == 0) fetch_in(0, sla, ra);
           }
