@@ -955,25 +955,34 @@ k_sweep(const UnitDev* units, SweepArgs a) {
         for (int b = 0; b < NB; ++b)
           if (b < nb_g && S.nb_c[8 * g + b] >= 0) newmask |= 1u << b;
         if (newmask) {
+          // class by class (uniform bounds), each lane over its positions
           auto rebuild = [&](auto* out) {
-            int c = 0;
-            for (int q = tp; q < N; q += TP) {
-              while (q >= class_off[c + 1]) ++c;
-              const int rho = q - class_off[c];
-              const int tq = rho & (TP - 1), kq = rho >> lg;
-              const uint32_t cur = sp[q * G + g];
-              const uint32_t f = fl[q * G + g];
-              uint32_t val = cur;
+            int cs[8], ts[8], ks[8];
 #pragma unroll
-              for (int b = 0; b < NB; ++b) {
-                if ((newmask >> b) & 1u) {
-                  const int cs = S.nb_c[8 * g + b], ts = S.nb_tp[8 * g + b], ks = S.kstar[8 * g + b];
-                  const bool after = c > cs || (c == cs && (tq > ts || (tq == ts && kq > ks)));
-                  if (after) val ^= f & (1u << b);
+            for (int b = 0; b < NB; ++b) {
+              cs[b] = S.nb_c[8 * g + b];
+              ts[b] = S.nb_tp[8 * g + b];
+              ks[b] = S.kstar[8 * g + b];
+            }
+            for (int c = 0; c < n_classes; ++c) {
+              const int cb0 = class_off[c], cb1 = class_off[c + 1];
+              for (int q = cb0 + tp; q < cb1; q += TP) {
+                const int rho = q - cb0;
+                const int tq = rho & (TP - 1), kq = rho >> lg;
+                const uint32_t at = lds_at<NB>((uint32_t)q, G, g);
+                const uint32_t cur = sp[at];
+                const uint32_t f = fl[at];
+                uint32_t val = cur;
+#pragma unroll
+                for (int b = 0; b < NB; ++b) {
+                  if ((newmask >> b) & 1u) {
+                    const bool after = c > cs[b] || (c == cs[b] && (tq > ts[b] || (tq == ts[b] && kq > ks[b])));
+                    if (after) val ^= f & (1u << b);
+                  }
                 }
+                const uint32_t old = out[at];
+                out[at] = (uint8_t)((old & ~newmask) | (val & newmask));
               }
-              const uint32_t old = out[q * G + g];
-              out[q * G + g] = (uint8_t)((old & ~newmask) | (val & newmask));
             }
           };
           // explicit address spaces: LDS or global stores, never flat ones
@@ -996,13 +1005,16 @@ k_sweep(const UnitDev* units, SweepArgs a) {
     if (LADDER && x_ladder > 0 && xphase == x_spr - 1) {
       const uint32_t step = (uint32_t)xstep;
       const int L = x_ladder;
-      // this thread's pair and its uniform, drawn before the energies arrive
+      // this thread's pair and its uniform (drawn after the energy is
+      // published, while the ladder's energies travel)
       const int i = (int)(step & 1u) + 2 * tid;
       double u = 0.0;
-      if (i < L - 1) {
-        const uint4 w = philox4x32_10((uint32_t)i, step, 0u, DOM_EXCHANGE, x_seed, x_stream);
-        u = canonical2(w.x, w.y);
-      }
+      auto exchange_uniform = [&]() {
+        if (i < L - 1) {
+          const uint4 w = philox4x32_10((uint32_t)i, step, 0u, DOM_EXCHANGE, x_seed, x_stream);
+          u = canonical2(w.x, w.y);
+        }
+      };
       if (x_upl > 1) {
         double* cb = x_cbuf + xbuf * L;
         if (tid < nrep) {
@@ -1025,6 +1037,7 @@ k_sweep(const UnitDev* units, SweepArgs a) {
           }
           early_ready = true;
         }
+        exchange_uniform();
         if (tid < L) {
           double e = 0.0;
           if (!await_energy(U, cb + tid, e)) S.abort_flag = 1;
@@ -1036,8 +1049,9 @@ k_sweep(const UnitDev* units, SweepArgs a) {
           const double nan = __longlong_as_double(0x7ff8000000000000ll);
           store_agent(x_cbuf + (xbuf == 0 ? 2 : xbuf - 1) * L + x_slot0 + tid, nan);
         }
-      } else if (tid < L) {
-        S.lcost[tid] = S.rcost[tid];
+      } else {
+        exchange_uniform();
+        if (tid < L) S.lcost[tid] = S.rcost[tid];
       }
       lds_barrier();
       if (i < L - 1) {
