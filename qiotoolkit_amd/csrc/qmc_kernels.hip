@@ -653,7 +653,8 @@ k_sweep(const UnitDev* units, SweepArgs a) {
   lds_u32* vpre = (lds_u32*)(smem_raw + a.pre_off + (size_t)a.pre_rounds * NB * nthreads * 4);
   lds_f64* dpre = (lds_f64*)(smem_raw + a.pre_off + (size_t)a.pre_rounds * (NB + 1) * nthreads * 4);
   lds_u32* mpre = (lds_u32*)(smem_raw + a.pre_off + (size_t)a.pre_rounds * ((NB + 1) * 4 + NB * 8) * nthreads);
-  const bool pre_draws = a.pre_draws != 0;
+  // (8 replicas per byte lane: fields only, the draws' registers would spill)
+  const bool pre_draws = NB <= 4 && a.pre_draws != 0;
   const int nc0 = class_off[1] - class_off[0];
   const int K0 = (nc0 + TP - 1) >> lg;
   const bool early_ok = kEarly && U.ladder > 0 && U.upl > 1 && n_classes >= 2 && K0 <= a.pre_rounds;
@@ -664,7 +665,7 @@ k_sweep(const UnitDev* units, SweepArgs a) {
     const int rho = (k << lg) + tp;
     const int q = rho < nc0 ? class_off[0] + rho : N;
     CT h[8];
-    if constexpr (kEarly) {
+    if constexpr (kEarly && NB <= 4) {
       if (pre_draws) {
         uint32_t wv[8];
         pair_gather<PAIR, NB, E>(sp, e, G, g, wv);

@@ -311,9 +311,13 @@ class Model:
             pass
 
     def n_vars(self):
-        n = C.c_int32()
-        _check_rc(lib().qiotk_model_vars(self.h, C.byref(n), None))
-        return n.value
+        # a model never changes after creation: asked once
+        nv = getattr(self, "_n_vars", None)
+        if nv is None:
+            n = C.c_int32()
+            _check_rc(lib().qiotk_model_vars(self.h, C.byref(n), None))
+            nv = self._n_vars = n.value
+        return nv
 
     def names(self):
         import numpy as np
