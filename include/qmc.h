@@ -86,8 +86,10 @@ int qmc_ctx_compute_units(qmc_ctx ctx, int32_t* compute_units);
  * CSR; ids sorted inside each term, no constant terms.  lanes = default
  * class-round width of walkers on it (0: each walker set picks its own from
  * the largest colour class and its work-group count: from 64 lanes, doubled
- * while its work-groups hold fewer than 8 waves per compute unit and one
- * round does not cover the largest class). */
+ * while its work-groups hold fewer than 16 waves per compute unit and one
+ * round does not cover the largest class, or 8 lanes per position of it for
+ * integer problems in sliced entries, whose small classes share positions
+ * among lanes; at most 512 lanes x ceil(replicas_per_group / 8)). */
 int qmc_problem_create(qmc_ctx ctx, int kind, int32_t n_vars, int64_t n_terms,
                        const int64_t* term_offsets, const int32_t* term_vars,
                        const double* coeffs, int32_t lanes, qmc_problem* out);

@@ -11,9 +11,9 @@
 #include <limits>
 #include <map>
 #include <memory>
-#include <unordered_set>
 #include <mutex>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "err_buf.h"
@@ -163,7 +163,7 @@ struct qmc_ctx_s {
   size_t stage_off = 0;
   DevBuf<uint8_t> dstage;
   std::vector<ScatterRec> pending;  // staged, not yet enqueued (src = ring offset)
-  std::unordered_set<const void*> pending_dst;
+  std::unordered_map<const void*, size_t> pending_dst;  // destination -> its record in `pending`
   size_t pend_begin = 0;            // ring offset of the first pending byte
   hipError_t sticky = hipSuccess;   // a failed flush, reported by sync_stream()
   // beta schedules on the device, shared by walker sets that plan the same
@@ -271,9 +271,19 @@ static hipError_t upload_async(qmc_ctx c, void* dst, const void* src, size_t byt
     e = hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->raw_stream);
     return e != hipSuccess ? e : hipStreamSynchronize(c->raw_stream);
   }
-  // a second upload to the same place waits for the first to be enqueued
-  // (the scatter applies one batch's copies in no particular order)
-  if (c->pending_dst.count(dst) && (e = flush_locked(c)) != hipSuccess) return e;
+  // A second upload to the same place: of the same size, it replaces the
+  // first one's staged bytes (nothing has been enqueued from them yet);
+  // otherwise it waits for the first to be enqueued (the scatter applies one
+  // batch's copies in no particular order).
+  auto same = c->pending_dst.find(dst);
+  if (same != c->pending_dst.end()) {
+    const ScatterRec& r = c->pending[same->second];
+    if (r.bytes == bytes) {
+      std::memcpy(c->stage + r.src, src, bytes);
+      return hipSuccess;
+    }
+    if ((e = flush_locked(c)) != hipSuccess) return e;
+  }
   size_t at = (c->stage_off + 15) & ~(size_t)15;
   if (at + bytes > kStageBytes) {
     // every earlier copy out of the ring is enqueued on the stream
@@ -286,7 +296,7 @@ static hipError_t upload_async(qmc_ctx c, void* dst, const void* src, size_t byt
   std::memcpy(c->stage + at, src, bytes);
   c->stage_off = at + bytes;
   c->pending.push_back(ScatterRec{(uint64_t)at, (uint64_t)(uintptr_t)dst, (uint64_t)bytes});
-  c->pending_dst.insert(dst);
+  c->pending_dst[dst] = c->pending.size() - 1;
   return hipSuccess;
 }
 
@@ -1035,14 +1045,18 @@ static int set_progress(qmc_walkers w, int64_t sweep0, int64_t n) {
 }
 
 // Class-round width of a walker set on a problem created without one: from
-// 64, lanes double while the set's work-groups hold fewer than 8 waves per
-// compute unit, one round does not yet cover the largest colour class, and
-// lanes x G stays within 512.  Few work-groups get wide ones (shorter class
-// rounds, more waves to hide latency); many get narrow ones.
+// 64, lanes double while the set's work-groups hold fewer than 16 waves per
+// compute unit, one round does not yet cover the largest colour class (or,
+// where small classes share a position among up to 8 lanes -- integer terms
+// in sliced entries -- 8 lanes of each of its positions), and lanes x G
+// stays within 512.  Few work-groups get wide ones (shorter class rounds,
+// more waves to hide latency); many get narrow ones.
 static int default_lanes(qmc_problem p, int64_t n_units, int G) {
-  const int64_t want = 8 * (int64_t)p->ctx->n_cus;
+  const int64_t want = 16 * (int64_t)p->ctx->n_cus;
+  const bool shared = !p->pairwise && p->coupling == QMC_COUPLING_I32 && p->max_others <= 3;
+  const int64_t cover = (int64_t)p->max_class * (shared ? 8 : 1);
   int l = 64;
-  while (l * 2 * G <= 512 && l < p->max_class && n_units * (l * G / 64) < want) l *= 2;
+  while (l * 2 * G <= 512 && l < cover && n_units * (l * G / 64) < want) l *= 2;
   return l;
 }
 

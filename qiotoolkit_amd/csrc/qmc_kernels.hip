@@ -897,45 +897,102 @@ k_sweep(const UnitDev* units, SweepArgs a) {
         }
       }
       lds_barrier();
-      if (tid < nrep) {
-        const int gg = tid >> 3, bb = tid & 7;
-        const int W = TP >> 6;
-        // every wave's slots read at once (one LDS round trip), then
-        // combined in wave order; slots of absent waves are read, not used
-        AT mnv[kMaxWavesPerGroup], smv[kMaxWavesPerGroup];
-        int wlv[kMaxWavesPerGroup], wkv[kMaxWavesPerGroup];
-#pragma unroll
-        for (int w = 0; w < kMaxWavesPerGroup; ++w) {
-          mnv[w] = Slots<AT>::mn(S, par, gg, bb)[w];
-          smv[w] = Slots<AT>::sum(S, par, gg, bb)[w];
-          wlv[w] = S.wlane[par][gg][bb][w];
-          wkv[w] = S.wk[par][gg][bb][w];
-        }
-        const double before = S.rcost[tid], best_before = S.rbest[tid];
-        AT run = AT(0), bestc = AT(0);
-        int bt = -1, bk = -1;
-#pragma unroll
-        for (int w = 0; w < kMaxWavesPerGroup; ++w) {
-          if (w < W) {
-            const AT cnd = run + mnv[w];
-            if (bt < 0 || cnd < bestc) {
-              bestc = cnd;
+      if constexpr (std::is_same<AT, int32_t>::value) {
+        // Integer sums (exact in any order): wave 0 combines every
+        // replica's wave partials at once, lane = replica x W + wave --
+        // segmented prefix sums of the totals, then a segmented prefix
+        // minimum of (earlier totals + the wave's minimum prefix), ties to
+        // the earlier wave: the wave-order fold, in log2(W) steps.
+        if (tid < 64) {
+          const int W = TP >> 6;  // waves per byte group: 1, 2, 4 or 8
+          const int lw = W >= 8 ? 3 : (W >= 4 ? 2 : (W >= 2 ? 1 : 0));
+          const int rr = lane >> lw, w = lane & (W - 1);
+          const bool live = rr < nrep;
+          const int gg = rr >> 3, bb = rr & 7;
+          int32_t sm = 0, m = 0, wl = 0, wk = -1;
+          if (live) {
+            sm = S.wsum.i[par][gg][bb][w];
+            m = S.wmin.i[par][gg][bb][w];
+            wl = S.wlane[par][gg][bb][w];
+            wk = S.wk[par][gg][bb][w];
+          }
+          int32_t incl = sm;
+          const int32_t o1 = dpp_z<DPP_ROW_SHR1>(incl);
+          incl = (W > 1 && w >= 1) ? incl + o1 : incl;
+          const int32_t o2 = dpp_z<DPP_ROW_SHR2>(incl);
+          incl = (W > 2 && w >= 2) ? incl + o2 : incl;
+          const int32_t o4 = dpp_z<DPP_ROW_SHR4>(incl);
+          incl = (W > 4 && w >= 4) ? incl + o4 : incl;
+          int32_t cnd = incl - sm + m;
+          int32_t wi = w;
+          auto min_step_seg = [&](auto shift_of, int sh) {
+            const int32_t oc = shift_of(cnd), owi = shift_of(wi), owl = shift_of(wl), owk = shift_of(wk);
+            // the earlier part wins unless this lane's candidate is lower
+            const bool take = sh < W && w >= sh && !(cnd < oc);
+            cnd = take ? oc : cnd;
+            wi = take ? owi : wi;
+            wl = take ? owl : wl;
+            wk = take ? owk : wk;
+          };
+          min_step_seg([](int32_t x) { return dpp_z<DPP_ROW_SHR1>(x); }, 1);
+          min_step_seg([](int32_t x) { return dpp_z<DPP_ROW_SHR2>(x); }, 2);
+          min_step_seg([](int32_t x) { return dpp_z<DPP_ROW_SHR4>(x); }, 4);
+          if (live && w == W - 1) {
+            const double before = S.rcost[rr], best_before = S.rbest[rr];
+            S.rcost[rr] = before + (double)incl;
+            const double cand = before + (double)cnd;
+            if (cand < best_before) {
+              // the state there is rebuilt once, at the end of the sweep
+              S.rbest[rr] = cand;
+              S.nb_c[rr] = c;
               // the lane's position (lanes share positions in split classes)
-              bt = (w * 64 + wlv[w]) >> lgs;
-              bk = wkv[w];
+              S.nb_tp[rr] = (wi * 64 + wl) >> lgs;
+              S.kstar[rr] = wk;
+              S.any_new_best = 1;
             }
-            run += smv[w];
           }
         }
-        S.rcost[tid] = before + (double)run;
-        const double cand = before + (double)bestc;
-        if (cand < best_before) {
-          // the state there is rebuilt once, at the end of the sweep
-          S.rbest[tid] = cand;
-          S.nb_c[tid] = c;
-          S.nb_tp[tid] = bt;
-          S.kstar[tid] = bk;
-          S.any_new_best = 1;
+      } else {
+        if (tid < nrep) {
+          const int gg = tid >> 3, bb = tid & 7;
+          const int W = TP >> 6;
+          // every wave's slots read at once (one LDS round trip), then
+          // combined in wave order; slots of absent waves are read, not used
+          AT mnv[kMaxWavesPerGroup], smv[kMaxWavesPerGroup];
+          int wlv[kMaxWavesPerGroup], wkv[kMaxWavesPerGroup];
+#pragma unroll
+          for (int w = 0; w < kMaxWavesPerGroup; ++w) {
+            mnv[w] = Slots<AT>::mn(S, par, gg, bb)[w];
+            smv[w] = Slots<AT>::sum(S, par, gg, bb)[w];
+            wlv[w] = S.wlane[par][gg][bb][w];
+            wkv[w] = S.wk[par][gg][bb][w];
+          }
+          const double before = S.rcost[tid], best_before = S.rbest[tid];
+          AT run = AT(0), bestc = AT(0);
+          int bt = -1, bk = -1;
+#pragma unroll
+          for (int w = 0; w < kMaxWavesPerGroup; ++w) {
+            if (w < W) {
+              const AT cnd = run + mnv[w];
+              if (bt < 0 || cnd < bestc) {
+                bestc = cnd;
+                // the lane's position (lanes share positions in split classes)
+                bt = (w * 64 + wlv[w]) >> lgs;
+                bk = wkv[w];
+              }
+              run += smv[w];
+            }
+          }
+          S.rcost[tid] = before + (double)run;
+          const double cand = before + (double)bestc;
+          if (cand < best_before) {
+            // the state there is rebuilt once, at the end of the sweep
+            S.rbest[tid] = cand;
+            S.nb_c[tid] = c;
+            S.nb_tp[tid] = bt;
+            S.kstar[tid] = bk;
+            S.any_new_best = 1;
+          }
         }
       }
       // no barrier: the next class writes the other parity's slots, and the
