@@ -46,27 +46,45 @@ HBM_PEAK_GBS = 8000.0
 # kernel (--lds-probe) measures 106-109 TB/s.
 LDS_PEAK_GBS = 256 * 256 * 2.4
 
-# HBM bytes per sweep-kernel launch of the default shapes, from rocprofv3
-# --pmc FETCH_SIZE and WRITE_SIZE in separate runs, scaled by what the same
-# counters report for a plain 512 MiB copy (FETCH_SIZE x 2, WRITE_SIZE x 1).
-# Summaries: profiles/k_sweep_*_hbm_traffic.md
-MEASURED_TRAFFIC_BYTES = {"c2": int((198183.25 * 2 + 82106) * 1024)}
+# HBM bytes per step of the default shapes (c4: the three sweep launches of a
+# step; the others: the one sweep launch), from rocprofv3 --pmc FETCH_SIZE and
+# WRITE_SIZE in separate runs, in KiB, scaled by what the same counters
+# report for a plain 512 MiB copy (FETCH_SIZE x 2, WRITE_SIZE x 1).
+# Summaries: profiles/k_sweep_*.md
+MEASURED_TRAFFIC_BYTES = {
+    "c2": int((107108.3 * 2 + 82106.0) * 1024),
+    "c1": int((328.4 * 2 + 142.0) * 1024),
+    "c3": int((6100.3 * 2 + 1056.3) * 1024),
+    "c4": int((98178123.4 * 2 + 9075.9) * 1024),
+}
 
-# What limits each kernel, from its SQ counters (profiles/k_sweep_*_sq_counters.md):
-# VALU issue plus stalls on the per-CU stream of graph rows from L2 and on
-# barriers; the HBM figure is the algorithmic bytes priced against the HBM peak.
-LIMITER = ("on-chip: VALU issue, the L2->CU graph-row stream and the cross-work-group ladder exchange "
-           "(graph and state stay in L2/LDS); frac = algorithmic bytes over the HBM peak, above 1 when "
-           "on-chip reuse serves more bytes than HBM could")
+# What limits each kernel (rocprofv3 kernel traces, SQ and byte counters in
+# profiles/); frac prices the algorithmic bytes against the HBM peak, above 1
+# where on-chip reuse serves more bytes than HBM could.
+_BASIS = "; frac = algorithmic bytes over the HBM peak (not an HBM bound)"
+LIMITER = {
+    "c2": "on-chip: VALU issue (0.59 of the issue floor) and the cross-work-group ladder exchange; "
+          "graph in L2, state in LDS" + _BASIS,
+    "c1": "per-class latency (wave scans, barrier, combine: 2 classes of one round) with 64 work-groups "
+          "for 256 CUs; graph and state on chip" + _BASIS,
+    "c3": "per-class latency (20 colour classes of one 512-lane round each) and VALU issue of the "
+          "4-local fields; graph in L2, state in LDS" + _BASIS,
+    "c4": "the largest instances' work-groups (the last of three concurrent launches) and their graph "
+          "stream from memory (the batch's graphs exceed L2)" + _BASIS,
+}
+
+
+def limiter(name):
+    return LIMITER[name]
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def roofline(attempts, B, kernel_ms, kernel, traffic=None):
+def roofline(attempts, B, kernel_ms, kernel, traffic=None, workload="c2"):
     achieved = attempts * B / (kernel_ms * 1e-3) / 1e9
-    return {"bound": LIMITER, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+    return {"bound": limiter(workload), "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "bytes_per_attempt": B,
             "kernel": kernel, "kernel_ms": kernel_ms, "lds_frac": achieved / LDS_PEAK_GBS}
 
@@ -188,10 +206,12 @@ def run_sa(args, name, steps, warmup):
         eng.close()
     attempts = params["restarts"] * params["sweeps"] * n
     B = d * (4 + sc + 1) + 2
+    default_shape = not args.group and not args.lanes
     return {"value": attempts * steps / t, "unit": "spin-flip attempts/s", "steps": steps, "warmup": warmup,
             "ms_per_step": t * 1e3 / steps, "dtype": dtype,
             "config": {"workload": desc, "n_vars": n, "attempts_per_step": attempts},
-            "roofline": roofline(attempts, B, float(np.mean(dev_ms)), "k_sweep (device time of the solve)"),
+            "roofline": roofline(attempts, B, float(np.mean(dev_ms)), "k_sweep (device time of the solve)",
+                                 MEASURED_TRAFFIC_BYTES[name] if default_shape else None, name),
             "quality": {"best_cost_mean": float(np.mean(best)), "best_cost_min": float(np.min(best))}}
 
 
@@ -263,7 +283,11 @@ def run_c4(args):
     value = attempts * args.steps / t
     avg_ms = float(np.mean(dev_ms))
     B = 7 * (4 + 1 + 1) + 2   # d = 6 couplings + 1 field term, int8 coefficients
-    default_shape = sweeps == 1000 and len(models) == 4096 and tune is not None
+    # the profiled shape: the tuner's usual choice, 8 replicas per work-group
+    # and lane depth 8, one tier, one launch
+    default_shape = (sweeps == 1000 and len(models) == 4096 and tune is not None
+                     and extra.get("replicas_per_group") == 8 and extra.get("positions_per_lane") == 8
+                     and not extra.get("large_min_vars") and "sweeps_per_launch" not in extra)
     # plain form: the same solves, one instance per launch, on a sample --
     # with the batch's parameters, and with one replica per work-group (the
     # plain form's best shape; the grouping does not change results)
@@ -306,7 +330,7 @@ def run_c4(args):
                    "instances": len(models), "spins": sum(i[0] for i in insts), "restarts": 8,
                    "sweeps": sweeps, "params": extra, "parallelism": "none (1 GPU)"},
         "roofline": roofline(attempts, B, avg_ms, "k_sweep<ISING,pairwise,i32> (all launches of a step)",
-                             MEASURED_TRAFFIC_BYTES.get("c4") if default_shape else None),
+                             MEASURED_TRAFFIC_BYTES["c4"] if default_shape else None, "c4"),
         "plain_per_instance": {"value": plain_best, "by_shape": plain, "instances_sampled": len(idx),
                                "speedup_of_batched": value / plain_best if plain_best else None,
                                "identical_best_costs": same},
