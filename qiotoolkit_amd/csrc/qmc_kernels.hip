@@ -475,6 +475,14 @@ __device__ __forceinline__ void lds_barrier() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
+// Orders LDS accesses among the lanes of one wavefront (its LDS
+// instructions execute in order): no work-group barrier.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
+}
+
 // s_waitcnt vmcnt(0): every vector memory access of this lane issued so far,
 // stores included, has completed.
 __device__ __forceinline__ void wait_stores() { __builtin_amdgcn_s_waitcnt(0x0F70); }
@@ -710,11 +718,15 @@ k_sweep(const UnitDev* units, SweepArgs a) {
     // here, before the next publish; the only other access in flight is the
     // row fetched before that exchange, long arrived, so the wait is short.
     if (LADDER && x_upl > 1 && tid < nrep) wait_stores();
-    if (sched && tid < nrep) {
-      S.rbeta[tid] = beta_next;
-      if (s + 1 < n_run) beta_next = sched[sweep64 + 1 - sched_base];
+    if (sched) {
+      if (tid < nrep) {
+        S.rbeta[tid] = beta_next;
+        if (s + 1 < n_run) beta_next = sched[sweep64 + 1 - sched_base];
+      }
+      lds_barrier();
     }
-    lds_barrier();
+    // (otherwise the betas were last written before the barrier that ends
+    // the previous sweep, or before the launch's set-up barrier)
     double beta[8];
 #pragma unroll
     for (int b = 0; b < NB; ++b) beta[b] = (b < nb_g) ? S.rbeta[8 * g + b] : 0.0;
@@ -1101,8 +1113,6 @@ k_sweep(const UnitDev* units, SweepArgs a) {
           if (!await_energy(U, cb + tid, e)) S.abort_flag = 1;
           S.lcost[tid] = e;
         }
-        lds_barrier();
-        if (S.abort_flag) break;
         if (tid < nrep) {
           const double nan = __longlong_as_double(0x7ff8000000000000ll);
           store_agent(x_cbuf + (xbuf == 0 ? 2 : xbuf - 1) * L + x_slot0 + tid, nan);
@@ -1111,7 +1121,10 @@ k_sweep(const UnitDev* units, SweepArgs a) {
         exchange_uniform();
         if (tid < L) S.lcost[tid] = S.rcost[tid];
       }
-      lds_barrier();
+      // Energies, decisions and new betas of a ladder (at most 32 replicas)
+      // all live in wavefront 0: ordered within it, one work-group barrier
+      // at the end of the sweep publishes the betas (and an abort).
+      wave_lds_sync();
       if (i < L - 1) {
         const int sa = S.label[i], sb = S.label[i + 1];
         const double bi = S.lbeta[i], bj = S.lbeta[i + 1];
@@ -1123,7 +1136,7 @@ k_sweep(const UnitDev* units, SweepArgs a) {
           S.inv[sb] = i;
         }
       }
-      lds_barrier();
+      wave_lds_sync();
       if (tid < nrep) S.rbeta[tid] = S.lbeta[S.inv[x_slot0 + tid]];
     }
     if (LADDER && x_ladder > 0 && ++xphase == x_spr) {
@@ -1132,6 +1145,7 @@ k_sweep(const UnitDev* units, SweepArgs a) {
       xbuf = xbuf == 2 ? 0 : xbuf + 1;
     }
     lds_barrier();
+    if (LADDER && x_upl > 1 && S.abort_flag) break;
   }  // sweeps
 
   // ---- write back ----
