@@ -370,6 +370,9 @@ struct qmc_walkers_s {
   int32_t R = 0, B = 1, G = 1, n_units = 0, stride = 0;
   int32_t cap = 0, cap_units = 0;  // allocated replicas / work-groups
   int32_t lanes = 64;               // positions side by side in a class round
+  // work-groups expected to hold live replicas (population mode: the target
+  // population's, not the capacity's)
+  int32_t live_units = 0;
   int32_t per_seed = 0;
   uint32_t seed = 0, stream = 0;
   int32_t first_replica = 0;
@@ -434,7 +437,8 @@ struct qmc_batch_s {
     int kind, pairwise, coupling, nb, row_e, coop, ladder, ordered;
     int block = 64;
     int stride = 0;
-    double work = 0;  // attempts planned (for spreading groups over streams)
+    int64_t live = 0;  // work-groups expected to hold live replicas
+    double work = 0;   // attempts planned (for spreading groups over streams)
     std::vector<UnitDev> units;
     DevBuf<UnitDev> d_units;
     // cooperative groups: launches of units [chunk[i], chunk[i+1]) (at most
@@ -1079,6 +1083,7 @@ int qmc_walkers_create(qmc_problem p, int32_t n_replicas, int32_t replicas_per_g
   if (w->lanes < 64) return fail(QMC_INVALID_ARGUMENT, "replica bytes exceed the work-group");
   w->cap = w->R;
   w->cap_units = w->n_units;
+  w->live_units = w->n_units;
   w->stride = (int32_t)(((int64_t)(p->N + 1) * w->G + 3) / 4 * 4);
   w->seed = seed;
   w->stream = stream;
@@ -1292,6 +1297,8 @@ int qmc_walkers_pa_begin(qmc_walkers w, int32_t population, double alpha, double
   if (sweeps_per_step < 1) return fail(QMC_INVALID_ARGUMENT, "sweeps_per_step must be >= 1");
   qmc_problem p = w->prob;
   qmc_ctx c = p->ctx;
+  // the population's work-groups, not the capacity's, are the ones that run
+  w->live_units = std::min<int32_t>(w->n_units, (population + w->B - 1) / w->B);
   const int cap = w->cap;
   const size_t N = (size_t)p->N;
   HIP_TRY(w->pa.alloc(1));
@@ -1505,11 +1512,33 @@ int qmc_walkers_set_betas(qmc_walkers w, const double* betas) {
   return QMC_OK;
 }
 
+// Which kernel a launch runs, for occupancy queries.
+struct LaunchKernel {
+  int kind, pairwise, coupling, row_e;
+};
+
 // Dynamic LDS of a sweep launch, plus room for the early fields of split
 // ladders (pre_rounds rounds of colour class 0, see SweepArgs) when they fit
-// the 160 KiB; fills the launch arguments that depend on it.
-static size_t sweep_launch_smem(int stride, int ordered, int block, int nb, int pre_rounds, SweepArgs& a) {
+// the 160 KiB; fills the launch arguments that depend on it.  The lowest
+// states stay in LDS unless keeping them in global memory lets more
+// work-groups share a compute unit and the launch has more work-groups
+// (live_units) than fit with them in LDS: large problems, where the third
+// byte per position would leave one work-group per compute unit.
+// Cooperative launches (pre_rounds, live_units 0) keep the placement their
+// co-residency was checked with.
+static size_t sweep_launch_smem(int stride, int ordered, int block, int nb, int pre_rounds, SweepArgs& a,
+                                const LaunchKernel* k = nullptr, int64_t live_units = 0) {
   size_t smem = sweep_smem_bytes(stride, ordered, &a.best_lds);
+  if (a.best_lds && k && live_units > 0 && !ordered && pre_rounds == 0) {
+    const int fit = max_coresident_blocks(k->kind, k->pairwise, k->coupling, nb, k->row_e, block, smem);
+    if (fit > 0 && live_units > fit) {
+      const size_t lean = sweep_smem_bytes(stride, ordered, nullptr, 0);
+      if (max_coresident_blocks(k->kind, k->pairwise, k->coupling, nb, k->row_e, block, lean) > fit) {
+        smem = lean;
+        a.best_lds = 0;
+      }
+    }
+  }
   a.pre_rounds = 0;
   a.pre_off = 0;
   a.pre_draws = 0;
@@ -1547,7 +1576,9 @@ static int launch_units(qmc_walkers w, const UnitDev* d_units, int block, int32_
   a.n_sweeps = max_sweeps;
   a.lut = p->ctx->lut.p;
   a.unit_of_block = nullptr;
-  const size_t smem = sweep_launch_smem(w->stride, ordered, block, nb, coop ? pre_rounds : 0, a);
+  const LaunchKernel lk{p->kind, ordered ? p->pairwise : w->pair, p->coupling, w->row_e};
+  const size_t smem = sweep_launch_smem(w->stride, ordered, block, nb, coop ? pre_rounds : 0, a, &lk,
+                                        coop ? 0 : std::min(w->live_units, n_units));
   HIP_TRY(launch_sweep(p->kind, ordered ? p->pairwise : w->pair, p->coupling, nb, w->row_e, d_units, n_units, block,
                        smem, a, ordered, coop, ladder, st));
   return QMC_OK;
@@ -1889,6 +1920,7 @@ static int batch_rebuild(qmc_batch b) {
     g->stride = std::max(g->stride, w->stride);
     if (w->coop) g->pre_rounds = std::max(g->pre_rounds, early_rounds(w));
     g->work += (double)p->N * w->R * (double)w->planned;
+    g->live += w->live_units;
     b->member_at[i] = {g, g->units.size()};
     g->units.insert(g->units.end(), w->units.begin(), w->units.end());
   }
@@ -1935,7 +1967,9 @@ static int launch_group(qmc_batch_s::Group& g, int32_t max_sweeps, const double*
   a.n_sweeps = max_sweeps;
   a.lut = lut;
   a.unit_of_block = nullptr;
-  const size_t smem = sweep_launch_smem(g.stride, g.ordered, g.block, g.nb, g.coop ? g.pre_rounds : 0, a);
+  const LaunchKernel lk{g.kind, g.pairwise, g.coupling, g.row_e};
+  const size_t smem = sweep_launch_smem(g.stride, g.ordered, g.block, g.nb, g.coop ? g.pre_rounds : 0, a, &lk,
+                                        g.coop ? 0 : g.live);
   if (smem > 160 * 1024) return fail(QMC_MEMORY_LIMITED, "batch LDS footprint exceeds 160 KiB");
   const int n = (int)g.units.size();
   if (!g.coop) {

@@ -2351,11 +2351,11 @@ __global__ void __launch_bounds__(256) k_scatter(const uint8_t* __restrict__ sta
 
 // ---------------------------------------------------------------------------
 // launch helpers (host side of this translation unit)
-size_t sweep_smem_bytes(int max_stride, int ordered, int* best_lds) {
+size_t sweep_smem_bytes(int max_stride, int ordered, int* best_lds, int allow_best_lds) {
   if (best_lds) *best_lds = 0;
   if (ordered) return 8192 + 2 * (size_t)max_stride;
   const size_t with_best = sizeof(Smem) + 3 * (size_t)max_stride;
-  if (with_best <= 160 * 1024) {
+  if (allow_best_lds && with_best <= 160 * 1024) {
     if (best_lds) *best_lds = 1;
     return with_best;
   }

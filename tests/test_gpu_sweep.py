@@ -241,6 +241,19 @@ def test_row_widths_bit_exact(ctx, name, B):
         assert np.array_equal(got[k], want[k]), k
 
 
+def test_lowest_states_in_global_memory(ctx):
+    """1024 work-groups of a 12000-variable graph: with the lowest states in
+    LDS (three bytes per position) two work-groups fit a compute unit, with
+    them in global memory three, so the launch keeps them in global memory;
+    identical states, costs, lowest costs and states, accepted counts."""
+    betas = np.linspace(0.1, 3.0, 4)
+    op, qp, w, got, want = _run_pair(ctx, "ising", wl.er_ising(12000, 6.0, seed=41), R=1024, B=1, betas=betas,
+                                     lanes=64)
+    assert qp.info.coupling == _qmc.QMC_COUPLING_I32
+    for k in ("states", "cost", "lowest", "lowest_states", "accepted"):
+        assert np.array_equal(got[k], want[k]), k
+
+
 def test_sa_sweeps_f32_six_entry_rows(ctx):
     """fp32 couplings on the 3D lattice (every slice 6 wide) with 2 replicas
     per work-group: the 6-entry kernel; identical states, costs within 1e-5."""
