@@ -14,7 +14,9 @@ wall time).  Nested under their own keys, in the same run:
   c4  the uneven batch: 4096 ER instances, 8 restarts x 1000 SA sweeps each,
       autotuned shared launches against the plain one-instance-per-launch form
   c5  the serving stream: Poisson arrivals of mixed Ising / PUBO requests,
-      population annealing or SA, p50 / p99 latency and the sustained rate
+      population annealing or SA, p50 / p99 latency and the sustained rate,
+      offered at 5 requests/s (the highest rate the loop keeps up with on
+      one MI355X: at 6 requests/s the queue grows)
   c1  SA on the 32x32 +-1 lattice, 64 restarts x 1000 sweeps
   c3  SA on the 2000-variable random PUBO, 256 restarts x 2000 sweeps
 
@@ -502,7 +504,7 @@ def main():
                     help="lanes cover the largest colour class in this many rounds (0 = auto)")
     ap.add_argument("--instances", type=int, default=4096, help="c4: number of instances")
     ap.add_argument("--no-tune", action="store_true", help="c4: skip the autotuner")
-    ap.add_argument("--rate", type=float, default=4.0, help="c5: offered requests per second")
+    ap.add_argument("--rate", type=float, default=5.0, help="c5: offered requests per second")
     ap.add_argument("--duration", type=float, default=30.0, help="c5: seconds of arrivals")
     ap.add_argument("--max-active", type=int, default=64, help="c5: concurrently admitted requests")
     ap.add_argument("--sweeps-per-round", type=int, default=10, help="c5: sweeps per shared launch")
