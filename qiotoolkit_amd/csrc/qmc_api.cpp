@@ -130,6 +130,7 @@ struct DevBuf {
 }  // namespace
 
 constexpr int kAuxStreams = 3;
+constexpr int kMaxBlock = 512;  // threads of the largest sweep work-group
 
 // Pairwise problems whose coefficients are integers in [-128, 127] and whose
 // positions fit 24 bits hold 4-byte entries {position | int8 coefficient <<
@@ -1090,7 +1091,7 @@ int qmc_walkers_create(qmc_problem p, int32_t n_replicas, int32_t replicas_per_g
   w->first_replica = first_replica;
   w->per_seed = replicas_per_seed;
   w->nb = w->B > 4 ? 8 : (w->B > 2 ? 4 : (w->B > 1 ? 2 : 1));
-  const size_t smem = sweep_smem_bytes(w->stride, 0);
+  const size_t smem = sweep_smem_bytes(w->stride, 0, 8, kMaxBlock);
   if (smem > 160 * 1024) return fail(QMC_MEMORY_LIMITED, "problem too large for LDS residency at this group size");
   HIP_TRY(hipSetDevice(p->ctx->device));
   HIP_TRY(w->spins.alloc((size_t)w->n_units * w->stride));
@@ -1528,11 +1529,11 @@ struct LaunchKernel {
 // co-residency was checked with.
 static size_t sweep_launch_smem(int stride, int ordered, int block, int nb, int pre_rounds, SweepArgs& a,
                                 const LaunchKernel* k = nullptr, int64_t live_units = 0) {
-  size_t smem = sweep_smem_bytes(stride, ordered, &a.best_lds);
+  size_t smem = sweep_smem_bytes(stride, ordered, nb, block, &a.best_lds);
   if (a.best_lds && k && live_units > 0 && !ordered && pre_rounds == 0) {
     const int fit = max_coresident_blocks(k->kind, k->pairwise, k->coupling, nb, k->row_e, block, smem);
     if (fit > 0 && live_units > fit) {
-      const size_t lean = sweep_smem_bytes(stride, ordered, nullptr, 0);
+      const size_t lean = sweep_smem_bytes(stride, ordered, nb, block, nullptr, 0);
       if (max_coresident_blocks(k->kind, k->pairwise, k->coupling, nb, k->row_e, block, lean) > fit) {
         smem = lean;
         a.best_lds = 0;

@@ -205,9 +205,11 @@ inline size_t sweep_pre_bytes(int rounds, int block, int nb, bool draws) {
 
 // Dynamic LDS of a sweep launch whose largest unit spin array is max_stride
 // bytes; *best_lds (optional) tells whether the lowest states fit in it too.
-// allow_best_lds 0: the lowest states stay in global memory (two bytes of
-// LDS per position and byte group instead of three)
-size_t sweep_smem_bytes(int max_stride, int ordered, int* best_lds = nullptr, int allow_best_lds = 1);
+// Dynamic LDS of a sweep work-group of `threads` threads with nb replicas
+// per byte lane; allow_best_lds 0: the lowest states stay in global memory
+// (two bytes of LDS per position and byte group instead of three)
+size_t sweep_smem_bytes(int max_stride, int ordered, int nb, int threads, int* best_lds = nullptr,
+                        int allow_best_lds = 1);
 // pairwise: 0 general terms, 1 pairwise 8-byte entries, 2 pairwise int8 4-byte entries
 // row_e: entries per pairwise row read ahead by the kernel (4, 6 or 8; any
 // value is valid for every layout, since slices store at least 8 entries)

@@ -57,25 +57,39 @@ struct Smem {
   int32_t nb_c[kMaxRep];    // class of the sweep's new lowest cost, -1 none
   int32_t nb_tp[kMaxRep];   // its lane in the class sequence
   int32_t kstar[kMaxRep];   // its round (-1: before the lane's first flip)
-  // per-wave partials [class parity][byte group][replica in group][wave]:
-  // a class's combine reads its parity's slots while the next class fills
-  // the other ones
-  union {
-    double d[2][4][8][kMaxWavesPerGroup];
-    int32_t i[2][4][8][kMaxWavesPerGroup];
-  } wsum, wmin;
-  int32_t wlane[2][4][8][kMaxWavesPerGroup];
-  int32_t wk[2][4][8][kMaxWavesPerGroup];
 };
 
-template <typename AT> struct Slots;
-template <> struct Slots<double> {
-  __device__ static double* sum(Smem& s, int p, int g, int b) { return s.wsum.d[p][g][b]; }
-  __device__ static double* mn(Smem& s, int p, int g, int b) { return s.wmin.d[p][g][b]; }
-};
-template <> struct Slots<int32_t> {
-  __device__ static int32_t* sum(Smem& s, int p, int g, int b) { return s.wsum.i[p][g][b]; }
-  __device__ static int32_t* mn(Smem& s, int p, int g, int b) { return s.wmin.i[p][g][b]; }
+// Per-wave partials of the class scans, in LDS right after Smem: entry
+// ((p * GS + g) * 8 + b) * 8 + w for class parity p, byte group g, replica b
+// of the group and wave w of the group (up to 8) of four arrays -- the sums
+// and the minimum prefixes (int32 or fp64 in 8-byte slots), the lane and the
+// round of each minimum.  GS, the byte groups laid out, is 1 below 8
+// replicas per byte lane and else at most threads / 64 (and 4), so small
+// work-groups keep little LDS for them.  A class's combine reads its
+// parity's slots while the next class fills the other ones.
+__host__ __device__ constexpr int slot_groups(int nb, int threads) {
+  return nb < 8 ? 1 : (threads / 64 < 4 ? (threads / 64 < 1 ? 1 : threads / 64) : 4);
+}
+__host__ __device__ constexpr size_t slot_bytes(int groups) {
+  return (size_t)groups * 2 * 8 * kMaxWavesPerGroup * 24;
+}
+
+template <typename AT> struct Slots {
+  using lds_at = __attribute__((address_space(3))) AT;
+  using lds_i32 = __attribute__((address_space(3))) int32_t;
+  lds_at* sum;
+  lds_at* mn;
+  lds_i32* lane;
+  lds_i32* k;
+  int GS;
+  __device__ Slots(unsigned char* base, int GS_) : GS(GS_) {
+    const size_t n = (size_t)GS_ * 2 * 8 * kMaxWavesPerGroup;
+    sum = (lds_at*)(base);
+    mn = (lds_at*)(base + 8 * n);
+    lane = (lds_i32*)(base + 16 * n);
+    k = (lds_i32*)(base + 20 * n);
+  }
+  __device__ int at(int p, int g, int b, int w) const { return ((p * GS + g) * 8 + b) * kMaxWavesPerGroup + w; }
 };
 
 // ---------------------------------------------------------------------------
@@ -520,12 +534,15 @@ k_sweep(const UnitDev* units, SweepArgs a) {
   const bool gen_ell = !PAIR && rows.ent4 != nullptr;
   const bool slices = PAIR || gen_ell;
   gptr<int32_t> var_of_pos = glob(uni_ptr(U.var_of_pos));
-  lds_u8* sp = (lds_u8*)(smem_raw + sizeof(Smem));  // generic -> LDS address space
-  lds_u8* fl = sp + stride;
-  // lowest states: in LDS for the launch when they fit, else in HBM
-  uint8_t* bst = a.best_lds ? reinterpret_cast<uint8_t*>(smem_raw + sizeof(Smem) + 2 * stride) : U.best;
   const int tid = threadIdx.x;
   const int nthreads = blockDim.x;
+  const int GS = slot_groups(NB, nthreads);
+  const Slots<AT> W_(smem_raw + sizeof(Smem), GS);
+  unsigned char* const state_lds = smem_raw + sizeof(Smem) + slot_bytes(GS);
+  lds_u8* sp = (lds_u8*)state_lds;  // generic -> LDS address space
+  lds_u8* fl = sp + stride;
+  // lowest states: in LDS for the launch when they fit, else in HBM
+  uint8_t* bst = a.best_lds ? reinterpret_cast<uint8_t*>(state_lds + 2 * stride) : U.best;
   const bool active = tid < TP * G;
   const int g = (NB < 8 || !active) ? 0 : tid >> lg;
   const int tp = tid - (g << lg);
@@ -901,10 +918,11 @@ k_sweep(const UnitDev* units, SweepArgs a) {
           const int wk = __builtin_amdgcn_readlane(kmin[b], wl);
           const AT tot = lane_value(incl[b], 63);
           if (lane == 0 && b < nb_g) {
-            Slots<AT>::sum(S, par, g, b)[wave_in_group] = tot;
-            Slots<AT>::mn(S, par, g, b)[wave_in_group] = m;
-            S.wlane[par][g][b][wave_in_group] = wl;
-            S.wk[par][g][b][wave_in_group] = wk;
+            const int at = W_.at(par, g, b, wave_in_group);
+            W_.sum[at] = tot;
+            W_.mn[at] = m;
+            W_.lane[at] = wl;
+            W_.k[at] = wk;
           }
         }
       }
@@ -923,10 +941,11 @@ k_sweep(const UnitDev* units, SweepArgs a) {
           const int gg = rr >> 3, bb = rr & 7;
           int32_t sm = 0, m = 0, wl = 0, wk = -1;
           if (live) {
-            sm = S.wsum.i[par][gg][bb][w];
-            m = S.wmin.i[par][gg][bb][w];
-            wl = S.wlane[par][gg][bb][w];
-            wk = S.wk[par][gg][bb][w];
+            const int at = W_.at(par, gg, bb, w);
+            sm = W_.sum[at];
+            m = W_.mn[at];
+            wl = W_.lane[at];
+            wk = W_.k[at];
           }
           int32_t incl = sm;
           const int32_t o1 = dpp_z<DPP_ROW_SHR1>(incl);
@@ -972,12 +991,13 @@ k_sweep(const UnitDev* units, SweepArgs a) {
           // combined in wave order; slots of absent waves are read, not used
           AT mnv[kMaxWavesPerGroup], smv[kMaxWavesPerGroup];
           int wlv[kMaxWavesPerGroup], wkv[kMaxWavesPerGroup];
+          const int at0 = W_.at(par, gg, bb, 0);
 #pragma unroll
           for (int w = 0; w < kMaxWavesPerGroup; ++w) {
-            mnv[w] = Slots<AT>::mn(S, par, gg, bb)[w];
-            smv[w] = Slots<AT>::sum(S, par, gg, bb)[w];
-            wlv[w] = S.wlane[par][gg][bb][w];
-            wkv[w] = S.wk[par][gg][bb][w];
+            mnv[w] = W_.mn[at0 + w];
+            smv[w] = W_.sum[at0 + w];
+            wlv[w] = W_.lane[at0 + w];
+            wkv[w] = W_.k[at0 + w];
           }
           const double before = S.rcost[tid], best_before = S.rbest[tid];
           AT run = AT(0), bestc = AT(0);
@@ -2351,15 +2371,16 @@ __global__ void __launch_bounds__(256) k_scatter(const uint8_t* __restrict__ sta
 
 // ---------------------------------------------------------------------------
 // launch helpers (host side of this translation unit)
-size_t sweep_smem_bytes(int max_stride, int ordered, int* best_lds, int allow_best_lds) {
+size_t sweep_smem_bytes(int max_stride, int ordered, int nb, int threads, int* best_lds, int allow_best_lds) {
   if (best_lds) *best_lds = 0;
   if (ordered) return 8192 + 2 * (size_t)max_stride;
-  const size_t with_best = sizeof(Smem) + 3 * (size_t)max_stride;
+  const size_t fixed = sizeof(Smem) + slot_bytes(slot_groups(nb, threads));
+  const size_t with_best = fixed + 3 * (size_t)max_stride;
   if (allow_best_lds && with_best <= 160 * 1024) {
     if (best_lds) *best_lds = 1;
     return with_best;
   }
-  return sizeof(Smem) + 2 * (size_t)max_stride;
+  return fixed + 2 * (size_t)max_stride;
 }
 
 // The dynamic-LDS limit is raised once per kernel to the full 160 KiB, so
