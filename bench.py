@@ -15,8 +15,9 @@ wall time).  Nested under their own keys, in the same run:
       autotuned shared launches against the plain one-instance-per-launch form
   c5  the serving stream: Poisson arrivals of mixed Ising / PUBO requests,
       population annealing or SA, p50 / p99 latency and the sustained rate,
-      offered at 5 requests/s (the highest rate the loop keeps up with on
-      one MI355X: at 6 requests/s the queue grows)
+      offered at 6 requests/s (the highest rate the loop keeps up with on
+      one MI355X: over 60 s at 6 requests/s at most 2 requests wait for
+      admission and the latency stays flat; at 7 the device is 86 % busy)
   c1  SA on the 32x32 +-1 lattice, 64 restarts x 1000 sweeps
   c3  SA on the 2000-variable random PUBO, 256 restarts x 2000 sweeps
 
@@ -57,7 +58,7 @@ MEASURED_TRAFFIC_BYTES = {
     "c2": int((107108.3 * 2 + 82106.0) * 1024),
     "c1": int((328.4 * 2 + 142.0) * 1024),
     "c3": int((6100.3 * 2 + 1056.3) * 1024),
-    "c4": int((98178123.4 * 2 + 9075.9) * 1024),
+    "c4": int((107989455.8 * 2 + 9075.9) * 1024),
 }
 
 # What limits each kernel (rocprofv3 kernel traces, SQ and byte counters in
@@ -504,7 +505,7 @@ def main():
                     help="lanes cover the largest colour class in this many rounds (0 = auto)")
     ap.add_argument("--instances", type=int, default=4096, help="c4: number of instances")
     ap.add_argument("--no-tune", action="store_true", help="c4: skip the autotuner")
-    ap.add_argument("--rate", type=float, default=5.0, help="c5: offered requests per second")
+    ap.add_argument("--rate", type=float, default=6.0, help="c5: offered requests per second")
     ap.add_argument("--duration", type=float, default=30.0, help="c5: seconds of arrivals")
     ap.add_argument("--max-active", type=int, default=64, help="c5: concurrently admitted requests")
     ap.add_argument("--sweeps-per-round", type=int, default=10, help="c5: sweeps per shared launch")
