@@ -410,6 +410,7 @@ struct qmc_walkers_s {
   uint32_t pend_epoch = 0;
   bool pend_prog = false;
   int64_t pend_sweep0 = 0, pend_n = 0;
+  bool pend_keep_ladder = false;  // the pending set-up keeps the uploaded ladder labels and betas
   // population annealing with the population size on the device
   int pa_on = 0;
   PaArgs pa_args{};
@@ -1152,6 +1153,7 @@ static void add_setup_descs(qmc_walkers w, const UnitDev* du, std::vector<SetupD
     d.epoch = w->pend_epoch;
     d.clear = w->pend_zero ? 1 : 0;
     d.set_progress = w->pend_prog ? 1 : 0;
+    d.keep_ladder = w->pend_keep_ladder ? 1 : 0;
     d.sweep0 = w->pend_sweep0;
     d.n_sweeps = w->pend_n;
     out.push_back(d);
@@ -1164,6 +1166,7 @@ static int setup_done(qmc_walkers w) {
   w->pend_init = false;
   w->pend_zero = false;
   w->pend_prog = false;
+  w->pend_keep_ladder = false;
   if (w->pa_on && w->pa_args.cost0)
     HIP_TRY(hipMemcpyAsync(w->pa_args.cost0, w->cost.p, (size_t)w->cap * sizeof(double),
                            hipMemcpyDeviceToDevice, stream_of(w->prob->ctx)));
@@ -1638,6 +1641,7 @@ int qmc_walkers_plan(qmc_walkers w, int64_t sweep0, int32_t n_sweeps, const doub
     w->pend_prog = true;
     w->pend_sweep0 = sweep0;
     w->pend_n = n_sweeps;
+    w->pend_keep_ladder = false;
     w->planned = n_sweeps;
     return QMC_OK;
   }
@@ -1667,7 +1671,12 @@ static int set_ladder(qmc_walkers w, int32_t ladder, const double* ladder_betas,
 int qmc_walkers_plan_pt(qmc_walkers w, int32_t ladder, const double* ladder_betas, int64_t step0,
                         int32_t n_steps, int32_t sweeps_per_replica) {
   if (!w || !ladder_betas) return fail(QMC_INVALID_ARGUMENT, "null argument");
-  if (int rc0 = materialize(w)) return rc0;
+  // A pending one-launch set-up stays pending (a batch runs those of all
+  // its members in one launch): it keeps the labels and betas uploaded here
+  // and sets the progress recorded here.
+  const bool defer = w->pend_init && setup_fits(w->stride);
+  if (!defer)
+    if (int rc0 = materialize(w)) return rc0;
   if (ladder < 1 || ladder % w->B != 0)
     return fail(QMC_INVALID_ARGUMENT, "the ladder must be a multiple of replicas_per_group");
   if (sweeps_per_replica < 1) return fail(QMC_INVALID_ARGUMENT, "sweeps_per_replica must be >= 1");
@@ -1693,6 +1702,14 @@ int qmc_walkers_plan_pt(qmc_walkers w, int32_t ladder, const double* ladder_beta
   w->spr = sweeps_per_replica;
   w->sched.reset();
   if ((rc = upload_units(w))) return rc;
+  if (defer) {
+    w->pend_keep_ladder = true;
+    w->pend_prog = true;
+    w->pend_sweep0 = step0 * sweeps_per_replica;
+    w->pend_n = (int64_t)n_steps * sweeps_per_replica;
+    w->planned = w->pend_n;
+    return QMC_OK;
+  }
   return set_progress(w, step0 * sweeps_per_replica, (int64_t)n_steps * sweeps_per_replica);
 }
 

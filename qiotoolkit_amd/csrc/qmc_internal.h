@@ -251,6 +251,8 @@ struct SetupDesc {
   uint32_t epoch;
   int32_t clear;
   int32_t set_progress;
+  int32_t keep_ladder;  // clearing leaves beta and labels (a ladder plan uploaded them)
+  int32_t pad_;
 };
 // max_stride: largest spin array among the units; fails with
 // hipErrorInvalidValue when it does not fit in LDS (setup_fits says so).

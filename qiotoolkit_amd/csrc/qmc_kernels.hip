@@ -1442,8 +1442,10 @@ __global__ void __launch_bounds__(256) k_setup_units(const SetupDesc* __restrict
   if (D.clear) {
     for (int r = tid; r < (int)U.nrep; r += 256) {
       U.accepted[r] = 0ull;
-      U.beta[r] = 0.0;
-      U.label[r] = 0;
+      if (!D.keep_ladder) {
+        U.beta[r] = 0.0;
+        U.label[r] = 0;
+      }
     }
     if (U.rbase == 0 && tid == 0) *U.err = 0;
     if (!D.set_progress && tid < 2) U.progress[tid] = 0;
