@@ -414,8 +414,14 @@ def run_c5(args):
                        "max": float(lat.max()) if len(lat) else None},
         "requests_per_s": len(lat) / t_end,
         # completed requests per second between the first arrival and the
-        # last completion; equal to the offered rate when the loop keeps up
+        # last completion (the drain after the last arrival keeps it below
+        # the offered rate even when the loop keeps up)
         "sustained_rate_per_s": len(lat) / max(1e-9, t_end - (reqs[0]["t"] if reqs else 0.0)),
+        # the loop keeps up when the later arrivals wait no longer than the
+        # earlier ones and few requests wait for admission
+        "latency_p50_by_arrival_half_ms": [float(np.percentile(h, 50)) if len(h) else None
+                                           for h in (lat[:len(lat) // 2], lat[len(lat) // 2:])],
+        "most_waiting_for_admission": st.get("peak_pending"),
         "failed": fails,
         "server_stats": st,
     }
