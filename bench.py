@@ -358,6 +358,8 @@ def run_c5(args):
         pa_extra["replicas_per_group"] = args.pa_group
     if args.pa_order:
         pa_extra["sweep_order"] = args.pa_order
+    if args.pa_lanes:
+        pa_extra["lanes"] = args.pa_lanes
     reqs = c5_requests(args.rate, args.duration, pa_extra=pa_extra)
     log("c5: %d requests generated in %.1f s (%d ising, %d pubo, %d PA, %d SA)" % (
         len(reqs), time.perf_counter() - t_gen, sum(r["kind"] == "ising" for r in reqs),
@@ -518,6 +520,7 @@ def main():
     ap.add_argument("--hbm-budget-gib", type=float, default=0.0, help="c5: device memory for requests")
     ap.add_argument("--pa-group", type=int, default=0, help="c5: replicas per work-group of PA requests (0 = solver default)")
     ap.add_argument("--pa-order", default="", help="c5: sweep_order of PA requests (default: the solver's, index)")
+    ap.add_argument("--pa-lanes", type=int, default=0, help="c5: lanes of PA work-groups (0 = library default)")
     ap.add_argument("--plain-sample", type=int, default=128,
                     help="c4: instances run one per launch for the plain-form rate (0: none)")
     ap.add_argument("--no-cpu", action="store_true")
